@@ -1,0 +1,218 @@
+"""MI355X channel-flow LES time step (the hot path of wadaniel/channel_RL).
+
+Python front of the C ABI in include/chanles.h (libchanles.so, HIP for
+gfx950).  Method names follow the reference's Fortran subroutines; arrays are
+numpy float64 in the reference's Fortran shapes (order='F').  The HIP library
+is the only compute path: importing works without it, but creating a Channel
+raises if libchanles.so is missing or no GPU is present.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["Channel", "ChanError", "load_library", "LIB_PATH", "FP64", "FP32_BULK",
+           "WALL_GIVEN", "WALL_EQUILIBRIUM"]
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libchanles.so")
+
+FP64, FP32_BULK = 0, 1
+WALL_GIVEN, WALL_EQUILIBRIUM = 0, 1
+U, V, W, NU_T, P, ALPHA_X, RHS_U, RHS_V, RHS_W = range(9)
+
+EXPORTS = [
+    "chan_create", "chan_destroy", "chan_last_error", "chan_set_field", "chan_get_field",
+    "chan_set_scalars", "chan_get_scalars", "chan_apply_boundary_conditions",
+    "chan_compute_eddy_viscosity", "chan_compute_wall_model", "chan_compute_rhs",
+    "chan_compute_projection_step", "chan_solve_poisson_equation",
+    "chan_compute_mean_mass_flow_U", "chan_initialize_mass_flow",
+    "chan_compute_dPx_for_constant_mass_flow", "chan_compute_dt",
+    "chan_compute_time_step_RK3", "chan_check_divergence", "chan_set_wall_model",
+    "chan_compute_equilibrium_wall_stress", "chan_plane_statistics", "chan_run_steps",
+    "chan_synchronize",
+]
+
+
+class ChanError(RuntimeError):
+    pass
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("x", ctypes.POINTER(ctypes.c_double)), ("y", ctypes.POINTER(ctypes.c_double)),
+                ("z", ctypes.POINTER(ctypes.c_double)),
+                ("nu", ctypes.c_double), ("CFL", ctypes.c_double), ("dPdx", ctypes.c_double),
+                ("x_mass_cte", ctypes.c_int), ("LES_model", ctypes.c_int),
+                ("Dirichlet_nu_t", ctypes.c_int), ("precision", ctypes.c_int), ("device", ctypes.c_int)]
+
+
+_LIB = None
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+def load_library(path=LIB_PATH):
+    """Load libchanles.so (raises if it is not built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise ChanError(f"{path} is not built: run `python -m channel_rl_amd.build`")
+    L = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    L.chan_create.argtypes = [ctypes.POINTER(_Config), ctypes.POINTER(vp)]
+    L.chan_destroy.argtypes = [vp]
+    L.chan_destroy.restype = None
+    L.chan_last_error.argtypes = [vp]
+    L.chan_last_error.restype = ctypes.c_char_p
+    L.chan_set_field.argtypes = [vp, ctypes.c_int, _D]
+    L.chan_get_field.argtypes = [vp, ctypes.c_int, _D]
+    L.chan_set_scalars.argtypes = [vp] + [ctypes.c_double] * 4
+    L.chan_get_scalars.argtypes = [vp, _D]
+    for n in ("chan_apply_boundary_conditions", "chan_compute_eddy_viscosity", "chan_compute_wall_model",
+              "chan_compute_rhs", "chan_compute_projection_step", "chan_solve_poisson_equation",
+              "chan_initialize_mass_flow", "chan_compute_time_step_RK3", "chan_synchronize"):
+        getattr(L, n).argtypes = [vp]
+    L.chan_compute_mean_mass_flow_U.argtypes = [vp, _D]
+    L.chan_compute_dPx_for_constant_mass_flow.argtypes = [vp, _D]
+    L.chan_compute_dt.argtypes = [vp, _D]
+    L.chan_check_divergence.argtypes = [vp, _D, _D]
+    L.chan_set_wall_model.argtypes = [vp, ctypes.c_int] + [ctypes.c_double] * 3
+    L.chan_compute_equilibrium_wall_stress.argtypes = [vp] + [ctypes.c_double] * 3
+    L.chan_plane_statistics.argtypes = [vp, _D]
+    L.chan_run_steps.argtypes = [vp, ctypes.c_int]
+    _LIB = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(_D)
+
+
+class Channel:
+    """One channel resident on the GPU (a chan_ctx)."""
+
+    def __init__(self, x, y, z, nu, CFL=-1e-2, dPdx=0.0, x_mass_cte=0, LES_model=2,
+                 Dirichlet_nu_t=1, precision=FP64, device=0):
+        L = load_library()
+        self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z)]
+        self.nx, self.ny, self.nz = (len(a) for a in self._keep)
+        self.nxg, self.nyg, self.nzg = self.nx + 1, self.ny + 1, self.nz + 1
+        self.precision = precision
+        cfg = _Config(self.nx, self.ny, self.nz, _p(self._keep[0]), _p(self._keep[1]), _p(self._keep[2]),
+                      nu, CFL, dPdx, x_mass_cte, LES_model, Dirichlet_nu_t, precision, device)
+        h = ctypes.c_void_p()
+        rc = L.chan_create(ctypes.byref(cfg), ctypes.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = L.chan_last_error(h).decode() if h.value else "chan_create failed"
+            L.chan_destroy(h)
+            self._h = None
+            raise ChanError(f"chan_create: {msg} (status {rc})")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().chan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise ChanError(f"{what}: {load_library().chan_last_error(self._h).decode()} (status {rc})")
+
+    def shape(self, which):
+        nx, ny, nz, nxg, nyg, nzg = self.nx, self.ny, self.nz, self.nxg, self.nyg, self.nzg
+        return {U: (nx, nyg, nzg), V: (nxg, ny, nzg), W: (nxg, nyg, nz), NU_T: (nxg, nyg, nzg),
+                P: (nxg, nyg, nzg), ALPHA_X: (nx, 2, nzg), RHS_U: (nx - 2, nyg - 2, nzg - 2),
+                RHS_V: (nxg - 2, ny - 2, nzg - 2), RHS_W: (nxg - 2, nyg - 2, nz - 2)}[which]
+
+    def set_field(self, which, a):
+        a = np.asfortranarray(a, dtype=np.float64)
+        if a.shape != self.shape(which):
+            raise ValueError(f"field {which}: shape {a.shape} != {self.shape(which)}")
+        self._chk(load_library().chan_set_field(self._h, which, _p(a)), "chan_set_field")
+
+    def get_field(self, which):
+        a = np.zeros(self.shape(which), dtype=np.float64, order="F")
+        self._chk(load_library().chan_get_field(self._h, which, _p(a)), "chan_get_field")
+        return a
+
+    def set_state(self, U_, V_, W_):
+        self.set_field(U, U_); self.set_field(V, V_); self.set_field(W, W_)
+
+    def state(self):
+        return self.get_field(U), self.get_field(V), self.get_field(W)
+
+    def set_scalars(self, dt=0.0, t=0.0, dPdx=0.0, Qflow_x_0=0.0):
+        self._chk(load_library().chan_set_scalars(self._h, dt, t, dPdx, Qflow_x_0), "chan_set_scalars")
+
+    def scalars(self):
+        out = np.zeros(4)
+        self._chk(load_library().chan_get_scalars(self._h, _p(out)), "chan_get_scalars")
+        return dict(dt=out[0], t=out[1], dPdx=out[2], Qflow_x_0=out[3])
+
+    # the reference's subroutines
+    def apply_boundary_conditions(self):
+        self._chk(load_library().chan_apply_boundary_conditions(self._h), "apply_boundary_conditions")
+
+    def compute_eddy_viscosity(self):
+        self._chk(load_library().chan_compute_eddy_viscosity(self._h), "compute_eddy_viscosity")
+
+    def compute_wall_model(self):
+        self._chk(load_library().chan_compute_wall_model(self._h), "compute_wall_model")
+
+    def compute_rhs(self):
+        self._chk(load_library().chan_compute_rhs(self._h), "compute_rhs")
+        return self.get_field(RHS_U), self.get_field(RHS_V), self.get_field(RHS_W)
+
+    def compute_projection_step(self):
+        self._chk(load_library().chan_compute_projection_step(self._h), "compute_projection_step")
+
+    def solve_poisson_equation(self):
+        self._chk(load_library().chan_solve_poisson_equation(self._h), "solve_poisson_equation")
+
+    def compute_mean_mass_flow_U(self):
+        q = ctypes.c_double()
+        self._chk(load_library().chan_compute_mean_mass_flow_U(self._h, ctypes.byref(q)), "compute_mean_mass_flow_U")
+        return q.value
+
+    def initialize_mass_flow(self):
+        self._chk(load_library().chan_initialize_mass_flow(self._h), "initialize_mass_flow")
+
+    def compute_dPx_for_constant_mass_flow(self):
+        q = ctypes.c_double()
+        self._chk(load_library().chan_compute_dPx_for_constant_mass_flow(self._h, ctypes.byref(q)),
+                  "compute_dPx_for_constant_mass_flow")
+        return q.value
+
+    def compute_dt(self, sync=True):
+        q = ctypes.c_double()
+        self._chk(load_library().chan_compute_dt(self._h, ctypes.byref(q) if sync else None), "compute_dt")
+        return q.value if sync else None
+
+    def compute_time_step_RK3(self):
+        self._chk(load_library().chan_compute_time_step_RK3(self._h), "compute_time_step_RK3")
+
+    def check_divergence(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._chk(load_library().chan_check_divergence(self._h, ctypes.byref(a), ctypes.byref(b)), "check_divergence")
+        return a.value, b.value
+
+    def set_wall_model(self, mode, h_wm=0.0, kappa=0.41, B=5.2):
+        self._chk(load_library().chan_set_wall_model(self._h, mode, h_wm, kappa, B), "set_wall_model")
+
+    def compute_equilibrium_wall_stress(self, h_wm=0.0, kappa=0.41, B=5.2):
+        self._chk(load_library().chan_compute_equilibrium_wall_stress(self._h, h_wm, kappa, B),
+                  "compute_equilibrium_wall_stress")
+
+    def plane_statistics(self):
+        out = np.zeros(7 * self.ny)
+        self._chk(load_library().chan_plane_statistics(self._h, _p(out)), "plane_statistics")
+        return out.reshape(7, self.ny)
+
+    def run_steps(self, n):
+        self._chk(load_library().chan_run_steps(self._h, n), "run_steps")
+
+    def synchronize(self):
+        self._chk(load_library().chan_synchronize(self._h), "synchronize")

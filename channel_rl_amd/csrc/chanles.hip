@@ -1,0 +1,822 @@
+// chanles.hip -- the C ABI (include/chanles.h): context set-up, host<->device
+// copies in the reference's Fortran shapes, and the per-subroutine entry
+// points, including the graph-captured RK3 step.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/chanles.h"
+#include "kernels.hpp"
+
+using namespace chan;
+
+namespace {
+
+// ------------------------------------------------------------ pack/unpack
+// Fortran-packed host layout (n1,n2,n3) placed at box offset (o,o,o).
+template <typename T>
+__global__ void k_unpack(const double* src, T* dst, Dims d, int n1, int n2, int n3, int o)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long tot = (long)n1 * n2 * n3;
+    if (q >= tot) return;
+    const int i = (int)(q % n1), j = (int)((q / n1) % n2), k = (int)(q / ((long)n1 * n2));
+    dst[at(d, i + o, j + o, k + o)] = T(src[q]);
+}
+
+template <typename T>
+__global__ void k_pack(const T* src, double* dst, Dims d, int n1, int n2, int n3, int o)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long tot = (long)n1 * n2 * n3;
+    if (q >= tot) return;
+    const int i = (int)(q % n1), j = (int)((q / n1) % n2), k = (int)(q / ((long)n1 * n2));
+    dst[q] = (double)src[at(d, i + o, j + o, k + o)];
+}
+
+template <typename T>
+__global__ void k_fill(T* a, long n, T v)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) a[q] = v;
+}
+
+__global__ void k_advance_time(Scalars* sc) { sc->t = sc->t + sc->dt; }
+
+struct Shape { int n1, n2, n3, o; };
+
+}  // namespace
+
+struct chan_ctx {
+    std::string err;
+    int prec = CHAN_FP64;
+    int device = 0;
+    Dims d{};
+    // host copies of the set-up constants
+    std::vector<double> x, y, z, xg, yg, zg, w0, w1, kxx, kzz, dd, dl, du;
+    double dxmin = 0, dymin = 0, dzmin = 0;
+    double rk_t[3]{}, rk_coef[3][3]{};
+    double nu = 0, CFL = 0, dPdx0 = 0;
+    int x_mass_cte = 0, LES_model = 0, Dirichlet_nu_t = 1;
+    int wall_mode = CHAN_WALL_GIVEN;
+    double h_wm = 0.0, kappa = 0.41, Bc = 5.2;
+    hipStream_t stream = nullptr;
+    // device
+    Metric<double> md{};
+    Metric<float> mf{};
+    std::vector<void*> allocs;
+    void* state[3][3] = {};  // [buffer][U,V,W]
+    int cur = 0;
+    void* F1[3] = {};
+    void* rhs[3] = {};
+    void* nut = nullptr;
+    void* dsm[13] = {};
+    double* part = nullptr;
+    int nblk_dsm = 0, nblk_red = 0;
+    double* ratio = nullptr;
+    double* alpha = nullptr;
+    double* stage = nullptr;
+    Scalars* sc = nullptr;
+    double* mins3 = nullptr;
+    PoissonBufs pb{};
+    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+    bool use_graph = true;
+};
+
+namespace {
+
+int fail(chan_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(ctx, CHAN_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+int dalloc(chan_ctx* c, void** p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes ? bytes : 8);
+    if (e != hipSuccess) return fail(c, CHAN_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->allocs.push_back(*p);
+    e = hipMemset(*p, 0, bytes ? bytes : 8);
+    if (e != hipSuccess) return fail(c, CHAN_EHIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    return CHAN_OK;
+}
+
+template <typename V>
+int upload(chan_ctx* c, const V& v, const double** out)
+{
+    void* p;
+    int rc = dalloc(c, &p, v.size() * sizeof(double));
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    *out = (const double*)p;
+    return CHAN_OK;
+}
+
+int upload_f(chan_ctx* c, const std::vector<double>& v, const float** out)
+{
+    std::vector<float> f(v.begin(), v.end());
+    void* p;
+    int rc = dalloc(c, &p, f.size() * sizeof(float));
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(p, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+    *out = (const float*)p;
+    return CHAN_OK;
+}
+
+bool factor_plan(int n, FftPlan& pl)
+{
+    pl.n = n;
+    pl.nr = 0;
+    int m = n;
+    while (m > 1) {
+        int r = (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3 : (m % 5 == 0) ? 5 : 0;
+        if (r == 0 || pl.nr >= 12) return false;
+        pl.r[pl.nr++] = r;
+        m /= r;
+    }
+    return true;
+}
+
+int make_twiddles(chan_ctx* c, FftPlan& pl)
+{
+    std::vector<double2> tw(pl.n);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int t = 0; t < pl.n; t++) {
+        long double a = -2.0L * pi * (long double)t / (long double)pl.n;
+        tw[t] = make_double2((double)cosl(a), (double)sinl(a));
+    }
+    void* p;
+    int rc = dalloc(c, &p, tw.size() * sizeof(double2));
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(p, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice));
+    pl.tw = (const double2*)p;
+    return CHAN_OK;
+}
+
+size_t esize(const chan_ctx* c) { return c->prec == CHAN_FP64 ? sizeof(double) : sizeof(float); }
+
+Shape field_shape(const chan_ctx* c, int which)
+{
+    const Dims& d = c->d;
+    switch (which) {
+    case CHAN_U: return {d.nx, d.nyg, d.nzg, 1};
+    case CHAN_V: return {d.nxg, d.ny, d.nzg, 1};
+    case CHAN_W: return {d.nxg, d.nyg, d.nz, 1};
+    case CHAN_NU_T: case CHAN_P: return {d.nxg, d.nyg, d.nzg, 1};
+    case CHAN_ALPHA_X: return {d.nx, 2, d.nzg, 1};
+    case CHAN_RHS_U: return {d.nx - 2, d.nyg - 2, d.nzg - 2, 2};
+    case CHAN_RHS_V: return {d.nxg - 2, d.ny - 2, d.nzg - 2, 2};
+    case CHAN_RHS_W: return {d.nxg - 2, d.nyg - 2, d.nz - 2, 2};
+    }
+    return {0, 0, 0, 0};
+}
+
+void* field_ptr(chan_ctx* c, int which)
+{
+    switch (which) {
+    case CHAN_U: case CHAN_V: case CHAN_W: return c->state[c->cur][which];
+    case CHAN_NU_T: return c->nut;
+    case CHAN_P: return c->pb.p;
+    case CHAN_RHS_U: case CHAN_RHS_V: case CHAN_RHS_W: return c->rhs[which - CHAN_RHS_U];
+    }
+    return nullptr;
+}
+
+template <typename T>
+DsmBufs<T> dsm_bufs(chan_ctx* c)
+{
+    DsmBufs<T> b;
+    for (int q = 0; q < 3; q++) b.ac[q] = (T*)c->dsm[q];
+    for (int q = 0; q < 6; q++) b.ss[q] = (T*)c->dsm[3 + q];
+    b.smag = (T*)c->dsm[9];
+    b.uf = (T*)c->dsm[10];
+    b.vf = (T*)c->dsm[11];
+    b.wf = (T*)c->dsm[12];
+    b.part = c->part;
+    b.ratio = c->ratio;
+    b.nblk = c->nblk_dsm;
+    return b;
+}
+
+template <typename T>
+const Metric<T>& metric(const chan_ctx* c);
+template <>
+const Metric<double>& metric<double>(const chan_ctx* c) { return c->md; }
+template <>
+const Metric<float>& metric<float>(const chan_ctx* c) { return c->mf; }
+
+// ---------------------------------------------------------- stage helpers
+
+template <typename T>
+void enqueue_bc(chan_ctx* c, int buf)
+{
+    T** s = (T**)c->state[buf];
+    launch_bc<T>(s[0], c->d, KIND_U, c->stream);
+    launch_bc<T>(s[1], c->d, KIND_V, c->stream);
+    launch_bc<T>(s[2], c->d, KIND_W, c->stream);
+}
+
+template <typename T>
+void enqueue_eddy_viscosity(chan_ctx* c, int buf, int fuse_wm)
+{
+    T** s = (T**)c->state[buf];
+    const T nu = T(c->nu);
+    if (c->LES_model == 2) {
+        DsmBufs<T> b = dsm_bufs<T>(c);
+        launch_dsm<T>(s[0], s[1], s[2], (T*)c->nut, metric<T>(c), c->d, b, c->Dirichlet_nu_t == 1, fuse_wm,
+                      c->alpha, nu, c->stream);
+    } else {
+        k_fill<T><<<cdiv(c->d.nbox, 256), 256, 0, c->stream>>>((T*)c->nut, c->d.nbox, T(0));
+        if (fuse_wm) launch_wall_model<T>(s[0], (T*)c->nut, c->alpha, metric<T>(c), c->d, nu, c->stream);
+    }
+}
+
+template <typename T>
+void enqueue_projection(chan_ctx* c, int buf)
+{
+    T** s = (T**)c->state[buf];
+    launch_poisson_forward<T>(s[0], s[1], s[2], c->md, c->d, c->pb, c->stream);
+    launch_poisson_solve(c->d, c->pb, c->stream);
+    launch_poisson_inverse(c->d, c->pb, c->stream);
+    launch_project<T>(s[0], s[1], s[2], c->pb.p, c->md, c->d, c->stream);
+}
+
+// code/time_integration.f90:24-96 with three rotating state buffers:
+// X (start of step, also Uo), Y, Z; substeps X->Y, Y->Z, Z->Y.
+template <typename T>
+void enqueue_rk3(chan_ctx* c, int X)
+{
+    const int Y = (X + 1) % 3, Z = (X + 2) % 3;
+    const int in[3] = {X, Y, Z}, out[3] = {Y, Z, Y};
+    T** Xs = (T**)c->state[X];
+    for (int s = 0; s < 3; s++) {
+        T** I = (T**)c->state[in[s]];
+        T** O = (T**)c->state[out[s]];
+        if (c->wall_mode == CHAN_WALL_EQUILIBRIUM)
+            launch_equilibrium_alpha<T>(I[0], c->alpha, c->md, c->d, c->nu, c->h_wm, c->kappa, c->Bc, c->stream);
+        enqueue_eddy_viscosity<T>(c, in[s], 1);
+        double c1, c2 = 0.0;
+        if (s == 0) c1 = c->rk_coef[0][0];
+        else if (s == 1) { c1 = c->rk_coef[1][0]; c2 = c->rk_coef[1][1]; }
+        else { c1 = c->rk_coef[2][0]; c2 = c->rk_coef[2][2]; }
+        launch_rhs_rk<T>(I[0], I[1], I[2], (const T*)c->nut, metric<T>(c), c->d, T(c->nu), c->sc, s + 1, Xs[0],
+                         Xs[1], Xs[2], (T*)c->F1[0], (T*)c->F1[1], (T*)c->F1[2], O[0], O[1], O[2], c1, c2,
+                         c->stream);
+        enqueue_bc<T>(c, out[s]);
+        enqueue_projection<T>(c, out[s]);
+    }
+    if (c->x_mass_cte == 1) {
+        T** O = (T**)c->state[Y];
+        launch_mass_flow_correct<T>(O[0], c->d, c->part, c->nblk_red, c->sc, 1, c->stream);
+    }
+    k_advance_time<<<1, 1, 0, c->stream>>>(c->sc);
+}
+
+int check_launch(chan_ctx* c)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(c, CHAN_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return CHAN_OK;
+}
+
+template <typename T>
+int rk3_step(chan_ctx* c)
+{
+    const int X = c->cur;
+    if (!c->use_graph) {
+        enqueue_rk3<T>(c, X);
+        int rc = check_launch(c);
+        if (rc) return rc;
+    } else {
+        if (!c->graph[X]) {
+            hipGraph_t g;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_rk3<T>(c, X);
+            HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+            hipError_t e = hipGraphInstantiate(&c->graph[X], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return fail(c, CHAN_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        HIPCHK(c, hipGraphLaunch(c->graph[X], c->stream));
+    }
+    c->cur = (X + 1) % 3;
+    return CHAN_OK;
+}
+
+template <typename T>
+void enqueue_dt(chan_ctx* c)
+{
+    T** s = (T**)c->state[c->cur];
+    launch_dt<T>(s[0], s[1], s[2], c->md, c->d, c->part, c->nblk_red, c->sc, c->mins3, c->stream);
+}
+
+int set_device(chan_ctx* c)
+{
+    HIPCHK(c, hipSetDevice(c->device));
+    return CHAN_OK;
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+
+extern "C" {
+
+const char* chan_last_error(const chan_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int chan_create(const chan_config* cfg, chan_ctx** out)
+{
+    if (!cfg || !out) return CHAN_EINVAL;
+    *out = nullptr;
+    chan_ctx* c = new chan_ctx();
+    *out = c;
+    const int nx = cfg->nx, ny = cfg->ny, nz = cfg->nz;
+    // code/initialization.f90:66-74 restrictions, one domain (nprocs = 1)
+    if (nx % 2 != 0) return fail(c, CHAN_EINVAL, "Error: nx must be even");
+    if (nz % 2 != 0) return fail(c, CHAN_EINVAL, "Error: nz must be even");
+    if (nx < 6 || nz < 6 || ny < 4) return fail(c, CHAN_EINVAL, "Error: grid too small (nx,nz >= 6, ny >= 4)");
+    if (nz - 2 < 2) return fail(c, CHAN_EINVAL, "Error: nslices_z must be at least 2");
+    if (!cfg->x || !cfg->y || !cfg->z) return fail(c, CHAN_EINVAL, "Error: missing grid coordinates");
+    if (cfg->precision != CHAN_FP64 && cfg->precision != CHAN_FP32_BULK)
+        return fail(c, CHAN_EINVAL, "Error: invalid precision");
+    if (cfg->LES_model != 0 && cfg->LES_model != 2)
+        return fail(c, CHAN_EINVAL, "Error: LES_model must be 0 (none) or 2 (dynamic Smagorinsky)");
+    c->prec = cfg->precision;
+    c->device = cfg->device;
+    c->nu = cfg->nu; c->CFL = cfg->CFL; c->dPdx0 = cfg->dPdx;
+    c->x_mass_cte = cfg->x_mass_cte; c->LES_model = cfg->LES_model; c->Dirichlet_nu_t = cfg->Dirichlet_nu_t;
+    if (const char* e = getenv("CHANLES_NO_GRAPH")) c->use_graph = (e[0] == '0');
+
+    Dims& d = c->d;
+    d.nx = nx; d.ny = ny; d.nz = nz;
+    d.nxg = nx + 1; d.nyg = ny + 1; d.nzg = nz + 1;
+    d.nxp = nx - 2; d.nzp = nz - 2; d.nkx = d.nxp / 2 + 1;
+    d.sx = (d.nxg + 15) / 16 * 16;
+    d.sxy = (long)d.sx * d.nyg;
+    d.nbox = d.sxy * d.nzg;
+
+    // grids, code/initialization.f90:187-240 (1-based vectors, element 0 unused)
+    int nxg = d.nxg, nyg = d.nyg, nzg = d.nzg;
+    c->x.assign(nx + 2, 0.0); c->y.assign(ny + 2, 0.0); c->z.assign(nz + 2, 0.0);
+    for (int i = 1; i <= nx; i++) c->x[i] = cfg->x[i - 1];
+    for (int j = 1; j <= ny; j++) c->y[j] = cfg->y[j - 1];
+    for (int k = 1; k <= nz; k++) c->z[k] = cfg->z[k - 1];
+    auto centres = [](const std::vector<double>& f, int n, std::vector<double>& g) {
+        const int nm = n - 1;
+        std::vector<double> m(nm + 1);
+        for (int i = 1; i <= nm; i++) m[i] = 0.5 * (f[i] + f[i + 1]);
+        g.assign(nm + 3, 0.0);
+        for (int i = 1; i <= nm; i++) g[i + 1] = m[i];
+        g[1] = m[1] - 2.0 * (m[1] - f[1]);
+        g[nm + 2] = m[nm] + 2.0 * (f[n] - m[nm]);
+    };
+    centres(c->x, nx, c->xg); centres(c->y, ny, c->yg); centres(c->z, nz, c->zg);
+    auto minspacing = [](const std::vector<double>& g, int ng) {
+        double m = g[2] - g[1];
+        for (int i = 2; i < ng; i++) m = std::fmin(m, g[i + 1] - g[i]);
+        return m;
+    };
+    c->dxmin = minspacing(c->xg, nxg); c->dymin = minspacing(c->yg, nyg); c->dzmin = minspacing(c->zg, nzg);
+    for (int i = 1; i <= nx - 1; i++)
+        if (!(c->x[i + 1] > c->x[i])) return fail(c, CHAN_EINVAL, "Error: x faces must increase");
+    for (int j = 1; j <= ny - 1; j++)
+        if (!(c->y[j + 1] > c->y[j])) return fail(c, CHAN_EINVAL, "Error: y faces must increase");
+    for (int k = 1; k <= nz - 1; k++)
+        if (!(c->z[k + 1] > c->z[k])) return fail(c, CHAN_EINVAL, "Error: z faces must increase");
+
+    // modified wavenumbers, dx = dxmin (code/initialization.f90:250-308)
+    const double pi = 4.0 * std::atan(1.0);
+    auto modified = [&](int np, double dx, std::vector<double>& kk) {
+        kk.assign(np, 0.0);
+        int ch = (int)std::ceil((double)np / 2.0);
+        for (int i = 0; i <= ch && i < np; i++) kk[i] = 2.0 * (std::cos(2.0 * pi * (double)i / (double)np) - 1.0) / (dx * dx);
+        for (int i = ch + 1; i <= np - 1; i++)
+            kk[i] = 2.0 * (std::cos(2.0 * pi * (double)(-np + i) / (double)np) - 1.0) / (dx * dx);
+    };
+    modified(d.nxp, c->dxmin, c->kxx);
+    modified(d.nzp, c->dzmin, c->kzz);
+
+    // Dyy rows j=2..nyg-1 packed at 0..ny-2 (code/initialization.f90:355-385)
+    const std::vector<double>&Y = c->y, &YG = c->yg;
+    auto abc = [&](int j, double& a, double& b, double& cc) {
+        a = 1.0 / (Y[j] - Y[j - 1]) / (YG[j + 1] - YG[j]);
+        b = 1.0 / (Y[j] - Y[j - 1]) * (-1.0 / (YG[j + 1] - YG[j]) - 1.0 / (YG[j] - YG[j - 1]));
+        cc = 1.0 / (Y[j] - Y[j - 1]) / (YG[j] - YG[j - 1]);
+    };
+    const int nr = ny - 1;
+    c->dd.assign(nr, 0.0); c->dl.assign(nr, 0.0); c->du.assign(nr, 0.0);
+    for (int j = 3; j <= nyg - 2; j++) {
+        double a, b, cc;
+        abc(j, a, b, cc);
+        c->dd[j - 2] = b;
+        c->du[j - 2] = a;       // Dyy(j,j+1)
+        c->dl[j - 3] = cc;      // Dyy(j,j-1) = DL(j-1)
+    }
+    {
+        double a, b, cc;
+        abc(2, a, b, cc);
+        c->dd[0] = b + cc;
+        c->du[0] = a;
+        abc(nyg - 1, a, b, cc);
+        c->dd[nr - 1] = a + b;
+        c->dl[nr - 2] = cc;
+    }
+    c->w0.assign(ny + 2, 0.0); c->w1.assign(ny + 2, 0.0);
+    for (int j = 1; j <= ny; j++) {
+        c->w0[j] = (YG[j + 1] - Y[j]) / (YG[j + 1] - YG[j]);
+        c->w1[j] = 1.0 - c->w0[j];
+    }
+    // Wray RK3 (code/initialization.f90:415-427)
+    c->rk_t[0] = 8.0 / 15.0; c->rk_t[1] = 2.0 / 3.0; c->rk_t[2] = 1.0;
+    std::memset(c->rk_coef, 0, sizeof(c->rk_coef));
+    c->rk_coef[0][0] = 8.0 / 15.0;
+    c->rk_coef[1][0] = 1.0 / 4.0; c->rk_coef[1][1] = 5.0 / 12.0;
+    c->rk_coef[2][0] = 1.0 / 4.0; c->rk_coef[2][1] = 0.0; c->rk_coef[2][2] = 3.0 / 4.0;
+
+    FftPlan fx{}, fz{};
+    if (!factor_plan(d.nxp, fx) || !factor_plan(d.nzp, fz))
+        return fail(c, CHAN_EINVAL, "Error: nx-2 and nz-2 must factor into 2, 3 and 5");
+
+    int rc = set_device(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+
+    // metrics
+    if ((rc = upload(c, c->x, &c->md.x)) || (rc = upload(c, c->y, &c->md.y)) || (rc = upload(c, c->z, &c->md.z)) ||
+        (rc = upload(c, c->xg, &c->md.xg)) || (rc = upload(c, c->yg, &c->md.yg)) ||
+        (rc = upload(c, c->zg, &c->md.zg)) || (rc = upload(c, c->w0, &c->md.w0)) ||
+        (rc = upload(c, c->w1, &c->md.w1)))
+        return rc;
+    if ((rc = upload_f(c, c->x, &c->mf.x)) || (rc = upload_f(c, c->y, &c->mf.y)) ||
+        (rc = upload_f(c, c->z, &c->mf.z)) || (rc = upload_f(c, c->xg, &c->mf.xg)) ||
+        (rc = upload_f(c, c->yg, &c->mf.yg)) || (rc = upload_f(c, c->zg, &c->mf.zg)) ||
+        (rc = upload_f(c, c->w0, &c->mf.w0)) || (rc = upload_f(c, c->w1, &c->mf.w1)))
+        return rc;
+
+    const size_t fb = (size_t)d.nbox * esize(c);
+    for (int b = 0; b < 3; b++)
+        for (int q = 0; q < 3; q++)
+            if ((rc = dalloc(c, &c->state[b][q], fb))) return rc;
+    for (int q = 0; q < 3; q++) {
+        if ((rc = dalloc(c, &c->F1[q], fb))) return rc;
+        if ((rc = dalloc(c, &c->rhs[q], fb))) return rc;
+    }
+    if ((rc = dalloc(c, &c->nut, fb))) return rc;
+    if (c->LES_model == 2)
+        for (int q = 0; q < 13; q++)
+            if ((rc = dalloc(c, &c->dsm[q], fb))) return rc;
+    // partial sums: DSM rows use one slot per (x,z) block of the L:M kernel
+    c->nblk_dsm = cdiv(d.nxg - 2, 64) * cdiv(d.nzg - 2, 4);
+    c->nblk_red = 1024;
+    size_t npart = std::max((size_t)(d.nyg + 1) * 2 * c->nblk_dsm, (size_t)3 * c->nblk_red);
+    if ((rc = dalloc(c, (void**)&c->part, npart * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, (void**)&c->ratio, (size_t)3 * d.nyg * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, (void**)&c->alpha, (size_t)d.nx * 2 * d.nzg * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, (void**)&c->stage, (size_t)d.nbox * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, (void**)&c->sc, sizeof(Scalars)))) return rc;
+    if ((rc = dalloc(c, (void**)&c->pb.p, (size_t)d.nbox * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, (void**)&c->pb.ph, (size_t)(d.ny - 1) * d.nzp * d.nkx * sizeof(double2)))) return rc;
+    if ((rc = upload(c, c->kxx, &c->pb.kxx)) || (rc = upload(c, c->kzz, &c->pb.kzz)) ||
+        (rc = upload(c, c->dd, &c->pb.dd)) || (rc = upload(c, c->dl, &c->pb.dl)) || (rc = upload(c, c->du, &c->pb.du)))
+        return rc;
+    std::vector<double> mins = {c->dxmin, c->dymin, c->dzmin};
+    const double* mp;
+    if ((rc = upload(c, mins, &mp))) return rc;
+    c->mins3 = (double*)mp;
+    c->pb.fx = fx;
+    c->pb.fz = fz;
+    if ((rc = make_twiddles(c, c->pb.fx)) || (rc = make_twiddles(c, c->pb.fz))) return rc;
+    c->pb.bx = std::max(1, std::min(16, 2048 / d.nxp));
+    c->pb.bz = std::max(1, std::min(16, 2048 / d.nzp));
+    Scalars s0{};
+    s0.dt = 0.0; s0.t = 0.0; s0.dPdx = cfg->dPdx; s0.Qflow_x_0 = 0.0;
+    s0.cfl = cfg->CFL; s0.nu = cfg->nu; s0.dt_max = 1.0e-3;
+    HIPCHK(c, hipMemcpy(c->sc, &s0, sizeof(s0), hipMemcpyHostToDevice));
+    HIPCHK(c, hipDeviceSynchronize());
+    return CHAN_OK;
+}
+
+void chan_destroy(chan_ctx* c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    for (int q = 0; q < 3; q++)
+        if (c->graph[q]) (void)hipGraphExecDestroy(c->graph[q]);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int chan_set_field(chan_ctx* c, int which, const double* src)
+{
+    if (!c || !src) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    Shape s = field_shape(c, which);
+    if (s.n1 <= 0) return fail(c, CHAN_EINVAL, "chan_set_field: unknown field");
+    const size_t n = (size_t)s.n1 * s.n2 * s.n3;
+    if (which == CHAN_ALPHA_X) {
+        HIPCHK(c, hipMemcpyAsync(c->alpha, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return CHAN_OK;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->stage, src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    void* dst = field_ptr(c, which);
+    if (which == CHAN_P || c->prec == CHAN_FP64)
+        k_unpack<double><<<cdiv(n, 256), 256, 0, c->stream>>>(c->stage, (double*)dst, c->d, s.n1, s.n2, s.n3, s.o);
+    else
+        k_unpack<float><<<cdiv(n, 256), 256, 0, c->stream>>>(c->stage, (float*)dst, c->d, s.n1, s.n2, s.n3, s.o);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHAN_OK;
+}
+
+int chan_get_field(chan_ctx* c, int which, double* dst)
+{
+    if (!c || !dst) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    Shape s = field_shape(c, which);
+    if (s.n1 <= 0) return fail(c, CHAN_EINVAL, "chan_get_field: unknown field");
+    const size_t n = (size_t)s.n1 * s.n2 * s.n3;
+    if (which == CHAN_ALPHA_X) {
+        HIPCHK(c, hipMemcpyAsync(dst, c->alpha, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return CHAN_OK;
+    }
+    void* src = field_ptr(c, which);
+    if (which == CHAN_P || c->prec == CHAN_FP64)
+        k_pack<double><<<cdiv(n, 256), 256, 0, c->stream>>>((const double*)src, c->stage, c->d, s.n1, s.n2, s.n3, s.o);
+    else
+        k_pack<float><<<cdiv(n, 256), 256, 0, c->stream>>>((const float*)src, c->stage, c->d, s.n1, s.n2, s.n3, s.o);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(dst, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHAN_OK;
+}
+
+int chan_set_scalars(chan_ctx* c, double dt, double t, double dPdx, double Q)
+{
+    if (!c) return CHAN_EINVAL;
+    Scalars s0{};
+    int rc = set_device(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(&s0, c->sc, sizeof(s0), hipMemcpyDeviceToHost));
+    s0.dt = dt; s0.t = t; s0.dPdx = dPdx; s0.Qflow_x_0 = Q;
+    HIPCHK(c, hipMemcpy(c->sc, &s0, sizeof(s0), hipMemcpyHostToDevice));
+    return CHAN_OK;
+}
+
+int chan_get_scalars(chan_ctx* c, double out[4])
+{
+    if (!c || !out) return CHAN_EINVAL;
+    Scalars s0{};
+    int rc = set_device(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(&s0, c->sc, sizeof(s0), hipMemcpyDeviceToHost));
+    out[0] = s0.dt; out[1] = s0.t; out[2] = s0.dPdx; out[3] = s0.Qflow_x_0;
+    return CHAN_OK;
+}
+
+#define DISPATCH(c, fn, ...) ((c)->prec == CHAN_FP64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+
+int chan_apply_boundary_conditions(chan_ctx* c)
+{
+    if (!c) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    DISPATCH(c, enqueue_bc, c, c->cur);
+    return check_launch(c);
+}
+
+int chan_compute_eddy_viscosity(chan_ctx* c)
+{
+    if (!c) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    DISPATCH(c, enqueue_eddy_viscosity, c, c->cur, 0);
+    return check_launch(c);
+}
+
+int chan_compute_wall_model(chan_ctx* c)
+{
+    if (!c) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    if (c->prec == CHAN_FP64)
+        launch_wall_model<double>((double*)c->state[c->cur][0], (double*)c->nut, c->alpha, c->md, c->d, c->nu,
+                                  c->stream);
+    else
+        launch_wall_model<float>((float*)c->state[c->cur][0], (float*)c->nut, c->alpha, c->mf, c->d, (float)c->nu,
+                                 c->stream);
+    return check_launch(c);
+}
+
+int chan_compute_rhs(chan_ctx* c)
+{
+    if (!c) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    void** s = c->state[c->cur];
+    if (c->prec == CHAN_FP64) {
+        double** S = (double**)s;
+        launch_rhs_rk<double>(S[0], S[1], S[2], (double*)c->nut, c->md, c->d, c->nu, c->sc, 0, nullptr, nullptr,
+                              nullptr, nullptr, nullptr, nullptr, (double*)c->rhs[0], (double*)c->rhs[1],
+                              (double*)c->rhs[2], 0.0, 0.0, c->stream);
+    } else {
+        float** S = (float**)s;
+        launch_rhs_rk<float>(S[0], S[1], S[2], (float*)c->nut, c->mf, c->d, (float)c->nu, c->sc, 0, nullptr, nullptr,
+                             nullptr, nullptr, nullptr, nullptr, (float*)c->rhs[0], (float*)c->rhs[1],
+                             (float*)c->rhs[2], 0.0, 0.0, c->stream);
+    }
+    return check_launch(c);
+}
+
+int chan_compute_projection_step(chan_ctx* c)
+{
+    if (!c) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    DISPATCH(c, enqueue_projection, c, c->cur);
+    return check_launch(c);
+}
+
+int chan_solve_poisson_equation(chan_ctx* c)
+{
+    if (!c) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    launch_poisson_forward_field(c->pb.p, c->d, c->pb, c->stream);
+    launch_poisson_solve(c->d, c->pb, c->stream);
+    launch_poisson_inverse(c->d, c->pb, c->stream);
+    return check_launch(c);
+}
+
+int chan_initialize_mass_flow(chan_ctx* c)
+{
+    if (!c) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    if (c->prec == CHAN_FP64)
+        launch_mass_flow_init<double>((double*)c->state[c->cur][0], c->md, c->d, c->part, c->nblk_red, c->sc,
+                                      c->stream);
+    else
+        launch_mass_flow_init<float>((float*)c->state[c->cur][0], c->md, c->d, c->part, c->nblk_red, c->sc,
+                                     c->stream);
+    return check_launch(c);
+}
+
+int chan_compute_mean_mass_flow_U(chan_ctx* c, double* Q)
+{
+    if (!c || !Q) return CHAN_EINVAL;
+    double keep[4];
+    int rc = chan_get_scalars(c, keep);
+    if (rc) return rc;
+    if ((rc = chan_initialize_mass_flow(c))) return rc;
+    double now[4];
+    if ((rc = chan_get_scalars(c, now))) return rc;
+    *Q = now[3];
+    return chan_set_scalars(c, keep[0], keep[1], keep[2], keep[3]);
+}
+
+int chan_compute_dPx_for_constant_mass_flow(chan_ctx* c, double* dPdx)
+{
+    if (!c) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    double keep[4];
+    if ((rc = chan_get_scalars(c, keep))) return rc;
+    if (c->prec == CHAN_FP64)
+        launch_mass_flow_correct<double>((double*)c->state[c->cur][0], c->d, c->part, c->nblk_red, c->sc, 0,
+                                         c->stream);
+    else
+        launch_mass_flow_correct<float>((float*)c->state[c->cur][0], c->d, c->part, c->nblk_red, c->sc, 0, c->stream);
+    if ((rc = check_launch(c))) return rc;
+    Scalars s0{};
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(&s0, c->sc, sizeof(s0), hipMemcpyDeviceToHost));
+    if (dPdx) *dPdx = s0.last_dPdx;
+    return chan_set_scalars(c, keep[0], keep[1], keep[2], keep[3]);
+}
+
+int chan_compute_dt(chan_ctx* c, double* dt)
+{
+    if (!c) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    DISPATCH(c, enqueue_dt, c);
+    if ((rc = check_launch(c))) return rc;
+    if (dt) {
+        double s[4];
+        if ((rc = chan_get_scalars(c, s))) return rc;
+        *dt = s[0];
+    }
+    return CHAN_OK;
+}
+
+int chan_compute_time_step_RK3(chan_ctx* c)
+{
+    if (!c) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    return DISPATCH(c, rk3_step, c);
+}
+
+int chan_check_divergence(chan_ctx* c, double* max_div, double* max_abs)
+{
+    if (!c) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    void** s = c->state[c->cur];
+    if (c->prec == CHAN_FP64)
+        launch_divergence_check<double>((double*)s[0], (double*)s[1], (double*)s[2], c->md, c->d, c->part,
+                                        c->nblk_red, c->sc, c->stream);
+    else
+        launch_divergence_check<float>((float*)s[0], (float*)s[1], (float*)s[2], c->md, c->d, c->part, c->nblk_red,
+                                       c->sc, c->stream);
+    if ((rc = check_launch(c))) return rc;
+    Scalars s0{};
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(&s0, c->sc, sizeof(s0), hipMemcpyDeviceToHost));
+    if (max_div) *max_div = s0.maxdiv;
+    if (max_abs) *max_abs = s0.maxabsdiv;
+    return CHAN_OK;
+}
+
+int chan_set_wall_model(chan_ctx* c, int mode, double h_wm, double kappa, double B)
+{
+    if (!c) return CHAN_EINVAL;
+    if (mode != CHAN_WALL_GIVEN && mode != CHAN_WALL_EQUILIBRIUM) return fail(c, CHAN_EINVAL, "invalid wall mode");
+    if (!(kappa > 0)) return fail(c, CHAN_EINVAL, "kappa must be positive");
+    c->wall_mode = mode; c->h_wm = h_wm; c->kappa = kappa; c->Bc = B;
+    for (int q = 0; q < 3; q++)
+        if (c->graph[q]) { (void)hipGraphExecDestroy(c->graph[q]); c->graph[q] = nullptr; }
+    return CHAN_OK;
+}
+
+int chan_compute_equilibrium_wall_stress(chan_ctx* c, double h_wm, double kappa, double B)
+{
+    if (!c) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    if (c->prec == CHAN_FP64)
+        launch_equilibrium_alpha<double>((double*)c->state[c->cur][0], c->alpha, c->md, c->d, c->nu, h_wm, kappa, B,
+                                         c->stream);
+    else
+        launch_equilibrium_alpha<float>((float*)c->state[c->cur][0], c->alpha, c->md, c->d, c->nu, h_wm, kappa, B,
+                                        c->stream);
+    return check_launch(c);
+}
+
+int chan_plane_statistics(chan_ctx* c, double* out)
+{
+    if (!c || !out) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    void** s = c->state[c->cur];
+    if (c->prec == CHAN_FP64)
+        launch_plane_stats<double>((double*)s[0], (double*)s[1], (double*)s[2], c->md, c->d, c->stage, c->stream);
+    else
+        launch_plane_stats<float>((float*)s[0], (float*)s[1], (float*)s[2], c->md, c->d, c->stage, c->stream);
+    if ((rc = check_launch(c))) return rc;
+    HIPCHK(c, hipMemcpyAsync(out, c->stage, (size_t)7 * c->d.ny * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHAN_OK;
+}
+
+int chan_run_steps(chan_ctx* c, int nsteps)
+{
+    if (!c || nsteps < 0) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    for (int n = 0; n < nsteps; n++) {
+        DISPATCH(c, enqueue_dt, c);
+        if ((rc = check_launch(c))) return rc;
+        if ((rc = chan_compute_time_step_RK3(c))) return rc;
+    }
+    return CHAN_OK;
+}
+
+int chan_synchronize(chan_ctx* c)
+{
+    if (!c) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHAN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+// kernels.hpp -- host-callable launchers of the channel kernels.
+#pragma once
+#include "common.hpp"
+
+namespace chan {
+
+// field kinds for the ghost fill
+enum FieldKind { KIND_U = 0, KIND_V = 1, KIND_W = 2, KIND_C_DIRICHLET = 3, KIND_C_NEUMANN = 4 };
+
+// Work buffers of the dynamic Smagorinsky model (box layout, type T).
+template <typename T>
+struct DsmBufs {
+    T* ac[3];     // centre velocities
+    T* ss[6];     // |S| S_ij
+    T* smag;      // |S|
+    T* uf; T* vf; T* wf;  // filtered staggered velocities
+    double* part;  // per-block partial sums: [nyg][2][nblk]
+    double* ratio; // <LM>/<MM> per row (index j-1); also LM, MM at +nyg, +2nyg
+    int nblk;      // partial-sum blocks per row
+};
+
+template <typename T>
+void launch_bc(T* F, const Dims& d, int kind, hipStream_t s);
+
+// mode 0: rhs only into R*; mode 1..3: RK substep (Wray, code/time_integration.f90:34-86)
+template <typename T>
+void launch_rhs_rk(const T* U, const T* V, const T* W, const T* nut, const Metric<T>& m, const Dims& d,
+                   T nu, const Scalars* sc, int mode, const T* Uo, const T* Vo, const T* Wo, T* F1u,
+                   T* F1v, T* F1w, T* Ou, T* Ov, T* Ow, double c1, double c2, hipStream_t s);
+
+template <typename T>
+void launch_dsm(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, const Dims& d,
+                DsmBufs<T>& b, int dirichlet, int fuse_wm, const double* alpha_x, T nu, hipStream_t s);
+
+template <typename T>
+void launch_wall_model(const T* U, T* nut, const double* alpha_x, const Metric<T>& m, const Dims& d, T nu,
+                       hipStream_t s);
+
+template <typename T>
+void launch_equilibrium_alpha(const T* U, double* alpha_x, const Metric<double>& md, const Dims& d, double nu,
+                              double h_wm, double kappa, double B, hipStream_t s);
+
+template <typename T>
+void launch_mass_flow_init(const T* U, const Metric<double>& md, const Dims& d, double* part, int nblk,
+                           Scalars* sc, hipStream_t s);
+
+template <typename T>
+void launch_mass_flow_correct(T* U, const Dims& d, double* part, int nblk, Scalars* sc, int apply,
+                              hipStream_t s);
+
+template <typename T>
+void launch_dt(const T* U, const T* V, const T* W, const Metric<double>& md, const Dims& d, double* part,
+               int nblk, Scalars* sc, const double* mins3, hipStream_t s);
+
+template <typename T>
+void launch_divergence_check(const T* U, const T* V, const T* W, const Metric<double>& md, const Dims& d,
+                             double* part, int nblk, Scalars* sc, hipStream_t s);
+
+template <typename T>
+void launch_plane_stats(const T* U, const T* V, const T* W, const Metric<double>& md, const Dims& d,
+                        double* out, hipStream_t s);
+
+template <typename T>
+void launch_scale_shift(T* F, const Dims& d, hipStream_t s);
+
+// ---------------------------------------------------------------- Poisson
+struct FftPlan {
+    int n;          // transform length
+    int nr;         // number of radix passes
+    int r[12];      // radices, first pass first
+    const double2* tw;  // exp(-2 pi i t / n), t = 0..n-1 (device)
+};
+
+struct PoissonBufs {
+    double2* ph;        // spectral pressure [ny-1][nzp][nkx]
+    double* p;          // pseudo-pressure (box layout, double)
+    const double* kxx;  // nxp
+    const double* kzz;  // nzp
+    const double* dd;   // Dyy diagonal, rows j=2..nyg-1 at [0..ny-2]
+    const double* dl;   // Dyy(j+1,j), j=2..nyg-2
+    const double* du;   // Dyy(j,j+1)
+    FftPlan fx, fz;
+    int bx, bz;         // transforms per block in the x and z passes
+};
+
+template <typename T>
+void launch_poisson_forward(const T* U, const T* V, const T* W, const Metric<double>& md, const Dims& d,
+                            PoissonBufs& pb, hipStream_t s);
+void launch_poisson_solve(const Dims& d, PoissonBufs& pb, hipStream_t s);
+void launch_poisson_inverse(const Dims& d, PoissonBufs& pb, hipStream_t s);
+// forward x pass of a given centre field (for the standalone Poisson solve)
+void launch_poisson_forward_field(const double* rhs, const Dims& d, PoissonBufs& pb, hipStream_t s);
+
+template <typename T>
+void launch_project(T* U, T* V, T* W, const double* p, const Metric<double>& md, const Dims& d, hipStream_t s);
+
+}  // namespace chan

@@ -1,0 +1,907 @@
+// stencil.hip -- ghost fill, RHS + Wray RK3 combine, dynamic Smagorinsky,
+// wall model, mass-flow forcing, dt and divergence reductions.
+//
+// FP64 arithmetic keeps the reference's operation order and is compiled
+// without FMA contraction, so the stencil stages reproduce the reference's
+// rounding.  Reductions are two-level (fixed block partials, then one block
+// in fixed order) and therefore deterministic from run to run.
+#include "kernels.hpp"
+
+namespace chan {
+
+static constexpr int RED_THREADS = 256;
+
+// ------------------------------------------------------------ ghost fill
+
+// code/boundary_conditions.f90:20-48 as one pass: the reference applies
+// periodic x, periodic z, then the y wall condition; the composition maps
+// every ghost cell to sign * F(interior image).  Sources are never ghosts,
+// so the pass is free of read/write races.
+template <typename T>
+__global__ void k_bc(T* F, Dims d, int kind)
+{
+    const bool xf = kind == KIND_U, zf = kind == KIND_W, yf = kind == KIND_V;
+    const int n1 = xf ? d.nx : d.nxg, n2 = yf ? d.ny : d.nyg, n3 = zf ? d.nz : d.nzg;
+    const int gx = xf ? 2 : 3, gz = zf ? 2 : 3;
+    const long ny_cnt = 2L * n1 * n3;
+    const long nz_cnt = (long)gz * n1 * (n2 - 2);
+    const long nx_cnt = (long)gx * (n2 - 2) * (n3 - gz);
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    auto ghost_x = [&](int g) { return g == 0 ? 1 : (xf ? n1 : (g == 1 ? n1 - 1 : n1)); };
+    auto ghost_z = [&](int g) { return g == 0 ? 1 : (zf ? n3 : (g == 1 ? n3 - 1 : n3)); };
+    if (q < ny_cnt) {
+        i = (int)(q % n1) + 1;
+        long r = q / n1;
+        k = (int)(r % n3) + 1;
+        j = (r / n3) == 0 ? 1 : n2;
+    } else if ((q -= ny_cnt) < nz_cnt) {
+        i = (int)(q % n1) + 1;
+        long r = q / n1;
+        j = (int)(r % (n2 - 2)) + 2;
+        k = ghost_z((int)(r / (n2 - 2)));
+    } else if ((q -= nz_cnt) < nx_cnt) {
+        j = (int)(q % (n2 - 2)) + 2;
+        long r = q / (n2 - 2);
+        k = (int)(r % (n3 - gz)) + 2;
+        i = ghost_x((int)(r / (n3 - gz)));
+    } else {
+        return;
+    }
+    const int si = xf ? img_face(i, n1) : img_centre(i, n1);
+    const int sk = zf ? img_face(k, n3) : img_centre(k, n3);
+    int sj = j;
+    T sign = T(1);
+    if (j == 1 || j == n2) {
+        if (yf) { F[at(d, i, j, k)] = T(0); return; }
+        sj = (j == 1) ? 2 : n2 - 1;
+        sign = (kind == KIND_C_NEUMANN) ? T(1) : T(-1);
+    }
+    F[at(d, i, j, k)] = sign * F[at(d, si, sj, sk)];
+}
+
+template <typename T>
+void launch_bc(T* F, const Dims& d, int kind, hipStream_t s)
+{
+    const bool xf = kind == KIND_U, zf = kind == KIND_W, yf = kind == KIND_V;
+    const int n1 = xf ? d.nx : d.nxg, n2 = yf ? d.ny : d.nyg, n3 = zf ? d.nz : d.nzg;
+    const int gx = xf ? 2 : 3, gz = zf ? 2 : 3;
+    long tot = 2L * n1 * n3 + (long)gz * n1 * (n2 - 2) + (long)gx * (n2 - 2) * (n3 - gz);
+    k_bc<T><<<cdiv(tot, 256), 256, 0, s>>>(F, d, kind);
+}
+
+// ------------------------------------------------------------------- RHS
+
+// code/equations.f90:24-167
+template <typename T>
+__device__ __forceinline__ T rhs_u_at(const T* U, const T* V, const T* W, const T* nt, const Metric<T>& m,
+                                      const Dims& d, T nu, T dPdx, int i, int j, int k)
+{
+    auto u = [&](int a, int b, int c) { return U[at(d, a, b, c)]; };
+    auto v = [&](int a, int b, int c) { return V[at(d, a, b, c)]; };
+    auto w = [&](int a, int b, int c) { return W[at(d, a, b, c)]; };
+    auto n = [&](int a, int b, int c) { return nt[at(d, a, b, c)]; };
+    const T h = T(0.5), q = T(0.25), one = T(1);
+    const T *x = m.x, *y = m.y, *z = m.z, *xg = m.xg, *yg = m.yg, *zg = m.zg, *w0 = m.w0, *w1 = m.w1;
+    T uc = h * (u(i, j, k) + u(i + 1, j, k));
+    T ucm = h * (u(i - 1, j, k) + u(i, j, k));
+    T t1 = -(uc * uc - ucm * ucm) / (xg[i + 1] - xg[i]);
+    T uvp = (w0[j] * u(i, j, k) + w1[j] * u(i, j + 1, k)) * (h * (v(i, j, k) + v(i + 1, j, k)));
+    T uvm = (w0[j - 1] * u(i, j - 1, k) + w1[j - 1] * u(i, j, k)) * (h * (v(i, j - 1, k) + v(i + 1, j - 1, k)));
+    T t2 = -(uvp - uvm) / (y[j] - y[j - 1]);
+    T uwp = (h * (u(i, j, k) + u(i, j, k + 1))) * (h * (w(i, j, k) + w(i + 1, j, k)));
+    T uwm = (h * (u(i, j, k - 1) + u(i, j, k))) * (h * (w(i, j, k - 1) + w(i + 1, j, k - 1)));
+    T t3 = -(uwp - uwm) / (z[k] - z[k - 1]);
+    T dz_1 = zg[k] - zg[k - 1], dz_2 = zg[k + 1] - zg[k], dz_3 = z[k] - z[k - 1];
+    T dy_3 = y[j] - y[j - 1];
+    T dx_1 = x[i] - x[i - 1], dx_2 = x[i + 1] - x[i], dx_3 = xg[i + 1] - xg[i];
+    T nu_x1 = nu + n(i, j, k);
+    T nu_x2 = nu + n(i + 1, j, k);
+    T nu_y1 = nu + h * (w0[j - 1] * n(i, j - 1, k) + w1[j - 1] * n(i, j, k) + w0[j - 1] * n(i + 1, j - 1, k) +
+                        w1[j - 1] * n(i + 1, j, k));
+    T nu_y2 = nu + h * (w0[j] * n(i, j, k) + w1[j] * n(i, j + 1, k) + w0[j] * n(i + 1, j, k) +
+                        w1[j] * n(i + 1, j + 1, k));
+    T nu_z1 = nu + q * (n(i, j, k) + n(i, j, k - 1) + n(i + 1, j, k) + n(i + 1, j, k - 1));
+    T nu_z2 = nu + q * (n(i, j, k) + n(i, j, k + 1) + n(i + 1, j, k) + n(i + 1, j, k + 1));
+    T dudy_p = (u(i, j + 1, k) - u(i, j, k)) / (yg[j + 1] - yg[j]);
+    T dudy_m = (u(i, j, k) - u(i, j - 1, k)) / (yg[j] - yg[j - 1]);
+    T t4 = one / dx_3 * (nu_x2 * one / dx_2 * (u(i + 1, j, k) - u(i, j, k)) -
+                         nu_x1 * one / dx_1 * (u(i, j, k) - u(i - 1, j, k))) +
+           one / dy_3 * (nu_y2 * dudy_p - nu_y1 * dudy_m) +
+           one / dz_3 * (nu_z2 * one / dz_2 * (u(i, j, k + 1) - u(i, j, k)) -
+                         nu_z1 * one / dz_1 * (u(i, j, k) - u(i, j, k - 1)));
+    return (((t1 + t2) + t3) + t4) + dPdx;
+}
+
+// code/equations.f90:174-320
+template <typename T>
+__device__ __forceinline__ T rhs_v_at(const T* U, const T* V, const T* W, const T* nt, const Metric<T>& m,
+                                      const Dims& d, T nu, int i, int j, int k)
+{
+    auto u = [&](int a, int b, int c) { return U[at(d, a, b, c)]; };
+    auto v = [&](int a, int b, int c) { return V[at(d, a, b, c)]; };
+    auto w = [&](int a, int b, int c) { return W[at(d, a, b, c)]; };
+    auto n = [&](int a, int b, int c) { return nt[at(d, a, b, c)]; };
+    const T h = T(0.5), one = T(1);
+    const T *x = m.x, *y = m.y, *z = m.z, *xg = m.xg, *yg = m.yg, *zg = m.zg, *w0 = m.w0, *w1 = m.w1;
+    T vc = h * (v(i, j, k) + v(i, j + 1, k));
+    T vcm = h * (v(i, j - 1, k) + v(i, j, k));
+    T t1 = -(vc * vc - vcm * vcm) / (yg[j + 1] - yg[j]);
+    T uvp = (w0[j] * u(i, j, k) + w1[j] * u(i, j + 1, k)) * (h * (v(i, j, k) + v(i + 1, j, k)));
+    T uvm = (w0[j] * u(i - 1, j, k) + w1[j] * u(i - 1, j + 1, k)) * (h * (v(i - 1, j, k) + v(i, j, k)));
+    T t2 = -(uvp - uvm) / (x[i] - x[i - 1]);
+    T vwp = (h * (v(i, j, k) + v(i, j, k + 1))) * (w0[j] * w(i, j, k) + w1[j] * w(i, j + 1, k));
+    T vwm = (h * (v(i, j, k - 1) + v(i, j, k))) * (w0[j] * w(i, j, k - 1) + w1[j] * w(i, j + 1, k - 1));
+    T t3 = -(vwp - vwm) / (z[k] - z[k - 1]);
+    T dz_1 = zg[k] - zg[k - 1], dz_2 = zg[k + 1] - zg[k], dz_3 = z[k] - z[k - 1];
+    T dy_1 = y[j] - y[j - 1], dy_2 = y[j + 1] - y[j], dy_3 = yg[j + 1] - yg[j];
+    T dx_1 = xg[i] - xg[i - 1], dx_2 = xg[i + 1] - xg[i], dx_3 = x[i] - x[i - 1];
+    T nu_x1 = nu + h * (w0[j] * n(i - 1, j, k) + w1[j] * n(i - 1, j + 1, k) + w0[j] * n(i, j, k) +
+                        w1[j] * n(i, j + 1, k));
+    T nu_x2 = nu + h * (w0[j] * n(i, j, k) + w1[j] * n(i, j + 1, k) + w0[j] * n(i + 1, j, k) +
+                        w1[j] * n(i + 1, j + 1, k));
+    T nu_y1 = nu + n(i, j, k);
+    T nu_y2 = nu + n(i, j + 1, k);
+    T nu_z1 = nu + h * (w0[j] * n(i, j, k - 1) + w1[j] * n(i, j + 1, k - 1) + w0[j] * n(i, j, k) +
+                        w1[j] * n(i, j + 1, k));
+    T nu_z2 = nu + h * (w0[j] * n(i, j, k) + w1[j] * n(i, j + 1, k) + w0[j] * n(i, j, k + 1) +
+                        w1[j] * n(i, j + 1, k + 1));
+    T t4 = one / dx_3 * (nu_x2 * one / dx_2 * (v(i + 1, j, k) - v(i, j, k)) -
+                         nu_x1 * one / dx_1 * (v(i, j, k) - v(i - 1, j, k))) +
+           one / dy_3 * (nu_y2 * one / dy_2 * (v(i, j + 1, k) - v(i, j, k)) -
+                         nu_y1 * one / dy_1 * (v(i, j, k) - v(i, j - 1, k))) +
+           one / dz_3 * (nu_z2 * one / dz_2 * (v(i, j, k + 1) - v(i, j, k)) -
+                         nu_z1 * one / dz_1 * (v(i, j, k) - v(i, j, k - 1)));
+    return ((t1 + t2) + t3) + t4;
+}
+
+// code/equations.f90:327-468
+template <typename T>
+__device__ __forceinline__ T rhs_w_at(const T* U, const T* V, const T* W, const T* nt, const Metric<T>& m,
+                                      const Dims& d, T nu, int i, int j, int k)
+{
+    auto u = [&](int a, int b, int c) { return U[at(d, a, b, c)]; };
+    auto v = [&](int a, int b, int c) { return V[at(d, a, b, c)]; };
+    auto w = [&](int a, int b, int c) { return W[at(d, a, b, c)]; };
+    auto n = [&](int a, int b, int c) { return nt[at(d, a, b, c)]; };
+    const T h = T(0.5), q = T(0.25), one = T(1);
+    const T *x = m.x, *y = m.y, *z = m.z, *xg = m.xg, *yg = m.yg, *zg = m.zg, *w0 = m.w0, *w1 = m.w1;
+    T wc = h * (w(i, j, k) + w(i, j, k + 1));
+    T wcm = h * (w(i, j, k - 1) + w(i, j, k));
+    T t1 = -(wc * wc - wcm * wcm) / (zg[k + 1] - zg[k]);
+    T uwp = (h * (u(i, j, k) + u(i, j, k + 1))) * (h * (w(i, j, k) + w(i + 1, j, k)));
+    T uwm = (h * (u(i - 1, j, k) + u(i - 1, j, k + 1))) * (h * (w(i - 1, j, k) + w(i, j, k)));
+    T t2 = -(uwp - uwm) / (x[i] - x[i - 1]);
+    T vwp = (h * (v(i, j, k) + v(i, j, k + 1))) * (w0[j] * w(i, j, k) + w1[j] * w(i, j + 1, k));
+    T vwm = (h * (v(i, j - 1, k) + v(i, j - 1, k + 1))) * (w0[j - 1] * w(i, j - 1, k) + w1[j - 1] * w(i, j, k));
+    T t3 = -(vwp - vwm) / (y[j] - y[j - 1]);
+    T dz_1 = z[k] - z[k - 1], dz_2 = z[k + 1] - z[k], dz_3 = zg[k + 1] - zg[k];
+    T dy_3 = y[j] - y[j - 1];
+    T dx_1 = xg[i] - xg[i - 1], dx_2 = xg[i + 1] - xg[i], dx_3 = x[i] - x[i - 1];
+    T nu_x1 = nu + q * (n(i, j, k) + n(i, j, k + 1) + n(i - 1, j, k) + n(i - 1, j, k + 1));
+    T nu_x2 = nu + q * (n(i, j, k) + n(i, j, k + 1) + n(i + 1, j, k) + n(i + 1, j, k + 1));
+    T nu_y1 = nu + h * (w0[j - 1] * n(i, j - 1, k) + w1[j - 1] * n(i, j, k) + w0[j - 1] * n(i, j - 1, k + 1) +
+                        w1[j - 1] * n(i, j, k + 1));
+    T nu_y2 = nu + h * (w0[j] * n(i, j, k) + w1[j] * n(i, j + 1, k) + w0[j] * n(i, j, k + 1) +
+                        w1[j] * n(i, j + 1, k + 1));
+    T nu_z1 = nu + n(i, j, k);
+    T nu_z2 = nu + n(i, j, k + 1);
+    T dwdy_p = (w(i, j + 1, k) - w(i, j, k)) / (yg[j + 1] - yg[j]);
+    T dwdy_m = (w(i, j, k) - w(i, j - 1, k)) / (yg[j] - yg[j - 1]);
+    T t4 = one / dx_3 * (nu_x2 * one / dx_2 * (w(i + 1, j, k) - w(i, j, k)) -
+                         nu_x1 * one / dx_1 * (w(i, j, k) - w(i - 1, j, k))) +
+           one / dy_3 * (nu_y2 * dwdy_p - nu_y1 * dwdy_m) +
+           one / dz_3 * (nu_z2 * one / dz_2 * (w(i, j, k + 1) - w(i, j, k)) -
+                         nu_z1 * one / dz_1 * (w(i, j, k) - w(i, j, k - 1)));
+    return ((t1 + t2) + t3) + t4;
+}
+
+// One thread per box cell computes whichever of rhs_u/v/w is defined there.
+// mode 0 writes the right-hand sides; mode s = 1,2,3 is Wray RK3 substep s
+// (code/time_integration.f90:42-44,59-61,76-81; b2 = rk_coef(3,2) = 0 so the
+// second stage's F is not kept).
+template <typename T>
+__global__ void __launch_bounds__(256) k_rhs_rk(const T* __restrict__ U, const T* __restrict__ V,
+                                                 const T* __restrict__ W, const T* __restrict__ nt,
+                                                 Metric<T> m, Dims d, T nu, const Scalars* sc, int mode,
+                                                 const T* Uo, const T* Vo, const T* Wo, T* F1u, T* F1v, T* F1w,
+                                                 T* Ou, T* Ov, T* Ow, T c1, T c2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 2;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
+    const int k = blockIdx.z + 2;
+    if (i > d.nxg - 1 || j > d.nyg - 1) return;
+    const T dPdx = T(sc->dPdx);
+    const T dt = T(sc->dt);
+    const long p = at(d, i, j, k);
+    const bool du = (i <= d.nx - 1) && (k <= d.nzg - 1);
+    const bool dv = (j <= d.ny - 1) && (k <= d.nzg - 1);
+    const bool dw = (k <= d.nz - 1);
+    T ru = T(0), rv = T(0), rw = T(0);
+    if (du) ru = rhs_u_at<T>(U, V, W, nt, m, d, nu, dPdx, i, j, k);
+    if (dv) rv = rhs_v_at<T>(U, V, W, nt, m, d, nu, i, j, k);
+    if (dw) rw = rhs_w_at<T>(U, V, W, nt, m, d, nu, i, j, k);
+    if (mode == 0) {
+        if (du) Ou[p] = ru;
+        if (dv) Ov[p] = rv;
+        if (dw) Ow[p] = rw;
+        return;
+    }
+    if (mode == 1) {
+        // U = Uo + dt*rk_coef(1,1)*Fu1 with Uo = U at substep 1
+        if (du) { F1u[p] = ru; Ou[p] = U[p] + dt * c1 * ru; }
+        if (dv) { F1v[p] = rv; Ov[p] = V[p] + dt * c1 * rv; }
+        if (dw) { F1w[p] = rw; Ow[p] = W[p] + dt * c1 * rw; }
+    } else {
+        // U = Uo + dt*(c1*Fu1 + c2*Fu_s)
+        if (du) Ou[p] = Uo[p] + dt * (c1 * F1u[p] + c2 * ru);
+        if (dv) Ov[p] = Vo[p] + dt * (c1 * F1v[p] + c2 * rv);
+        if (dw) Ow[p] = Wo[p] + dt * (c1 * F1w[p] + c2 * rw);
+    }
+}
+
+template <typename T>
+void launch_rhs_rk(const T* U, const T* V, const T* W, const T* nut, const Metric<T>& m, const Dims& d, T nu,
+                   const Scalars* sc, int mode, const T* Uo, const T* Vo, const T* Wo, T* F1u, T* F1v, T* F1w,
+                   T* Ou, T* Ov, T* Ow, double c1, double c2, hipStream_t s)
+{
+    dim3 blk(64, 4, 1);
+    dim3 grd(cdiv(d.nxg - 2, 64), cdiv(d.nyg - 2, 4), d.nzg - 2);
+    k_rhs_rk<T><<<grd, blk, 0, s>>>(U, V, W, nut, m, d, nu, sc, mode, Uo, Vo, Wo, F1u, F1v, F1w, Ou, Ov, Ow,
+                                    T(c1), T(c2));
+}
+
+// --------------------------------------------------------------- SGS: DSM
+
+// code/subgrid.f90:1014-1072 (rate of strain at a centre, off-diagonals
+// averaged from the four surrounding edges; |S| as sqrt(2 S_ij S_ij)).
+template <typename T>
+__device__ __forceinline__ void strain_at(const T* U, const T* V, const T* W, const Metric<T>& m, const Dims& d,
+                                          int i, int j, int k, T s[6], T& smag)
+{
+    auto u = [&](int a, int b, int c) { return U[at(d, a, b, c)]; };
+    auto v = [&](int a, int b, int c) { return V[at(d, a, b, c)]; };
+    auto w = [&](int a, int b, int c) { return W[at(d, a, b, c)]; };
+    const T h = T(0.5), q = T(0.25);
+    const T *x = m.x, *y = m.y, *z = m.z, *xg = m.xg, *yg = m.yg, *zg = m.zg;
+    s[0] = (u(i, j, k) - u(i - 1, j, k)) / (x[i] - x[i - 1]);
+    s[1] = (v(i, j, k) - v(i, j - 1, k)) / (y[j] - y[j - 1]);
+    s[2] = (w(i, j, k) - w(i, j, k - 1)) / (z[k] - z[k - 1]);
+    auto e4 = [&](int a, int b, int c) {
+        return h * ((u(a - 1, b, c) - u(a - 1, b - 1, c)) / (yg[b] - yg[b - 1]) +
+                    (v(a, b - 1, c) - v(a - 1, b - 1, c)) / (xg[a] - xg[a - 1]));
+    };
+    auto e5 = [&](int a, int b, int c) {
+        return h * ((u(a - 1, b, c) - u(a - 1, b, c - 1)) / (zg[c] - zg[c - 1]) +
+                    (w(a, b, c - 1) - w(a - 1, b, c - 1)) / (xg[a] - xg[a - 1]));
+    };
+    auto e6 = [&](int a, int b, int c) {
+        return h * ((v(a, b - 1, c) - v(a, b - 1, c - 1)) / (zg[c] - zg[c - 1]) +
+                    (w(a, b, c - 1) - w(a, b - 1, c - 1)) / (yg[b] - yg[b - 1]));
+    };
+    s[3] = q * (e4(i, j, k) + e4(i + 1, j, k) + e4(i, j + 1, k) + e4(i + 1, j + 1, k));
+    s[4] = q * (e5(i, j, k) + e5(i + 1, j, k) + e5(i, j, k + 1) + e5(i + 1, j, k + 1));
+    s[5] = q * (e6(i, j, k) + e6(i, j + 1, k) + e6(i, j, k + 1) + e6(i, j + 1, k + 1));
+    T a = T(0);
+    a = a + T(2) * s[0] * s[0];
+    a = a + T(2) * s[1] * s[1];
+    a = a + T(2) * s[2] * s[2];
+    a = a + T(4) * s[3] * s[3];
+    a = a + T(4) * s[4] * s[4];
+    a = a + T(4) * s[5] * s[5];
+    smag = sqrt(a);
+}
+
+// code/subgrid.f90:394-416, 464-468: centre velocities and |S| S_ij at every
+// centre of rows 2..nyg-1, periodic images evaluated at their source.
+template <typename T>
+__global__ void __launch_bounds__(256) k_dsm_prep(const T* __restrict__ U, const T* __restrict__ V,
+                                                   const T* __restrict__ W, Metric<T> m, Dims d, DsmBufs<T> b)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
+    const int k = blockIdx.z + 1;
+    if (i > d.nxg || j > d.nyg - 1) return;
+    const int ic = img_centre(i, d.nxg), kc = img_centre(k, d.nzg);
+    const T h = T(0.5);
+    const long p = at(d, i, j, k);
+    b.ac[0][p] = h * (U[at(d, ic - 1, j, kc)] + U[at(d, ic, j, kc)]);
+    b.ac[1][p] = h * (V[at(d, ic, j - 1, kc)] + V[at(d, ic, j, kc)]);
+    b.ac[2][p] = h * (W[at(d, ic, j, kc - 1)] + W[at(d, ic, j, kc)]);
+    T s[6], sm;
+    strain_at<T>(U, V, W, m, d, ic, j, kc, s, sm);
+    b.smag[p] = sm;
+#pragma unroll
+    for (int c = 0; c < 6; c++) b.ss[c][p] = sm * s[c];
+}
+
+// code/subgrid.f90:140-190 applied over a staggered field's whole y range
+// (ghost rows included), code/subgrid.f90:438-453 with periodic images.
+template <typename T>
+__device__ __forceinline__ T filter_full_at(const T* F, const Dims& d, int ii, int j, int kk, int n2)
+{
+    const T fil[3][3] = {{T(1.0 / 36.0), T(1.0 / 9.0), T(1.0 / 36.0)},
+                         {T(1.0 / 9.0), T(4.0 / 9.0), T(1.0 / 9.0)},
+                         {T(1.0 / 36.0), T(1.0 / 9.0), T(1.0 / 36.0)}};
+    const T c6 = T(1.0 / 6.0), c23 = T(2.0 / 3.0), c13 = T(1.0 / 3.0);
+    T acc = T(0);
+#pragma unroll
+    for (int a = -1; a <= 1; a++)
+#pragma unroll
+        for (int c = -1; c <= 1; c++) {
+            T yv;
+            if (j == 1)
+                yv = c23 * F[at(d, ii + a, 1, kk + c)] + c13 * F[at(d, ii + a, 2, kk + c)];
+            else if (j == n2)
+                yv = c13 * F[at(d, ii + a, n2 - 1, kk + c)] + c23 * F[at(d, ii + a, n2, kk + c)];
+            else
+                yv = c6 * F[at(d, ii + a, j - 1, kk + c)] + c23 * F[at(d, ii + a, j, kk + c)] +
+                     c6 * F[at(d, ii + a, j + 1, kk + c)];
+            acc = acc + yv * fil[a + 1][c + 1];
+        }
+    return acc;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_dsm_filtvel(const T* __restrict__ U, const T* __restrict__ V,
+                                                      const T* __restrict__ W, Dims d, T* Uf, T* Vf, T* Wf)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > d.nxg || j > d.nyg) return;
+    const long p = at(d, i, j, k);
+    if (i <= d.nx) Uf[p] = filter_full_at<T>(U, d, img_face(i, d.nx), j, img_centre(k, d.nzg), d.nyg);
+    if (j <= d.ny) Vf[p] = filter_full_at<T>(V, d, img_centre(i, d.nxg), j, img_centre(k, d.nzg), d.ny);
+    if (k <= d.nz) Wf[p] = filter_full_at<T>(W, d, img_centre(i, d.nxg), j, img_face(k, d.nz), d.nyg);
+}
+
+// code/subgrid.f90:418-510: Leonard term L_ij, model tensor M_ij, and the
+// per-row partial sums of L:M and M:M over the plane i,k = 2..nxg-1,2..nzg-1.
+// The test filter on rows 2..nyg-1 is one-sided on the first/last row.
+template <typename T>
+__global__ void __launch_bounds__(256) k_dsm_lm(Metric<T> m, Dims d, DsmBufs<T> b)
+{
+    __shared__ double red[2][256];
+    const int tx = threadIdx.x, tz = threadIdx.z;
+    const int tid = tz * blockDim.x + tx;
+    const int i = blockIdx.x * blockDim.x + tx + 2;
+    const int j = blockIdx.y + 2;
+    const int k = blockIdx.z * blockDim.z + tz + 2;
+    double lm = 0.0, mm = 0.0;
+    if (i <= d.nxg - 1 && k <= d.nzg - 1) {
+        const T fil[3][3] = {{T(1.0 / 36.0), T(1.0 / 9.0), T(1.0 / 36.0)},
+                             {T(1.0 / 9.0), T(4.0 / 9.0), T(1.0 / 9.0)},
+                             {T(1.0 / 36.0), T(1.0 / 9.0), T(1.0 / 36.0)}};
+        const T c6 = T(1.0 / 6.0), c23 = T(2.0 / 3.0), c13 = T(1.0 / 3.0);
+        const int j0 = 2, j1 = d.nyg - 1;
+        T fa[3] = {T(0), T(0), T(0)};
+        T fp[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+        T fs[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+        const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+        for (int a = -1; a <= 1; a++)
+            for (int c = -1; c <= 1; c++) {
+                const T f = fil[a + 1][c + 1];
+                const long pm = at(d, i + a, j - 1, k + c), p0 = at(d, i + a, j, k + c),
+                           pp = at(d, i + a, j + 1, k + c);
+                T am[3], a0[3], ap[3];
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    a0[q] = b.ac[q][p0];
+                    am[q] = (j > j0) ? b.ac[q][pm] : T(0);
+                    ap[q] = (j < j1) ? b.ac[q][pp] : T(0);
+                }
+                auto ycomb = [&](T vm, T v0, T vp) {
+                    if (j == j0) return c23 * v0 + c13 * vp;
+                    if (j == j1) return c13 * vm + c23 * v0;
+                    return c6 * vm + c23 * v0 + c6 * vp;
+                };
+#pragma unroll
+                for (int q = 0; q < 6; q++) {
+                    T prm = am[pa[q]] * am[pb[q]], pr0 = a0[pa[q]] * a0[pb[q]], prp = ap[pa[q]] * ap[pb[q]];
+                    fp[q] = fp[q] + ycomb(prm, pr0, prp) * f;
+                }
+#pragma unroll
+                for (int q = 0; q < 3; q++) fa[q] = fa[q] + ycomb(am[q], a0[q], ap[q]) * f;
+#pragma unroll
+                for (int q = 0; q < 6; q++) {
+                    T sm_ = (j > j0) ? b.ss[q][pm] : T(0);
+                    T s0_ = b.ss[q][p0];
+                    T sp_ = (j < j1) ? b.ss[q][pp] : T(0);
+                    fs[q] = fs[q] + ycomb(sm_, s0_, sp_) * f;
+                }
+            }
+        T sh[6], shm;
+        strain_at<T>(b.uf, b.vf, b.wf, m, d, i, j, k, sh, shm);
+        T L[6], M[6];
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            L[q] = fp[q] - fa[pa[q]] * fa[pb[q]];
+            M[q] = T(4) * shm * sh[q] - fs[q];
+        }
+        T tlm = L[0] * M[0] + L[1] * M[1] + L[2] * M[2] + T(2) * (L[3] * M[3] + L[4] * M[4] + L[5] * M[5]);
+        T tmm = M[0] * M[0] + M[1] * M[1] + M[2] * M[2] + T(2) * (M[3] * M[3] + M[4] * M[4] + M[5] * M[5]);
+        lm = (double)tlm;
+        mm = (double)tmm;
+    }
+    red[0][tid] = lm;
+    red[1][tid] = mm;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (tid < st) {
+            red[0][tid] += red[0][tid + st];
+            red[1][tid] += red[1][tid + st];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const int blk = blockIdx.z * gridDim.x + blockIdx.x;
+        b.part[((long)(j - 1) * 2 + 0) * b.nblk + blk] = red[0][0];
+        b.part[((long)(j - 1) * 2 + 1) * b.nblk + blk] = red[1][0];
+    }
+}
+
+// <L:M>/<M:M> per row, blocks summed in a fixed order.
+__global__ void k_dsm_ratio(const double* part, int nblk, int nyg, double* ratio)
+{
+    __shared__ double red[2][256];
+    const int j = blockIdx.x + 2;
+    const int t = threadIdx.x;
+    double a = 0.0, c = 0.0;
+    for (int q = t; q < nblk; q += blockDim.x) {
+        a += part[((long)(j - 1) * 2 + 0) * nblk + q];
+        c += part[((long)(j - 1) * 2 + 1) * nblk + q];
+    }
+    red[0][t] = a;
+    red[1][t] = c;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (t < st) { red[0][t] += red[0][t + st]; red[1][t] += red[1][t + st]; }
+        __syncthreads();
+    }
+    if (t == 0) {
+        ratio[j - 1] = red[0][0] / red[1][0];
+        ratio[nyg + j - 1] = red[0][0];
+        ratio[2 * nyg + j - 1] = red[1][0];
+    }
+}
+
+// code/wallmodel.f90:32-37: wall-model eddy viscosity at a centre from the
+// two adjacent x faces; wall = 0 bottom (rows 1,2), 1 top (rows nyg-1,nyg).
+template <typename T>
+__device__ __forceinline__ T wm_nut_at(const T* U, const double* alpha, const Metric<T>& m, const Dims& d, T nu,
+                                       int ic, int k, int wall)
+{
+    const T dyw = m.yg[2] - m.yg[1];
+    auto tau = [&](int f) {
+        const T a = T(alpha[(f - 1) + (long)d.nx * (wall + 2L * (k - 1))]);
+        const T du = wall == 0 ? (U[at(d, f, 2, k)] - U[at(d, f, 1, k)])
+                               : (U[at(d, f, d.nyg - 1, k)] - U[at(d, f, d.nyg, k)]);
+        return a / (du / dyw) - nu;
+    };
+    return T(0.5) * (tau(ic - 1) + tau(ic));
+}
+
+// code/subgrid.f90:512-535: nu_t = max(-1/2 <LM>/<MM> |S|, 0), y ghost by
+// Dirichlet_nu_t; with fuse_wm the wall rows come from the wall model
+// (code/wallmodel.f90:39-42), which the reference applies right after.
+template <typename T>
+__global__ void __launch_bounds__(256) k_dsm_nut(T* nut, Dims d, DsmBufs<T> b, int dirichlet, int fuse_wm,
+                                                  const T* U, const double* alpha, Metric<T> m, T nu)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > d.nxg || j > d.nyg) return;
+    const long p = at(d, i, j, k);
+    if (fuse_wm && (j <= 2 || j >= d.nyg - 1)) {
+        nut[p] = wm_nut_at<T>(U, alpha, m, d, nu, img_centre(i, d.nxg), k, j <= 2 ? 0 : 1);
+        return;
+    }
+    const int jj = (j == 1) ? 2 : (j == d.nyg ? d.nyg - 1 : j);
+    const T r = T(b.ratio[jj - 1]);
+    const T v = fmax(T(-0.5) * r * b.smag[at(d, i, jj, k)], T(0));
+    nut[p] = (jj != j && dirichlet) ? -v : v;
+}
+
+template <typename T>
+void launch_dsm(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, const Dims& d, DsmBufs<T>& b,
+                int dirichlet, int fuse_wm, const double* alpha_x, T nu, hipStream_t s)
+{
+    {
+        dim3 blk(64, 4, 1), grd(cdiv(d.nxg, 64), cdiv(d.nyg - 2, 4), d.nzg);
+        k_dsm_prep<T><<<grd, blk, 0, s>>>(U, V, W, m, d, b);
+    }
+    {
+        dim3 blk(64, 4, 1), grd(cdiv(d.nxg, 64), cdiv(d.nyg, 4), d.nzg);
+        k_dsm_filtvel<T><<<grd, blk, 0, s>>>(U, V, W, d, b.uf, b.vf, b.wf);
+    }
+    {
+        dim3 blk(64, 1, 4), grd(cdiv(d.nxg - 2, 64), d.nyg - 2, cdiv(d.nzg - 2, 4));
+        k_dsm_lm<T><<<grd, blk, 0, s>>>(m, d, b);
+    }
+    k_dsm_ratio<<<d.nyg - 2, 256, 0, s>>>(b.part, b.nblk, d.nyg, b.ratio);
+    {
+        dim3 blk(64, 4, 1), grd(cdiv(d.nxg, 64), cdiv(d.nyg, 4), d.nzg);
+        k_dsm_nut<T><<<grd, blk, 0, s>>>(nut, d, b, dirichlet, fuse_wm, U, alpha_x, m, nu);
+    }
+}
+
+// code/wallmodel.f90:23-46 on its own: wall rows from alpha_x, then the
+// x-periodic fill of every other row of nu_t.
+template <typename T>
+__global__ void k_wall_model(const T* U, T* nut, const double* alpha, Metric<T> m, Dims d, T nu)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > d.nxg || j > d.nyg) return;
+    const bool wallrow = (j <= 2 || j >= d.nyg - 1);
+    if (wallrow) {
+        nut[at(d, i, j, k)] = wm_nut_at<T>(U, alpha, m, d, nu, img_centre(i, d.nxg), k, j <= 2 ? 0 : 1);
+    } else if (i == 1 || i >= d.nxg - 1) {
+        nut[at(d, i, j, k)] = nut[at(d, img_centre(i, d.nxg), j, k)];
+    }
+}
+
+template <typename T>
+void launch_wall_model(const T* U, T* nut, const double* alpha_x, const Metric<T>& m, const Dims& d, T nu,
+                       hipStream_t s)
+{
+    dim3 blk(64, 4, 1), grd(cdiv(d.nxg, 64), cdiv(d.nyg, 4), d.nzg);
+    k_wall_model<T><<<grd, blk, 0, s>>>(U, nut, alpha_x, m, d, nu);
+}
+
+// ------------------------------------------------ equilibrium wall stress
+
+// Log law U+ = ln(y+)/kappa + B above the crossover with U+ = y+, linear
+// below; Newton on f(ut) = ut (ln(h ut/nu)/kappa + B) - |u|.
+__device__ double loglaw_utau(double u, double h, double nu, double kappa, double B)
+{
+    double au = fabs(u);
+    if (au == 0.0) return 0.0;
+    double ut = sqrt(nu * au / h);
+    double yc = 11.0;
+    for (int it = 0; it < 50; it++) yc = yc - (yc - log(yc) / kappa - B) / (1.0 - 1.0 / (kappa * yc));
+    if (h * ut / nu <= yc) return ut;
+    double g = ut;
+    for (int it = 0; it < 60; it++) {
+        double lg = log(h * g / nu);
+        double f = g * (lg / kappa + B) - au;
+        double fp = lg / kappa + B + 1.0 / kappa;
+        double gn = g - f / fp;
+        g = gn > 0.5 * g ? gn : 0.5 * g;
+    }
+    return g;
+}
+
+template <typename T>
+__global__ void k_equilibrium_alpha(const T* U, double* alpha, Metric<double> m, Dims d, double nu, double h_wm,
+                                    double kappa, double B)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int k = blockIdx.y + 1;
+    if (i > d.nx) return;
+    const double *y = m.y, *yg = m.yg;
+    int jb = 2;
+    double wb = 1.0, hb = yg[2] - y[1];
+    if (h_wm > 0.0) {
+        double yt = y[1] + h_wm;
+        while (jb < d.nyg - 2 && yg[jb + 1] < yt) jb++;
+        wb = (yg[jb + 1] - yt) / (yg[jb + 1] - yg[jb]);
+        hb = h_wm;
+    }
+    const int jt = d.nyg + 1 - jb;
+    const double ub = wb * (double)U[at(d, i, jb, k)] + (1.0 - wb) * (double)U[at(d, i, jb + 1, k)];
+    const double ut = wb * (double)U[at(d, i, jt, k)] + (1.0 - wb) * (double)U[at(d, i, jt - 1, k)];
+    const double tb = loglaw_utau(ub, hb, nu, kappa, B);
+    const double tt = loglaw_utau(ut, h_wm > 0.0 ? hb : y[d.ny] - yg[d.nyg - 1], nu, kappa, B);
+    alpha[(i - 1) + (long)d.nx * (0 + 2L * (k - 1))] = (ub >= 0 ? 1.0 : -1.0) * tb * tb;
+    alpha[(i - 1) + (long)d.nx * (1 + 2L * (k - 1))] = (ut >= 0 ? 1.0 : -1.0) * tt * tt;
+}
+
+template <typename T>
+void launch_equilibrium_alpha(const T* U, double* alpha_x, const Metric<double>& md, const Dims& d, double nu,
+                              double h_wm, double kappa, double B, hipStream_t s)
+{
+    dim3 blk(64), grd(cdiv(d.nx, 64), d.nzg);
+    k_equilibrium_alpha<T><<<grd, blk, 0, s>>>(U, alpha_x, md, d, nu, h_wm, kappa, B);
+}
+
+// ------------------------------------------------------------- reductions
+
+// block sum of two doubles into part[blk], part[nblk + blk]
+__device__ __forceinline__ void block_sum2(double a, double b, double* part, int nblk)
+{
+    __shared__ double red[2][RED_THREADS];
+    const int t = threadIdx.x;
+    red[0][t] = a; red[1][t] = b;
+    __syncthreads();
+    for (int st = RED_THREADS / 2; st > 0; st >>= 1) {
+        if (t < st) { red[0][t] += red[0][t + st]; red[1][t] += red[1][t + st]; }
+        __syncthreads();
+    }
+    if (t == 0) { part[blockIdx.x] = red[0][0]; part[nblk + blockIdx.x] = red[1][0]; }
+}
+
+__device__ __forceinline__ void final_sum2(const double* part, int nblk, double& a, double& b)
+{
+    __shared__ double red[2][RED_THREADS];
+    const int t = threadIdx.x;
+    double x = 0.0, y = 0.0;
+    for (int q = t; q < nblk; q += RED_THREADS) { x += part[q]; y += part[nblk + q]; }
+    red[0][t] = x; red[1][t] = y;
+    __syncthreads();
+    for (int st = RED_THREADS / 2; st > 0; st >>= 1) {
+        if (t < st) { red[0][t] += red[0][t + st]; red[1][t] += red[1][t + st]; }
+        __syncthreads();
+    }
+    a = red[0][0]; b = red[1][0];
+}
+
+// code/mass_flow.f90:24-62 (trapezoid bulk flow between yg(2) and yg(nyg-1))
+template <typename T>
+__global__ void k_massflow_u(const T* U, Metric<double> m, Dims d, double* part, int nblk)
+{
+    const long n1 = d.nx - 2, n2 = d.nyg - 3, n3 = d.nzp;
+    const long tot = n1 * n2 * n3;
+    double q = 0.0, nrm = 0.0;
+    for (long t = (long)blockIdx.x * RED_THREADS + threadIdx.x; t < tot; t += (long)nblk * RED_THREADS) {
+        const int i = (int)(t % n1) + 2;
+        const int j = (int)((t / n1) % n2) + 3;
+        const int k = (int)(t / (n1 * n2)) + 2;
+        const double dy = m.yg[j] - m.yg[j - 1];
+        q += ((double)U[at(d, i, j, k)] + (double)U[at(d, i, j - 1, k)]) * 0.5 * dy;
+        nrm += (1.0 + 1.0) * 0.5 * dy;
+    }
+    block_sum2(q, nrm, part, nblk);
+}
+
+__global__ void k_massflow_u_final(const double* part, int nblk, Scalars* sc)
+{
+    double a, b;
+    final_sum2(part, nblk, a, b);
+    if (threadIdx.x == 0) sc->Qflow_x_0 = a / b;
+}
+
+template <typename T>
+void launch_mass_flow_init(const T* U, const Metric<double>& md, const Dims& d, double* part, int nblk, Scalars* sc,
+                           hipStream_t s)
+{
+    k_massflow_u<T><<<nblk, RED_THREADS, 0, s>>>(U, md, d, part, nblk);
+    k_massflow_u_final<<<1, RED_THREADS, 0, s>>>(part, nblk, sc);
+}
+
+// code/mass_flow.f90:113-166: mean of 0.5(U(:,3,:)+U(:,nyg-2,:)) over the
+// unique x faces and z centres; dPdx = Qflow_x_0 - mean.
+template <typename T>
+__global__ void k_dpx_partial(const T* U, Dims d, double* part, int nblk)
+{
+    const long n1 = d.nx - 2, n3 = d.nzp, tot = n1 * n3;
+    double s = 0.0, c = 0.0;
+    for (long t = (long)blockIdx.x * RED_THREADS + threadIdx.x; t < tot; t += (long)nblk * RED_THREADS) {
+        const int i = (int)(t % n1) + 2, k = (int)(t / n1) + 2;
+        s += 0.5 * ((double)U[at(d, i, 3, k)] + (double)U[at(d, i, d.nyg - 2, k)]);
+        c += 1.0;
+    }
+    block_sum2(s, c, part, nblk);
+}
+
+__global__ void k_dpx_final(const double* part, int nblk, Scalars* sc)
+{
+    double a, b;
+    final_sum2(part, nblk, a, b);
+    if (threadIdx.x == 0) {
+        sc->last_dPdx = sc->Qflow_x_0 - a / b;
+        sc->dPdx = 0.0;  // code/time_integration.f90:92
+    }
+}
+
+// U(2:nx-1,2:nyg-1,2:nzg-1) += dPdx (code/time_integration.f90:91)
+template <typename T>
+__global__ void k_shift_u(T* U, Dims d, const Scalars* sc)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 2;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
+    const int k = blockIdx.z + 2;
+    if (i > d.nx - 1 || j > d.nyg - 1) return;
+    const T s = T(sc->last_dPdx);
+    const long p = at(d, i, j, k);
+    U[p] = U[p] + s;
+}
+
+template <typename T>
+void launch_mass_flow_correct(T* U, const Dims& d, double* part, int nblk, Scalars* sc, int apply, hipStream_t s)
+{
+    k_dpx_partial<T><<<nblk, RED_THREADS, 0, s>>>(U, d, part, nblk);
+    k_dpx_final<<<1, RED_THREADS, 0, s>>>(part, nblk, sc);
+    if (apply) {
+        dim3 blk(64, 4, 1), grd(cdiv(d.nx - 2, 64), cdiv(d.nyg - 2, 4), d.nzg - 2);
+        k_shift_u<T><<<grd, blk, 0, s>>>(U, d, sc);
+        launch_bc<T>(U, d, KIND_U, s);
+    }
+}
+
+// code/time_integration.f90:102-150 (max of dx/|u| as the reference; max is
+// order independent, so the result is exact).
+__device__ __forceinline__ void block_max3(double a, double b, double c, double* part, int nblk)
+{
+    __shared__ double red[3][RED_THREADS];
+    const int t = threadIdx.x;
+    red[0][t] = a; red[1][t] = b; red[2][t] = c;
+    __syncthreads();
+    for (int st = RED_THREADS / 2; st > 0; st >>= 1) {
+        if (t < st) {
+            red[0][t] = fmax(red[0][t], red[0][t + st]);
+            red[1][t] = fmax(red[1][t], red[1][t + st]);
+            red[2][t] = fmax(red[2][t], red[2][t + st]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        part[blockIdx.x] = red[0][0];
+        part[nblk + blockIdx.x] = red[1][0];
+        part[2 * nblk + blockIdx.x] = red[2][0];
+    }
+}
+
+template <typename T>
+__global__ void k_dt_partial(const T* U, const T* V, const T* W, Metric<double> m, Dims d, double* part, int nblk)
+{
+    const long n1 = d.nxg - 2, n2 = d.nyg - 2, n3 = d.nzg - 2, tot = n1 * n2 * n3;
+    double lu = 0.0, lv = 0.0, lw = 0.0;
+    for (long t = (long)blockIdx.x * RED_THREADS + threadIdx.x; t < tot; t += (long)nblk * RED_THREADS) {
+        const int i = (int)(t % n1) + 2;
+        const int j = (int)((t / n1) % n2) + 2;
+        const int k = (int)(t / (n1 * n2)) + 2;
+        lu = fmax(lu, (m.xg[i + 1] - m.xg[i]) / fabs((double)U[at(d, i, j, k)]));
+        lv = fmax(lv, (m.yg[j + 1] - m.yg[j]) / fabs((double)V[at(d, i, j, k)]));
+        lw = fmax(lw, (m.zg[k + 1] - m.zg[k]) / fabs((double)W[at(d, i, j, k)]));
+    }
+    block_max3(lu, lv, lw, part, nblk);
+}
+
+__global__ void k_dt_final(const double* part, int nblk, Scalars* sc, const double* mins3)
+{
+    __shared__ double red[3][RED_THREADS];
+    const int t = threadIdx.x;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int q = t; q < nblk; q += RED_THREADS) {
+        a = fmax(a, part[q]); b = fmax(b, part[nblk + q]); c = fmax(c, part[2 * nblk + q]);
+    }
+    red[0][t] = a; red[1][t] = b; red[2][t] = c;
+    __syncthreads();
+    for (int st = RED_THREADS / 2; st > 0; st >>= 1) {
+        if (t < st) {
+            red[0][t] = fmax(red[0][t], red[0][t + st]);
+            red[1][t] = fmax(red[1][t], red[1][t + st]);
+            red[2][t] = fmax(red[2][t], red[2][t + st]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double cfl = sc->cfl, nu = sc->nu;
+        double dconv = fmin(fmin(cfl * red[0][0], cfl * red[1][0]), cfl * red[2][0]);
+        double dvis = fmin(fmin(cfl * (mins3[0] * mins3[0]) / nu, cfl * (mins3[1] * mins3[1]) / nu),
+                           cfl * (mins3[2] * mins3[2]) / nu);
+        double dt = fmin(dconv, dvis);
+        dt = fmin(dt, sc->dt_max);
+        if (cfl < 0) dt = -cfl;
+        sc->dt = dt;
+    }
+}
+
+template <typename T>
+void launch_dt(const T* U, const T* V, const T* W, const Metric<double>& md, const Dims& d, double* part, int nblk,
+               Scalars* sc, const double* mins3, hipStream_t s)
+{
+    k_dt_partial<T><<<nblk, RED_THREADS, 0, s>>>(U, V, W, md, d, part, nblk);
+    k_dt_final<<<1, RED_THREADS, 0, s>>>(part, nblk, sc, mins3);
+}
+
+// code/projection.f90:223-245 (signed max) plus max |div|
+template <typename T>
+__global__ void k_div_partial(const T* U, const T* V, const T* W, Metric<double> m, Dims d, double* part, int nblk)
+{
+    const long n1 = d.nxg - 2, n2 = d.nyg - 2, n3 = d.nzg - 2, tot = n1 * n2 * n3;
+    double mx = 0.0, ma = 0.0;
+    for (long t = (long)blockIdx.x * RED_THREADS + threadIdx.x; t < tot; t += (long)nblk * RED_THREADS) {
+        const int i = (int)(t % n1) + 2;
+        const int j = (int)((t / n1) % n2) + 2;
+        const int k = (int)(t / (n1 * n2)) + 2;
+        double v = ((double)U[at(d, i, j, k)] - (double)U[at(d, i - 1, j, k)]) / (m.x[i] - m.x[i - 1]) +
+                   ((double)V[at(d, i, j, k)] - (double)V[at(d, i, j - 1, k)]) / (m.y[j] - m.y[j - 1]) +
+                   ((double)W[at(d, i, j, k)] - (double)W[at(d, i, j, k - 1)]) / (m.z[k] - m.z[k - 1]);
+        mx = fmax(mx, v);
+        ma = fmax(ma, fabs(v));
+    }
+    block_max3(mx, ma, 0.0, part, nblk);
+}
+
+__global__ void k_div_final(const double* part, int nblk, Scalars* sc)
+{
+    __shared__ double red[2][RED_THREADS];
+    const int t = threadIdx.x;
+    double a = 0.0, b = 0.0;
+    for (int q = t; q < nblk; q += RED_THREADS) { a = fmax(a, part[q]); b = fmax(b, part[nblk + q]); }
+    red[0][t] = a; red[1][t] = b;
+    __syncthreads();
+    for (int st = RED_THREADS / 2; st > 0; st >>= 1) {
+        if (t < st) { red[0][t] = fmax(red[0][t], red[0][t + st]); red[1][t] = fmax(red[1][t], red[1][t + st]); }
+        __syncthreads();
+    }
+    if (t == 0) { sc->maxdiv = red[0][0]; sc->maxabsdiv = red[1][0]; }
+}
+
+template <typename T>
+void launch_divergence_check(const T* U, const T* V, const T* W, const Metric<double>& md, const Dims& d,
+                             double* part, int nblk, Scalars* sc, hipStream_t s)
+{
+    k_div_partial<T><<<nblk, RED_THREADS, 0, s>>>(U, V, W, md, d, part, nblk);
+    k_div_final<<<1, RED_THREADS, 0, s>>>(part, nblk, sc);
+}
+
+// code/statistics.f90:52-64 as plane means over the unique periodic points:
+// out[q*ny + j-1] for q = U, V, W, U2, V2, W2, UV.  One block per row.
+template <typename T>
+__global__ void k_plane_stats(const T* U, const T* V, const T* W, Metric<double> m, Dims d, double* out)
+{
+    __shared__ double red[7][RED_THREADS];
+    const int j = blockIdx.x + 1;
+    const int t = threadIdx.x;
+    double s[7] = {0, 0, 0, 0, 0, 0, 0};
+    const long n1 = d.nxp, n3 = d.nzp, tot = n1 * n3;
+    for (long q = t; q < tot; q += RED_THREADS) {
+        const int i = (int)(q % n1) + 2, k = (int)(q / n1) + 2;
+        const double u = U[at(d, i, j, k)], v = V[at(d, i, j, k)], w = W[at(d, i, j, k)];
+        s[0] += u; s[3] += u * u;
+        s[1] += v; s[4] += v * v;
+        s[2] += w; s[5] += w * w;
+        const double uy0 = m.w0[j] * (double)U[at(d, i - 1, j, k)] + m.w1[j] * (double)U[at(d, i - 1, j + 1, k)];
+        const double uy1 = m.w0[j] * (double)U[at(d, i, j, k)] + m.w1[j] * (double)U[at(d, i, j + 1, k)];
+        s[6] += v * (0.5 * (uy0 + uy1));
+    }
+    for (int c = 0; c < 7; c++) red[c][t] = s[c];
+    __syncthreads();
+    for (int st = RED_THREADS / 2; st > 0; st >>= 1) {
+        if (t < st)
+            for (int c = 0; c < 7; c++) red[c][t] += red[c][t + st];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int c = 0; c < 7; c++) out[(long)c * d.ny + j - 1] = red[c][0] / ((double)d.nxp * (double)d.nzp);
+}
+
+template <typename T>
+void launch_plane_stats(const T* U, const T* V, const T* W, const Metric<double>& md, const Dims& d, double* out,
+                        hipStream_t s)
+{
+    k_plane_stats<T><<<d.ny, RED_THREADS, 0, s>>>(U, V, W, md, d, out);
+}
+
+// ------------------------------------------------------- instantiations
+#define CHAN_INST(T)                                                                                            \
+    template void launch_bc<T>(T*, const Dims&, int, hipStream_t);                                              \
+    template void launch_rhs_rk<T>(const T*, const T*, const T*, const T*, const Metric<T>&, const Dims&, T,    \
+                                   const Scalars*, int, const T*, const T*, const T*, T*, T*, T*, T*, T*, T*,   \
+                                   double, double, hipStream_t);                                                \
+    template void launch_dsm<T>(const T*, const T*, const T*, T*, const Metric<T>&, const Dims&, DsmBufs<T>&,   \
+                                int, int, const double*, T, hipStream_t);                                       \
+    template void launch_wall_model<T>(const T*, T*, const double*, const Metric<T>&, const Dims&, T,           \
+                                       hipStream_t);                                                            \
+    template void launch_equilibrium_alpha<T>(const T*, double*, const Metric<double>&, const Dims&, double,    \
+                                              double, double, double, hipStream_t);                             \
+    template void launch_mass_flow_init<T>(const T*, const Metric<double>&, const Dims&, double*, int,          \
+                                           Scalars*, hipStream_t);                                              \
+    template void launch_mass_flow_correct<T>(T*, const Dims&, double*, int, Scalars*, int, hipStream_t);       \
+    template void launch_dt<T>(const T*, const T*, const T*, const Metric<double>&, const Dims&, double*, int,  \
+                               Scalars*, const double*, hipStream_t);                                           \
+    template void launch_divergence_check<T>(const T*, const T*, const T*, const Metric<double>&, const Dims&, \
+                                             double*, int, Scalars*, hipStream_t);                              \
+    template void launch_plane_stats<T>(const T*, const T*, const T*, const Metric<double>&, const Dims&,       \
+                                        double*, hipStream_t);
+
+CHAN_INST(double)
+CHAN_INST(float)
+
+}  // namespace chan

@@ -1,0 +1,119 @@
+/*
+ * chanles.h -- C ABI of the MI355X channel-flow LES time step.
+ *
+ * One context holds one channel resident on one GPU.  Host arrays use the
+ * reference's Fortran shapes and column-major order (x fastest), so a
+ * Fortran caller passes its arrays unchanged (see the iso_c_binding module
+ * channel_rl_amd/fortran/chanles_mod.f90):
+ *   U (nx,nyg,nzg)  V (nxg,ny,nzg)  W (nxg,nyg,nz)  nu_t, P (nxg,nyg,nzg)
+ *   alpha_x (nx,2,nzg)      with nxg = nx+1, nyg = ny+1, nzg = nz+1
+ *   rhs_u (2:nx-1,2:nyg-1,2:nzg-1)  rhs_v (2:nxg-1,2:ny-1,2:nzg-1)
+ *   rhs_w (2:nxg-1,2:nyg-1,2:nz-1)                 (interior, packed)
+ * (code/initialization.f90:133-136,156-159,467,476).
+ *
+ * Every call returns CHAN_OK (0) or a negative status; chan_last_error()
+ * gives the message.  The Fortran wrappers turn a non-zero status into
+ * `Stop`, the reference's error convention (code/initialization.f90:66-74).
+ * Calls are enqueued on the context's stream; calls that return a value to
+ * the host synchronise.  A context is not re-entrant.
+ */
+#ifndef CHANLES_H
+#define CHANLES_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct chan_ctx chan_ctx;
+
+enum { CHAN_OK = 0, CHAN_EINVAL = -1, CHAN_EHIP = -2, CHAN_ESTATE = -3, CHAN_ENOMEM = -4 };
+
+/* arithmetic of the bulk (stencil/SGS/RK) arrays; the Poisson path is FP64 */
+enum { CHAN_FP64 = 0, CHAN_FP32_BULK = 1 };
+
+/* field selectors for chan_set_field / chan_get_field */
+enum {
+    CHAN_U = 0, CHAN_V = 1, CHAN_W = 2, CHAN_NU_T = 3, CHAN_P = 4, CHAN_ALPHA_X = 5,
+    CHAN_RHS_U = 6, CHAN_RHS_V = 7, CHAN_RHS_W = 8
+};
+
+/* wall-stress source for the wall model (the reference's alpha_x hook) */
+enum { CHAN_WALL_GIVEN = 0, CHAN_WALL_EQUILIBRIUM = 1 };
+
+/* The reference's input-file parameters (code/input_output.f90:20-91) plus
+ * the grid (faces; code/input_output.f90:108-124 / the .init mesh). */
+typedef struct {
+    int nx, ny, nz;            /* face counts, the input line `nx ny nz`   */
+    const double* x;           /* nx face coordinates                      */
+    const double* y;           /* ny                                       */
+    const double* z;           /* nz                                       */
+    double nu;                 /* viscosity                                */
+    double CFL;                /* CFL, or fixed dt = -CFL when negative    */
+    double dPdx;               /* pressure gradient                        */
+    int x_mass_cte;            /* 1: constant mass flow in x               */
+    int LES_model;             /* 0 none, 2 dynamic Smagorinsky            */
+    int Dirichlet_nu_t;        /* 1 Dirichlet, 0 Neumann ghost for nu_t    */
+    int precision;             /* CHAN_FP64 or CHAN_FP32_BULK              */
+    int device;                /* HIP device ordinal                       */
+} chan_config;
+
+/* Allocates the device state and sets up the constants of
+ * code/initialization.f90:187-427 (grids, kxx/kzz, Dyy, weights, RK3). */
+int  chan_create(const chan_config* cfg, chan_ctx** out);
+void chan_destroy(chan_ctx* ctx);
+const char* chan_last_error(const chan_ctx* ctx);
+
+/* host <-> device copies in the shapes listed above */
+int chan_set_field(chan_ctx* ctx, int which, const double* src);
+int chan_get_field(chan_ctx* ctx, int which, double* dst);
+/* scalars: dt, t, dPdx, Qflow_x_0 */
+int chan_set_scalars(chan_ctx* ctx, double dt, double t, double dPdx, double Qflow_x_0);
+int chan_get_scalars(chan_ctx* ctx, double out4[4]);
+
+/* code/boundary_conditions.f90:20 apply_boundary_conditions (on U,V,W) */
+int chan_apply_boundary_conditions(chan_ctx* ctx);
+/* code/subgrid.f90:23 compute_eddy_viscosity(U,V,W,nu_t) */
+int chan_compute_eddy_viscosity(chan_ctx* ctx);
+/* code/wallmodel.f90:23 compute_wall_model(U,V,W,nu_t) from alpha_x */
+int chan_compute_wall_model(chan_ctx* ctx);
+/* code/equations.f90:24,174,327 compute_rhs_u/_v/_w into CHAN_RHS_* */
+int chan_compute_rhs(chan_ctx* ctx);
+/* code/projection.f90:20 compute_projection_step (U,V,W updated, P kept) */
+int chan_compute_projection_step(chan_ctx* ctx);
+/* code/projection.f90:74 solve_poisson_equation on P in place */
+int chan_solve_poisson_equation(chan_ctx* ctx);
+/* code/mass_flow.f90:24 compute_mean_mass_flow_U(U,Q) */
+int chan_compute_mean_mass_flow_U(chan_ctx* ctx, double* Q);
+/* code/initialization.f90:442: Qflow_x_0 = compute_mean_mass_flow_U(U) */
+int chan_initialize_mass_flow(chan_ctx* ctx);
+/* code/mass_flow.f90:113 compute_dPx_for_constant_mass_flow(U,dPdx) */
+int chan_compute_dPx_for_constant_mass_flow(chan_ctx* ctx, double* dPdx);
+/* code/time_integration.f90:102 compute_dt; dt may be NULL (no host sync) */
+int chan_compute_dt(chan_ctx* ctx, double* dt);
+/* code/time_integration.f90:24 compute_time_step_RK3 (graph-captured) */
+int chan_compute_time_step_RK3(chan_ctx* ctx);
+/* code/projection.f90:223 check_divergence: signed max and max |div| */
+int chan_check_divergence(chan_ctx* ctx, double* max_div, double* max_abs_div);
+
+/* Wall-stress source: CHAN_WALL_GIVEN keeps alpha_x as set by the host (the
+ * reference's RL hook, code/smarties_stat.f90:231-258); CHAN_WALL_EQUILIBRIUM
+ * evaluates the equilibrium log law U+ = ln(y+)/kappa + B at height h_wm
+ * (h_wm <= 0: first off-wall cell) before each substep's wall model.  No
+ * reference counterpart: the log-law closure is new. */
+int chan_set_wall_model(chan_ctx* ctx, int mode, double h_wm, double kappa, double B);
+/* evaluate the equilibrium wall stress into alpha_x once */
+int chan_compute_equilibrium_wall_stress(chan_ctx* ctx, double h_wm, double kappa, double B);
+
+/* code/statistics.f90:52-64: xz-plane means over unique points, rows
+ * j=1..ny of U, V, W, U^2, V^2, W^2, UV (out: 7*ny doubles). */
+int chan_plane_statistics(chan_ctx* ctx, double* out);
+
+/* code/app_main.f90:108-118: nsteps x (compute_dt; compute_time_step_RK3)
+ * without host synchronisation. */
+int chan_run_steps(chan_ctx* ctx, int nsteps);
+int chan_synchronize(chan_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

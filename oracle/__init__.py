@@ -1,0 +1,261 @@
+"""ctypes front of the FP64 CPU oracle (oracle/oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker.  The product (channel_rl_amd/)
+never imports it.
+
+Arrays are numpy float64 in the reference's Fortran shapes, order='F':
+U (nx,nyg,nzg), V (nxg,ny,nzg), W (nxg,nyg,nz), nu_t/P (nxg,nyg,nzg),
+alpha_x (nx,2,nzg).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liborc.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liborc.so is not built (run oracle/build.sh)")
+        L = ctypes.CDLL(path)
+        vp = ctypes.c_void_p
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [ctypes.c_int] * 3 + [_D] * 3 + [ctypes.c_double] * 3 + [ctypes.c_int] * 3
+        L.orc_destroy.argtypes = [vp]
+        for name in ("orc_apply_boundary_conditions", "orc_compute_projection_step",
+                     "orc_compute_time_step_RK3"):
+            getattr(L, name).argtypes = [vp]
+        for name in ("orc_compute_rhs_u", "orc_compute_rhs_v", "orc_compute_rhs_w",
+                     "orc_compute_eddy_viscosity"):
+            getattr(L, name).argtypes = [vp, _D, _D, _D, _D]
+        L.orc_compute_wall_model.argtypes = [vp, _D, _D]
+        L.orc_solve_poisson_equation.argtypes = [vp, _D]
+        L.orc_compute_mean_mass_flow_U.argtypes = [vp, _D]
+        L.orc_compute_mean_mass_flow_U.restype = ctypes.c_double
+        L.orc_compute_dPx_for_constant_mass_flow.argtypes = [vp, _D]
+        L.orc_compute_dPx_for_constant_mass_flow.restype = ctypes.c_double
+        L.orc_compute_dt.argtypes = [vp]
+        L.orc_compute_dt.restype = ctypes.c_double
+        L.orc_check_divergence.argtypes = [vp, _D, _D]
+        L.orc_field_size.argtypes = [vp, ctypes.c_int]
+        L.orc_field_size.restype = ctypes.c_long
+        L.orc_set_field.argtypes = [vp, ctypes.c_int, _D]
+        L.orc_get_field.argtypes = [vp, ctypes.c_int, _D]
+        L.orc_set_scalars.argtypes = [vp] + [ctypes.c_double] * 4
+        L.orc_get_scalars.argtypes = [vp, _D]
+        L.orc_get_grid.argtypes = [vp] + [_D] * 6
+        L.orc_get_poisson.argtypes = [vp] + [_D] * 5
+        L.orc_get_rk.argtypes = [vp, _D, _D]
+        L.orc_fft.argtypes = [ctypes.c_int, _D, ctypes.c_int]
+        L.orc_zgtsv.argtypes = [ctypes.c_int, _D, _D, _D, _D]
+        L.orc_zgtsv.restype = ctypes.c_int
+        L.orc_filter_xzy.argtypes = [ctypes.c_int] * 3 + [_D, _D]
+        L.orc_compute_Sij.argtypes = [vp, _D, _D, _D, _D, _D]
+        L.orc_dsm_parts.argtypes = [vp, _D, _D, _D, _D, _D, _D, _D]
+        L.orc_loglaw_utau.argtypes = [ctypes.c_double] * 5
+        L.orc_loglaw_utau.restype = ctypes.c_double
+        L.orc_equilibrium_alpha.argtypes = [vp, _D, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.orc_plane_stats.argtypes = [vp, _D]
+        L.orc_set_wall_model.argtypes = [vp, ctypes.c_int] + [ctypes.c_double] * 3
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.ndim == 1), "need float64 Fortran-ordered"
+    return a.ctypes.data_as(_D)
+
+
+U_, V_, W_, NUT, P_, ALPHA = 0, 1, 2, 3, 4, 5
+
+
+class Oracle:
+    """One channel in the oracle; method names follow the reference subroutines."""
+
+    def __init__(self, x, y, z, nu, CFL=-1e-2, dPdx=0.0, x_mass_cte=0, LES_model=2,
+                 Dirichlet_nu_t=1):
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        self.y = np.ascontiguousarray(y, dtype=np.float64)
+        self.z = np.ascontiguousarray(z, dtype=np.float64)
+        self.nx, self.ny, self.nz = len(x), len(y), len(z)
+        self.nxg, self.nyg, self.nzg = self.nx + 1, self.ny + 1, self.nz + 1
+        self.nu = nu
+        L = lib()
+        self._h = L.orc_create(self.nx, self.ny, self.nz, _p(self.x), _p(self.y), _p(self.z),
+                               nu, CFL, dPdx, x_mass_cte, LES_model, Dirichlet_nu_t)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_destroy(self._h)
+            self._h = None
+
+    # shapes
+    def shape(self, which):
+        nx, ny, nz, nxg, nyg, nzg = self.nx, self.ny, self.nz, self.nxg, self.nyg, self.nzg
+        return {U_: (nx, nyg, nzg), V_: (nxg, ny, nzg), W_: (nxg, nyg, nz), NUT: (nxg, nyg, nzg),
+                P_: (nxg, nyg, nzg), ALPHA: (nx, 2, nzg)}[which]
+
+    def zeros(self, which):
+        return np.zeros(self.shape(which), dtype=np.float64, order="F")
+
+    def set(self, which, a):
+        a = np.asfortranarray(a, dtype=np.float64)
+        assert a.shape == self.shape(which)
+        lib().orc_set_field(self._h, which, _p(a))
+
+    def get(self, which):
+        a = self.zeros(which)
+        lib().orc_get_field(self._h, which, _p(a))
+        return a
+
+    def set_state(self, U, V, W):
+        self.set(U_, U); self.set(V_, V); self.set(W_, W)
+
+    def state(self):
+        return self.get(U_), self.get(V_), self.get(W_)
+
+    def set_scalars(self, dt=0.0, t=0.0, dPdx=0.0, Qflow_x_0=0.0):
+        lib().orc_set_scalars(self._h, dt, t, dPdx, Qflow_x_0)
+
+    def scalars(self):
+        out = np.zeros(4)
+        lib().orc_get_scalars(self._h, _p(out))
+        return dict(dt=out[0], t=out[1], dPdx=out[2], Qflow_x_0=out[3])
+
+    def grid(self):
+        xg, yg, zg = np.zeros(self.nxg), np.zeros(self.nyg), np.zeros(self.nzg)
+        w0, w1, m = np.zeros(self.ny), np.zeros(self.ny), np.zeros(3)
+        lib().orc_get_grid(self._h, _p(xg), _p(yg), _p(zg), _p(w0), _p(w1), _p(m))
+        return dict(xg=xg, yg=yg, zg=zg, w0=w0, w1=w1, dxmin=m[0], dymin=m[1], dzmin=m[2])
+
+    def poisson(self):
+        n = self.ny - 1
+        d, l, u = np.zeros(n), np.zeros(max(n - 1, 1)), np.zeros(max(n - 1, 1))
+        kxx, kzz = np.zeros(self.nx - 2), np.zeros(self.nz - 2)
+        lib().orc_get_poisson(self._h, _p(d), _p(l), _p(u), _p(kxx), _p(kzz))
+        return dict(d=d, l=l[: n - 1], u=u[: n - 1], kxx=kxx, kzz=kzz)
+
+    def rk(self):
+        t, c = np.zeros(3), np.zeros(9)
+        lib().orc_get_rk(self._h, _p(t), _p(c))
+        return t, c.reshape(3, 3)
+
+    # reference subroutines
+    def apply_boundary_conditions(self):
+        lib().orc_apply_boundary_conditions(self._h)
+
+    def compute_rhs(self, U, V, W):
+        L = lib()
+        U, V, W = (np.asfortranarray(a, dtype=np.float64) for a in (U, V, W))
+        ru, rv, rw = self.zeros(U_), self.zeros(V_), self.zeros(W_)
+        L.orc_compute_rhs_u(self._h, _p(U), _p(V), _p(W), _p(ru))
+        L.orc_compute_rhs_v(self._h, _p(U), _p(V), _p(W), _p(rv))
+        L.orc_compute_rhs_w(self._h, _p(U), _p(V), _p(W), _p(rw))
+        return ru, rv, rw
+
+    def compute_eddy_viscosity(self, U, V, W):
+        U, V, W = (np.asfortranarray(a, dtype=np.float64) for a in (U, V, W))
+        nut = self.zeros(NUT)
+        lib().orc_compute_eddy_viscosity(self._h, _p(U), _p(V), _p(W), _p(nut))
+        return nut
+
+    def compute_wall_model(self, U, nu_t):
+        U = np.asfortranarray(U, dtype=np.float64)
+        nut = np.array(nu_t, dtype=np.float64, order="F", copy=True)
+        lib().orc_compute_wall_model(self._h, _p(U), _p(nut))
+        return nut
+
+    def compute_projection_step(self):
+        lib().orc_compute_projection_step(self._h)
+
+    def solve_poisson_equation(self, rhs_p):
+        p = np.array(rhs_p, dtype=np.float64, order="F", copy=True)
+        lib().orc_solve_poisson_equation(self._h, _p(p))
+        return p
+
+    def compute_mean_mass_flow_U(self, U):
+        return lib().orc_compute_mean_mass_flow_U(self._h, _p(np.asfortranarray(U, dtype=np.float64)))
+
+    def compute_dPx_for_constant_mass_flow(self, U):
+        return lib().orc_compute_dPx_for_constant_mass_flow(self._h, _p(np.asfortranarray(U, dtype=np.float64)))
+
+    def compute_dt(self):
+        return lib().orc_compute_dt(self._h)
+
+    def compute_time_step_RK3(self):
+        lib().orc_compute_time_step_RK3(self._h)
+
+    def check_divergence(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        lib().orc_check_divergence(self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def compute_Sij(self, U, V, W):
+        U, V, W = (np.asfortranarray(a, dtype=np.float64) for a in (U, V, W))
+        n = (self.nxg, self.nyg, self.nzg)
+        Sij = np.zeros(n + (6,), order="F")
+        S = np.zeros(n, order="F")
+        lib().orc_compute_Sij(self._h, _p(U), _p(V), _p(W), _p(Sij), _p(S))
+        return Sij, S
+
+    def dsm_parts(self, U, V, W):
+        U, V, W = (np.asfortranarray(a, dtype=np.float64) for a in (U, V, W))
+        n = (self.nxg, self.nyg, self.nzg)
+        L = np.zeros(n + (6,), order="F")
+        M = np.zeros(n + (6,), order="F")
+        LM, MM = np.zeros(self.nyg), np.zeros(self.nyg)
+        lib().orc_dsm_parts(self._h, _p(U), _p(V), _p(W), _p(L), _p(M), _p(LM), _p(MM))
+        return L, M, LM, MM
+
+    def equilibrium_alpha(self, U, h_wm=0.0, kappa=0.41, B=5.2):
+        lib().orc_equilibrium_alpha(self._h, _p(np.asfortranarray(U, dtype=np.float64)), h_wm, kappa, B)
+        return self.get(ALPHA)
+
+    def set_wall_model(self, mode, h_wm=0.0, kappa=0.41, B=5.2):
+        lib().orc_set_wall_model(self._h, mode, h_wm, kappa, B)
+
+    def plane_stats(self):
+        out = np.zeros(7 * self.ny)
+        lib().orc_plane_stats(self._h, _p(out))
+        return out.reshape(7, self.ny)
+
+
+def fft(a, sign=-1):
+    a = np.asarray(a, dtype=np.complex128)
+    buf = np.empty(2 * len(a))
+    buf[0::2], buf[1::2] = a.real, a.imag
+    lib().orc_fft(len(a), _p(buf), sign)
+    return buf[0::2] + 1j * buf[1::2]
+
+
+def zgtsv(dl, d, du, b):
+    n = len(d)
+
+    def il(a, m):
+        a = np.asarray(a, dtype=np.complex128)
+        buf = np.zeros(2 * max(m, 1))
+        buf[0:2 * m:2], buf[1:2 * m:2] = a.real[:m], a.imag[:m]
+        return buf
+    DL, D, DU, B = il(dl, n - 1), il(d, n), il(du, n - 1), il(b, n)
+    info = lib().orc_zgtsv(n, _p(DL), _p(D), _p(DU), _p(B))
+    return B[0::2][:n] + 1j * B[1::2][:n], info
+
+
+def filter_xzy(a):
+    a = np.asfortranarray(a, dtype=np.float64)
+    n1, n2, n3 = a.shape
+    out = np.zeros((n1 - 2, n2, n3 - 2), order="F")
+    lib().orc_filter_xzy(n1, n2, n3, _p(a), _p(out))
+    return out
+
+
+def loglaw_utau(u, h, nu, kappa=0.41, B=5.2):
+    return lib().orc_loglaw_utau(u, h, nu, kappa, B)

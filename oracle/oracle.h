@@ -1,0 +1,80 @@
+/*
+ * oracle.h -- FP64 CPU restatement of the channel_RL LES time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (channel_rl_amd/,
+ * include/) links, loads or calls this code.  It is the checker that the
+ * parity tests, __graft_entry__.smoke() and bench.py's cpu_baseline leg use.
+ *
+ * Arrays use the reference's Fortran shapes and column-major order (x
+ * fastest), 1-based in the formulas:
+ *   U (nx ,nyg,nzg)   V (nxg,ny ,nzg)   W (nxg,nyg,nz )
+ *   nu_t, P (nxg,nyg,nzg)            alpha_x (nx,2,nzg)
+ * with nxg=nx+1, nyg=ny+1, nzg=nz+1 (code/initialization.f90:91-111).
+ * Right-hand sides are returned in arrays of the field's own full shape,
+ * only the reference's interior range being written.
+ */
+#ifndef CHANNEL_ORACLE_H
+#define CHANNEL_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc orc;
+
+orc*  orc_create(int nx, int ny, int nz, const double* x, const double* y, const double* z,
+                 double nu, double CFL, double dPdx, int x_mass_cte, int LES_model,
+                 int Dirichlet_nu_t);
+void  orc_destroy(orc* o);
+
+/* setup constants (code/initialization.f90) */
+void  orc_get_grid(const orc* o, double* xg, double* yg, double* zg, double* w0, double* w1,
+                   double* mins3);
+void  orc_get_poisson(const orc* o, double* dyy_d, double* dyy_l, double* dyy_u,
+                      double* kxx, double* kzz);
+void  orc_get_rk(const orc* o, double* rk_t3, double* rk_coef9);
+
+/* state access: which = 0 U, 1 V, 2 W, 3 nu_t, 4 P, 5 alpha_x */
+long  orc_field_size(const orc* o, int which);
+void  orc_set_field(orc* o, int which, const double* src);
+void  orc_get_field(const orc* o, int which, double* dst);
+void  orc_set_scalars(orc* o, double dt, double t, double dPdx, double Qflow_x_0);
+void  orc_get_scalars(const orc* o, double* out4); /* dt, t, dPdx, Qflow_x_0 */
+
+/* reference subroutines */
+void   orc_apply_boundary_conditions(orc* o);
+void   orc_compute_rhs_u(const orc* o, const double* U, const double* V, const double* W, double* rhs);
+void   orc_compute_rhs_v(const orc* o, const double* U, const double* V, const double* W, double* rhs);
+void   orc_compute_rhs_w(const orc* o, const double* U, const double* V, const double* W, double* rhs);
+void   orc_compute_eddy_viscosity(orc* o, const double* U, const double* V, const double* W, double* nu_t);
+void   orc_compute_wall_model(orc* o, const double* U, double* nu_t);
+void   orc_compute_projection_step(orc* o);
+void   orc_solve_poisson_equation(orc* o, double* rhs_p);
+double orc_compute_mean_mass_flow_U(const orc* o, const double* U);
+double orc_compute_dPx_for_constant_mass_flow(const orc* o, const double* U);
+double orc_compute_dt(orc* o);
+void   orc_compute_time_step_RK3(orc* o);
+void   orc_set_wall_model(orc* o, int mode, double h_wm, double kappa, double B);
+void   orc_check_divergence(const orc* o, double* max_div, double* max_abs_div);
+
+/* helpers exposed for pinning */
+void   orc_fft(int n, double* data /* interleaved re,im */, int sign);
+int    orc_zgtsv(int n, double* dl, double* d, double* du, double* b); /* complex, interleaved */
+void   orc_filter_xzy(int n1, int n2, int n3, const double* in, double* out);
+void   orc_compute_Sij(const orc* o, const double* U, const double* V, const double* W,
+                       double* Sij /* 6 x (nxg,nyg,nzg) */, double* S /* (nxg,nyg,nzg) */);
+void   orc_dsm_parts(orc* o, const double* U, const double* V, const double* W,
+                     double* L /* 6 x box */, double* M /* 6 x box */,
+                     double* LM_plane /* nyg */, double* MM_plane /* nyg */);
+
+/* equilibrium log-law wall stress (project-defined closure, no reference counterpart) */
+double orc_loglaw_utau(double u, double h, double nu, double kappa, double B);
+void   orc_equilibrium_alpha(orc* o, const double* U, double h_wm, double kappa, double B);
+
+/* xz-plane statistics (code/statistics.f90:52-64) */
+void   orc_plane_stats(const orc* o, double* out /* 7 x ny */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
