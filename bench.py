@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Channel LES throughput on one MI355X: cell-RK3-steps per second.
+
+One step is compute_dt + compute_time_step_RK3 (three substeps of dynamic
+Smagorinsky, equilibrium wall model, RHS + RK combine, FFT/tridiagonal
+pressure projection, then the constant-mass-flow correction) over the whole
+channel resident in HBM.  Default workload: 192x97x192 (faces 194 97 194),
+Re_tau 2000, FP64, seeded log-law + 10% initial field (synthetic).
+
+Prints one JSON line: value = unique cells x steps / second / 1e6 (Mcells/s),
+plus the roofline of the dominant kernel (HIP-event durations measured in this
+run on the library's stream) and the CPU oracle timed on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+
+# Algorithmic HBM values per unique cell per launch (FP64: x8 bytes; FP32
+# bulk: bulk values x4, pressure values x8), the compulsory traffic of each
+# kernel: every input read once, every output written once.
+#   k_rhs_rk : reads U,V,W,nu_t (+Uo,F1 in substeps 2,3), writes U* (+F1 in
+#              substep 1): 10, 13, 13 -> 12 on average per launch
+#   k_dsm_lm : reads centre velocities (3), |S|S_ij (6), filtered u,v,w (3)
+#   k_dsm_prep: reads U,V,W, writes 3 + 6 + 1
+#   k_dsm_filtvel: reads U,V,W, writes 3
+#   k_dsm_nut: reads |S|, (U for the wall rows), writes nu_t
+#   k_fftx_fwd: reads U,V,W (divergence), writes p-hat (complex, half spectrum = 1 double/cell)
+#   k_fftz: reads and writes p-hat (1 complex per 2 cells each way: 2 doubles/cell)
+#   k_tridiag: reads p-hat twice, writes twice (forward + back substitution): 4
+#   k_fftx_inv: reads p-hat (1), writes p (1)
+#   k_project: reads p, U,V,W, writes U,V,W: 7
+ALG_VALUES = {
+    "k_rhs_rk": (12, 0),
+    "k_dsm_lm": (12, 0),
+    "k_dsm_prep": (13, 0),
+    "k_dsm_filtvel": (6, 0),
+    "k_dsm_nut": (2, 0),
+    "k_fftx_fwd": (3, 1),
+    "k_fftz": (0, 2),
+    "k_tridiag": (0, 4),
+    "k_fftx_inv": (0, 2),
+    "k_project": (6, 1),
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="wmles", choices=["small", "wmles", "large"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline (0: skip)")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs one channel on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is designed, not built: see DESIGN.md)", file=sys.stderr)
+        return 2
+    import channel_rl_amd as cr
+    from channel_rl_amd import cases
+
+    prec = cr.FP64 if args.precision == "fp64" else cr.FP32_BULK
+    case, field = cases.config(args.config, seed=1234)
+    ch = cr.Channel(case.x, case.y, case.z, precision=prec, **case.kwargs())
+    # initial field: boundary conditions, projection to div = 0, initial bulk flow
+    ch.set_state(*field)
+    ch.apply_boundary_conditions()
+    ch.compute_projection_step()
+    ch.initialize_mass_flow()
+    q0 = ch.scalars()["Qflow_x_0"]
+    ch.set_scalars(dt=0.0, t=0.0, dPdx=case.dPdx, Qflow_x_0=q0)
+    ch.set_wall_model(cr.WALL_EQUILIBRIUM, h_wm=case.h_wm)
+    cells = case.cells
+    log(f"[bench] {case.name} faces {case.nx} {case.ny} {case.nz}, {cells} cells, {args.precision}")
+
+    ch.run_steps(args.warmup)
+    ch.synchronize()
+    log(f"[bench] warmup {args.warmup} steps done")
+    t0 = time.perf_counter()
+    ch.run_steps(args.steps)
+    ch.synchronize()
+    t1 = time.perf_counter()
+    sec = t1 - t0
+    ms_step = 1e3 * sec / args.steps
+    value = cells * args.steps / sec / 1e6
+    log(f"[bench] timed {args.steps} steps: {ms_step:.3f} ms/step, {value:.1f} Mcells/s")
+    _, maxdiv = ch.check_divergence()
+
+    # per-kernel HIP-event durations on the library's stream
+    prof = ch.profile_steps(args.profile_steps)
+    tot = sum(v[0] for v in prof.values())
+    ranked = sorted(prof.items(), key=lambda kv: -kv[1][0])
+    for name, (ms, n) in ranked[:12]:
+        log(f"[bench]   {name:22s} {ms / args.profile_steps:8.3f} ms/step  {1e3 * ms / n:9.2f} us/launch  "
+            f"{100 * ms / tot:5.1f}%")
+    dom = next(n for n, _ in ranked if n in ALG_VALUES)
+    ms_tot, launches = prof[dom]
+    avg_s = ms_tot / launches / 1e3
+    bulk_b = 8 if prec == cr.FP64 else 4
+    vb, vp = ALG_VALUES[dom]
+    alg_bytes = cells * (vb * bulk_b + vp * 8)
+    achieved = alg_bytes / avg_s / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": alg_bytes,
+                "step_ms_profiled": round(tot / args.profile_steps, 3)}
+
+    cpu = None
+    if args.cpu_steps > 0:
+        import oracle
+        o = oracle.Oracle(case.x, case.y, case.z, **case.kwargs())
+        U, V, W = ch.state()
+        o.set_state(U, V, W)
+        o.set_scalars(dt=0.0, t=0.0, dPdx=0.0, Qflow_x_0=q0)
+        o.set_wall_model(1, h_wm=case.h_wm)
+        log(f"[bench] cpu baseline: oracle, {args.cpu_steps} step(s) at the same size")
+        c0 = time.perf_counter()
+        for _ in range(args.cpu_steps):
+            o.compute_dt()
+            o.compute_time_step_RK3()
+        c1 = time.perf_counter()
+        cpu = {"value": round(cells * args.cpu_steps / (c1 - c0) / 1e6, 4), "unit": "Mcells/s", "cores": 1,
+               "kind": "port", "sample": f"{args.cpu_steps} RK3 step(s) of the same {case.nx}x{case.ny}x{case.nz} "
+               f"channel, FP64 C oracle, 1 thread, {c1 - c0:.1f} s"}
+        log(f"[bench] cpu: {cpu['value']} Mcells/s")
+
+    out = {
+        "metric": "channel cell-RK3-steps/s (Mcells/s)",
+        "value": round(value, 2),
+        "unit": "Mcells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == cr.FP64 else "f32-bulk/f64-poisson",
+        "data": "synthetic",
+        "config": {"workload": f"channel LES RK3 step, {case.nx - 2}x{case.ny}x{case.nz - 2} "
+                               f"(faces {case.nx} {case.ny} {case.nz}), Re_tau 2000, dynamic Smagorinsky, "
+                               "equilibrium wall model, constant mass flow",
+                   "cells": cells, "parallelism": "single GPU", "max_abs_div": maxdiv},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

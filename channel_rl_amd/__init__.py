@@ -29,7 +29,7 @@ EXPORTS = [
     "chan_compute_dPx_for_constant_mass_flow", "chan_compute_dt",
     "chan_compute_time_step_RK3", "chan_check_divergence", "chan_set_wall_model",
     "chan_compute_equilibrium_wall_stress", "chan_plane_statistics", "chan_run_steps",
-    "chan_synchronize",
+    "chan_synchronize", "chan_profile_steps", "chan_profile_count", "chan_profile_entry",
 ]
 
 
@@ -80,6 +80,10 @@ def load_library(path=LIB_PATH):
     L.chan_compute_equilibrium_wall_stress.argtypes = [vp] + [ctypes.c_double] * 3
     L.chan_plane_statistics.argtypes = [vp, _D]
     L.chan_run_steps.argtypes = [vp, ctypes.c_int]
+    L.chan_profile_steps.argtypes = [vp, ctypes.c_int]
+    L.chan_profile_count.argtypes = [vp]
+    L.chan_profile_entry.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _D,
+                                     ctypes.POINTER(ctypes.c_long)]
     _LIB = L
     return L
 
@@ -213,6 +217,20 @@ class Channel:
 
     def run_steps(self, n):
         self._chk(load_library().chan_run_steps(self._h, n), "run_steps")
+
+    def profile_steps(self, n):
+        """Per-kernel HIP-event timing of n un-captured steps:
+        {kernel: (total_ms, launches)}."""
+        L = load_library()
+        self._chk(L.chan_profile_steps(self._h, n), "profile_steps")
+        out = {}
+        buf = ctypes.create_string_buffer(128)
+        for q in range(L.chan_profile_count(self._h)):
+            ms, cnt = ctypes.c_double(), ctypes.c_long()
+            self._chk(L.chan_profile_entry(self._h, q, buf, 128, ctypes.byref(ms), ctypes.byref(cnt)),
+                      "profile_entry")
+            out[buf.value.decode()] = (ms.value, cnt.value)
+        return out
 
     def synchronize(self):
         self._chk(load_library().chan_synchronize(self._h), "synchronize")

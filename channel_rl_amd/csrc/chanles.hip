@@ -11,8 +11,13 @@
 
 #include "../../include/chanles.h"
 #include "kernels.hpp"
+#include "prof.hpp"
 
 using namespace chan;
+
+namespace chan {
+thread_local Prof* g_prof = nullptr;
+}
 
 namespace {
 
@@ -85,6 +90,8 @@ struct chan_ctx {
     PoissonBufs pb{};
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
     bool use_graph = true;
+    Prof prof;
+    std::vector<std::pair<std::string, std::pair<double, long>>> prof_out;
 };
 
 namespace {
@@ -807,6 +814,45 @@ int chan_run_steps(chan_ctx* c, int nsteps)
         if ((rc = check_launch(c))) return rc;
         if ((rc = chan_compute_time_step_RK3(c))) return rc;
     }
+    return CHAN_OK;
+}
+
+int chan_profile_steps(chan_ctx* c, int nsteps)
+{
+    if (!c || nsteps < 1) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->prof.acc.clear();
+    const bool g = c->use_graph;
+    c->use_graph = false;
+    g_prof = &c->prof;
+    for (int n = 0; n < nsteps && rc == CHAN_OK; n++) {
+        DISPATCH(c, enqueue_dt, c);
+        rc = check_launch(c);
+        if (!rc) rc = chan_compute_time_step_RK3(c);
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (!rc && e != hipSuccess) rc = fail(c, CHAN_EHIP, std::string("profile sync: ") + hipGetErrorString(e));
+        c->prof.collect();
+    }
+    g_prof = nullptr;
+    c->use_graph = g;
+    c->prof_out.assign(c->prof.acc.begin(), c->prof.acc.end());
+    return rc;
+}
+
+int chan_profile_count(const chan_ctx* c) { return c ? (int)c->prof_out.size() : 0; }
+
+int chan_profile_entry(const chan_ctx* c, int idx, char* name, int len, double* total_ms, long* launches)
+{
+    if (!c || idx < 0 || idx >= (int)c->prof_out.size()) return CHAN_EINVAL;
+    const auto& e = c->prof_out[idx];
+    if (name && len > 0) {
+        std::strncpy(name, e.first.c_str(), (size_t)len - 1);
+        name[len - 1] = 0;
+    }
+    if (total_ms) *total_ms = e.second.first;
+    if (launches) *launches = e.second.second;
     return CHAN_OK;
 }
 

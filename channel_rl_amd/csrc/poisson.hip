@@ -15,6 +15,7 @@
 // contiguous kx runs and the y solve reads one mode per lane, coalesced.
 // FFTs are Stockham autosort passes of radix 2/3/4/5 in LDS.
 #include "kernels.hpp"
+#include "prof.hpp"
 
 namespace chan {
 
@@ -295,31 +296,45 @@ void launch_poisson_forward(const T* U, const T* V, const T* W, const Metric<dou
                             PoissonBufs& pb, hipStream_t s)
 {
     const int npairs = (d.ny - 1) * (d.nzp / 2);
+    PROF_BEGIN("k_fftx_fwd", s);
     k_fftx_fwd<T, false><<<cdiv(npairs, pb.bx), 256, fft_lds(d.nxp, pb.bx), s>>>(U, V, W, nullptr, md, d, pb);
+    PROF_END(s);
+    PROF_BEGIN("k_fftz", s);
     k_fftz<-1><<<dim3(cdiv(d.nkx, pb.bz), d.ny - 1), 256, fft_lds(d.nzp, pb.bz), s>>>(d, pb);
+    PROF_END(s);
 }
 
 void launch_poisson_forward_field(const double* rhs, const Dims& d, PoissonBufs& pb, hipStream_t s)
 {
     const int npairs = (d.ny - 1) * (d.nzp / 2);
     Metric<double> none{};
+    PROF_BEGIN("k_fftx_fwd", s);
     k_fftx_fwd<double, true><<<cdiv(npairs, pb.bx), 256, fft_lds(d.nxp, pb.bx), s>>>(nullptr, nullptr, nullptr, rhs,
                                                                                        none, d, pb);
+    PROF_END(s);
+    PROF_BEGIN("k_fftz", s);
     k_fftz<-1><<<dim3(cdiv(d.nkx, pb.bz), d.ny - 1), 256, fft_lds(d.nzp, pb.bz), s>>>(d, pb);
+    PROF_END(s);
 }
 
 void launch_poisson_solve(const Dims& d, PoissonBufs& pb, hipStream_t s)
 {
     const int nmodes = d.nkx * d.nzp;
+    PROF_BEGIN("k_tridiag", s);
     k_tridiag<<<cdiv(nmodes, TRI_THREADS), TRI_THREADS, (size_t)(d.ny - 1) * TRI_THREADS * sizeof(double), s>>>(d,
                                                                                                               pb);
+    PROF_END(s);
 }
 
 void launch_poisson_inverse(const Dims& d, PoissonBufs& pb, hipStream_t s)
 {
+    PROF_BEGIN("k_fftz", s);
     k_fftz<+1><<<dim3(cdiv(d.nkx, pb.bz), d.ny - 1), 256, fft_lds(d.nzp, pb.bz), s>>>(d, pb);
+    PROF_END(s);
     const int npairs = (d.ny - 1) * (d.nzp / 2);
+    PROF_BEGIN("k_fftx_inv", s);
     k_fftx_inv<<<cdiv(npairs, pb.bx), 256, fft_lds(d.nxp, pb.bx), s>>>(d, pb);
+    PROF_END(s);
 }
 
 // code/projection.f90:192-214: u -= grad p on the interior.
@@ -344,7 +359,9 @@ template <typename T>
 void launch_project(T* U, T* V, T* W, const double* p, const Metric<double>& md, const Dims& d, hipStream_t s)
 {
     dim3 blk(64, 4, 1), grd(cdiv(d.nxg - 2, 64), cdiv(d.nyg - 2, 4), d.nzg - 2);
+    PROF_BEGIN("k_project", s);
     k_project<T><<<grd, blk, 0, s>>>(U, V, W, p, md, d);
+    PROF_END(s);
     launch_bc<T>(U, d, KIND_U, s);
     launch_bc<T>(V, d, KIND_V, s);
     launch_bc<T>(W, d, KIND_W, s);

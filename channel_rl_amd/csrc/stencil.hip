@@ -6,6 +6,7 @@
 // rounding.  Reductions are two-level (fixed block partials, then one block
 // in fixed order) and therefore deterministic from run to run.
 #include "kernels.hpp"
+#include "prof.hpp"
 
 namespace chan {
 
@@ -67,7 +68,9 @@ void launch_bc(T* F, const Dims& d, int kind, hipStream_t s)
     const int n1 = xf ? d.nx : d.nxg, n2 = yf ? d.ny : d.nyg, n3 = zf ? d.nz : d.nzg;
     const int gx = xf ? 2 : 3, gz = zf ? 2 : 3;
     long tot = 2L * n1 * n3 + (long)gz * n1 * (n2 - 2) + (long)gx * (n2 - 2) * (n3 - gz);
+    PROF_BEGIN("k_bc", s);
     k_bc<T><<<cdiv(tot, 256), 256, 0, s>>>(F, d, kind);
+    PROF_END(s);
 }
 
 // ------------------------------------------------------------------- RHS
@@ -247,8 +250,10 @@ void launch_rhs_rk(const T* U, const T* V, const T* W, const T* nut, const Metri
 {
     dim3 blk(64, 4, 1);
     dim3 grd(cdiv(d.nxg - 2, 64), cdiv(d.nyg - 2, 4), d.nzg - 2);
+    PROF_BEGIN("k_rhs_rk", s);
     k_rhs_rk<T><<<grd, blk, 0, s>>>(U, V, W, nut, m, d, nu, sc, mode, Uo, Vo, Wo, F1u, F1v, F1w, Ou, Ov, Ow,
                                     T(c1), T(c2));
+    PROF_END(s);
 }
 
 // --------------------------------------------------------------- SGS: DSM
@@ -510,20 +515,30 @@ void launch_dsm(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, 
 {
     {
         dim3 blk(64, 4, 1), grd(cdiv(d.nxg, 64), cdiv(d.nyg - 2, 4), d.nzg);
+        PROF_BEGIN("k_dsm_prep", s);
         k_dsm_prep<T><<<grd, blk, 0, s>>>(U, V, W, m, d, b);
+        PROF_END(s);
     }
     {
         dim3 blk(64, 4, 1), grd(cdiv(d.nxg, 64), cdiv(d.nyg, 4), d.nzg);
+        PROF_BEGIN("k_dsm_filtvel", s);
         k_dsm_filtvel<T><<<grd, blk, 0, s>>>(U, V, W, d, b.uf, b.vf, b.wf);
+        PROF_END(s);
     }
     {
         dim3 blk(64, 1, 4), grd(cdiv(d.nxg - 2, 64), d.nyg - 2, cdiv(d.nzg - 2, 4));
+        PROF_BEGIN("k_dsm_lm", s);
         k_dsm_lm<T><<<grd, blk, 0, s>>>(m, d, b);
+        PROF_END(s);
     }
+    PROF_BEGIN("k_dsm_ratio", s);
     k_dsm_ratio<<<d.nyg - 2, 256, 0, s>>>(b.part, b.nblk, d.nyg, b.ratio);
+    PROF_END(s);
     {
         dim3 blk(64, 4, 1), grd(cdiv(d.nxg, 64), cdiv(d.nyg, 4), d.nzg);
+        PROF_BEGIN("k_dsm_nut", s);
         k_dsm_nut<T><<<grd, blk, 0, s>>>(nut, d, b, dirichlet, fuse_wm, U, alpha_x, m, nu);
+        PROF_END(s);
     }
 }
 
@@ -549,20 +564,21 @@ void launch_wall_model(const T* U, T* nut, const double* alpha_x, const Metric<T
                        hipStream_t s)
 {
     dim3 blk(64, 4, 1), grd(cdiv(d.nxg, 64), cdiv(d.nyg, 4), d.nzg);
+    PROF_BEGIN("k_wall_model", s);
     k_wall_model<T><<<grd, blk, 0, s>>>(U, nut, alpha_x, m, d, nu);
+    PROF_END(s);
 }
 
 // ------------------------------------------------ equilibrium wall stress
 
 // Log law U+ = ln(y+)/kappa + B above the crossover with U+ = y+, linear
 // below; Newton on f(ut) = ut (ln(h ut/nu)/kappa + B) - |u|.
-__device__ double loglaw_utau(double u, double h, double nu, double kappa, double B)
+// yc is the crossover y+ where ln(y+)/kappa + B = y+ (computed once on the host).
+__device__ double loglaw_utau(double u, double h, double nu, double kappa, double B, double yc)
 {
     double au = fabs(u);
     if (au == 0.0) return 0.0;
     double ut = sqrt(nu * au / h);
-    double yc = 11.0;
-    for (int it = 0; it < 50; it++) yc = yc - (yc - log(yc) / kappa - B) / (1.0 - 1.0 / (kappa * yc));
     if (h * ut / nu <= yc) return ut;
     double g = ut;
     for (int it = 0; it < 60; it++) {
@@ -570,17 +586,21 @@ __device__ double loglaw_utau(double u, double h, double nu, double kappa, doubl
         double f = g * (lg / kappa + B) - au;
         double fp = lg / kappa + B + 1.0 / kappa;
         double gn = g - f / fp;
-        g = gn > 0.5 * g ? gn : 0.5 * g;
+        gn = gn > 0.5 * g ? gn : 0.5 * g;
+        if (fabs(gn - g) <= 1e-15 * gn) { g = gn; break; }
+        g = gn;
     }
     return g;
 }
 
+// one thread per (x face, z centre, wall)
 template <typename T>
 __global__ void k_equilibrium_alpha(const T* U, double* alpha, Metric<double> m, Dims d, double nu, double h_wm,
-                                    double kappa, double B)
+                                    double kappa, double B, double yc)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int k = blockIdx.y + 1;
+    const int wall = blockIdx.z;
     if (i > d.nx) return;
     const double *y = m.y, *yg = m.yg;
     int jb = 2;
@@ -591,21 +611,35 @@ __global__ void k_equilibrium_alpha(const T* U, double* alpha, Metric<double> m,
         wb = (yg[jb + 1] - yt) / (yg[jb + 1] - yg[jb]);
         hb = h_wm;
     }
-    const int jt = d.nyg + 1 - jb;
-    const double ub = wb * (double)U[at(d, i, jb, k)] + (1.0 - wb) * (double)U[at(d, i, jb + 1, k)];
-    const double ut = wb * (double)U[at(d, i, jt, k)] + (1.0 - wb) * (double)U[at(d, i, jt - 1, k)];
-    const double tb = loglaw_utau(ub, hb, nu, kappa, B);
-    const double tt = loglaw_utau(ut, h_wm > 0.0 ? hb : y[d.ny] - yg[d.nyg - 1], nu, kappa, B);
-    alpha[(i - 1) + (long)d.nx * (0 + 2L * (k - 1))] = (ub >= 0 ? 1.0 : -1.0) * tb * tb;
-    alpha[(i - 1) + (long)d.nx * (1 + 2L * (k - 1))] = (ut >= 0 ? 1.0 : -1.0) * tt * tt;
+    double uw, h;
+    if (wall == 0) {
+        uw = wb * (double)U[at(d, i, jb, k)] + (1.0 - wb) * (double)U[at(d, i, jb + 1, k)];
+        h = hb;
+    } else {
+        const int jt = d.nyg + 1 - jb;
+        uw = wb * (double)U[at(d, i, jt, k)] + (1.0 - wb) * (double)U[at(d, i, jt - 1, k)];
+        h = h_wm > 0.0 ? hb : y[d.ny] - yg[d.nyg - 1];
+    }
+    const double t = loglaw_utau(uw, h, nu, kappa, B, yc);
+    alpha[(i - 1) + (long)d.nx * (wall + 2L * (k - 1))] = (uw >= 0 ? 1.0 : -1.0) * t * t;
+}
+
+static double loglaw_crossover(double kappa, double B)
+{
+    double yc = 11.0;
+    for (int it = 0; it < 50; it++) yc = yc - (yc - std::log(yc) / kappa - B) / (1.0 - 1.0 / (kappa * yc));
+    return yc;
 }
 
 template <typename T>
 void launch_equilibrium_alpha(const T* U, double* alpha_x, const Metric<double>& md, const Dims& d, double nu,
                               double h_wm, double kappa, double B, hipStream_t s)
 {
-    dim3 blk(64), grd(cdiv(d.nx, 64), d.nzg);
-    k_equilibrium_alpha<T><<<grd, blk, 0, s>>>(U, alpha_x, md, d, nu, h_wm, kappa, B);
+    dim3 blk(64), grd(cdiv(d.nx, 64), d.nzg, 2);
+    const double yc = loglaw_crossover(kappa, B);
+    PROF_BEGIN("k_equilibrium_alpha", s);
+    k_equilibrium_alpha<T><<<grd, blk, 0, s>>>(U, alpha_x, md, d, nu, h_wm, kappa, B, yc);
+    PROF_END(s);
 }
 
 // ------------------------------------------------------------- reductions
@@ -668,8 +702,12 @@ template <typename T>
 void launch_mass_flow_init(const T* U, const Metric<double>& md, const Dims& d, double* part, int nblk, Scalars* sc,
                            hipStream_t s)
 {
+    PROF_BEGIN("k_massflow_u", s);
     k_massflow_u<T><<<nblk, RED_THREADS, 0, s>>>(U, md, d, part, nblk);
+    PROF_END(s);
+    PROF_BEGIN("k_massflow_u_final", s);
     k_massflow_u_final<<<1, RED_THREADS, 0, s>>>(part, nblk, sc);
+    PROF_END(s);
 }
 
 // code/mass_flow.f90:113-166: mean of 0.5(U(:,3,:)+U(:,nyg-2,:)) over the
@@ -713,11 +751,17 @@ __global__ void k_shift_u(T* U, Dims d, const Scalars* sc)
 template <typename T>
 void launch_mass_flow_correct(T* U, const Dims& d, double* part, int nblk, Scalars* sc, int apply, hipStream_t s)
 {
+    PROF_BEGIN("k_dpx_partial", s);
     k_dpx_partial<T><<<nblk, RED_THREADS, 0, s>>>(U, d, part, nblk);
+    PROF_END(s);
+    PROF_BEGIN("k_dpx_final", s);
     k_dpx_final<<<1, RED_THREADS, 0, s>>>(part, nblk, sc);
+    PROF_END(s);
     if (apply) {
         dim3 blk(64, 4, 1), grd(cdiv(d.nx - 2, 64), cdiv(d.nyg - 2, 4), d.nzg - 2);
+        PROF_BEGIN("k_shift_u", s);
         k_shift_u<T><<<grd, blk, 0, s>>>(U, d, sc);
+        PROF_END(s);
         launch_bc<T>(U, d, KIND_U, s);
     }
 }
@@ -795,8 +839,12 @@ template <typename T>
 void launch_dt(const T* U, const T* V, const T* W, const Metric<double>& md, const Dims& d, double* part, int nblk,
                Scalars* sc, const double* mins3, hipStream_t s)
 {
+    PROF_BEGIN("k_dt_partial", s);
     k_dt_partial<T><<<nblk, RED_THREADS, 0, s>>>(U, V, W, md, d, part, nblk);
+    PROF_END(s);
+    PROF_BEGIN("k_dt_final", s);
     k_dt_final<<<1, RED_THREADS, 0, s>>>(part, nblk, sc, mins3);
+    PROF_END(s);
 }
 
 // code/projection.f90:223-245 (signed max) plus max |div|
@@ -837,8 +885,12 @@ template <typename T>
 void launch_divergence_check(const T* U, const T* V, const T* W, const Metric<double>& md, const Dims& d,
                              double* part, int nblk, Scalars* sc, hipStream_t s)
 {
+    PROF_BEGIN("k_div_partial", s);
     k_div_partial<T><<<nblk, RED_THREADS, 0, s>>>(U, V, W, md, d, part, nblk);
+    PROF_END(s);
+    PROF_BEGIN("k_div_final", s);
     k_div_final<<<1, RED_THREADS, 0, s>>>(part, nblk, sc);
+    PROF_END(s);
 }
 
 // code/statistics.f90:52-64 as plane means over the unique periodic points:
@@ -876,7 +928,9 @@ template <typename T>
 void launch_plane_stats(const T* U, const T* V, const T* W, const Metric<double>& md, const Dims& d, double* out,
                         hipStream_t s)
 {
+    PROF_BEGIN("k_plane_stats", s);
     k_plane_stats<T><<<d.ny, RED_THREADS, 0, s>>>(U, V, W, md, d, out);
+    PROF_END(s);
 }
 
 // ------------------------------------------------------- instantiations

@@ -113,6 +113,15 @@ int chan_plane_statistics(chan_ctx* ctx, double* out);
 int chan_run_steps(chan_ctx* ctx, int nsteps);
 int chan_synchronize(chan_ctx* ctx);
 
+/* Measurement (the reference times nmonitor steps with MPI_WTIME,
+ * code/monitor.f90:57,95,102; here per kernel): runs nsteps of
+ * compute_dt + compute_time_step_RK3 without graph capture, bracketing every
+ * kernel launch with HIP events on the context's stream, and keeps the
+ * per-kernel totals, read back with chan_profile_count / chan_profile_entry. */
+int chan_profile_steps(chan_ctx* ctx, int nsteps);
+int chan_profile_count(const chan_ctx* ctx);
+int chan_profile_entry(const chan_ctx* ctx, int idx, char* name, int len, double* total_ms, long* launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1104,7 +1104,9 @@ double orc_loglaw_utau(double u, double h, double nu, double kappa, double B)
         double f = g * (lg / kappa + B) - au;
         double fp = lg / kappa + B + 1.0 / kappa;
         double gn = g - f / fp;
-        g = gn > 0.5 * g ? gn : 0.5 * g;
+        gn = gn > 0.5 * g ? gn : 0.5 * g;
+        if (fabs(gn - g) <= 1e-15 * gn) { g = gn; break; }
+        g = gn;
     }
     return g;
 }

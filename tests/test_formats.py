@@ -1,0 +1,64 @@
+"""Host-side formats and generators: the reference's snapshot/.init layout,
+its input-parameter file, the grids and the seeded initial field."""
+import math
+
+import numpy as np
+
+from channel_rl_amd import cases, snapshot
+
+
+def test_snapshot_round_trip(tmp_path):
+    c, (U, V, W) = cases.small_case(10, 9, 12, seed=1)
+    P = np.random.default_rng(0).standard_normal((c.nxg, c.nyg, c.nzg))
+    path = str(tmp_path / "s.init")
+    snapshot.write(path, c.x, c.y, c.z, U, V, W, P, P * 2)
+    r = snapshot.read(path, with_extra=True)
+    for name, a in (("U", U), ("V", V), ("W", W), ("P", P), ("nu_t", 2 * P)):
+        assert np.array_equal(r[name], a)
+    assert np.array_equal(r["xm"], 0.5 * (c.x[1:] + c.x[:-1]))
+
+
+def test_snapshot_layout_matches_the_reference_reader(tmp_path):
+    # scripts/check_file.m: int32 n + float64 mesh arrays, then int32[3] dims per field
+    c, (U, V, W) = cases.small_case(10, 9, 12, seed=1)
+    path = str(tmp_path / "s.init")
+    snapshot.write(path, c.x, c.y, c.z, U, V, W)
+    raw = open(path, "rb").read()
+    n = np.frombuffer(raw[:4], "<i4")[0]
+    assert n == 10
+    off = 4 + 8 * 10
+    assert np.frombuffer(raw[off:off + 4], "<i4")[0] == 9
+    mesh = 4 * 6 + 8 * (10 + 9 + 12 + 9 + 8 + 11)
+    assert tuple(np.frombuffer(raw[mesh:mesh + 12], "<i4")) == (10, 10, 13)
+    total = mesh + 3 * 12 + 8 * (U.size + V.size + W.size)
+    assert len(raw) == total
+
+
+def test_parameter_file_format(tmp_path):
+    p = tmp_path / "input_parameters"
+    snapshot.write_parameters(str(p), 34, 33, 34, 0.5, 1 / 180, 1.0, 1, 2, 10, 11, 5, 5, "a.init", "out")
+    lines = p.read_text().splitlines()
+    assert len(lines) == 22 and lines[0].startswith("!") and lines[1].split() == ["34", "33", "34"]
+
+
+def test_grids():
+    y = cases.tanh_y(33, 1.5)
+    assert y[0] == 0.0 and y[-1] == 2.0 and np.all(np.diff(y) > 0)
+    assert np.allclose(y, 2.0 - y[::-1], atol=1e-14)
+    j = np.arange(33)
+    assert np.allclose(y, 1 + np.tanh(1.5 * (2 * j / 32 - 1)) / math.tanh(1.5), atol=1e-15)
+    x = cases.periodic_faces(34, 2 * math.pi)
+    assert math.isclose(x[-2], 2 * math.pi) and math.isclose(x[-1] - x[-2], x[1] - x[0])
+
+
+def test_noise_is_seeded_and_bounded():
+    a = cases.uniform_noise(1234, 1, (5, 6, 7))
+    b = cases.uniform_noise(1234, 1, (5, 6, 7))
+    c = cases.uniform_noise(1235, 1, (5, 6, 7))
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    big = cases.uniform_noise(7, 2, (64, 64, 64))
+    assert big.min() >= -1.0 and big.max() < 1.0 and abs(big.mean()) < 0.01
+
+
+def test_bulk_velocity_correlation():
+    assert 15.0 < cases.bulk_velocity_plus(180.0) < 16.5
