@@ -480,11 +480,9 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
         if ((rc = dalloc(c, &c->rhs[q], fb))) return rc;
     }
     if ((rc = dalloc(c, &c->nut, fb))) return rc;
-    if (c->LES_model == 2)
-        for (int q = 0; q < 13; q++)
-            if ((rc = dalloc(c, &c->dsm[q], fb))) return rc;
-    // partial sums: DSM rows use one slot per (x,z) block of the L:M kernel
-    c->nblk_dsm = cdiv(d.nxg - 2, 64) * cdiv(d.nzg - 2, 4);
+    if (c->LES_model == 2 && (rc = dalloc(c, &c->dsm[9], fb))) return rc;  // |S|
+    // partial sums: DSM rows use one slot per (x,z) block of the fused kernel
+    c->nblk_dsm = dsm_fused_nblk(d);
     c->nblk_red = 1024;
     size_t npart = std::max((size_t)(d.nyg + 1) * 2 * c->nblk_dsm, (size_t)3 * c->nblk_red);
     if ((rc = dalloc(c, (void**)&c->part, npart * sizeof(double)))) return rc;

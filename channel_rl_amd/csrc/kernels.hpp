@@ -32,6 +32,12 @@ template <typename T>
 void launch_dsm(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, const Dims& d,
                 DsmBufs<T>& b, int dirichlet, int fuse_wm, const double* alpha_x, T nu, hipStream_t s);
 
+// fused pass A of the dynamic Smagorinsky model (dsm_fused.hip)
+int dsm_fused_nblk(const Dims& d);
+template <typename T>
+void launch_dsm_fused(const T* U, const T* V, const T* W, const Metric<T>& m, const Dims& d, DsmBufs<T>& b,
+                      hipStream_t s);
+
 template <typename T>
 void launch_wall_model(const T* U, T* nut, const double* alpha_x, const Metric<T>& m, const Dims& d, T nu,
                        hipStream_t s);

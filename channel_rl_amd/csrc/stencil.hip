@@ -257,194 +257,7 @@ void launch_rhs_rk(const T* U, const T* V, const T* W, const T* nut, const Metri
 }
 
 // --------------------------------------------------------------- SGS: DSM
-
-// code/subgrid.f90:1014-1072 (rate of strain at a centre, off-diagonals
-// averaged from the four surrounding edges; |S| as sqrt(2 S_ij S_ij)).
-template <typename T>
-__device__ __forceinline__ void strain_at(const T* U, const T* V, const T* W, const Metric<T>& m, const Dims& d,
-                                          int i, int j, int k, T s[6], T& smag)
-{
-    auto u = [&](int a, int b, int c) { return U[at(d, a, b, c)]; };
-    auto v = [&](int a, int b, int c) { return V[at(d, a, b, c)]; };
-    auto w = [&](int a, int b, int c) { return W[at(d, a, b, c)]; };
-    const T h = T(0.5), q = T(0.25);
-    const T *x = m.x, *y = m.y, *z = m.z, *xg = m.xg, *yg = m.yg, *zg = m.zg;
-    s[0] = (u(i, j, k) - u(i - 1, j, k)) / (x[i] - x[i - 1]);
-    s[1] = (v(i, j, k) - v(i, j - 1, k)) / (y[j] - y[j - 1]);
-    s[2] = (w(i, j, k) - w(i, j, k - 1)) / (z[k] - z[k - 1]);
-    auto e4 = [&](int a, int b, int c) {
-        return h * ((u(a - 1, b, c) - u(a - 1, b - 1, c)) / (yg[b] - yg[b - 1]) +
-                    (v(a, b - 1, c) - v(a - 1, b - 1, c)) / (xg[a] - xg[a - 1]));
-    };
-    auto e5 = [&](int a, int b, int c) {
-        return h * ((u(a - 1, b, c) - u(a - 1, b, c - 1)) / (zg[c] - zg[c - 1]) +
-                    (w(a, b, c - 1) - w(a - 1, b, c - 1)) / (xg[a] - xg[a - 1]));
-    };
-    auto e6 = [&](int a, int b, int c) {
-        return h * ((v(a, b - 1, c) - v(a, b - 1, c - 1)) / (zg[c] - zg[c - 1]) +
-                    (w(a, b, c - 1) - w(a, b - 1, c - 1)) / (yg[b] - yg[b - 1]));
-    };
-    s[3] = q * (e4(i, j, k) + e4(i + 1, j, k) + e4(i, j + 1, k) + e4(i + 1, j + 1, k));
-    s[4] = q * (e5(i, j, k) + e5(i + 1, j, k) + e5(i, j, k + 1) + e5(i + 1, j, k + 1));
-    s[5] = q * (e6(i, j, k) + e6(i, j + 1, k) + e6(i, j, k + 1) + e6(i, j + 1, k + 1));
-    T a = T(0);
-    a = a + T(2) * s[0] * s[0];
-    a = a + T(2) * s[1] * s[1];
-    a = a + T(2) * s[2] * s[2];
-    a = a + T(4) * s[3] * s[3];
-    a = a + T(4) * s[4] * s[4];
-    a = a + T(4) * s[5] * s[5];
-    smag = sqrt(a);
-}
-
-// code/subgrid.f90:394-416, 464-468: centre velocities and |S| S_ij at every
-// centre of rows 2..nyg-1, periodic images evaluated at their source.
-template <typename T>
-__global__ void __launch_bounds__(256) k_dsm_prep(const T* __restrict__ U, const T* __restrict__ V,
-                                                   const T* __restrict__ W, Metric<T> m, Dims d, DsmBufs<T> b)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
-    const int k = blockIdx.z + 1;
-    if (i > d.nxg || j > d.nyg - 1) return;
-    const int ic = img_centre(i, d.nxg), kc = img_centre(k, d.nzg);
-    const T h = T(0.5);
-    const long p = at(d, i, j, k);
-    b.ac[0][p] = h * (U[at(d, ic - 1, j, kc)] + U[at(d, ic, j, kc)]);
-    b.ac[1][p] = h * (V[at(d, ic, j - 1, kc)] + V[at(d, ic, j, kc)]);
-    b.ac[2][p] = h * (W[at(d, ic, j, kc - 1)] + W[at(d, ic, j, kc)]);
-    T s[6], sm;
-    strain_at<T>(U, V, W, m, d, ic, j, kc, s, sm);
-    b.smag[p] = sm;
-#pragma unroll
-    for (int c = 0; c < 6; c++) b.ss[c][p] = sm * s[c];
-}
-
-// code/subgrid.f90:140-190 applied over a staggered field's whole y range
-// (ghost rows included), code/subgrid.f90:438-453 with periodic images.
-template <typename T>
-__device__ __forceinline__ T filter_full_at(const T* F, const Dims& d, int ii, int j, int kk, int n2)
-{
-    const T fil[3][3] = {{T(1.0 / 36.0), T(1.0 / 9.0), T(1.0 / 36.0)},
-                         {T(1.0 / 9.0), T(4.0 / 9.0), T(1.0 / 9.0)},
-                         {T(1.0 / 36.0), T(1.0 / 9.0), T(1.0 / 36.0)}};
-    const T c6 = T(1.0 / 6.0), c23 = T(2.0 / 3.0), c13 = T(1.0 / 3.0);
-    T acc = T(0);
-#pragma unroll
-    for (int a = -1; a <= 1; a++)
-#pragma unroll
-        for (int c = -1; c <= 1; c++) {
-            T yv;
-            if (j == 1)
-                yv = c23 * F[at(d, ii + a, 1, kk + c)] + c13 * F[at(d, ii + a, 2, kk + c)];
-            else if (j == n2)
-                yv = c13 * F[at(d, ii + a, n2 - 1, kk + c)] + c23 * F[at(d, ii + a, n2, kk + c)];
-            else
-                yv = c6 * F[at(d, ii + a, j - 1, kk + c)] + c23 * F[at(d, ii + a, j, kk + c)] +
-                     c6 * F[at(d, ii + a, j + 1, kk + c)];
-            acc = acc + yv * fil[a + 1][c + 1];
-        }
-    return acc;
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_dsm_filtvel(const T* __restrict__ U, const T* __restrict__ V,
-                                                      const T* __restrict__ W, Dims d, T* Uf, T* Vf, T* Wf)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    if (i > d.nxg || j > d.nyg) return;
-    const long p = at(d, i, j, k);
-    if (i <= d.nx) Uf[p] = filter_full_at<T>(U, d, img_face(i, d.nx), j, img_centre(k, d.nzg), d.nyg);
-    if (j <= d.ny) Vf[p] = filter_full_at<T>(V, d, img_centre(i, d.nxg), j, img_centre(k, d.nzg), d.ny);
-    if (k <= d.nz) Wf[p] = filter_full_at<T>(W, d, img_centre(i, d.nxg), j, img_face(k, d.nz), d.nyg);
-}
-
-// code/subgrid.f90:418-510: Leonard term L_ij, model tensor M_ij, and the
-// per-row partial sums of L:M and M:M over the plane i,k = 2..nxg-1,2..nzg-1.
-// The test filter on rows 2..nyg-1 is one-sided on the first/last row.
-template <typename T>
-__global__ void __launch_bounds__(256) k_dsm_lm(Metric<T> m, Dims d, DsmBufs<T> b)
-{
-    __shared__ double red[2][256];
-    const int tx = threadIdx.x, tz = threadIdx.z;
-    const int tid = tz * blockDim.x + tx;
-    const int i = blockIdx.x * blockDim.x + tx + 2;
-    const int j = blockIdx.y + 2;
-    const int k = blockIdx.z * blockDim.z + tz + 2;
-    double lm = 0.0, mm = 0.0;
-    if (i <= d.nxg - 1 && k <= d.nzg - 1) {
-        const T fil[3][3] = {{T(1.0 / 36.0), T(1.0 / 9.0), T(1.0 / 36.0)},
-                             {T(1.0 / 9.0), T(4.0 / 9.0), T(1.0 / 9.0)},
-                             {T(1.0 / 36.0), T(1.0 / 9.0), T(1.0 / 36.0)}};
-        const T c6 = T(1.0 / 6.0), c23 = T(2.0 / 3.0), c13 = T(1.0 / 3.0);
-        const int j0 = 2, j1 = d.nyg - 1;
-        T fa[3] = {T(0), T(0), T(0)};
-        T fp[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-        T fs[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-        const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
-        for (int a = -1; a <= 1; a++)
-            for (int c = -1; c <= 1; c++) {
-                const T f = fil[a + 1][c + 1];
-                const long pm = at(d, i + a, j - 1, k + c), p0 = at(d, i + a, j, k + c),
-                           pp = at(d, i + a, j + 1, k + c);
-                T am[3], a0[3], ap[3];
-#pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    a0[q] = b.ac[q][p0];
-                    am[q] = (j > j0) ? b.ac[q][pm] : T(0);
-                    ap[q] = (j < j1) ? b.ac[q][pp] : T(0);
-                }
-                auto ycomb = [&](T vm, T v0, T vp) {
-                    if (j == j0) return c23 * v0 + c13 * vp;
-                    if (j == j1) return c13 * vm + c23 * v0;
-                    return c6 * vm + c23 * v0 + c6 * vp;
-                };
-#pragma unroll
-                for (int q = 0; q < 6; q++) {
-                    T prm = am[pa[q]] * am[pb[q]], pr0 = a0[pa[q]] * a0[pb[q]], prp = ap[pa[q]] * ap[pb[q]];
-                    fp[q] = fp[q] + ycomb(prm, pr0, prp) * f;
-                }
-#pragma unroll
-                for (int q = 0; q < 3; q++) fa[q] = fa[q] + ycomb(am[q], a0[q], ap[q]) * f;
-#pragma unroll
-                for (int q = 0; q < 6; q++) {
-                    T sm_ = (j > j0) ? b.ss[q][pm] : T(0);
-                    T s0_ = b.ss[q][p0];
-                    T sp_ = (j < j1) ? b.ss[q][pp] : T(0);
-                    fs[q] = fs[q] + ycomb(sm_, s0_, sp_) * f;
-                }
-            }
-        T sh[6], shm;
-        strain_at<T>(b.uf, b.vf, b.wf, m, d, i, j, k, sh, shm);
-        T L[6], M[6];
-#pragma unroll
-        for (int q = 0; q < 6; q++) {
-            L[q] = fp[q] - fa[pa[q]] * fa[pb[q]];
-            M[q] = T(4) * shm * sh[q] - fs[q];
-        }
-        T tlm = L[0] * M[0] + L[1] * M[1] + L[2] * M[2] + T(2) * (L[3] * M[3] + L[4] * M[4] + L[5] * M[5]);
-        T tmm = M[0] * M[0] + M[1] * M[1] + M[2] * M[2] + T(2) * (M[3] * M[3] + M[4] * M[4] + M[5] * M[5]);
-        lm = (double)tlm;
-        mm = (double)tmm;
-    }
-    red[0][tid] = lm;
-    red[1][tid] = mm;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-        if (tid < st) {
-            red[0][tid] += red[0][tid + st];
-            red[1][tid] += red[1][tid + st];
-        }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        const int blk = blockIdx.z * gridDim.x + blockIdx.x;
-        b.part[((long)(j - 1) * 2 + 0) * b.nblk + blk] = red[0][0];
-        b.part[((long)(j - 1) * 2 + 1) * b.nblk + blk] = red[1][0];
-    }
-}
+// Pass A (filters, L_ij, M_ij, plane partial sums, |S|) is dsm_fused.hip.
 
 // <L:M>/<M:M> per row, blocks summed in a fixed order.
 __global__ void k_dsm_ratio(const double* part, int nblk, int nyg, double* ratio)
@@ -505,7 +318,7 @@ __global__ void __launch_bounds__(256) k_dsm_nut(T* nut, Dims d, DsmBufs<T> b, i
     }
     const int jj = (j == 1) ? 2 : (j == d.nyg ? d.nyg - 1 : j);
     const T r = T(b.ratio[jj - 1]);
-    const T v = fmax(T(-0.5) * r * b.smag[at(d, i, jj, k)], T(0));
+    const T v = fmax(T(-0.5) * r * b.smag[at(d, img_centre(i, d.nxg), jj, img_centre(k, d.nzg))], T(0));
     nut[p] = (jj != j && dirichlet) ? -v : v;
 }
 
@@ -513,24 +326,7 @@ template <typename T>
 void launch_dsm(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, const Dims& d, DsmBufs<T>& b,
                 int dirichlet, int fuse_wm, const double* alpha_x, T nu, hipStream_t s)
 {
-    {
-        dim3 blk(64, 4, 1), grd(cdiv(d.nxg, 64), cdiv(d.nyg - 2, 4), d.nzg);
-        PROF_BEGIN("k_dsm_prep", s);
-        k_dsm_prep<T><<<grd, blk, 0, s>>>(U, V, W, m, d, b);
-        PROF_END(s);
-    }
-    {
-        dim3 blk(64, 4, 1), grd(cdiv(d.nxg, 64), cdiv(d.nyg, 4), d.nzg);
-        PROF_BEGIN("k_dsm_filtvel", s);
-        k_dsm_filtvel<T><<<grd, blk, 0, s>>>(U, V, W, d, b.uf, b.vf, b.wf);
-        PROF_END(s);
-    }
-    {
-        dim3 blk(64, 1, 4), grd(cdiv(d.nxg - 2, 64), d.nyg - 2, cdiv(d.nzg - 2, 4));
-        PROF_BEGIN("k_dsm_lm", s);
-        k_dsm_lm<T><<<grd, blk, 0, s>>>(m, d, b);
-        PROF_END(s);
-    }
+    launch_dsm_fused<T>(U, V, W, m, d, b, s);
     PROF_BEGIN("k_dsm_ratio", s);
     k_dsm_ratio<<<d.nyg - 2, 256, 0, s>>>(b.part, b.nblk, d.nyg, b.ratio);
     PROF_END(s);
