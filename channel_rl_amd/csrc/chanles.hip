@@ -80,6 +80,7 @@ struct chan_ctx {
     void* rhs[3] = {};
     void* nut = nullptr;
     void* dsm[13] = {};
+    const void* rcp[2][6] = {};  // reciprocal spacings [double, float][dx, dy, dz, dxg, dyg, dzg]
     double* part = nullptr;
     int nblk_dsm = 0, nblk_red = 0;
     double* ratio = nullptr;
@@ -204,15 +205,13 @@ template <typename T>
 DsmBufs<T> dsm_bufs(chan_ctx* c)
 {
     DsmBufs<T> b;
-    for (int q = 0; q < 3; q++) b.ac[q] = (T*)c->dsm[q];
-    for (int q = 0; q < 6; q++) b.ss[q] = (T*)c->dsm[3 + q];
     b.smag = (T*)c->dsm[9];
-    b.uf = (T*)c->dsm[10];
-    b.vf = (T*)c->dsm[11];
-    b.wf = (T*)c->dsm[12];
     b.part = c->part;
     b.ratio = c->ratio;
     b.nblk = c->nblk_dsm;
+    const int pi = sizeof(T) == sizeof(double) ? 0 : 1;
+    b.rdx = (const T*)c->rcp[pi][0]; b.rdy = (const T*)c->rcp[pi][1]; b.rdz = (const T*)c->rcp[pi][2];
+    b.rdxg = (const T*)c->rcp[pi][3]; b.rdyg = (const T*)c->rcp[pi][4]; b.rdzg = (const T*)c->rcp[pi][5];
     return b;
 }
 
@@ -481,6 +480,20 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
     }
     if ((rc = dalloc(c, &c->nut, fb))) return rc;
     if (c->LES_model == 2 && (rc = dalloc(c, &c->dsm[9], fb))) return rc;  // |S|
+    {
+        // 1-based reciprocal spacings for the dynamic model
+        const std::vector<double>* f[6] = {&c->x, &c->y, &c->z, &c->xg, &c->yg, &c->zg};
+        const int nf[6] = {nx, ny, nz, nxg, nyg, nzg};
+        for (int q = 0; q < 6; q++) {
+            std::vector<double> r(nf[q] + 2, 0.0);
+            for (int i = 2; i <= nf[q]; i++) r[i] = 1.0 / ((*f[q])[i] - (*f[q])[i - 1]);
+            const double* pd;
+            const float* pf;
+            if ((rc = upload(c, r, &pd)) || (rc = upload_f(c, r, &pf))) return rc;
+            c->rcp[0][q] = pd;
+            c->rcp[1][q] = pf;
+        }
+    }
     // partial sums: DSM rows use one slot per (x,z) block of the fused kernel
     c->nblk_dsm = dsm_fused_nblk(d);
     c->nblk_red = 1024;

@@ -10,13 +10,12 @@ enum FieldKind { KIND_U = 0, KIND_V = 1, KIND_W = 2, KIND_C_DIRICHLET = 3, KIND_
 // Work buffers of the dynamic Smagorinsky model (box layout, type T).
 template <typename T>
 struct DsmBufs {
-    T* ac[3];     // centre velocities
-    T* ss[6];     // |S| S_ij
-    T* smag;      // |S|
-    T* uf; T* vf; T* wf;  // filtered staggered velocities
+    T* smag;       // |S| (box layout)
     double* part;  // per-block partial sums: [nyg][2][nblk]
     double* ratio; // <LM>/<MM> per row (index j-1); also LM, MM at +nyg, +2nyg
     int nblk;      // partial-sum blocks per row
+    // reciprocal spacings, 1-based: 1/(x(i)-x(i-1)), 1/(xg(i)-xg(i-1)), ...
+    const T *rdx, *rdy, *rdz, *rdxg, *rdyg, *rdzg;
 };
 
 template <typename T>

@@ -4,16 +4,16 @@
 //   x pass : divergence of u* computed on load, two z-rows packed into one
 //            complex FFT of length nxp (real-to-complex by the two-for-one
 //            split), modes kx = 0..nxp/2 kept.
-//   z pass : complex FFT of length nzp per (j, kx), in place.
-//   y solve: one thread per (kx,kz) mode, Thomas elimination of the real
-//            tridiagonal (Dyy + kxx + kzz) on the complex right-hand side;
-//            the 00 mode's first diagonal is scaled by 3/2 as in the
-//            reference.
+//   z pass : complex FFT of length nzp per (j, kx) line, in place.
+//   y solve: Thomas elimination of the real tridiagonal (Dyy + kxx + kzz)
+//            per mode, real and imaginary parts on separate lanes; the 00
+//            mode's first diagonal is scaled by 3/2 as in the reference.
 //   inverse: z pass, then complex-to-real x pass scaled by 1/(nxp nzp),
 //            written with the periodic images P(nxg-1)=P(2), P(nzg-1)=P(2).
-// Spectral layout ph[j-2][kz][kx] keeps kx fastest so that the z pass reads
-// contiguous kx runs and the y solve reads one mode per lane, coalesced.
-// FFTs are Stockham autosort passes of radix 2/3/4/5 in LDS.
+// Spectral layout ph[j-2][kx][kz]: the z pass reads whole contiguous lines,
+// the y solve reads one mode per lane (kz fastest), the x passes touch
+// 2B-long kz runs.  FFTs are in-place Stockham passes of radix 2/3/4/5 in
+// LDS, staged through registers, with the twiddle table copied to LDS.
 #include "kernels.hpp"
 #include "prof.hpp"
 
@@ -32,76 +32,109 @@ __device__ __forceinline__ double2 cmuli(double2 a, int sign)
     return sign < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
 }
 
-__device__ __forceinline__ double2 twid(const FftPlan& pl, int t, int sign)
+static constexpr int FFT_THREADS = 256;
+static constexpr int FFT_MAXNB = 2048;  // B*N per block
+
+// R-point DFT in registers (exp(sign 2 pi i / R) convention)
+template <int R>
+__device__ __forceinline__ void dft(double2 (&v)[R], const double2* tw, int N, int sign)
 {
-    double2 w = pl.tw[t];
-    return sign < 0 ? w : cconj(w);
+    if constexpr (R == 2) {
+        double2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        double2 s13 = cadd(v[1], v[3]), d13 = cmuli(csub(v[1], v[3]), sign);
+        v[0] = cadd(s02, s13);
+        v[1] = cadd(d02, d13);
+        v[2] = csub(s02, s13);
+        v[3] = csub(d02, d13);
+    } else if constexpr (R == 3) {
+        const double c = -0.5, sn = 0.86602540378443864676;
+        double2 s12 = cadd(v[1], v[2]), d12 = csub(v[1], v[2]);
+        double2 t1 = make_double2(v[0].x + c * s12.x, v[0].y + c * s12.y);
+        double2 t2 = cmuli(make_double2(sn * d12.x, sn * d12.y), sign);
+        v[0] = cadd(v[0], s12);
+        v[1] = cadd(t1, t2);
+        v[2] = csub(t1, t2);
+    } else {  // generic (5) from the table
+        double2 o[R];
+        const int st = N / R;
+#pragma unroll
+        for (int qq = 0; qq < R; qq++) {
+            double2 acc = v[0];
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                double2 w = tw[((r * qq) % R) * st];
+                acc = cadd(acc, cmul(v[r], sign < 0 ? w : cconj(w)));
+            }
+            o[qq] = acc;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = o[r];
+    }
 }
 
-// In-LDS batched Stockham FFT of B transforms of length pl.n.
-// NB layout: element (n,b) at n*B + b; otherwise at b*N + n.
-// Result is left in `a` (tmp is scratch of the same size).
-template <bool NB>
-__device__ void block_fft(double2* a, double2* tmp, const FftPlan& pl, int B, int sign)
+// One in-place Stockham pass of radix R over B transforms (element (n,b) at
+// b*ld + n).  Every thread loads all of its butterflies' inputs, the block
+// synchronises, then the outputs are stored: no second buffer is needed.
+template <int R>
+__device__ __forceinline__ void fft_pass(double2* a, const double2* tw, int N, int B, int ld, int Ns, int sign)
 {
-    const int N = pl.n;
-    double2* src = a;
-    double2* dst = tmp;
+    constexpr int MB = (FFT_MAXNB / R + FFT_THREADS - 1) / FFT_THREADS;
+    const int NR = N / R, nbf = NR * B, tws = N / (Ns * R);
+    double2 v[MB][R];
+#pragma unroll
+    for (int t = 0; t < MB; t++) {
+        const int q = threadIdx.x + t * FFT_THREADS;
+        if (q < nbf) {
+            const int jj = q % NR, b = q / NR;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[t][r] = a[b * ld + jj + r * NR];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < MB; t++) {
+        const int q = threadIdx.x + t * FFT_THREADS;
+        if (q < nbf) {
+            const int jj = q % NR, b = q / NR;
+            const int k = jj % Ns;
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    double2 w = tw[r * k * tws];
+                    v[t][r] = cmul(v[t][r], sign < 0 ? w : cconj(w));
+                }
+            }
+            dft<R>(v[t], tw, N, sign);
+            const int idxD = (jj / Ns) * Ns * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) a[b * ld + idxD + r * Ns] = v[t][r];
+        }
+    }
+    __syncthreads();
+}
+
+// In-place batched Stockham FFT of B transforms of length pl.n held in `a`
+// (element (n,b) at b*ld + n), twiddles tw[t] = exp(-2 pi i t/N) in LDS.
+__device__ void block_fft(double2* a, const double2* tw, const FftPlan& pl, int B, int ld, int sign)
+{
     int Ns = 1;
     for (int p = 0; p < pl.nr; p++) {
         const int R = pl.r[p];
-        const int NR = N / R;
-        const int nbf = NR * B;
-        const int tws = N / (Ns * R);
-        for (int q = threadIdx.x; q < nbf; q += blockDim.x) {
-            int b, jj;
-            if (NB) { b = q % B; jj = q / B; } else { jj = q % NR; b = q / NR; }
-            const int k = jj % Ns;
-            double2 v[5];
-            for (int r = 0; r < R; r++) v[r] = NB ? src[(jj + r * NR) * B + b] : src[b * N + jj + r * NR];
-            if (Ns > 1)
-                for (int r = 1; r < R; r++) v[r] = cmul(v[r], twid(pl, r * k * tws, sign));
-            double2 o[5];
-            if (R == 2) {
-                o[0] = cadd(v[0], v[1]);
-                o[1] = csub(v[0], v[1]);
-            } else if (R == 4) {
-                double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-                double2 s13 = cadd(v[1], v[3]), d13 = cmuli(csub(v[1], v[3]), sign);
-                o[0] = cadd(s02, s13);
-                o[1] = cadd(d02, d13);
-                o[2] = csub(s02, s13);
-                o[3] = csub(d02, d13);
-            } else if (R == 3) {
-                const double c = -0.5, sn = 0.86602540378443864676;
-                double2 s12 = cadd(v[1], v[2]), d12 = csub(v[1], v[2]);
-                double2 t1 = make_double2(v[0].x + c * s12.x, v[0].y + c * s12.y);
-                double2 t2 = cmuli(make_double2(sn * d12.x, sn * d12.y), sign);
-                o[0] = cadd(v[0], s12);
-                o[1] = cadd(t1, t2);
-                o[2] = csub(t1, t2);
-            } else {  // generic radix (5) from the table
-                const int st = N / R;
-                for (int qq = 0; qq < R; qq++) {
-                    double2 acc = v[0];
-                    for (int r = 1; r < R; r++) acc = cadd(acc, cmul(v[r], twid(pl, ((r * qq) % R) * st, sign)));
-                    o[qq] = acc;
-                }
-            }
-            const int idxD = (jj / Ns) * Ns * R + k;
-            for (int r = 0; r < R; r++) {
-                const int n = idxD + r * Ns;
-                if (NB) dst[n * B + b] = o[r]; else dst[b * N + n] = o[r];
-            }
-        }
-        __syncthreads();
-        double2* t = src; src = dst; dst = t;
+        if (R == 4) fft_pass<4>(a, tw, pl.n, B, ld, Ns, sign);
+        else if (R == 2) fft_pass<2>(a, tw, pl.n, B, ld, Ns, sign);
+        else if (R == 3) fft_pass<3>(a, tw, pl.n, B, ld, Ns, sign);
+        else fft_pass<5>(a, tw, pl.n, B, ld, Ns, sign);
         Ns *= R;
     }
-    if (src != a) {
-        for (int q = threadIdx.x; q < N * B; q += blockDim.x) a[q] = src[q];
-        __syncthreads();
-    }
+}
+
+__device__ __forceinline__ void load_twiddles(double2* tw, const FftPlan& pl)
+{
+    for (int q = threadIdx.x; q < pl.n; q += blockDim.x) tw[q] = pl.tw[q];
 }
 
 // divergence of the staggered velocity at centre (i,j,k), code/projection.f90:45-53
@@ -114,168 +147,193 @@ __device__ __forceinline__ double div_at(const T* U, const T* V, const T* W, con
            ((double)W[at(d, i, j, k)] - (double)W[at(d, i, j, k - 1)]) / (m.z[k] - m.z[k - 1]);
 }
 
-// split the packed transform Z = F[a + i b] into F[a], F[b] for kx = 0..nkx-1
-__device__ __forceinline__ void store_split(const double2* z, int N, int nkx, double2* outA, double2* outB, int b_,
-                                            bool valid)
+// pair pr -> (row jr = j-2, first z index kA-2) of the two packed z-rows
+__device__ __forceinline__ void pair_rows(const Dims& d, int pr, int& jr, int& kz)
 {
-    for (int q = threadIdx.x; q < nkx; q += blockDim.x) {
-        if (!valid) continue;
-        double2 Z = z[b_ * N + q];
-        double2 Zc = cconj(z[b_ * N + ((N - q) % N)]);
-        double2 s = cadd(Z, Zc), df = csub(Z, Zc);
-        outA[q] = make_double2(0.5 * s.x, 0.5 * s.y);
-        outB[q] = make_double2(0.5 * df.y, -0.5 * df.x);
-    }
+    jr = pr / (d.nzp / 2);
+    kz = 2 * (pr % (d.nzp / 2));
 }
 
 template <typename T, bool FROM_FIELD>
-__global__ void k_fftx_fwd(const T* U, const T* V, const T* W, const double* rhs, Metric<double> m, Dims d,
-                           PoissonBufs pb)
+__global__ void __launch_bounds__(FFT_THREADS) k_fftx_fwd(const T* U, const T* V, const T* W, const double* rhs,
+                                                          Metric<double> m, Dims d, PoissonBufs pb)
 {
     extern __shared__ double2 lds[];
-    const int N = d.nxp, B = pb.bx;
-    double2* a = lds;
-    double2* tmp = lds + (long)N * B;
+    const int N = d.nxp, B = pb.bx, ld = N + 1;
+    double2* tw = lds;
+    double2* a = lds + N;
     const int npairs = (d.ny - 1) * (d.nzp / 2);
     const int pair0 = blockIdx.x * B;
+    load_twiddles(tw, pb.fx);
     for (int q = threadIdx.x; q < N * B; q += blockDim.x) {
         const int b = q / N, n = q % N;
         const int pr = pair0 + b;
         double2 val = make_double2(0.0, 0.0);
         if (pr < npairs) {
-            const int j = pr / (d.nzp / 2) + 2;
-            const int kA = 2 * (pr % (d.nzp / 2)) + 2;
-            const int i = n + 2;
-            if (FROM_FIELD) {
+            int jr, kz;
+            pair_rows(d, pr, jr, kz);
+            const int j = jr + 2, kA = kz + 2, i = n + 2;
+            if (FROM_FIELD)
                 val = make_double2(rhs[at(d, i, j, kA)], rhs[at(d, i, j, kA + 1)]);
-            } else {
+            else
                 val = make_double2(div_at<T>(U, V, W, m, d, i, j, kA), div_at<T>(U, V, W, m, d, i, j, kA + 1));
-            }
         }
-        a[q] = val;
+        a[b * ld + n] = val;
     }
     __syncthreads();
-    block_fft<false>(a, tmp, pb.fx, B, -1);
-    for (int b = 0; b < B; b++) {
+    block_fft(a, tw, pb.fx, B, ld, -1);
+    // split Z = F[a + i b] into F[a], F[b]; consecutive lanes write consecutive kz
+    for (int q = threadIdx.x; q < d.nkx * B; q += blockDim.x) {
+        const int b = q % B, kx = q / B;
         const int pr = pair0 + b;
-        if (pr >= npairs) break;
-        const int jr = pr / (d.nzp / 2);
-        const int kA = 2 * (pr % (d.nzp / 2));
-        double2* outA = pb.ph + ((long)jr * d.nzp + kA) * d.nkx;
-        double2* outB = outA + d.nkx;
-        store_split(a, N, d.nkx, outA, outB, b, true);
+        if (pr >= npairs) continue;
+        int jr, kz;
+        pair_rows(d, pr, jr, kz);
+        const double2 Z = a[b * ld + kx];
+        const double2 Zc = cconj(a[b * ld + ((N - kx) % N)]);
+        const double2 s = cadd(Z, Zc), df = csub(Z, Zc);
+        double2* out = pb.ph + ((long)jr * d.nkx + kx) * d.nzp + kz;
+        out[0] = make_double2(0.5 * s.x, 0.5 * s.y);
+        out[1] = make_double2(0.5 * df.y, -0.5 * df.x);
     }
 }
 
 template <int SIGN>
-__global__ void k_fftz(Dims d, PoissonBufs pb)
+__global__ void __launch_bounds__(FFT_THREADS) k_fftz(Dims d, PoissonBufs pb)
 {
     extern __shared__ double2 lds[];
-    const int N = d.nzp, B = pb.bz;
-    double2* a = lds;
-    double2* tmp = lds + (long)N * B;
-    const int kx0 = blockIdx.x * B;
-    const int jr = blockIdx.y;
-    double2* base = pb.ph + (long)jr * d.nzp * d.nkx;
+    const int N = d.nzp, B = pb.bz, ld = N + 1;
+    double2* tw = lds;
+    double2* a = lds + N;
+    const long nlines = (long)(d.ny - 1) * d.nkx;
+    const long line0 = (long)blockIdx.x * B;
+    load_twiddles(tw, pb.fz);
     for (int q = threadIdx.x; q < N * B; q += blockDim.x) {
-        const int b = q % B, n = q / B;
-        const int kx = kx0 + b;
-        a[q] = kx < d.nkx ? base[(long)n * d.nkx + kx] : make_double2(0.0, 0.0);
+        const int b = q / N, n = q % N;
+        a[b * ld + n] = (line0 + b < nlines) ? pb.ph[(line0 + b) * N + n] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    block_fft<true>(a, tmp, pb.fz, B, SIGN);
+    block_fft(a, tw, pb.fz, B, ld, SIGN);
     for (int q = threadIdx.x; q < N * B; q += blockDim.x) {
-        const int b = q % B, n = q / B;
-        const int kx = kx0 + b;
-        if (kx < d.nkx) base[(long)n * d.nkx + kx] = a[q];
+        const int b = q / N, n = q % N;
+        if (line0 + b < nlines) pb.ph[(line0 + b) * N + n] = a[b * ld + n];
     }
 }
 
-// code/projection.f90:90-113: (Dyy + kxx + kzz) p = r per mode, Thomas
-// elimination (the matrix is real and weakly diagonally dominant).
+// code/projection.f90:90-113: (Dyy + kxx + kzz) p = r per mode.  Lane pairs
+// share a mode (real, imaginary part); c' is kept in LDS for the back
+// substitution, d' is written over the right-hand side.
 static constexpr int TRI_THREADS = 64;
+static constexpr int TRI_UNROLL = 8;
 
 __global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
 {
     extern __shared__ double cp[];  // [row][lane]
     const int lane = threadIdx.x;
-    const int mode = blockIdx.x * TRI_THREADS + lane;
+    const long g = (long)blockIdx.x * TRI_THREADS + lane;
     const int nmodes = d.nkx * d.nzp;
-    if (mode >= nmodes) return;
-    const int kx = mode % d.nkx, kz = mode / d.nkx;
+    if (g >= 2L * nmodes) return;
+    const int mode = (int)(g >> 1);
+    const int kx = mode / d.nzp, kz = mode % d.nzp;
     const int n = d.ny - 1;
-    const long plane = (long)nmodes;
-    double2* ph = pb.ph + mode;
+    const long plane = 2L * nmodes;
+    double* ph = reinterpret_cast<double*>(pb.ph) + g;
     const double lx = pb.kxx[kx], lz = pb.kzz[kz];
     double den = (pb.dd[0] + lx) + lz;
     if (kx == 0 && kz == 0) den = 3.0 / 2.0 * den;
     double inv = 1.0 / den;
     double c = pb.du[0] * inv;
     cp[lane] = c;
-    double2 r = ph[0];
-    double2 dp = make_double2(r.x * inv, r.y * inv);
+    double dp = ph[0] * inv;
     ph[0] = dp;
-    for (int row = 1; row < n; row++) {
+    int row = 1;
+    for (; row + TRI_UNROLL <= n; row += TRI_UNROLL) {
+        double r[TRI_UNROLL];
+#pragma unroll
+        for (int u = 0; u < TRI_UNROLL; u++) r[u] = ph[(row + u) * plane];
+#pragma unroll
+        for (int u = 0; u < TRI_UNROLL; u++) {
+            const int rr = row + u;
+            const double l = pb.dl[rr - 1];
+            den = ((pb.dd[rr] + lx) + lz) - l * c;
+            inv = 1.0 / den;
+            dp = (r[u] - l * dp) * inv;
+            ph[rr * plane] = dp;
+            if (rr < n - 1) {
+                c = pb.du[rr] * inv;
+                cp[rr * TRI_THREADS + lane] = c;
+            }
+        }
+    }
+    for (; row < n; row++) {
         const double l = pb.dl[row - 1];
         den = ((pb.dd[row] + lx) + lz) - l * c;
         inv = 1.0 / den;
-        r = ph[row * plane];
-        dp = make_double2((r.x - l * dp.x) * inv, (r.y - l * dp.y) * inv);
+        dp = (ph[row * plane] - l * dp) * inv;
         ph[row * plane] = dp;
         if (row < n - 1) {
             c = pb.du[row] * inv;
             cp[row * TRI_THREADS + lane] = c;
         }
     }
-    double2 xn = dp;
-    for (int row = n - 2; row >= 0; row--) {
-        const double cr = cp[row * TRI_THREADS + lane];
-        double2 dr = ph[row * plane];
-        xn = make_double2(dr.x - cr * xn.x, dr.y - cr * xn.y);
+    double xn = dp;
+    row = n - 2;
+    for (; row - TRI_UNROLL + 1 >= 0; row -= TRI_UNROLL) {
+        double r[TRI_UNROLL];
+#pragma unroll
+        for (int u = 0; u < TRI_UNROLL; u++) r[u] = ph[(row - u) * plane];
+#pragma unroll
+        for (int u = 0; u < TRI_UNROLL; u++) {
+            xn = r[u] - cp[(row - u) * TRI_THREADS + lane] * xn;
+            ph[(row - u) * plane] = xn;
+        }
+    }
+    for (; row >= 0; row--) {
+        xn = ph[row * plane] - cp[row * TRI_THREADS + lane] * xn;
         ph[row * plane] = xn;
     }
 }
 
-__global__ void k_fftx_inv(Dims d, PoissonBufs pb)
+__global__ void __launch_bounds__(FFT_THREADS) k_fftx_inv(Dims d, PoissonBufs pb)
 {
     extern __shared__ double2 lds[];
-    const int N = d.nxp, B = pb.bx;
-    double2* a = lds;
-    double2* tmp = lds + (long)N * B;
+    const int N = d.nxp, B = pb.bx, ld = N + 1;
+    double2* tw = lds;
+    double2* a = lds + N;
     const int npairs = (d.ny - 1) * (d.nzp / 2);
     const int pair0 = blockIdx.x * B;
     const int half = N / 2;
-    for (int q = threadIdx.x; q < N * B; q += blockDim.x) {
-        const int b = q / N, n = q % N;
+    load_twiddles(tw, pb.fx);
+    // consecutive lanes read consecutive kz of one kx line
+    for (int q = threadIdx.x; q < d.nkx * B; q += blockDim.x) {
+        const int b = q % B, n = q / B;
         const int pr = pair0 + b;
-        double2 z = make_double2(0.0, 0.0);
+        double2 xa = make_double2(0.0, 0.0), xb = xa;
         if (pr < npairs) {
-            const int jr = pr / (d.nzp / 2);
-            const int kA = 2 * (pr % (d.nzp / 2));
-            const double2* inA = pb.ph + ((long)jr * d.nzp + kA) * d.nkx;
-            const double2* inB = inA + d.nkx;
-            double2 xa, xb;
-            if (n <= half) {
-                xa = inA[n]; xb = inB[n];
-                if (n == 0 || n == half) { xa.y = 0.0; xb.y = 0.0; }
-            } else {
-                xa = cconj(inA[N - n]); xb = cconj(inB[N - n]);
-            }
-            z = make_double2(xa.x - xb.y, xa.y + xb.x);
+            int jr, kz;
+            pair_rows(d, pr, jr, kz);
+            const double2* in = pb.ph + ((long)jr * d.nkx + n) * d.nzp + kz;
+            xa = in[0];
+            xb = in[1];
         }
-        a[q] = z;
+        if (n == 0 || n == half) { xa.y = 0.0; xb.y = 0.0; }
+        a[b * ld + n] = make_double2(xa.x - xb.y, xa.y + xb.x);
+        if (n > 0 && n < half) {
+            const double2 ca = cconj(xa), cb = cconj(xb);
+            a[b * ld + N - n] = make_double2(ca.x - cb.y, ca.y + cb.x);
+        }
     }
     __syncthreads();
-    block_fft<false>(a, tmp, pb.fx, B, +1);
+    block_fft(a, tw, pb.fx, B, ld, +1);
     const double scale = (double)((long)d.nxp * d.nzp);
     for (int q = threadIdx.x; q < N * B; q += blockDim.x) {
         const int b = q / N, n = q % N;
         const int pr = pair0 + b;
         if (pr >= npairs) continue;
-        const int j = pr / (d.nzp / 2) + 2;
-        const int kA = 2 * (pr % (d.nzp / 2)) + 2;
-        const double pa = a[q].x / scale, pbv = a[q].y / scale;
-        const int i = n + 2;
+        int jr, kz;
+        pair_rows(d, pr, jr, kz);
+        const int j = jr + 2, kA = kz + 2, i = n + 2;
+        const double pa = a[b * ld + n].x / scale, pbv = a[b * ld + n].y / scale;
         pb.p[at(d, i, j, kA)] = pa;
         pb.p[at(d, i, j, kA + 1)] = pbv;
         if (n == 0) {
@@ -289,7 +347,7 @@ __global__ void k_fftx_inv(Dims d, PoissonBufs pb)
     }
 }
 
-static size_t fft_lds(int N, int B) { return (size_t)2 * N * B * sizeof(double2); }
+static size_t fft_lds(int N, int B) { return (size_t)(N + (size_t)(N + 1) * B) * sizeof(double2); }
 
 template <typename T>
 void launch_poisson_forward(const T* U, const T* V, const T* W, const Metric<double>& md, const Dims& d,
@@ -297,10 +355,11 @@ void launch_poisson_forward(const T* U, const T* V, const T* W, const Metric<dou
 {
     const int npairs = (d.ny - 1) * (d.nzp / 2);
     PROF_BEGIN("k_fftx_fwd", s);
-    k_fftx_fwd<T, false><<<cdiv(npairs, pb.bx), 256, fft_lds(d.nxp, pb.bx), s>>>(U, V, W, nullptr, md, d, pb);
+    k_fftx_fwd<T, false><<<cdiv(npairs, pb.bx), FFT_THREADS, fft_lds(d.nxp, pb.bx), s>>>(U, V, W, nullptr, md, d,
+                                                                                          pb);
     PROF_END(s);
     PROF_BEGIN("k_fftz", s);
-    k_fftz<-1><<<dim3(cdiv(d.nkx, pb.bz), d.ny - 1), 256, fft_lds(d.nzp, pb.bz), s>>>(d, pb);
+    k_fftz<-1><<<cdiv((long)(d.ny - 1) * d.nkx, pb.bz), FFT_THREADS, fft_lds(d.nzp, pb.bz), s>>>(d, pb);
     PROF_END(s);
 }
 
@@ -309,31 +368,32 @@ void launch_poisson_forward_field(const double* rhs, const Dims& d, PoissonBufs&
     const int npairs = (d.ny - 1) * (d.nzp / 2);
     Metric<double> none{};
     PROF_BEGIN("k_fftx_fwd", s);
-    k_fftx_fwd<double, true><<<cdiv(npairs, pb.bx), 256, fft_lds(d.nxp, pb.bx), s>>>(nullptr, nullptr, nullptr, rhs,
-                                                                                       none, d, pb);
+    k_fftx_fwd<double, true><<<cdiv(npairs, pb.bx), FFT_THREADS, fft_lds(d.nxp, pb.bx), s>>>(nullptr, nullptr,
+                                                                                              nullptr, rhs, none, d,
+                                                                                              pb);
     PROF_END(s);
     PROF_BEGIN("k_fftz", s);
-    k_fftz<-1><<<dim3(cdiv(d.nkx, pb.bz), d.ny - 1), 256, fft_lds(d.nzp, pb.bz), s>>>(d, pb);
+    k_fftz<-1><<<cdiv((long)(d.ny - 1) * d.nkx, pb.bz), FFT_THREADS, fft_lds(d.nzp, pb.bz), s>>>(d, pb);
     PROF_END(s);
 }
 
 void launch_poisson_solve(const Dims& d, PoissonBufs& pb, hipStream_t s)
 {
-    const int nmodes = d.nkx * d.nzp;
+    const long lanes = 2L * d.nkx * d.nzp;
     PROF_BEGIN("k_tridiag", s);
-    k_tridiag<<<cdiv(nmodes, TRI_THREADS), TRI_THREADS, (size_t)(d.ny - 1) * TRI_THREADS * sizeof(double), s>>>(d,
-                                                                                                              pb);
+    k_tridiag<<<cdiv(lanes, TRI_THREADS), TRI_THREADS, (size_t)(d.ny - 1) * TRI_THREADS * sizeof(double), s>>>(d,
+                                                                                                             pb);
     PROF_END(s);
 }
 
 void launch_poisson_inverse(const Dims& d, PoissonBufs& pb, hipStream_t s)
 {
     PROF_BEGIN("k_fftz", s);
-    k_fftz<+1><<<dim3(cdiv(d.nkx, pb.bz), d.ny - 1), 256, fft_lds(d.nzp, pb.bz), s>>>(d, pb);
+    k_fftz<+1><<<cdiv((long)(d.ny - 1) * d.nkx, pb.bz), FFT_THREADS, fft_lds(d.nzp, pb.bz), s>>>(d, pb);
     PROF_END(s);
     const int npairs = (d.ny - 1) * (d.nzp / 2);
     PROF_BEGIN("k_fftx_inv", s);
-    k_fftx_inv<<<cdiv(npairs, pb.bx), 256, fft_lds(d.nxp, pb.bx), s>>>(d, pb);
+    k_fftx_inv<<<cdiv(npairs, pb.bx), FFT_THREADS, fft_lds(d.nxp, pb.bx), s>>>(d, pb);
     PROF_END(s);
 }
 

@@ -73,188 +73,7 @@ void launch_bc(T* F, const Dims& d, int kind, hipStream_t s)
     PROF_END(s);
 }
 
-// ------------------------------------------------------------------- RHS
-
-// code/equations.f90:24-167
-template <typename T>
-__device__ __forceinline__ T rhs_u_at(const T* U, const T* V, const T* W, const T* nt, const Metric<T>& m,
-                                      const Dims& d, T nu, T dPdx, int i, int j, int k)
-{
-    auto u = [&](int a, int b, int c) { return U[at(d, a, b, c)]; };
-    auto v = [&](int a, int b, int c) { return V[at(d, a, b, c)]; };
-    auto w = [&](int a, int b, int c) { return W[at(d, a, b, c)]; };
-    auto n = [&](int a, int b, int c) { return nt[at(d, a, b, c)]; };
-    const T h = T(0.5), q = T(0.25), one = T(1);
-    const T *x = m.x, *y = m.y, *z = m.z, *xg = m.xg, *yg = m.yg, *zg = m.zg, *w0 = m.w0, *w1 = m.w1;
-    T uc = h * (u(i, j, k) + u(i + 1, j, k));
-    T ucm = h * (u(i - 1, j, k) + u(i, j, k));
-    T t1 = -(uc * uc - ucm * ucm) / (xg[i + 1] - xg[i]);
-    T uvp = (w0[j] * u(i, j, k) + w1[j] * u(i, j + 1, k)) * (h * (v(i, j, k) + v(i + 1, j, k)));
-    T uvm = (w0[j - 1] * u(i, j - 1, k) + w1[j - 1] * u(i, j, k)) * (h * (v(i, j - 1, k) + v(i + 1, j - 1, k)));
-    T t2 = -(uvp - uvm) / (y[j] - y[j - 1]);
-    T uwp = (h * (u(i, j, k) + u(i, j, k + 1))) * (h * (w(i, j, k) + w(i + 1, j, k)));
-    T uwm = (h * (u(i, j, k - 1) + u(i, j, k))) * (h * (w(i, j, k - 1) + w(i + 1, j, k - 1)));
-    T t3 = -(uwp - uwm) / (z[k] - z[k - 1]);
-    T dz_1 = zg[k] - zg[k - 1], dz_2 = zg[k + 1] - zg[k], dz_3 = z[k] - z[k - 1];
-    T dy_3 = y[j] - y[j - 1];
-    T dx_1 = x[i] - x[i - 1], dx_2 = x[i + 1] - x[i], dx_3 = xg[i + 1] - xg[i];
-    T nu_x1 = nu + n(i, j, k);
-    T nu_x2 = nu + n(i + 1, j, k);
-    T nu_y1 = nu + h * (w0[j - 1] * n(i, j - 1, k) + w1[j - 1] * n(i, j, k) + w0[j - 1] * n(i + 1, j - 1, k) +
-                        w1[j - 1] * n(i + 1, j, k));
-    T nu_y2 = nu + h * (w0[j] * n(i, j, k) + w1[j] * n(i, j + 1, k) + w0[j] * n(i + 1, j, k) +
-                        w1[j] * n(i + 1, j + 1, k));
-    T nu_z1 = nu + q * (n(i, j, k) + n(i, j, k - 1) + n(i + 1, j, k) + n(i + 1, j, k - 1));
-    T nu_z2 = nu + q * (n(i, j, k) + n(i, j, k + 1) + n(i + 1, j, k) + n(i + 1, j, k + 1));
-    T dudy_p = (u(i, j + 1, k) - u(i, j, k)) / (yg[j + 1] - yg[j]);
-    T dudy_m = (u(i, j, k) - u(i, j - 1, k)) / (yg[j] - yg[j - 1]);
-    T t4 = one / dx_3 * (nu_x2 * one / dx_2 * (u(i + 1, j, k) - u(i, j, k)) -
-                         nu_x1 * one / dx_1 * (u(i, j, k) - u(i - 1, j, k))) +
-           one / dy_3 * (nu_y2 * dudy_p - nu_y1 * dudy_m) +
-           one / dz_3 * (nu_z2 * one / dz_2 * (u(i, j, k + 1) - u(i, j, k)) -
-                         nu_z1 * one / dz_1 * (u(i, j, k) - u(i, j, k - 1)));
-    return (((t1 + t2) + t3) + t4) + dPdx;
-}
-
-// code/equations.f90:174-320
-template <typename T>
-__device__ __forceinline__ T rhs_v_at(const T* U, const T* V, const T* W, const T* nt, const Metric<T>& m,
-                                      const Dims& d, T nu, int i, int j, int k)
-{
-    auto u = [&](int a, int b, int c) { return U[at(d, a, b, c)]; };
-    auto v = [&](int a, int b, int c) { return V[at(d, a, b, c)]; };
-    auto w = [&](int a, int b, int c) { return W[at(d, a, b, c)]; };
-    auto n = [&](int a, int b, int c) { return nt[at(d, a, b, c)]; };
-    const T h = T(0.5), one = T(1);
-    const T *x = m.x, *y = m.y, *z = m.z, *xg = m.xg, *yg = m.yg, *zg = m.zg, *w0 = m.w0, *w1 = m.w1;
-    T vc = h * (v(i, j, k) + v(i, j + 1, k));
-    T vcm = h * (v(i, j - 1, k) + v(i, j, k));
-    T t1 = -(vc * vc - vcm * vcm) / (yg[j + 1] - yg[j]);
-    T uvp = (w0[j] * u(i, j, k) + w1[j] * u(i, j + 1, k)) * (h * (v(i, j, k) + v(i + 1, j, k)));
-    T uvm = (w0[j] * u(i - 1, j, k) + w1[j] * u(i - 1, j + 1, k)) * (h * (v(i - 1, j, k) + v(i, j, k)));
-    T t2 = -(uvp - uvm) / (x[i] - x[i - 1]);
-    T vwp = (h * (v(i, j, k) + v(i, j, k + 1))) * (w0[j] * w(i, j, k) + w1[j] * w(i, j + 1, k));
-    T vwm = (h * (v(i, j, k - 1) + v(i, j, k))) * (w0[j] * w(i, j, k - 1) + w1[j] * w(i, j + 1, k - 1));
-    T t3 = -(vwp - vwm) / (z[k] - z[k - 1]);
-    T dz_1 = zg[k] - zg[k - 1], dz_2 = zg[k + 1] - zg[k], dz_3 = z[k] - z[k - 1];
-    T dy_1 = y[j] - y[j - 1], dy_2 = y[j + 1] - y[j], dy_3 = yg[j + 1] - yg[j];
-    T dx_1 = xg[i] - xg[i - 1], dx_2 = xg[i + 1] - xg[i], dx_3 = x[i] - x[i - 1];
-    T nu_x1 = nu + h * (w0[j] * n(i - 1, j, k) + w1[j] * n(i - 1, j + 1, k) + w0[j] * n(i, j, k) +
-                        w1[j] * n(i, j + 1, k));
-    T nu_x2 = nu + h * (w0[j] * n(i, j, k) + w1[j] * n(i, j + 1, k) + w0[j] * n(i + 1, j, k) +
-                        w1[j] * n(i + 1, j + 1, k));
-    T nu_y1 = nu + n(i, j, k);
-    T nu_y2 = nu + n(i, j + 1, k);
-    T nu_z1 = nu + h * (w0[j] * n(i, j, k - 1) + w1[j] * n(i, j + 1, k - 1) + w0[j] * n(i, j, k) +
-                        w1[j] * n(i, j + 1, k));
-    T nu_z2 = nu + h * (w0[j] * n(i, j, k) + w1[j] * n(i, j + 1, k) + w0[j] * n(i, j, k + 1) +
-                        w1[j] * n(i, j + 1, k + 1));
-    T t4 = one / dx_3 * (nu_x2 * one / dx_2 * (v(i + 1, j, k) - v(i, j, k)) -
-                         nu_x1 * one / dx_1 * (v(i, j, k) - v(i - 1, j, k))) +
-           one / dy_3 * (nu_y2 * one / dy_2 * (v(i, j + 1, k) - v(i, j, k)) -
-                         nu_y1 * one / dy_1 * (v(i, j, k) - v(i, j - 1, k))) +
-           one / dz_3 * (nu_z2 * one / dz_2 * (v(i, j, k + 1) - v(i, j, k)) -
-                         nu_z1 * one / dz_1 * (v(i, j, k) - v(i, j, k - 1)));
-    return ((t1 + t2) + t3) + t4;
-}
-
-// code/equations.f90:327-468
-template <typename T>
-__device__ __forceinline__ T rhs_w_at(const T* U, const T* V, const T* W, const T* nt, const Metric<T>& m,
-                                      const Dims& d, T nu, int i, int j, int k)
-{
-    auto u = [&](int a, int b, int c) { return U[at(d, a, b, c)]; };
-    auto v = [&](int a, int b, int c) { return V[at(d, a, b, c)]; };
-    auto w = [&](int a, int b, int c) { return W[at(d, a, b, c)]; };
-    auto n = [&](int a, int b, int c) { return nt[at(d, a, b, c)]; };
-    const T h = T(0.5), q = T(0.25), one = T(1);
-    const T *x = m.x, *y = m.y, *z = m.z, *xg = m.xg, *yg = m.yg, *zg = m.zg, *w0 = m.w0, *w1 = m.w1;
-    T wc = h * (w(i, j, k) + w(i, j, k + 1));
-    T wcm = h * (w(i, j, k - 1) + w(i, j, k));
-    T t1 = -(wc * wc - wcm * wcm) / (zg[k + 1] - zg[k]);
-    T uwp = (h * (u(i, j, k) + u(i, j, k + 1))) * (h * (w(i, j, k) + w(i + 1, j, k)));
-    T uwm = (h * (u(i - 1, j, k) + u(i - 1, j, k + 1))) * (h * (w(i - 1, j, k) + w(i, j, k)));
-    T t2 = -(uwp - uwm) / (x[i] - x[i - 1]);
-    T vwp = (h * (v(i, j, k) + v(i, j, k + 1))) * (w0[j] * w(i, j, k) + w1[j] * w(i, j + 1, k));
-    T vwm = (h * (v(i, j - 1, k) + v(i, j - 1, k + 1))) * (w0[j - 1] * w(i, j - 1, k) + w1[j - 1] * w(i, j, k));
-    T t3 = -(vwp - vwm) / (y[j] - y[j - 1]);
-    T dz_1 = z[k] - z[k - 1], dz_2 = z[k + 1] - z[k], dz_3 = zg[k + 1] - zg[k];
-    T dy_3 = y[j] - y[j - 1];
-    T dx_1 = xg[i] - xg[i - 1], dx_2 = xg[i + 1] - xg[i], dx_3 = x[i] - x[i - 1];
-    T nu_x1 = nu + q * (n(i, j, k) + n(i, j, k + 1) + n(i - 1, j, k) + n(i - 1, j, k + 1));
-    T nu_x2 = nu + q * (n(i, j, k) + n(i, j, k + 1) + n(i + 1, j, k) + n(i + 1, j, k + 1));
-    T nu_y1 = nu + h * (w0[j - 1] * n(i, j - 1, k) + w1[j - 1] * n(i, j, k) + w0[j - 1] * n(i, j - 1, k + 1) +
-                        w1[j - 1] * n(i, j, k + 1));
-    T nu_y2 = nu + h * (w0[j] * n(i, j, k) + w1[j] * n(i, j + 1, k) + w0[j] * n(i, j, k + 1) +
-                        w1[j] * n(i, j + 1, k + 1));
-    T nu_z1 = nu + n(i, j, k);
-    T nu_z2 = nu + n(i, j, k + 1);
-    T dwdy_p = (w(i, j + 1, k) - w(i, j, k)) / (yg[j + 1] - yg[j]);
-    T dwdy_m = (w(i, j, k) - w(i, j - 1, k)) / (yg[j] - yg[j - 1]);
-    T t4 = one / dx_3 * (nu_x2 * one / dx_2 * (w(i + 1, j, k) - w(i, j, k)) -
-                         nu_x1 * one / dx_1 * (w(i, j, k) - w(i - 1, j, k))) +
-           one / dy_3 * (nu_y2 * dwdy_p - nu_y1 * dwdy_m) +
-           one / dz_3 * (nu_z2 * one / dz_2 * (w(i, j, k + 1) - w(i, j, k)) -
-                         nu_z1 * one / dz_1 * (w(i, j, k) - w(i, j, k - 1)));
-    return ((t1 + t2) + t3) + t4;
-}
-
-// One thread per box cell computes whichever of rhs_u/v/w is defined there.
-// mode 0 writes the right-hand sides; mode s = 1,2,3 is Wray RK3 substep s
-// (code/time_integration.f90:42-44,59-61,76-81; b2 = rk_coef(3,2) = 0 so the
-// second stage's F is not kept).
-template <typename T>
-__global__ void __launch_bounds__(256) k_rhs_rk(const T* __restrict__ U, const T* __restrict__ V,
-                                                 const T* __restrict__ W, const T* __restrict__ nt,
-                                                 Metric<T> m, Dims d, T nu, const Scalars* sc, int mode,
-                                                 const T* Uo, const T* Vo, const T* Wo, T* F1u, T* F1v, T* F1w,
-                                                 T* Ou, T* Ov, T* Ow, T c1, T c2)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 2;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
-    const int k = blockIdx.z + 2;
-    if (i > d.nxg - 1 || j > d.nyg - 1) return;
-    const T dPdx = T(sc->dPdx);
-    const T dt = T(sc->dt);
-    const long p = at(d, i, j, k);
-    const bool du = (i <= d.nx - 1) && (k <= d.nzg - 1);
-    const bool dv = (j <= d.ny - 1) && (k <= d.nzg - 1);
-    const bool dw = (k <= d.nz - 1);
-    T ru = T(0), rv = T(0), rw = T(0);
-    if (du) ru = rhs_u_at<T>(U, V, W, nt, m, d, nu, dPdx, i, j, k);
-    if (dv) rv = rhs_v_at<T>(U, V, W, nt, m, d, nu, i, j, k);
-    if (dw) rw = rhs_w_at<T>(U, V, W, nt, m, d, nu, i, j, k);
-    if (mode == 0) {
-        if (du) Ou[p] = ru;
-        if (dv) Ov[p] = rv;
-        if (dw) Ow[p] = rw;
-        return;
-    }
-    if (mode == 1) {
-        // U = Uo + dt*rk_coef(1,1)*Fu1 with Uo = U at substep 1
-        if (du) { F1u[p] = ru; Ou[p] = U[p] + dt * c1 * ru; }
-        if (dv) { F1v[p] = rv; Ov[p] = V[p] + dt * c1 * rv; }
-        if (dw) { F1w[p] = rw; Ow[p] = W[p] + dt * c1 * rw; }
-    } else {
-        // U = Uo + dt*(c1*Fu1 + c2*Fu_s)
-        if (du) Ou[p] = Uo[p] + dt * (c1 * F1u[p] + c2 * ru);
-        if (dv) Ov[p] = Vo[p] + dt * (c1 * F1v[p] + c2 * rv);
-        if (dw) Ow[p] = Wo[p] + dt * (c1 * F1w[p] + c2 * rw);
-    }
-}
-
-template <typename T>
-void launch_rhs_rk(const T* U, const T* V, const T* W, const T* nut, const Metric<T>& m, const Dims& d, T nu,
-                   const Scalars* sc, int mode, const T* Uo, const T* Vo, const T* Wo, T* F1u, T* F1v, T* F1w,
-                   T* Ou, T* Ov, T* Ow, double c1, double c2, hipStream_t s)
-{
-    dim3 blk(64, 4, 1);
-    dim3 grd(cdiv(d.nxg - 2, 64), cdiv(d.nyg - 2, 4), d.nzg - 2);
-    PROF_BEGIN("k_rhs_rk", s);
-    k_rhs_rk<T><<<grd, blk, 0, s>>>(U, V, W, nut, m, d, nu, sc, mode, Uo, Vo, Wo, F1u, F1v, F1w, Ou, Ov, Ow,
-                                    T(c1), T(c2));
-    PROF_END(s);
-}
+// The RHS + RK combine kernel is rhs.hip.
 
 // --------------------------------------------------------------- SGS: DSM
 // Pass A (filters, L_ij, M_ij, plane partial sums, |S|) is dsm_fused.hip.
@@ -732,9 +551,6 @@ void launch_plane_stats(const T* U, const T* V, const T* W, const Metric<double>
 // ------------------------------------------------------- instantiations
 #define CHAN_INST(T)                                                                                            \
     template void launch_bc<T>(T*, const Dims&, int, hipStream_t);                                              \
-    template void launch_rhs_rk<T>(const T*, const T*, const T*, const T*, const Metric<T>&, const Dims&, T,    \
-                                   const Scalars*, int, const T*, const T*, const T*, T*, T*, T*, T*, T*, T*,   \
-                                   double, double, hipStream_t);                                                \
     template void launch_dsm<T>(const T*, const T*, const T*, T*, const Metric<T>&, const Dims&, DsmBufs<T>&,   \
                                 int, int, const double*, T, hipStream_t);                                       \
     template void launch_wall_model<T>(const T*, T*, const double*, const Metric<T>&, const Dims&, T,           \
