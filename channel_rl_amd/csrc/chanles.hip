@@ -279,7 +279,6 @@ void enqueue_rk3(chan_ctx* c, int X)
         launch_rhs_rk<T>(I[0], I[1], I[2], (const T*)c->nut, metric<T>(c), c->d, T(c->nu), c->sc, s + 1, Xs[0],
                          Xs[1], Xs[2], (T*)c->F1[0], (T*)c->F1[1], (T*)c->F1[2], O[0], O[1], O[2], c1, c2,
                          c->stream);
-        enqueue_bc<T>(c, out[s]);
         enqueue_projection<T>(c, out[s]);
     }
     if (c->x_mass_cte == 1) {

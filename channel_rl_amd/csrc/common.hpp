@@ -44,6 +44,46 @@ __host__ __device__ inline int img_centre(int i, int ng)
     return i == 1 ? ng - 2 : (i == ng - 1 ? 2 : (i == ng ? 3 : i));
 }
 
+// Store v at the canonical cell (i,j,k) of a velocity field and at every
+// ghost cell whose boundary-condition source it is, so that a producer
+// kernel leaves the field exactly as code/boundary_conditions.f90:20-48
+// would: periodic images in x and z, and in y the wall mirror -v for U and
+// W (V's wall rows stay 0).  kind: 0 U (x faces), 1 V, 2 W (z faces).
+template <typename T>
+__device__ __forceinline__ void store_img(T* F, const Dims& d, int kind, int i, int j, int k, T v)
+{
+    int xs[2], zs[2], ys[2];
+    T sg[2];
+    int nx_ = 1, nz_ = 1, ny_ = 1;
+    xs[0] = i;
+    if (kind == 0) {  // faces: 1 <- nx-1, nx <- 2
+        if (i == d.nx - 1) xs[nx_++] = 1;
+        if (i == 2) xs[nx_++] = d.nx;
+    } else {          // centres: 1 <- nxg-2, nxg-1 <- 2, nxg <- 3
+        if (i == d.nxg - 2) xs[nx_++] = 1;
+        if (i == 2) xs[nx_++] = d.nxg - 1;
+        if (i == 3) xs[nx_++] = d.nxg;
+    }
+    zs[0] = k;
+    if (kind == 2) {
+        if (k == d.nz - 1) zs[nz_++] = 1;
+        if (k == 2) zs[nz_++] = d.nz;
+    } else {
+        if (k == d.nzg - 2) zs[nz_++] = 1;
+        if (k == 2) zs[nz_++] = d.nzg - 1;
+        if (k == 3) zs[nz_++] = d.nzg;
+    }
+    ys[0] = j;
+    sg[0] = v;
+    if (kind != 1) {
+        if (j == 2) { ys[ny_] = 1; sg[ny_++] = -v; }
+        if (j == d.nyg - 1) { ys[ny_] = d.nyg; sg[ny_++] = -v; }
+    }
+    for (int a = 0; a < nx_; a++)
+        for (int c = 0; c < nz_; c++)
+            for (int b = 0; b < ny_; b++) F[at(d, xs[a], ys[b], zs[c])] = sg[b];
+}
+
 // Device scalars shared by the kernels of one channel (dt and dPdx are read
 // from here, so a captured step needs no host round trip).
 struct Scalars {

@@ -397,7 +397,9 @@ void launch_poisson_inverse(const Dims& d, PoissonBufs& pb, hipStream_t s)
     PROF_END(s);
 }
 
-// code/projection.f90:192-214: u -= grad p on the interior.
+// code/projection.f90:192-214: u -= grad p on the canonical cells, each
+// stored with its boundary-condition images (the reference applies
+// apply_boundary_conditions right after, code/time_integration.f90:49).
 template <typename T>
 __global__ void __launch_bounds__(256) k_project(T* U, T* V, T* W, const double* __restrict__ p, Metric<double> m,
                                                   Dims d)
@@ -405,26 +407,22 @@ __global__ void __launch_bounds__(256) k_project(T* U, T* V, T* W, const double*
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 2;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
     const int k = blockIdx.z + 2;
-    if (i > d.nxg - 1 || j > d.nyg - 1) return;
+    if (i > d.nxg - 2 || j > d.nyg - 1) return;
     const long q = at(d, i, j, k);
     const double p0 = p[q];
-    if (i <= d.nx - 1 && k <= d.nzg - 1)
-        U[q] = T((double)U[q] - (p[at(d, i + 1, j, k)] - p0) / (m.xg[i + 1] - m.xg[i]));
-    if (j <= d.ny - 1 && k <= d.nzg - 1)
-        V[q] = T((double)V[q] - (p[at(d, i, j + 1, k)] - p0) / (m.yg[j + 1] - m.yg[j]));
-    if (k <= d.nz - 1) W[q] = T((double)W[q] - (p[at(d, i, j, k + 1)] - p0) / (m.zg[k + 1] - m.zg[k]));
+    store_img(U, d, 0, i, j, k, T((double)U[q] - (p[at(d, i + 1, j, k)] - p0) / (m.xg[i + 1] - m.xg[i])));
+    if (j <= d.ny - 1)
+        store_img(V, d, 1, i, j, k, T((double)V[q] - (p[at(d, i, j + 1, k)] - p0) / (m.yg[j + 1] - m.yg[j])));
+    store_img(W, d, 2, i, j, k, T((double)W[q] - (p[at(d, i, j, k + 1)] - p0) / (m.zg[k + 1] - m.zg[k])));
 }
 
 template <typename T>
 void launch_project(T* U, T* V, T* W, const double* p, const Metric<double>& md, const Dims& d, hipStream_t s)
 {
-    dim3 blk(64, 4, 1), grd(cdiv(d.nxg - 2, 64), cdiv(d.nyg - 2, 4), d.nzg - 2);
+    dim3 blk(64, 4, 1), grd(cdiv(d.nxg - 3, 64), cdiv(d.nyg - 2, 4), d.nzg - 3);
     PROF_BEGIN("k_project", s);
     k_project<T><<<grd, blk, 0, s>>>(U, V, W, p, md, d);
     PROF_END(s);
-    launch_bc<T>(U, d, KIND_U, s);
-    launch_bc<T>(V, d, KIND_V, s);
-    launch_bc<T>(W, d, KIND_W, s);
 }
 
 template void launch_poisson_forward<double>(const double*, const double*, const double*, const Metric<double>&,

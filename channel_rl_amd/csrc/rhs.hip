@@ -189,15 +189,16 @@ __global__ void __launch_bounds__(TX* TY)
                 if (dv) Ov[p] = rv;
                 if (dw) Ow[p] = rw;
             } else if (mode == 1) {
-                // U = Uo + dt*rk_coef(1,1)*Fu1 with Uo = U at substep 1
-                if (du) { F1u[p] = ru; Ou[p] = F(0, i, j, k) + dt * c1 * ru; }
-                if (dv) { F1v[p] = rv; Ov[p] = F(1, i, j, k) + dt * c1 * rv; }
-                if (dw) { F1w[p] = rw; Ow[p] = F(2, i, j, k) + dt * c1 * rw; }
+                // U = Uo + dt*rk_coef(1,1)*Fu1 with Uo = U at substep 1; the
+                // new field is stored with its boundary-condition images
+                if (du) { F1u[p] = ru; store_img(Ou, d, 0, i, j, k, F(0, i, j, k) + dt * c1 * ru); }
+                if (dv) { F1v[p] = rv; store_img(Ov, d, 1, i, j, k, F(1, i, j, k) + dt * c1 * rv); }
+                if (dw) { F1w[p] = rw; store_img(Ow, d, 2, i, j, k, F(2, i, j, k) + dt * c1 * rw); }
             } else {
                 // U = Uo + dt*(c1*Fu1 + c2*Fu_s); rk_coef(3,2) = 0 so the second stage's F is not kept
-                if (du) Ou[p] = Uo[p] + dt * (c1 * F1u[p] + c2 * ru);
-                if (dv) Ov[p] = Vo[p] + dt * (c1 * F1v[p] + c2 * rv);
-                if (dw) Ow[p] = Wo[p] + dt * (c1 * F1w[p] + c2 * rw);
+                if (du) store_img(Ou, d, 0, i, j, k, Uo[p] + dt * (c1 * F1u[p] + c2 * ru));
+                if (dv) store_img(Ov, d, 1, i, j, k, Vo[p] + dt * (c1 * F1v[p] + c2 * rv));
+                if (dw) store_img(Ow, d, 2, i, j, k, Wo[p] + dt * (c1 * F1w[p] + c2 * rw));
             }
         }
         __syncthreads();

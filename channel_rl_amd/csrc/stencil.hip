@@ -360,7 +360,7 @@ __global__ void k_shift_u(T* U, Dims d, const Scalars* sc)
     if (i > d.nx - 1 || j > d.nyg - 1) return;
     const T s = T(sc->last_dPdx);
     const long p = at(d, i, j, k);
-    U[p] = U[p] + s;
+    store_img(U, d, 0, i, j, k, U[p] + s);
 }
 
 template <typename T>
@@ -373,11 +373,11 @@ void launch_mass_flow_correct(T* U, const Dims& d, double* part, int nblk, Scala
     k_dpx_final<<<1, RED_THREADS, 0, s>>>(part, nblk, sc);
     PROF_END(s);
     if (apply) {
-        dim3 blk(64, 4, 1), grd(cdiv(d.nx - 2, 64), cdiv(d.nyg - 2, 4), d.nzg - 2);
+        // canonical cells only (k up to nzg-2), images written by store_img
+        dim3 blk(64, 4, 1), grd(cdiv(d.nx - 2, 64), cdiv(d.nyg - 2, 4), d.nzg - 3);
         PROF_BEGIN("k_shift_u", s);
         k_shift_u<T><<<grd, blk, 0, s>>>(U, d, sc);
         PROF_END(s);
-        launch_bc<T>(U, d, KIND_U, s);
     }
 }
 
