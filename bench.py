@@ -29,20 +29,18 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
 # kernel: every input read once, every output written once.
 #   k_rhs_rk : reads U,V,W,nu_t (+Uo,F1 in substeps 2,3), writes U* (+F1 in
 #              substep 1): 10, 13, 13 -> 12 on average per launch
-#   k_dsm_lm : reads centre velocities (3), |S|S_ij (6), filtered u,v,w (3)
-#   k_dsm_prep: reads U,V,W, writes 3 + 6 + 1
-#   k_dsm_filtvel: reads U,V,W, writes 3
-#   k_dsm_nut: reads |S|, (U for the wall rows), writes nu_t
+#   k_dsm_fused: reads U,V,W, writes |S| (pass A of the dynamic model; the
+#              filters, L_ij, M_ij and the plane sums stay on chip)
+#   k_dsm_nut: reads |S|, writes nu_t (pass B)
 #   k_fftx_fwd: reads U,V,W (divergence), writes p-hat (complex, half spectrum = 1 double/cell)
 #   k_fftz: reads and writes p-hat (1 complex per 2 cells each way: 2 doubles/cell)
 #   k_tridiag: reads p-hat twice, writes twice (forward + back substitution): 4
 #   k_fftx_inv: reads p-hat (1), writes p (1)
 #   k_project: reads p, U,V,W, writes U,V,W: 7
+# A whole step moves 122 values per cell (976 B FP64, 632 B FP32 bulk).
 ALG_VALUES = {
     "k_rhs_rk": (12, 0),
-    "k_dsm_lm": (12, 0),
-    "k_dsm_prep": (13, 0),
-    "k_dsm_filtvel": (6, 0),
+    "k_dsm_fused": (4, 0),
     "k_dsm_nut": (2, 0),
     "k_fftx_fwd": (3, 1),
     "k_fftz": (0, 2),
@@ -118,10 +116,20 @@ def main():
     vb, vp = ALG_VALUES[dom]
     alg_bytes = cells * (vb * bulk_b + vp * 8)
     achieved = alg_bytes / avg_s / 1e9
+    # HBM bytes per launch of the same kernel from the two rocprofv3 counter
+    # passes over this workload, scaled by the copy kernel (null if absent)
+    from channel_rl_amd import rocprof_summary
+    tag = f"{case.nx - 2}x{case.ny}x{case.nz - 2}_{args.precision}"
+    traffic = rocprof_summary.load_traffic(os.path.join(ROOT, "profiles", f"hbm_traffic_channel_step_{tag}.md")).get(dom)
+    copy_gbs, copy_ms = ch.stream_copy(1 << 30, reps=10)
+    log(f"[bench] stream copy 1 GiB: {copy_gbs:.0f} GB/s (read + write), {copy_ms:.3f} ms")
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None if traffic is None else int(traffic),
                 "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": alg_bytes,
-                "step_ms_profiled": round(tot / args.profile_steps, 3)}
+                "step_ms_profiled": round(tot / args.profile_steps, 3),
+                "copy_kernel_gbs": round(copy_gbs, 1),
+                "step_alg_gbs": round(cells * (976 if prec == cr.FP64 else 632) / (ms_step * 1e-3) / 1e9, 1)}
 
     cpu = None
     if args.cpu_steps > 0:

@@ -30,6 +30,7 @@ EXPORTS = [
     "chan_compute_time_step_RK3", "chan_check_divergence", "chan_set_wall_model",
     "chan_compute_equilibrium_wall_stress", "chan_plane_statistics", "chan_run_steps",
     "chan_synchronize", "chan_profile_steps", "chan_profile_count", "chan_profile_entry",
+    "chan_stream_copy",
 ]
 
 
@@ -84,6 +85,7 @@ def load_library(path=LIB_PATH):
     L.chan_profile_count.argtypes = [vp]
     L.chan_profile_entry.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _D,
                                      ctypes.POINTER(ctypes.c_long)]
+    L.chan_stream_copy.argtypes = [vp, ctypes.c_long, ctypes.c_int, _D]
     _LIB = L
     return L
 
@@ -231,6 +233,12 @@ class Channel:
                       "profile_entry")
             out[buf.value.decode()] = (ms.value, cnt.value)
         return out
+
+    def stream_copy(self, nbytes, reps=20):
+        """Achieved bandwidth of a plain device copy: (GB/s counting read + write, ms per copy)."""
+        ms = ctypes.c_double()
+        self._chk(load_library().chan_stream_copy(self._h, int(nbytes), int(reps), ctypes.byref(ms)), "stream_copy")
+        return 2.0 * nbytes / (ms.value * 1e-3) / 1e9, ms.value
 
     def synchronize(self):
         self._chk(load_library().chan_synchronize(self._h), "synchronize")

@@ -52,6 +52,12 @@ __global__ void k_fill(T* a, long n, T v)
 
 __global__ void k_advance_time(Scalars* sc) { sc->t = sc->t + sc->dt; }
 
+// plain streaming copy (16-byte vectors), the bandwidth reference
+__global__ void __launch_bounds__(256) k_stream_copy(const double2* __restrict__ a, double2* __restrict__ b, long n)
+{
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) b[q] = a[q];
+}
+
 struct Shape { int n1, n2, n3, o; };
 
 }  // namespace
@@ -852,6 +858,40 @@ int chan_profile_steps(chan_ctx* c, int nsteps)
 }
 
 int chan_profile_count(const chan_ctx* c) { return c ? (int)c->prof_out.size() : 0; }
+
+int chan_stream_copy(chan_ctx* c, long nbytes, int reps, double* ms_per_copy)
+{
+    if (!c || nbytes < 16 || reps < 1) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    const long n = nbytes / 16;
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(c, hipMalloc(&a, n * 16));
+    hipError_t e = hipMalloc(&b, n * 16);
+    if (e != hipSuccess) {
+        (void)hipFree(a);
+        return fail(c, CHAN_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipMemsetAsync(a, 0, n * 16, c->stream);
+    const int grid = 256 * 32;
+    k_stream_copy<<<grid, 256, 0, c->stream>>>((const double2*)a, (double2*)b, n);
+    (void)hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps; r++) k_stream_copy<<<grid, 256, 0, c->stream>>>((const double2*)a, (double2*)b, n);
+    (void)hipEventRecord(e1, c->stream);
+    e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(c, CHAN_EHIP, std::string("stream copy: ") + hipGetErrorString(e));
+    if (ms_per_copy) *ms_per_copy = ms / reps;
+    return CHAN_OK;
+}
 
 int chan_profile_entry(const chan_ctx* c, int idx, char* name, int len, double* total_ms, long* launches)
 {

@@ -1,0 +1,116 @@
+"""Summaries of rocprofv3 output for the channel step kernels.
+
+    python -m channel_rl_amd.rocprof_summary stats DIR/run_kernel_stats.csv
+    python -m channel_rl_amd.rocprof_summary traffic FETCH_DIR/run_counter_collection.csv \
+        WRITE_DIR/run_counter_collection.csv
+
+`stats` prints the per-kernel table of `rocprofv3 --kernel-trace --stats`.
+`traffic` turns two `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs (KiB per
+dispatch) into HBM bytes per launch.  Each counter is scaled by what it reads
+for k_stream_copy, whose bytes are known (the same run copies nbytes into a
+second buffer): factor = nbytes / (counter KiB x 1024).  On gfx950 FETCH_SIZE
+reads half of a streaming copy's loads (factor 2) and WRITE_SIZE all of its
+stores (factor 1).
+
+The traffic table's rows `| kernel | launches | fetch KiB | write KiB | bytes |`
+are what `load_traffic` reads back.
+"""
+import csv
+import collections
+import os
+import statistics
+import sys
+
+COPY_KERNEL = "k_stream_copy"
+COPY_BYTES = 1 << 30  # bench.py copies 1 GiB (read) into 1 GiB (write)
+
+
+def short_name(name):
+    """`void chan::k_rhs_rk<double, 64, 4>(...)` -> `k_rhs_rk<double, 64, 4>`."""
+    n = name.replace("(anonymous namespace)::", "").replace("void ", "")
+    depth = 0
+    for i, ch in enumerate(n):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            n = n[:i]
+            break
+    return n.replace("chan::", "").strip()
+
+
+def base_name(name):
+    return short_name(name).split("<")[0]
+
+
+def read_stats(path):
+    rows = []
+    for r in csv.DictReader(open(path)):
+        rows.append((short_name(r["Name"]), int(r["Calls"]), float(r["TotalDurationNs"]),
+                     float(r["AverageNs"]), float(r["Percentage"])))
+    return rows
+
+
+def stats_table(path):
+    out = ["| kernel | calls | total ms | average us | % |", "|---|---:|---:|---:|---:|"]
+    for n, c, tot, avg, pct in read_stats(path):
+        out.append(f"| {n} | {c} | {tot / 1e6:.3f} | {avg / 1e3:.2f} | {pct:.2f} |")
+    return "\n".join(out)
+
+
+def read_counter(path):
+    d = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        d[short_name(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+    return d
+
+
+def traffic(fetch_path, write_path):
+    """{kernel: (launches, fetch KiB, write KiB, scaled bytes per launch)} and
+    the two scale factors."""
+    f, w = read_counter(fetch_path), read_counter(write_path)
+    copy = next(k for k in f if base_name(k) == COPY_KERNEL)
+    # the first copy launch is the warm-up; every launch moves the same bytes
+    sf = COPY_BYTES / (statistics.median(f[copy]) * 1024.0)
+    sw = COPY_BYTES / (statistics.median(w[copy]) * 1024.0)
+    res = {}
+    for k in f:
+        fk = statistics.mean(f[k])
+        wk = statistics.mean(w.get(k, [0.0]))
+        res[k] = (len(f[k]), fk, wk, fk * 1024.0 * sf + wk * 1024.0 * sw)
+    return res, sf, sw
+
+
+def traffic_table(fetch_path, write_path):
+    res, sf, sw = traffic(fetch_path, write_path)
+    out = [f"copy-kernel scale factors: FETCH_SIZE x {sf:.3f}, WRITE_SIZE x {sw:.3f}", "",
+           "| kernel | launches | fetch KiB | write KiB | bytes |", "|---|---:|---:|---:|---:|"]
+    for k, (n, fk, wk, b) in sorted(res.items(), key=lambda kv: -kv[1][3]):
+        out.append(f"| {k} | {n} | {fk:.1f} | {wk:.1f} | {b:.0f} |")
+    return "\n".join(out)
+
+
+def load_traffic(path):
+    """Scaled HBM bytes per launch by kernel base name from a traffic table."""
+    res = {}
+    if not os.path.exists(path):
+        return res
+    for line in open(path):
+        p = [s.strip() for s in line.strip().strip("|").split("|")]
+        if len(p) == 5 and p[0].startswith("k_"):
+            try:
+                res[base_name(p[0])] = float(p[4])
+            except ValueError:
+                pass
+    return res
+
+
+if __name__ == "__main__":
+    if len(sys.argv) >= 3 and sys.argv[1] == "stats":
+        print(stats_table(sys.argv[2]))
+    elif len(sys.argv) >= 4 and sys.argv[1] == "traffic":
+        print(traffic_table(sys.argv[2], sys.argv[3]))
+    else:
+        print(__doc__)
+        sys.exit(2)

@@ -121,6 +121,10 @@ int chan_synchronize(chan_ctx* ctx);
 int chan_profile_steps(chan_ctx* ctx, int nsteps);
 int chan_profile_count(const chan_ctx* ctx);
 int chan_profile_entry(const chan_ctx* ctx, int idx, char* name, int len, double* total_ms, long* launches);
+/* Measurement: a plain streaming copy of nbytes on the context's device,
+ * timed over reps launches with HIP events (achievable HBM bandwidth, and
+ * the reference point that scales the hardware traffic counters). */
+int chan_stream_copy(chan_ctx* ctx, long nbytes, int reps, double* ms_per_copy);
 
 #ifdef __cplusplus
 }

@@ -62,3 +62,31 @@ def test_noise_is_seeded_and_bounded():
 
 def test_bulk_velocity_correlation():
     assert 15.0 < cases.bulk_velocity_plus(180.0) < 16.5
+
+
+def test_rocprof_summary_names_and_traffic_table(tmp_path):
+    from channel_rl_amd import rocprof_summary as rs
+    assert rs.short_name("void chan::k_rhs_rk<double, 64, 4>(double const*, chan::Dims)") == "k_rhs_rk<double, 64, 4>"
+    assert rs.short_name("(anonymous namespace)::k_stream_copy(HIP_vector_type<double, 2u> const*, long)") == "k_stream_copy"
+    assert rs.base_name("chan::k_fftz<-1>(chan::Dims, chan::PoissonBufs)") == "k_fftz"
+    hdr = "Correlation_Id,Kernel_Name,Counter_Name,Counter_Value\n"
+    f = tmp_path / "f.csv"
+    w = tmp_path / "w.csv"
+    f.write_text(hdr + '1,"(anonymous namespace)::k_stream_copy(x)",FETCH_SIZE,524288\n'
+                 '2,"void chan::k_dsm_fused<double, 30, 6>(x)",FETCH_SIZE,100\n')
+    w.write_text(hdr + '1,"(anonymous namespace)::k_stream_copy(x)",WRITE_SIZE,1048576\n'
+                 '2,"void chan::k_dsm_fused<double, 30, 6>(x)",WRITE_SIZE,10\n')
+    res, sf, sw = rs.traffic(str(f), str(w))
+    assert sf == 2.0 and sw == 1.0
+    assert res["k_dsm_fused<double, 30, 6>"][3] == 100 * 1024 * 2 + 10 * 1024
+    t = tmp_path / "t.md"
+    t.write_text(rs.traffic_table(str(f), str(w)))
+    assert rs.load_traffic(str(t))["k_dsm_fused"] == 100 * 1024 * 2 + 10 * 1024
+
+
+def test_committed_traffic_table_parses():
+    import os
+    from channel_rl_amd import rocprof_summary as rs
+    p = os.path.join(os.path.dirname(__file__), "..", "profiles", "hbm_traffic_channel_step_192x97x192_fp64.md")
+    t = rs.load_traffic(p)
+    assert t["k_dsm_fused"] > 0 and t["k_rhs_rk"] > 0
