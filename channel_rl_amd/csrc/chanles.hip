@@ -53,9 +53,18 @@ __global__ void k_fill(T* a, long n, T v)
 __global__ void k_advance_time(Scalars* sc) { sc->t = sc->t + sc->dt; }
 
 // plain streaming copy (16-byte vectors), the bandwidth reference
+// Each block copies 4 x 256 consecutive 16-byte elements: four coalesced
+// loads in flight per lane, then the four stores.
 __global__ void __launch_bounds__(256) k_stream_copy(const double2* __restrict__ a, double2* __restrict__ b, long n)
 {
-    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) b[q] = a[q];
+    const long base = (long)blockIdx.x * 1024 + threadIdx.x;
+    double2 v[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+        if (base + r * 256 < n) v[r] = a[base + r * 256];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+        if (base + r * 256 < n) b[base + r * 256] = v[r];
 }
 
 struct Shape { int n1, n2, n3, o; };
@@ -865,6 +874,8 @@ int chan_stream_copy(chan_ctx* c, long nbytes, int reps, double* ms_per_copy)
     int rc = set_device(c);
     if (rc) return rc;
     const long n = nbytes / 16;
+    const long grid = (n + 1023) / 1024;
+    if (grid > 0x7fffffffL) return fail(c, CHAN_EINVAL, "stream copy: size too large");
     void *a = nullptr, *b = nullptr;
     HIPCHK(c, hipMalloc(&a, n * 16));
     hipError_t e = hipMalloc(&b, n * 16);
@@ -876,10 +887,9 @@ int chan_stream_copy(chan_ctx* c, long nbytes, int reps, double* ms_per_copy)
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     (void)hipMemsetAsync(a, 0, n * 16, c->stream);
-    const int grid = 256 * 32;
-    k_stream_copy<<<grid, 256, 0, c->stream>>>((const double2*)a, (double2*)b, n);
+    k_stream_copy<<<(unsigned)grid, 256, 0, c->stream>>>((const double2*)a, (double2*)b, n);
     (void)hipEventRecord(e0, c->stream);
-    for (int r = 0; r < reps; r++) k_stream_copy<<<grid, 256, 0, c->stream>>>((const double2*)a, (double2*)b, n);
+    for (int r = 0; r < reps; r++) k_stream_copy<<<(unsigned)grid, 256, 0, c->stream>>>((const double2*)a, (double2*)b, n);
     (void)hipEventRecord(e1, c->stream);
     e = hipEventSynchronize(e1);
     float ms = 0.f;

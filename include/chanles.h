@@ -58,7 +58,9 @@ typedef struct {
 } chan_config;
 
 /* Allocates the device state and sets up the constants of
- * code/initialization.f90:187-427 (grids, kxx/kzz, Dyy, weights, RK3). */
+ * code/initialization.f90:187-427 (grids, kxx/kzz, Dyy, weights, RK3).
+ * On failure *out still holds a context carrying the message for
+ * chan_last_error; release it with chan_destroy. */
 int  chan_create(const chan_config* cfg, chan_ctx** out);
 void chan_destroy(chan_ctx* ctx);
 const char* chan_last_error(const chan_ctx* ctx);
