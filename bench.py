@@ -87,6 +87,8 @@ def main():
     ch.set_scalars(dt=0.0, t=0.0, dPdx=case.dPdx, Qflow_x_0=q0)
     ch.set_wall_model(cr.WALL_EQUILIBRIUM, h_wm=case.h_wm)
     cells = case.cells
+    tune = ch.autotune(reps=5)
+    log(f"[bench] autotuned RHS tile shape {ch.tile_shape()}: " + " ".join(f"{1e3 * t:.1f}" for t in tune) + " us")
     log(f"[bench] {case.name} faces {case.nx} {case.ny} {case.nz}, {cells} cells, {args.precision}")
 
     ch.run_steps(args.warmup)

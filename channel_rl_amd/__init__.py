@@ -30,8 +30,10 @@ EXPORTS = [
     "chan_compute_time_step_RK3", "chan_check_divergence", "chan_set_wall_model",
     "chan_compute_equilibrium_wall_stress", "chan_plane_statistics", "chan_run_steps",
     "chan_synchronize", "chan_profile_steps", "chan_profile_count", "chan_profile_entry",
-    "chan_stream_copy",
+    "chan_stream_copy", "chan_tile_shape_count", "chan_set_tile_shape", "chan_get_tile_shape",
+    "chan_autotune",
 ]
+KERNEL_RHS = 0
 
 
 class ChanError(RuntimeError):
@@ -86,6 +88,10 @@ def load_library(path=LIB_PATH):
     L.chan_profile_entry.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _D,
                                      ctypes.POINTER(ctypes.c_long)]
     L.chan_stream_copy.argtypes = [vp, ctypes.c_long, ctypes.c_int, _D]
+    L.chan_tile_shape_count.argtypes = [ctypes.c_int]
+    L.chan_set_tile_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.chan_get_tile_shape.argtypes = [vp, ctypes.c_int]
+    L.chan_autotune.argtypes = [vp, ctypes.c_int, _D, ctypes.c_int]
     _LIB = L
     return L
 
@@ -242,3 +248,18 @@ class Channel:
 
     def synchronize(self):
         self._chk(load_library().chan_synchronize(self._h), "synchronize")
+
+    def tile_shape(self, kernel=KERNEL_RHS):
+        return load_library().chan_get_tile_shape(self._h, kernel)
+
+    def set_tile_shape(self, shape, kernel=KERNEL_RHS):
+        self._chk(load_library().chan_set_tile_shape(self._h, kernel, int(shape)), "set_tile_shape")
+
+    def autotune(self, reps=10):
+        """Time every RHS tile shape on this channel and keep the fastest;
+        returns the average ms per launch of each shape."""
+        L = load_library()
+        n = L.chan_tile_shape_count(KERNEL_RHS)
+        t = np.zeros(n)
+        self._chk(L.chan_autotune(self._h, int(reps), _p(t), n), "autotune")
+        return t

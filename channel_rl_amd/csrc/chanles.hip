@@ -106,6 +106,7 @@ struct chan_ctx {
     PoissonBufs pb{};
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
     bool use_graph = true;
+    int rhs_shape = 0;  // index into the RHS kernel's tile shapes
     Prof prof;
     std::vector<std::pair<std::string, std::pair<double, long>>> prof_out;
 };
@@ -237,6 +238,18 @@ const Metric<double>& metric<double>(const chan_ctx* c) { return c->md; }
 template <>
 const Metric<float>& metric<float>(const chan_ctx* c) { return c->mf; }
 
+template <typename T>
+RhsTabs<T> rhs_tabs(const chan_ctx* c)
+{
+    const int pi = sizeof(T) == sizeof(double) ? 0 : 1;
+    const Metric<T>& m = metric<T>(c);
+    RhsTabs<T> t;
+    t.rdx = (const T*)c->rcp[pi][0]; t.rdy = (const T*)c->rcp[pi][1]; t.rdz = (const T*)c->rcp[pi][2];
+    t.rdxg = (const T*)c->rcp[pi][3]; t.rdyg = (const T*)c->rcp[pi][4]; t.rdzg = (const T*)c->rcp[pi][5];
+    t.w0 = m.w0; t.w1 = m.w1;
+    return t;
+}
+
 // ---------------------------------------------------------- stage helpers
 
 template <typename T>
@@ -291,9 +304,9 @@ void enqueue_rk3(chan_ctx* c, int X)
         if (s == 0) c1 = c->rk_coef[0][0];
         else if (s == 1) { c1 = c->rk_coef[1][0]; c2 = c->rk_coef[1][1]; }
         else { c1 = c->rk_coef[2][0]; c2 = c->rk_coef[2][2]; }
-        launch_rhs_rk<T>(I[0], I[1], I[2], (const T*)c->nut, metric<T>(c), c->d, T(c->nu), c->sc, s + 1, Xs[0],
+        launch_rhs_rk<T>(I[0], I[1], I[2], (const T*)c->nut, rhs_tabs<T>(c), c->d, T(c->nu), c->sc, s + 1, Xs[0],
                          Xs[1], Xs[2], (T*)c->F1[0], (T*)c->F1[1], (T*)c->F1[2], O[0], O[1], O[2], c1, c2,
-                         c->stream);
+                         c->rhs_shape, c->stream);
         enqueue_projection<T>(c, out[s]);
     }
     if (c->x_mass_cte == 1) {
@@ -301,6 +314,13 @@ void enqueue_rk3(chan_ctx* c, int X)
         launch_mass_flow_correct<T>(O[0], c->d, c->part, c->nblk_red, c->sc, 1, c->stream);
     }
     k_advance_time<<<1, 1, 0, c->stream>>>(c->sc);
+}
+
+// the captured steps bake in launch shapes and options: drop them on change
+void drop_graphs(chan_ctx* c)
+{
+    for (int q = 0; q < 3; q++)
+        if (c->graph[q]) { (void)hipGraphExecDestroy(c->graph[q]); c->graph[q] = nullptr; }
 }
 
 int check_launch(chan_ctx* c)
@@ -495,7 +515,7 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
     if ((rc = dalloc(c, &c->nut, fb))) return rc;
     if (c->LES_model == 2 && (rc = dalloc(c, &c->dsm[9], fb))) return rc;  // |S|
     {
-        // 1-based reciprocal spacings for the dynamic model
+        // 1-based reciprocal spacings for the RHS and the dynamic model
         const std::vector<double>* f[6] = {&c->x, &c->y, &c->z, &c->xg, &c->yg, &c->zg};
         const int nf[6] = {nx, ny, nz, nxg, nyg, nzg};
         for (int q = 0; q < 6; q++) {
@@ -665,14 +685,14 @@ int chan_compute_rhs(chan_ctx* c)
     void** s = c->state[c->cur];
     if (c->prec == CHAN_FP64) {
         double** S = (double**)s;
-        launch_rhs_rk<double>(S[0], S[1], S[2], (double*)c->nut, c->md, c->d, c->nu, c->sc, 0, nullptr, nullptr,
+        launch_rhs_rk<double>(S[0], S[1], S[2], (double*)c->nut, rhs_tabs<double>(c), c->d, c->nu, c->sc, 0, nullptr, nullptr,
                               nullptr, nullptr, nullptr, nullptr, (double*)c->rhs[0], (double*)c->rhs[1],
-                              (double*)c->rhs[2], 0.0, 0.0, c->stream);
+                              (double*)c->rhs[2], 0.0, 0.0, c->rhs_shape, c->stream);
     } else {
         float** S = (float**)s;
-        launch_rhs_rk<float>(S[0], S[1], S[2], (float*)c->nut, c->mf, c->d, (float)c->nu, c->sc, 0, nullptr, nullptr,
+        launch_rhs_rk<float>(S[0], S[1], S[2], (float*)c->nut, rhs_tabs<float>(c), c->d, (float)c->nu, c->sc, 0, nullptr, nullptr,
                              nullptr, nullptr, nullptr, nullptr, (float*)c->rhs[0], (float*)c->rhs[1],
-                             (float*)c->rhs[2], 0.0, 0.0, c->stream);
+                             (float*)c->rhs[2], 0.0, 0.0, c->rhs_shape, c->stream);
     }
     return check_launch(c);
 }
@@ -794,8 +814,7 @@ int chan_set_wall_model(chan_ctx* c, int mode, double h_wm, double kappa, double
     if (mode != CHAN_WALL_GIVEN && mode != CHAN_WALL_EQUILIBRIUM) return fail(c, CHAN_EINVAL, "invalid wall mode");
     if (!(kappa > 0)) return fail(c, CHAN_EINVAL, "kappa must be positive");
     c->wall_mode = mode; c->h_wm = h_wm; c->kappa = kappa; c->Bc = B;
-    for (int q = 0; q < 3; q++)
-        if (c->graph[q]) { (void)hipGraphExecDestroy(c->graph[q]); c->graph[q] = nullptr; }
+    drop_graphs(c);
     return CHAN_OK;
 }
 
@@ -867,6 +886,84 @@ int chan_profile_steps(chan_ctx* c, int nsteps)
 }
 
 int chan_profile_count(const chan_ctx* c) { return c ? (int)c->prof_out.size() : 0; }
+
+int chan_tile_shape_count(int kernel)
+{
+    return kernel == CHAN_KERNEL_RHS ? rhs_shape_count() : 0;
+}
+
+int chan_set_tile_shape(chan_ctx* c, int kernel, int shape)
+{
+    if (!c) return CHAN_EINVAL;
+    if (kernel != CHAN_KERNEL_RHS) return fail(c, CHAN_EINVAL, "unknown kernel for tile shapes");
+    if (shape < 0 || shape >= rhs_shape_count()) return fail(c, CHAN_EINVAL, "tile shape out of range");
+    c->rhs_shape = shape;
+    drop_graphs(c);
+    return CHAN_OK;
+}
+
+int chan_get_tile_shape(const chan_ctx* c, int kernel)
+{
+    if (!c || kernel != CHAN_KERNEL_RHS) return CHAN_EINVAL;
+    return c->rhs_shape;
+}
+
+extern "C++" {
+namespace {
+template <typename T>
+void tune_rhs(chan_ctx* c)
+{
+    T** I = (T**)c->state[c->cur];
+    T** R = (T**)c->rhs;
+    launch_rhs_rk<T>(I[0], I[1], I[2], (const T*)c->nut, rhs_tabs<T>(c), c->d, T(c->nu), c->sc, 2, I[0], I[1], I[2],
+                     (T*)c->F1[0], (T*)c->F1[1], (T*)c->F1[2], R[0], R[1], R[2], c->rk_coef[1][0], c->rk_coef[1][1],
+                     c->rhs_shape, c->stream);
+}
+int tune_rhs_launch(chan_ctx* c)
+{
+    if (c->prec == CHAN_FP64) tune_rhs<double>(c);
+    else tune_rhs<float>(c);
+    return check_launch(c);
+}
+}  // namespace
+}
+
+int chan_autotune(chan_ctx* c, int reps, double* times_ms, int max_out)
+{
+    if (!c || reps < 1) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    const int n = rhs_shape_count();
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    int best = c->rhs_shape;
+    double best_ms = 1e300;
+    const int keep = c->rhs_shape;
+    for (int q = 0; q < n && rc == CHAN_OK; q++) {
+        c->rhs_shape = q;
+        // one untimed launch, then reps timed ones, of the substep-2 form of
+        // the kernel (reads the state, Uo = the state, and F1; its output goes
+        // to the rhs buffers, so the state is unchanged)
+        float ms = 0.f;
+        for (int r = 0; r <= reps && rc == CHAN_OK; r++) {
+            if (r == 1) (void)hipEventRecord(e0, c->stream);
+            rc = tune_rhs_launch(c);
+        }
+        (void)hipEventRecord(e1, c->stream);
+        hipError_t e = hipEventSynchronize(e1);
+        if (!rc && e != hipSuccess) rc = fail(c, CHAN_EHIP, std::string("autotune: ") + hipGetErrorString(e));
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        const double avg = (double)ms / reps;
+        if (times_ms && q < max_out) times_ms[q] = avg;
+        if (avg < best_ms) { best_ms = avg; best = q; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->rhs_shape = rc == CHAN_OK ? best : keep;
+    drop_graphs(c);
+    return rc;
+}
 
 int chan_stream_copy(chan_ctx* c, long nbytes, int reps, double* ms_per_copy)
 {

@@ -21,11 +21,21 @@ struct DsmBufs {
 template <typename T>
 void launch_bc(T* F, const Dims& d, int kind, hipStream_t s);
 
+// 1-based metric tables of the RHS kernel: reciprocal spacings
+// r[i] = 1/(a(i)-a(i-1)) (0 where undefined) and the y interpolation weights
+template <typename T>
+struct RhsTabs {
+    const T *rdx, *rdy, *rdz, *rdxg, *rdyg, *rdzg;
+    const T *w0, *w1;
+};
+
 // mode 0: rhs only into R*; mode 1..3: RK substep (Wray, code/time_integration.f90:34-86)
 template <typename T>
-void launch_rhs_rk(const T* U, const T* V, const T* W, const T* nut, const Metric<T>& m, const Dims& d,
+void launch_rhs_rk(const T* U, const T* V, const T* W, const T* nut, const RhsTabs<T>& m, const Dims& d,
                    T nu, const Scalars* sc, int mode, const T* Uo, const T* Vo, const T* Wo, T* F1u,
-                   T* F1v, T* F1w, T* Ou, T* Ov, T* Ow, double c1, double c2, hipStream_t s);
+                   T* F1v, T* F1w, T* Ou, T* Ov, T* Ow, double c1, double c2, int shape, hipStream_t s);
+// number of RHS tile shapes (shape index 0 .. count-1)
+int rhs_shape_count();
 
 template <typename T>
 void launch_dsm(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, const Dims& d,

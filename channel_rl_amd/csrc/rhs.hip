@@ -11,142 +11,160 @@
 
 namespace chan {
 
+// Metric factors of one cell: reciprocal spacings from the 1-based tables
+// r[i] = 1/(a(i)-a(i-1)) of the faces and centres.  The reference divides by
+// the spacings (code/equations.f90:41-160); multiplying by the correctly
+// rounded reciprocal instead differs by at most an ulp per factor and turns
+// ~36 FP64 divisions per cell into multiplications.  x and y factors depend
+// on the thread's (i, j) only and are kept in registers over the z march;
+// z factors are uniform per plane.
+template <typename T>
+struct RhsXY {
+    T rx_f, rx_fp;         // 1/(x(i)-x(i-1)), 1/(x(i+1)-x(i))
+    T rx_c, rx_cp;         // 1/(xg(i)-xg(i-1)), 1/(xg(i+1)-xg(i))
+    T ry_f, ry_fp;         // 1/(y(j)-y(j-1)), 1/(y(j+1)-y(j))
+    T ry_c, ry_cp;         // 1/(yg(j)-yg(j-1)), 1/(yg(j+1)-yg(j))
+    T w0j, w1j, w0m, w1m;  // weight_y_0/1 at j and j-1
+};
+template <typename T>
+struct RhsZ {
+    T rz_f, rz_fp;         // 1/(z(k)-z(k-1)), 1/(z(k+1)-z(k))
+    T rz_c, rz_cp;         // 1/(zg(k)-zg(k-1)), 1/(zg(k+1)-zg(k))
+};
+
 // A(f, i, j, k) returns field f (0 U, 1 V, 2 W, 3 nu_t) at Fortran (i,j,k).
+// code/equations.f90:24-167: convection in divergence form, variable-viscosity
+// diffusion with dx_1 = x(i)-x(i-1), dx_2 = x(i+1)-x(i), dx_3 = xg(i+1)-xg(i),
+// dy_3 = y(j)-y(j-1), dz_1 = zg(k)-zg(k-1), dz_2 = zg(k+1)-zg(k), dz_3 = z(k)-z(k-1)
 template <typename T, typename A>
-__device__ __forceinline__ T rhs_u_gen(const A& F, const Metric<T>& m, T nu, T dPdx, int i, int j, int k)
+__device__ __forceinline__ T rhs_u_gen(const A& F, const RhsXY<T>& g, const RhsZ<T>& gz, T nu, T dPdx, int i, int j,
+                                       int k)
 {
     auto u = [&](int a, int b, int c) { return F(0, a, b, c); };
     auto v = [&](int a, int b, int c) { return F(1, a, b, c); };
     auto w = [&](int a, int b, int c) { return F(2, a, b, c); };
     auto n = [&](int a, int b, int c) { return F(3, a, b, c); };
-    const T h = T(0.5), q = T(0.25), one = T(1);
-    const T *x = m.x, *y = m.y, *z = m.z, *xg = m.xg, *yg = m.yg, *zg = m.zg, *w0 = m.w0, *w1 = m.w1;
+    const T h = T(0.5), q = T(0.25);
     T uc = h * (u(i, j, k) + u(i + 1, j, k));
     T ucm = h * (u(i - 1, j, k) + u(i, j, k));
-    T t1 = -(uc * uc - ucm * ucm) / (xg[i + 1] - xg[i]);
-    T uvp = (w0[j] * u(i, j, k) + w1[j] * u(i, j + 1, k)) * (h * (v(i, j, k) + v(i + 1, j, k)));
-    T uvm = (w0[j - 1] * u(i, j - 1, k) + w1[j - 1] * u(i, j, k)) * (h * (v(i, j - 1, k) + v(i + 1, j - 1, k)));
-    T t2 = -(uvp - uvm) / (y[j] - y[j - 1]);
+    T t1 = -(uc * uc - ucm * ucm) * g.rx_cp;
+    T uvp = (g.w0j * u(i, j, k) + g.w1j * u(i, j + 1, k)) * (h * (v(i, j, k) + v(i + 1, j, k)));
+    T uvm = (g.w0m * u(i, j - 1, k) + g.w1m * u(i, j, k)) * (h * (v(i, j - 1, k) + v(i + 1, j - 1, k)));
+    T t2 = -(uvp - uvm) * g.ry_f;
     T uwp = (h * (u(i, j, k) + u(i, j, k + 1))) * (h * (w(i, j, k) + w(i + 1, j, k)));
     T uwm = (h * (u(i, j, k - 1) + u(i, j, k))) * (h * (w(i, j, k - 1) + w(i + 1, j, k - 1)));
-    T t3 = -(uwp - uwm) / (z[k] - z[k - 1]);
-    T dz_1 = zg[k] - zg[k - 1], dz_2 = zg[k + 1] - zg[k], dz_3 = z[k] - z[k - 1];
-    T dy_3 = y[j] - y[j - 1];
-    T dx_1 = x[i] - x[i - 1], dx_2 = x[i + 1] - x[i], dx_3 = xg[i + 1] - xg[i];
+    T t3 = -(uwp - uwm) * gz.rz_f;
     T nu_x1 = nu + n(i, j, k);
     T nu_x2 = nu + n(i + 1, j, k);
-    T nu_y1 = nu + h * (w0[j - 1] * n(i, j - 1, k) + w1[j - 1] * n(i, j, k) + w0[j - 1] * n(i + 1, j - 1, k) +
-                        w1[j - 1] * n(i + 1, j, k));
-    T nu_y2 = nu + h * (w0[j] * n(i, j, k) + w1[j] * n(i, j + 1, k) + w0[j] * n(i + 1, j, k) +
-                        w1[j] * n(i + 1, j + 1, k));
+    T nu_y1 = nu + h * (g.w0m * n(i, j - 1, k) + g.w1m * n(i, j, k) + g.w0m * n(i + 1, j - 1, k) +
+                        g.w1m * n(i + 1, j, k));
+    T nu_y2 = nu + h * (g.w0j * n(i, j, k) + g.w1j * n(i, j + 1, k) + g.w0j * n(i + 1, j, k) +
+                        g.w1j * n(i + 1, j + 1, k));
     T nu_z1 = nu + q * (n(i, j, k) + n(i, j, k - 1) + n(i + 1, j, k) + n(i + 1, j, k - 1));
     T nu_z2 = nu + q * (n(i, j, k) + n(i, j, k + 1) + n(i + 1, j, k) + n(i + 1, j, k + 1));
-    T dudy_p = (u(i, j + 1, k) - u(i, j, k)) / (yg[j + 1] - yg[j]);
-    T dudy_m = (u(i, j, k) - u(i, j - 1, k)) / (yg[j] - yg[j - 1]);
-    T t4 = one / dx_3 * (nu_x2 * one / dx_2 * (u(i + 1, j, k) - u(i, j, k)) -
-                         nu_x1 * one / dx_1 * (u(i, j, k) - u(i - 1, j, k))) +
-           one / dy_3 * (nu_y2 * dudy_p - nu_y1 * dudy_m) +
-           one / dz_3 * (nu_z2 * one / dz_2 * (u(i, j, k + 1) - u(i, j, k)) -
-                         nu_z1 * one / dz_1 * (u(i, j, k) - u(i, j, k - 1)));
+    T dudy_p = (u(i, j + 1, k) - u(i, j, k)) * g.ry_cp;
+    T dudy_m = (u(i, j, k) - u(i, j - 1, k)) * g.ry_c;
+    T t4 = g.rx_cp * (nu_x2 * g.rx_fp * (u(i + 1, j, k) - u(i, j, k)) -
+                      nu_x1 * g.rx_f * (u(i, j, k) - u(i - 1, j, k))) +
+           g.ry_f * (nu_y2 * dudy_p - nu_y1 * dudy_m) +
+           gz.rz_f * (nu_z2 * gz.rz_cp * (u(i, j, k + 1) - u(i, j, k)) -
+                      nu_z1 * gz.rz_c * (u(i, j, k) - u(i, j, k - 1)));
     return (((t1 + t2) + t3) + t4) + dPdx;
 }
 
+// code/equations.f90:174-320, with dx_1 = xg(i)-xg(i-1), dx_2 = xg(i+1)-xg(i),
+// dx_3 = x(i)-x(i-1), dy_1 = y(j)-y(j-1), dy_2 = y(j+1)-y(j), dy_3 = yg(j+1)-yg(j),
+// dz as for u
 template <typename T, typename A>
-__device__ __forceinline__ T rhs_v_gen(const A& F, const Metric<T>& m, T nu, int i, int j, int k)
+__device__ __forceinline__ T rhs_v_gen(const A& F, const RhsXY<T>& g, const RhsZ<T>& gz, T nu, int i, int j, int k)
 {
     auto u = [&](int a, int b, int c) { return F(0, a, b, c); };
     auto v = [&](int a, int b, int c) { return F(1, a, b, c); };
     auto w = [&](int a, int b, int c) { return F(2, a, b, c); };
     auto n = [&](int a, int b, int c) { return F(3, a, b, c); };
-    const T h = T(0.5), one = T(1);
-    const T *x = m.x, *y = m.y, *z = m.z, *xg = m.xg, *yg = m.yg, *zg = m.zg, *w0 = m.w0, *w1 = m.w1;
+    const T h = T(0.5);
     T vc = h * (v(i, j, k) + v(i, j + 1, k));
     T vcm = h * (v(i, j - 1, k) + v(i, j, k));
-    T t1 = -(vc * vc - vcm * vcm) / (yg[j + 1] - yg[j]);
-    T uvp = (w0[j] * u(i, j, k) + w1[j] * u(i, j + 1, k)) * (h * (v(i, j, k) + v(i + 1, j, k)));
-    T uvm = (w0[j] * u(i - 1, j, k) + w1[j] * u(i - 1, j + 1, k)) * (h * (v(i - 1, j, k) + v(i, j, k)));
-    T t2 = -(uvp - uvm) / (x[i] - x[i - 1]);
-    T vwp = (h * (v(i, j, k) + v(i, j, k + 1))) * (w0[j] * w(i, j, k) + w1[j] * w(i, j + 1, k));
-    T vwm = (h * (v(i, j, k - 1) + v(i, j, k))) * (w0[j] * w(i, j, k - 1) + w1[j] * w(i, j + 1, k - 1));
-    T t3 = -(vwp - vwm) / (z[k] - z[k - 1]);
-    T dz_1 = zg[k] - zg[k - 1], dz_2 = zg[k + 1] - zg[k], dz_3 = z[k] - z[k - 1];
-    T dy_1 = y[j] - y[j - 1], dy_2 = y[j + 1] - y[j], dy_3 = yg[j + 1] - yg[j];
-    T dx_1 = xg[i] - xg[i - 1], dx_2 = xg[i + 1] - xg[i], dx_3 = x[i] - x[i - 1];
-    T nu_x1 = nu + h * (w0[j] * n(i - 1, j, k) + w1[j] * n(i - 1, j + 1, k) + w0[j] * n(i, j, k) +
-                        w1[j] * n(i, j + 1, k));
-    T nu_x2 = nu + h * (w0[j] * n(i, j, k) + w1[j] * n(i, j + 1, k) + w0[j] * n(i + 1, j, k) +
-                        w1[j] * n(i + 1, j + 1, k));
+    T t1 = -(vc * vc - vcm * vcm) * g.ry_cp;
+    T uvp = (g.w0j * u(i, j, k) + g.w1j * u(i, j + 1, k)) * (h * (v(i, j, k) + v(i + 1, j, k)));
+    T uvm = (g.w0j * u(i - 1, j, k) + g.w1j * u(i - 1, j + 1, k)) * (h * (v(i - 1, j, k) + v(i, j, k)));
+    T t2 = -(uvp - uvm) * g.rx_f;
+    T vwp = (h * (v(i, j, k) + v(i, j, k + 1))) * (g.w0j * w(i, j, k) + g.w1j * w(i, j + 1, k));
+    T vwm = (h * (v(i, j, k - 1) + v(i, j, k))) * (g.w0j * w(i, j, k - 1) + g.w1j * w(i, j + 1, k - 1));
+    T t3 = -(vwp - vwm) * gz.rz_f;
+    T nu_x1 = nu + h * (g.w0j * n(i - 1, j, k) + g.w1j * n(i - 1, j + 1, k) + g.w0j * n(i, j, k) +
+                        g.w1j * n(i, j + 1, k));
+    T nu_x2 = nu + h * (g.w0j * n(i, j, k) + g.w1j * n(i, j + 1, k) + g.w0j * n(i + 1, j, k) +
+                        g.w1j * n(i + 1, j + 1, k));
     T nu_y1 = nu + n(i, j, k);
     T nu_y2 = nu + n(i, j + 1, k);
-    T nu_z1 = nu + h * (w0[j] * n(i, j, k - 1) + w1[j] * n(i, j + 1, k - 1) + w0[j] * n(i, j, k) +
-                        w1[j] * n(i, j + 1, k));
-    T nu_z2 = nu + h * (w0[j] * n(i, j, k) + w1[j] * n(i, j + 1, k) + w0[j] * n(i, j, k + 1) +
-                        w1[j] * n(i, j + 1, k + 1));
-    T t4 = one / dx_3 * (nu_x2 * one / dx_2 * (v(i + 1, j, k) - v(i, j, k)) -
-                         nu_x1 * one / dx_1 * (v(i, j, k) - v(i - 1, j, k))) +
-           one / dy_3 * (nu_y2 * one / dy_2 * (v(i, j + 1, k) - v(i, j, k)) -
-                         nu_y1 * one / dy_1 * (v(i, j, k) - v(i, j - 1, k))) +
-           one / dz_3 * (nu_z2 * one / dz_2 * (v(i, j, k + 1) - v(i, j, k)) -
-                         nu_z1 * one / dz_1 * (v(i, j, k) - v(i, j, k - 1)));
+    T nu_z1 = nu + h * (g.w0j * n(i, j, k - 1) + g.w1j * n(i, j + 1, k - 1) + g.w0j * n(i, j, k) +
+                        g.w1j * n(i, j + 1, k));
+    T nu_z2 = nu + h * (g.w0j * n(i, j, k) + g.w1j * n(i, j + 1, k) + g.w0j * n(i, j, k + 1) +
+                        g.w1j * n(i, j + 1, k + 1));
+    T t4 = g.rx_f * (nu_x2 * g.rx_cp * (v(i + 1, j, k) - v(i, j, k)) -
+                     nu_x1 * g.rx_c * (v(i, j, k) - v(i - 1, j, k))) +
+           g.ry_cp * (nu_y2 * g.ry_fp * (v(i, j + 1, k) - v(i, j, k)) -
+                      nu_y1 * g.ry_f * (v(i, j, k) - v(i, j - 1, k))) +
+           gz.rz_f * (nu_z2 * gz.rz_cp * (v(i, j, k + 1) - v(i, j, k)) -
+                      nu_z1 * gz.rz_c * (v(i, j, k) - v(i, j, k - 1)));
     return ((t1 + t2) + t3) + t4;
 }
 
+// code/equations.f90:327-468, with dx as for v, dy_3 = y(j)-y(j-1),
+// dz_1 = z(k)-z(k-1), dz_2 = z(k+1)-z(k), dz_3 = zg(k+1)-zg(k)
 template <typename T, typename A>
-__device__ __forceinline__ T rhs_w_gen(const A& F, const Metric<T>& m, T nu, int i, int j, int k)
+__device__ __forceinline__ T rhs_w_gen(const A& F, const RhsXY<T>& g, const RhsZ<T>& gz, T nu, int i, int j, int k)
 {
     auto u = [&](int a, int b, int c) { return F(0, a, b, c); };
     auto v = [&](int a, int b, int c) { return F(1, a, b, c); };
     auto w = [&](int a, int b, int c) { return F(2, a, b, c); };
     auto n = [&](int a, int b, int c) { return F(3, a, b, c); };
-    const T h = T(0.5), q = T(0.25), one = T(1);
-    const T *x = m.x, *y = m.y, *z = m.z, *xg = m.xg, *yg = m.yg, *zg = m.zg, *w0 = m.w0, *w1 = m.w1;
+    const T h = T(0.5), q = T(0.25);
     T wc = h * (w(i, j, k) + w(i, j, k + 1));
     T wcm = h * (w(i, j, k - 1) + w(i, j, k));
-    T t1 = -(wc * wc - wcm * wcm) / (zg[k + 1] - zg[k]);
+    T t1 = -(wc * wc - wcm * wcm) * gz.rz_cp;
     T uwp = (h * (u(i, j, k) + u(i, j, k + 1))) * (h * (w(i, j, k) + w(i + 1, j, k)));
     T uwm = (h * (u(i - 1, j, k) + u(i - 1, j, k + 1))) * (h * (w(i - 1, j, k) + w(i, j, k)));
-    T t2 = -(uwp - uwm) / (x[i] - x[i - 1]);
-    T vwp = (h * (v(i, j, k) + v(i, j, k + 1))) * (w0[j] * w(i, j, k) + w1[j] * w(i, j + 1, k));
-    T vwm = (h * (v(i, j - 1, k) + v(i, j - 1, k + 1))) * (w0[j - 1] * w(i, j - 1, k) + w1[j - 1] * w(i, j, k));
-    T t3 = -(vwp - vwm) / (y[j] - y[j - 1]);
-    T dz_1 = z[k] - z[k - 1], dz_2 = z[k + 1] - z[k], dz_3 = zg[k + 1] - zg[k];
-    T dy_3 = y[j] - y[j - 1];
-    T dx_1 = xg[i] - xg[i - 1], dx_2 = xg[i + 1] - xg[i], dx_3 = x[i] - x[i - 1];
+    T t2 = -(uwp - uwm) * g.rx_f;
+    T vwp = (h * (v(i, j, k) + v(i, j, k + 1))) * (g.w0j * w(i, j, k) + g.w1j * w(i, j + 1, k));
+    T vwm = (h * (v(i, j - 1, k) + v(i, j - 1, k + 1))) * (g.w0m * w(i, j - 1, k) + g.w1m * w(i, j, k));
+    T t3 = -(vwp - vwm) * g.ry_f;
     T nu_x1 = nu + q * (n(i, j, k) + n(i, j, k + 1) + n(i - 1, j, k) + n(i - 1, j, k + 1));
     T nu_x2 = nu + q * (n(i, j, k) + n(i, j, k + 1) + n(i + 1, j, k) + n(i + 1, j, k + 1));
-    T nu_y1 = nu + h * (w0[j - 1] * n(i, j - 1, k) + w1[j - 1] * n(i, j, k) + w0[j - 1] * n(i, j - 1, k + 1) +
-                        w1[j - 1] * n(i, j, k + 1));
-    T nu_y2 = nu + h * (w0[j] * n(i, j, k) + w1[j] * n(i, j + 1, k) + w0[j] * n(i, j, k + 1) +
-                        w1[j] * n(i, j + 1, k + 1));
+    T nu_y1 = nu + h * (g.w0m * n(i, j - 1, k) + g.w1m * n(i, j, k) + g.w0m * n(i, j - 1, k + 1) +
+                        g.w1m * n(i, j, k + 1));
+    T nu_y2 = nu + h * (g.w0j * n(i, j, k) + g.w1j * n(i, j + 1, k) + g.w0j * n(i, j, k + 1) +
+                        g.w1j * n(i, j + 1, k + 1));
     T nu_z1 = nu + n(i, j, k);
     T nu_z2 = nu + n(i, j, k + 1);
-    T dwdy_p = (w(i, j + 1, k) - w(i, j, k)) / (yg[j + 1] - yg[j]);
-    T dwdy_m = (w(i, j, k) - w(i, j - 1, k)) / (yg[j] - yg[j - 1]);
-    T t4 = one / dx_3 * (nu_x2 * one / dx_2 * (w(i + 1, j, k) - w(i, j, k)) -
-                         nu_x1 * one / dx_1 * (w(i, j, k) - w(i - 1, j, k))) +
-           one / dy_3 * (nu_y2 * dwdy_p - nu_y1 * dwdy_m) +
-           one / dz_3 * (nu_z2 * one / dz_2 * (w(i, j, k + 1) - w(i, j, k)) -
-                         nu_z1 * one / dz_1 * (w(i, j, k) - w(i, j, k - 1)));
+    T dwdy_p = (w(i, j + 1, k) - w(i, j, k)) * g.ry_cp;
+    T dwdy_m = (w(i, j, k) - w(i, j - 1, k)) * g.ry_c;
+    T t4 = g.rx_f * (nu_x2 * g.rx_cp * (w(i + 1, j, k) - w(i, j, k)) -
+                     nu_x1 * g.rx_c * (w(i, j, k) - w(i - 1, j, k))) +
+           g.ry_f * (nu_y2 * dwdy_p - nu_y1 * dwdy_m) +
+           gz.rz_cp * (nu_z2 * gz.rz_fp * (w(i, j, k + 1) - w(i, j, k)) -
+                       nu_z1 * gz.rz_f * (w(i, j, k) - w(i, j, k - 1)));
     return ((t1 + t2) + t3) + t4;
 }
-
-template <int TX, int TY>
-struct RhsTile {
-    static constexpr int PX = TX + 2, PY = TY + 2, NP = PX * PY;
-};
 
 // mode 0: rhs_u/v/w into Ou/Ov/Ow on the reference's interior ranges;
 // mode s = 1,2,3: Wray substep s.  In RK modes the x and z ranges stop at the
 // last unique cell (nxg-2, nzg-2): the image columns the reference computes
 // there are overwritten by the boundary conditions that follow.
+//
+// Pipelining: while plane k is computed from the LDS ring (planes k-1, k,
+// k+1), the loads of plane k+2 are already in flight into registers; they
+// go to the ring slot of plane k-1 once every thread is done with plane k.
+// In substeps 2 and 3 the thread's Uo and F1 values are fetched before the
+// stencil as well.
 template <typename T, int TX, int TY>
 __global__ void __launch_bounds__(TX* TY)
     k_rhs_rk(const T* __restrict__ U, const T* __restrict__ V, const T* __restrict__ W, const T* __restrict__ nt,
-             Metric<T> m, Dims d, T nu, const Scalars* sc, int mode, const T* Uo, const T* Vo, const T* Wo, T* F1u,
+             RhsTabs<T> tb, Dims d, T nu, const Scalars* sc, int mode, const T* Uo, const T* Vo, const T* Wo, T* F1u,
              T* F1v, T* F1w, T* Ou, T* Ov, T* Ow, T c1, T c2, int kchunk)
 {
-    using Tl = RhsTile<TX, TY>;
-    constexpr int PX = Tl::PX, NP = Tl::NP;
+    constexpr int PX = TX + 2, PY = TY + 2, NP = PX * PY, NT = TX * TY, NL = (NP + NT - 1) / NT;
     __shared__ T sh[4][3][NP];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
     const int i0 = blockIdx.x * TX + 2, j0 = blockIdx.y * TY + 2;
@@ -156,78 +174,130 @@ __global__ void __launch_bounds__(TX* TY)
     const int ke = min(kb + kchunk - 1, kmax);
     const int i = i0 + tx, j = j0 + ty;
     const T* src[4] = {U, V, W, nt};
+    // this thread's share of a halo plane: LDS index and box offset (-1: outside)
+    int lq[NL];
+    long lo[NL];
+#pragma unroll
+    for (int r = 0; r < NL; r++) {
+        const int q = tid + r * NT;
+        const int lx = q % PX, ly = q / PX;
+        const int ii = i0 - 1 + lx, jj = j0 - 1 + ly;
+        lq[r] = q < NP ? q : -1;
+        lo[r] = (q < NP && ii <= d.nxg && jj <= d.nyg) ? at(d, ii, jj, 1) : -1;
+    }
+    T pre[NL][4];
+    auto gload = [&](int p) {
+        const long po = (long)(p - 1) * d.sxy;
+#pragma unroll
+        for (int r = 0; r < NL; r++)
+#pragma unroll
+            for (int f = 0; f < 4; f++) pre[r][f] = lo[r] >= 0 ? src[f][lo[r] + po] : T(0);
+    };
     auto slot = [&](int p) { return (p - kb + 3) % 3; };
-    auto load = [&](int p) {
+    auto sstore = [&](int p) {
         const int s = slot(p);
-        for (int q = tid; q < NP; q += TX * TY) {
-            const int lx = q % PX, ly = q / PX;
-            const int ii = i0 - 1 + lx, jj = j0 - 1 + ly;
-            const bool ok = ii <= d.nxg && jj <= d.nyg;
-            const long o = ok ? at(d, ii, jj, p) : 0;
-            for (int f = 0; f < 4; f++) sh[f][s][q] = ok ? src[f][o] : T(0);
-        }
+#pragma unroll
+        for (int r = 0; r < NL; r++)
+            if (lq[r] >= 0)
+#pragma unroll
+                for (int f = 0; f < 4; f++) sh[f][s][lq[r]] = pre[r][f];
     };
     auto F = [&](int f, int a, int b, int c) -> T { return sh[f][slot(c)][(a - i0 + 1) + PX * (b - j0 + 1)]; };
     const T dPdx = T(sc->dPdx);
     const T dt = T(sc->dt);
-    load(kb - 1);
-    load(kb);
+    RhsXY<T> g;
+    {
+        const int ic = min(i, imax), jc = min(j, d.nyg - 1);
+        g.rx_f = tb.rdx[ic]; g.rx_fp = tb.rdx[ic + 1]; g.rx_c = tb.rdxg[ic]; g.rx_cp = tb.rdxg[ic + 1];
+        g.ry_f = tb.rdy[jc]; g.ry_fp = tb.rdy[jc + 1]; g.ry_c = tb.rdyg[jc]; g.ry_cp = tb.rdyg[jc + 1];
+        g.w0j = tb.w0[jc]; g.w1j = tb.w1[jc]; g.w0m = tb.w0[jc - 1]; g.w1m = tb.w1[jc - 1];
+    }
+    gload(kb - 1);
+    sstore(kb - 1);
+    gload(kb);
+    sstore(kb);
+    gload(kb + 1);
+    sstore(kb + 1);
+    __syncthreads();
+    const bool act = i <= imax && j <= d.nyg - 1;
     for (int k = kb; k <= ke; k++) {
-        load(k + 1);
-        __syncthreads();
-        if (i <= imax && j <= d.nyg - 1) {
-            const long p = at(d, i, j, k);
-            const bool du = (i <= d.nx - 1) && (k <= d.nzg - 1);
-            const bool dv = (j <= d.ny - 1) && (k <= d.nzg - 1);
-            const bool dw = (k <= d.nz - 1);
-            T ru = T(0), rv = T(0), rw = T(0);
-            if (du) ru = rhs_u_gen<T>(F, m, nu, dPdx, i, j, k);
-            if (dv) rv = rhs_v_gen<T>(F, m, nu, i, j, k);
-            if (dw) rw = rhs_w_gen<T>(F, m, nu, i, j, k);
-            if (mode == 0) {
-                if (du) Ou[p] = ru;
-                if (dv) Ov[p] = rv;
-                if (dw) Ow[p] = rw;
-            } else if (mode == 1) {
-                // U = Uo + dt*rk_coef(1,1)*Fu1 with Uo = U at substep 1; the
-                // new field is stored with its boundary-condition images
-                if (du) { F1u[p] = ru; store_img(Ou, d, 0, i, j, k, F(0, i, j, k) + dt * c1 * ru); }
-                if (dv) { F1v[p] = rv; store_img(Ov, d, 1, i, j, k, F(1, i, j, k) + dt * c1 * rv); }
-                if (dw) { F1w[p] = rw; store_img(Ow, d, 2, i, j, k, F(2, i, j, k) + dt * c1 * rw); }
-            } else {
-                // U = Uo + dt*(c1*Fu1 + c2*Fu_s); rk_coef(3,2) = 0 so the second stage's F is not kept
-                if (du) store_img(Ou, d, 0, i, j, k, Uo[p] + dt * (c1 * F1u[p] + c2 * ru));
-                if (dv) store_img(Ov, d, 1, i, j, k, Vo[p] + dt * (c1 * F1v[p] + c2 * rv));
-                if (dw) store_img(Ow, d, 2, i, j, k, Wo[p] + dt * (c1 * F1w[p] + c2 * rw));
-            }
+        const bool more = k + 2 <= ke + 1;
+        if (more) gload(k + 2);
+        const long p = at(d, i, j, k);
+        const bool du = act && (i <= d.nx - 1) && (k <= d.nzg - 1);
+        const bool dv = act && (j <= d.ny - 1) && (k <= d.nzg - 1);
+        const bool dw = act && (k <= d.nz - 1);
+        T uo = T(0), vo = T(0), wo = T(0), fu = T(0), fv = T(0), fw = T(0);
+        if (mode >= 2) {
+            if (du) { uo = Uo[p]; fu = F1u[p]; }
+            if (dv) { vo = Vo[p]; fv = F1v[p]; }
+            if (dw) { wo = Wo[p]; fw = F1w[p]; }
+        }
+        RhsZ<T> gz;
+        gz.rz_f = tb.rdz[k]; gz.rz_fp = tb.rdz[k + 1]; gz.rz_c = tb.rdzg[k]; gz.rz_cp = tb.rdzg[k + 1];
+        T ru = T(0), rv = T(0), rw = T(0);
+        if (du) ru = rhs_u_gen<T>(F, g, gz, nu, dPdx, i, j, k);
+        if (dv) rv = rhs_v_gen<T>(F, g, gz, nu, i, j, k);
+        if (dw) rw = rhs_w_gen<T>(F, g, gz, nu, i, j, k);
+        if (mode == 0) {
+            if (du) Ou[p] = ru;
+            if (dv) Ov[p] = rv;
+            if (dw) Ow[p] = rw;
+        } else if (mode == 1) {
+            // U = Uo + dt*rk_coef(1,1)*Fu1 with Uo = U at substep 1; the
+            // new field is stored with its boundary-condition images
+            if (du) { F1u[p] = ru; store_img(Ou, d, 0, i, j, k, F(0, i, j, k) + dt * c1 * ru); }
+            if (dv) { F1v[p] = rv; store_img(Ov, d, 1, i, j, k, F(1, i, j, k) + dt * c1 * rv); }
+            if (dw) { F1w[p] = rw; store_img(Ow, d, 2, i, j, k, F(2, i, j, k) + dt * c1 * rw); }
+        } else {
+            // U = Uo + dt*(c1*Fu1 + c2*Fu_s); rk_coef(3,2) = 0 so the second stage's F is not kept
+            if (du) store_img(Ou, d, 0, i, j, k, uo + dt * (c1 * fu + c2 * ru));
+            if (dv) store_img(Ov, d, 1, i, j, k, vo + dt * (c1 * fv + c2 * rv));
+            if (dw) store_img(Ow, d, 2, i, j, k, wo + dt * (c1 * fw + c2 * rw));
         }
         __syncthreads();
+        if (more) {
+            sstore(k + 2);
+            __syncthreads();
+        }
     }
 }
 
-static constexpr int RHS_TX = 64, RHS_TY = 4, RHS_KCHUNK = 16;
+// Tile shapes (x cells, y cells) and z-chunk lengths the launcher offers;
+// chan_autotune times them on the channel at hand.
+struct RhsShape { int tx, ty, kchunk; };
+static const RhsShape kRhsShapes[] = {
+    {64, 4, 16}, {64, 4, 8}, {64, 8, 8}, {64, 8, 16}, {32, 8, 8}, {32, 8, 16}, {64, 4, 32}, {64, 8, 32},
+};
+int rhs_shape_count() { return (int)(sizeof(kRhsShapes) / sizeof(kRhsShapes[0])); }
 
 template <typename T>
-void launch_rhs_rk(const T* U, const T* V, const T* W, const T* nut, const Metric<T>& m, const Dims& d, T nu,
+void launch_rhs_rk(const T* U, const T* V, const T* W, const T* nut, const RhsTabs<T>& m, const Dims& d, T nu,
                    const Scalars* sc, int mode, const T* Uo, const T* Vo, const T* Wo, T* F1u, T* F1v, T* F1w,
-                   T* Ou, T* Ov, T* Ow, double c1, double c2, hipStream_t s)
+                   T* Ou, T* Ov, T* Ow, double c1, double c2, int shape, hipStream_t s)
 {
+    const RhsShape sh = kRhsShapes[(shape >= 0 && shape < rhs_shape_count()) ? shape : 0];
     const int ni = (mode == 0 ? d.nxg - 1 : d.nxg - 2) - 1;
     const int nk = (mode == 0 ? d.nzg - 1 : d.nzg - 2) - 1;
-    dim3 blk(RHS_TX, RHS_TY, 1);
-    dim3 grd(cdiv(ni, RHS_TX), cdiv(d.nyg - 2, RHS_TY), cdiv(nk, RHS_KCHUNK));
+    dim3 blk(sh.tx, sh.ty, 1);
+    dim3 grd(cdiv(ni, sh.tx), cdiv(d.nyg - 2, sh.ty), cdiv(nk, sh.kchunk));
     PROF_BEGIN("k_rhs_rk", s);
-    k_rhs_rk<T, RHS_TX, RHS_TY><<<grd, blk, 0, s>>>(U, V, W, nut, m, d, nu, sc, mode, Uo, Vo, Wo, F1u, F1v, F1w, Ou,
-                                                    Ov, Ow, T(c1), T(c2), RHS_KCHUNK);
+#define RHS_GO(TX, TY)                                                                                          \
+    k_rhs_rk<T, TX, TY><<<grd, blk, 0, s>>>(U, V, W, nut, m, d, nu, sc, mode, Uo, Vo, Wo, F1u, F1v, F1w, Ou, Ov, \
+                                            Ow, T(c1), T(c2), sh.kchunk)
+    if (sh.tx == 64 && sh.ty == 4) RHS_GO(64, 4);
+    else if (sh.tx == 64 && sh.ty == 8) RHS_GO(64, 8);
+    else RHS_GO(32, 8);
+#undef RHS_GO
     PROF_END(s);
 }
 
 template void launch_rhs_rk<double>(const double*, const double*, const double*, const double*,
-                                    const Metric<double>&, const Dims&, double, const Scalars*, int, const double*,
+                                    const RhsTabs<double>&, const Dims&, double, const Scalars*, int, const double*,
                                     const double*, const double*, double*, double*, double*, double*, double*,
-                                    double*, double, double, hipStream_t);
-template void launch_rhs_rk<float>(const float*, const float*, const float*, const float*, const Metric<float>&,
+                                    double*, double, double, int, hipStream_t);
+template void launch_rhs_rk<float>(const float*, const float*, const float*, const float*, const RhsTabs<float>&,
                                    const Dims&, float, const Scalars*, int, const float*, const float*, const float*,
-                                   float*, float*, float*, float*, float*, float*, double, double, hipStream_t);
+                                   float*, float*, float*, float*, float*, float*, double, double, int, hipStream_t);
 
 }  // namespace chan

@@ -128,6 +128,19 @@ int chan_profile_entry(const chan_ctx* ctx, int idx, char* name, int len, double
  * the reference point that scales the hardware traffic counters). */
 int chan_stream_copy(chan_ctx* ctx, long nbytes, int reps, double* ms_per_copy);
 
+/* Tile shapes and the autotuner.  No reference counterpart (the reference's
+ * loops have no tiles).  CHAN_KERNEL_RHS selects the fused RHS + RK kernel,
+ * whose shapes differ in (x, y) tile and z-chunk length.  chan_autotune times
+ * every shape on this channel (reps launches each of the kernel's substep-2
+ * form, writing into the rhs buffers: the state is not changed), keeps the
+ * fastest for the step and, when times_ms is not NULL, writes the average ms
+ * per launch of the first max_out shapes. */
+enum { CHAN_KERNEL_RHS = 0 };
+int chan_tile_shape_count(int kernel);
+int chan_set_tile_shape(chan_ctx* ctx, int kernel, int shape);
+int chan_get_tile_shape(const chan_ctx* ctx, int kernel);  /* shape index, or < 0 */
+int chan_autotune(chan_ctx* ctx, int reps, double* times_ms, int max_out);
+
 #ifdef __cplusplus
 }
 #endif
