@@ -29,11 +29,17 @@ struct DsmTile {
     static constexpr int LX = TXO + 4, LY = TYO + 4, NL = LX * LY;  // level-0 plane
 };
 
+// Two blocks fit a CU's LDS; the register budget is held to the same two
+// waves per SIMD.
 template <typename T, int TXO, int TYO>
-__global__ void __launch_bounds__((TXO + 2) * (TYO + 2))
+__global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_waves_per_eu(2)))
     k_dsm_fused(const T* __restrict__ U, const T* __restrict__ V, const T* __restrict__ W, Dims d, DsmBufs<T> b,
                 int kchunk)
 {
+    // The model's sums are reordered against the reference anyway (separable
+    // filter, per-row partials); contracting a*b+c into FMAs here saves a
+    // third of the FP64 instructions and changes results only by rounding.
+#pragma clang fp contract(fast)
     using Tl = DsmTile<TXO, TYO>;
     constexpr int CX = Tl::CX, NC = Tl::NC, LX = Tl::LX, NL = Tl::NL;
     __shared__ T l0[3][3][NL];  // [slot][U,V,W][position]
@@ -51,30 +57,63 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2))
     const int ke = min(kb + kchunk - 1, d.nzg - 1);
     const int ci = i0 - 1 + cx, cj = j0 - 1 + cy;  // column position (x unwrapped)
     const int nxp = d.nxp, nzp = d.nzp;
-    auto wx = [&](int q) { int r = (q - 2) % nxp; return 2 + (r < 0 ? r + nxp : r); };
-    auto wz = [&](int q) { int r = (q - 2) % nzp; return 2 + (r < 0 ? r + nzp : r); };
+    // periodic wrap onto the unique points 2..nxp+1; every index used here is
+    // within one period of that range
+    auto wx = [&](int q) { return q < 2 ? q + nxp : (q > nxp + 1 ? q - nxp : q); };
+    auto wz = [&](int q) { return q < 2 ? q + nzp : (q > nzp + 1 ? q - nzp : q); };
     const T h = T(0.5), qq = T(0.25);
     const T s6 = T(1.0 / 6.0), s23 = T(2.0 / 3.0), s13 = T(1.0 / 3.0);
     const T *rdx = b.rdx, *rdy = b.rdy, *rdz = b.rdz, *rdxg = b.rdxg, *rdyg = b.rdyg, *rdzg = b.rdzg;
 
     auto slot = [&](int p) { return (p - kb + 3) % 3; };  // ring slot of plane p >= kb-2
 
-    auto load_plane = [&](int p) {
-        const int s = slot(p), pz = wz(p);
-        for (int q = tid; q < NL; q += NC) {
-            const int lx = q % LX, ly = q / LX;
-            const int ii = wx(i0 - 2 + lx), jj = j0 - 2 + ly;
+    // this thread's share of a level-0 plane: box offset within a plane
+    // (-1 outside the walls) and whether V exists there; plane p's values are
+    // fetched into registers (gload) one iteration before they are stored
+    // into the ring (sstore), so the loads overlap the previous plane's work
+    constexpr int NLD = (NL + NC - 1) / NC;
+    long lo[NLD];
+    bool lv[NLD];
+#pragma unroll
+    for (int r = 0; r < NLD; r++) {
+        const int q = tid + r * NC;
+        const int lx = q % LX, ly = q / LX;
+        const int ii = wx(i0 - 2 + lx), jj = j0 - 2 + ly;
+        const bool in = q < NL && jj >= 1 && jj <= d.nyg;
+        lo[r] = in ? at(d, ii, jj, 1) : -1;
+        lv[r] = in && jj <= d.ny;
+    }
+    T pre[NLD][3];
+    auto gload = [&](int p) {
+        const long po = (long)(wz(p) - 1) * d.sxy;
+#pragma unroll
+        for (int r = 0; r < NLD; r++) {
             T u = T(0), v = T(0), w = T(0);
-            if (jj >= 1 && jj <= d.nyg) {
-                const long o = at(d, ii, jj, pz);
-                u = U[o];
-                w = W[o];
-                if (jj <= d.ny) v = V[o];
+            if (lo[r] >= 0) {
+                u = U[lo[r] + po];
+                w = W[lo[r] + po];
+                if (lv[r]) v = V[lo[r] + po];
             }
-            l0[s][0][q] = u;
-            l0[s][1][q] = v;
-            l0[s][2][q] = w;
+            pre[r][0] = u;
+            pre[r][1] = v;
+            pre[r][2] = w;
         }
+    };
+    auto sstore = [&](int p) {
+        const int s = slot(p);
+#pragma unroll
+        for (int r = 0; r < NLD; r++) {
+            const int q = tid + r * NC;
+            if (q < NL) {
+                l0[s][0][q] = pre[r][0];
+                l0[s][1][q] = pre[r][1];
+                l0[s][2][q] = pre[r][2];
+            }
+        }
+    };
+    auto load_plane = [&](int p) {
+        gload(p);
+        sstore(p);
     };
     auto L0 = [&](int f, int p, int dx, int dy) -> T { return l0[slot(p)][f][(cx + 1 + dx) + LX * (cy + 1 + dy)]; };
 
@@ -172,9 +211,11 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2))
     double acc_lm = 0.0, acc_mm = 0.0;
     const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
 
+    gload(kb + 2);
     for (int k = kb; k <= ke; k++) {
         __syncthreads();
-        load_plane(k + 2);
+        sstore(k + 2);
+        if (k + 3 <= ke + 2) gload(k + 3);
         __syncthreads();
         yz_combine(k + 1);
         __syncthreads();
@@ -193,15 +234,16 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2))
             const T wyp = (j == d.nyg - 1) ? T(0) : (j == 2 ? s13 : s6);
             const T wyv[3] = {wym, s23, wyp};
             const T wxv[3] = {s6, s23, s6};
-            T F[15];
-            for (int q = 0; q < 15; q++) F[q] = T(0);
-            for (int dy = -1; dy <= 1; dy++) {
-                for (int dx = -1; dx <= 1; dx++) {
-                    const T wgt = wxv[dx + 1] * wyv[dy + 1];
-                    const int col = (cx + dx) + CX * (cy + dy);
-                    for (int q = 0; q < 15; q++) F[q] = F[q] + wgt * zq[q][col];
-                }
-            }
+            // 3x3 x-y taps of one z-combined quantity
+            auto tap = [&](int q) {
+                T f = T(0);
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dx = -1; dx <= 1; dx++)
+                        f = f + (wxv[dx + 1] * wyv[dy + 1]) * zq[q][(cx + dx) + CX * (cy + dy)];
+                return f;
+            };
+            T F[3];
+            for (int q = 0; q < 3; q++) F[q] = tap(q);
             // hat(S) from the filtered velocities at the true index (code/subgrid.f90:456)
             const int i = ci;
             auto uf = [&](int dx, int dy, int dz) { return lf[slot(k + dz)][0][(cx + dx) + CX * (cy + dy)]; };
@@ -235,8 +277,8 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2))
             const T shm4 = T(4) * sqrt(acc);
             T lm = T(0), mm = T(0);
             for (int q = 0; q < 6; q++) {
-                const T L = F[3 + q] - F[pa[q]] * F[pb[q]];
-                const T M = shm4 * sh[q] - F[9 + q];
+                const T L = tap(3 + q) - F[pa[q]] * F[pb[q]];
+                const T M = shm4 * sh[q] - tap(9 + q);
                 const T wgt = q < 3 ? T(1) : T(2);
                 lm = lm + wgt * (L * M);
                 mm = mm + wgt * (M * M);
