@@ -17,7 +17,7 @@ __all__ = ["Channel", "ChanError", "load_library", "LIB_PATH", "FP64", "FP32_BUL
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libchanles.so")
 
 FP64, FP32_BULK = 0, 1
-WALL_GIVEN, WALL_EQUILIBRIUM = 0, 1
+WALL_GIVEN, WALL_EQUILIBRIUM = 0, 1  # WALL_MLP = 2 below
 U, V, W, NU_T, P, ALPHA_X, RHS_U, RHS_V, RHS_W = range(9)
 
 EXPORTS = [
@@ -31,9 +31,13 @@ EXPORTS = [
     "chan_compute_equilibrium_wall_stress", "chan_plane_statistics", "chan_run_steps",
     "chan_synchronize", "chan_profile_steps", "chan_profile_count", "chan_profile_entry",
     "chan_stream_copy", "chan_tile_shape_count", "chan_set_tile_shape", "chan_get_tile_shape",
-    "chan_autotune",
+    "chan_autotune", "chan_mlp_create", "chan_mlp_destroy", "chan_mlp_last_error", "chan_mlp_get_params",
+    "chan_mlp_set_params", "chan_mlp_fit_normalization", "chan_mlp_forward", "chan_mlp_gradients",
+    "chan_mlp_train", "chan_wall_samples", "chan_set_wall_network", "chan_wall_model_stats", "chan_evaluate_wall_stress",
 ]
 KERNEL_RHS = 0
+WALL_MLP = 2
+MLP_NPARAM, MLP_NNORM = 8770, 12
 
 
 class ChanError(RuntimeError):
@@ -92,12 +96,110 @@ def load_library(path=LIB_PATH):
     L.chan_set_tile_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.chan_get_tile_shape.argtypes = [vp, ctypes.c_int]
     L.chan_autotune.argtypes = [vp, ctypes.c_int, _D, ctypes.c_int]
+    _F = ctypes.POINTER(ctypes.c_float)
+    L.chan_mlp_create.argtypes = [ctypes.c_int, ctypes.c_ulonglong, ctypes.POINTER(vp)]
+    L.chan_mlp_destroy.argtypes = [vp]
+    L.chan_mlp_destroy.restype = None
+    L.chan_mlp_last_error.argtypes = [vp]
+    L.chan_mlp_last_error.restype = ctypes.c_char_p
+    L.chan_mlp_get_params.argtypes = [vp, _F, _F]
+    L.chan_mlp_set_params.argtypes = [vp, _F, _F]
+    L.chan_mlp_fit_normalization.argtypes = [vp, _F, _F, ctypes.c_int]
+    L.chan_mlp_forward.argtypes = [vp, _F, ctypes.c_int, _F]
+    L.chan_mlp_gradients.argtypes = [vp, _F, _F, ctypes.c_int, _F, _D]
+    L.chan_mlp_train.argtypes = [vp, _F, _F, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                 ctypes.c_ulonglong, _D]
+    L.chan_wall_samples.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _F, _F]
+    L.chan_set_wall_network.argtypes = [vp, vp, ctypes.c_double]
+    L.chan_wall_model_stats.argtypes = [vp, _D]
+    L.chan_evaluate_wall_stress.argtypes = [vp]
     _LIB = L
     return L
 
 
 def _p(a):
     return a.ctypes.data_as(_D)
+
+
+def _pf(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _f32(a, cols):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim != 2 or a.shape[1] != cols:
+        raise ValueError(f"expected an (n, {cols}) array, got {a.shape}")
+    return a
+
+
+class WallNetwork:
+    """The learned wall-stress closure (a chan_mlp): 4 -> 64 -> 64 -> 64 -> 2,
+    SiLU, BF16 weights with FP32 accumulation, trained with Adam on the GPU.
+    Features X are (n, 4) and outputs Y (n, 2), raw (unstandardised)."""
+
+    def __init__(self, seed=0, device=0):
+        L = load_library()
+        h = ctypes.c_void_p()
+        rc = L.chan_mlp_create(device, seed, ctypes.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = L.chan_mlp_last_error(h).decode() if h.value else "chan_mlp_create failed"
+            L.chan_mlp_destroy(h)
+            self._h = None
+            raise ChanError(f"chan_mlp_create: {msg} (status {rc})")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().chan_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise ChanError(f"{what}: {load_library().chan_mlp_last_error(self._h).decode()} (status {rc})")
+
+    def params(self):
+        p = np.zeros(MLP_NPARAM, np.float32)
+        n = np.zeros(MLP_NNORM, np.float32)
+        self._chk(load_library().chan_mlp_get_params(self._h, _pf(p), _pf(n)), "get_params")
+        return p, n
+
+    def set_params(self, params=None, norm=None):
+        pp = None if params is None else np.ascontiguousarray(params, np.float32)
+        nn = None if norm is None else np.ascontiguousarray(norm, np.float32)
+        if pp is not None and pp.shape != (MLP_NPARAM,):
+            raise ValueError("params must hold 8770 values")
+        if nn is not None and nn.shape != (MLP_NNORM,):
+            raise ValueError("norm must hold 12 values")
+        self._chk(load_library().chan_mlp_set_params(self._h, None if pp is None else _pf(pp),
+                                                     None if nn is None else _pf(nn)), "set_params")
+
+    def fit_normalization(self, X, Y):
+        X, Y = _f32(X, 4), _f32(Y, 2)
+        self._chk(load_library().chan_mlp_fit_normalization(self._h, _pf(X), _pf(Y), len(X)), "fit_normalization")
+
+    def forward(self, X):
+        X = _f32(X, 4)
+        Y = np.zeros((len(X), 2), np.float32)
+        self._chk(load_library().chan_mlp_forward(self._h, _pf(X), len(X), _pf(Y)), "forward")
+        return Y
+
+    def gradients(self, X, Y):
+        X, Y = _f32(X, 4), _f32(Y, 2)
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = ctypes.c_double()
+        self._chk(load_library().chan_mlp_gradients(self._h, _pf(X), _pf(Y), len(X), _pf(g), ctypes.byref(loss)),
+                  "gradients")
+        return g, loss.value
+
+    def train(self, X, Y, epochs=20, batch=1024, lr=1e-3, seed=0):
+        X, Y = _f32(X, 4), _f32(Y, 2)
+        losses = np.zeros(max(epochs, 1))
+        self._chk(load_library().chan_mlp_train(self._h, _pf(X), _pf(Y), len(X), int(epochs), int(batch),
+                                                float(lr), int(seed), _p(losses)), "train")
+        return losses[:epochs]
 
 
 class Channel:
@@ -110,6 +212,7 @@ class Channel:
         self.nx, self.ny, self.nz = (len(a) for a in self._keep)
         self.nxg, self.nyg, self.nzg = self.nx + 1, self.ny + 1, self.nz + 1
         self.precision = precision
+        self.nu = nu
         cfg = _Config(self.nx, self.ny, self.nz, _p(self._keep[0]), _p(self._keep[1]), _p(self._keep[2]),
                       nu, CFL, dPdx, x_mass_cte, LES_model, Dirichlet_nu_t, precision, device)
         h = ctypes.c_void_p()
@@ -248,6 +351,32 @@ class Channel:
 
     def synchronize(self):
         self._chk(load_library().chan_synchronize(self._h), "synchronize")
+
+    def wall_samples(self, h_wm=0.0, nu=None, kappa=0.41, B=5.2):
+        """Features (2*nx*nzg, 4) and equilibrium targets (2*nx*nzg, 2) of every
+        wall cell at height h_wm, with viscosity nu (default: the channel's)."""
+        n = 2 * self.nx * self.nzg
+        X = np.zeros((n, 4), np.float32)
+        Y = np.zeros((n, 2), np.float32)
+        nu = self.nu if nu is None else nu
+        self._chk(load_library().chan_wall_samples(self._h, h_wm, nu, kappa, B, _pf(X), _pf(Y)), "wall_samples")
+        return X, Y
+
+    def set_wall_network(self, net, max_rel_err=0.05):
+        self._net = net  # keep it alive while attached
+        self._chk(load_library().chan_set_wall_network(self._h, None if net is None else net._h, max_rel_err),
+                  "set_wall_network")
+
+    def evaluate_wall_stress(self):
+        """Evaluate the selected wall-stress source once into alpha_x."""
+        self._chk(load_library().chan_evaluate_wall_stress(self._h), "evaluate_wall_stress")
+
+    def wall_model_stats(self):
+        """(mean relative deviation from the equilibrium stress, fallback fraction)
+        of the last learned-wall-model evaluation."""
+        out = np.zeros(2)
+        self._chk(load_library().chan_wall_model_stats(self._h, _p(out)), "wall_model_stats")
+        return float(out[0]), float(out[1])
 
     def tile_shape(self, kernel=KERNEL_RHS):
         return load_library().chan_get_tile_shape(self._h, kernel)

@@ -107,6 +107,11 @@ struct chan_ctx {
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
     bool use_graph = true;
     int rhs_shape = 0;  // index into the RHS kernel's tile shapes
+    // learned wall model
+    chan_mlp* net = nullptr;
+    double wm_max_err = 0.05;
+    double* wm_stats = nullptr;   // per 16-cell group: error sums, fallbacks
+    float* wm_samples = nullptr;  // 2*nx*nzg x (4 + 2) floats, on demand
     Prof prof;
     std::vector<std::pair<std::string, std::pair<double, long>>> prof_out;
 };
@@ -297,8 +302,15 @@ void enqueue_rk3(chan_ctx* c, int X)
     for (int s = 0; s < 3; s++) {
         T** I = (T**)c->state[in[s]];
         T** O = (T**)c->state[out[s]];
-        if (c->wall_mode == CHAN_WALL_EQUILIBRIUM)
+        if (c->wall_mode == CHAN_WALL_EQUILIBRIUM) {
             launch_equilibrium_alpha<T>(I[0], c->alpha, c->md, c->d, c->nu, c->h_wm, c->kappa, c->Bc, c->stream);
+        } else if (c->wall_mode == CHAN_WALL_MLP) {
+            const float *np = nullptr, *nn = nullptr;
+            int ndev = 0;
+            if (mlp_device_params(c->net, &np, &nn, &ndev))
+                launch_wm_mlp<T>(np, nn, I[0], I[2], c->alpha, c->sc, c->md, c->d, c->nu, c->h_wm, c->kappa, c->Bc,
+                                 c->x_mass_cte, c->wm_max_err, c->wm_stats, c->stream);
+        }
         enqueue_eddy_viscosity<T>(c, in[s], 1);
         double c1, c2 = 0.0;
         if (s == 0) c1 = c->rk_coef[0][0];
@@ -535,6 +547,7 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
     if ((rc = dalloc(c, (void**)&c->part, npart * sizeof(double)))) return rc;
     if ((rc = dalloc(c, (void**)&c->ratio, (size_t)3 * d.nyg * sizeof(double)))) return rc;
     if ((rc = dalloc(c, (void**)&c->alpha, (size_t)d.nx * 2 * d.nzg * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, (void**)&c->wm_stats, (size_t)2 * wm_mlp_groups(d) * sizeof(double)))) return rc;
     if ((rc = dalloc(c, (void**)&c->stage, (size_t)d.nbox * sizeof(double)))) return rc;
     if ((rc = dalloc(c, (void**)&c->sc, sizeof(Scalars)))) return rc;
     if ((rc = dalloc(c, (void**)&c->pb.p, (size_t)d.nbox * sizeof(double)))) return rc;
@@ -811,7 +824,9 @@ int chan_check_divergence(chan_ctx* c, double* max_div, double* max_abs)
 int chan_set_wall_model(chan_ctx* c, int mode, double h_wm, double kappa, double B)
 {
     if (!c) return CHAN_EINVAL;
-    if (mode != CHAN_WALL_GIVEN && mode != CHAN_WALL_EQUILIBRIUM) return fail(c, CHAN_EINVAL, "invalid wall mode");
+    if (mode != CHAN_WALL_GIVEN && mode != CHAN_WALL_EQUILIBRIUM && mode != CHAN_WALL_MLP)
+        return fail(c, CHAN_EINVAL, "invalid wall mode");
+    if (mode == CHAN_WALL_MLP && !c->net) return fail(c, CHAN_ESTATE, "no wall network set (chan_set_wall_network)");
     if (!(kappa > 0)) return fail(c, CHAN_EINVAL, "kappa must be positive");
     c->wall_mode = mode; c->h_wm = h_wm; c->kappa = kappa; c->Bc = B;
     drop_graphs(c);
@@ -886,6 +901,91 @@ int chan_profile_steps(chan_ctx* c, int nsteps)
 }
 
 int chan_profile_count(const chan_ctx* c) { return c ? (int)c->prof_out.size() : 0; }
+
+int chan_wall_samples(chan_ctx* c, double h_wm, double nu, double kappa, double B, float* X, float* Y)
+{
+    if (!c || !X || !Y) return CHAN_EINVAL;
+    if (!(nu > 0.0) || !(kappa > 0.0)) return fail(c, CHAN_EINVAL, "wall samples: nu and kappa must be positive");
+    int rc = set_device(c);
+    if (rc) return rc;
+    const long ncell = 2L * c->d.nx * c->d.nzg;
+    if (!c->wm_samples && (rc = dalloc(c, (void**)&c->wm_samples, (size_t)ncell * 6 * sizeof(float)))) return rc;
+    float* dX = c->wm_samples;
+    float* dY = c->wm_samples + ncell * 4;
+    void** s = c->state[c->cur];
+    if (c->prec == CHAN_FP64)
+        launch_wm_samples<double>((double*)s[0], (double*)s[2], c->alpha, c->sc, c->md, c->d, nu, h_wm, kappa, B,
+                                  c->x_mass_cte, dX, dY, c->stream);
+    else
+        launch_wm_samples<float>((float*)s[0], (float*)s[2], c->alpha, c->sc, c->md, c->d, nu, h_wm, kappa, B,
+                                 c->x_mass_cte, dX, dY, c->stream);
+    if ((rc = check_launch(c))) return rc;
+    HIPCHK(c, hipMemcpyAsync(X, dX, (size_t)ncell * 4 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(Y, dY, (size_t)ncell * 2 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHAN_OK;
+}
+
+int chan_evaluate_wall_stress(chan_ctx* c)
+{
+    if (!c) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    void** s = c->state[c->cur];
+    if (c->wall_mode == CHAN_WALL_EQUILIBRIUM) {
+        if (c->prec == CHAN_FP64)
+            launch_equilibrium_alpha<double>((double*)s[0], c->alpha, c->md, c->d, c->nu, c->h_wm, c->kappa, c->Bc,
+                                             c->stream);
+        else
+            launch_equilibrium_alpha<float>((float*)s[0], c->alpha, c->md, c->d, c->nu, c->h_wm, c->kappa, c->Bc,
+                                            c->stream);
+    } else if (c->wall_mode == CHAN_WALL_MLP) {
+        const float *np = nullptr, *nn = nullptr;
+        int ndev = 0;
+        if (!mlp_device_params(c->net, &np, &nn, &ndev)) return fail(c, CHAN_ESTATE, "no wall network set");
+        if (c->prec == CHAN_FP64)
+            launch_wm_mlp<double>(np, nn, (double*)s[0], (double*)s[2], c->alpha, c->sc, c->md, c->d, c->nu,
+                                  c->h_wm, c->kappa, c->Bc, c->x_mass_cte, c->wm_max_err, c->wm_stats, c->stream);
+        else
+            launch_wm_mlp<float>(np, nn, (float*)s[0], (float*)s[2], c->alpha, c->sc, c->md, c->d, c->nu, c->h_wm,
+                                 c->kappa, c->Bc, c->x_mass_cte, c->wm_max_err, c->wm_stats, c->stream);
+    }
+    return check_launch(c);
+}
+
+int chan_set_wall_network(chan_ctx* c, chan_mlp* net, double max_rel_err)
+{
+    if (!c) return CHAN_EINVAL;
+    if (!(max_rel_err >= 0.0)) return fail(c, CHAN_EINVAL, "max_rel_err must be >= 0");
+    if (net) {
+        const float *p, *n;
+        int dev = -1;
+        if (!mlp_device_params(net, &p, &n, &dev)) return fail(c, CHAN_EINVAL, "invalid network");
+        if (dev != c->device) return fail(c, CHAN_EINVAL, "the network lives on another device");
+    }
+    if (!net && c->wall_mode == CHAN_WALL_MLP) return fail(c, CHAN_ESTATE, "the MLP wall model is in use");
+    c->net = net;
+    c->wm_max_err = max_rel_err;
+    drop_graphs(c);
+    return CHAN_OK;
+}
+
+int chan_wall_model_stats(chan_ctx* c, double out2[2])
+{
+    if (!c || !out2) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    const long g = wm_mlp_groups(c->d);
+    std::vector<double> h((size_t)2 * g);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->wm_stats, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double e = 0.0, f = 0.0;
+    for (long q = 0; q < g; q++) { e += h[q]; f += h[g + q]; }
+    const double ncell = 2.0 * c->d.nx * c->d.nzg;
+    out2[0] = e / ncell;
+    out2[1] = f / ncell;
+    return CHAN_OK;
+}
 
 int chan_tile_shape_count(int kernel)
 {

@@ -2,6 +2,8 @@
 #pragma once
 #include "common.hpp"
 
+struct chan_mlp;
+
 namespace chan {
 
 // field kinds for the ghost fill
@@ -105,6 +107,20 @@ void launch_poisson_solve(const Dims& d, PoissonBufs& pb, hipStream_t s);
 void launch_poisson_inverse(const Dims& d, PoissonBufs& pb, hipStream_t s);
 // forward x pass of a given centre field (for the standalone Poisson solve)
 void launch_poisson_forward_field(const double* rhs, const Dims& d, PoissonBufs& pb, hipStream_t s);
+
+// the learned wall model (mlp.hip); mlp_device_params gives a network's
+// device parameters, standardisation constants and device
+bool mlp_device_params(const ::chan_mlp* net, const float** p, const float** norm, int* device);
+void launch_mlp_infer(const float* p, const float* norm, const float* X, float* Y, int n, hipStream_t s);
+template <typename T>
+void launch_wm_samples(const T* U, const T* W, const double* alpha, const Scalars* sc, const Metric<double>& m,
+                       const Dims& d, double nu, double h_wm, double kappa, double B, int x_mass_cte, float* X, float* Y,
+                       hipStream_t s);
+long wm_mlp_groups(const Dims& d);
+template <typename T>
+void launch_wm_mlp(const float* p, const float* norm, const T* U, const T* W, double* alpha, const Scalars* sc,
+                   const Metric<double>& m, const Dims& d, double nu, double h_wm, double kappa, double B,
+                   int x_mass_cte, double max_err, double* stats, hipStream_t s);
 
 template <typename T>
 void launch_project(T* U, T* V, T* W, const double* p, const Metric<double>& md, const Dims& d, hipStream_t s);

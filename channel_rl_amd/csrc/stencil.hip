@@ -7,6 +7,7 @@
 // in fixed order) and therefore deterministic from run to run.
 #include "kernels.hpp"
 #include "prof.hpp"
+#include "wallmodel.hpp"
 
 namespace chan {
 
@@ -186,28 +187,6 @@ void launch_wall_model(const T* U, T* nut, const double* alpha_x, const Metric<T
 
 // ------------------------------------------------ equilibrium wall stress
 
-// Log law U+ = ln(y+)/kappa + B above the crossover with U+ = y+, linear
-// below; Newton on f(ut) = ut (ln(h ut/nu)/kappa + B) - |u|.
-// yc is the crossover y+ where ln(y+)/kappa + B = y+ (computed once on the host).
-__device__ double loglaw_utau(double u, double h, double nu, double kappa, double B, double yc)
-{
-    double au = fabs(u);
-    if (au == 0.0) return 0.0;
-    double ut = sqrt(nu * au / h);
-    if (h * ut / nu <= yc) return ut;
-    double g = ut;
-    for (int it = 0; it < 60; it++) {
-        double lg = log(h * g / nu);
-        double f = g * (lg / kappa + B) - au;
-        double fp = lg / kappa + B + 1.0 / kappa;
-        double gn = g - f / fp;
-        gn = gn > 0.5 * g ? gn : 0.5 * g;
-        if (fabs(gn - g) <= 1e-15 * gn) { g = gn; break; }
-        g = gn;
-    }
-    return g;
-}
-
 // one thread per (x face, z centre, wall)
 template <typename T>
 __global__ void k_equilibrium_alpha(const T* U, double* alpha, Metric<double> m, Dims d, double nu, double h_wm,
@@ -237,13 +216,6 @@ __global__ void k_equilibrium_alpha(const T* U, double* alpha, Metric<double> m,
     }
     const double t = loglaw_utau(uw, h, nu, kappa, B, yc);
     alpha[(i - 1) + (long)d.nx * (wall + 2L * (k - 1))] = (uw >= 0 ? 1.0 : -1.0) * t * t;
-}
-
-static double loglaw_crossover(double kappa, double B)
-{
-    double yc = 11.0;
-    for (int it = 0; it < 50; it++) yc = yc - (yc - std::log(yc) / kappa - B) / (1.0 - 1.0 / (kappa * yc));
-    return yc;
 }
 
 template <typename T>

@@ -1,0 +1,85 @@
+// wallmodel.hpp -- wall-stress closures shared by the equilibrium and the
+// learned (MLP) wall models: the log law and the sampling of the flow at the
+// matching height.
+#pragma once
+#include <cmath>
+#include "common.hpp"
+
+namespace chan {
+
+// Log law U+ = ln(y+)/kappa + B above the crossover with U+ = y+, linear
+// below; Newton on f(ut) = ut (ln(h ut/nu)/kappa + B) - |u|.
+// yc is the crossover y+ where ln(y+)/kappa + B = y+ (computed once on the host).
+__device__ inline double loglaw_utau(double u, double h, double nu, double kappa, double B, double yc)
+{
+    double au = fabs(u);
+    if (au == 0.0) return 0.0;
+    double ut = sqrt(nu * au / h);
+    if (h * ut / nu <= yc) return ut;
+    double g = ut;
+    for (int it = 0; it < 60; it++) {
+        double lg = log(h * g / nu);
+        double f = g * (lg / kappa + B) - au;
+        double fp = lg / kappa + B + 1.0 / kappa;
+        double gn = g - f / fp;
+        gn = gn > 0.5 * g ? gn : 0.5 * g;
+        if (fabs(gn - g) <= 1e-15 * gn) { g = gn; break; }
+        g = gn;
+    }
+    return g;
+}
+
+inline double loglaw_crossover(double kappa, double B)
+{
+    double yc = 11.0;
+    for (int it = 0; it < 50; it++) yc = yc - (yc - std::log(yc) / kappa - B) / (1.0 - 1.0 / (kappa * yc));
+    return yc;
+}
+
+// The flow seen by a wall cell at the matching height.  Wall cells are the
+// entries of alpha_x (nx, 2, nzg): x face i, z centre k, wall 0 (bottom) or
+// 1 (top) (code/wallmodel.f90:23-46).  U lives there; W (x centres, z faces)
+// is averaged to that point.  Below/above the height the two bracketing
+// centre rows are interpolated linearly; h_wm <= 0 takes the first off-wall
+// centre (the reference's wall-model point).
+struct WallSample {
+    double u, w;      // wall-parallel velocity components at the height
+    double dudy;      // d|u_par|/dn between the bracketing rows, n away from the wall
+    double h;         // the height
+};
+
+template <typename T>
+__device__ inline WallSample wall_sample(const T* U, const T* W, const Metric<double>& m, const Dims& d, int i, int k,
+                                         int wall, double h_wm)
+{
+    const double *y = m.y, *yg = m.yg;
+    int jb = 2;
+    double wb = 1.0, hb = yg[2] - y[1];
+    if (h_wm > 0.0) {
+        const double yt = y[1] + h_wm;
+        while (jb < d.nyg - 2 && yg[jb + 1] < yt) jb++;
+        wb = (yg[jb + 1] - yt) / (yg[jb + 1] - yg[jb]);
+        hb = h_wm;
+    }
+    // z centre k as its canonical image for the W average over faces k-1, k
+    const int kc = k == 1 ? d.nzg - 2 : (k == d.nzg ? 3 : k);
+    auto uw_at = [&](int j, double& u, double& w) {
+        u = (double)U[at(d, i, j, k)];
+        w = 0.25 * ((double)W[at(d, i, j, kc - 1)] + (double)W[at(d, i, j, kc)] + (double)W[at(d, i + 1, j, kc - 1)] +
+                    (double)W[at(d, i + 1, j, kc)]);
+    };
+    const int ja = wall == 0 ? jb : d.nyg + 1 - jb;       // nearer row
+    const int jn = wall == 0 ? jb + 1 : d.nyg - jb;       // farther row
+    double ua, wa, un, wn;
+    uw_at(ja, ua, wa);
+    uw_at(jn, un, wn);
+    WallSample s;
+    s.u = wb * ua + (1.0 - wb) * un;
+    s.w = wb * wa + (1.0 - wb) * wn;
+    const double dn = wall == 0 ? yg[jb + 1] - yg[jb] : yg[ja] - yg[jn];
+    s.dudy = (sqrt(un * un + wn * wn) - sqrt(ua * ua + wa * wa)) / dn;
+    s.h = (h_wm > 0.0 || wall == 0) ? hb : y[d.ny] - yg[d.nyg - 1];
+    return s;
+}
+
+}  // namespace chan

@@ -25,6 +25,7 @@ extern "C" {
 #endif
 
 typedef struct chan_ctx chan_ctx;
+typedef struct chan_mlp chan_mlp;
 
 enum { CHAN_OK = 0, CHAN_EINVAL = -1, CHAN_EHIP = -2, CHAN_ESTATE = -3, CHAN_ENOMEM = -4 };
 
@@ -38,7 +39,7 @@ enum {
 };
 
 /* wall-stress source for the wall model (the reference's alpha_x hook) */
-enum { CHAN_WALL_GIVEN = 0, CHAN_WALL_EQUILIBRIUM = 1 };
+enum { CHAN_WALL_GIVEN = 0, CHAN_WALL_EQUILIBRIUM = 1, CHAN_WALL_MLP = 2 };
 
 /* The reference's input-file parameters (code/input_output.f90:20-91) plus
  * the grid (faces; code/input_output.f90:108-124 / the .init mesh). */
@@ -127,6 +128,60 @@ int chan_profile_entry(const chan_ctx* ctx, int idx, char* name, int len, double
  * timed over reps launches with HIP events (achievable HBM bandwidth, and
  * the reference point that scales the hardware traffic counters). */
 int chan_stream_copy(chan_ctx* ctx, long nbytes, int reps, double* ms_per_copy);
+
+/* ---------------------------------------------------------------------
+ * The learned wall-stress closure.  No reference counterpart: it produces
+ * the alpha_x the reference receives from its RL agent
+ * (code/smarties_stat.f90:231-258).  A 4 -> 64 -> 64 -> 64 -> 2 network with
+ * SiLU activations, BF16 weights and FP32 accumulation, evaluated and trained
+ * by the library's own kernels (matrix-core hidden layers, Adam).
+ *
+ * Per wall cell (x face i, z centre k, wall; the entries of alpha_x) the
+ * features are, before standardisation,
+ *   f0 = asinh(h|u_par|/nu/10), f1 = asinh(h^2 d|u_par|/dn/nu/10),
+ *   f2 = asinh(y+_prev/10) with y+_prev = h sqrt(|alpha_x|)/nu (the previous
+ *   wall stress), f3 = asinh(h^3 dpdx/nu^2/10),
+ * and the outputs o0 = log(1 + h^2 tau_par/nu^2), o1 = h^2 tau_perp/nu^2 /
+ * exp(o0): the stress along and across u_par, rotated back to
+ * (tau_wx, tau_wz).  The equilibrium log law (tau_par = u_tau^2, tau_perp = 0)
+ * is the training target, the judge of the network's error and the fallback.
+ *
+ * Parameters: 8770 FP32 values, W1[4][64] b1[64] W2[64][64] b2[64]
+ * W3[64][64] b3[64] W4[64][2] b4[2] (W[k][n], input k, output n); and 12
+ * standardisation constants: feature means and scales (4 + 4), output means
+ * and scales (2 + 2).  Features/outputs passed to the calls below are raw
+ * (the network standardises them).  X is n x 4, Y is n x 2, row-major. */
+int  chan_mlp_create(int device, unsigned long long seed, chan_mlp** out);
+void chan_mlp_destroy(chan_mlp* net);
+const char* chan_mlp_last_error(const chan_mlp* net);
+int chan_mlp_get_params(chan_mlp* net, float* params /* 8770 or NULL */, float* norm /* 12 or NULL */);
+int chan_mlp_set_params(chan_mlp* net, const float* params, const float* norm);
+/* standardisation constants from data: means and standard deviations */
+int chan_mlp_fit_normalization(chan_mlp* net, const float* X, const float* Y, int n);
+int chan_mlp_forward(chan_mlp* net, const float* X, int n, float* Y);
+/* gradient of L = 1/(2n) sum (y - t)^2 over standardised outputs, no update */
+int chan_mlp_gradients(chan_mlp* net, const float* X, const float* Y, int n, float* grad, double* loss);
+/* Adam (lr, beta 0.9/0.999, eps 1e-8), shuffled minibatches (batch a
+ * multiple of 32); epoch_loss (epochs values or NULL) gets each epoch's
+ * mean loss during training */
+int chan_mlp_train(chan_mlp* net, const float* X, const float* Y, int n, int epochs, int batch, float lr,
+                   unsigned long long seed, double* epoch_loss);
+
+/* Samples of every wall cell of the channel's current state at height h_wm
+ * (<= 0: first off-wall cell) with viscosity nu: features X (2*nx*nzg x 4)
+ * and equilibrium targets Y (2*nx*nzg x 2), in alpha_x order. */
+int chan_wall_samples(chan_ctx* ctx, double h_wm, double nu, double kappa, double B, float* X, float* Y);
+/* the network used by CHAN_WALL_MLP (not owned; same device as ctx; do not
+ * train it while a step that uses it runs) and the largest relative deviation
+ * |(tau_par, tau_perp) - (u_tau^2, 0)| / u_tau^2 from the equilibrium stress
+ * it may have before a cell falls back to the equilibrium value */
+int chan_set_wall_network(chan_ctx* ctx, chan_mlp* net, double max_rel_err);
+/* evaluate the selected wall-stress source (equilibrium or network) once on
+ * the current state into alpha_x (CHAN_WALL_GIVEN: nothing to do) */
+int chan_evaluate_wall_stress(chan_ctx* ctx);
+/* of the last CHAN_WALL_MLP evaluation: mean relative deviation from the
+ * equilibrium stress over the wall cells, fraction of cells that fell back */
+int chan_wall_model_stats(chan_ctx* ctx, double out2[2]);
 
 /* Tile shapes and the autotuner.  No reference counterpart (the reference's
  * loops have no tiles).  CHAN_KERNEL_RHS selects the fused RHS + RK kernel,

@@ -1,0 +1,175 @@
+"""The learned wall-stress closure on the GPU, against the CPU oracle
+(oracle/mlp.py) and the equilibrium log law.
+
+Tolerances: the GPU accumulates in FP32 where the oracle uses FP64 with the
+same BF16 rounding points, so a BF16 rounding of an activation can land on
+the other side of a tie; forward outputs agree to 2e-3 (absolute, on
+standardised outputs of order 1), gradients to 2e-2 relative (per parameter
+block, in norm), one Adam step to 1e-6 absolute on the parameters (the step is
+lr * m_hat / sqrt(v_hat) ~ lr, so this is 1e-3 of the step).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import mlp as om
+from channel_rl_amd import FP64, WALL_EQUILIBRIUM, WALL_MLP, Channel, WallNetwork, cases, wallnet
+from channel_rl_amd import ALPHA_X
+from helpers import projected_state
+
+pytestmark = pytest.mark.gpu
+
+BLOCKS = [("W1", om.W1, om.B1), ("b1", om.B1, om.W2), ("W2", om.W2, om.B2), ("b2", om.B2, om.W3),
+          ("W3", om.W3, om.B3), ("b3", om.B3, om.W4), ("W4", om.W4, om.B4), ("b4", om.B4, om.NPARAM)]
+
+
+def _net_and_data(seed, n):
+    net = WallNetwork(seed=seed)
+    rng = np.random.default_rng(seed)
+    p, _ = net.params()
+    p = p.astype(np.float64)
+    p[om.B1:om.W2] = rng.normal(0, 0.1, 64)
+    p[om.B2:om.W3] = rng.normal(0, 0.1, 64)
+    p[om.B3:om.W4] = rng.normal(0, 0.1, 64)
+    p[om.B4:] = rng.normal(0, 0.1, 2)
+    norm = np.array([0.1, -0.2, 0.3, 0.0, 1.5, 0.7, 1.0, 2.0, 0.2, -0.1, 1.3, 0.8], np.float32)
+    net.set_params(p.astype(np.float32), norm)
+    X = rng.normal(0, 1, (n, 4)).astype(np.float32) * [1.5, 0.7, 1.0, 2.0] + [0.1, -0.2, 0.3, 0.0]
+    T = rng.normal(0, 1, (n, 2)).astype(np.float32)
+    return net, p.astype(np.float32).astype(np.float64), norm.astype(np.float64), X.astype(np.float32), T
+
+
+def test_forward_matches_oracle(gpu):
+    net, p, norm, X, _ = _net_and_data(1, 1000)
+    got = net.forward(X).astype(np.float64)
+    ref = om.forward(p, norm, X)
+    err = np.abs((got - ref) / norm[10:12])
+    assert err.max() <= 2e-3, err.max()
+
+
+def test_gradients_match_oracle(gpu):
+    net, p, norm, X, T = _net_and_data(2, 333)   # not a multiple of 32: padded batch
+    g, loss = net.gradients(X, T)
+    gr, lr = om.gradients(p, norm, X, T)
+    assert abs(loss - lr) <= 1e-3 * lr
+    for name, a, b in BLOCKS:
+        e = np.linalg.norm(g[a:b] - gr[a:b]) / np.linalg.norm(gr[a:b])
+        assert e <= 2e-2, (name, e)
+
+
+def test_one_adam_step_matches_oracle(gpu):
+    net, p, norm, X, T = _net_and_data(3, 256)
+    net.train(X, T, epochs=1, batch=256, lr=1e-3, seed=0)
+    got, _ = net.params()
+    gr, _ = om.gradients(p, norm, X, T)
+    ref, _, _ = om.adam_step(p, np.zeros_like(p), np.zeros_like(p), gr, 1, lr=1e-3)
+    # entries whose gradient is tiny can flip the sign of m_hat / sqrt(v_hat)
+    big = np.abs(gr) > 1e-3 * np.abs(gr).max()
+    assert np.max(np.abs(got[big] - ref[big])) <= 1e-6 + 1e-3 * 1e-3
+
+
+def _grid1(case):
+    """1-based y faces and y centres (element 0 unused), as the kernels index them."""
+    o = oracle.Oracle(case.x, case.y, case.z, **case.kwargs())
+    return np.concatenate([[0.0], case.y]), np.concatenate([[0.0], o.grid()["yg"]])
+
+
+def test_wall_samples_match_oracle(gpu):
+    case, field = cases.small_case(18, 17, 18, seed=9, x_mass_cte=0)
+    U, V, W = projected_state(case, field)
+    g = Channel(case.x, case.y, case.z, precision=FP64, **case.kwargs())
+    g.set_state(U, V, W)
+    rng = np.random.default_rng(5)
+    alpha = np.asfortranarray(rng.uniform(0.0, 2.0, (case.nx, 2, case.nzg)))
+    g.set_field(ALPHA_X, alpha)
+    g.set_scalars(dt=1e-3, t=0.0, dPdx=0.37, Qflow_x_0=0.0)
+    y, yg = _grid1(case)
+    for h in (0.0, 0.13):
+        X, Y = g.wall_samples(h_wm=h, nu=case.nu)
+        Xr, Yr = om.wall_samples(U, W, alpha, y, yg, case.nu, h, 0.41, 5.2, 0.37, oracle.loglaw_utau)
+        assert np.max(np.abs(X - Xr)) <= 1e-5 * max(1.0, np.abs(Xr).max())
+        assert np.max(np.abs(Y - Yr)) <= 1e-5 * max(1.0, np.abs(Yr).max())
+
+
+@pytest.fixture(scope="module")
+def trained():
+    X, Y = wallnet.solver_samples(400_000, seed=7, develop_steps=20)
+    Xv, Yv = wallnet.solver_samples(20_000, seed=8, develop_steps=80)
+    net, losses = wallnet.train(X, Y, epochs=10, batch=1024, lr=1e-3, seed=0)
+    return net, losses, Xv, Yv
+
+
+def test_training_gains_over_untrained_network(gpu, trained):
+    net, losses, Xv, Yv = trained
+    assert np.all(np.isfinite(losses)) and losses[-1] < 0.05 * losses[0], losses
+    err = wallnet.stress_error(net, Xv, Yv)
+    # untrained network with the same standardisation
+    raw = WallNetwork(seed=0)
+    raw.set_params(None, net.params()[1])
+    err0 = wallnet.stress_error(raw, Xv, Yv)
+    assert np.mean(err) <= 0.02, np.mean(err)
+    assert np.mean(err) < 0.1 * np.mean(err0), (np.mean(err), np.mean(err0))
+
+
+def _small_channel(seed, mode, net=None, steps=0):
+    case, field = cases.config("small", seed=seed)
+    g = Channel(case.x, case.y, case.z, precision=FP64, **case.kwargs())
+    g.set_state(*field)
+    g.apply_boundary_conditions()
+    g.compute_projection_step()
+    g.initialize_mass_flow()
+    g.set_scalars(dt=0.0, t=0.0, dPdx=case.dPdx, Qflow_x_0=g.scalars()["Qflow_x_0"])
+    if net is not None:
+        g.set_wall_network(net, max_rel_err=0.05)
+    g.set_wall_model(mode, h_wm=0.0)
+    g.run_steps(steps)
+    return g
+
+
+def test_mlp_wall_model_in_the_step(gpu, trained):
+    net = trained[0]
+    # warm start: the developed flow of the equilibrium run, then the network
+    g = _small_channel(3, WALL_EQUILIBRIUM, net, steps=60)
+    ref = _small_channel(3, WALL_EQUILIBRIUM, steps=60)
+    g.set_wall_model(WALL_MLP, h_wm=0.0)
+    g.run_steps(20)
+    ref.run_steps(20)
+    err, fb = g.wall_model_stats()
+    assert err <= 0.05 and fb <= 0.2, (err, fb)
+    a_ml, a_eq = g.get_field(ALPHA_X), ref.get_field(ALPHA_X)
+    assert np.all(np.isfinite(a_ml))
+    assert np.mean(np.abs(a_ml - a_eq)) <= 0.05 * np.mean(np.abs(a_eq))
+    # cold start from the seeded field: whatever the network does in the
+    # transient, the applied stress stays within the fallback bound
+    g = _small_channel(3, WALL_MLP, net, steps=20)
+    ref = _small_channel(3, WALL_EQUILIBRIUM, steps=20)
+    a_ml, a_eq = g.get_field(ALPHA_X), ref.get_field(ALPHA_X)
+    assert np.all(np.isfinite(a_ml))
+    assert np.mean(np.abs(a_ml - a_eq)) <= 0.1 * np.mean(np.abs(a_eq))
+
+
+def test_fallback_gives_the_equilibrium_stress(gpu):
+    # a network that is far off everywhere falls back on every cell, and the
+    # wall stress is then the 2-D equilibrium value u_tau^2 u/|u_par|
+    case, field = cases.small_case(18, 17, 18, seed=4)
+    U, V, W = projected_state(case, field)
+    net = WallNetwork(seed=1)
+    p, norm = net.params()
+    p[:] = 0.0
+    p[om.B4] = -30.0  # o0 far below any target
+    net.set_params(p, norm)
+    g = Channel(case.x, case.y, case.z, precision=FP64, **case.kwargs())
+    g.set_state(U, V, W)
+    g.set_wall_network(net, max_rel_err=0.05)
+    for h in (0.0, 0.2):
+        g.set_field(ALPHA_X, np.zeros((case.nx, 2, case.nzg), order="F"))
+        g.set_wall_model(WALL_MLP, h_wm=h)
+        g.evaluate_wall_stress()
+        err, fb = g.wall_model_stats()
+        assert fb == 1.0 and err > 0.05
+        y, yg = _grid1(case)
+        _, _, F = om.wall_samples(U, W, np.zeros((case.nx, 2, case.nzg)), y, yg, case.nu, h, 0.41, 5.2, 0.0,
+                                  oracle.loglaw_utau, flow=True)
+        ref = om.equilibrium_alpha_2d(F, case.nu, 0.41, 5.2, oracle.loglaw_utau)
+        got = g.get_field(ALPHA_X).ravel(order="F")
+        assert np.max(np.abs(got - ref)) <= 1e-12 * np.abs(ref).max()
