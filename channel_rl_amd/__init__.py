@@ -34,6 +34,8 @@ EXPORTS = [
     "chan_autotune", "chan_mlp_create", "chan_mlp_destroy", "chan_mlp_last_error", "chan_mlp_get_params",
     "chan_mlp_set_params", "chan_mlp_fit_normalization", "chan_mlp_forward", "chan_mlp_gradients",
     "chan_mlp_train", "chan_wall_samples", "chan_set_wall_network", "chan_wall_model_stats", "chan_evaluate_wall_stress",
+    "chan_ensemble_create", "chan_ensemble_destroy", "chan_ensemble_last_error", "chan_ensemble_size",
+    "chan_ensemble_run_steps", "chan_ensemble_synchronize",
 ]
 KERNEL_RHS = 0
 WALL_MLP = 2
@@ -109,10 +111,19 @@ def load_library(path=LIB_PATH):
     L.chan_mlp_gradients.argtypes = [vp, _F, _F, ctypes.c_int, _F, _D]
     L.chan_mlp_train.argtypes = [vp, _F, _F, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                  ctypes.c_ulonglong, _D]
-    L.chan_wall_samples.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _F, _F]
+    L.chan_wall_samples.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _F, _F,
+                                    _D]
     L.chan_set_wall_network.argtypes = [vp, vp, ctypes.c_double]
     L.chan_wall_model_stats.argtypes = [vp, _D]
     L.chan_evaluate_wall_stress.argtypes = [vp]
+    L.chan_ensemble_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
+    L.chan_ensemble_destroy.argtypes = [vp]
+    L.chan_ensemble_destroy.restype = None
+    L.chan_ensemble_last_error.argtypes = [vp]
+    L.chan_ensemble_last_error.restype = ctypes.c_char_p
+    L.chan_ensemble_size.argtypes = [vp]
+    L.chan_ensemble_run_steps.argtypes = [vp, ctypes.c_int]
+    L.chan_ensemble_synchronize.argtypes = [vp]
     _LIB = L
     return L
 
@@ -130,6 +141,44 @@ def _f32(a, cols):
     if a.ndim != 2 or a.shape[1] != cols:
         raise ValueError(f"expected an (n, {cols}) array, got {a.shape}")
     return a
+
+
+class Ensemble:
+    """Channels advanced together, one graph launch per ensemble step."""
+
+    def __init__(self, channels):
+        L = load_library()
+        self.channels = list(channels)  # the members must outlive the ensemble
+        arr = (ctypes.c_void_p * len(self.channels))(*[c._h.value for c in self.channels])
+        h = ctypes.c_void_p()
+        rc = L.chan_ensemble_create(arr, len(self.channels), ctypes.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = L.chan_ensemble_last_error(h).decode() if h.value else "chan_ensemble_create failed"
+            L.chan_ensemble_destroy(h)
+            self._h = None
+            raise ChanError(f"chan_ensemble_create: {msg} (status {rc})")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().chan_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise ChanError(f"{what}: {load_library().chan_ensemble_last_error(self._h).decode()} (status {rc})")
+
+    def __len__(self):
+        return load_library().chan_ensemble_size(self._h)
+
+    def run_steps(self, n):
+        self._chk(load_library().chan_ensemble_run_steps(self._h, int(n)), "run_steps")
+
+    def synchronize(self):
+        self._chk(load_library().chan_ensemble_synchronize(self._h), "synchronize")
 
 
 class WallNetwork:
@@ -352,15 +401,18 @@ class Channel:
     def synchronize(self):
         self._chk(load_library().chan_synchronize(self._h), "synchronize")
 
-    def wall_samples(self, h_wm=0.0, nu=None, kappa=0.41, B=5.2):
+    def wall_samples(self, h_wm=0.0, nu=None, kappa=0.41, B=5.2, stress=False):
         """Features (2*nx*nzg, 4) and equilibrium targets (2*nx*nzg, 2) of every
-        wall cell at height h_wm, with viscosity nu (default: the channel's)."""
+        wall cell at height h_wm, with viscosity nu (default: the channel's);
+        stress=True also returns the equilibrium stress u_tau^2 per cell."""
         n = 2 * self.nx * self.nzg
         X = np.zeros((n, 4), np.float32)
         Y = np.zeros((n, 2), np.float32)
+        T = np.zeros(n)
         nu = self.nu if nu is None else nu
-        self._chk(load_library().chan_wall_samples(self._h, h_wm, nu, kappa, B, _pf(X), _pf(Y)), "wall_samples")
-        return X, Y
+        self._chk(load_library().chan_wall_samples(self._h, h_wm, nu, kappa, B, _pf(X), _pf(Y), _p(T)),
+                  "wall_samples")
+        return (X, Y, T) if stress else (X, Y)
 
     def set_wall_network(self, net, max_rel_err=0.05):
         self._net = net  # keep it alive while attached

@@ -146,6 +146,30 @@ def config(name, seed=1234):
     raise KeyError(name)
 
 
+ENSEMBLE_GRIDS = ((34, 33, 34), (66, 49, 66), (98, 65, 98))
+
+
+def ensemble(n=64, seed=42, field_seed=1234):
+    """The batched ensemble: n channels, 2pi x 2 x pi, grids (faces) drawn
+    uniformly from ENSEMBLE_GRIDS and Re_tau log-uniform in [180, 5200] with
+    the seeded generator `seed`; nu = 1/Re_tau (u_tau = 1 units), constant
+    mass flow, DSM, wall model at h_wm = 0.1, initial field log law + 10%
+    noise (noise seed field_seed + channel index).  Returns [(Case, field)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(n):
+        nx, ny, nz = ENSEMBLE_GRIDS[int(rng.integers(len(ENSEMBLE_GRIDS)))]
+        re_tau = float(np.exp(rng.uniform(np.log(180.0), np.log(5200.0))))
+        nu = 1.0 / re_tau
+        c = Case(f"ensemble{q}", nx, ny, nz, 2 * math.pi, math.pi, tanh_y(ny, 1.5), nu=nu, dPdx=1.0, CFL=0.5,
+                 h_wm=0.1, steps=1000)
+        c.re_tau = re_tau
+        ub = bulk_velocity_plus(re_tau)
+        f = c.initial_field(lambda d, nu=nu: loglaw_plus(d / nu), ub, 0.10, field_seed + q)
+        out.append((c, f))
+    return out
+
+
 def small_case(nx, ny, nz, seed=7, stretched=True, LES_model=2, x_mass_cte=1, CFL=0.5, amp=0.1):
     """A test-sized channel with the small configuration's physics."""
     re_tau = 180.0

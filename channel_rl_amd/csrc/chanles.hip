@@ -112,6 +112,7 @@ struct chan_ctx {
     double wm_max_err = 0.05;
     double* wm_stats = nullptr;   // per 16-cell group: error sums, fallbacks
     float* wm_samples = nullptr;  // 2*nx*nzg x (4 + 2) floats, on demand
+    long generation = 0;          // bumped whenever captured steps go stale
     Prof prof;
     std::vector<std::pair<std::string, std::pair<double, long>>> prof_out;
 };
@@ -333,6 +334,7 @@ void drop_graphs(chan_ctx* c)
 {
     for (int q = 0; q < 3; q++)
         if (c->graph[q]) { (void)hipGraphExecDestroy(c->graph[q]); c->graph[q] = nullptr; }
+    c->generation++;
 }
 
 int check_launch(chan_ctx* c)
@@ -902,26 +904,29 @@ int chan_profile_steps(chan_ctx* c, int nsteps)
 
 int chan_profile_count(const chan_ctx* c) { return c ? (int)c->prof_out.size() : 0; }
 
-int chan_wall_samples(chan_ctx* c, double h_wm, double nu, double kappa, double B, float* X, float* Y)
+int chan_wall_samples(chan_ctx* c, double h_wm, double nu, double kappa, double B, float* X, float* Y, double* tau)
 {
     if (!c || !X || !Y) return CHAN_EINVAL;
     if (!(nu > 0.0) || !(kappa > 0.0)) return fail(c, CHAN_EINVAL, "wall samples: nu and kappa must be positive");
     int rc = set_device(c);
     if (rc) return rc;
     const long ncell = 2L * c->d.nx * c->d.nzg;
-    if (!c->wm_samples && (rc = dalloc(c, (void**)&c->wm_samples, (size_t)ncell * 6 * sizeof(float)))) return rc;
+    // features (4), targets (2) as floats, then the stress as doubles
+    if (!c->wm_samples && (rc = dalloc(c, (void**)&c->wm_samples, (size_t)ncell * 8 * sizeof(float)))) return rc;
     float* dX = c->wm_samples;
     float* dY = c->wm_samples + ncell * 4;
+    double* dT = (double*)(c->wm_samples + ncell * 6);
     void** s = c->state[c->cur];
     if (c->prec == CHAN_FP64)
         launch_wm_samples<double>((double*)s[0], (double*)s[2], c->alpha, c->sc, c->md, c->d, nu, h_wm, kappa, B,
-                                  c->x_mass_cte, dX, dY, c->stream);
+                                  c->x_mass_cte, dX, dY, dT, c->stream);
     else
         launch_wm_samples<float>((float*)s[0], (float*)s[2], c->alpha, c->sc, c->md, c->d, nu, h_wm, kappa, B,
-                                 c->x_mass_cte, dX, dY, c->stream);
+                                 c->x_mass_cte, dX, dY, dT, c->stream);
     if ((rc = check_launch(c))) return rc;
     HIPCHK(c, hipMemcpyAsync(X, dX, (size_t)ncell * 4 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(Y, dY, (size_t)ncell * 2 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (tau) HIPCHK(c, hipMemcpyAsync(tau, dT, (size_t)ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return CHAN_OK;
 }
@@ -1119,6 +1124,152 @@ int chan_synchronize(chan_ctx* c)
     int rc = set_device(c);
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHAN_OK;
+}
+
+
+// ------------------------------------------------------------------ ensemble
+// Many independent channels (any sizes and precisions, one device) advanced
+// together: one step of every member is captured into one graph, each member
+// on its own stream between a fork and a join, so the members' kernels are
+// independent graph branches and one graph launch runs the whole ensemble
+// step.  Three graphs cover the members' state rotation.
+struct chan_ensemble {
+    std::string err;
+    std::vector<chan_ctx*> ch;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr, done = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+    std::vector<int> cur0;    // members' rotation when the graphs were captured
+    std::vector<long> gen0;   // members' generation then
+    long step = 0;            // steps since the capture base
+};
+
+static int efail(chan_ensemble* e, int code, const std::string& msg)
+{
+    if (e) e->err = msg;
+    return code;
+}
+
+static void ens_drop(chan_ensemble* e)
+{
+    for (int q = 0; q < 3; q++)
+        if (e->graph[q]) { (void)hipGraphExecDestroy(e->graph[q]); e->graph[q] = nullptr; }
+    e->cur0.clear();
+    e->gen0.clear();
+    e->step = 0;
+}
+
+int chan_ensemble_create(chan_ctx* const* members, int n, chan_ensemble** out)
+{
+    if (!out) return CHAN_EINVAL;
+    *out = nullptr;
+    if (!members || n < 1) return CHAN_EINVAL;
+    chan_ensemble* e = new chan_ensemble();
+    *out = e;
+    for (int q = 0; q < n; q++) {
+        if (!members[q]) return efail(e, CHAN_EINVAL, "null member");
+        for (int r = 0; r < q; r++)
+            if (members[r] == members[q]) return efail(e, CHAN_EINVAL, "a channel appears twice");
+        if (members[q]->device != members[0]->device) return efail(e, CHAN_EINVAL, "members on different devices");
+        e->ch.push_back(members[q]);
+    }
+    e->device = members[0]->device;
+    hipError_t h = hipSetDevice(e->device);
+    if (h == hipSuccess) h = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (h == hipSuccess) h = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
+    if (h == hipSuccess) h = hipEventCreateWithFlags(&e->done, hipEventDisableTiming);
+    e->join.assign(n, nullptr);
+    for (int q = 0; q < n && h == hipSuccess; q++) h = hipEventCreateWithFlags(&e->join[q], hipEventDisableTiming);
+    if (h != hipSuccess) return efail(e, CHAN_EHIP, std::string("ensemble set-up: ") + hipGetErrorString(h));
+    return CHAN_OK;
+}
+
+void chan_ensemble_destroy(chan_ensemble* e)
+{
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    ens_drop(e);
+    for (hipEvent_t ev : e->join)
+        if (ev) (void)hipEventDestroy(ev);
+    if (e->fork) (void)hipEventDestroy(e->fork);
+    if (e->done) (void)hipEventDestroy(e->done);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+const char* chan_ensemble_last_error(const chan_ensemble* e) { return e ? e->err.c_str() : "null ensemble"; }
+
+int chan_ensemble_size(const chan_ensemble* e) { return e ? (int)e->ch.size() : 0; }
+
+int chan_ensemble_run_steps(chan_ensemble* e, int nsteps)
+{
+    if (!e || nsteps < 0) return CHAN_EINVAL;
+    hipError_t h = hipSetDevice(e->device);
+    if (h != hipSuccess) return efail(e, CHAN_EHIP, hipGetErrorString(h));
+    const int n = (int)e->ch.size();
+    // the graphs are valid while every member's rotation and settings are
+    // where the capture left them
+    bool stale = e->cur0.empty();
+    for (int q = 0; q < n && !stale; q++)
+        stale = e->ch[q]->generation != e->gen0[q] || e->ch[q]->cur != (e->cur0[q] + e->step) % 3;
+    if (stale) {
+        ens_drop(e);
+        for (chan_ctx* c : e->ch) { e->cur0.push_back(c->cur); e->gen0.push_back(c->generation); }
+    }
+    // order after whatever the members' own streams hold
+    for (int q = 0; q < n; q++) {
+        h = hipEventRecord(e->join[q], e->ch[q]->stream);
+        if (h == hipSuccess) h = hipStreamWaitEvent(e->stream, e->join[q], 0);
+        if (h != hipSuccess) return efail(e, CHAN_EHIP, std::string("ensemble order: ") + hipGetErrorString(h));
+    }
+    for (int it = 0; it < nsteps; it++) {
+        const int o = (int)(e->step % 3);
+        if (!e->graph[o]) {
+            hipGraph_t g;
+            h = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+            if (h == hipSuccess) h = hipEventRecord(e->fork, e->stream);
+            for (int q = 0; q < n && h == hipSuccess; q++) {
+                chan_ctx* c = e->ch[q];
+                h = hipStreamWaitEvent(c->stream, e->fork, 0);
+                if (h != hipSuccess) break;
+                DISPATCH(c, enqueue_dt, c);
+                DISPATCH(c, enqueue_rk3, c, c->cur);
+                h = hipEventRecord(e->join[q], c->stream);
+                if (h == hipSuccess) h = hipStreamWaitEvent(e->stream, e->join[q], 0);
+            }
+            hipError_t h2 = hipStreamEndCapture(e->stream, &g);
+            if (h == hipSuccess) h = h2;
+            if (h == hipSuccess) {
+                h = hipGraphInstantiate(&e->graph[o], g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+            }
+            if (h != hipSuccess) {
+                ens_drop(e);
+                return efail(e, CHAN_EHIP, std::string("ensemble capture: ") + hipGetErrorString(h));
+            }
+        }
+        h = hipGraphLaunch(e->graph[o], e->stream);
+        if (h != hipSuccess) return efail(e, CHAN_EHIP, std::string("ensemble launch: ") + hipGetErrorString(h));
+        for (chan_ctx* c : e->ch) c->cur = (c->cur + 1) % 3;
+        e->step++;
+    }
+    // members' later work is ordered after the ensemble's steps
+    h = hipEventRecord(e->done, e->stream);
+    for (int q = 0; q < n && h == hipSuccess; q++) h = hipStreamWaitEvent(e->ch[q]->stream, e->done, 0);
+    if (h != hipSuccess) return efail(e, CHAN_EHIP, std::string("ensemble order: ") + hipGetErrorString(h));
+    return CHAN_OK;
+}
+
+int chan_ensemble_synchronize(chan_ensemble* e)
+{
+    if (!e) return CHAN_EINVAL;
+    hipError_t h = hipSetDevice(e->device);
+    if (h == hipSuccess) h = hipStreamSynchronize(e->stream);
+    if (h != hipSuccess) return efail(e, CHAN_EHIP, std::string("ensemble sync: ") + hipGetErrorString(h));
     return CHAN_OK;
 }
 

@@ -115,7 +115,7 @@ void launch_mlp_infer(const float* p, const float* norm, const float* X, float* 
 template <typename T>
 void launch_wm_samples(const T* U, const T* W, const double* alpha, const Scalars* sc, const Metric<double>& m,
                        const Dims& d, double nu, double h_wm, double kappa, double B, int x_mass_cte, float* X, float* Y,
-                       hipStream_t s);
+                       double* Tq, hipStream_t s);
 long wm_mlp_groups(const Dims& d);
 template <typename T>
 void launch_wm_mlp(const float* p, const float* norm, const T* U, const T* W, double* alpha, const Scalars* sc,

@@ -331,11 +331,11 @@ __device__ __forceinline__ double wm_driving_dpdx(const Scalars* sc, int x_mass_
 
 // Training/validation samples of every wall cell at height h_wm, viscosity
 // nu (features and the equilibrium targets o0, o1; the lagged stress is the
-// context's alpha_x).
+// context's alpha_x) and the equilibrium stress u_tau^2 itself (Tq).
 template <typename T>
 __global__ void k_wm_samples(const T* U, const T* W, const double* alpha, const Scalars* sc, Metric<double> m, Dims d,
                              double nu, double h_wm, double kappa, double B, double yc, int x_mass_cte, float* X,
-                             float* Y)
+                             float* Y, double* Tq)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int k = blockIdx.y + 1, wall = blockIdx.z;
@@ -348,6 +348,7 @@ __global__ void k_wm_samples(const T* U, const T* W, const double* alpha, const 
     for (int q = 0; q < 4; q++) X[c * 4 + q] = f[q];
     Y[c * 2 + 0] = (float)log1p(ws.h * ws.h * ut * ut / (nu * nu));
     Y[c * 2 + 1] = 0.0f;
+    Tq[c] = ut * ut;
 }
 
 // The MLP wall model of one substep: per wall cell, the network's tau_wx
@@ -426,11 +427,11 @@ void launch_mlp_infer(const float* p, const float* norm, const float* X, float* 
 template <typename T>
 void launch_wm_samples(const T* U, const T* W, const double* alpha, const Scalars* sc, const Metric<double>& m,
                        const Dims& d, double nu, double h_wm, double kappa, double B, int x_mass_cte, float* X, float* Y,
-                       hipStream_t s)
+                       double* Tq, hipStream_t s)
 {
     dim3 blk(64), grd(cdiv(d.nx, 64), d.nzg, 2);
     k_wm_samples<T><<<grd, blk, 0, s>>>(U, W, alpha, sc, m, d, nu, h_wm, kappa, B, loglaw_crossover(kappa, B),
-                                         x_mass_cte, X, Y);
+                                         x_mass_cte, X, Y, Tq);
 }
 
 long wm_mlp_groups(const Dims& d) { return (2L * d.nx * d.nzg + 15) / 16; }
@@ -450,10 +451,10 @@ void launch_wm_mlp(const float* p, const float* norm, const T* U, const T* W, do
 
 template void launch_wm_samples<double>(const double*, const double*, const double*, const Scalars*,
                                         const Metric<double>&, const Dims&, double, double, double, double, int,
-                                        float*, float*, hipStream_t);
+                                        float*, float*, double*, hipStream_t);
 template void launch_wm_samples<float>(const float*, const float*, const double*, const Scalars*,
                                        const Metric<double>&, const Dims&, double, double, double, double, int, float*,
-                                       float*, hipStream_t);
+                                       float*, double*, hipStream_t);
 template void launch_wm_mlp<double>(const float*, const float*, const double*, const double*, double*,
                                     const Scalars*, const Metric<double>&, const Dims&, double, double, double,
                                     double, int, double, double*, hipStream_t);

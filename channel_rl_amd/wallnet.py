@@ -14,14 +14,20 @@ the equilibrium log law's stress.
 """
 import numpy as np
 
-from . import Channel, WallNetwork, WALL_EQUILIBRIUM, cases
+from . import ALPHA_X, Channel, WallNetwork, WALL_EQUILIBRIUM, cases
 
 HEIGHTS = (0.0, 0.05, 0.1, 0.15, 0.2, 0.3, 0.4, 0.5)
 
 
 def solver_samples(n_samples, seed=7, develop_steps=20, every=4, heights=HEIGHTS, nu_factor_max=30.0,
                    kappa=0.41, B=5.2, precision=0):
-    """(X, Y) with n_samples rows drawn from runs of the small channel."""
+    """(X, Y) with n_samples rows drawn from runs of the small channel.
+
+    Each round draws a viscosity factor per height, takes the equilibrium
+    stress of the current flow at that height and viscosity, advances one step
+    and samples again with that stress as the lagged wall stress (the wall
+    model sees the previous evaluation's stress in the same way), then
+    advances the remaining steps of the round."""
     case, field = cases.config("small", seed=seed)
     ch = Channel(case.x, case.y, case.z, precision=precision, **case.kwargs())
     ch.set_state(*field)
@@ -34,33 +40,48 @@ def solver_samples(n_samples, seed=7, develop_steps=20, every=4, heights=HEIGHTS
     ch.run_steps(develop_steps)
     rng = np.random.default_rng(seed)
     Xs, Ys, n = [], [], 0
+    shape = (ch.nx, 2, ch.nzg)
     while n < n_samples:
-        ch.run_steps(every)
-        for h in heights:
-            f = float(np.exp(rng.uniform(0.0, np.log(nu_factor_max))))
-            X, Y = ch.wall_samples(h_wm=h, nu=case.nu / f, kappa=kappa, B=B)
+        nus = [case.nu / float(np.exp(rng.uniform(0.0, np.log(nu_factor_max)))) for _ in heights]
+        prev = [ch.wall_samples(h_wm=h, nu=nu, kappa=kappa, B=B, stress=True)[2] for h, nu in zip(heights, nus)]
+        ch.run_steps(1)
+        for h, nu, tp in zip(heights, nus, prev):
+            ch.set_field(ALPHA_X, np.asfortranarray(tp.reshape(shape, order="F")))
+            X, Y = ch.wall_samples(h_wm=h, nu=nu, kappa=kappa, B=B)
             Xs.append(X)
             Ys.append(Y)
             n += len(X)
+        ch.run_steps(max(every - 1, 0))
     X = np.concatenate(Xs)[:n_samples]
     Y = np.concatenate(Ys)[:n_samples]
     ch.close()
     return X, Y
 
 
-def train(X, Y, epochs=20, batch=1024, lr=1e-3, lr_final=1e-5, seed=0, device=0):
+def train(X, Y, epochs=20, batch=1024, lr=1e-3, lr_final=1e-5, seed=0, device=0, input_noise=1.0):
     """A WallNetwork fitted to (X, Y): standardisation from the data, then
     Adam with the learning rate lr in the first epoch, decaying on a cosine to
     lr_final in the last (the optimiser state carries over between epochs);
-    returns (network, per-epoch loss)."""
+    returns (network, per-epoch loss).  input_noise > 0 adds, every epoch,
+    fresh Gaussian noise of that many standard deviations to the features
+    f1..f3 (wall-normal gradient, lagged stress, pressure gradient): the
+    equilibrium target is a function of the local Reynolds number f0, and the
+    noise keeps the network from keying on how the other features happen to
+    correlate with it in the training flow."""
     net = WallNetwork(seed=seed, device=device)
     net.fit_normalization(X, Y)
     X = np.ascontiguousarray(X, np.float32)
     Y = np.ascontiguousarray(Y, np.float32)
+    sd = X.std(0)
+    rng = np.random.default_rng(seed)
     losses = []
     for e in range(epochs):
         c = 0.5 * (1.0 + np.cos(np.pi * e / max(epochs - 1, 1)))
-        losses.append(net.train(X, Y, epochs=1, batch=batch, lr=lr_final + (lr - lr_final) * c, seed=seed + e)[0])
+        Xe = X
+        if input_noise > 0.0:
+            Xe = X.copy()
+            Xe[:, 1:] += (input_noise * sd[1:] * rng.standard_normal((len(X), 3))).astype(np.float32)
+        losses.append(net.train(Xe, Y, epochs=1, batch=batch, lr=lr_final + (lr - lr_final) * c, seed=seed + e)[0])
     return net, np.array(losses)
 
 

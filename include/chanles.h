@@ -26,6 +26,7 @@ extern "C" {
 
 typedef struct chan_ctx chan_ctx;
 typedef struct chan_mlp chan_mlp;
+typedef struct chan_ensemble chan_ensemble;
 
 enum { CHAN_OK = 0, CHAN_EINVAL = -1, CHAN_EHIP = -2, CHAN_ESTATE = -3, CHAN_ENOMEM = -4 };
 
@@ -168,9 +169,12 @@ int chan_mlp_train(chan_mlp* net, const float* X, const float* Y, int n, int epo
                    unsigned long long seed, double* epoch_loss);
 
 /* Samples of every wall cell of the channel's current state at height h_wm
- * (<= 0: first off-wall cell) with viscosity nu: features X (2*nx*nzg x 4)
- * and equilibrium targets Y (2*nx*nzg x 2), in alpha_x order. */
-int chan_wall_samples(chan_ctx* ctx, double h_wm, double nu, double kappa, double B, float* X, float* Y);
+ * (<= 0: first off-wall cell) with viscosity nu: features X (2*nx*nzg x 4),
+ * equilibrium targets Y (2*nx*nzg x 2) and, if tau is not NULL, the
+ * equilibrium stress u_tau^2 (2*nx*nzg), in alpha_x order.  The lagged-stress
+ * feature uses the context's alpha_x. */
+int chan_wall_samples(chan_ctx* ctx, double h_wm, double nu, double kappa, double B, float* X, float* Y,
+                      double* tau);
 /* the network used by CHAN_WALL_MLP (not owned; same device as ctx; do not
  * train it while a step that uses it runs) and the largest relative deviation
  * |(tau_par, tau_perp) - (u_tau^2, 0)| / u_tau^2 from the equilibrium stress
@@ -182,6 +186,25 @@ int chan_evaluate_wall_stress(chan_ctx* ctx);
 /* of the last CHAN_WALL_MLP evaluation: mean relative deviation from the
  * equilibrium stress over the wall cells, fraction of cells that fell back */
 int chan_wall_model_stats(chan_ctx* ctx, double out2[2]);
+
+/* ---------------------------------------------------------------------
+ * Ensembles: many independent channels (any sizes, precisions and wall
+ * models; one device) advanced together.  No reference counterpart (the
+ * reference runs one channel per MPI job).  One step of every member is
+ * captured into one graph in which each member's kernels form an independent
+ * branch; chan_ensemble_run_steps launches that graph once per step
+ * (compute_dt + compute_time_step_RK3 of every member).  Members stay usable
+ * on their own: the ensemble orders its steps after the members' pending
+ * work and the members' later work after its steps.  Changing a member's
+ * settings (wall model, tile shape) or stepping it alone is detected and the
+ * graphs are captured again.  The ensemble does not own the members; destroy
+ * it before them. */
+int  chan_ensemble_create(chan_ctx* const* members, int n, chan_ensemble** out);
+void chan_ensemble_destroy(chan_ensemble* ens);
+const char* chan_ensemble_last_error(const chan_ensemble* ens);
+int chan_ensemble_size(const chan_ensemble* ens);
+int chan_ensemble_run_steps(chan_ensemble* ens, int nsteps);
+int chan_ensemble_synchronize(chan_ensemble* ens);
 
 /* Tile shapes and the autotuner.  No reference counterpart (the reference's
  * loops have no tiles).  CHAN_KERNEL_RHS selects the fused RHS + RK kernel,

@@ -93,9 +93,9 @@ def test_wall_samples_match_oracle(gpu):
 
 @pytest.fixture(scope="module")
 def trained():
-    X, Y = wallnet.solver_samples(400_000, seed=7, develop_steps=20)
+    X, Y = wallnet.solver_samples(1_000_000, seed=7, develop_steps=20)
     Xv, Yv = wallnet.solver_samples(20_000, seed=8, develop_steps=80)
-    net, losses = wallnet.train(X, Y, epochs=10, batch=1024, lr=1e-3, seed=0)
+    net, losses = wallnet.train(X, Y, epochs=20, batch=1024, lr=1e-3, seed=0)
     return net, losses, Xv, Yv
 
 
@@ -109,6 +109,21 @@ def test_training_gains_over_untrained_network(gpu, trained):
     err0 = wallnet.stress_error(raw, Xv, Yv)
     assert np.mean(err) <= 0.02, np.mean(err)
     assert np.mean(err) < 0.1 * np.mean(err0), (np.mean(err), np.mean(err0))
+
+
+def test_trained_network_generalises_to_ensemble_flows(gpu, trained):
+    # channels of other grids and Reynolds numbers (the batched ensemble's
+    # draw), wall model at h = 0.1: the network trained on the small channel
+    # stays within 2% of the equilibrium stress on average
+    from channel_rl_amd.ensemble import make_channel
+    net = trained[0]
+    errs = []
+    for c, f in cases.ensemble(6):
+        ch = make_channel(c, f)
+        ch.run_steps(10)
+        X, Y = ch.wall_samples(h_wm=c.h_wm)
+        errs.append(np.mean(wallnet.stress_error(net, X, Y)))
+    assert np.mean(errs) <= 0.03, errs
 
 
 def _small_channel(seed, mode, net=None, steps=0):
