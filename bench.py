@@ -59,11 +59,77 @@ def parse():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline (0: skip)")
+    ap.add_argument("--ensemble", type=int, default=64, help="channels of the batched ensemble (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=10)
     return ap.parse_args()
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def measure_batched(ch, case, args, cr, cases):
+    """The learned wall model and the batched ensemble: train the network on
+    data of the small channel (10^6 samples, 20 epochs), run this channel with
+    it, then the ensemble's channels plain (FP64, equilibrium wall model, each
+    channel's own captured step, one channel after the other) and batched
+    (FP32 bulk + FP64 Poisson, the network in the loop, one graph launch per
+    ensemble step)."""
+    from channel_rl_amd import ensemble as E
+    from channel_rl_amd import wallnet
+    t0 = time.perf_counter()
+    X, Y = wallnet.solver_samples(1_000_000, seed=7)
+    t1 = time.perf_counter()
+    net, losses = wallnet.train(X, Y, epochs=20)
+    t2 = time.perf_counter()
+    Xv, Yv = wallnet.solver_samples(40_000, seed=8, develop_steps=80)
+    val = float(np.mean(wallnet.stress_error(net, Xv, Yv)))
+    log(f"[bench] wall network: samples {t1 - t0:.1f} s, training {t2 - t1:.1f} s, loss {losses[-1]:.3g}, "
+        f"validation error {val:.4f}")
+    # this channel with the network as its wall model
+    ch.set_wall_network(net, 0.05)
+    ch.set_wall_model(cr.WALL_MLP, h_wm=case.h_wm)
+    ch.run_steps(5)
+    ch.synchronize()
+    n = 50
+    s0 = time.perf_counter()
+    ch.run_steps(n)
+    ch.synchronize()
+    ms_mlp = 1e3 * (time.perf_counter() - s0) / n
+    err1, fb1 = ch.wall_model_stats()
+    log(f"[bench] {case.name} with the network: {ms_mlp:.3f} ms/step, error {err1:.4f}, fallback {fb1:.4f}")
+    mem = cases.ensemble(args.ensemble)
+    ncell = sum(c.cells for c, _ in mem)
+    k = args.ensemble_steps
+    plain = [E.make_channel(c, f, precision=cr.FP64, wall=cr.WALL_EQUILIBRIUM) for c, f in mem]
+    tp = E.time_sequential(plain, k)
+    for c in plain:
+        c.close()
+    chans = [E.make_channel(c, f, precision=cr.FP32_BULK, wall=cr.WALL_MLP, net=net) for c, f in mem]
+    ens = cr.Ensemble(chans)
+    tb = E.time_batched(ens, k)
+    werr, wfb = E.wall_error(chans)
+    ens.close()
+    for c in chans:
+        c.close()
+    out = {
+        "workload": f"{len(mem)} channels, faces drawn from 34 33 34 / 66 49 66 / 98 65 98, Re_tau 180..5200, "
+                    f"{ncell} cells, wall model at h_wm = 0.1",
+        "steps": k,
+        "plain_channel_steps_per_s": round(len(mem) * k / tp, 1),
+        "batched_channel_steps_per_s": round(len(mem) * k / tb, 1),
+        "speedup": round(tp / tb, 3),
+        "batched_mcells_per_s": round(ncell * k / tb / 1e6, 1),
+        "ensemble_wall_stress_error": round(werr, 4),
+        "ensemble_fallback_fraction": round(wfb, 4),
+        "network": {"samples": len(X), "epochs": 20, "train_s": round(t2 - t1, 2), "final_loss": float(losses[-1]),
+                    "validation_error": round(val, 4)},
+        "single_channel_mlp_ms_per_step": round(ms_mlp, 4),
+        "single_channel_mlp_error": round(err1, 4),
+    }
+    log(f"[bench] ensemble: plain {out['plain_channel_steps_per_s']} channel-steps/s, batched "
+        f"{out['batched_channel_steps_per_s']} channel-steps/s, wall error {werr:.4f}, fallback {wfb:.4f}")
+    return out
 
 
 def main():
@@ -133,6 +199,10 @@ def main():
                 "copy_kernel_gbs": round(copy_gbs, 1),
                 "step_alg_gbs": round(cells * (976 if prec == cr.FP64 else 632) / (ms_step * 1e-3) / 1e9, 1)}
 
+    batched = None
+    if args.ensemble > 0:
+        batched = measure_batched(ch, case, args, cr, cases)
+
     cpu = None
     if args.cpu_steps > 0:
         import oracle
@@ -171,6 +241,7 @@ def main():
                    "cells": cells, "parallelism": "single GPU", "max_abs_div": maxdiv},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
     return 0
