@@ -276,6 +276,9 @@ void enqueue_eddy_viscosity(chan_ctx* c, int buf, int fuse_wm)
         DsmBufs<T> b = dsm_bufs<T>(c);
         launch_dsm<T>(s[0], s[1], s[2], (T*)c->nut, metric<T>(c), c->d, b, c->Dirichlet_nu_t == 1, fuse_wm,
                       c->alpha, nu, c->stream);
+    } else if (c->LES_model == 3) {
+        launch_amd<T>(s[0], s[1], s[2], (T*)c->nut, metric<T>(c), c->d, c->Dirichlet_nu_t, c->sc, fuse_wm, c->alpha,
+                      nu, c->stream);
     } else {
         k_fill<T><<<cdiv(c->d.nbox, 256), 256, 0, c->stream>>>((T*)c->nut, c->d.nbox, T(0));
         if (fuse_wm) launch_wall_model<T>(s[0], (T*)c->nut, c->alpha, metric<T>(c), c->d, nu, c->stream);
@@ -404,8 +407,10 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
     if (!cfg->x || !cfg->y || !cfg->z) return fail(c, CHAN_EINVAL, "Error: missing grid coordinates");
     if (cfg->precision != CHAN_FP64 && cfg->precision != CHAN_FP32_BULK)
         return fail(c, CHAN_EINVAL, "Error: invalid precision");
-    if (cfg->LES_model != 0 && cfg->LES_model != 2)
-        return fail(c, CHAN_EINVAL, "Error: LES_model must be 0 (none) or 2 (dynamic Smagorinsky)");
+    if (cfg->LES_model != 0 && cfg->LES_model != 2 && cfg->LES_model != 3)
+        return fail(c, CHAN_EINVAL, "Error: LES_model must be 0 (none), 2 (dynamic Smagorinsky) or 3 (AMD)");
+    if (cfg->Dirichlet_nu_t < 0 || cfg->Dirichlet_nu_t > 2 || (cfg->Dirichlet_nu_t == 2 && cfg->LES_model != 3))
+        return fail(c, CHAN_EINVAL, "Error: Dirichlet_nu_t must be 1 or 0 (2: AMD only)");
     c->prec = cfg->precision;
     c->device = cfg->device;
     c->nu = cfg->nu; c->CFL = cfg->CFL; c->dPdx0 = cfg->dPdx;

@@ -43,6 +43,11 @@ template <typename T>
 void launch_dsm(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, const Dims& d,
                 DsmBufs<T>& b, int dirichlet, int fuse_wm, const double* alpha_x, T nu, hipStream_t s);
 
+// AMD eddy viscosity (amd.hip), dirichlet = Dirichlet_nu_t (0, 1, 2)
+template <typename T>
+void launch_amd(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, const Dims& d, int dirichlet,
+                const Scalars* sc, int fuse_wm, const double* alpha, T nu, hipStream_t s);
+
 // fused pass A of the dynamic Smagorinsky model (dsm_fused.hip)
 int dsm_fused_nblk(const Dims& d);
 template <typename T>

@@ -104,22 +104,6 @@ __global__ void k_dsm_ratio(const double* part, int nblk, int nyg, double* ratio
     }
 }
 
-// code/wallmodel.f90:32-37: wall-model eddy viscosity at a centre from the
-// two adjacent x faces; wall = 0 bottom (rows 1,2), 1 top (rows nyg-1,nyg).
-template <typename T>
-__device__ __forceinline__ T wm_nut_at(const T* U, const double* alpha, const Metric<T>& m, const Dims& d, T nu,
-                                       int ic, int k, int wall)
-{
-    const T dyw = m.yg[2] - m.yg[1];
-    auto tau = [&](int f) {
-        const T a = T(alpha[(f - 1) + (long)d.nx * (wall + 2L * (k - 1))]);
-        const T du = wall == 0 ? (U[at(d, f, 2, k)] - U[at(d, f, 1, k)])
-                               : (U[at(d, f, d.nyg - 1, k)] - U[at(d, f, d.nyg, k)]);
-        return a / (du / dyw) - nu;
-    };
-    return T(0.5) * (tau(ic - 1) + tau(ic));
-}
-
 // code/subgrid.f90:512-535: nu_t = max(-1/2 <LM>/<MM> |S|, 0), y ghost by
 // Dirichlet_nu_t; with fuse_wm the wall rows come from the wall model
 // (code/wallmodel.f90:39-42), which the reference applies right after.

@@ -53,8 +53,9 @@ typedef struct {
     double CFL;                /* CFL, or fixed dt = -CFL when negative    */
     double dPdx;               /* pressure gradient                        */
     int x_mass_cte;            /* 1: constant mass flow in x               */
-    int LES_model;             /* 0 none, 2 dynamic Smagorinsky            */
-    int Dirichlet_nu_t;        /* 1 Dirichlet, 0 Neumann ghost for nu_t    */
+    int LES_model;             /* 0 none, 2 dynamic Smagorinsky, 3 AMD     */
+    int Dirichlet_nu_t;        /* 1 Dirichlet, 0 Neumann ghost for nu_t,
+                                  2 Dirichlet with wall offset (AMD)       */
     int precision;             /* CHAN_FP64 or CHAN_FP32_BULK              */
     int device;                /* HIP device ordinal                       */
 } chan_config;
@@ -76,7 +77,8 @@ int chan_get_scalars(chan_ctx* ctx, double out4[4]);
 
 /* code/boundary_conditions.f90:20 apply_boundary_conditions (on U,V,W) */
 int chan_apply_boundary_conditions(chan_ctx* ctx);
-/* code/subgrid.f90:23 compute_eddy_viscosity(U,V,W,nu_t) */
+/* code/subgrid.f90:23 compute_eddy_viscosity(U,V,W,nu_t): LES_model 2 dynamic
+ * Smagorinsky (code/subgrid.f90:379-537), 3 AMD (code/subgrid.f90:632-874) */
 int chan_compute_eddy_viscosity(chan_ctx* ctx);
 /* code/wallmodel.f90:23 compute_wall_model(U,V,W,nu_t) from alpha_x */
 int chan_compute_wall_model(chan_ctx* ctx);

@@ -721,11 +721,140 @@ void orc_dsm_parts(orc* o, const double* U, const double* V, const double* W, do
     dsm_core(o, U, V, W, 0, L, M, LMp, MMp);
 }
 
+/* code/subgrid.f90:632-874 -- anisotropic minimum dissipation (AMD) at one
+ * interior centre (i,j,k):
+ *   nu_t = -C N / D, C = 0.3,
+ *   N = sum (dh_k u_i)(dh_k u_j) S_ij (dh: the undivided difference, i.e. the
+ *       spacing times the derivative), D = sum (d_l u_m)^2,
+ * with every staggered difference carried to the centre by the reference's
+ * face-to-centre averages (code/interpolation.f90:24-95, plain averages) in
+ * the order its calls use them, and the terms accumulated in the order of
+ * its statements.  ax(a, b) = 0.5*(a + b). */
+#define AX(a, b) (0.5 * ((a) + (b)))
+static double amd_point(const orc* o, const double* U, const double* V, const double* W, const double* Sij, int i,
+                        int j, int k)
+{
+    const double *x = o->x, *y = o->y, *z = o->z, *xg = o->xg, *yg = o->yg, *zg = o->zg;
+    const size_t nc = nC(o);
+    const double S11 = AC(Sij, i, j, k), S22 = AC(Sij + nc, i, j, k), S33 = AC(Sij + 2 * nc, i, j, k);
+    const double S12 = AC(Sij + 3 * nc, i, j, k), S13 = AC(Sij + 4 * nc, i, j, k), S23 = AC(Sij + 5 * nc, i, j, k);
+    /* undivided differences at their natural positions */
+#define dUx(a, b, c) (AU(U, a, b, c) - AU(U, (a) - 1, b, c))
+#define dUy(a, b, c) (AU(U, a, b, c) - AU(U, a, (b) - 1, c))
+#define dUz(a, b, c) (AU(U, a, b, c) - AU(U, a, b, (c) - 1))
+#define dVx(a, b, c) (AV(V, a, b, c) - AV(V, (a) - 1, b, c))
+#define dVy(a, b, c) (AV(V, a, b, c) - AV(V, a, (b) - 1, c))
+#define dVz(a, b, c) (AV(V, a, b, c) - AV(V, a, b, (c) - 1))
+#define dWx(a, b, c) (AW(W, a, b, c) - AW(W, (a) - 1, b, c))
+#define dWy(a, b, c) (AW(W, a, b, c) - AW(W, a, (b) - 1, c))
+#define dWz(a, b, c) (AW(W, a, b, c) - AW(W, a, b, (c) - 1))
+    /* U: x then y / x then z; V: y then x / y then z; W: z then x / z then y */
+    const double Uxh = dUx(i, j, k);
+    const double Uyh = AX(AX(dUy(i - 1, j, k), dUy(i, j, k)), AX(dUy(i - 1, j + 1, k), dUy(i, j + 1, k)));
+    const double Uzh = AX(AX(dUz(i - 1, j, k), dUz(i, j, k)), AX(dUz(i - 1, j, k + 1), dUz(i, j, k + 1)));
+    const double Vyh = dVy(i, j, k);
+    const double Vxh = AX(AX(dVx(i, j - 1, k), dVx(i, j, k)), AX(dVx(i + 1, j - 1, k), dVx(i + 1, j, k)));
+    const double Vzh = AX(AX(dVz(i, j - 1, k), dVz(i, j, k)), AX(dVz(i, j - 1, k + 1), dVz(i, j, k + 1)));
+    const double Wzh = dWz(i, j, k);
+    const double Wxh = AX(AX(dWx(i, j, k - 1), dWx(i, j, k)), AX(dWx(i + 1, j, k - 1), dWx(i + 1, j, k)));
+    const double Wyh = AX(AX(dWy(i, j, k - 1), dWy(i, j, k)), AX(dWy(i, j + 1, k - 1), dWy(i, j + 1, k)));
+    /* derivatives: the differences divided by their spacings before averaging */
+#define q(a, b) ((a) / (b))
+    const double Uyd = AX(AX(q(dUy(i - 1, j, k), yg[j] - yg[j - 1]), q(dUy(i, j, k), yg[j] - yg[j - 1])),
+                          AX(q(dUy(i - 1, j + 1, k), yg[j + 1] - yg[j]), q(dUy(i, j + 1, k), yg[j + 1] - yg[j])));
+    const double Uzd = AX(AX(q(dUz(i - 1, j, k), zg[k] - zg[k - 1]), q(dUz(i, j, k), zg[k] - zg[k - 1])),
+                          AX(q(dUz(i - 1, j, k + 1), zg[k + 1] - zg[k]), q(dUz(i, j, k + 1), zg[k + 1] - zg[k])));
+    const double Vxd = AX(AX(q(dVx(i, j - 1, k), xg[i] - xg[i - 1]), q(dVx(i, j, k), xg[i] - xg[i - 1])),
+                          AX(q(dVx(i + 1, j - 1, k), xg[i + 1] - xg[i]), q(dVx(i + 1, j, k), xg[i + 1] - xg[i])));
+    const double Vzd = AX(AX(q(dVz(i, j - 1, k), zg[k] - zg[k - 1]), q(dVz(i, j, k), zg[k] - zg[k - 1])),
+                          AX(q(dVz(i, j - 1, k + 1), zg[k + 1] - zg[k]), q(dVz(i, j, k + 1), zg[k + 1] - zg[k])));
+    const double Wxd = AX(AX(q(dWx(i, j, k - 1), xg[i] - xg[i - 1]), q(dWx(i, j, k), xg[i] - xg[i - 1])),
+                          AX(q(dWx(i + 1, j, k - 1), xg[i + 1] - xg[i]), q(dWx(i + 1, j, k), xg[i + 1] - xg[i])));
+    const double Wyd = AX(AX(q(dWy(i, j, k - 1), yg[j] - yg[j - 1]), q(dWy(i, j, k), yg[j] - yg[j - 1])),
+                          AX(q(dWy(i, j + 1, k - 1), yg[j + 1] - yg[j]), q(dWy(i, j + 1, k), yg[j + 1] - yg[j])));
+#undef q
+    /* cross terms formed before the last average */
+    const double z12 = AX(AX(dUz(i - 1, j, k), dUz(i, j, k)) * AX(dVz(i, j - 1, k), dVz(i, j, k)),
+                          AX(dUz(i - 1, j, k + 1), dUz(i, j, k + 1)) * AX(dVz(i, j - 1, k + 1), dVz(i, j, k + 1)));
+    const double y13 = AX(AX(dUy(i - 1, j, k), dUy(i, j, k)) * AX(dWy(i, j, k - 1), dWy(i, j, k)),
+                          AX(dUy(i - 1, j + 1, k), dUy(i, j + 1, k)) * AX(dWy(i, j + 1, k - 1), dWy(i, j + 1, k)));
+    const double x23 = AX(AX(dVx(i, j - 1, k), dVx(i, j, k)) * AX(dWx(i, j, k - 1), dWx(i, j, k)),
+                          AX(dVx(i + 1, j - 1, k), dVx(i + 1, j, k)) * AX(dWx(i + 1, j, k - 1), dWx(i + 1, j, k)));
+    double N = Uxh * Uxh * S11;
+    N = N + Uyh * Uyh * S11;
+    N = N + Uzh * Uzh * S11;
+    N = N + Vyh * Vyh * S22;
+    N = N + Vxh * Vxh * S22;
+    N = N + Vzh * Vzh * S22;
+    N = N + (Wzh * Wzh) * S33;
+    N = N + Wxh * Wxh * S33;
+    N = N + Wyh * Wyh * S33;
+    N = N + 2.0 * (Vxh * Uxh) * S12;
+    N = N + 2.0 * (Uyh * Vyh) * S12;
+    N = N + 2.0 * z12 * S12;
+    N = N + 2.0 * (Wxh * Uxh) * S13;
+    N = N + 2.0 * (Uzh * Wzh) * S13;
+    N = N + 2.0 * y13 * S13;
+    N = N + 2.0 * (Wyh * Vyh) * S23;
+    N = N + 2.0 * (Vzh * Wzh) * S23;
+    N = N + 2.0 * x23 * S23;
+    const double dx = x[i] - x[i - 1], dy = y[j] - y[j - 1], dz = z[k] - z[k - 1];
+    double D = Uxh * Uxh / (dx * dx);
+    D = D + Uyd * Uyd;
+    D = D + Uzd * Uzd;
+    D = D + Vyh * Vyh / (dy * dy);
+    D = D + Vxd * Vxd;
+    D = D + Vzd * Vzd;
+    D = D + (Wzh * Wzh) / (dz * dz);
+    D = D + Wxd * Wxd;
+    D = D + Wyd * Wyd;
+#undef dUx
+#undef dUy
+#undef dUz
+#undef dVx
+#undef dVy
+#undef dVz
+#undef dWx
+#undef dWy
+#undef dWz
+    return -0.3 * N / D;
+}
+#undef AX
+
+static void amd_core(orc* o, const double* U, const double* V, const double* W, double* nu_t)
+{
+    const int nxg = o->nxg, nyg = o->nyg, nzg = o->nzg;
+    compute_Sij_full(o, U, V, W, o->nx, o->ny, o->nz, o->Sijb, o->Sbuf);
+    memset(nu_t, 0, nC(o) * sizeof(double));
+    for (int k = 2; k <= nzg - 1; k++)
+        for (int j = 2; j <= nyg - 1; j++)
+            for (int i = 2; i <= nxg - 1; i++) AC(nu_t, i, j, k) = amd_point(o, U, V, W, o->Sijb, i, j, k);
+    /* clipping; 0/0 (a uniform neighbourhood) gives NaN, clipped to 0 (fmax) */
+    for (int k = 2; k <= nzg - 1; k++)
+        for (int j = 2; j <= nyg - 1; j++)
+            for (int i = 2; i <= nxg - 1; i++) AC(nu_t, i, j, k) = fmax(AC(nu_t, i, j, k), 0.0);
+    periodic_x(nu_t, nxg, nyg, nzg, 0);
+    periodic_z(nu_t, nxg, nyg, nzg, 0);
+    if (o->Dirichlet_nu_t == 2) {
+        /* code/subgrid.f90:868-869: Dirichlet with a wall offset */
+        const double off = 2.0 * 0.38 * sqrt(o->dPdx) * (o->y[2] - o->y[1]) / 2.0;
+        for (int k = 1; k <= nzg; k++)
+            for (int i = 1; i <= nxg; i++) {
+                AC(nu_t, i, 1, k) = -AC(nu_t, i, 2, k) + off;
+                AC(nu_t, i, nyg, k) = -AC(nu_t, i, nyg - 1, k) + off;
+            }
+    } else {
+        wall_y(nu_t, nxg, nyg, nzg, 0, o->Dirichlet_nu_t == 1);
+    }
+}
+
 /* code/subgrid.f90:23-47 */
 void orc_compute_eddy_viscosity(orc* o, const double* U, const double* V, const double* W, double* nu_t)
 {
     if (o->LES_model == 2) {
         dsm_core(o, U, V, W, nu_t, o->Lij, o->Mij, 0, 0);
+    } else if (o->LES_model == 3) {
+        amd_core(o, U, V, W, nu_t);
     } else {
         memset(nu_t, 0, nC(o) * sizeof(double));
     }

@@ -205,3 +205,26 @@ def test_rhs_tile_shapes_agree_bitwise(gpu):
     assert 0 <= g.tile_shape() < n and t[g.tile_shape()] == t.min()
     for a, b in zip(g.state(), before):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n,dirichlet", [((10, 9, 10), 1), ((18, 17, 18), 0), ((34, 33, 34), 2)])
+def test_amd_matches_oracle(gpu, n, dirichlet):
+    case, field = cases.small_case(*n, seed=6, LES_model=3)
+    case.Dirichlet_nu_t = dirichlet
+    U, V, W = projected_state(case, field)
+    o = oracle_for(case)
+    g = Channel(case.x, case.y, case.z, precision=FP64, **case.kwargs())
+    g.set_state(U, V, W)
+    o.set_scalars(dPdx=0.36)
+    g.set_scalars(dPdx=0.36)
+    ref = o.compute_eddy_viscosity(U, V, W)
+    g.compute_eddy_viscosity()
+    got = g.get_field(NU_T)
+    assert np.all(np.isfinite(got)) and np.abs(ref).max() > 0
+    assert rel(got, ref) <= 1e-12, rel(got, ref)
+
+
+def test_rk3_with_amd(gpu):
+    case, o, g = run_pair(18, 17, 18, 3, LES_model=3)
+    for a, b in zip(g.state(), o.state()):
+        assert rel(a, b) <= 1e-10, rel(a, b)

@@ -397,6 +397,70 @@ def test_dsm_constant_field_is_zero():
     assert np.all(nut == 0.0)  # <M:M> = 0 -> 0/0 clipped to 0
 
 
+# ------------------------------------------------------------ AMD model
+
+def _amd_expected(A, dxyz):
+    """AMD of the linear field u_i = A_ik x_k on a uniform grid, from the
+    model's tensor definition: -C sum_ij S_ij sum_k (D_k A_ik)(D_k A_jk) /
+    sum A_ml^2, clipped at 0."""
+    S = 0.5 * (A + A.T)
+    G = A * np.asarray(dxyz)[None, :]          # undivided differences dh_k u_i
+    N = np.einsum("ij,ik,jk->", S, G, G)
+    return max(-0.3 * N / np.sum(A * A), 0.0), N
+
+
+def test_amd_linear_field_closed_form():
+    o = mk(14, 13, 10, LES_model=3)
+    dxyz = (o.x[1] - o.x[0], o.y[1] - o.y[0], o.z[1] - o.z[0])
+    assert len({round(d, 12) for d in dxyz}) == 3  # distinct spacings catch swapped directions
+    A = np.array([[0.9, 0.35, -0.25], [0.5, -1.7, 0.45], [-0.65, 0.8, 0.6]])
+    nu_e, N = _amd_expected(A, dxyz)
+    if N > 0:
+        A = -A
+        nu_e, N = _amd_expected(A, dxyz)
+    assert nu_e > 0
+    U, V, W = fields(o, lambda x, y, z: A[0, 0] * x + A[0, 1] * y + A[0, 2] * z,
+                     lambda x, y, z: A[1, 0] * x + A[1, 1] * y + A[1, 2] * z,
+                     lambda x, y, z: A[2, 0] * x + A[2, 1] * y + A[2, 2] * z)
+    nut = o.compute_eddy_viscosity(U, V, W)
+    inner = nut[1:-2, 1:-1, 1:-2]
+    assert np.max(np.abs(inner - nu_e)) <= 1e-10 * nu_e
+    # Dirichlet ghost rows mirror the first interior row
+    assert np.allclose(nut[1:-2, 0, 1:-2], -nut[1:-2, 1, 1:-2])
+
+
+def test_amd_shear_and_constant_fields_are_zero():
+    o = mk(10, 9, 10, LES_model=3)
+    # constant field: 0/0, clipped to 0
+    U, V, W = fields(o, lambda x, y, z: 1.0 + 0 * x, zero, zero)
+    assert np.all(o.compute_eddy_viscosity(U, V, W) == 0.0)
+    # plane shear U = y: no normal strain, and dh_y U dh_y V S12 = 0 -> N = 0
+    U, V, W = fields(o, lambda x, y, z: y + 0 * x, zero, zero)
+    assert np.all(o.compute_eddy_viscosity(U, V, W) == 0.0)
+
+
+def test_amd_galilean_invariance():
+    o = mk(12, 11, 12, stretched=True, LES_model=3)
+    rng = np.random.default_rng(3)
+    U, V, W = (np.asfortranarray(rng.standard_normal(s)) for s in ((o.nx, o.nyg, o.nzg), (o.nxg, o.ny, o.nzg),
+                                                                   (o.nxg, o.nyg, o.nz)))
+    a = o.compute_eddy_viscosity(U, V, W)
+    b = o.compute_eddy_viscosity(U + 0.75, V - 0.5, W + 0.25)
+    assert np.max(np.abs(a - b)) <= 1e-12 * np.abs(a).max()
+    assert np.abs(a).max() > 0
+
+
+def test_amd_wall_offset_ghosts():
+    o = mk(10, 9, 10, LES_model=3, Dirichlet_nu_t=2, dPdx=0.49)
+    rng = np.random.default_rng(4)
+    U, V, W = (np.asfortranarray(rng.standard_normal(s)) for s in ((o.nx, o.nyg, o.nzg), (o.nxg, o.ny, o.nzg),
+                                                                   (o.nxg, o.nyg, o.nz)))
+    nut = o.compute_eddy_viscosity(U, V, W)
+    off = 0.38 * 0.7 * (o.y[1] - o.y[0])
+    assert np.allclose(nut[:, 0, :], -nut[:, 1, :] + off)
+    assert np.allclose(nut[:, -1, :], -nut[:, -2, :] + off)
+
+
 # ------------------------------------------------------------ wall model
 
 def test_wall_model_identity():
