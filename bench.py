@@ -61,11 +61,42 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--ensemble", type=int, default=64, help="channels of the batched ensemble (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=10)
+    ap.add_argument("--profile-compare", type=int, default=2000,
+                    help="steps of the FP32-bulk vs FP64 mean-profile comparison (0: skip)")
     return ap.parse_args()
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def compare_fp32_profiles(case, field, args, cr):
+    """The FP32 bulk against FP64 over a long run of this channel: both from
+    the same initial field, plane statistics averaged every 10 steps over the
+    second half; relative differences of the mean U+ profile, <u'v'>+ and
+    u_tau, and the FP32 run's ms/step."""
+    from channel_rl_amd import statistics as S
+    from channel_rl_amd.ensemble import make_channel
+    n = args.profile_compare
+    res, ms = {}, {}
+    for prec in (cr.FP64, cr.FP32_BULK):
+        c = make_channel(case, field, precision=prec, wall=cr.WALL_EQUILIBRIUM)
+        t0 = time.perf_counter()
+        c.run_steps(n // 2)
+        a = S.ProfileAverager(c)
+        a.run(n - n // 2, 10)
+        c.synchronize()
+        ms[prec] = 1e3 * (time.perf_counter() - t0) / n
+        res[prec] = a.result()
+        c.close()
+    d = S.profile_difference(res[cr.FP64], res[cr.FP32_BULK])
+    out = {"steps": n, "Uplus_rel_diff": d["Uplus"], "uvplus_rel_diff": d["uvplus"], "u_tau_rel_diff": d["u_tau"],
+           "Re_tau_fp64": round(float(res[cr.FP64]["Re_tau"]), 2),
+           "fp32_ms_per_step_with_stats": round(ms[cr.FP32_BULK], 4),
+           "fp64_ms_per_step_with_stats": round(ms[cr.FP64], 4)}
+    log(f"[bench] FP32 bulk vs FP64 over {n} steps: U+ {d['Uplus']:.2e}, <u'v'>+ {d['uvplus']:.2e}, "
+        f"u_tau {d['u_tau']:.2e}")
+    return out
 
 
 def measure_batched(ch, case, args, cr, cases):
@@ -199,6 +230,10 @@ def main():
                 "copy_kernel_gbs": round(copy_gbs, 1),
                 "step_alg_gbs": round(cells * (976 if prec == cr.FP64 else 632) / (ms_step * 1e-3) / 1e9, 1)}
 
+    fp32 = None
+    if args.profile_compare > 0:
+        fp32 = compare_fp32_profiles(case, field, args, cr)
+
     batched = None
     if args.ensemble > 0:
         batched = measure_batched(ch, case, args, cr, cases)
@@ -242,6 +277,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "fp32_bulk": fp32,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -35,7 +35,7 @@ EXPORTS = [
     "chan_mlp_set_params", "chan_mlp_fit_normalization", "chan_mlp_forward", "chan_mlp_gradients",
     "chan_mlp_train", "chan_wall_samples", "chan_set_wall_network", "chan_wall_model_stats", "chan_evaluate_wall_stress",
     "chan_ensemble_create", "chan_ensemble_destroy", "chan_ensemble_last_error", "chan_ensemble_size",
-    "chan_ensemble_run_steps", "chan_ensemble_synchronize",
+    "chan_ensemble_run_steps", "chan_ensemble_synchronize", "chan_wall_shear",
 ]
 KERNEL_RHS = 0
 WALL_MLP = 2
@@ -124,6 +124,7 @@ def load_library(path=LIB_PATH):
     L.chan_ensemble_size.argtypes = [vp]
     L.chan_ensemble_run_steps.argtypes = [vp, ctypes.c_int]
     L.chan_ensemble_synchronize.argtypes = [vp]
+    L.chan_wall_shear.argtypes = [vp, _D]
     _LIB = L
     return L
 
@@ -418,6 +419,12 @@ class Channel:
         self._net = net  # keep it alive while attached
         self._chk(load_library().chan_set_wall_network(self._h, None if net is None else net._h, max_rel_err),
                   "set_wall_network")
+
+    def wall_shear(self):
+        """dict(tau_bottom, tau_top, u_tau, Re_tau) of the current state."""
+        out = np.zeros(4)
+        self._chk(load_library().chan_wall_shear(self._h, _p(out)), "wall_shear")
+        return dict(tau_bottom=out[0], tau_top=out[1], u_tau=out[2], Re_tau=out[3])
 
     def evaluate_wall_stress(self):
         """Evaluate the selected wall-stress source once into alpha_x."""

@@ -870,6 +870,33 @@ int chan_plane_statistics(chan_ctx* c, double* out)
     return CHAN_OK;
 }
 
+int chan_wall_shear(chan_ctx* c, double out4[4])
+{
+    if (!c || !out4) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    void** s = c->state[c->cur];
+    double* dout = c->stage;  // two doubles of the staging area
+    if (c->prec == CHAN_FP64)
+        launch_wall_shear<double>((double*)s[0], (double*)c->nut, c->md, c->d, c->nu, c->part, c->nblk_red, dout,
+                                  c->stream);
+    else
+        launch_wall_shear<float>((float*)s[0], (float*)c->nut, c->md, c->d, c->nu, c->part, c->nblk_red, dout,
+                                 c->stream);
+    if ((rc = check_launch(c))) return rc;
+    double t[2];
+    HIPCHK(c, hipMemcpyAsync(t, dout, sizeof(t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // code/statistics.f90:126-130 with Ly the channel height
+    const double Ly = c->y[c->d.ny] - c->y[1];
+    const double ut = std::sqrt((t[0] + t[1]) / Ly);
+    out4[0] = t[0];
+    out4[1] = t[1];
+    out4[2] = ut;
+    out4[3] = ut * Ly / 2.0 / c->nu;
+    return CHAN_OK;
+}
+
 int chan_run_steps(chan_ctx* c, int nsteps)
 {
     if (!c || nsteps < 0) return CHAN_EINVAL;

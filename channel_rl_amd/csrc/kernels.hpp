@@ -83,6 +83,10 @@ void launch_plane_stats(const T* U, const T* V, const T* W, const Metric<double>
                         double* out, hipStream_t s);
 
 template <typename T>
+void launch_wall_shear(const T* U, const T* nut, const Metric<double>& md, const Dims& d, double nu, double* part,
+                       int nblk, double* out, hipStream_t s);
+
+template <typename T>
 void launch_scale_shift(T* F, const Dims& d, hipStream_t s);
 
 // ---------------------------------------------------------------- Poisson

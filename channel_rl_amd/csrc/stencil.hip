@@ -504,6 +504,47 @@ void launch_plane_stats(const T* U, const T* V, const T* W, const Metric<double>
     PROF_END(s);
 }
 
+// code/statistics.f90:95-124: mean wall shear stress (nu + nu_t) dU/dy at
+// each wall over the unique x faces and z centres, nu_t taken to the wall
+// face by the x average and the weighted y interpolation of
+// code/interpolation.f90.  The reference evaluates the bottom wall for both
+// sums; here the top wall uses its own rows (U(nyg-1) - U(nyg)) / (yg(nyg) -
+// yg(nyg-1)) and its own wall face (y face ny).  out: tau_bottom, tau_top.
+template <typename T>
+__global__ void k_wall_shear_partial(const T* U, const T* nut, Metric<double> m, Dims d, double nu, double* part,
+                                     int nblk)
+{
+    const long n1 = d.nx - 2, n3 = d.nzp, tot = n1 * n3;
+    const int jt = d.nyg - 1, ny = d.ny;
+    double sb = 0.0, st = 0.0;
+    for (long t = (long)blockIdx.x * RED_THREADS + threadIdx.x; t < tot; t += (long)nblk * RED_THREADS) {
+        const int i = (int)(t % n1) + 2, k = (int)(t / n1) + 2;
+        auto tx = [&](int j) { return 0.5 * ((double)nut[at(d, i, j, k)] + (double)nut[at(d, i + 1, j, k)]); };
+        const double nb = m.w0[1] * tx(1) + m.w1[1] * tx(2);
+        const double nt = m.w0[ny] * tx(jt) + m.w1[ny] * tx(jt + 1);
+        sb += ((double)U[at(d, i, 2, k)] - (double)U[at(d, i, 1, k)]) / (m.yg[2] - m.yg[1]) * (nu + nb);
+        st += ((double)U[at(d, i, jt, k)] - (double)U[at(d, i, jt + 1, k)]) / (m.yg[jt + 1] - m.yg[jt]) * (nu + nt);
+    }
+    block_sum2(sb, st, part, nblk);
+}
+
+__global__ void k_wall_shear_final(const double* part, int nblk, double ncell, double* out)
+{
+    double a, b;
+    final_sum2(part, nblk, a, b);
+    if (threadIdx.x == 0) { out[0] = a / ncell; out[1] = b / ncell; }
+}
+
+template <typename T>
+void launch_wall_shear(const T* U, const T* nut, const Metric<double>& md, const Dims& d, double nu, double* part,
+                       int nblk, double* out, hipStream_t s)
+{
+    PROF_BEGIN("k_wall_shear", s);
+    k_wall_shear_partial<T><<<nblk, RED_THREADS, 0, s>>>(U, nut, md, d, nu, part, nblk);
+    k_wall_shear_final<<<1, RED_THREADS, 0, s>>>(part, nblk, (double)(d.nx - 2) * (double)d.nzp, out);
+    PROF_END(s);
+}
+
 // ------------------------------------------------------- instantiations
 #define CHAN_INST(T)                                                                                            \
     template void launch_bc<T>(T*, const Dims&, int, hipStream_t);                                              \
@@ -521,7 +562,9 @@ void launch_plane_stats(const T* U, const T* V, const T* W, const Metric<double>
     template void launch_divergence_check<T>(const T*, const T*, const T*, const Metric<double>&, const Dims&, \
                                              double*, int, Scalars*, hipStream_t);                              \
     template void launch_plane_stats<T>(const T*, const T*, const T*, const Metric<double>&, const Dims&,       \
-                                        double*, hipStream_t);
+                                        double*, hipStream_t);                                                  \
+    template void launch_wall_shear<T>(const T*, const T*, const Metric<double>&, const Dims&, double, double*, \
+                                       int, double*, hipStream_t);
 
 CHAN_INST(double)
 CHAN_INST(float)

@@ -114,6 +114,13 @@ int chan_compute_equilibrium_wall_stress(chan_ctx* ctx, double h_wm, double kapp
  * j=1..ny of U, V, W, U^2, V^2, W^2, UV (out: 7*ny doubles). */
 int chan_plane_statistics(chan_ctx* ctx, double* out);
 
+/* code/statistics.f90:95-130: mean wall shear (nu + nu_t) dU/dy over each
+ * wall (nu_t of the context, as left by the last step), u_tau =
+ * sqrt((tau_b + tau_t) / Ly) and Re_tau = u_tau Ly / 2 / nu.  The reference
+ * sums the bottom wall twice; the top wall uses its own rows here.
+ * out: tau_bottom, tau_top, u_tau, Re_tau. */
+int chan_wall_shear(chan_ctx* ctx, double out4[4]);
+
 /* code/app_main.f90:108-118: nsteps x (compute_dt; compute_time_step_RK3)
  * without host synchronisation. */
 int chan_run_steps(chan_ctx* ctx, int nsteps);

@@ -64,6 +64,7 @@ def lib():
         L.orc_loglaw_utau.restype = ctypes.c_double
         L.orc_equilibrium_alpha.argtypes = [vp, _D, ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.orc_plane_stats.argtypes = [vp, _D]
+        L.orc_wall_shear.argtypes = [vp, _D]
         L.orc_set_wall_model.argtypes = [vp, ctypes.c_int] + [ctypes.c_double] * 3
         _LIB = L
     return _LIB
@@ -226,6 +227,12 @@ class Oracle:
         out = np.zeros(7 * self.ny)
         lib().orc_plane_stats(self._h, _p(out))
         return out.reshape(7, self.ny)
+
+    def wall_shear(self):
+        """dict(tau_bottom, tau_top, u_tau, Re_tau) from U and nu_t."""
+        out = np.zeros(4)
+        lib().orc_wall_shear(self._h, _p(out))
+        return dict(tau_bottom=out[0], tau_top=out[1], u_tau=out[2], Re_tau=out[3])
 
 
 def fft(a, sign=-1):

@@ -1273,6 +1273,31 @@ void orc_equilibrium_alpha(orc* o, const double* U, double h_wm, double kappa, d
  * code/statistics.f90:52-64: per row j = 1..ny the means of U, V, W, their
  * squares, and <u v> with u interpolated to V's position (y-weights, then the
  * x average).  out = [U, V, W, U2, V2, W2, UV] x ny. */
+/* code/statistics.f90:95-130 with the top wall on its own rows (the
+ * reference evaluates the bottom wall twice): out = tau_b, tau_t, u_tau,
+ * Re_tau, from U and the oracle's nu_t. */
+void orc_wall_shear(const orc* o, double* out)
+{
+    int nx = o->nx, ny = o->ny, nyg = o->nyg, nzg = o->nzg;
+    const double *U = o->U, *nu_t = o->nu_t, *yg = o->yg, *w0 = o->w0, *w1 = o->w1;
+    double sb = 0.0, st = 0.0;
+    for (int k = 2; k <= nzg - 2; k++)
+        for (int i = 2; i <= nx - 1; i++) {
+#define TX(j) (0.5 * (AC(nu_t, i, j, k) + AC(nu_t, i + 1, j, k)))
+            double nb = w0[1] * TX(1) + w1[1] * TX(2);
+            double nt = w0[ny] * TX(nyg - 1) + w1[ny] * TX(nyg);
+#undef TX
+            sb += (AU(U, i, 2, k) - AU(U, i, 1, k)) / (yg[2] - yg[1]) * (o->nu + nb);
+            st += (AU(U, i, nyg - 1, k) - AU(U, i, nyg, k)) / (yg[nyg] - yg[nyg - 1]) * (o->nu + nt);
+        }
+    double nq = (double)(nx - 2) * (double)o->nzp;
+    double Ly = o->y[ny] - o->y[1];
+    out[0] = sb / nq;
+    out[1] = st / nq;
+    out[2] = sqrt((out[0] + out[1]) / Ly);
+    out[3] = out[2] * Ly / 2.0 / o->nu;
+}
+
 void orc_plane_stats(const orc* o, double* out)
 {
     int nx = o->nx, ny = o->ny, nxg = o->nxg, nzg = o->nzg, nz = o->nz;

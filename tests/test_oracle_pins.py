@@ -461,6 +461,37 @@ def test_amd_wall_offset_ghosts():
     assert np.allclose(nut[:, -1, :], -nut[:, -2, :] + off)
 
 
+# ------------------------------------------------------------ statistics
+
+def test_wall_shear_of_poiseuille_flow():
+    # U = A y (2 - y) evaluated at every centre (ghost rows included): the
+    # centred difference across each wall is exact for a quadratic, so
+    # tau = 2 A nu at both walls, u_tau = sqrt(2 A nu), Re_tau = u_tau / nu
+    A, nu = 1.7, 0.01
+    o = mk(10, 13, 10, nu=nu)
+    U, V, W = fields(o, lambda x, y, z: A * y * (2 - y) + 0 * x, zero, zero)
+    o.set_state(U, V, W)
+    o.set(oracle.NUT, np.zeros(o.shape(oracle.NUT), order="F"))
+    w = o.wall_shear()
+    assert abs(w["tau_bottom"] - 2 * A * nu) < 1e-13 and abs(w["tau_top"] - 2 * A * nu) < 1e-13
+    assert abs(w["u_tau"] - np.sqrt(2 * A * nu)) < 1e-13
+    assert abs(w["Re_tau"] - np.sqrt(2 * A * nu) / nu) < 1e-10
+    # an eddy viscosity constant in the wall rows adds in proportion
+    o.set(oracle.NUT, np.full(o.shape(oracle.NUT), 0.5 * nu, order="F"))
+    assert abs(o.wall_shear()["tau_bottom"] - 2 * A * 1.5 * nu) < 1e-13
+
+
+def test_plane_statistics_of_known_fields():
+    o = mk(10, 9, 10)
+    U, V, W = fields(o, lambda x, y, z: 2.0 + np.sin(x) + 0 * y, lambda x, y, z: 0.5 + 0 * x * y * z,
+                     lambda x, y, z: np.cos(2 * z) + 0 * x * y)
+    o.set_state(U, V, W)
+    st = o.plane_stats()
+    # sin over the unique x faces and cos(2z) over the unique z faces average to 0
+    assert np.allclose(st[0], 2.0) and np.allclose(st[1], 0.5) and np.allclose(st[2], 0.0, atol=1e-14)
+    assert np.allclose(st[3], 4.0 + 0.5) and np.allclose(st[4], 0.25) and np.allclose(st[5], 0.5)
+
+
 # ------------------------------------------------------------ wall model
 
 def test_wall_model_identity():
