@@ -35,7 +35,7 @@ EXPORTS = [
     "chan_mlp_set_params", "chan_mlp_fit_normalization", "chan_mlp_forward", "chan_mlp_gradients",
     "chan_mlp_train", "chan_wall_samples", "chan_set_wall_network", "chan_wall_model_stats", "chan_evaluate_wall_stress",
     "chan_ensemble_create", "chan_ensemble_destroy", "chan_ensemble_last_error", "chan_ensemble_size",
-    "chan_ensemble_run_steps", "chan_ensemble_synchronize", "chan_wall_shear",
+    "chan_ensemble_run_steps", "chan_ensemble_synchronize", "chan_wall_shear", "chan_rl_observe", "chan_rl_act",
 ]
 KERNEL_RHS = 0
 WALL_MLP = 2
@@ -125,6 +125,8 @@ def load_library(path=LIB_PATH):
     L.chan_ensemble_run_steps.argtypes = [vp, ctypes.c_int]
     L.chan_ensemble_synchronize.argtypes = [vp]
     L.chan_wall_shear.argtypes = [vp, _D]
+    L.chan_rl_observe.argtypes = [vp, ctypes.c_int, ctypes.c_double, _D, _D, _D]
+    L.chan_rl_act.argtypes = [vp, ctypes.c_int, _D]
     _LIB = L
     return L
 
@@ -425,6 +427,19 @@ class Channel:
         out = np.zeros(4)
         self._chk(load_library().chan_wall_shear(self._h, _p(out)), "wall_shear")
         return dict(tau_bottom=out[0], tau_top=out[1], u_tau=out[2], Re_tau=out[3])
+
+    def rl_observe(self, nagents, dPdx_ref, draws):
+        """States (2, nagents, 6) and rewards (2, nagents) of the RL agents;
+        draws (2, nagents, 2) uniform numbers."""
+        d = np.ascontiguousarray(draws, dtype=np.float64).reshape(2 * nagents * 2)
+        st = np.zeros(2 * nagents * 6)
+        rw = np.zeros(2 * nagents)
+        self._chk(load_library().chan_rl_observe(self._h, nagents, dPdx_ref, _p(d), _p(st), _p(rw)), "rl_observe")
+        return st.reshape(2, nagents, 6), rw.reshape(2, nagents)
+
+    def rl_act(self, nagents, factors):
+        f = np.ascontiguousarray(factors, dtype=np.float64).reshape(2 * nagents)
+        self._chk(load_library().chan_rl_act(self._h, nagents, _p(f)), "rl_act")
 
     def evaluate_wall_stress(self):
         """Evaluate the selected wall-stress source once into alpha_x."""

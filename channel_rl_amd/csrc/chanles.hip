@@ -897,6 +897,53 @@ int chan_wall_shear(chan_ctx* c, double out4[4])
     return CHAN_OK;
 }
 
+static int rl_check(chan_ctx* c, int na)
+{
+    if (na < 1 || (c->d.nx - 2) % na != 0) return fail(c, CHAN_EINVAL, "nagents must divide nx-2");
+    if (c->d.nyg < 6) return fail(c, CHAN_EINVAL, "the agents sample rows up to 5: ny >= 5 needed");
+    if ((long)22 * na > c->d.nbox) return fail(c, CHAN_EINVAL, "too many agents for the staging area");
+    return CHAN_OK;
+}
+
+int chan_rl_observe(chan_ctx* c, int nagents, double dPdx_ref, const double* draws, double* states, double* rewards)
+{
+    if (!c || !draws || !states || !rewards) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc || (rc = rl_check(c, nagents))) return rc;
+    const int na = nagents, nxa = (c->d.nx - 2) / na;
+    double* dd = c->stage;
+    double* ds = dd + 4L * na;
+    double* dr = ds + 12L * na;
+    HIPCHK(c, hipMemcpyAsync(dd, draws, 4L * na * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    void** s = c->state[c->cur];
+    if (c->prec == CHAN_FP64)
+        launch_rl_observe<double>((double*)s[0], (double*)c->nut, c->md, c->d, c->nu, na, nxa, dPdx_ref, dd, ds, dr,
+                                  c->stream);
+    else
+        launch_rl_observe<float>((float*)s[0], (float*)c->nut, c->md, c->d, c->nu, na, nxa, dPdx_ref, dd, ds, dr,
+                                 c->stream);
+    if ((rc = check_launch(c))) return rc;
+    HIPCHK(c, hipMemcpyAsync(states, ds, 12L * na * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(rewards, dr, 2L * na * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHAN_OK;
+}
+
+int chan_rl_act(chan_ctx* c, int nagents, const double* factors)
+{
+    if (!c || !factors) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc || (rc = rl_check(c, nagents))) return rc;
+    const int na = nagents, nxa = (c->d.nx - 2) / na;
+    double* df = c->stage + 18L * na;
+    double* da = df + 2L * na;
+    HIPCHK(c, hipMemcpyAsync(df, factors, 2L * na * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    launch_rl_actions(c->alpha, df, c->d, na, nxa, da, c->stream);
+    if ((rc = check_launch(c))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CHAN_OK;
+}
+
 int chan_run_steps(chan_ctx* c, int nsteps)
 {
     if (!c || nsteps < 0) return CHAN_EINVAL;

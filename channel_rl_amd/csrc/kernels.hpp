@@ -86,6 +86,13 @@ template <typename T>
 void launch_wall_shear(const T* U, const T* nut, const Metric<double>& md, const Dims& d, double nu, double* part,
                        int nblk, double* out, hipStream_t s);
 
+// the RL agents' observations and actions (rl.hip)
+template <typename T>
+void launch_rl_observe(const T* U, const T* nut, const Metric<double>& m, const Dims& d, double nu, int na, int nxa,
+                       double dPdx_ref, const double* draws, double* states, double* rewards, hipStream_t s);
+void launch_rl_actions(double* alpha, const double* factors, const Dims& d, int na, int nxa, double* acts,
+                       hipStream_t s);
+
 template <typename T>
 void launch_scale_shift(T* F, const Dims& d, hipStream_t s);
 

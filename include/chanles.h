@@ -121,6 +121,23 @@ int chan_plane_statistics(chan_ctx* ctx, double* out);
  * out: tau_bottom, tau_top, u_tau, Re_tau. */
 int chan_wall_shear(chan_ctx* ctx, double out4[4]);
 
+/* code/smarties_stat.f90:24-178 on the device: the states (6 per agent and
+ * wall) and rewards of nagents agents per wall, agent a at x face
+ * a*(nx-2)/nagents + 2, z centre 2, from U and nu_t of the current state.
+ * draws (2*nagents*2): per (wall, agent), wall-major, the two uniform numbers
+ * the reference draws with RANDOM_NUMBER: the first < 1/2 picks sampling row
+ * 3, else 4; the second weighs that row against the next.  states
+ * [wall][agent][6] = (dU l / u_tau, U/u_tau - dU l / u_tau ln(l u_tau / nu),
+ * U, l, u_tau, dPdx_ref); rewards [wall][agent].  nagents must divide
+ * nx-2.  Replaces the per-step host copy of the wall planes. */
+int chan_rl_observe(chan_ctx* ctx, int nagents, double dPdx_ref, const double* draws, double* states,
+                    double* rewards);
+/* code/smarties_stat.f90:190,213,231-258: actions = alpha_x at each agent
+ * times factors [wall][agent], interpolated linearly in x between agents
+ * (periodic), x images, then the z interpolation of the reference, into
+ * alpha_x on the device. */
+int chan_rl_act(chan_ctx* ctx, int nagents, const double* factors);
+
 /* code/app_main.f90:108-118: nsteps x (compute_dt; compute_time_step_RK3)
  * without host synchronisation. */
 int chan_run_steps(chan_ctx* ctx, int nsteps);

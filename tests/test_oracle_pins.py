@@ -492,6 +492,47 @@ def test_plane_statistics_of_known_fields():
     assert np.allclose(st[3], 4.0 + 0.5) and np.allclose(st[4], 0.25) and np.allclose(st[5], 0.5)
 
 
+# ------------------------------------------------------------ RL agents
+
+def test_rl_states_of_a_linear_profile():
+    from oracle import rl
+    s_, nu, dref = 3.0, 1e-3, 0.01
+    o = mk(18, 13, 10, nu=nu)
+    U, V, W = fields(o, lambda x, y, z: s_ * np.where(y < 1.0, y, 2.0 - y) + 0 * x, zero, zero)
+    nut = np.zeros((o.nxg, o.nyg, o.nzg))
+    yg = np.concatenate([[0.0], centres(o.y)])
+    draws = np.array([[[0.2, 0.3], [0.7, 0.9]], [[0.4, 0.5], [0.6, 0.1]]])
+    st, rw = rl.observe(U, nut, yg, nu, 2, dref, draws)
+    ut = np.sqrt(nu * s_)
+    for w in range(2):
+        for a in range(2):
+            yind = 3 if draws[w, a, 0] < 0.5 else 4
+            yr = draws[w, a, 1]
+            l = yr * yg[yind] + (1 - yr) * yg[yind + 1]   # distance from the wall (symmetric grid)
+            assert abs(st[w, a, 3] - l) < 1e-13 and abs(st[w, a, 4] - ut) < 1e-13
+            assert abs(st[w, a, 2] - s_ * l) < 1e-12            # linear interpolation of a linear profile
+            assert abs(st[w, a, 0] - s_ * l / ut) < 1e-10
+            assert abs(st[w, a, 1] - (s_ * l / ut - s_ * l / ut * np.log(l * ut / nu))) < 1e-9
+            assert abs(rw[w, a] - (100 - 5000 * abs(ut - 0.1) / 0.1)) < 1e-9
+
+
+def test_rl_actions_interpolate_between_agents():
+    from oracle import rl
+    nx, nzg = 14, 11
+    alpha = np.full((nx, 2, nzg), 0.5)
+    f = np.array([[1.0, 2.0, 4.0], [0.5, 1.5, 1.0]])
+    out = rl.act(alpha, 3, f)
+    nxa = (nx - 2) // 3
+    for w in range(2):
+        for a in range(3):
+            # at each agent's face the action itself; halfway to the next, the mean
+            assert np.allclose(out[a * nxa + 1, w, :], 0.5 * f[w, a], rtol=1e-15)
+        assert np.allclose(out[0, w], out[nx - 2, w]) and np.allclose(out[nx - 1, w], out[1, w])
+    # equal factors: uniform alpha scaled
+    out = rl.act(alpha, 3, np.full((2, 3), 1.2))
+    assert np.allclose(out, 0.6, rtol=1e-15)
+
+
 # ------------------------------------------------------------ wall model
 
 def test_wall_model_identity():
