@@ -36,6 +36,7 @@ EXPORTS = [
     "chan_mlp_train", "chan_wall_samples", "chan_set_wall_network", "chan_wall_model_stats", "chan_evaluate_wall_stress",
     "chan_ensemble_create", "chan_ensemble_destroy", "chan_ensemble_last_error", "chan_ensemble_size",
     "chan_ensemble_run_steps", "chan_ensemble_synchronize", "chan_wall_shear", "chan_rl_observe", "chan_rl_act",
+    "chan_compute_pressure",
 ]
 KERNEL_RHS = 0
 WALL_MLP = 2
@@ -127,6 +128,7 @@ def load_library(path=LIB_PATH):
     L.chan_wall_shear.argtypes = [vp, _D]
     L.chan_rl_observe.argtypes = [vp, ctypes.c_int, ctypes.c_double, _D, _D, _D]
     L.chan_rl_act.argtypes = [vp, ctypes.c_int, _D]
+    L.chan_compute_pressure.argtypes = [vp]
     _LIB = L
     return L
 
@@ -427,6 +429,11 @@ class Channel:
         out = np.zeros(4)
         self._chk(load_library().chan_wall_shear(self._h, _p(out)), "wall_shear")
         return dict(tau_bottom=out[0], tau_top=out[1], u_tau=out[2], Re_tau=out[3])
+
+    def compute_pressure(self):
+        """The true pressure of the current state (code/pressure.f90:21)."""
+        self._chk(load_library().chan_compute_pressure(self._h), "compute_pressure")
+        return self.get_field(P)
 
     def rl_observe(self, nagents, dPdx_ref, draws):
         """States (2, nagents, 6) and rewards (2, nagents) of the RL agents;

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 FSRC = os.path.join(HERE, "fortran")
 LIB = os.path.join(HERE, "libchanles.so")
 
-HIP_SOURCES = ["chanles.hip", "stencil.hip", "poisson.hip", "dsm_fused.hip", "rhs.hip", "mlp.hip", "amd.hip", "rl.hip"]
+HIP_SOURCES = ["chanles.hip", "stencil.hip", "poisson.hip", "dsm_fused.hip", "rhs.hip", "mlp.hip", "amd.hip", "rl.hip", "pressure.hip"]
 
 # gfx950 only; FP64 stencils keep the reference's operation order, so no FMA
 # contraction; packed FP32 instructions are kept off.

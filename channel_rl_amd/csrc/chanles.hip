@@ -113,6 +113,7 @@ struct chan_ctx {
     double* wm_stats = nullptr;   // per 16-cell group: error sums, fallbacks
     float* wm_samples = nullptr;  // 2*nx*nzg x (4 + 2) floats, on demand
     long generation = 0;          // bumped whenever captured steps go stale
+    double* pbc = nullptr;        // wall data of the true pressure, 2 planes nxg x nzg
     Prof prof;
     std::vector<std::pair<std::string, std::pair<double, long>>> prof_out;
 };
@@ -734,6 +735,33 @@ int chan_solve_poisson_equation(chan_ctx* c)
     launch_poisson_forward_field(c->pb.p, c->d, c->pb, c->stream);
     launch_poisson_solve(c->d, c->pb, c->stream);
     launch_poisson_inverse(c->d, c->pb, c->stream);
+    return check_launch(c);
+}
+
+extern "C++" template <typename T>
+void enqueue_pressure(chan_ctx* c)
+{
+    T** S = (T**)c->state[c->cur];
+    T** R = (T**)c->rhs;
+    launch_rhs_rk<T>(S[0], S[1], S[2], (const T*)c->nut, rhs_tabs<T>(c), c->d, T(c->nu), c->sc, 0, nullptr, nullptr,
+                     nullptr, nullptr, nullptr, nullptr, R[0], R[1], R[2], 0.0, 0.0, c->rhs_shape, c->stream);
+    double* bc1 = c->pbc;
+    double* bc2 = c->pbc + (long)c->d.nxg * c->d.nzg;
+    launch_pressure_rhs<T>(S[1], (const T*)c->nut, R[0], R[1], R[2], c->md, c->d, c->nu, bc1, bc2, c->pb.p,
+                           c->stream);
+    launch_poisson_forward_field(c->pb.p, c->d, c->pb, c->stream);
+    launch_poisson_solve(c->d, c->pb, c->stream);
+    launch_poisson_inverse(c->d, c->pb, c->stream);
+    launch_pressure_ghosts(c->pb.p, bc1, bc2, c->md, c->d, c->stream);
+}
+
+int chan_compute_pressure(chan_ctx* c)
+{
+    if (!c) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    if (!c->pbc && (rc = dalloc(c, (void**)&c->pbc, (size_t)2 * c->d.nxg * c->d.nzg * sizeof(double)))) return rc;
+    DISPATCH(c, enqueue_pressure, c);
     return check_launch(c);
 }
 

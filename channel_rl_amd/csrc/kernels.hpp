@@ -86,6 +86,13 @@ template <typename T>
 void launch_wall_shear(const T* U, const T* nut, const Metric<double>& md, const Dims& d, double nu, double* part,
                        int nblk, double* out, hipStream_t s);
 
+// the true pressure (pressure.hip)
+template <typename T>
+void launch_pressure_rhs(const T* V, const T* nut, const T* Ru, const T* Rv, const T* Rw, const Metric<double>& m,
+                         const Dims& d, double nu, double* bc1, double* bc2, double* P, hipStream_t s);
+void launch_pressure_ghosts(double* P, const double* bc1, const double* bc2, const Metric<double>& m, const Dims& d,
+                            hipStream_t s);
+
 // the RL agents' observations and actions (rl.hip)
 template <typename T>
 void launch_rl_observe(const T* U, const T* nut, const Metric<double>& m, const Dims& d, double nu, int na, int nxa,

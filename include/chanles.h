@@ -88,6 +88,12 @@ int chan_compute_rhs(chan_ctx* ctx);
 int chan_compute_projection_step(chan_ctx* ctx);
 /* code/projection.f90:74 solve_poisson_equation on P in place */
 int chan_solve_poisson_equation(chan_ctx* ctx);
+/* code/pressure.f90:21 compute_pressure: the true pressure of the current
+ * U, V, W and nu_t into CHAN_P (Laplacian(p) = div of the momentum
+ * right-hand sides, dP/dy = d/dy((nu + nu_t) dV/dy) at the walls), with
+ * every periodic image and wall ghost row set.  The next projection
+ * overwrites CHAN_P with the pseudo-pressure. */
+int chan_compute_pressure(chan_ctx* ctx);
 /* code/mass_flow.f90:24 compute_mean_mass_flow_U(U,Q) */
 int chan_compute_mean_mass_flow_U(chan_ctx* ctx, double* Q);
 /* code/initialization.f90:442: Qflow_x_0 = compute_mean_mass_flow_U(U) */

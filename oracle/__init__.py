@@ -65,6 +65,7 @@ def lib():
         L.orc_equilibrium_alpha.argtypes = [vp, _D, ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.orc_plane_stats.argtypes = [vp, _D]
         L.orc_wall_shear.argtypes = [vp, _D]
+        L.orc_compute_pressure.argtypes = [vp, _D]
         L.orc_set_wall_model.argtypes = [vp, ctypes.c_int] + [ctypes.c_double] * 3
         _LIB = L
     return _LIB
@@ -227,6 +228,12 @@ class Oracle:
         out = np.zeros(7 * self.ny)
         lib().orc_plane_stats(self._h, _p(out))
         return out.reshape(7, self.ny)
+
+    def compute_pressure(self):
+        """The pressure (nxg, nyg, nzg) of the oracle's U, V, W and nu_t."""
+        P = self.zeros(P_)
+        lib().orc_compute_pressure(self._h, _p(P))
+        return P
 
     def wall_shear(self):
         """dict(tau_bottom, tau_top, u_tau, Re_tau) from U and nu_t."""

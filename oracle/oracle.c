@@ -996,7 +996,31 @@ int orc_zgtsv(int n, double* dl, double* d, double* du, double* b)
 /* code/projection.f90:74-128 with code/projection.f90:133-151 collapsed to
  * the periodic images P(nxg-1)=P(2), P(:,:,nzg-1)=P(:,:,2).  rhs_p is a
  * centre box; in: divergence on (2:nxg-1,2:nyg-1,2:nzg-1); out: pressure. */
-void orc_solve_poisson_equation(orc* o, double* rhs_p)
+/* 2-D forward transform of a plane over the unique points (i,k) = (2..nxp+1,
+ * 2..nzp+1) of an (nxg x nzg) array, into hat[kz*nxp + kx] */
+static void plane_fft(const orc* o, const double* F, cplx* hat, cplx* line, cplx* lout, cplx* work, const cplx* twx,
+                      const cplx* twz)
+{
+    int nxp = o->nxp, nzp = o->nzp, nxg = o->nxg;
+    for (int k = 0; k < nzp; k++) {
+        for (int i = 0; i < nxp; i++) line[i] = F[(size_t)(i + 1) + (size_t)nxg * (k + 1)];
+        fft_rec(nxp, line, 1, lout, twx, 1, nxp, work);
+        for (int i = 0; i < nxp; i++) hat[(size_t)k * nxp + i] = lout[i];
+    }
+    for (int i = 0; i < nxp; i++) {
+        for (int k = 0; k < nzp; k++) line[k] = hat[(size_t)k * nxp + i];
+        fft_rec(nzp, line, 1, lout, twz, 1, nzp, work);
+        for (int k = 0; k < nzp; k++) hat[(size_t)k * nxp + i] = lout[k];
+    }
+}
+
+static void poisson_core(orc* o, double* rhs_p, const double* bc1, const double* bc2);
+
+void orc_solve_poisson_equation(orc* o, double* rhs_p) { poisson_core(o, rhs_p, 0, 0); }
+
+/* code/projection.f90:74-128; with bc1/bc2 (nxg x nzg planes, 1-based) also
+ * the inhomogeneous Neumann rows of code/pressure.f90:134-139 */
+static void poisson_core(orc* o, double* rhs_p, const double* bc1, const double* bc2)
 {
     int nxp = o->nxp, nzp = o->nzp, nyg = o->nyg, nxg = o->nxg, nzg = o->nzg;
     int nr = nyg - 2;
@@ -1021,12 +1045,29 @@ void orc_solve_poisson_equation(orc* o, double* rhs_p)
     }
     cplx *D = (cplx*)malloc(sizeof(cplx) * nr), *DL = (cplx*)malloc(sizeof(cplx) * nr),
          *DU = (cplx*)malloc(sizeof(cplx) * nr), *B = (cplx*)malloc(sizeof(cplx) * nr);
+    cplx *b1h = 0, *b2h = 0;
+    double cb1 = 0.0, cb2 = 0.0;
+    if (bc1 && bc2) {
+        b1h = (cplx*)malloc(sizeof(cplx) * plane);
+        b2h = (cplx*)malloc(sizeof(cplx) * plane);
+        plane_fft(o, bc1, b1h, line, lout, work, twxf, twzf);
+        plane_fft(o, bc2, b2h, line, lout, work, twxf, twzf);
+        /* coef_bc_1 = c at j = 2, coef_bc_2 = a at j = nyg-1
+         * (code/initialization.f90:370-385) */
+        const double *y = o->y, *yg = o->yg;
+        cb1 = 1.0 / (y[2] - y[1]) / (yg[2] - yg[1]);
+        cb2 = 1.0 / (y[nyg - 1] - y[nyg - 2]) / (yg[nyg] - yg[nyg - 1]);
+    }
     for (int kz = 0; kz < nzp; kz++)
         for (int kx = 0; kx < nxp; kx++) {
             for (int j = 2; j <= nyg - 1; j++) D[j - 2] = (o->dyy_d[j] + o->kxx[kx]) + o->kzz[kz];
             for (int j = 2; j <= nyg - 2; j++) { DL[j - 2] = o->dyy_l[j]; DU[j - 2] = o->dyy_u[j]; }
             if (kx == 0 && kz == 0) D[0] = 3.0 / 2.0 * D[0];
             for (int j = 0; j < nr; j++) B[j] = hat[plane * j + (size_t)kz * nxp + kx];
+            if (b1h) {
+                B[0] = B[0] + cb1 * (o->yg[2] - o->yg[1]) * b1h[(size_t)kz * nxp + kx];
+                B[nr - 1] = B[nr - 1] - cb2 * (o->yg[nyg] - o->yg[nyg - 1]) * b2h[(size_t)kz * nxp + kx];
+            }
             zgtsv_c(nr, DL, D, DU, B);
             for (int j = 0; j < nr; j++) hat[plane * j + (size_t)kz * nxp + kx] = B[j];
         }
@@ -1051,6 +1092,98 @@ void orc_solve_poisson_equation(orc* o, double* rhs_p)
         for (int i = 2; i <= nxg - 1; i++) AC(rhs_p, i, j, nzg - 1) = AC(rhs_p, i, j, 2);
     free(twxf); free(twzf); free(twxb); free(twzb);
     free(hat); free(line); free(lout); free(work); free(D); free(DL); free(DU); free(B);
+    free(b1h); free(b2h);
+}
+
+/* code/pressure.f90:236-340 compute_pressure_BC: bc = d/dy((nu + nu_t) dV/dy)
+ * at each wall, one-sided second-order differences with metric factors;
+ * planes (nxg x nzg), set on i = 2..nxg-1, k = 2..nzg-1. */
+static void pressure_bc(const orc* o, const double* V, const double* nut, double* bc1, double* bc2)
+{
+    int nxg = o->nxg, nyg = o->nyg, ny = o->ny, nzg = o->nzg;
+    const double *y = o->y, *yg = o->yg, nu = o->nu, dzeta = 1.0;
+    double g = (-3.0 * y[1] + 4.0 * y[2] - y[3]) / (2.0 * dzeta);
+    double g2 = (2.0 * y[1] - 5.0 * y[2] + 4.0 * y[3] - 1.0 * y[4]) / (dzeta * dzeta);
+    for (int k = 2; k <= nzg - 1; k++)
+        for (int i = 2; i <= nxg - 1; i++) {
+            double nui = nu + 0.5 * (AC(nut, i, 1, k) + AC(nut, i, 2, k));
+            double dnu = (AC(nut, i, 2, k) - AC(nut, i, 1, k)) / (yg[2] - yg[1]);
+            double dV = (-3.0 * AV(V, i, 1, k) + 4.0 * AV(V, i, 2, k) - AV(V, i, 3, k)) / (2.0 * dzeta) * 1.0 / g;
+            double ddV = (2.0 * AV(V, i, 1, k) - 5.0 * AV(V, i, 2, k) + 4.0 * AV(V, i, 3, k) - 1.0 * AV(V, i, 4, k)) /
+                         (dzeta * dzeta);
+            ddV = (ddV - dV * g2) / (g * g);
+            bc1[(size_t)(i - 1) + (size_t)nxg * (k - 1)] = dnu * dV + nui * ddV;
+        }
+    g = (3.0 * y[ny] - 4.0 * y[ny - 1] + y[ny - 2]) / (2.0 * dzeta);
+    g2 = (2.0 * y[ny] - 5.0 * y[ny - 1] + 4.0 * y[ny - 2] - 1.0 * y[ny - 3]) / (dzeta * dzeta);
+    for (int k = 2; k <= nzg - 1; k++)
+        for (int i = 2; i <= nxg - 1; i++) {
+            double nui = nu + 0.5 * (AC(nut, i, nyg, k) + AC(nut, i, nyg - 1, k));
+            double dnu = (AC(nut, i, nyg, k) - AC(nut, i, nyg - 1, k)) / (yg[nyg] - yg[nyg - 1]);
+            double dV = (3.0 * AV(V, i, ny, k) - 4.0 * AV(V, i, ny - 1, k) + AV(V, i, ny - 2, k)) / (2.0 * dzeta) *
+                        1.0 / g;
+            double ddV = (2.0 * AV(V, i, ny, k) - 5.0 * AV(V, i, ny - 1, k) + 4.0 * AV(V, i, ny - 2, k) -
+                          1.0 * AV(V, i, ny - 3, k)) / (dzeta * dzeta);
+            ddV = (ddV - dV * g2) / (g * g);
+            bc2[(size_t)(i - 1) + (size_t)nxg * (k - 1)] = dnu * dV + nui * ddV;
+        }
+}
+
+/* code/pressure.f90:21-176 compute_pressure: the pressure of the current
+ * U, V, W and nu_t from Laplacian(p) = div(rhs_u, rhs_v, rhs_w) with the
+ * wall conditions dP/dy = d/dy(nu dV/dy).  The ghost planes follow the
+ * whole-domain periodic images (x: 1 <- nxg-2, nxg-1 <- 2, nxg <- 3; z the
+ * same), and the wall ghost rows P(1) = P(2) - dy bc_1, P(nyg) = P(nyg-1) +
+ * dy bc_2 on the interior (i, k) as the reference intends. */
+void orc_compute_pressure(orc* o, double* P)
+{
+    int nx = o->nx, ny = o->ny, nz = o->nz, nxg = o->nxg, nyg = o->nyg, nzg = o->nzg;
+    const double *x = o->x, *y = o->y, *z = o->z, *yg = o->yg;
+    size_t nc = nC(o);
+    double* ru = dalloc(nc);
+    double* rv = dalloc(nc);
+    double* rw = dalloc(nc);
+    double* bc1 = dalloc((size_t)nxg * nzg);
+    double* bc2 = dalloc((size_t)nxg * nzg);
+    pressure_bc(o, o->V, o->nu_t, bc1, bc2);
+    /* rhs of u, v, w on their interior ranges (code/equations.f90), in the
+     * fields' own shapes */
+    orc_compute_rhs_u(o, o->U, o->V, o->W, ru);
+    orc_compute_rhs_v(o, o->U, o->V, o->W, rv);
+    orc_compute_rhs_w(o, o->U, o->V, o->W, rw);
+#define RU(i, j, k) ru[IX(i, j, k, nx, nyg)]
+#define RV(i, j, k) rv[IX(i, j, k, nxg, ny)]
+#define RW(i, j, k) rw[IX(i, j, k, nxg, nyg)]
+    /* boundaries: x-periodic rhs_u (faces), wall rows of rhs_v from bc,
+     * z-periodic rhs_w (faces) */
+    for (int k = 1; k <= nzg; k++)
+        for (int j = 1; j <= nyg; j++) { RU(1, j, k) = RU(nx - 1, j, k); RU(nx, j, k) = RU(2, j, k); }
+    for (int k = 2; k <= nzg - 1; k++)
+        for (int i = 2; i <= nxg - 1; i++) {
+            RV(i, 1, k) = bc1[(size_t)(i - 1) + (size_t)nxg * (k - 1)];
+            RV(i, ny, k) = bc2[(size_t)(i - 1) + (size_t)nxg * (k - 1)];
+        }
+    for (int j = 1; j <= nyg; j++)
+        for (int i = 1; i <= nxg; i++) { RW(i, j, 1) = RW(i, j, nz - 1); RW(i, j, nz) = RW(i, j, 2); }
+    memset(P, 0, nc * sizeof(double));
+    for (int k = 2; k <= nzg - 1; k++)
+        for (int j = 2; j <= nyg - 1; j++)
+            for (int i = 2; i <= nxg - 1; i++)
+                AC(P, i, j, k) = (RU(i, j, k) - RU(i - 1, j, k)) / (x[i] - x[i - 1]) +
+                                 (RV(i, j, k) - RV(i, j - 1, k)) / (y[j] - y[j - 1]) +
+                                 (RW(i, j, k) - RW(i, j, k - 1)) / (z[k] - z[k - 1]);
+#undef RU
+#undef RV
+#undef RW
+    poisson_core(o, P, bc1, bc2);
+    for (int k = 2; k <= nzg - 1; k++)
+        for (int i = 2; i <= nxg - 1; i++) {
+            AC(P, i, 1, k) = AC(P, i, 2, k) - (yg[2] - yg[1]) * bc1[(size_t)(i - 1) + (size_t)nxg * (k - 1)];
+            AC(P, i, nyg, k) = AC(P, i, nyg - 1, k) + (yg[nyg] - yg[nyg - 1]) * bc2[(size_t)(i - 1) + (size_t)nxg * (k - 1)];
+        }
+    periodic_x(P, nxg, nyg, nzg, 0);
+    periodic_z(P, nxg, nyg, nzg, 0);
+    free(ru); free(rv); free(rw); free(bc1); free(bc2);
 }
 
 /* code/projection.f90:20-55 and :192-214 */

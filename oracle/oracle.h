@@ -47,6 +47,7 @@ void   orc_compute_rhs_u(const orc* o, const double* U, const double* V, const d
 void   orc_compute_rhs_v(const orc* o, const double* U, const double* V, const double* W, double* rhs);
 void   orc_compute_rhs_w(const orc* o, const double* U, const double* V, const double* W, double* rhs);
 void   orc_wall_shear(const orc* o, double* out4);
+void   orc_compute_pressure(orc* o, double* P);
 void   orc_compute_eddy_viscosity(orc* o, const double* U, const double* V, const double* W, double* nu_t);
 void   orc_compute_wall_model(orc* o, const double* U, double* nu_t);
 void   orc_compute_projection_step(orc* o);

@@ -228,3 +228,17 @@ def test_rk3_with_amd(gpu):
     case, o, g = run_pair(18, 17, 18, 3, LES_model=3)
     for a, b in zip(g.state(), o.state()):
         assert rel(a, b) <= 1e-10, rel(a, b)
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_true_pressure_matches_oracle(gpu, n):
+    case, o, g, (U, V, W) = make_pair(*n)
+    nut = random_nut(case, 4)
+    o.set(NU_T, nut)
+    g.set_field(NU_T, nut)
+    o.set_scalars(dPdx=0.3)
+    g.set_scalars(dPdx=0.3)
+    ref = o.compute_pressure()
+    got = g.compute_pressure()
+    assert np.all(np.isfinite(got))
+    assert rel(got, ref) <= 1e-11, rel(got, ref)

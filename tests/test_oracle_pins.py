@@ -461,6 +461,45 @@ def test_amd_wall_offset_ghosts():
     assert np.allclose(nut[:, -1, :], -nut[:, -2, :] + off)
 
 
+# ------------------------------------------------------------ true pressure
+
+def test_pressure_makes_the_momentum_rhs_divergence_free():
+    # the pressure solves Laplacian(p) = div(rhs) with dP/dy = bc at the
+    # walls: the corrected fluxes rhs - grad p are discretely divergence free
+    # at every unique centre (wall faces: rhs_v = bc = dP/dy, so zero flux)
+    from helpers import projected_state
+    case, field = cases.small_case(18, 17, 18, seed=3)
+    U, V, W = projected_state(case, field)
+    o = oracle.Oracle(case.x, case.y, case.z, **case.kwargs())
+    o.set_state(U, V, W)
+    rng = np.random.default_rng(5)
+    o.set(oracle.NUT, np.asfortranarray(rng.uniform(0.0, 2e-3, o.shape(oracle.NUT))))
+    o.set_scalars(dPdx=0.3)
+    P = o.compute_pressure()
+    RU, RV, RW = o.compute_rhs(U, V, W)
+    nx, ny, nz, nxg, nyg, nzg = o.nx, o.ny, o.nz, o.nxg, o.nyg, o.nzg
+    RU[0], RU[nx - 1] = RU[nx - 2], RU[1]
+    RW[:, :, 0], RW[:, :, nz - 1] = RW[:, :, nz - 2], RW[:, :, 1]
+    g = o.grid()
+    xg, yg, zg = g["xg"], g["yg"], g["zg"]
+    x, y, z = o.x, o.y, o.z
+    cu = RU - (P[1:, :, :] - P[:-1, :, :]) / (xg[1:] - xg[:-1])[:, None, None]
+    cv = RV - (P[:, 1:, :] - P[:, :-1, :]) / (yg[1:] - yg[:-1])[None, :, None]
+    cv[:, 0, :] = 0.0
+    cv[:, ny - 1, :] = 0.0
+    cw = RW - (P[:, :, 1:] - P[:, :, :-1]) / (zg[1:] - zg[:-1])[None, None, :]
+    I, J, K = slice(1, nxg - 2), slice(1, nyg - 1), slice(1, nzg - 2)   # unique centres, 0-based
+    div = ((cu[1:nxg - 2] - cu[0:nxg - 3])[:, J, K] / (x[1:nxg - 2] - x[0:nxg - 3])[:, None, None] +
+           (cv[I, 1:nyg - 1] - cv[I, 0:nyg - 2])[:, :, K] / (y[1:] - y[:-1])[None, :, None] +
+           (cw[I, J, 1:nzg - 2] - cw[I, J, 0:nzg - 3]) / (z[1:nzg - 2] - z[0:nzg - 3])[None, None, :])
+    raw = ((RU[1:nxg - 2] - RU[0:nxg - 3])[:, J, K] / (x[1:nxg - 2] - x[0:nxg - 3])[:, None, None])
+    assert np.abs(raw).max() > 0
+    assert np.abs(div).max() <= 1e-9 * np.abs(raw).max(), (np.abs(div).max(), np.abs(raw).max())
+    # the wall ghost rows carry the wall data: P(2) - P(1) = dy bc with the
+    # same bc on every image of a cell
+    assert np.allclose(P[0], P[nxg - 3]) and np.allclose(P[:, :, nzg - 2], P[:, :, 1])
+
+
 # ------------------------------------------------------------ statistics
 
 def test_wall_shear_of_poiseuille_flow():
