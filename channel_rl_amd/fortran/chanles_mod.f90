@@ -123,6 +123,15 @@ Module chanles_c
        Type(c_ptr), Value :: ctx
        Real(c_double), Intent(Out) :: out(*)
      End Function chan_plane_statistics
+     Integer(c_int) Function chan_compute_pressure(ctx) Bind(c, name='chan_compute_pressure')
+       Import :: c_ptr, c_int
+       Type(c_ptr), Value :: ctx
+     End Function chan_compute_pressure
+     Integer(c_int) Function chan_wall_shear(ctx, out4) Bind(c, name='chan_wall_shear')
+       Import :: c_ptr, c_int, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Intent(Out) :: out4(4)
+     End Function chan_wall_shear
      Integer(c_int) Function chan_synchronize(ctx) Bind(c, name='chan_synchronize')
        Import :: c_ptr, c_int
        Type(c_ptr), Value :: ctx
@@ -299,5 +308,20 @@ Contains
     Real(c_double), Intent(Out) :: stats(ny,7)
     Call check(chan_plane_statistics(ctx, stats), 'plane_statistics')
   End Subroutine compute_plane_statistics
+
+  ! code/statistics.f90:95-130: u_tau and Re_tau from the wall shear
+  Subroutine compute_wall_shear(utau_, Retau_)
+    Real(c_double), Intent(Out) :: utau_, Retau_
+    Real(c_double) :: w(4)
+    Call check(chan_wall_shear(ctx, w), 'wall_shear')
+    utau_ = w(3)
+    Retau_ = w(4)
+  End Subroutine compute_wall_shear
+
+  ! code/pressure.f90:21 compute_pressure: the true pressure into the
+  ! device P (read back with chanles_get_state)
+  Subroutine compute_pressure
+    Call check(chan_compute_pressure(ctx), 'compute_pressure')
+  End Subroutine compute_pressure
 
 End Module chanles_gpu

@@ -12,7 +12,7 @@
 ! The initial field is the .init file named in the parameters                  !
 ! (code/input_output.f90:390-570 format, read without the added noise).        !
 ! Snapshots are written every nsave steps in the same format                   !
-! (code/input_output.f90:579-730), with the pseudo-pressure as P.              !
+! (code/input_output.f90:579-730), with the true pressure as P.                !
 !------------------------------------------------------------------------------!
 Program channel_driver
   Use, Intrinsic :: iso_c_binding
@@ -23,7 +23,7 @@ Program channel_driver
   Character(200) :: filepar, filein, fileout, dummy_line, arg
   Integer(Int32) :: nxi, nyi, nzi, x_mass_cte, LES_model, nsteps, nsave, nstats, nmonitor
   Integer(Int32) :: nstep_init, random_init, Dirichlet_nu_t, precision, istep, j
-  Real(Int64) :: CFL, nu, dPdx_in, h_wm, maxdiv, maxabs
+  Real(Int64) :: CFL, nu, dPdx_in, h_wm, maxdiv, maxabs, utau, Retau
   Real(Int64), Allocatable :: x(:), y(:), z(:), U(:,:,:), V(:,:,:), W(:,:,:), P(:,:,:), nut(:,:,:)
   Real(Int64), Allocatable :: stats(:,:)
   Logical :: use_wm
@@ -100,13 +100,19 @@ Program channel_driver
      If (Mod(istep, nstats) == 0) Then
         Call compute_plane_statistics(stats)
         If (Any(stats /= stats)) Stop 'Error: NaNs!'
-        Write(*,'(A,ES14.6,A,ES14.6)') '  U centre', stats(nyi/2+1, 1), '  <uv> near wall', stats(3, 7)
+        Call compute_wall_shear(utau, Retau)
+        Write(*,'(A,ES14.6,A,ES14.6,A,ES14.6,A,F10.2)') '  U centre', stats(nyi/2+1, 1), &
+             '  <uv> near wall', stats(3, 7), '  u_tau', utau, '  Re_tau', Retau
      End If
      If (Mod(istep, nsave) == 0) Then
+        ! the snapshot holds the true pressure, as the reference's output
+        ! (code/input_output.f90:589)
+        Call compute_pressure
         Call chanles_get_state(U, V, W, P, nut)
         Call write_snapshot(istep + nstep_init)
      End If
   End Do
+  Call compute_pressure
   Call chanles_get_state(U, V, W, P, nut)
   Call write_snapshot(nsteps + nstep_init)
   Call chanles_finalize

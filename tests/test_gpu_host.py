@@ -39,6 +39,9 @@ def test_fortran_driver_matches_oracle(gpu, tmp_path):
         o.compute_time_step_RK3()
     for name, ref in zip("UVW", o.state()):
         assert rel(out[name], ref) <= 1e-10, (name, rel(out[name], ref))
+    # the snapshot's P is the true pressure of the final state
+    # (code/input_output.f90:589 -> code/pressure.f90:21)
+    assert rel(out["P"], o.compute_pressure()) <= 1e-9, rel(out["P"], o.compute_pressure())
 
 
 def test_fp32_bulk_against_fp64_oracle(gpu):
