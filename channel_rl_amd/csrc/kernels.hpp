@@ -121,12 +121,15 @@ struct PoissonBufs {
     const double* du;   // Dyy(j,j+1)
     FftPlan fx, fz;
     int bx, bz;         // transforms per block in the x and z passes
+    double2* tri;       // per mode and row: (c'_j, 1/den_j) of the elimination, [row][mode]
 };
 
 template <typename T>
 void launch_poisson_forward(const T* U, const T* V, const T* W, const Metric<double>& md, const Dims& d,
                             PoissonBufs& pb, hipStream_t s);
 void launch_poisson_solve(const Dims& d, PoissonBufs& pb, hipStream_t s);
+// the elimination's pivots of every mode (once per channel)
+void launch_tridiag_setup(const Dims& d, PoissonBufs& pb, hipStream_t s);
 void launch_poisson_inverse(const Dims& d, PoissonBufs& pb, hipStream_t s);
 // forward x pass of a given centre field (for the standalone Poisson solve)
 void launch_poisson_forward_field(const double* rhs, const Dims& d, PoissonBufs& pb, hipStream_t s);

@@ -223,73 +223,87 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftz(Dims d, PoissonBufs pb)
 // code/projection.f90:90-113: (Dyy + kxx + kzz) p = r per mode.  Lane pairs
 // share a mode (real, imaginary part); c' is kept in LDS for the back
 // substitution, d' is written over the right-hand side.
+// Thomas algorithm per (kx, kz) mode.  The matrix Dyy + kxx + kzz (with the
+// 00 mode's first row scaled by 3/2, code/projection.f90:107) is real and the
+// same for every solve of the run, so the forward elimination's pivots
+// 1/den_j and the modified upper diagonal c'_j are computed once per mode
+// (k_tridiag_setup, the same arithmetic the elimination would do) and kept
+// in HBM as pairs [row][mode] = (c'_j, 1/den_j).  A solve then reads the
+// right-hand side once forward and once backward with no divisions and no
+// LDS, so occupancy is not limited by the number of rows.
 static constexpr int TRI_THREADS = 64;
-static constexpr int TRI_UNROLL = 8;
+static constexpr int TRI_UNROLL = 12;
 
-__global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
+__global__ void __launch_bounds__(TRI_THREADS) k_tridiag_setup(Dims d, PoissonBufs pb)
 {
-    extern __shared__ double cp[];  // [row][lane]
-    const int lane = threadIdx.x;
-    const long g = (long)blockIdx.x * TRI_THREADS + lane;
     const int nmodes = d.nkx * d.nzp;
-    if (g >= 2L * nmodes) return;
-    const int mode = (int)(g >> 1);
+    const int mode = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mode >= nmodes) return;
     const int kx = mode / d.nzp, kz = mode % d.nzp;
     const int n = d.ny - 1;
-    const long plane = 2L * nmodes;
-    double* ph = reinterpret_cast<double*>(pb.ph) + g;
     const double lx = pb.kxx[kx], lz = pb.kzz[kz];
     double den = (pb.dd[0] + lx) + lz;
     if (kx == 0 && kz == 0) den = 3.0 / 2.0 * den;
     double inv = 1.0 / den;
     double c = pb.du[0] * inv;
-    cp[lane] = c;
-    double dp = ph[0] * inv;
-    ph[0] = dp;
-    int row = 1;
-    for (; row + TRI_UNROLL <= n; row += TRI_UNROLL) {
-        double r[TRI_UNROLL];
-#pragma unroll
-        for (int u = 0; u < TRI_UNROLL; u++) r[u] = ph[(row + u) * plane];
-#pragma unroll
-        for (int u = 0; u < TRI_UNROLL; u++) {
-            const int rr = row + u;
-            const double l = pb.dl[rr - 1];
-            den = ((pb.dd[rr] + lx) + lz) - l * c;
-            inv = 1.0 / den;
-            dp = (r[u] - l * dp) * inv;
-            ph[rr * plane] = dp;
-            if (rr < n - 1) {
-                c = pb.du[rr] * inv;
-                cp[rr * TRI_THREADS + lane] = c;
-            }
-        }
-    }
-    for (; row < n; row++) {
+    pb.tri[mode] = make_double2(c, inv);
+    for (int row = 1; row < n; row++) {
         const double l = pb.dl[row - 1];
         den = ((pb.dd[row] + lx) + lz) - l * c;
         inv = 1.0 / den;
-        dp = (ph[row * plane] - l * dp) * inv;
-        ph[row * plane] = dp;
-        if (row < n - 1) {
-            c = pb.du[row] * inv;
-            cp[row * TRI_THREADS + lane] = c;
+        c = row < n - 1 ? pb.du[row] * inv : 0.0;
+        pb.tri[(long)row * nmodes + mode] = make_double2(c, inv);
+    }
+}
+
+// one lane per (mode, real/imaginary part)
+__global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
+{
+    const long g = (long)blockIdx.x * TRI_THREADS + threadIdx.x;
+    const int nmodes = d.nkx * d.nzp;
+    if (g >= 2L * nmodes) return;
+    const int mode = (int)(g >> 1);
+    const int n = d.ny - 1;
+    const long plane = 2L * nmodes;
+    double* ph = reinterpret_cast<double*>(pb.ph) + g;
+    const double2* tri = pb.tri + mode;
+    double dp = ph[0] * tri[0].y;
+    ph[0] = dp;
+    int row = 1;
+    for (; row + TRI_UNROLL <= n; row += TRI_UNROLL) {
+        double r[TRI_UNROLL], iv[TRI_UNROLL];
+#pragma unroll
+        for (int u = 0; u < TRI_UNROLL; u++) {
+            r[u] = ph[(row + u) * plane];
+            iv[u] = tri[(long)(row + u) * nmodes].y;
         }
+#pragma unroll
+        for (int u = 0; u < TRI_UNROLL; u++) {
+            dp = (r[u] - pb.dl[row + u - 1] * dp) * iv[u];
+            ph[(row + u) * plane] = dp;
+        }
+    }
+    for (; row < n; row++) {
+        dp = (ph[row * plane] - pb.dl[row - 1] * dp) * tri[(long)row * nmodes].y;
+        ph[row * plane] = dp;
     }
     double xn = dp;
     row = n - 2;
     for (; row - TRI_UNROLL + 1 >= 0; row -= TRI_UNROLL) {
-        double r[TRI_UNROLL];
-#pragma unroll
-        for (int u = 0; u < TRI_UNROLL; u++) r[u] = ph[(row - u) * plane];
+        double r[TRI_UNROLL], cc[TRI_UNROLL];
 #pragma unroll
         for (int u = 0; u < TRI_UNROLL; u++) {
-            xn = r[u] - cp[(row - u) * TRI_THREADS + lane] * xn;
+            r[u] = ph[(row - u) * plane];
+            cc[u] = tri[(long)(row - u) * nmodes].x;
+        }
+#pragma unroll
+        for (int u = 0; u < TRI_UNROLL; u++) {
+            xn = r[u] - cc[u] * xn;
             ph[(row - u) * plane] = xn;
         }
     }
     for (; row >= 0; row--) {
-        xn = ph[row * plane] - cp[row * TRI_THREADS + lane] * xn;
+        xn = ph[row * plane] - tri[(long)row * nmodes].x * xn;
         ph[row * plane] = xn;
     }
 }
@@ -377,12 +391,16 @@ void launch_poisson_forward_field(const double* rhs, const Dims& d, PoissonBufs&
     PROF_END(s);
 }
 
+void launch_tridiag_setup(const Dims& d, PoissonBufs& pb, hipStream_t s)
+{
+    k_tridiag_setup<<<cdiv((long)d.nkx * d.nzp, TRI_THREADS), TRI_THREADS, 0, s>>>(d, pb);
+}
+
 void launch_poisson_solve(const Dims& d, PoissonBufs& pb, hipStream_t s)
 {
     const long lanes = 2L * d.nkx * d.nzp;
     PROF_BEGIN("k_tridiag", s);
-    k_tridiag<<<cdiv(lanes, TRI_THREADS), TRI_THREADS, (size_t)(d.ny - 1) * TRI_THREADS * sizeof(double), s>>>(d,
-                                                                                                             pb);
+    k_tridiag<<<cdiv(lanes, TRI_THREADS), TRI_THREADS, 0, s>>>(d, pb);
     PROF_END(s);
 }
 
