@@ -5,18 +5,21 @@
 //
 // Design: a block owns an x-y tile of TXO x TYO output cells and marches in
 // z over a chunk of planes.  Threads map one-to-one onto the tile plus a halo
-// of one ("columns").  Per plane:
-//   level 0  U,V,W of the tile plus a halo of two, three planes, in LDS,
-//            loaded with periodic index wrap (no ghost cells are read);
-//   level 1  per column: centre velocities a, |S| S_ij (in registers, three
-//            planes), and the filtered staggered velocities (in LDS, three
-//            planes), the latter from a y-z combined plane shared in LDS;
-//   filter   the test filter is separable: the z taps are combined in
-//            registers, then the x and y taps read the combined plane from
-//            LDS (y weights one-sided on rows 2 and nyg-1 as the reference).
-// Spacing divisions are multiplications by precomputed reciprocals.  The
-// sums run in a different order than the reference (separable filter), so
-// the model agrees with it to rounding, not bit for bit.
+// of one ("columns"), x fastest.  Per plane, three phases separated by three
+// barriers:
+//   1  y-z taps of the staggered-velocity filter for every level-0 position
+//      (level 0 = U,V,W of the tile plus a halo of two, three planes, in LDS,
+//      loaded with periodic index wrap: no ghost cells are read);
+//   2  level 1 per column: centre velocities a, |S| S_ij (in registers, three
+//      planes), the x taps of the filtered staggered velocities (to LDS, three
+//      planes), and the z taps of the filter inputs (to LDS, one plane);
+//   3  per output: the y taps of the filter inputs from LDS, the x taps by
+//      cross-lane moves, hat(S) from the filtered velocities, L:M and M:M;
+//      the next level-0 plane is stored and the one after it fetched.
+// The test filter is separable (y weights one-sided on rows 2 and nyg-1 as
+// the reference).  Spacing divisions are multiplications by precomputed
+// reciprocals.  The sums run in a different order than the reference, so the
+// model agrees with it to rounding, not bit for bit.
 // Global traffic: U,V,W read once per tile (plus halo), |S| written once.
 #include "kernels.hpp"
 #include "prof.hpp"
@@ -27,7 +30,29 @@ template <int TXO, int TYO>
 struct DsmTile {
     static constexpr int CX = TXO + 2, CY = TYO + 2, NC = CX * CY;  // columns
     static constexpr int LX = TXO + 4, LY = TYO + 4, NL = LX * LY;  // level-0 plane
+    static constexpr int NG = LX * (LY - 2);                         // y-z combined rows 1..LY-2
 };
+
+// Cross-lane moves along x.  Columns are x-fastest with CX columns per row
+// and CX divides the wave, so lane l's x neighbours are lanes l-1 and l+1; a
+// whole-wave shift by one lane (DPP wave_shr:1 / wave_shl:1) delivers them.
+// Only the ends of a row (halo columns cx = 0 and CX-1, which produce no
+// output) receive a lane of another row, or 0 at the ends of the wave.
+// The whole wave must be active.
+constexpr int DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138;
+template <int CTRL>
+__device__ inline double lane_shift(double v)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, (unsigned long long)(unsigned)lo | ((unsigned long long)(unsigned)hi << 32));
+}
+template <int CTRL>
+__device__ inline float lane_shift(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, true));
+}
 
 // Two blocks fit a CU's LDS; the register budget is held to the same two
 // waves per SIMD.
@@ -41,14 +66,12 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     // third of the FP64 instructions and changes results only by rounding.
 #pragma clang fp contract(fast)
     using Tl = DsmTile<TXO, TYO>;
-    constexpr int CX = Tl::CX, NC = Tl::NC, LX = Tl::LX, NL = Tl::NL;
+    constexpr int CX = Tl::CX, CY = Tl::CY, NC = Tl::NC, LX = Tl::LX, NL = Tl::NL, NG = Tl::NG;
+    static_assert(64 % CX == 0 && NC % 64 == 0, "rows of columns must tile the waves");
     __shared__ T l0[3][3][NL];  // [slot][U,V,W][position]
     __shared__ T lf[3][3][NC];  // [slot][Uf,Vf,Wf][column]
     __shared__ T zq[15][NC];    // z-combined filter inputs: a(3), a_a a_b (6), |S|S_ij (6)
-    // y-z combined level 0 (filtered-velocity input); shares zq's memory, it
-    // is dead before zq is written
-    static_assert(3 * NL <= 15 * NC, "gq must fit in zq");
-    T(*gq)[NL] = reinterpret_cast<T(*)[NL]>(&zq[0][0]);
+    __shared__ T gq[3][NG];     // y-z combined level 0, level-0 rows 1..LY-2
 
     const int tid = threadIdx.x;
     const int cx = tid % CX, cy = tid / CX;
@@ -119,26 +142,43 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
 
     // y and z taps of the filter of the staggered velocities at plane p
     // (code/subgrid.f90:140-190 over the whole y range, one-sided at the
-    // first/last row of each field), for every level-0 position.
+    // first/last row of each field), for the level-0 positions of rows
+    // 1..LY-2.  Blocks clear of both walls (every row interior to U, V and W)
+    // take the uniform weights without per-position tests.
+    const bool interior_rows = j0 - 1 >= 2 && j0 - 2 + Tl::LY - 2 <= d.ny - 1;
     auto yz_combine = [&](int p) {
-        for (int q = tid; q < NL; q += NC) {
-            const int ly = q / LX;
-            const int jj = j0 - 2 + ly;
-            for (int f = 0; f < 3; f++) {
-                const int n2 = (f == 1) ? d.ny : d.nyg;
-                T acc = T(0);
-                if (jj >= 1 && jj <= n2 && ly >= 1 && ly <= Tl::LY - 2) {
-                    const T wz3[3] = {s6, s23, s6};
-                    for (int dc = -1; dc <= 1; dc++) {
-                        const T* pl = l0[slot(p + dc)][f];
-                        T yv;
-                        if (jj == 1) yv = s23 * pl[q] + s13 * pl[q + LX];
-                        else if (jj == n2) yv = s13 * pl[q - LX] + s23 * pl[q];
-                        else yv = s6 * pl[q - LX] + s23 * pl[q] + s6 * pl[q + LX];
-                        acc = acc + wz3[dc + 1] * yv;
+        const T wz3[3] = {s6, s23, s6};
+        const T* pl[3][3];
+        for (int dc = -1; dc <= 1; dc++)
+            for (int f = 0; f < 3; f++) pl[dc + 1][f] = l0[slot(p + dc)][f];
+        for (int g = tid; g < NG; g += NC) {
+            const int q = g + LX;
+            if (interior_rows) {
+                for (int f = 0; f < 3; f++) {
+                    T acc = T(0);
+                    for (int dc = 0; dc < 3; dc++) {
+                        const T* P = pl[dc][f];
+                        acc = acc + wz3[dc] * (s6 * P[q - LX] + s23 * P[q] + s6 * P[q + LX]);
                     }
+                    gq[f][g] = acc;
                 }
-                gq[f][q] = acc;
+            } else {
+                const int jj = j0 - 2 + q / LX;
+                for (int f = 0; f < 3; f++) {
+                    const int n2 = (f == 1) ? d.ny : d.nyg;
+                    T acc = T(0);
+                    if (jj >= 1 && jj <= n2) {
+                        for (int dc = 0; dc < 3; dc++) {
+                            const T* P = pl[dc][f];
+                            T yv;
+                            if (jj == 1) yv = s23 * P[q] + s13 * P[q + LX];
+                            else if (jj == n2) yv = s13 * P[q - LX] + s23 * P[q];
+                            else yv = s6 * P[q - LX] + s23 * P[q] + s6 * P[q + LX];
+                            acc = acc + wz3[dc] * yv;
+                        }
+                    }
+                    gq[f][g] = acc;
+                }
             }
         }
     };
@@ -186,11 +226,11 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
         }
         // x taps of the staggered-velocity filter (Simpson 1/6, 2/3, 1/6)
         const int sf = slot(p);
-        const int g0 = (cx + 1) + LX * (cy + 1);
+        const int g0 = (cx + 1) + LX * cy;  // gq index of level-0 position (cx+1, cy+1)
         for (int f = 0; f < 3; f++) lf[sf][f][tid] = s6 * gq[f][g0 - 1] + s23 * gq[f][g0] + s6 * gq[f][g0 + 1];
     };
 
-    // pre-roll: level 0 planes kb-2..kb+1, level 1 at kb-1 and kb
+    // pre-roll: level 0 planes kb-2..kb+2, level 1 at kb-1 and kb
     T am[3], sm_m[6], a0[3], s0[6], ap[3], sp[6];
     T smag_m, smag_0, smag_p;
     load_plane(kb - 2);
@@ -206,46 +246,43 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     yz_combine(kb);
     __syncthreads();
     level1(kb, a0, s0, smag_0);
+    __syncthreads();
+    load_plane(kb + 2);
+    if (kb + 3 <= ke + 2) gload(kb + 3);
+    __syncthreads();
 
     const bool out_col = cx >= 1 && cx <= TXO && cy >= 1 && cy <= TYO && ci <= d.nxg - 1 && cj <= d.nyg - 1;
     double acc_lm = 0.0, acc_mm = 0.0;
     const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+    // y taps of the filter inputs for this column's row (one-sided at rows 2
+    // and nyg-1); halo rows read their own row and are never used
+    const T wym = (cj == 2) ? T(0) : (cj == d.nyg - 1 ? s13 : s6);
+    const T wyp = (cj == d.nyg - 1) ? T(0) : (cj == 2 ? s13 : s6);
+    const int om = cx + CX * (cy > 0 ? cy - 1 : cy), op = cx + CX * (cy < CY - 1 ? cy + 1 : cy);
 
-    gload(kb + 2);
     for (int k = kb; k <= ke; k++) {
-        __syncthreads();
-        sstore(k + 2);
-        if (k + 3 <= ke + 2) gload(k + 3);
-        __syncthreads();
+        // phase 1: l0 holds planes k..k+2
         yz_combine(k + 1);
         __syncthreads();
+        // phase 2
         level1(k + 1, ap, sp, smag_p);
-        __syncthreads();
-        // z taps of the filter inputs
-        for (int c = 0; c < 3; c++) zq[c][tid] = s6 * am[c] + s23 * a0[c] + s6 * ap[c];
+        T zown[15];  // z taps of the filter inputs
+        for (int c = 0; c < 3; c++) zown[c] = s6 * am[c] + s23 * a0[c] + s6 * ap[c];
         for (int c = 0; c < 6; c++) {
-            zq[3 + c][tid] = s6 * (am[pa[c]] * am[pb[c]]) + s23 * (a0[pa[c]] * a0[pb[c]]) + s6 * (ap[pa[c]] * ap[pb[c]]);
-            zq[9 + c][tid] = s6 * sm_m[c] + s23 * s0[c] + s6 * sp[c];
+            zown[3 + c] = s6 * (am[pa[c]] * am[pb[c]]) + s23 * (a0[pa[c]] * a0[pb[c]]) + s6 * (ap[pa[c]] * ap[pb[c]]);
+            zown[9 + c] = s6 * sm_m[c] + s23 * s0[c] + s6 * sp[c];
         }
+        for (int c = 0; c < 15; c++) zq[c][tid] = zown[c];
         __syncthreads();
+        // phase 3: x-y taps of the 15 filter inputs, every lane (cross-lane moves)
+        T tp[15];
+        for (int c = 0; c < 15; c++) {
+            const T y = wym * zq[c][om] + s23 * zown[c] + wyp * zq[c][op];
+            tp[c] = s6 * lane_shift<DPP_WAVE_SHR1>(y) + s23 * y + s6 * lane_shift<DPP_WAVE_SHL1>(y);
+        }
         if (out_col) {
-            const int j = cj;
-            const T wym = (j == 2) ? T(0) : (j == d.nyg - 1 ? s13 : s6);
-            const T wyp = (j == d.nyg - 1) ? T(0) : (j == 2 ? s13 : s6);
-            const T wyv[3] = {wym, s23, wyp};
-            const T wxv[3] = {s6, s23, s6};
-            // 3x3 x-y taps of one z-combined quantity
-            auto tap = [&](int q) {
-                T f = T(0);
-                for (int dy = -1; dy <= 1; dy++)
-                    for (int dx = -1; dx <= 1; dx++)
-                        f = f + (wxv[dx + 1] * wyv[dy + 1]) * zq[q][(cx + dx) + CX * (cy + dy)];
-                return f;
-            };
-            T F[3];
-            for (int q = 0; q < 3; q++) F[q] = tap(q);
+            const int i = ci, j = cj;
             // hat(S) from the filtered velocities at the true index (code/subgrid.f90:456)
-            const int i = ci;
             auto uf = [&](int dx, int dy, int dz) { return lf[slot(k + dz)][0][(cx + dx) + CX * (cy + dy)]; };
             auto vf = [&](int dx, int dy, int dz) { return lf[slot(k + dz)][1][(cx + dx) + CX * (cy + dy)]; };
             auto wf = [&](int dx, int dy, int dz) { return lf[slot(k + dz)][2][(cx + dx) + CX * (cy + dy)]; };
@@ -277,8 +314,8 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
             const T shm4 = T(4) * sqrt(acc);
             T lm = T(0), mm = T(0);
             for (int q = 0; q < 6; q++) {
-                const T L = tap(3 + q) - F[pa[q]] * F[pb[q]];
-                const T M = shm4 * sh[q] - tap(9 + q);
+                const T L = tp[3 + q] - tp[pa[q]] * tp[pb[q]];
+                const T M = shm4 * sh[q] - tp[9 + q];
                 const T wgt = q < 3 ? T(1) : T(2);
                 lm = lm + wgt * (L * M);
                 mm = mm + wgt * (M * M);
@@ -287,6 +324,10 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
             acc_mm += (double)mm;
             b.smag[at(d, i, j, k)] = smag_0;
         }
+        // next level-0 plane into the slot of plane k (last read in phase 2)
+        if (k + 3 <= ke + 2) sstore(k + 3);
+        if (k + 4 <= ke + 2) gload(k + 4);
+        __syncthreads();
         // rotate the level-1 ring
         for (int c = 0; c < 3; c++) { am[c] = a0[c]; a0[c] = ap[c]; }
         for (int c = 0; c < 6; c++) { sm_m[c] = s0[c]; s0[c] = sp[c]; }
@@ -294,7 +335,6 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
         smag_0 = smag_p;
     }
     // per-row partial sums over the block's columns, in a fixed order
-    __syncthreads();
     double* red = reinterpret_cast<double*>(&l0[0][0][0]);
     red[tid] = out_col ? acc_lm : 0.0;
     red[NC + tid] = out_col ? acc_mm : 0.0;
