@@ -66,7 +66,7 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     // third of the FP64 instructions and changes results only by rounding.
 #pragma clang fp contract(fast)
     using Tl = DsmTile<TXO, TYO>;
-    constexpr int CX = Tl::CX, CY = Tl::CY, NC = Tl::NC, LX = Tl::LX, NL = Tl::NL, NG = Tl::NG;
+    constexpr int CX = Tl::CX, NC = Tl::NC, LX = Tl::LX, NL = Tl::NL, NG = Tl::NG;
     static_assert(64 % CX == 0 && NC % 64 == 0, "rows of columns must tile the waves");
     __shared__ T l0[3][3][NL];  // [slot][U,V,W][position]
     __shared__ T lf[3][3][NC];  // [slot][Uf,Vf,Wf][column]
@@ -140,45 +140,54 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     };
     auto L0 = [&](int f, int p, int dx, int dy) -> T { return l0[slot(p)][f][(cx + 1 + dx) + LX * (cy + 1 + dy)]; };
 
-    // y and z taps of the filter of the staggered velocities at plane p
-    // (code/subgrid.f90:140-190 over the whole y range, one-sided at the
-    // first/last row of each field), for the level-0 positions of rows
-    // 1..LY-2.  Blocks clear of both walls (every row interior to U, V and W)
-    // take the uniform weights without per-position tests.
+    // y and z taps of the filter of the staggered velocities (code/subgrid.f90:
+    // 140-190 over the whole y range, one-sided at the first/last row of each
+    // field) for the level-0 positions of rows 1..LY-2.  Each thread owns
+    // positions g = tid + r NC; the y taps of a plane are taken once and kept
+    // for the z taps in a register ring of two planes (ycm, yc0).  Blocks
+    // clear of both walls (every row interior to U, V and W) take the uniform
+    // weights without per-position tests.
+    constexpr int NGT = (NG + NC - 1) / NC;
     const bool interior_rows = j0 - 1 >= 2 && j0 - 2 + Tl::LY - 2 <= d.ny - 1;
-    auto yz_combine = [&](int p) {
-        const T wz3[3] = {s6, s23, s6};
-        const T* pl[3][3];
-        for (int dc = -1; dc <= 1; dc++)
-            for (int f = 0; f < 3; f++) pl[dc + 1][f] = l0[slot(p + dc)][f];
-        for (int g = tid; g < NG; g += NC) {
+    T ycm[NGT][3], yc0[NGT][3];
+    auto ycomb = [&](int p, T out[NGT][3]) {
+        const int s = slot(p);
+#pragma unroll
+        for (int r = 0; r < NGT; r++) {
+            const int g = tid + r * NC;
             const int q = g + LX;
+            for (int f = 0; f < 3; f++) out[r][f] = T(0);
+            if (g >= NG) continue;
             if (interior_rows) {
                 for (int f = 0; f < 3; f++) {
-                    T acc = T(0);
-                    for (int dc = 0; dc < 3; dc++) {
-                        const T* P = pl[dc][f];
-                        acc = acc + wz3[dc] * (s6 * P[q - LX] + s23 * P[q] + s6 * P[q + LX]);
-                    }
-                    gq[f][g] = acc;
+                    const T* P = l0[s][f];
+                    out[r][f] = s6 * P[q - LX] + s23 * P[q] + s6 * P[q + LX];
                 }
             } else {
                 const int jj = j0 - 2 + q / LX;
                 for (int f = 0; f < 3; f++) {
                     const int n2 = (f == 1) ? d.ny : d.nyg;
-                    T acc = T(0);
-                    if (jj >= 1 && jj <= n2) {
-                        for (int dc = 0; dc < 3; dc++) {
-                            const T* P = pl[dc][f];
-                            T yv;
-                            if (jj == 1) yv = s23 * P[q] + s13 * P[q + LX];
-                            else if (jj == n2) yv = s13 * P[q - LX] + s23 * P[q];
-                            else yv = s6 * P[q - LX] + s23 * P[q] + s6 * P[q + LX];
-                            acc = acc + wz3[dc] * yv;
-                        }
-                    }
-                    gq[f][g] = acc;
+                    const T* P = l0[s][f];
+                    if (jj < 1 || jj > n2) out[r][f] = T(0);
+                    else if (jj == 1) out[r][f] = s23 * P[q] + s13 * P[q + LX];
+                    else if (jj == n2) out[r][f] = s13 * P[q - LX] + s23 * P[q];
+                    else out[r][f] = s6 * P[q - LX] + s23 * P[q] + s6 * P[q + LX];
                 }
+            }
+        }
+    };
+    // gq at plane p from the ring (planes p-1, p) and the y taps of plane p+1;
+    // the ring advances to (p, p+1)
+    auto yz_combine = [&](int p) {
+        T ycp[NGT][3];
+        ycomb(p + 1, ycp);
+#pragma unroll
+        for (int r = 0; r < NGT; r++) {
+            const int g = tid + r * NC;
+            for (int f = 0; f < 3; f++) {
+                if (g < NG) gq[f][g] = s6 * ycm[r][f] + s23 * yc0[r][f] + s6 * ycp[r][f];
+                ycm[r][f] = yc0[r][f];
+                yc0[r][f] = ycp[r][f];
             }
         }
     };
@@ -237,6 +246,8 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     load_plane(kb - 1);
     load_plane(kb);
     __syncthreads();
+    ycomb(kb - 2, ycm);
+    ycomb(kb - 1, yc0);
     yz_combine(kb - 1);
     __syncthreads();
     level1(kb - 1, am, sm_m, smag_m);
@@ -255,10 +266,10 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     double acc_lm = 0.0, acc_mm = 0.0;
     const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
     // y taps of the filter inputs for this column's row (one-sided at rows 2
-    // and nyg-1); halo rows read their own row and are never used
+    // and nyg-1), taken on the output rows 1..TYO only
     const T wym = (cj == 2) ? T(0) : (cj == d.nyg - 1 ? s13 : s6);
     const T wyp = (cj == d.nyg - 1) ? T(0) : (cj == 2 ? s13 : s6);
-    const int om = cx + CX * (cy > 0 ? cy - 1 : cy), op = cx + CX * (cy < CY - 1 ? cy + 1 : cy);
+    const int om = cx + CX * (cy - 1), op = cx + CX * (cy + 1);
 
     for (int k = kb; k <= ke; k++) {
         // phase 1: l0 holds planes k..k+2
@@ -276,8 +287,13 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
         __syncthreads();
         // phase 3: x-y taps of the 15 filter inputs, every lane (cross-lane moves)
         T tp[15];
+        if (cy >= 1 && cy <= TYO) {
+            for (int c = 0; c < 15; c++) tp[c] = wym * zq[c][om] + s23 * zown[c] + wyp * zq[c][op];
+        } else {
+            for (int c = 0; c < 15; c++) tp[c] = T(0);
+        }
         for (int c = 0; c < 15; c++) {
-            const T y = wym * zq[c][om] + s23 * zown[c] + wyp * zq[c][op];
+            const T y = tp[c];
             tp[c] = s6 * lane_shift<DPP_WAVE_SHR1>(y) + s23 * y + s6 * lane_shift<DPP_WAVE_SHL1>(y);
         }
         if (out_col) {
