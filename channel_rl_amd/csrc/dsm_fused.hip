@@ -10,9 +10,9 @@
 //   1  y-z taps of the staggered-velocity filter for every level-0 position
 //      (level 0 = U,V,W of the tile plus a halo of two, three planes, in LDS,
 //      loaded with periodic index wrap: no ghost cells are read);
-//   2  level 1 per column: centre velocities a, |S| S_ij (in registers, three
-//      planes), the x taps of the filtered staggered velocities (to LDS, three
-//      planes), and the z taps of the filter inputs (to LDS, one plane);
+//   2  level 1 per column: centre velocities a, |S| S_ij, the z taps of the
+//      filter inputs (running sums in registers, the finished plane to LDS),
+//      the x taps of the filtered staggered velocities (to LDS, three planes);
 //   3  per output: the y taps of the filter inputs from LDS, the x taps by
 //      cross-lane moves, hat(S) from the filtered velocities, L:M and M:M;
 //      the next level-0 plane is stored and the one after it fetched.
@@ -143,13 +143,14 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     // y and z taps of the filter of the staggered velocities (code/subgrid.f90:
     // 140-190 over the whole y range, one-sided at the first/last row of each
     // field) for the level-0 positions of rows 1..LY-2.  Each thread owns
-    // positions g = tid + r NC; the y taps of a plane are taken once and kept
-    // for the z taps in a register ring of two planes (ycm, yc0).  Blocks
+    // positions g = tid + r NC; the y taps of a plane are taken once and
+    // enter the z taps of three planes through two running sums (ga: the
+    // next plane's, two terms in; gb: the one after, one term in).  Blocks
     // clear of both walls (every row interior to U, V and W) take the uniform
     // weights without per-position tests.
     constexpr int NGT = (NG + NC - 1) / NC;
     const bool interior_rows = j0 - 1 >= 2 && j0 - 2 + Tl::LY - 2 <= d.ny - 1;
-    T ycm[NGT][3], yc0[NGT][3];
+    T ga[NGT][3], gb[NGT][3];
     auto ycomb = [&](int p, T out[NGT][3]) {
         const int s = slot(p);
 #pragma unroll
@@ -176,8 +177,8 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
             }
         }
     };
-    // gq at plane p from the ring (planes p-1, p) and the y taps of plane p+1;
-    // the ring advances to (p, p+1)
+    // gq at plane p: the running sums hold planes p-1 and p, the y taps of
+    // plane p+1 complete it
     auto yz_combine = [&](int p) {
         T ycp[NGT][3];
         ycomb(p + 1, ycp);
@@ -185,19 +186,23 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
         for (int r = 0; r < NGT; r++) {
             const int g = tid + r * NC;
             for (int f = 0; f < 3; f++) {
-                if (g < NG) gq[f][g] = s6 * ycm[r][f] + s23 * yc0[r][f] + s6 * ycp[r][f];
-                ycm[r][f] = yc0[r][f];
-                yc0[r][f] = ycp[r][f];
+                if (g < NG) gq[f][g] = ga[r][f] + s6 * ycp[r][f];
+                ga[r][f] = gb[r][f] + s23 * ycp[r][f];
+                gb[r][f] = s6 * ycp[r][f];
             }
         }
     };
 
-    // level 1 of this column at plane p: a (3), |S| S_ij (6), |S|; filtered velocities to LDS
-    auto level1 = [&](int p, T a[3], T ss[6], T& sm) {
+    const bool out_col = cx >= 1 && cx <= TXO && cy >= 1 && cy <= TYO && ci <= d.nxg - 1 && cj <= d.nyg - 1;
+    const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+
+    // level 1 of this column at plane p: the filter inputs Q = a (3),
+    // a_a a_b (6), |S| S_ij (6); |S| of the output planes to global memory,
+    // the x taps of the filtered velocities to LDS
+    auto level1 = [&](int p, T Q[15]) {
         const int pc = wz(p), ic = wx(ci);
-        for (int c = 0; c < 3; c++) a[c] = T(0);
-        for (int c = 0; c < 6; c++) ss[c] = T(0);
-        sm = T(0);
+        T a[3] = {T(0), T(0), T(0)}, ss[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+        T sm = T(0);
         if (cj >= 2 && cj <= d.nyg - 1) {
             // code/subgrid.f90:394-396 centre velocities
             a[0] = h * (L0(0, p, -1, 0) + L0(0, p, 0, 0));
@@ -233,38 +238,60 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
             sm = sqrt(acc);
             for (int c = 0; c < 6; c++) ss[c] = sm * s[c];
         }
+        for (int c = 0; c < 3; c++) Q[c] = a[c];
+        for (int c = 0; c < 6; c++) {
+            Q[3 + c] = a[pa[c]] * a[pb[c]];
+            Q[9 + c] = ss[c];
+        }
+        if (out_col && p >= kb && p <= ke) b.smag[at(d, ci, cj, p)] = sm;
         // x taps of the staggered-velocity filter (Simpson 1/6, 2/3, 1/6)
         const int sf = slot(p);
         const int g0 = (cx + 1) + LX * cy;  // gq index of level-0 position (cx+1, cy+1)
         for (int f = 0; f < 3; f++) lf[sf][f][tid] = s6 * gq[f][g0 - 1] + s23 * gq[f][g0] + s6 * gq[f][g0 + 1];
     };
 
+    // z taps of the filter inputs as running sums: za for plane k (planes k-1
+    // and k in), zb for plane k+1 (plane k in)
+    T za[15], zb[15], Q[15];
     // pre-roll: level 0 planes kb-2..kb+2, level 1 at kb-1 and kb
-    T am[3], sm_m[6], a0[3], s0[6], ap[3], sp[6];
-    T smag_m, smag_0, smag_p;
     load_plane(kb - 2);
     load_plane(kb - 1);
     load_plane(kb);
     __syncthreads();
-    ycomb(kb - 2, ycm);
-    ycomb(kb - 1, yc0);
+    {
+        T yc[NGT][3];
+        ycomb(kb - 2, yc);
+#pragma unroll
+        for (int r = 0; r < NGT; r++)
+            for (int f = 0; f < 3; f++) ga[r][f] = s6 * yc[r][f];
+        ycomb(kb - 1, yc);
+#pragma unroll
+        for (int r = 0; r < NGT; r++)
+            for (int f = 0; f < 3; f++) {
+                ga[r][f] = ga[r][f] + s23 * yc[r][f];
+                gb[r][f] = s6 * yc[r][f];
+            }
+    }
     yz_combine(kb - 1);
     __syncthreads();
-    level1(kb - 1, am, sm_m, smag_m);
+    level1(kb - 1, Q);
+    for (int c = 0; c < 15; c++) za[c] = s6 * Q[c];
     __syncthreads();
     load_plane(kb + 1);
     __syncthreads();
     yz_combine(kb);
     __syncthreads();
-    level1(kb, a0, s0, smag_0);
+    level1(kb, Q);
+    for (int c = 0; c < 15; c++) {
+        za[c] = za[c] + s23 * Q[c];
+        zb[c] = s6 * Q[c];
+    }
     __syncthreads();
     load_plane(kb + 2);
     if (kb + 3 <= ke + 2) gload(kb + 3);
     __syncthreads();
 
-    const bool out_col = cx >= 1 && cx <= TXO && cy >= 1 && cy <= TYO && ci <= d.nxg - 1 && cj <= d.nyg - 1;
     double acc_lm = 0.0, acc_mm = 0.0;
-    const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
     // y taps of the filter inputs for this column's row (one-sided at rows 2
     // and nyg-1), taken on the output rows 1..TYO only
     const T wym = (cj == 2) ? T(0) : (cj == d.nyg - 1 ? s13 : s6);
@@ -276,12 +303,12 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
         yz_combine(k + 1);
         __syncthreads();
         // phase 2
-        level1(k + 1, ap, sp, smag_p);
-        T zown[15];  // z taps of the filter inputs
-        for (int c = 0; c < 3; c++) zown[c] = s6 * am[c] + s23 * a0[c] + s6 * ap[c];
-        for (int c = 0; c < 6; c++) {
-            zown[3 + c] = s6 * (am[pa[c]] * am[pb[c]]) + s23 * (a0[pa[c]] * a0[pb[c]]) + s6 * (ap[pa[c]] * ap[pb[c]]);
-            zown[9 + c] = s6 * sm_m[c] + s23 * s0[c] + s6 * sp[c];
+        level1(k + 1, Q);
+        T zown[15];  // z taps of the filter inputs at plane k
+        for (int c = 0; c < 15; c++) {
+            zown[c] = za[c] + s6 * Q[c];
+            za[c] = zb[c] + s23 * Q[c];
+            zb[c] = s6 * Q[c];
         }
         for (int c = 0; c < 15; c++) zq[c][tid] = zown[c];
         __syncthreads();
@@ -338,17 +365,11 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
             }
             acc_lm += (double)lm;
             acc_mm += (double)mm;
-            b.smag[at(d, i, j, k)] = smag_0;
         }
         // next level-0 plane into the slot of plane k (last read in phase 2)
         if (k + 3 <= ke + 2) sstore(k + 3);
         if (k + 4 <= ke + 2) gload(k + 4);
         __syncthreads();
-        // rotate the level-1 ring
-        for (int c = 0; c < 3; c++) { am[c] = a0[c]; a0[c] = ap[c]; }
-        for (int c = 0; c < 6; c++) { sm_m[c] = s0[c]; s0[c] = sp[c]; }
-        smag_m = smag_0;
-        smag_0 = smag_p;
     }
     // per-row partial sums over the block's columns, in a fixed order
     double* red = reinterpret_cast<double*>(&l0[0][0][0]);
@@ -368,7 +389,6 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
             b.part[((long)(j - 1) * 2 + 1) * b.nblk + blk] = sm2;
         }
     }
-    (void)smag_m;
 }
 
 static constexpr int DSM_TXO = 30, DSM_TYO = 6, DSM_KCHUNK = 16;
