@@ -94,4 +94,22 @@ struct Scalars {
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
+// Tile order across the 8 XCDs of MI355X.  The dispatcher deals workgroups
+// to the XCDs in turn (workgroup n to XCD n mod 8) and each XCD has its own
+// L2, so neighbouring tiles in launch order land in different L2s and their
+// shared halo rows are fetched from HBM once per XCD.  Remapping the linear
+// workgroup id so that each XCD receives a contiguous run of tiles (x
+// fastest, then y, then z) keeps neighbours, and their halos, in one L2.
+// A bijection on 0..n-1 for any n.
+struct TileId { int x, y, z; };
+__device__ inline TileId xcd_tile()
+{
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int n = gx * gy * gridDim.z;
+    const int h = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const int q = n / 8, r = n % 8, x = h % 8;
+    const int l = x * q + (x < r ? x : r) + h / 8;
+    return TileId{l % gx, (l / gx) % gy, l / (gx * gy)};
+}
+
 }  // namespace chan

@@ -75,8 +75,9 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
 
     const int tid = threadIdx.x;
     const int cx = tid % CX, cy = tid / CX;
-    const int i0 = blockIdx.x * TXO + 2, j0 = blockIdx.y * TYO + 2;
-    const int kb = blockIdx.z * kchunk + 2;
+    const TileId tile = xcd_tile();
+    const int i0 = tile.x * TXO + 2, j0 = tile.y * TYO + 2;
+    const int kb = tile.z * kchunk + 2;
     const int ke = min(kb + kchunk - 1, d.nzg - 1);
     const int ci = i0 - 1 + cx, cj = j0 - 1 + cy;  // column position (x unwrapped)
     const int nxp = d.nxp, nzp = d.nzp;
@@ -384,7 +385,7 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
                 sl += red[c + CX * cyy];
                 sm2 += red[NC + c + CX * cyy];
             }
-            const int blk = blockIdx.z * gridDim.x + blockIdx.x;
+            const int blk = tile.z * gridDim.x + tile.x;
             b.part[((long)(j - 1) * 2 + 0) * b.nblk + blk] = sl;
             b.part[((long)(j - 1) * 2 + 1) * b.nblk + blk] = sm2;
         }

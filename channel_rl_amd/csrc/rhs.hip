@@ -167,10 +167,11 @@ __global__ void __launch_bounds__(TX* TY)
     constexpr int PX = TX + 2, PY = TY + 2, NP = PX * PY, NT = TX * TY, NL = (NP + NT - 1) / NT;
     __shared__ T sh[4][3][NP];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-    const int i0 = blockIdx.x * TX + 2, j0 = blockIdx.y * TY + 2;
+    const TileId tile = xcd_tile();
+    const int i0 = tile.x * TX + 2, j0 = tile.y * TY + 2;
     const int imax = mode == 0 ? d.nxg - 1 : d.nxg - 2;
     const int kmax = mode == 0 ? d.nzg - 1 : d.nzg - 2;
-    const int kb = blockIdx.z * kchunk + 2;
+    const int kb = tile.z * kchunk + 2;
     const int ke = min(kb + kchunk - 1, kmax);
     const int i = i0 + tx, j = j0 + ty;
     const T* src[4] = {U, V, W, nt};
