@@ -3,7 +3,7 @@
 //
 // A block owns a TX x TY tile of (x,y) cells and marches in z.  U, V, W and
 // nu_t for planes k-1, k, k+1 of the tile plus a one-cell halo sit in LDS
-// (a three-plane ring), so every value is read from HBM once per tile and
+// (a four-slot ring), so every value is read from HBM once per tile and
 // the ~120 stencil operands of a cell come from LDS.  The arithmetic is the
 // reference's operation order (no FMA contraction in FP64).
 #include "kernels.hpp"
@@ -153,9 +153,10 @@ __device__ __forceinline__ T rhs_w_gen(const A& F, const RhsXY<T>& g, const RhsZ
 // last unique cell (nxg-2, nzg-2): the image columns the reference computes
 // there are overwritten by the boundary conditions that follow.
 //
-// Pipelining: while plane k is computed from the LDS ring (planes k-1, k,
-// k+1), the loads of plane k+2 are already in flight into registers; they
-// go to the ring slot of plane k-1 once every thread is done with plane k.
+// Pipelining: while plane k is computed from the LDS ring (four slots,
+// planes k-1, k, k+1 in use), the loads of plane k+2 are already in flight
+// into registers; they go to the free slot (plane k-2's) right after, so one
+// barrier per plane separates the ring's writes from its reads.
 // In substeps 2 and 3 the thread's Uo and F1 values are fetched before the
 // stencil as well.
 template <typename T, int TX, int TY>
@@ -165,7 +166,7 @@ __global__ void __launch_bounds__(TX* TY)
              T* F1v, T* F1w, T* Ou, T* Ov, T* Ow, T c1, T c2, int kchunk)
 {
     constexpr int PX = TX + 2, PY = TY + 2, NP = PX * PY, NT = TX * TY, NL = (NP + NT - 1) / NT;
-    __shared__ T sh[4][3][NP];
+    __shared__ T sh[4][4][NP];  // [field][slot][position]
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
     const TileId tile = xcd_tile();
     const int i0 = tile.x * TX + 2, j0 = tile.y * TY + 2;
@@ -194,7 +195,7 @@ __global__ void __launch_bounds__(TX* TY)
 #pragma unroll
             for (int f = 0; f < 4; f++) pre[r][f] = lo[r] >= 0 ? src[f][lo[r] + po] : T(0);
     };
-    auto slot = [&](int p) { return (p - kb + 3) % 3; };
+    auto slot = [&](int p) { return (p - kb + 4) & 3; };
     auto sstore = [&](int p) {
         const int s = slot(p);
 #pragma unroll
@@ -256,11 +257,10 @@ __global__ void __launch_bounds__(TX* TY)
             if (dv) store_img(Ov, d, 1, i, j, k, vo + dt * (c1 * fv + c2 * rv));
             if (dw) store_img(Ow, d, 2, i, j, k, wo + dt * (c1 * fw + c2 * rw));
         }
+        // plane k+2 goes to the slot of plane k-2, last read before the
+        // previous barrier
+        if (more) sstore(k + 2);
         __syncthreads();
-        if (more) {
-            sstore(k + 2);
-            __syncthreads();
-        }
     }
 }
 
