@@ -154,9 +154,10 @@ __device__ __forceinline__ T rhs_w_gen(const A& F, const RhsXY<T>& g, const RhsZ
 // there are overwritten by the boundary conditions that follow.
 //
 // Pipelining: while plane k is computed from the LDS ring (four slots,
-// planes k-1, k, k+1 in use), the loads of plane k+2 are already in flight
-// into registers; they go to the free slot (plane k-2's) right after, so one
-// barrier per plane separates the ring's writes from its reads.
+// planes k-1, k, k+1 in use), the loads of planes k+2 and k+3 are in flight
+// into two register buffers; plane k+2 goes to the free slot (plane k-2's)
+// right after, so one barrier per plane separates the ring's writes from
+// its reads, and every load has two planes of work to arrive.
 // In substeps 2 and 3 the thread's Uo and F1 values are fetched before the
 // stencil as well.
 template <typename T, int TX, int TY>
@@ -187,8 +188,10 @@ __global__ void __launch_bounds__(TX* TY)
         lq[r] = q < NP ? q : -1;
         lo[r] = (q < NP && ii <= d.nxg && jj <= d.nyg) ? at(d, ii, jj, 1) : -1;
     }
-    T pre[NL][4];
-    auto gload = [&](int p) {
+    // two register buffers: the plane stored at the end of an iteration was
+    // fetched two iterations earlier
+    T preA[NL][4], preB[NL][4];
+    auto gload = [&](int p, T (&pre)[NL][4]) {
         const long po = (long)(p - 1) * d.sxy;
 #pragma unroll
         for (int r = 0; r < NL; r++)
@@ -196,7 +199,7 @@ __global__ void __launch_bounds__(TX* TY)
             for (int f = 0; f < 4; f++) pre[r][f] = lo[r] >= 0 ? src[f][lo[r] + po] : T(0);
     };
     auto slot = [&](int p) { return (p - kb + 4) & 3; };
-    auto sstore = [&](int p) {
+    auto sstore = [&](int p, const T (&pre)[NL][4]) {
         const int s = slot(p);
 #pragma unroll
         for (int r = 0; r < NL; r++)
@@ -214,17 +217,20 @@ __global__ void __launch_bounds__(TX* TY)
         g.ry_f = tb.rdy[jc]; g.ry_fp = tb.rdy[jc + 1]; g.ry_c = tb.rdyg[jc]; g.ry_cp = tb.rdyg[jc + 1];
         g.w0j = tb.w0[jc]; g.w1j = tb.w1[jc]; g.w0m = tb.w0[jc - 1]; g.w1m = tb.w1[jc - 1];
     }
-    gload(kb - 1);
-    sstore(kb - 1);
-    gload(kb);
-    sstore(kb);
-    gload(kb + 1);
-    sstore(kb + 1);
+    gload(kb - 1, preA);
+    sstore(kb - 1, preA);
+    gload(kb, preA);
+    sstore(kb, preA);
+    gload(kb + 1, preA);
+    sstore(kb + 1, preA);
+    if (kb + 2 <= ke + 1) gload(kb + 2, preA);
     __syncthreads();
     const bool act = i <= imax && j <= d.nyg - 1;
-    for (int k = kb; k <= ke; k++) {
-        const bool more = k + 2 <= ke + 1;
-        if (more) gload(k + 2);
+    // one plane of output from the ring (planes k-1, k, k+1).  The thread's
+    // own Uo, F1 loads are issued before the prefetch (fetch): memory
+    // counters drain in issue order, so waiting for them does not wait for
+    // the prefetched planes.
+    auto plane = [&](int k, auto&& fetch) {
         const long p = at(d, i, j, k);
         const bool du = act && (i <= d.nx - 1) && (k <= d.nzg - 1);
         const bool dv = act && (j <= d.ny - 1) && (k <= d.nzg - 1);
@@ -235,6 +241,7 @@ __global__ void __launch_bounds__(TX* TY)
             if (dv) { vo = Vo[p]; fv = F1v[p]; }
             if (dw) { wo = Wo[p]; fw = F1w[p]; }
         }
+        fetch();
         RhsZ<T> gz;
         gz.rz_f = tb.rdz[k]; gz.rz_fp = tb.rdz[k + 1]; gz.rz_c = tb.rdzg[k]; gz.rz_cp = tb.rdzg[k + 1];
         T ru = T(0), rv = T(0), rw = T(0);
@@ -257,9 +264,17 @@ __global__ void __launch_bounds__(TX* TY)
             if (dv) store_img(Ov, d, 1, i, j, k, vo + dt * (c1 * fv + c2 * rv));
             if (dw) store_img(Ow, d, 2, i, j, k, wo + dt * (c1 * fw + c2 * rw));
         }
-        // plane k+2 goes to the slot of plane k-2, last read before the
-        // previous barrier
-        if (more) sstore(k + 2);
+    };
+    // unrolled by two so the register buffers alternate without copies;
+    // plane k+2 goes to the slot of plane k-2, last read before the
+    // previous barrier
+    for (int k = kb; k <= ke; k += 2) {
+        plane(k, [&] { if (k + 3 <= ke + 1) gload(k + 3, preB); });
+        if (k + 2 <= ke + 1) sstore(k + 2, preA);
+        __syncthreads();
+        if (k + 1 > ke) break;
+        plane(k + 1, [&] { if (k + 4 <= ke + 1) gload(k + 4, preA); });
+        if (k + 3 <= ke + 1) sstore(k + 3, preB);
         __syncthreads();
     }
 }
