@@ -87,7 +87,28 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     auto wz = [&](int q) { return q < 2 ? q + nzp : (q > nzp + 1 ? q - nzp : q); };
     const T h = T(0.5), qq = T(0.25);
     const T s6 = T(1.0 / 6.0), s23 = T(2.0 / 3.0), s13 = T(1.0 / 3.0);
-    const T *rdx = b.rdx, *rdy = b.rdy, *rdz = b.rdz, *rdxg = b.rdxg, *rdyg = b.rdyg, *rdzg = b.rdzg;
+    const T *rdz = b.rdz, *rdzg = b.rdzg;
+    // x and y reciprocal spacings of the block's columns and rows, in LDS:
+    // read per plane from here rather than from global memory, whose loads
+    // would wait behind the velocity prefetch (memory counters drain in
+    // issue order).  Level 1 uses the canonical x image, the outputs the
+    // true index (code/subgrid.f90:456); z factors are uniform per plane.
+    __shared__ T mx[6][CX];  // rdx, rdxg(i), rdxg(i+1) at wx(ci); the same at ci
+    __shared__ T my[3][TYO + 2];  // rdy, rdyg(j), rdyg(j+1)
+    if (tid < CX) {
+        const int ic = wx(i0 - 1 + tid), it = min(max(i0 - 1 + tid, 2), d.nxg - 1);
+        mx[0][tid] = b.rdx[ic];
+        mx[1][tid] = b.rdxg[ic];
+        mx[2][tid] = b.rdxg[ic + 1];
+        mx[3][tid] = b.rdx[it];
+        mx[4][tid] = b.rdxg[it];
+        mx[5][tid] = b.rdxg[it + 1];
+    } else if (tid < CX + TYO + 2) {
+        const int r = tid - CX, jt = min(max(j0 - 1 + r, 2), d.nyg - 1);
+        my[0][r] = b.rdy[jt];
+        my[1][r] = b.rdyg[jt];
+        my[2][r] = b.rdyg[jt + 1];
+    }
 
     auto slot = [&](int p) { return (p - kb + 3) % 3; };  // ring slot of plane p >= kb-2
 
@@ -201,7 +222,7 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     // a_a a_b (6), |S| S_ij (6); |S| of the output planes to global memory,
     // the x taps of the filtered velocities to LDS
     auto level1 = [&](int p, T Q[15]) {
-        const int pc = wz(p), ic = wx(ci);
+        const int pc = wz(p);
         T a[3] = {T(0), T(0), T(0)}, ss[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
         T sm = T(0);
         if (cj >= 2 && cj <= d.nyg - 1) {
@@ -209,23 +230,22 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
             a[0] = h * (L0(0, p, -1, 0) + L0(0, p, 0, 0));
             a[1] = h * (L0(1, p, 0, -1) + L0(1, p, 0, 0));
             a[2] = h * (L0(2, p - 1, 0, 0) + L0(2, p, 0, 0));
-            // code/subgrid.f90:1027-1070 at the canonical image (ic, cj, pc)
-            const int jj = cj;
+            // code/subgrid.f90:1027-1070 at the canonical image (wx(ci), cj, pc)
             T s[6];
-            s[0] = (L0(0, p, 0, 0) - L0(0, p, -1, 0)) * rdx[ic];
-            s[1] = (L0(1, p, 0, 0) - L0(1, p, 0, -1)) * rdy[jj];
+            s[0] = (L0(0, p, 0, 0) - L0(0, p, -1, 0)) * mx[0][cx];
+            s[1] = (L0(1, p, 0, 0) - L0(1, p, 0, -1)) * my[0][cy];
             s[2] = (L0(2, p, 0, 0) - L0(2, p - 1, 0, 0)) * rdz[pc];
             auto e4 = [&](int da, int db) {
-                return h * ((L0(0, p, da - 1, db) - L0(0, p, da - 1, db - 1)) * rdyg[jj + db] +
-                            (L0(1, p, da, db - 1) - L0(1, p, da - 1, db - 1)) * rdxg[ic + da]);
+                return h * ((L0(0, p, da - 1, db) - L0(0, p, da - 1, db - 1)) * my[1 + db][cy] +
+                            (L0(1, p, da, db - 1) - L0(1, p, da - 1, db - 1)) * mx[1 + da][cx]);
             };
             auto e5 = [&](int da, int dc) {
                 return h * ((L0(0, p + dc, da - 1, 0) - L0(0, p + dc - 1, da - 1, 0)) * rdzg[pc + dc] +
-                            (L0(2, p + dc - 1, da, 0) - L0(2, p + dc - 1, da - 1, 0)) * rdxg[ic + da]);
+                            (L0(2, p + dc - 1, da, 0) - L0(2, p + dc - 1, da - 1, 0)) * mx[1 + da][cx]);
             };
             auto e6 = [&](int db, int dc) {
                 return h * ((L0(1, p + dc, 0, db - 1) - L0(1, p + dc - 1, 0, db - 1)) * rdzg[pc + dc] +
-                            (L0(2, p + dc - 1, 0, db) - L0(2, p + dc - 1, 0, db - 1)) * rdyg[jj + db]);
+                            (L0(2, p + dc - 1, 0, db) - L0(2, p + dc - 1, 0, db - 1)) * my[1 + db][cy]);
             };
             s[3] = qq * (e4(0, 0) + e4(1, 0) + e4(0, 1) + e4(1, 1));
             s[4] = qq * (e5(0, 0) + e5(1, 0) + e5(0, 1) + e5(1, 1));
@@ -325,26 +345,25 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
             tp[c] = s6 * lane_shift<DPP_WAVE_SHR1>(y) + s23 * y + s6 * lane_shift<DPP_WAVE_SHL1>(y);
         }
         if (out_col) {
-            const int i = ci, j = cj;
             // hat(S) from the filtered velocities at the true index (code/subgrid.f90:456)
             auto uf = [&](int dx, int dy, int dz) { return lf[slot(k + dz)][0][(cx + dx) + CX * (cy + dy)]; };
             auto vf = [&](int dx, int dy, int dz) { return lf[slot(k + dz)][1][(cx + dx) + CX * (cy + dy)]; };
             auto wf = [&](int dx, int dy, int dz) { return lf[slot(k + dz)][2][(cx + dx) + CX * (cy + dy)]; };
             T sh[6];
-            sh[0] = (uf(0, 0, 0) - uf(-1, 0, 0)) * rdx[i];
-            sh[1] = (vf(0, 0, 0) - vf(0, -1, 0)) * rdy[j];
+            sh[0] = (uf(0, 0, 0) - uf(-1, 0, 0)) * mx[3][cx];
+            sh[1] = (vf(0, 0, 0) - vf(0, -1, 0)) * my[0][cy];
             sh[2] = (wf(0, 0, 0) - wf(0, 0, -1)) * rdz[k];
             auto e4 = [&](int da, int db) {
-                return h * ((uf(da - 1, db, 0) - uf(da - 1, db - 1, 0)) * rdyg[j + db] +
-                            (vf(da, db - 1, 0) - vf(da - 1, db - 1, 0)) * rdxg[i + da]);
+                return h * ((uf(da - 1, db, 0) - uf(da - 1, db - 1, 0)) * my[1 + db][cy] +
+                            (vf(da, db - 1, 0) - vf(da - 1, db - 1, 0)) * mx[4 + da][cx]);
             };
             auto e5 = [&](int da, int dc) {
                 return h * ((uf(da - 1, 0, dc) - uf(da - 1, 0, dc - 1)) * rdzg[k + dc] +
-                            (wf(da, 0, dc - 1) - wf(da - 1, 0, dc - 1)) * rdxg[i + da]);
+                            (wf(da, 0, dc - 1) - wf(da - 1, 0, dc - 1)) * mx[4 + da][cx]);
             };
             auto e6 = [&](int db, int dc) {
                 return h * ((vf(0, db - 1, dc) - vf(0, db - 1, dc - 1)) * rdzg[k + dc] +
-                            (wf(0, db, dc - 1) - wf(0, db - 1, dc - 1)) * rdyg[j + db]);
+                            (wf(0, db, dc - 1) - wf(0, db - 1, dc - 1)) * my[1 + db][cy]);
             };
             sh[3] = qq * (e4(0, 0) + e4(1, 0) + e4(0, 1) + e4(1, 1));
             sh[4] = qq * (e5(0, 0) + e5(1, 0) + e5(0, 1) + e5(1, 1));
