@@ -507,6 +507,8 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
     FftPlan fx{}, fz{};
     if (!factor_plan(d.nxp, fx) || !factor_plan(d.nzp, fz))
         return fail(c, CHAN_EINVAL, "Error: nx-2 and nz-2 must factor into 2, 3 and 5");
+    // one block holds a whole line (and its batch) in LDS
+    if (d.nxp > 2048 || d.nzp > 2048) return fail(c, CHAN_EINVAL, "Error: nx-2 and nz-2 must be at most 2048");
 
     int rc = set_device(c);
     if (rc) return rc;
@@ -586,6 +588,12 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
 void chan_destroy(chan_ctx* c)
 {
     if (!c) return;
+    // a context rejected by chan_create's argument checks holds no device
+    // resources: release it without touching the runtime
+    if (!c->stream && c->allocs.empty()) {
+        delete c;
+        return;
+    }
     (void)hipSetDevice(c->device);
     for (int q = 0; q < 3; q++)
         if (c->graph[q]) (void)hipGraphExecDestroy(c->graph[q]);

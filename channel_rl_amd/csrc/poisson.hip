@@ -208,9 +208,20 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftz(Dims d, PoissonBufs pb)
     const long nlines = (long)(d.ny - 1) * d.nkx;
     const long line0 = (long)blockIdx.x * B;
     load_twiddles(tw, pb.fz);
-    for (int q = threadIdx.x; q < N * B; q += blockDim.x) {
+    // every load of the block's lines issued before the first LDS store, so
+    // the HBM round trip is paid once
+    constexpr int MQ = FFT_MAXNB / FFT_THREADS;
+    double2 v[MQ];
+#pragma unroll
+    for (int t = 0; t < MQ; t++) {
+        const int q = threadIdx.x + t * FFT_THREADS;
         const int b = q / N, n = q % N;
-        a[b * ld + n] = (line0 + b < nlines) ? pb.ph[(line0 + b) * N + n] : make_double2(0.0, 0.0);
+        v[t] = (q < N * B && line0 + b < nlines) ? pb.ph[(line0 + b) * N + n] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int t = 0; t < MQ; t++) {
+        const int q = threadIdx.x + t * FFT_THREADS;
+        if (q < N * B) a[(q / N) * ld + q % N] = v[t];
     }
     __syncthreads();
     block_fft(a, tw, pb.fz, B, ld, SIGN);
@@ -318,18 +329,31 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_inv(Dims d, PoissonBufs pb
     const int pair0 = blockIdx.x * B;
     const int half = N / 2;
     load_twiddles(tw, pb.fx);
-    // consecutive lanes read consecutive kz of one kx line
-    for (int q = threadIdx.x; q < d.nkx * B; q += blockDim.x) {
+    // consecutive lanes read consecutive kz of one kx line; all loads are
+    // issued before the first LDS store
+    constexpr int MQ = (FFT_MAXNB / 2 + FFT_THREADS - 1) / FFT_THREADS + 1;
+    double2 va[MQ], vb[MQ];
+#pragma unroll
+    for (int t = 0; t < MQ; t++) {
+        const int q = threadIdx.x + t * FFT_THREADS;
         const int b = q % B, n = q / B;
         const int pr = pair0 + b;
-        double2 xa = make_double2(0.0, 0.0), xb = xa;
-        if (pr < npairs) {
+        va[t] = make_double2(0.0, 0.0);
+        vb[t] = va[t];
+        if (q < d.nkx * B && pr < npairs) {
             int jr, kz;
             pair_rows(d, pr, jr, kz);
             const double2* in = pb.ph + ((long)jr * d.nkx + n) * d.nzp + kz;
-            xa = in[0];
-            xb = in[1];
+            va[t] = in[0];
+            vb[t] = in[1];
         }
+    }
+#pragma unroll
+    for (int t = 0; t < MQ; t++) {
+        const int q = threadIdx.x + t * FFT_THREADS;
+        if (q >= d.nkx * B) break;
+        const int b = q % B, n = q / B;
+        double2 xa = va[t], xb = vb[t];
         if (n == 0 || n == half) { xa.y = 0.0; xb.y = 0.0; }
         a[b * ld + n] = make_double2(xa.x - xb.y, xa.y + xb.x);
         if (n > 0 && n < half) {

@@ -62,6 +62,8 @@ typedef struct {
 
 /* Allocates the device state and sets up the constants of
  * code/initialization.f90:187-427 (grids, kxx/kzz, Dyy, weights, RK3).
+ * nx-2 and nz-2 (the periodic points) must factor into 2, 3 and 5 and be at
+ * most 2048 (CHAN_EINVAL otherwise).
  * On failure *out still holds a context carrying the message for
  * chan_last_error; release it with chan_destroy. */
 int  chan_create(const chan_config* cfg, chan_ctx** out);

@@ -51,3 +51,24 @@ def test_python_front_is_loud_without_library(tmp_path):
             channel_rl_amd.load_library(str(tmp_path / "missing.so"))
         finally:
             channel_rl_amd._LIB = channel_rl_amd._LIB_saved
+
+
+def test_create_rejects_invalid_grids():
+    """chan_create's argument checks run before any device call, so they are
+    exercised here: odd nx, too small a grid, periodic lengths that do not
+    factor into 2, 3, 5 or exceed one block's line (2048)."""
+    import numpy as np
+    import pytest
+    from channel_rl_amd import Channel, ChanError
+
+    def grid(nx, ny, nz):
+        return (np.linspace(0, 6.28, nx), 1.0 + np.tanh(1.5 * np.linspace(-1, 1, ny)) / np.tanh(1.5),
+                np.linspace(0, 3.14, nz))
+
+    for (nx, ny, nz), msg in [((33, 17, 34), "nx must be even"), ((4, 17, 34), "grid too small"),
+                              ((16, 17, 36), "factor into 2, 3 and 5"), ((2162, 9, 34), "at most 2048"),
+                              ((34, 9, 2162), "at most 2048")]:
+        with pytest.raises(ChanError, match=msg):
+            Channel(*grid(nx, ny, nz), nu=1e-3)
+    with pytest.raises(ChanError, match="LES_model"):
+        Channel(*grid(34, 17, 34), nu=1e-3, LES_model=1)
