@@ -23,6 +23,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+# MI355X vector peaks (vendor figures): FP64 78.6 TFLOP/s, FP32 157.3 TFLOP/s
+VALU_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+# algorithmic flops per unique cell and launch, counted from the implemented
+# formulas (an FMA counts 2): the DSM pass A is centre velocities and |S| S_ij
+# (126), the separable 3x3x3 test filter of 15 inputs and 3 staggered
+# velocities (18 x 15 = 270), hat(S) and 4|hat(S)| (109), L_ij, M_ij and the
+# two contractions (62)
+ALG_FLOPS = {"k_dsm_fused": 567}
 
 # Algorithmic HBM values per unique cell per launch (FP64: x8 bytes; FP32
 # bulk: bulk values x4, pressure values x8), the compulsory traffic of each
@@ -229,6 +237,13 @@ def main():
                 "step_ms_profiled": round(tot / args.profile_steps, 3),
                 "copy_kernel_gbs": round(copy_gbs, 1),
                 "step_alg_gbs": round(cells * (976 if prec == cr.FP64 else 632) / (ms_step * 1e-3) / 1e9, 1)}
+    if dom in ALG_FLOPS:
+        # the same kernel against the vector ALU peak of its arithmetic type
+        # (no MFMA on this path): its bound is issue and latency, not HBM
+        pk = VALU_PEAK_TFLOPS["fp64" if prec == cr.FP64 else "fp32"]
+        tf = ALG_FLOPS[dom] * cells / avg_s / 1e12
+        roofline["compute"] = {"unit": "TFLOP/s", "alg_flop_per_cell": ALG_FLOPS[dom], "achieved": round(tf, 2),
+                               "peak": pk, "frac": round(tf / pk, 4)}
 
     fp32 = None
     if args.profile_compare > 0:
