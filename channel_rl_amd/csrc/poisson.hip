@@ -76,6 +76,17 @@ __device__ __forceinline__ void dft(double2 (&v)[R], const double2* tw, int N, i
     }
 }
 
+// Quotient by a block-uniform divisor 1 <= dv <= 2^16 for 0 <= q < 2^16: one
+// high multiply by m = floor((2^32-1)/dv) + 1 (exact in that range; checked
+// exhaustively for dv <= 2048 on the host).  Replaces the ~40-instruction
+// integer division in the butterfly index arithmetic.
+struct Quo16 {
+    unsigned m;
+    int dv;
+};
+__device__ __forceinline__ Quo16 quo16(int dv) { return Quo16{dv > 1 ? 0xFFFFFFFFu / (unsigned)dv + 1u : 0u, dv}; }
+__device__ __forceinline__ int quo(int q, const Quo16& v) { return v.dv > 1 ? (int)__umulhi((unsigned)q, v.m) : q; }
+
 // One in-place Stockham pass of radix R over B transforms (element (n,b) at
 // b*ld + n).  Every thread loads all of its butterflies' inputs, the block
 // synchronises, then the outputs are stored: no second buffer is needed.
@@ -84,14 +95,17 @@ __device__ __forceinline__ void fft_pass(double2* a, const double2* tw, int N, i
 {
     constexpr int MB = (FFT_MAXNB / R + FFT_THREADS - 1) / FFT_THREADS;
     const int NR = N / R, nbf = NR * B, tws = N / (Ns * R);
+    const Quo16 qNR = quo16(NR), qNs = quo16(Ns);
     double2 v[MB][R];
+    int jjv[MB], bv[MB];
 #pragma unroll
     for (int t = 0; t < MB; t++) {
         const int q = threadIdx.x + t * FFT_THREADS;
+        bv[t] = quo(q, qNR);
+        jjv[t] = q - bv[t] * NR;
         if (q < nbf) {
-            const int jj = q % NR, b = q / NR;
 #pragma unroll
-            for (int r = 0; r < R; r++) v[t][r] = a[b * ld + jj + r * NR];
+            for (int r = 0; r < R; r++) v[t][r] = a[bv[t] * ld + jjv[t] + r * NR];
         }
     }
     __syncthreads();
@@ -99,8 +113,9 @@ __device__ __forceinline__ void fft_pass(double2* a, const double2* tw, int N, i
     for (int t = 0; t < MB; t++) {
         const int q = threadIdx.x + t * FFT_THREADS;
         if (q < nbf) {
-            const int jj = q % NR, b = q / NR;
-            const int k = jj % Ns;
+            const int jj = jjv[t], b = bv[t];
+            const int js = quo(jj, qNs);
+            const int k = jj - js * Ns;
             if (Ns > 1) {
 #pragma unroll
                 for (int r = 1; r < R; r++) {
@@ -109,7 +124,7 @@ __device__ __forceinline__ void fft_pass(double2* a, const double2* tw, int N, i
                 }
             }
             dft<R>(v[t], tw, N, sign);
-            const int idxD = (jj / Ns) * Ns * R + k;
+            const int idxD = js * Ns * R + k;
 #pragma unroll
             for (int r = 0; r < R; r++) a[b * ld + idxD + r * Ns] = v[t][r];
         }
