@@ -175,13 +175,14 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_fwd(const T* U, const T* V
 {
     extern __shared__ double2 lds[];
     const int N = d.nxp, B = pb.bx, ld = N + 1;
+    const Quo16 qN = quo16(N), qB = quo16(B);
     double2* tw = lds;
     double2* a = lds + N;
     const int npairs = (d.ny - 1) * (d.nzp / 2);
     const int pair0 = blockIdx.x * B;
     load_twiddles(tw, pb.fx);
     for (int q = threadIdx.x; q < N * B; q += blockDim.x) {
-        const int b = q / N, n = q % N;
+        const int b = quo(q, qN), n = q - b * N;
         const int pr = pair0 + b;
         double2 val = make_double2(0.0, 0.0);
         if (pr < npairs) {
@@ -199,7 +200,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_fwd(const T* U, const T* V
     block_fft(a, tw, pb.fx, B, ld, -1);
     // split Z = F[a + i b] into F[a], F[b]; consecutive lanes write consecutive kz
     for (int q = threadIdx.x; q < d.nkx * B; q += blockDim.x) {
-        const int b = q % B, kx = q / B;
+        const int kx = quo(q, qB), b = q - kx * B;
         const int pr = pair0 + b;
         if (pr >= npairs) continue;
         int jr, kz;
@@ -218,6 +219,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftz(Dims d, PoissonBufs pb)
 {
     extern __shared__ double2 lds[];
     const int N = d.nzp, B = pb.bz, ld = N + 1;
+    const Quo16 qN = quo16(N);
     double2* tw = lds;
     double2* a = lds + N;
     const long nlines = (long)(d.ny - 1) * d.nkx;
@@ -230,18 +232,19 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftz(Dims d, PoissonBufs pb)
 #pragma unroll
     for (int t = 0; t < MQ; t++) {
         const int q = threadIdx.x + t * FFT_THREADS;
-        const int b = q / N, n = q % N;
+        const int b = quo(q, qN), n = q - b * N;
         v[t] = (q < N * B && line0 + b < nlines) ? pb.ph[(line0 + b) * N + n] : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int t = 0; t < MQ; t++) {
         const int q = threadIdx.x + t * FFT_THREADS;
-        if (q < N * B) a[(q / N) * ld + q % N] = v[t];
+        const int b = quo(q, qN);
+        if (q < N * B) a[b * ld + q - b * N] = v[t];
     }
     __syncthreads();
     block_fft(a, tw, pb.fz, B, ld, SIGN);
     for (int q = threadIdx.x; q < N * B; q += blockDim.x) {
-        const int b = q / N, n = q % N;
+        const int b = quo(q, qN), n = q - b * N;
         if (line0 + b < nlines) pb.ph[(line0 + b) * N + n] = a[b * ld + n];
     }
 }
@@ -338,6 +341,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_inv(Dims d, PoissonBufs pb
 {
     extern __shared__ double2 lds[];
     const int N = d.nxp, B = pb.bx, ld = N + 1;
+    const Quo16 qN = quo16(N), qB = quo16(B);
     double2* tw = lds;
     double2* a = lds + N;
     const int npairs = (d.ny - 1) * (d.nzp / 2);
@@ -351,7 +355,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_inv(Dims d, PoissonBufs pb
 #pragma unroll
     for (int t = 0; t < MQ; t++) {
         const int q = threadIdx.x + t * FFT_THREADS;
-        const int b = q % B, n = q / B;
+        const int n = quo(q, qB), b = q - n * B;
         const int pr = pair0 + b;
         va[t] = make_double2(0.0, 0.0);
         vb[t] = va[t];
@@ -367,7 +371,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_inv(Dims d, PoissonBufs pb
     for (int t = 0; t < MQ; t++) {
         const int q = threadIdx.x + t * FFT_THREADS;
         if (q >= d.nkx * B) break;
-        const int b = q % B, n = q / B;
+        const int n = quo(q, qB), b = q - n * B;
         double2 xa = va[t], xb = vb[t];
         if (n == 0 || n == half) { xa.y = 0.0; xb.y = 0.0; }
         a[b * ld + n] = make_double2(xa.x - xb.y, xa.y + xb.x);
@@ -380,7 +384,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_inv(Dims d, PoissonBufs pb
     block_fft(a, tw, pb.fx, B, ld, +1);
     const double scale = (double)((long)d.nxp * d.nzp);
     for (int q = threadIdx.x; q < N * B; q += blockDim.x) {
-        const int b = q / N, n = q % N;
+        const int b = quo(q, qN), n = q - b * N;
         const int pr = pair0 + b;
         if (pr >= npairs) continue;
         int jr, kz;
