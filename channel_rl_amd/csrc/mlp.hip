@@ -82,6 +82,13 @@ __device__ void mlp_load_lds(MlpLds& s, const float* __restrict__ p, const float
     if (tid < 2) { s.ym[tid] = norm[8 + tid]; s.ys[tid] = norm[10 + tid]; }
 }
 
+// Compiler-only fence at the top of a grid-stride loop over sample groups:
+// the network image in LDS is loop-invariant, and without it the compiler
+// keeps every weight a lane reads (~200 values) in registers across the
+// loop, which holds the kernel to one wave per SIMD.  Re-reading them from
+// LDS per group costs little and frees the registers for occupancy.
+__device__ __forceinline__ void lds_reload_fence() { asm volatile("" ::: "memory"); }
+
 // activations of one sample held by a lane: a[4t + r] <-> feature 16t + 4g + r
 struct Act { float a[16]; };
 
@@ -149,6 +156,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const float* __restrict__ p, 
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const int ngroups = (n + 15) / 16;
     for (int grp = blockIdx.x * 4 + wave; grp < ngroups; grp += gridDim.x * 4) {
+        lds_reload_fence();
         const int smp = grp * 16 + lane % 16;
         const bool ok = smp < n;
         float x[4];
@@ -185,6 +193,7 @@ __global__ void __launch_bounds__(256) k_mlp_fwd_bwd(const float* __restrict__ p
     const int ngroups = B / 16;
     const float inv = 1.0f / (float)nvalid;
     for (int grp = blockIdx.x * 4 + wave; grp < ngroups; grp += gridDim.x * 4) {
+        lds_reload_fence();
         const int sl = grp * 16 + lane % 16;  // slot in the batch
         const bool ok = sl < nvalid;
         const long src = ok ? (long)idx[sl] : 0;
@@ -371,6 +380,7 @@ __global__ void __launch_bounds__(256)
     const long ngroups = (ncell + 15) / 16;
     const double dpdx = wm_driving_dpdx(sc, x_mass_cte);
     for (long grp = (long)blockIdx.x * 4 + wave; grp < ngroups; grp += (long)gridDim.x * 4) {
+        lds_reload_fence();
         const long c = grp * 16 + lane % 16;
         const bool ok = c < ncell;
         float x[4] = {0.f, 0.f, 0.f, 0.f};
@@ -394,7 +404,8 @@ __global__ void __launch_bounds__(256)
             const double tpar = sc2 * expm1(o0);
             const double tper = sc2 * exp(o0) * (double)(y[1] * s.ys[1] + s.ym[1]);
             const double tx = tpar * cs - tper * sn;
-            const double ut = loglaw_utau(up, ws.h, nu, kappa, B, yc);
+            // the network's own u_tau starts the equilibrium Newton solve
+            const double ut = loglaw_utau(up, ws.h, nu, kappa, B, yc, tpar > 0.0 ? sqrt(tpar) : 0.0);
             const double teq = ut * ut;
             const double tx_eq = teq * cs;
             // deviation of the whole wall-parallel stress vector

@@ -10,13 +10,17 @@ namespace chan {
 // Log law U+ = ln(y+)/kappa + B above the crossover with U+ = y+, linear
 // below; Newton on f(ut) = ut (ln(h ut/nu)/kappa + B) - |u|.
 // yc is the crossover y+ where ln(y+)/kappa + B = y+ (computed once on the host).
-__device__ inline double loglaw_utau(double u, double h, double nu, double kappa, double B, double yc)
+// guess > 0 (finite) starts Newton there instead of at the viscous estimate:
+// f is increasing and convex on the log branch, so either start converges to
+// the same root (to the 1e-15 stopping test); a close guess saves iterations.
+__device__ inline double loglaw_utau(double u, double h, double nu, double kappa, double B, double yc,
+                                     double guess = 0.0)
 {
     double au = fabs(u);
     if (au == 0.0) return 0.0;
     double ut = sqrt(nu * au / h);
     if (h * ut / nu <= yc) return ut;
-    double g = ut;
+    double g = (guess > 0.0 && guess < 1e300) ? guess : ut;
     for (int it = 0; it < 60; it++) {
         double lg = log(h * g / nu);
         double f = g * (lg / kappa + B) - au;
