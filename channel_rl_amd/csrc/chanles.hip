@@ -97,7 +97,7 @@ struct chan_ctx {
     void* dsm[13] = {};
     const void* rcp[2][6] = {};  // reciprocal spacings [double, float][dx, dy, dz, dxg, dyg, dzg]
     double* part = nullptr;
-    int nblk_dsm = 0, nblk_red = 0;
+    int nblk_dsm = 0, nblk_red = 0, dsm_kchunk = 16;
     double* ratio = nullptr;
     double* alpha = nullptr;
     double* stage = nullptr;
@@ -232,6 +232,7 @@ DsmBufs<T> dsm_bufs(chan_ctx* c)
     b.part = c->part;
     b.ratio = c->ratio;
     b.nblk = c->nblk_dsm;
+    b.kchunk = c->dsm_kchunk;
     const int pi = sizeof(T) == sizeof(double) ? 0 : 1;
     b.rdx = (const T*)c->rcp[pi][0]; b.rdy = (const T*)c->rcp[pi][1]; b.rdz = (const T*)c->rcp[pi][2];
     b.rdxg = (const T*)c->rcp[pi][3]; b.rdyg = (const T*)c->rcp[pi][4]; b.rdzg = (const T*)c->rcp[pi][5];
@@ -551,7 +552,8 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
         }
     }
     // partial sums: DSM rows use one slot per (x,z) block of the fused kernel
-    c->nblk_dsm = dsm_fused_nblk(d);
+    c->dsm_kchunk = c->prec == CHAN_FP64 ? dsm_fused_kchunk<double>(d, c->device) : dsm_fused_kchunk<float>(d, c->device);
+    c->nblk_dsm = dsm_fused_nblk(d, c->dsm_kchunk);
     c->nblk_red = 1024;
     size_t npart = std::max((size_t)(d.nyg + 1) * 2 * c->nblk_dsm, (size_t)3 * c->nblk_red);
     if ((rc = dalloc(c, (void**)&c->part, npart * sizeof(double)))) return rc;

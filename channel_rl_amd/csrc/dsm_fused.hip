@@ -21,6 +21,7 @@
 // reciprocals.  The sums run in a different order than the reference, so the
 // model agrees with it to rounding, not bit for bit.
 // Global traffic: U,V,W read once per tile (plus halo), |S| written once.
+#include <algorithm>
 #include "kernels.hpp"
 #include "prof.hpp"
 
@@ -411,11 +412,40 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     }
 }
 
-static constexpr int DSM_TXO = 30, DSM_TYO = 6, DSM_KCHUNK = 16;
+static constexpr int DSM_TXO = 30, DSM_TYO = 6;
 
-int dsm_fused_nblk(const Dims& d)
+// z-chunk length: every block costs its planes plus the two planes of the
+// pre-roll, and blocks run in rounds of (resident blocks per CU) x (CUs)
+// slots; the chunk that minimises rounds x (planes + 2) wins, so that the
+// last round is as full as it can be (e.g. 192x97x192 FP64, 112 tiles per
+// chunk on 512 slots: 22 planes, 2 rounds, against 16 planes, 3 rounds).
+template <typename T>
+int dsm_fused_kchunk(const Dims& d, int device)
 {
-    return cdiv(d.nxg - 2, DSM_TXO) * cdiv(d.nzg - 2, DSM_KCHUNK);
+    int per_cu = 0, ncu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dsm_fused<T, DSM_TXO, DSM_TYO>,
+                                                     (DSM_TXO + 2) * (DSM_TYO + 2), 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu < 1) ncu = 1;
+    const long slots = (long)per_cu * ncu;
+    const long tiles = (long)cdiv(d.nxg - 2, DSM_TXO) * cdiv(d.nyg - 2, DSM_TYO);
+    const int nk = d.nzg - 2;
+    int best = 16;
+    long best_cost = -1;
+    for (int kc = 8; kc <= 32; kc++) {
+        const long rounds = (tiles * cdiv(nk, kc) + slots - 1) / slots;
+        const long cost = rounds * (std::min(kc, nk) + 2);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = kc;
+        }
+    }
+    return best;
+}
+
+int dsm_fused_nblk(const Dims& d, int kchunk)
+{
+    return cdiv(d.nxg - 2, DSM_TXO) * cdiv(d.nzg - 2, kchunk);
 }
 
 template <typename T>
@@ -424,12 +454,14 @@ void launch_dsm_fused(const T* U, const T* V, const T* W, const Metric<T>& m, co
 {
     (void)m;
     dim3 blk((DSM_TXO + 2) * (DSM_TYO + 2));
-    dim3 grd(cdiv(d.nxg - 2, DSM_TXO), cdiv(d.nyg - 2, DSM_TYO), cdiv(d.nzg - 2, DSM_KCHUNK));
+    dim3 grd(cdiv(d.nxg - 2, DSM_TXO), cdiv(d.nyg - 2, DSM_TYO), cdiv(d.nzg - 2, b.kchunk));
     PROF_BEGIN("k_dsm_fused", s);
-    k_dsm_fused<T, DSM_TXO, DSM_TYO><<<grd, blk, 0, s>>>(U, V, W, d, b, DSM_KCHUNK);
+    k_dsm_fused<T, DSM_TXO, DSM_TYO><<<grd, blk, 0, s>>>(U, V, W, d, b, b.kchunk);
     PROF_END(s);
 }
 
+template int dsm_fused_kchunk<double>(const Dims&, int);
+template int dsm_fused_kchunk<float>(const Dims&, int);
 template void launch_dsm_fused<double>(const double*, const double*, const double*, const Metric<double>&,
                                        const Dims&, DsmBufs<double>&, hipStream_t);
 template void launch_dsm_fused<float>(const float*, const float*, const float*, const Metric<float>&, const Dims&,

@@ -16,6 +16,7 @@ struct DsmBufs {
     double* part;  // per-block partial sums: [nyg][2][nblk]
     double* ratio; // <LM>/<MM> per row (index j-1); also LM, MM at +nyg, +2nyg
     int nblk;      // partial-sum blocks per row
+    int kchunk;    // z planes per block of the fused pass A
     // reciprocal spacings, 1-based: 1/(x(i)-x(i-1)), 1/(xg(i)-xg(i-1)), ...
     const T *rdx, *rdy, *rdz, *rdxg, *rdyg, *rdzg;
 };
@@ -48,8 +49,11 @@ template <typename T>
 void launch_amd(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, const Dims& d, int dirichlet,
                 const Scalars* sc, int fuse_wm, const double* alpha, T nu, hipStream_t s);
 
-// fused pass A of the dynamic Smagorinsky model (dsm_fused.hip)
-int dsm_fused_nblk(const Dims& d);
+// fused pass A of the dynamic Smagorinsky model (dsm_fused.hip): z-chunk
+// length for this grid and device, and the partial-sum blocks per row it gives
+template <typename T>
+int dsm_fused_kchunk(const Dims& d, int device);
+int dsm_fused_nblk(const Dims& d, int kchunk);
 template <typename T>
 void launch_dsm_fused(const T* U, const T* V, const T* W, const Metric<T>& m, const Dims& d, DsmBufs<T>& b,
                       hipStream_t s);
