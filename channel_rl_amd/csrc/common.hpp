@@ -94,6 +94,26 @@ struct Scalars {
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
+// z-chunk length in [lo, hi] for a kernel whose blocks each march `kc` of
+// nk planes over one of `tiles` (x,y) tiles, with two planes of pre-roll:
+// blocks run in rounds of `slots` (resident blocks per CU x CUs), so the
+// chunk that minimises rounds x (planes + 2) keeps the last round as full as
+// it can be.
+inline int round_filling_chunk(long tiles, int nk, long slots, int lo, int hi)
+{
+    int best = lo;
+    long best_cost = -1;
+    for (int kc = lo; kc <= hi; kc++) {
+        const long rounds = (tiles * cdiv(nk, kc) + slots - 1) / slots;
+        const long cost = rounds * ((kc < nk ? kc : nk) + 2);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = kc;
+        }
+    }
+    return best;
+}
+
 // Tile order across the 8 XCDs of MI355X.  The dispatcher deals workgroups
 // to the XCDs in turn (workgroup n to XCD n mod 8) and each XCD has its own
 // L2, so neighbouring tiles in launch order land in different L2s and their

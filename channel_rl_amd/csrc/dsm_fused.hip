@@ -21,7 +21,6 @@
 // reciprocals.  The sums run in a different order than the reference, so the
 // model agrees with it to rounding, not bit for bit.
 // Global traffic: U,V,W read once per tile (plus halo), |S| written once.
-#include <algorithm>
 #include "kernels.hpp"
 #include "prof.hpp"
 
@@ -414,11 +413,9 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
 
 static constexpr int DSM_TXO = 30, DSM_TYO = 6;
 
-// z-chunk length: every block costs its planes plus the two planes of the
-// pre-roll, and blocks run in rounds of (resident blocks per CU) x (CUs)
-// slots; the chunk that minimises rounds x (planes + 2) wins, so that the
-// last round is as full as it can be (e.g. 192x97x192 FP64, 112 tiles per
-// chunk on 512 slots: 22 planes, 2 rounds, against 16 planes, 3 rounds).
+// z-chunk length filling the last round of blocks (e.g. 192x97x192 FP64,
+// 112 tiles per chunk on 512 slots: 22 planes, 2 rounds, against 16 planes,
+// 3 rounds).
 template <typename T>
 int dsm_fused_kchunk(const Dims& d, int device)
 {
@@ -427,20 +424,8 @@ int dsm_fused_kchunk(const Dims& d, int device)
                                                      (DSM_TXO + 2) * (DSM_TYO + 2), 0) != hipSuccess || per_cu < 1)
         per_cu = 1;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu < 1) ncu = 1;
-    const long slots = (long)per_cu * ncu;
     const long tiles = (long)cdiv(d.nxg - 2, DSM_TXO) * cdiv(d.nyg - 2, DSM_TYO);
-    const int nk = d.nzg - 2;
-    int best = 16;
-    long best_cost = -1;
-    for (int kc = 8; kc <= 32; kc++) {
-        const long rounds = (tiles * cdiv(nk, kc) + slots - 1) / slots;
-        const long cost = rounds * (std::min(kc, nk) + 2);
-        if (best_cost < 0 || cost < best_cost) {
-            best_cost = cost;
-            best = kc;
-        }
-    }
-    return best;
+    return round_filling_chunk(tiles, d.nzg - 2, (long)per_cu * ncu, 8, 32);
 }
 
 int dsm_fused_nblk(const Dims& d, int kchunk)

@@ -280,11 +280,27 @@ __global__ void __launch_bounds__(TX* TY)
 }
 
 // Tile shapes (x cells, y cells) and z-chunk lengths the launcher offers;
-// chan_autotune times them on the channel at hand.
+// chan_autotune times them on the channel at hand.  kchunk 0: the chunk that
+// fills the last round of blocks on this grid and device
+// (round_filling_chunk with the kernel's occupancy).
 struct RhsShape { int tx, ty, kchunk; };
 static const RhsShape kRhsShapes[] = {
     {64, 4, 16}, {64, 4, 8}, {64, 8, 8}, {64, 8, 16}, {32, 8, 8}, {32, 8, 16}, {64, 4, 32}, {64, 8, 32},
+    {64, 4, 0},  {32, 8, 0}, {64, 8, 0},
 };
+
+template <typename T, int TX, int TY>
+int rhs_round_chunk(int ni, int nj, int nk)
+{
+    int per_cu = 0, ncu = 0, dev = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rhs_rk<T, TX, TY>, TX * TY, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+        ncu = 1;
+    return round_filling_chunk((long)cdiv(ni, TX) * cdiv(nj, TY), nk, (long)per_cu * ncu, 6, 32);
+}
 int rhs_shape_count() { return (int)(sizeof(kRhsShapes) / sizeof(kRhsShapes[0])); }
 
 template <typename T>
@@ -296,11 +312,14 @@ void launch_rhs_rk(const T* U, const T* V, const T* W, const T* nut, const RhsTa
     const int ni = (mode == 0 ? d.nxg - 1 : d.nxg - 2) - 1;
     const int nk = (mode == 0 ? d.nzg - 1 : d.nzg - 2) - 1;
     dim3 blk(sh.tx, sh.ty, 1);
-    dim3 grd(cdiv(ni, sh.tx), cdiv(d.nyg - 2, sh.ty), cdiv(nk, sh.kchunk));
     PROF_BEGIN("k_rhs_rk", s);
-#define RHS_GO(TX, TY)                                                                                          \
-    k_rhs_rk<T, TX, TY><<<grd, blk, 0, s>>>(U, V, W, nut, m, d, nu, sc, mode, Uo, Vo, Wo, F1u, F1v, F1w, Ou, Ov, \
-                                            Ow, T(c1), T(c2), sh.kchunk)
+#define RHS_GO(TX, TY)                                                                                              \
+    do {                                                                                                            \
+        const int kc = sh.kchunk > 0 ? sh.kchunk : rhs_round_chunk<T, TX, TY>(ni, d.nyg - 2, nk);                 \
+        dim3 grd(cdiv(ni, TX), cdiv(d.nyg - 2, TY), cdiv(nk, kc));                                                  \
+        k_rhs_rk<T, TX, TY><<<grd, blk, 0, s>>>(U, V, W, nut, m, d, nu, sc, mode, Uo, Vo, Wo, F1u, F1v, F1w, Ou, Ov, \
+                                                Ow, T(c1), T(c2), kc);                                              \
+    } while (0)
     if (sh.tx == 64 && sh.ty == 4) RHS_GO(64, 4);
     else if (sh.tx == 64 && sh.ty == 8) RHS_GO(64, 8);
     else RHS_GO(32, 8);
