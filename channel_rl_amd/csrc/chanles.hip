@@ -576,7 +576,7 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
     if ((rc = make_twiddles(c, c->pb.fx)) || (rc = make_twiddles(c, c->pb.fz))) return rc;
     c->pb.bx = std::max(1, std::min(16, 2048 / d.nxp));
     c->pb.bz = std::max(1, std::min(16, 2048 / d.nzp));
-    if ((rc = dalloc(c, (void**)&c->pb.tri, (size_t)(d.ny - 1) * d.nzp * d.nkx * sizeof(double2)))) return rc;
+    if ((rc = dalloc(c, (void**)&c->pb.tri, (size_t)2 * (d.ny - 1) * d.nzp * d.nkx * sizeof(double)))) return rc;
     launch_tridiag_setup(d, c->pb, c->stream);
     if ((rc = check_launch(c))) return rc;
     Scalars s0{};

@@ -125,7 +125,7 @@ struct PoissonBufs {
     const double* du;   // Dyy(j,j+1)
     FftPlan fx, fz;
     int bx, bz;         // transforms per block in the x and z passes
-    double2* tri;       // per mode and row: (c'_j, 1/den_j) of the elimination, [row][mode]
+    double* tri;        // pivots of the elimination, [row][mode]: c'_j, then 1/den_j at +rows*modes
 };
 
 template <typename T>

@@ -249,19 +249,19 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftz(Dims d, PoissonBufs pb)
     }
 }
 
-// code/projection.f90:90-113: (Dyy + kxx + kzz) p = r per mode.  Lane pairs
-// share a mode (real, imaginary part); c' is kept in LDS for the back
-// substitution, d' is written over the right-hand side.
-// Thomas algorithm per (kx, kz) mode.  The matrix Dyy + kxx + kzz (with the
-// 00 mode's first row scaled by 3/2, code/projection.f90:107) is real and the
-// same for every solve of the run, so the forward elimination's pivots
-// 1/den_j and the modified upper diagonal c'_j are computed once per mode
-// (k_tridiag_setup, the same arithmetic the elimination would do) and kept
-// in HBM as pairs [row][mode] = (c'_j, 1/den_j).  A solve then reads the
-// right-hand side once forward and once backward with no divisions and no
-// LDS, so occupancy is not limited by the number of rows.
+// code/projection.f90:90-113: (Dyy + kxx + kzz) p = r per (kx, kz) mode by
+// the Thomas algorithm, real and imaginary parts on neighbouring lanes.  The
+// matrix (with the 00 mode's first row scaled by 3/2, code/projection.f90:107)
+// is real and the same for every solve of the run, so the forward
+// elimination's pivots 1/den_j and the modified upper diagonal c'_j are
+// computed once per mode (k_tridiag_setup, the same arithmetic the
+// elimination would do) and kept in HBM as two [row][mode] planes, each sweep
+// reading only the one it needs.  A solve has no divisions and no LDS; the
+// last TRI_KEEP rows of the forward sweep stay in registers for the back
+// substitution instead of going through memory.
 static constexpr int TRI_THREADS = 64;
 static constexpr int TRI_UNROLL = 12;
+static constexpr int TRI_KEEP = 48;
 
 __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_setup(Dims d, PoissonBufs pb)
 {
@@ -270,18 +270,22 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_setup(Dims d, PoissonBu
     if (mode >= nmodes) return;
     const int kx = mode / d.nzp, kz = mode % d.nzp;
     const int n = d.ny - 1;
+    double* tc = pb.tri;
+    double* ti = pb.tri + (long)n * nmodes;
     const double lx = pb.kxx[kx], lz = pb.kzz[kz];
     double den = (pb.dd[0] + lx) + lz;
     if (kx == 0 && kz == 0) den = 3.0 / 2.0 * den;
     double inv = 1.0 / den;
     double c = pb.du[0] * inv;
-    pb.tri[mode] = make_double2(c, inv);
+    tc[mode] = c;
+    ti[mode] = inv;
     for (int row = 1; row < n; row++) {
         const double l = pb.dl[row - 1];
         den = ((pb.dd[row] + lx) + lz) - l * c;
         inv = 1.0 / den;
         c = row < n - 1 ? pb.du[row] * inv : 0.0;
-        pb.tri[(long)row * nmodes + mode] = make_double2(c, inv);
+        tc[(long)row * nmodes + mode] = c;
+        ti[(long)row * nmodes + mode] = inv;
     }
 }
 
@@ -293,37 +297,72 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
     if (g >= 2L * nmodes) return;
     const int mode = (int)(g >> 1);
     const int n = d.ny - 1;
+    const int nk = n < TRI_KEEP ? n : TRI_KEEP;  // rows n-nk..n-1 stay in registers
+    const int n0 = n - nk;
     const long plane = 2L * nmodes;
     double* ph = reinterpret_cast<double*>(pb.ph) + g;
-    const double2* tri = pb.tri + mode;
-    double dp = ph[0] * tri[0].y;
-    ph[0] = dp;
-    int row = 1;
-    for (; row + TRI_UNROLL <= n; row += TRI_UNROLL) {
+    const double* tc = pb.tri + mode;
+    const double* ti = pb.tri + (long)n * nmodes + mode;
+    // forward: row 0 has no lower coupling (l = 0 leaves r * inv exact)
+    double dp = 0.0;
+    int row = 0;
+    for (; row + TRI_UNROLL <= n0; row += TRI_UNROLL) {
         double r[TRI_UNROLL], iv[TRI_UNROLL];
 #pragma unroll
         for (int u = 0; u < TRI_UNROLL; u++) {
             r[u] = ph[(row + u) * plane];
-            iv[u] = tri[(long)(row + u) * nmodes].y;
+            iv[u] = ti[(long)(row + u) * nmodes];
         }
 #pragma unroll
         for (int u = 0; u < TRI_UNROLL; u++) {
-            dp = (r[u] - pb.dl[row + u - 1] * dp) * iv[u];
+            const double l = row + u > 0 ? pb.dl[row + u - 1] : 0.0;
+            dp = (r[u] - l * dp) * iv[u];
             ph[(row + u) * plane] = dp;
         }
     }
-    for (; row < n; row++) {
-        dp = (ph[row * plane] - pb.dl[row - 1] * dp) * tri[(long)row * nmodes].y;
+    for (; row < n0; row++) {
+        const double l = row > 0 ? pb.dl[row - 1] : 0.0;
+        dp = (ph[row * plane] - l * dp) * ti[(long)row * nmodes];
         ph[row * plane] = dp;
     }
-    double xn = dp;
-    row = n - 2;
+    double keep[TRI_KEEP];
+    {
+        double r[TRI_KEEP], iv[TRI_KEEP];
+#pragma unroll
+        for (int u = 0; u < TRI_KEEP; u++) {
+            r[u] = u < nk ? ph[(n0 + u) * plane] : 0.0;
+            iv[u] = u < nk ? ti[(long)(n0 + u) * nmodes] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < TRI_KEEP; u++) {
+            if (u < nk) {
+                const double l = n0 + u > 0 ? pb.dl[n0 + u - 1] : 0.0;
+                dp = (r[u] - l * dp) * iv[u];
+            }
+            keep[u] = dp;
+        }
+    }
+    // back substitution: x(n-1) = dp(n-1), x(j) = dp(j) - c'(j) x(j+1)
+    double xn = 0.0;
+    {
+        double cc[TRI_KEEP];
+#pragma unroll
+        for (int u = 0; u < TRI_KEEP; u++) cc[u] = u < nk ? tc[(long)(n0 + u) * nmodes] : 0.0;
+#pragma unroll
+        for (int u = TRI_KEEP - 1; u >= 0; u--) {
+            if (u < nk) {
+                xn = u == nk - 1 ? keep[u] : keep[u] - cc[u] * xn;
+                ph[(n0 + u) * plane] = xn;
+            }
+        }
+    }
+    row = n0 - 1;
     for (; row - TRI_UNROLL + 1 >= 0; row -= TRI_UNROLL) {
         double r[TRI_UNROLL], cc[TRI_UNROLL];
 #pragma unroll
         for (int u = 0; u < TRI_UNROLL; u++) {
             r[u] = ph[(row - u) * plane];
-            cc[u] = tri[(long)(row - u) * nmodes].x;
+            cc[u] = tc[(long)(row - u) * nmodes];
         }
 #pragma unroll
         for (int u = 0; u < TRI_UNROLL; u++) {
@@ -332,7 +371,7 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
         }
     }
     for (; row >= 0; row--) {
-        xn = ph[row * plane] - tri[(long)row * nmodes].x * xn;
+        xn = ph[row * plane] - tc[(long)row * nmodes] * xn;
         ph[row * plane] = xn;
     }
 }
