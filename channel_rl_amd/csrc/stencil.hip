@@ -198,8 +198,11 @@ __global__ void k_equilibrium_alpha(const T* U, double* alpha, Metric<double> m,
         uw = wb * (double)U[at(d, i, jt, k)] + (1.0 - wb) * (double)U[at(d, i, jt - 1, k)];
         h = h_wm > 0.0 ? hb : y[d.ny] - yg[d.nyg - 1];
     }
-    const double t = loglaw_utau(uw, h, nu, kappa, B, yc);
-    alpha[(i - 1) + (long)d.nx * (wall + 2L * (k - 1))] = (uw >= 0 ? 1.0 : -1.0) * t * t;
+    // Newton starts from the cell's previous u_tau (|alpha_x| = u_tau^2): the
+    // flow moves little in a step, so a few iterations reach the same root
+    const long ia = (i - 1) + (long)d.nx * (wall + 2L * (k - 1));
+    const double t = loglaw_utau(uw, h, nu, kappa, B, yc, sqrt(fabs(alpha[ia])));
+    alpha[ia] = (uw >= 0 ? 1.0 : -1.0) * t * t;
 }
 
 template <typename T>
