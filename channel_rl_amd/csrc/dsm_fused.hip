@@ -5,15 +5,16 @@
 //
 // Design: a block owns an x-y tile of TXO x TYO output cells and marches in
 // z over a chunk of planes.  Threads map one-to-one onto the tile plus a halo
-// of one ("columns"), x fastest.  Per plane, three phases separated by three
-// barriers:
-//   1  y-z taps of the staggered-velocity filter for every level-0 position
-//      (level 0 = U,V,W of the tile plus a halo of two, three planes, in LDS,
-//      loaded with periodic index wrap: no ghost cells are read);
-//   2  level 1 per column: centre velocities a, |S| S_ij, the z taps of the
-//      filter inputs (running sums in registers, the finished plane to LDS),
-//      the x taps of the filtered staggered velocities (to LDS, three planes);
-//   3  per output: the y taps of the filter inputs from LDS, the x taps by
+// of one ("columns"), x fastest.  Level 0 = U,V,W of the tile plus a halo
+// of two, three planes, in LDS, loaded with periodic index wrap (no ghost
+// cells are read).  Per plane, two phases separated by two barriers:
+//   1  per column: the y-z taps of the staggered-velocity filter at the
+//      column's level-0 position (the row ends also at the halo position
+//      beyond them), their x taps by cross-lane moves (filtered velocities
+//      to LDS, three planes); level 1: centre velocities a, |S| S_ij, and
+//      the z taps of the filter inputs (running sums in registers, the
+//      finished plane to LDS);
+//   2  per output: the y taps of the filter inputs from LDS, the x taps by
 //      cross-lane moves, hat(S) from the filtered velocities, L:M and M:M;
 //      the next level-0 plane is stored and the one after it fetched.
 // The test filter is separable (y weights one-sided on rows 2 and nyg-1 as
@@ -30,7 +31,6 @@ template <int TXO, int TYO>
 struct DsmTile {
     static constexpr int CX = TXO + 2, CY = TYO + 2, NC = CX * CY;  // columns
     static constexpr int LX = TXO + 4, LY = TYO + 4, NL = LX * LY;  // level-0 plane
-    static constexpr int NG = LX * (LY - 2);                         // y-z combined rows 1..LY-2
 };
 
 // Cross-lane moves along x.  Columns are x-fastest with CX columns per row
@@ -66,12 +66,11 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     // third of the FP64 instructions and changes results only by rounding.
 #pragma clang fp contract(fast)
     using Tl = DsmTile<TXO, TYO>;
-    constexpr int CX = Tl::CX, NC = Tl::NC, LX = Tl::LX, NL = Tl::NL, NG = Tl::NG;
+    constexpr int CX = Tl::CX, NC = Tl::NC, LX = Tl::LX, NL = Tl::NL;
     static_assert(64 % CX == 0 && NC % 64 == 0, "rows of columns must tile the waves");
     __shared__ T l0[3][3][NL];  // [slot][U,V,W][position]
     __shared__ T lf[3][3][NC];  // [slot][Uf,Vf,Wf][column]
     __shared__ T zq[15][NC];    // z-combined filter inputs: a(3), a_a a_b (6), |S|S_ij (6)
-    __shared__ T gq[3][NG];     // y-z combined level 0, level-0 rows 1..LY-2
 
     const int tid = threadIdx.x;
     const int cx = tid % CX, cy = tid / CX;
@@ -164,54 +163,57 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
 
     // y and z taps of the filter of the staggered velocities (code/subgrid.f90:
     // 140-190 over the whole y range, one-sided at the first/last row of each
-    // field) for the level-0 positions of rows 1..LY-2.  Each thread owns
-    // positions g = tid + r NC; the y taps of a plane are taken once and
-    // enter the z taps of three planes through two running sums (ga: the
-    // next plane's, two terms in; gb: the one after, one term in).  Blocks
-    // clear of both walls (every row interior to U, V and W) take the uniform
-    // weights without per-position tests.
-    constexpr int NGT = (NG + NC - 1) / NC;
-    const bool interior_rows = j0 - 1 >= 2 && j0 - 2 + Tl::LY - 2 <= d.ny - 1;
-    T ga[NGT][3], gb[NGT][3];
-    auto ycomb = [&](int p, T out[NGT][3]) {
+    // field).  Each column takes them at its own level-0 position (cx+1,
+    // cy+1) and, for the x taps beyond the row ends, the first and last
+    // column also at the halo position next to them (lx = 0, LX-1; the other
+    // lanes repeat their own).  The y taps of a plane are taken once and
+    // enter the z taps of three planes through two running sums (ga: the next
+    // plane's, two terms in; gb: the one after, one term in).  The y weights
+    // depend on the column's row only: zero outside the field's rows.
+    T wy[2][3];  // [U and W, V][below, centre, above]
+    {
+        const int jj = cj;
+        for (int t = 0; t < 2; t++) {
+            const int n2 = t == 0 ? d.nyg : d.ny;
+            const bool in = jj >= 1 && jj <= n2;
+            wy[t][0] = !in || jj == 1 ? T(0) : (jj == n2 ? s13 : s6);
+            wy[t][1] = in ? s23 : T(0);
+            wy[t][2] = !in || jj == n2 ? T(0) : (jj == 1 ? s13 : s6);
+        }
+    }
+    const int qown = (cx + 1) + LX * (cy + 1);
+    const int qext = (cx == 0 ? 0 : (cx == CX - 1 ? LX - 1 : cx + 1)) + LX * (cy + 1);
+    T ga[2][3], gb[2][3];  // [own, halo position][U,V,W]
+    auto ycomb = [&](int p, T out[2][3]) {
         const int s = slot(p);
-#pragma unroll
-        for (int r = 0; r < NGT; r++) {
-            const int g = tid + r * NC;
-            const int q = g + LX;
-            for (int f = 0; f < 3; f++) out[r][f] = T(0);
-            if (g >= NG) continue;
-            if (interior_rows) {
-                for (int f = 0; f < 3; f++) {
-                    const T* P = l0[s][f];
-                    out[r][f] = s6 * P[q - LX] + s23 * P[q] + s6 * P[q + LX];
-                }
-            } else {
-                const int jj = j0 - 2 + q / LX;
-                for (int f = 0; f < 3; f++) {
-                    const int n2 = (f == 1) ? d.ny : d.nyg;
-                    const T* P = l0[s][f];
-                    if (jj < 1 || jj > n2) out[r][f] = T(0);
-                    else if (jj == 1) out[r][f] = s23 * P[q] + s13 * P[q + LX];
-                    else if (jj == n2) out[r][f] = s13 * P[q - LX] + s23 * P[q];
-                    else out[r][f] = s6 * P[q - LX] + s23 * P[q] + s6 * P[q + LX];
-                }
-            }
+        for (int f = 0; f < 3; f++) {
+            const T* P = l0[s][f];
+            const T* w = wy[f == 1 ? 1 : 0];
+            out[0][f] = w[0] * P[qown - LX] + w[1] * P[qown] + w[2] * P[qown + LX];
+            out[1][f] = w[0] * P[qext - LX] + w[1] * P[qext] + w[2] * P[qext + LX];
         }
     };
-    // gq at plane p: the running sums hold planes p-1 and p, the y taps of
-    // plane p+1 complete it
-    auto yz_combine = [&](int p) {
-        T ycp[NGT][3];
-        ycomb(p + 1, ycp);
-#pragma unroll
-        for (int r = 0; r < NGT; r++) {
-            const int g = tid + r * NC;
-            for (int f = 0; f < 3; f++) {
-                if (g < NG) gq[f][g] = ga[r][f] + s6 * ycp[r][f];
-                ga[r][f] = gb[r][f] + s23 * ycp[r][f];
-                gb[r][f] = s6 * ycp[r][f];
+    // filtered staggered velocities of the column at plane p into LDS: the
+    // running sums hold planes p-1 and p, the y taps of plane p+1 complete
+    // the z taps; x taps from the neighbouring lanes (the halo position at
+    // the row ends).  Every lane of the wave must take part.
+    auto filt_vel = [&](int p) {
+        T yc[2][3];
+        ycomb(p + 1, yc);
+        const int sf = slot(p);
+        for (int f = 0; f < 3; f++) {
+            const T own = ga[0][f] + s6 * yc[0][f];
+            const T ext = ga[1][f] + s6 * yc[1][f];
+            for (int e = 0; e < 2; e++) {
+                ga[e][f] = gb[e][f] + s23 * yc[e][f];
+                gb[e][f] = s6 * yc[e][f];
             }
+            // the moves run on every lane before the selection: inside a
+            // conditional they would read disabled lanes as 0
+            const T sl = lane_shift<DPP_WAVE_SHR1>(own), sr = lane_shift<DPP_WAVE_SHL1>(own);
+            const T left = cx == 0 ? ext : sl;
+            const T right = cx == CX - 1 ? ext : sr;
+            lf[sf][f][tid] = s6 * left + s23 * own + s6 * right;
         }
     };
 
@@ -219,8 +221,7 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
 
     // level 1 of this column at plane p: the filter inputs Q = a (3),
-    // a_a a_b (6), |S| S_ij (6); |S| of the output planes to global memory,
-    // the x taps of the filtered velocities to LDS
+    // a_a a_b (6), |S| S_ij (6); |S| of the output planes to global memory
     auto level1 = [&](int p, T Q[15]) {
         const int pc = wz(p);
         T a[3] = {T(0), T(0), T(0)}, ss[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
@@ -265,10 +266,6 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
             Q[9 + c] = ss[c];
         }
         if (out_col && p >= kb && p <= ke) b.smag[at(d, ci, cj, p)] = sm;
-        // x taps of the staggered-velocity filter (Simpson 1/6, 2/3, 1/6)
-        const int sf = slot(p);
-        const int g0 = (cx + 1) + LX * cy;  // gq index of level-0 position (cx+1, cy+1)
-        for (int f = 0; f < 3; f++) lf[sf][f][tid] = s6 * gq[f][g0 - 1] + s23 * gq[f][g0] + s6 * gq[f][g0 + 1];
     };
 
     // z taps of the filter inputs as running sums: za for plane k (planes k-1
@@ -280,28 +277,24 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     load_plane(kb);
     __syncthreads();
     {
-        T yc[NGT][3];
+        T yc[2][3];
         ycomb(kb - 2, yc);
-#pragma unroll
-        for (int r = 0; r < NGT; r++)
-            for (int f = 0; f < 3; f++) ga[r][f] = s6 * yc[r][f];
+        for (int e = 0; e < 2; e++)
+            for (int f = 0; f < 3; f++) ga[e][f] = s6 * yc[e][f];
         ycomb(kb - 1, yc);
-#pragma unroll
-        for (int r = 0; r < NGT; r++)
+        for (int e = 0; e < 2; e++)
             for (int f = 0; f < 3; f++) {
-                ga[r][f] = ga[r][f] + s23 * yc[r][f];
-                gb[r][f] = s6 * yc[r][f];
+                ga[e][f] = ga[e][f] + s23 * yc[e][f];
+                gb[e][f] = s6 * yc[e][f];
             }
     }
-    yz_combine(kb - 1);
-    __syncthreads();
+    filt_vel(kb - 1);
     level1(kb - 1, Q);
     for (int c = 0; c < 15; c++) za[c] = s6 * Q[c];
     __syncthreads();
     load_plane(kb + 1);
     __syncthreads();
-    yz_combine(kb);
-    __syncthreads();
+    filt_vel(kb);
     level1(kb, Q);
     for (int c = 0; c < 15; c++) {
         za[c] = za[c] + s23 * Q[c];
@@ -321,9 +314,7 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
 
     for (int k = kb; k <= ke; k++) {
         // phase 1: l0 holds planes k..k+2
-        yz_combine(k + 1);
-        __syncthreads();
-        // phase 2
+        filt_vel(k + 1);
         level1(k + 1, Q);
         T zown[15];  // z taps of the filter inputs at plane k
         for (int c = 0; c < 15; c++) {
@@ -333,7 +324,7 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
         }
         for (int c = 0; c < 15; c++) zq[c][tid] = zown[c];
         __syncthreads();
-        // phase 3: x-y taps of the 15 filter inputs, every lane (cross-lane moves)
+        // phase 2: x-y taps of the 15 filter inputs, every lane (cross-lane moves)
         T tp[15];
         if (cy >= 1 && cy <= TYO) {
             for (int c = 0; c < 15; c++) tp[c] = wym * zq[c][om] + s23 * zown[c] + wyp * zq[c][op];
@@ -386,7 +377,7 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
             acc_lm += (double)lm;
             acc_mm += (double)mm;
         }
-        // next level-0 plane into the slot of plane k (last read in phase 2)
+        // next level-0 plane into the slot of plane k (last read in phase 1)
         if (k + 3 <= ke + 2) sstore(k + 3);
         if (k + 4 <= ke + 2) gload(k + 4);
         __syncthreads();
