@@ -402,7 +402,15 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     }
 }
 
-static constexpr int DSM_TXO = 30, DSM_TYO = 6;
+// Tile: 30 x-columns (a row of 32 columns is half a wave); y rows by type:
+// FP64 30x14 outputs (512 threads, 144 KB LDS, one block of 8 waves per CU:
+// less halo work per output than 30x6 at the same occupancy), FP32 30x6
+// (256 threads, 40 KB: three blocks per CU, as its registers allow).
+static constexpr int DSM_TXO = 30;
+template <typename T>
+struct DsmRows {
+    static constexpr int TYO = sizeof(T) == 8 ? 14 : 6;
+};
 
 // z-chunk length filling the last round of blocks (e.g. 192x97x192 FP64,
 // 112 tiles per chunk on 512 slots: 22 planes, 2 rounds, against 16 planes,
@@ -411,11 +419,13 @@ template <typename T>
 int dsm_fused_kchunk(const Dims& d, int device)
 {
     int per_cu = 0, ncu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dsm_fused<T, DSM_TXO, DSM_TYO>,
-                                                     (DSM_TXO + 2) * (DSM_TYO + 2), 0) != hipSuccess || per_cu < 1)
+    constexpr int TYO = DsmRows<T>::TYO;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dsm_fused<T, DSM_TXO, TYO>, (DSM_TXO + 2) * (TYO + 2),
+                                                     0) != hipSuccess ||
+        per_cu < 1)
         per_cu = 1;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu < 1) ncu = 1;
-    const long tiles = (long)cdiv(d.nxg - 2, DSM_TXO) * cdiv(d.nyg - 2, DSM_TYO);
+    const long tiles = (long)cdiv(d.nxg - 2, DSM_TXO) * cdiv(d.nyg - 2, TYO);
     return round_filling_chunk(tiles, d.nzg - 2, (long)per_cu * ncu, 8, 32);
 }
 
@@ -429,10 +439,11 @@ void launch_dsm_fused(const T* U, const T* V, const T* W, const Metric<T>& m, co
                       hipStream_t s)
 {
     (void)m;
-    dim3 blk((DSM_TXO + 2) * (DSM_TYO + 2));
-    dim3 grd(cdiv(d.nxg - 2, DSM_TXO), cdiv(d.nyg - 2, DSM_TYO), cdiv(d.nzg - 2, b.kchunk));
+    constexpr int TYO = DsmRows<T>::TYO;
+    dim3 blk((DSM_TXO + 2) * (TYO + 2));
+    dim3 grd(cdiv(d.nxg - 2, DSM_TXO), cdiv(d.nyg - 2, TYO), cdiv(d.nzg - 2, b.kchunk));
     PROF_BEGIN("k_dsm_fused", s);
-    k_dsm_fused<T, DSM_TXO, DSM_TYO><<<grd, blk, 0, s>>>(U, V, W, d, b, b.kchunk);
+    k_dsm_fused<T, DSM_TXO, TYO><<<grd, blk, 0, s>>>(U, V, W, d, b, b.kchunk);
     PROF_END(s);
 }
 
