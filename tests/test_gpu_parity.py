@@ -19,7 +19,9 @@ from helpers import interior, oracle_for, projected_state, rel
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [(10, 9, 10), (18, 17, 18), (34, 33, 34)]
+# periodic lengths nx-2, nz-2: 8, 16, 32 (radix 4 and 2), 20 and 30 (radix 5,
+# 3 and 2), so every FFT pass the kernels have runs against the oracle
+SIZES = [(10, 9, 10), (18, 17, 18), (34, 33, 34), (22, 13, 32)]
 
 
 def make_pair(nx, ny, nz, seed=5, **kw):
