@@ -183,7 +183,7 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     }
     const int qown = (cx + 1) + LX * (cy + 1);
     const int qext = (cx == 0 ? 0 : (cx == CX - 1 ? LX - 1 : cx + 1)) + LX * (cy + 1);
-    T ga[2][3], gb[2][3];  // [own, halo position][U,V,W]
+    T gA[2][3], gB[2][3];  // running sums [own, halo position][U,V,W]
     auto ycomb = [&](int p, T out[2][3]) {
         const int s = slot(p);
         for (int f = 0; f < 3; f++) {
@@ -194,10 +194,13 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
         }
     };
     // filtered staggered velocities of the column at plane p into LDS: the
-    // running sums hold planes p-1 and p, the y taps of plane p+1 complete
-    // the z taps; x taps from the neighbouring lanes (the halo position at
-    // the row ends).  Every lane of the wave must take part.
-    auto filt_vel = [&](int p) {
+    // running sums hold planes p-1 and p (ga) and p (gb), the y taps of plane
+    // p+1 complete the z taps; x taps from the neighbouring lanes (the halo
+    // position at the row ends).  Every lane of the wave must take part.
+    // The sums are updated in place with their roles swapped (gb becomes the
+    // next call's ga, ga its gb): callers alternate the two arrays, which
+    // keeps the march free of register copies.
+    auto filt_vel = [&](int p, T (&ga)[2][3], T (&gb)[2][3]) {
         T yc[2][3];
         ycomb(p + 1, yc);
         const int sf = slot(p);
@@ -205,8 +208,8 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
             const T own = ga[0][f] + s6 * yc[0][f];
             const T ext = ga[1][f] + s6 * yc[1][f];
             for (int e = 0; e < 2; e++) {
-                ga[e][f] = gb[e][f] + s23 * yc[e][f];
-                gb[e][f] = s6 * yc[e][f];
+                gb[e][f] = gb[e][f] + s23 * yc[e][f];
+                ga[e][f] = s6 * yc[e][f];
             }
             // the moves run on every lane before the selection: inside a
             // conditional they would read disabled lanes as 0
@@ -268,9 +271,10 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
         if (out_col && p >= kb && p <= ke) b.smag[at(d, ci, cj, p)] = sm;
     };
 
-    // z taps of the filter inputs as running sums: za for plane k (planes k-1
-    // and k in), zb for plane k+1 (plane k in)
-    T za[15], zb[15], Q[15];
+    // z taps of the filter inputs as running sums: for plane k (planes k-1
+    // and k in) and for plane k+1 (plane k in), in zA and zB with roles
+    // alternating as in filt_vel
+    T zA[15], zB[15], Q[15];
     // pre-roll: level 0 planes kb-2..kb+2, level 1 at kb-1 and kb
     load_plane(kb - 2);
     load_plane(kb - 1);
@@ -280,25 +284,25 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
         T yc[2][3];
         ycomb(kb - 2, yc);
         for (int e = 0; e < 2; e++)
-            for (int f = 0; f < 3; f++) ga[e][f] = s6 * yc[e][f];
+            for (int f = 0; f < 3; f++) gA[e][f] = s6 * yc[e][f];
         ycomb(kb - 1, yc);
         for (int e = 0; e < 2; e++)
             for (int f = 0; f < 3; f++) {
-                ga[e][f] = ga[e][f] + s23 * yc[e][f];
-                gb[e][f] = s6 * yc[e][f];
+                gA[e][f] = gA[e][f] + s23 * yc[e][f];
+                gB[e][f] = s6 * yc[e][f];
             }
     }
-    filt_vel(kb - 1);
+    filt_vel(kb - 1, gA, gB);
     level1(kb - 1, Q);
-    for (int c = 0; c < 15; c++) za[c] = s6 * Q[c];
+    for (int c = 0; c < 15; c++) zA[c] = s6 * Q[c];
     __syncthreads();
     load_plane(kb + 1);
     __syncthreads();
-    filt_vel(kb);
+    filt_vel(kb, gB, gA);
     level1(kb, Q);
     for (int c = 0; c < 15; c++) {
-        za[c] = za[c] + s23 * Q[c];
-        zb[c] = s6 * Q[c];
+        zA[c] = zA[c] + s23 * Q[c];
+        zB[c] = s6 * Q[c];
     }
     __syncthreads();
     load_plane(kb + 2);
@@ -312,15 +316,16 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     const T wyp = (cj == d.nyg - 1) ? T(0) : (cj == 2 ? s13 : s6);
     const int om = cx + CX * (cy - 1), op = cx + CX * (cy + 1);
 
-    for (int k = kb; k <= ke; k++) {
+    // one output plane k; za/zb and ga/gb as described above (roles swap)
+    auto plane = [&](int k, T (&za)[15], T (&zb)[15], T (&ga)[2][3], T (&gb)[2][3]) {
         // phase 1: l0 holds planes k..k+2
-        filt_vel(k + 1);
+        filt_vel(k + 1, ga, gb);
         level1(k + 1, Q);
         T zown[15];  // z taps of the filter inputs at plane k
         for (int c = 0; c < 15; c++) {
             zown[c] = za[c] + s6 * Q[c];
-            za[c] = zb[c] + s23 * Q[c];
-            zb[c] = s6 * Q[c];
+            zb[c] = zb[c] + s23 * Q[c];
+            za[c] = s6 * Q[c];
         }
         for (int c = 0; c < 15; c++) zq[c][tid] = zown[c];
         __syncthreads();
@@ -381,6 +386,12 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
         if (k + 3 <= ke + 2) sstore(k + 3);
         if (k + 4 <= ke + 2) gload(k + 4);
         __syncthreads();
+    };
+    // unrolled by two so the running sums alternate arrays without copies
+    for (int k = kb; k <= ke; k += 2) {
+        plane(k, zA, zB, gA, gB);
+        if (k + 1 > ke) break;
+        plane(k + 1, zB, zA, gB, gA);
     }
     // per-row partial sums over the block's columns, in a fixed order
     double* red = reinterpret_cast<double*>(&l0[0][0][0]);
