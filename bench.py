@@ -27,10 +27,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
 VALU_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 # algorithmic flops per unique cell and launch, counted from the implemented
 # formulas (an FMA counts 2): the DSM pass A is centre velocities and |S| S_ij
-# (126), the separable 3x3x3 test filter of 15 inputs and 3 staggered
-# velocities (18 x 15 = 270), hat(S) and 4|hat(S)| (109), L_ij, M_ij and the
-# two contractions (62)
-ALG_FLOPS = {"k_dsm_fused": 567}
+# (102: the off-diagonal strains 20 each), the separable 3x3x3 test filter of
+# 15 inputs and 3 staggered velocities (18 x 15 = 270), hat(S) and 4|hat(S)|
+# (85), L_ij, M_ij and the two contractions (62)
+ALG_FLOPS = {"k_dsm_fused": 519}
 
 # Algorithmic HBM values per unique cell per launch (FP64: x8 bytes; FP32
 # bulk: bulk values x4, pressure values x8), the compulsory traffic of each

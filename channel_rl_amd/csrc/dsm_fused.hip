@@ -54,6 +54,23 @@ __device__ inline float lane_shift(float v)
     return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, true));
 }
 
+// Off-diagonal rate of strain of code/subgrid.f90:1027-1070: the average
+// over the four surrounding edges of (dA/db + dB/da)/2,
+//   1/4 sum_{da,db} 1/2 ((A(da-1,db) - A(da-1,db-1)) rb[db] + (B(da,db-1) - B(da-1,db-1)) ra[da]),
+// with the sums regrouped by metric factor (four products instead of eight,
+// one scaling instead of five).  A(a, b): a in {-1,0}, b in {-1,0,1};
+// B(a, b): a in {-1,0,1}, b in {-1,0}; ra/rb the reciprocal spacings along
+// B's and A's difference direction.
+template <typename T, typename FA, typename FB>
+__device__ __forceinline__ T strain_offdiag(const FA& A, const FB& B, T ra0, T ra1, T rb0, T rb1)
+{
+    const T sA0 = (A(-1, 0) - A(-1, -1)) + (A(0, 0) - A(0, -1));
+    const T sA1 = (A(-1, 1) - A(-1, 0)) + (A(0, 1) - A(0, 0));
+    const T sB0 = (B(0, -1) - B(-1, -1)) + (B(0, 0) - B(-1, 0));
+    const T sB1 = (B(1, -1) - B(0, -1)) + (B(1, 0) - B(0, 0));
+    return T(0.125) * ((rb0 * sA0 + rb1 * sA1) + (ra0 * sB0 + ra1 * sB1));
+}
+
 // Two blocks fit a CU's LDS; the register budget is held to the same two
 // waves per SIMD.
 template <typename T, int TXO, int TYO>
@@ -84,7 +101,7 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     // within one period of that range
     auto wx = [&](int q) { return q < 2 ? q + nxp : (q > nxp + 1 ? q - nxp : q); };
     auto wz = [&](int q) { return q < 2 ? q + nzp : (q > nzp + 1 ? q - nzp : q); };
-    const T h = T(0.5), qq = T(0.25);
+    const T h = T(0.5);
     const T s6 = T(1.0 / 6.0), s23 = T(2.0 / 3.0), s13 = T(1.0 / 3.0);
     const T *rdz = b.rdz, *rdzg = b.rdzg;
     // x and y reciprocal spacings of the block's columns and rows, in LDS:
@@ -239,21 +256,14 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
             s[0] = (L0(0, p, 0, 0) - L0(0, p, -1, 0)) * mx[0][cx];
             s[1] = (L0(1, p, 0, 0) - L0(1, p, 0, -1)) * my[0][cy];
             s[2] = (L0(2, p, 0, 0) - L0(2, p - 1, 0, 0)) * rdz[pc];
-            auto e4 = [&](int da, int db) {
-                return h * ((L0(0, p, da - 1, db) - L0(0, p, da - 1, db - 1)) * my[1 + db][cy] +
-                            (L0(1, p, da, db - 1) - L0(1, p, da - 1, db - 1)) * mx[1 + da][cx]);
-            };
-            auto e5 = [&](int da, int dc) {
-                return h * ((L0(0, p + dc, da - 1, 0) - L0(0, p + dc - 1, da - 1, 0)) * rdzg[pc + dc] +
-                            (L0(2, p + dc - 1, da, 0) - L0(2, p + dc - 1, da - 1, 0)) * mx[1 + da][cx]);
-            };
-            auto e6 = [&](int db, int dc) {
-                return h * ((L0(1, p + dc, 0, db - 1) - L0(1, p + dc - 1, 0, db - 1)) * rdzg[pc + dc] +
-                            (L0(2, p + dc - 1, 0, db) - L0(2, p + dc - 1, 0, db - 1)) * my[1 + db][cy]);
-            };
-            s[3] = qq * (e4(0, 0) + e4(1, 0) + e4(0, 1) + e4(1, 1));
-            s[4] = qq * (e5(0, 0) + e5(1, 0) + e5(0, 1) + e5(1, 1));
-            s[5] = qq * (e6(0, 0) + e6(1, 0) + e6(0, 1) + e6(1, 1));
+            const T rz0 = rdzg[pc], rz1 = rdzg[pc + 1];
+            s[3] = strain_offdiag([&](int a, int c) { return L0(0, p, a, c); },
+                                  [&](int a, int c) { return L0(1, p, a, c); }, mx[1][cx], mx[2][cx], my[1][cy],
+                                  my[2][cy]);
+            s[4] = strain_offdiag([&](int a, int c) { return L0(0, p + c, a, 0); },
+                                  [&](int a, int c) { return L0(2, p + c, a, 0); }, mx[1][cx], mx[2][cx], rz0, rz1);
+            s[5] = strain_offdiag([&](int a, int c) { return L0(1, p + c, 0, a); },
+                                  [&](int a, int c) { return L0(2, p + c, 0, a); }, my[1][cy], my[2][cy], rz0, rz1);
             T acc = T(2) * s[0] * s[0];
             acc = acc + T(2) * s[1] * s[1];
             acc = acc + T(2) * s[2] * s[2];
@@ -349,21 +359,13 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
             sh[0] = (uf(0, 0, 0) - uf(-1, 0, 0)) * mx[3][cx];
             sh[1] = (vf(0, 0, 0) - vf(0, -1, 0)) * my[0][cy];
             sh[2] = (wf(0, 0, 0) - wf(0, 0, -1)) * rdz[k];
-            auto e4 = [&](int da, int db) {
-                return h * ((uf(da - 1, db, 0) - uf(da - 1, db - 1, 0)) * my[1 + db][cy] +
-                            (vf(da, db - 1, 0) - vf(da - 1, db - 1, 0)) * mx[4 + da][cx]);
-            };
-            auto e5 = [&](int da, int dc) {
-                return h * ((uf(da - 1, 0, dc) - uf(da - 1, 0, dc - 1)) * rdzg[k + dc] +
-                            (wf(da, 0, dc - 1) - wf(da - 1, 0, dc - 1)) * mx[4 + da][cx]);
-            };
-            auto e6 = [&](int db, int dc) {
-                return h * ((vf(0, db - 1, dc) - vf(0, db - 1, dc - 1)) * rdzg[k + dc] +
-                            (wf(0, db, dc - 1) - wf(0, db - 1, dc - 1)) * my[1 + db][cy]);
-            };
-            sh[3] = qq * (e4(0, 0) + e4(1, 0) + e4(0, 1) + e4(1, 1));
-            sh[4] = qq * (e5(0, 0) + e5(1, 0) + e5(0, 1) + e5(1, 1));
-            sh[5] = qq * (e6(0, 0) + e6(1, 0) + e6(0, 1) + e6(1, 1));
+            const T rz0 = rdzg[k], rz1 = rdzg[k + 1];
+            sh[3] = strain_offdiag([&](int a, int c) { return uf(a, c, 0); }, [&](int a, int c) { return vf(a, c, 0); },
+                                   mx[4][cx], mx[5][cx], my[1][cy], my[2][cy]);
+            sh[4] = strain_offdiag([&](int a, int c) { return uf(a, 0, c); }, [&](int a, int c) { return wf(a, 0, c); },
+                                   mx[4][cx], mx[5][cx], rz0, rz1);
+            sh[5] = strain_offdiag([&](int a, int c) { return vf(0, a, c); }, [&](int a, int c) { return wf(0, a, c); },
+                                   my[1][cy], my[2][cy], rz0, rz1);
             T acc = T(2) * sh[0] * sh[0];
             acc = acc + T(2) * sh[1] * sh[1];
             acc = acc + T(2) * sh[2] * sh[2];
