@@ -6,6 +6,7 @@
 // (a four-slot ring), so every value is read from HBM once per tile and
 // the ~120 stencil operands of a cell come from LDS.  The arithmetic is the
 // reference's operation order (no FMA contraction in FP64).
+#include <algorithm>
 #include "kernels.hpp"
 #include "prof.hpp"
 
@@ -160,6 +161,8 @@ __device__ __forceinline__ T rhs_w_gen(const A& F, const RhsXY<T>& g, const RhsZ
 // its reads, and every load has two planes of work to arrive.
 // In substeps 2 and 3 the thread's Uo and F1 values are fetched before the
 // stencil as well.
+static constexpr int RHS_KMAX = 32;  // longest z chunk a block marches
+
 template <typename T, int TX, int TY>
 __global__ void __launch_bounds__(TX* TY)
     k_rhs_rk(const T* __restrict__ U, const T* __restrict__ V, const T* __restrict__ W, const T* __restrict__ nt,
@@ -168,6 +171,11 @@ __global__ void __launch_bounds__(TX* TY)
 {
     constexpr int PX = TX + 2, PY = TY + 2, NP = PX * PY, NT = TX * TY, NL = (NP + NT - 1) / NT;
     __shared__ T sh[4][4][NP];  // [field][slot][position]
+    // z reciprocal spacings of the block's planes kb..ke+1, read from LDS per
+    // plane: a global load there would wait behind the prefetched planes
+    // (memory counters drain in issue order)
+    constexpr int KZ = RHS_KMAX + 2;
+    __shared__ T zt[2][KZ];  // 1/(z(k)-z(k-1)), 1/(zg(k)-zg(k-1)) at k = kb + q
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
     const TileId tile = xcd_tile();
     const int i0 = tile.x * TX + 2, j0 = tile.y * TY + 2;
@@ -217,6 +225,11 @@ __global__ void __launch_bounds__(TX* TY)
         g.ry_f = tb.rdy[jc]; g.ry_fp = tb.rdy[jc + 1]; g.ry_c = tb.rdyg[jc]; g.ry_cp = tb.rdyg[jc + 1];
         g.w0j = tb.w0[jc]; g.w1j = tb.w1[jc]; g.w0m = tb.w0[jc - 1]; g.w1m = tb.w1[jc - 1];
     }
+    for (int q = tid; q < KZ; q += TX * TY) {
+        const int kk = min(kb + q, d.nzg);
+        zt[0][q] = tb.rdz[kk];
+        zt[1][q] = tb.rdzg[kk];
+    }
     gload(kb - 1, preA);
     sstore(kb - 1, preA);
     gload(kb, preA);
@@ -243,7 +256,7 @@ __global__ void __launch_bounds__(TX* TY)
         }
         fetch();
         RhsZ<T> gz;
-        gz.rz_f = tb.rdz[k]; gz.rz_fp = tb.rdz[k + 1]; gz.rz_c = tb.rdzg[k]; gz.rz_cp = tb.rdzg[k + 1];
+        gz.rz_f = zt[0][k - kb]; gz.rz_fp = zt[0][k - kb + 1]; gz.rz_c = zt[1][k - kb]; gz.rz_cp = zt[1][k - kb + 1];
         T ru = T(0), rv = T(0), rw = T(0);
         if (du) ru = rhs_u_gen<T>(F, g, gz, nu, dPdx, i, j, k);
         if (dv) rv = rhs_v_gen<T>(F, g, gz, nu, i, j, k);
@@ -315,7 +328,7 @@ void launch_rhs_rk(const T* U, const T* V, const T* W, const T* nut, const RhsTa
     PROF_BEGIN("k_rhs_rk", s);
 #define RHS_GO(TX, TY)                                                                                              \
     do {                                                                                                            \
-        const int kc = sh.kchunk > 0 ? sh.kchunk : rhs_round_chunk<T, TX, TY>(ni, d.nyg - 2, nk);                 \
+        const int kc = std::min(RHS_KMAX, sh.kchunk > 0 ? sh.kchunk : rhs_round_chunk<T, TX, TY>(ni, d.nyg - 2, nk)); \
         dim3 grd(cdiv(ni, TX), cdiv(d.nyg - 2, TY), cdiv(nk, kc));                                                  \
         k_rhs_rk<T, TX, TY><<<grd, blk, 0, s>>>(U, V, W, nut, m, d, nu, sc, mode, Uo, Vo, Wo, F1u, F1v, F1w, Ou, Ov, \
                                                 Ow, T(c1), T(c2), kc);                                              \
