@@ -363,18 +363,23 @@ __device__ __forceinline__ void block_max3(double a, double b, double c, double*
     }
 }
 
+// Blocks walk whole x rows (j, k), so the row's y and z spacings are uniform
+// and no per-element index division is needed; the maximum does not depend
+// on the order, so the result is the same as any other traversal.
 template <typename T>
 __global__ void k_dt_partial(const T* U, const T* V, const T* W, Metric<double> m, Dims d, double* part, int nblk)
 {
-    const long n1 = d.nxg - 2, n2 = d.nyg - 2, n3 = d.nzg - 2, tot = n1 * n2 * n3;
+    const int n2 = d.nyg - 2, rows = n2 * (d.nzg - 2);
     double lu = 0.0, lv = 0.0, lw = 0.0;
-    for (long t = (long)blockIdx.x * RED_THREADS + threadIdx.x; t < tot; t += (long)nblk * RED_THREADS) {
-        const int i = (int)(t % n1) + 2;
-        const int j = (int)((t / n1) % n2) + 2;
-        const int k = (int)(t / (n1 * n2)) + 2;
-        lu = fmax(lu, (m.xg[i + 1] - m.xg[i]) / fabs((double)U[at(d, i, j, k)]));
-        lv = fmax(lv, (m.yg[j + 1] - m.yg[j]) / fabs((double)V[at(d, i, j, k)]));
-        lw = fmax(lw, (m.zg[k + 1] - m.zg[k]) / fabs((double)W[at(d, i, j, k)]));
+    for (int r = blockIdx.x; r < rows; r += nblk) {
+        const int j = r % n2 + 2, k = r / n2 + 2;
+        const double dyj = m.yg[j + 1] - m.yg[j], dzk = m.zg[k + 1] - m.zg[k];
+        const long base = at(d, 1, j, k);
+        for (int i = threadIdx.x + 2; i <= d.nxg - 1; i += RED_THREADS) {
+            lu = fmax(lu, (m.xg[i + 1] - m.xg[i]) / fabs((double)U[base + i - 1]));
+            lv = fmax(lv, dyj / fabs((double)V[base + i - 1]));
+            lw = fmax(lw, dzk / fabs((double)W[base + i - 1]));
+        }
     }
     block_max3(lu, lv, lw, part, nblk);
 }
