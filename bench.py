@@ -174,8 +174,8 @@ def measure_batched(ch, case, args, cr, cases):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs one channel on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is designed, not built: see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: one channel runs on one GPU; --gpus {args.gpus} is not supported "
+              "(sharding the channel across GPUs is not built)", file=sys.stderr)
         return 2
     import channel_rl_amd as cr
     from channel_rl_amd import cases

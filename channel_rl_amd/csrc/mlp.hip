@@ -319,7 +319,7 @@ __global__ void k_mlp_adam(float* __restrict__ p, float* __restrict__ m, float* 
 // precision of tau_par nearly uniform from the sublayer to the log region,
 // and o1 measures tau_perp relative to the same scale.  The equilibrium log law gives
 // tau_par = u_tau(|u_par|)^2, tau_perp = 0: it is the training target, the
-// judge of the network's error and the fallback.
+// yardstick of the network's error and the fallback.
 __device__ __forceinline__ void wm_features(const WallSample& ws, double nu, double tau_prev, double dpdx, float f[4])
 {
     const double up = sqrt(ws.u * ws.u + ws.w * ws.w);

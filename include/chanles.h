@@ -179,7 +179,7 @@ int chan_stream_copy(chan_ctx* ctx, long nbytes, int reps, double* ms_per_copy);
  * and the outputs o0 = log(1 + h^2 tau_par/nu^2), o1 = h^2 tau_perp/nu^2 /
  * exp(o0): the stress along and across u_par, rotated back to
  * (tau_wx, tau_wz).  The equilibrium log law (tau_par = u_tau^2, tau_perp = 0)
- * is the training target, the judge of the network's error and the fallback.
+ * is the training target, the yardstick of the network's error and the fallback.
  *
  * Parameters: 8770 FP32 values, W1[4][64] b1[64] W2[64][64] b2[64]
  * W3[64][64] b3[64] W4[64][2] b4[2] (W[k][n], input k, output n); and 12
