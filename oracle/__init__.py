@@ -67,8 +67,14 @@ def lib():
         L.orc_wall_shear.argtypes = [vp, _D]
         L.orc_compute_pressure.argtypes = [vp, _D]
         L.orc_set_wall_model.argtypes = [vp, ctypes.c_int] + [ctypes.c_double] * 3
+        L.orc_num_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
+
+
+def num_threads():
+    """OpenMP threads the oracle's parallel loops use (1 for a serial build)."""
+    return int(lib().orc_num_threads())
 
 
 def _p(a):

@@ -4,5 +4,5 @@
 # floating-point operation order.
 set -e
 cd "$(dirname "$0")"
-gcc -O2 -fPIC -shared -std=c11 -ffp-contract=off -fno-fast-math -Wall -Wextra -Wno-unused-parameter \
+gcc -O2 -fopenmp -fPIC -shared -std=c11 -ffp-contract=off -fno-fast-math -Wall -Wextra -Wno-unused-parameter \
     -o liborc.so oracle.c -lm

@@ -18,8 +18,32 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 typedef double complex cplx;
+
+/* Loops whose iterations write disjoint outputs, each output by the same
+ * operations in the same order as the serial loop, run over OpenMP threads
+ * when built with -fopenmp: the results are bit-identical for any thread
+ * count.  Sums and maxima over the domain stay serial. */
+#ifdef _OPENMP
+#define PAR_FOR _Pragma("omp parallel for schedule(static)")
+#define PAR_FOR2 _Pragma("omp parallel for collapse(2) schedule(static)")
+#else
+#define PAR_FOR
+#define PAR_FOR2
+#endif
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 struct orc {
     int nx, ny, nz, nxg, nyg, nzg, nxp, nzp;
@@ -331,6 +355,7 @@ void orc_compute_rhs_u(const orc* o, const double* U, const double* V, const dou
     const double *w0 = o->w0, *w1 = o->w1, *nt = o->nu_t;
     const double nu = o->nu;
 #define NT(i, j, k) AC(nt, i, j, k)
+    PAR_FOR
     for (int k = 2; k <= o->nzg - 1; k++) {
         double dz_1 = zg[k] - zg[k - 1], dz_2 = zg[k + 1] - zg[k], dz_3 = z[k] - z[k - 1];
         for (int j = 2; j <= o->nyg - 1; j++) {
@@ -382,6 +407,7 @@ void orc_compute_rhs_v(const orc* o, const double* U, const double* V, const dou
     const double *x = o->x, *y = o->y, *z = o->z, *xg = o->xg, *yg = o->yg, *zg = o->zg;
     const double *w0 = o->w0, *w1 = o->w1, *nt = o->nu_t;
     const double nu = o->nu;
+    PAR_FOR
     for (int k = 2; k <= o->nzg - 1; k++) {
         double dz_1 = zg[k] - zg[k - 1], dz_2 = zg[k + 1] - zg[k], dz_3 = z[k] - z[k - 1];
         for (int j = 2; j <= o->ny - 1; j++) {
@@ -434,6 +460,7 @@ void orc_compute_rhs_w(const orc* o, const double* U, const double* V, const dou
     const double *x = o->x, *y = o->y, *z = o->z, *xg = o->xg, *yg = o->yg, *zg = o->zg;
     const double *w0 = o->w0, *w1 = o->w1, *nt = o->nu_t;
     const double nu = o->nu;
+    PAR_FOR
     for (int k = 2; k <= o->nz - 1; k++) {
         double dz_1 = z[k] - z[k - 1], dz_2 = z[k + 1] - z[k], dz_3 = zg[k + 1] - zg[k];
         for (int j = 2; j <= o->nyg - 1; j++) {
@@ -495,7 +522,11 @@ void orc_filter_xzy(int n1, int n2, int n3, const double* in, double* out)
     int m1 = n1 - 2, m2 = n2;
 #define IN(i, j, k) in[IX(i, j, k, n1, n2)]
 #define OUT(i, j, k) out[IX(i, j, k, m1, m2)]
-    for (size_t q = 0; q < (size_t)m1 * m2 * (n3 - 2); q++) out[q] = 0.0;
+    const long nout = (long)m1 * m2 * (n3 - 2);
+    PAR_FOR
+    for (long q = 0; q < nout; q++) out[q] = 0.0;
+    /* each ii accumulates into its own output column OUT(ii-1, :, :) */
+    PAR_FOR
     for (int ii = 2; ii <= n1 - 1; ii++)
         for (int i = -1; i <= 1; i++)
             for (int kk = 2; kk <= n3 - 1; kk++)
@@ -520,10 +551,12 @@ void orc_filter_xzy(int n1, int n2, int n3, const double* in, double* out)
 static void filter_box_rows(const orc* o, const double* src, double* dst, double* tmp_in, double* tmp_out)
 {
     int n1 = o->nxg, n2 = o->nyg - 2, n3 = o->nzg;
+    PAR_FOR
     for (int k = 1; k <= n3; k++)
         for (int j = 1; j <= n2; j++)
             for (int i = 1; i <= n1; i++) tmp_in[IX(i, j, k, n1, n2)] = AC(src, i, j + 1, k);
     orc_filter_xzy(n1, n2, n3, tmp_in, tmp_out);
+    PAR_FOR
     for (int k = 2; k <= n3 - 1; k++)
         for (int j = 1; j <= n2; j++)
             for (int i = 2; i <= n1 - 1; i++) AC(dst, i, j + 1, k) = tmp_out[IX(i - 1, j, k - 1, n1 - 2, n2)];
@@ -542,6 +575,7 @@ static void compute_Sij_full(const orc* o, const double* U, const double* V, con
     size_t nc = nC(o);
     double *S1 = Sij, *S2 = Sij + nc, *S3 = Sij + 2 * nc, *S4 = Sij + 3 * nc, *S5 = Sij + 4 * nc,
            *S6 = Sij + 5 * nc;
+    PAR_FOR
     for (int k = 2; k <= o->nzg - 1; k++)
         for (int j = 2; j <= o->nyg - 1; j++)
             for (int i = 2; i <= o->nxg - 1; i++) {
@@ -549,6 +583,7 @@ static void compute_Sij_full(const orc* o, const double* U, const double* V, con
                 AC(S2, i, j, k) = (AV(V, i, j, k) - AV(V, i, j - 1, k)) / (y[j] - y[j - 1]);
                 AC(S3, i, j, k) = (AW(W, i, j, k) - AW(W, i, j, k - 1)) / (z[k] - z[k - 1]);
             }
+    PAR_FOR
     for (int k = 2; k <= o->nzg; k++)
         for (int j = 2; j <= o->nyg; j++)
             for (int i = 2; i <= o->nxg; i++) {
@@ -559,14 +594,27 @@ static void compute_Sij_full(const orc* o, const double* U, const double* V, con
                 AC(S6, i, j, k) = 0.5 * ((AV(V, i, j - 1, k) - AV(V, i, j - 1, k - 1)) / (zg[k] - zg[k - 1]) +
                                          (AW(W, i, j, k - 1) - AW(W, i, j - 1, k - 1)) / (yg[j] - yg[j - 1]));
             }
-    /* edge -> centre, in place in increasing index order (reads only later points) */
+    /* edge -> centre, in place in increasing index order (reads only later
+     * points, so every read sees an edge value).  One loop per component,
+     * each split across threads along the axis its stencil does not touch:
+     * S4 (x,y stencil) by k, S5 (x,z) by j, S6 (y,z) by i; within a thread
+     * the order stays increasing. */
+    PAR_FOR
     for (int k = 2; k <= o->nzg - 1; k++)
         for (int j = 2; j <= o->nyg - 1; j++)
-            for (int i = 2; i <= o->nxg - 1; i++) {
+            for (int i = 2; i <= o->nxg - 1; i++)
                 AC(S4, i, j, k) = 0.25 * (AC(S4, i, j, k) + AC(S4, i + 1, j, k) + AC(S4, i, j + 1, k) + AC(S4, i + 1, j + 1, k));
+    PAR_FOR
+    for (int j = 2; j <= o->nyg - 1; j++)
+        for (int k = 2; k <= o->nzg - 1; k++)
+            for (int i = 2; i <= o->nxg - 1; i++)
                 AC(S5, i, j, k) = 0.25 * (AC(S5, i, j, k) + AC(S5, i + 1, j, k) + AC(S5, i, j, k + 1) + AC(S5, i + 1, j, k + 1));
+    PAR_FOR
+    for (int i = 2; i <= o->nxg - 1; i++)
+        for (int k = 2; k <= o->nzg - 1; k++)
+            for (int j = 2; j <= o->nyg - 1; j++)
                 AC(S6, i, j, k) = 0.25 * (AC(S6, i, j, k) + AC(S6, i, j + 1, k) + AC(S6, i, j, k + 1) + AC(S6, i, j + 1, k + 1));
-            }
+    PAR_FOR
     for (int k = 2; k <= o->nzg - 1; k++)
         for (int j = 2; j <= o->nyg - 1; j++)
             for (int i = 2; i <= o->nxg - 1; i++) {
@@ -591,6 +639,7 @@ void orc_compute_Sij(const orc* o, const double* U, const double* V, const doubl
 static void filter_full(const double* F, int n1, int n2, int n3, double* Ff, double* tmp)
 {
     orc_filter_xzy(n1, n2, n3, F, tmp);
+    PAR_FOR
     for (int k = 2; k <= n3 - 1; k++)
         for (int j = 1; j <= n2; j++)
             for (int i = 2; i <= n1 - 1; i++) Ff[IX(i, j, k, n1, n2)] = tmp[IX(i - 1, j, k - 1, n1 - 2, n2)];
@@ -611,12 +660,15 @@ static void dsm_core(orc* o, const double* U, const double* V, const double* W, 
 
     /* Part 1: centre velocities (faces -> centres), periodic fill */
     memset(uc, 0, nc * sizeof(double)); memset(vc, 0, nc * sizeof(double)); memset(wc, 0, nc * sizeof(double));
+    PAR_FOR
     for (int k = 1; k <= nzg; k++)
         for (int j = 1; j <= nyg; j++)
             for (int i = 2; i <= nxg - 1; i++) AC(uc, i, j, k) = 0.5 * (AU(U, i - 1, j, k) + AU(U, i, j, k));
+    PAR_FOR
     for (int k = 1; k <= nzg; k++)
         for (int j = 2; j <= nyg - 1; j++)
             for (int i = 1; i <= nxg; i++) AC(vc, i, j, k) = 0.5 * (AV(V, i, j - 1, k) + AV(V, i, j, k));
+    PAR_FOR
     for (int k = 2; k <= nzg - 1; k++)
         for (int j = 1; j <= nyg; j++)
             for (int i = 1; i <= nxg; i++) AC(wc, i, j, k) = 0.5 * (AW(W, i, j, k - 1) + AW(W, i, j, k));
@@ -629,6 +681,7 @@ static void dsm_core(orc* o, const double* U, const double* V, const double* W, 
     for (int q = 0; q < 6; q++) {
         double* Lq = L + q * nc;
         memset(tmp, 0, nc * sizeof(double));
+        PAR_FOR
         for (int k = 1; k <= nzg; k++)
             for (int j = 2; j <= nyg - 1; j++)
                 for (int i = 1; i <= nxg; i++)
@@ -637,6 +690,7 @@ static void dsm_core(orc* o, const double* U, const double* V, const double* W, 
     }
     /* hat(u_i) into Sij components 0..2 */
     for (int c = 0; c < 3; c++) filter_box_rows(o, comp[c], Sij + c * nc, tin, tout);
+    PAR_FOR2
     for (int q = 0; q < 6; q++)
         for (int k = 2; k <= nzg - 1; k++)
             for (int j = 2; j <= nyg - 1; j++)
@@ -654,6 +708,7 @@ static void dsm_core(orc* o, const double* U, const double* V, const double* W, 
     periodic_x(Wf, nxg, nyg, o->nz, 0); periodic_z(Wf, nxg, nyg, o->nz, 1);
     compute_Sij_full(o, Uf, Vf, Wf, o->nx, o->ny, o->nz, Sij, Sh);
     const double fs2 = 2.0 * 2.0; /* fil_size**2, fil_size = 2 (code/subgrid.f90:156) */
+    PAR_FOR2
     for (int q = 0; q < 6; q++)
         for (int k = 2; k <= nzg - 1; k++)
             for (int j = 2; j <= nyg - 1; j++)
@@ -664,6 +719,7 @@ static void dsm_core(orc* o, const double* U, const double* V, const double* W, 
     compute_Sij_full(o, U, V, W, o->nx, o->ny, o->nz, Sij, S);
     for (int q = 0; q < 6; q++) {
         memset(tmp, 0, nc * sizeof(double));
+        PAR_FOR
         for (int k = 2; k <= nzg - 1; k++)
             for (int j = 2; j <= nyg - 1; j++)
                 for (int i = 2; i <= nxg - 1; i++) AC(tmp, i, j, k) = AC(S, i, j, k) * AC(Sij + q * nc, i, j, k);
@@ -671,6 +727,7 @@ static void dsm_core(orc* o, const double* U, const double* V, const double* W, 
         periodic_z(tmp, nxg, nyg, nzg, 0);
         filter_box_rows(o, tmp, Sij + q * nc, tin, tout);
     }
+    PAR_FOR2
     for (int q = 0; q < 6; q++)
         for (int k = 2; k <= nzg - 1; k++)
             for (int j = 2; j <= nyg - 1; j++)
@@ -681,6 +738,7 @@ static void dsm_core(orc* o, const double* U, const double* V, const double* W, 
     if (nu_t) memset(nu_t, 0, nc * sizeof(double));
 #define LL(q) AC(L + (q) * nc, i, j, k)
 #define MMq(q) AC(M + (q) * nc, i, j, k)
+    PAR_FOR
     for (int j = 2; j <= nyg - 1; j++) {
         double slm = 0.0, smm = 0.0;
         for (int k = 2; k <= nzg - 1; k++)
@@ -709,6 +767,7 @@ static void dsm_core(orc* o, const double* U, const double* V, const double* W, 
         periodic_x(nu_t, nxg, nyg, nzg, 0);
         periodic_z(nu_t, nxg, nyg, nzg, 0);
         /* clipping; 0/0 planes (M:M = 0) give NaN, clipped to 0 (fmax) */
+        PAR_FOR
         for (size_t q = 0; q < nc; q++) nu_t[q] = fmax(nu_t[q], 0.0);
         wall_y(nu_t, nxg, nyg, nzg, 0, o->Dirichlet_nu_t == 1);
     }
@@ -826,10 +885,12 @@ static void amd_core(orc* o, const double* U, const double* V, const double* W, 
     const int nxg = o->nxg, nyg = o->nyg, nzg = o->nzg;
     compute_Sij_full(o, U, V, W, o->nx, o->ny, o->nz, o->Sijb, o->Sbuf);
     memset(nu_t, 0, nC(o) * sizeof(double));
+    PAR_FOR
     for (int k = 2; k <= nzg - 1; k++)
         for (int j = 2; j <= nyg - 1; j++)
             for (int i = 2; i <= nxg - 1; i++) AC(nu_t, i, j, k) = amd_point(o, U, V, W, o->Sijb, i, j, k);
     /* clipping; 0/0 (a uniform neighbourhood) gives NaN, clipped to 0 (fmax) */
+    PAR_FOR
     for (int k = 2; k <= nzg - 1; k++)
         for (int j = 2; j <= nyg - 1; j++)
             for (int i = 2; i <= nxg - 1; i++) AC(nu_t, i, j, k) = fmax(AC(nu_t, i, j, k), 0.0);
@@ -1026,12 +1087,15 @@ static void poisson_core(orc* o, double* rhs_p, const double* bc1, const double*
     int nr = nyg - 2;
     size_t plane = (size_t)nxp * nzp;
     cplx* hat = (cplx*)malloc(sizeof(cplx) * plane * nr);   /* [j][kz][kx] */
-    cplx* line = (cplx*)malloc(sizeof(cplx) * (nxp > nzp ? nxp : nzp));
-    cplx* lout = (cplx*)malloc(sizeof(cplx) * (nxp > nzp ? nxp : nzp));
-    cplx* work = (cplx*)malloc(sizeof(cplx) * (nxp > nzp ? nxp : nzp));
     cplx *twxf = twiddles(nxp, -1), *twzf = twiddles(nzp, -1), *twxb = twiddles(nxp, +1), *twzb = twiddles(nzp, +1);
+    const int lmax = nxp > nzp ? nxp : nzp;
+    /* planes, modes and lines are independent: each iteration has its own
+     * line buffers */
+    PAR_FOR
     for (int j = 2; j <= nyg - 1; j++) {
         cplx* h = hat + plane * (j - 2);
+        cplx* line = (cplx*)malloc(sizeof(cplx) * 3 * (size_t)lmax);
+        cplx *lout = line + lmax, *work = lout + lmax;
         for (int k = 0; k < nzp; k++) {
             for (int i = 0; i < nxp; i++) line[i] = AC(rhs_p, i + 2, j, k + 2);
             fft_rec(nxp, line, 1, lout, twxf, 1, nxp, work);
@@ -1042,23 +1106,27 @@ static void poisson_core(orc* o, double* rhs_p, const double* bc1, const double*
             fft_rec(nzp, line, 1, lout, twzf, 1, nzp, work);
             for (int k = 0; k < nzp; k++) h[(size_t)k * nxp + i] = lout[k];
         }
+        free(line);
     }
-    cplx *D = (cplx*)malloc(sizeof(cplx) * nr), *DL = (cplx*)malloc(sizeof(cplx) * nr),
-         *DU = (cplx*)malloc(sizeof(cplx) * nr), *B = (cplx*)malloc(sizeof(cplx) * nr);
     cplx *b1h = 0, *b2h = 0;
     double cb1 = 0.0, cb2 = 0.0;
     if (bc1 && bc2) {
         b1h = (cplx*)malloc(sizeof(cplx) * plane);
         b2h = (cplx*)malloc(sizeof(cplx) * plane);
-        plane_fft(o, bc1, b1h, line, lout, work, twxf, twzf);
-        plane_fft(o, bc2, b2h, line, lout, work, twxf, twzf);
+        cplx* line = (cplx*)malloc(sizeof(cplx) * 3 * (size_t)lmax);
+        plane_fft(o, bc1, b1h, line, line + lmax, line + 2 * lmax, twxf, twzf);
+        plane_fft(o, bc2, b2h, line, line + lmax, line + 2 * lmax, twxf, twzf);
+        free(line);
         /* coef_bc_1 = c at j = 2, coef_bc_2 = a at j = nyg-1
          * (code/initialization.f90:370-385) */
         const double *y = o->y, *yg = o->yg;
         cb1 = 1.0 / (y[2] - y[1]) / (yg[2] - yg[1]);
         cb2 = 1.0 / (y[nyg - 1] - y[nyg - 2]) / (yg[nyg] - yg[nyg - 1]);
     }
-    for (int kz = 0; kz < nzp; kz++)
+    PAR_FOR
+    for (int kz = 0; kz < nzp; kz++) {
+        cplx* D = (cplx*)malloc(sizeof(cplx) * 4 * (size_t)nr);
+        cplx *DL = D + nr, *DU = DL + nr, *B = DU + nr;
         for (int kx = 0; kx < nxp; kx++) {
             for (int j = 2; j <= nyg - 1; j++) D[j - 2] = (o->dyy_d[j] + o->kxx[kx]) + o->kzz[kz];
             for (int j = 2; j <= nyg - 2; j++) { DL[j - 2] = o->dyy_l[j]; DU[j - 2] = o->dyy_u[j]; }
@@ -1071,9 +1139,14 @@ static void poisson_core(orc* o, double* rhs_p, const double* bc1, const double*
             zgtsv_c(nr, DL, D, DU, B);
             for (int j = 0; j < nr; j++) hat[plane * j + (size_t)kz * nxp + kx] = B[j];
         }
+        free(D);
+    }
     const double scale = (double)((long)nxp * nzp);
+    PAR_FOR
     for (int j = 2; j <= nyg - 1; j++) {
         cplx* h = hat + plane * (j - 2);
+        cplx* line = (cplx*)malloc(sizeof(cplx) * 3 * (size_t)lmax);
+        cplx *lout = line + lmax, *work = lout + lmax;
         for (int i = 0; i < nxp; i++) {
             for (int k = 0; k < nzp; k++) line[k] = h[(size_t)k * nxp + i];
             fft_rec(nzp, line, 1, lout, twzb, 1, nzp, work);
@@ -1084,6 +1157,7 @@ static void poisson_core(orc* o, double* rhs_p, const double* bc1, const double*
             fft_rec(nxp, line, 1, lout, twxb, 1, nxp, work);
             for (int i = 0; i < nxp; i++) AC(rhs_p, i + 2, j, k + 2) = creal(lout[i] / scale);
         }
+        free(line);
     }
     /* periodic images of the pseudo-pressure */
     for (int k = 2; k <= nzg; k++)
@@ -1091,8 +1165,7 @@ static void poisson_core(orc* o, double* rhs_p, const double* bc1, const double*
     for (int j = 2; j <= nyg - 1; j++)
         for (int i = 2; i <= nxg - 1; i++) AC(rhs_p, i, j, nzg - 1) = AC(rhs_p, i, j, 2);
     free(twxf); free(twzf); free(twxb); free(twzb);
-    free(hat); free(line); free(lout); free(work); free(D); free(DL); free(DU); free(B);
-    free(b1h); free(b2h);
+    free(hat); free(b1h); free(b2h);
 }
 
 /* code/pressure.f90:236-340 compute_pressure_BC: bc = d/dy((nu + nu_t) dV/dy)
@@ -1166,6 +1239,7 @@ void orc_compute_pressure(orc* o, double* P)
     for (int j = 1; j <= nyg; j++)
         for (int i = 1; i <= nxg; i++) { RW(i, j, 1) = RW(i, j, nz - 1); RW(i, j, nz) = RW(i, j, 2); }
     memset(P, 0, nc * sizeof(double));
+    PAR_FOR
     for (int k = 2; k <= nzg - 1; k++)
         for (int j = 2; j <= nyg - 1; j++)
             for (int i = 2; i <= nxg - 1; i++)
@@ -1192,6 +1266,7 @@ void orc_compute_projection_step(orc* o)
     int nxg = o->nxg, nyg = o->nyg, nzg = o->nzg;
     const double *x = o->x, *y = o->y, *z = o->z, *xg = o->xg, *yg = o->yg, *zg = o->zg;
     double *U = o->U, *V = o->V, *W = o->W, *P = o->P;
+    PAR_FOR
     for (int k = 2; k <= nzg - 1; k++)
         for (int j = 2; j <= nyg - 1; j++)
             for (int i = 2; i <= nxg - 1; i++)
@@ -1199,14 +1274,17 @@ void orc_compute_projection_step(orc* o)
                                  (AV(V, i, j, k) - AV(V, i, j - 1, k)) / (y[j] - y[j - 1]) +
                                  (AW(W, i, j, k) - AW(W, i, j, k - 1)) / (z[k] - z[k - 1]);
     orc_solve_poisson_equation(o, P);
+    PAR_FOR
     for (int k = 2; k <= nzg - 1; k++)
         for (int j = 2; j <= nyg - 1; j++)
             for (int i = 2; i <= o->nx - 1; i++)
                 AU(U, i, j, k) = AU(U, i, j, k) - (AC(P, i + 1, j, k) - AC(P, i, j, k)) / (xg[i + 1] - xg[i]);
+    PAR_FOR
     for (int k = 2; k <= nzg - 1; k++)
         for (int j = 2; j <= o->ny - 1; j++)
             for (int i = 2; i <= nxg - 1; i++)
                 AV(V, i, j, k) = AV(V, i, j, k) - (AC(P, i, j + 1, k) - AC(P, i, j, k)) / (yg[j + 1] - yg[j]);
+    PAR_FOR
     for (int k = 2; k <= o->nz - 1; k++)
         for (int j = 2; j <= nyg - 1; j++)
             for (int i = 2; i <= nxg - 1; i++)
@@ -1309,6 +1387,7 @@ void orc_compute_time_step_RK3(orc* o)
         orc_compute_rhs_v(o, o->U, o->V, o->W, o->Fv[s]);
         orc_compute_rhs_w(o, o->U, o->V, o->W, o->Fw[s]);
         const double* c = o->rk_coef[s];
+        PAR_FOR
         for (int k = 2; k <= o->nzg - 1; k++)
             for (int j = 2; j <= o->nyg - 1; j++)
                 for (int i = 2; i <= o->nx - 1; i++) {
@@ -1317,6 +1396,7 @@ void orc_compute_time_step_RK3(orc* o)
                     else if (s == 1) o->U[q] = o->Uo[q] + dt * (c[0] * o->Fu[0][q] + c[1] * o->Fu[1][q]);
                     else o->U[q] = o->Uo[q] + dt * (c[0] * o->Fu[0][q] + c[1] * o->Fu[1][q] + c[2] * o->Fu[2][q]);
                 }
+        PAR_FOR
         for (int k = 2; k <= o->nzg - 1; k++)
             for (int j = 2; j <= o->ny - 1; j++)
                 for (int i = 2; i <= o->nxg - 1; i++) {
@@ -1325,6 +1405,7 @@ void orc_compute_time_step_RK3(orc* o)
                     else if (s == 1) o->V[q] = o->Vo[q] + dt * (c[0] * o->Fv[0][q] + c[1] * o->Fv[1][q]);
                     else o->V[q] = o->Vo[q] + dt * (c[0] * o->Fv[0][q] + c[1] * o->Fv[1][q] + c[2] * o->Fv[2][q]);
                 }
+        PAR_FOR
         for (int k = 2; k <= o->nz - 1; k++)
             for (int j = 2; j <= o->nyg - 1; j++)
                 for (int i = 2; i <= o->nxg - 1; i++) {
@@ -1340,6 +1421,7 @@ void orc_compute_time_step_RK3(orc* o)
     }
     if (o->x_mass_cte == 1) {
         o->dPdx = orc_compute_dPx_for_constant_mass_flow(o, o->U);
+        PAR_FOR
         for (int k = 2; k <= o->nzg - 1; k++)
             for (int j = 2; j <= o->nyg - 1; j++)
                 for (int i = 2; i <= o->nx - 1; i++) AU(o->U, i, j, k) = AU(o->U, i, j, k) + o->dPdx;

@@ -76,6 +76,10 @@ void   orc_equilibrium_alpha(orc* o, const double* U, double h_wm, double kappa,
 /* xz-plane statistics (code/statistics.f90:52-64) */
 void   orc_plane_stats(const orc* o, double* out /* 7 x ny */);
 
+/* OpenMP threads of the loops with disjoint outputs (1 without -fopenmp);
+ * results are bit-identical for any count */
+int    orc_num_threads(void);
+
 #ifdef __cplusplus
 }
 #endif

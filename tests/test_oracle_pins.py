@@ -707,3 +707,53 @@ def test_rk3_amplification_of_advected_diffused_mode():
     mid = slice(6, o.nyg - 6)  # rows the walls cannot reach in one step
     got = W1[1:o.nxg - 1, mid, 1:o.nz - 1]
     assert np.max(np.abs(got - ref[:, None, None])) <= 1e-12
+
+
+# ------------------------------------------------------------ threading
+
+def _threaded_outputs():
+    """DSM and AMD eddy viscosities, right-hand sides, two RK3 steps with the
+    wall model and mass-flow correction, and the pressure, of a small
+    stretched channel."""
+    out = {}
+    for les in (2, 3):
+        case, field = cases.small_case(26, 17, 22, seed=5, LES_model=les)
+        o = oracle.Oracle(case.x, case.y, case.z, **case.kwargs())
+        o.set_state(*field)
+        o.apply_boundary_conditions()
+        U, V, W = o.state()
+        out[f"nut{les}"] = o.compute_eddy_viscosity(U, V, W)
+        for q, r in enumerate(o.compute_rhs(U, V, W)):
+            out[f"rhs{les}{q}"] = r
+        o.set_scalars(dt=0.0, dPdx=case.dPdx, Qflow_x_0=0.5)
+        o.set_wall_model(1, h_wm=0.1)
+        for _ in range(2):
+            o.compute_dt()
+            o.compute_time_step_RK3()
+        for q, a in enumerate(o.state()):
+            out[f"state{les}{q}"] = a
+        out[f"p{les}"] = o.compute_pressure()
+    return out
+
+
+def test_threaded_oracle_matches_one_thread(tmp_path):
+    """The OpenMP loops write disjoint outputs by the serial operations, so
+    the oracle on all threads equals the oracle on one thread bit for bit."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    path = str(tmp_path / "one_thread.npz")
+    env = dict(os.environ, OMP_NUM_THREADS="1", PYTHONPATH=root + os.pathsep + os.path.join(root, "tests"))
+    subprocess.check_call([sys.executable, os.path.abspath(__file__), path], env=env, timeout=300)
+    one = np.load(path)
+    many = _threaded_outputs()
+    assert sorted(one.files) == sorted(many)
+    for k in one.files:
+        assert np.array_equal(one[k], many[k], equal_nan=True), k
+
+
+if __name__ == "__main__":
+    import sys
+    assert oracle.num_threads() == 1
+    np.savez(sys.argv[1], **_threaded_outputs())
