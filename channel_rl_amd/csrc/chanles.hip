@@ -576,9 +576,11 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
     if ((rc = make_twiddles(c, c->pb.fx)) || (rc = make_twiddles(c, c->pb.fz))) return rc;
     c->pb.bx = std::max(1, std::min(16, 2048 / d.nxp));
     c->pb.bz = std::max(1, std::min(16, 2048 / d.nzp));
-    if ((rc = dalloc(c, (void**)&c->pb.tri, (size_t)2 * (d.ny - 1) * d.nzp * d.nkx * sizeof(double)))) return rc;
-    launch_tridiag_setup(d, c->pb, c->stream);
-    if ((rc = check_launch(c))) return rc;
+    if (tridiag_needs_table(d)) {
+        if ((rc = dalloc(c, (void**)&c->pb.tri, (size_t)(d.ny - 1) * d.nzp * d.nkx * sizeof(double)))) return rc;
+        launch_tridiag_setup(d, c->pb, c->stream);
+        if ((rc = check_launch(c))) return rc;
+    }
     Scalars s0{};
     s0.dt = 0.0; s0.t = 0.0; s0.dPdx = cfg->dPdx; s0.Qflow_x_0 = 0.0;
     s0.cfl = cfg->CFL; s0.nu = cfg->nu; s0.dt_max = 1.0e-3;

@@ -116,7 +116,7 @@ struct FftPlan {
 };
 
 struct PoissonBufs {
-    double2* ph;        // spectral pressure [ny-1][nzp][nkx]
+    double2* ph;        // spectral pressure [ny-1][nkx][nzp], kz fastest
     double* p;          // pseudo-pressure (box layout, double)
     const double* kxx;  // nxp
     const double* kzz;  // nzp
@@ -125,7 +125,7 @@ struct PoissonBufs {
     const double* du;   // Dyy(j,j+1)
     FftPlan fx, fz;
     int bx, bz;         // transforms per block in the x and z passes
-    double* tri;        // pivots of the elimination, [row][mode]: c'_j, then 1/den_j at +rows*modes
+    double* tri;        // pivots 1/den_j of the y elimination, [row][mode] (tall systems only)
 };
 
 template <typename T>
@@ -133,6 +133,9 @@ void launch_poisson_forward(const T* U, const T* V, const T* W, const Metric<dou
                             PoissonBufs& pb, hipStream_t s);
 void launch_poisson_solve(const Dims& d, PoissonBufs& pb, hipStream_t s);
 // the elimination's pivots of every mode (once per channel)
+// the y solve keeps systems of up to 96 rows in registers with pivots computed
+// in the kernel; taller systems need the pivot table PoissonBufs::tri
+bool tridiag_needs_table(const Dims& d);
 void launch_tridiag_setup(const Dims& d, PoissonBufs& pb, hipStream_t s);
 void launch_poisson_inverse(const Dims& d, PoissonBufs& pb, hipStream_t s);
 // forward x pass of a given centre field (for the standalone Poisson solve)

@@ -252,16 +252,20 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftz(Dims d, PoissonBufs pb)
 // code/projection.f90:90-113: (Dyy + kxx + kzz) p = r per (kx, kz) mode by
 // the Thomas algorithm, real and imaginary parts on neighbouring lanes.  The
 // matrix (with the 00 mode's first row scaled by 3/2, code/projection.f90:107)
-// is real and the same for every solve of the run, so the forward
-// elimination's pivots 1/den_j and the modified upper diagonal c'_j are
-// computed once per mode (k_tridiag_setup, the same arithmetic the
-// elimination would do) and kept in HBM as two [row][mode] planes, each sweep
-// reading only the one it needs.  A solve has no divisions and no LDS; the
-// last TRI_KEEP rows of the forward sweep stay in registers for the back
-// substitution instead of going through memory.
+// is real, so the elimination's pivots 1/den_j depend on the mode only, and
+// the modified upper diagonal is c'_j = Dyy_j,j+1 / den_j.
+//
+// Systems of at most TRI_KEEP rows (k_tridiag<false>) live in registers: the
+// right-hand sides are loaded first, all in flight at once; the pivot chain
+// (one division per row, no memory) runs while they arrive and keeps every
+// 1/den_j; the two sweeps then read registers only and each row is written
+// once.  A wave can hold few loads in flight, so a pivot table in memory
+// would cost more than the divisions.  Taller systems (k_tridiag<true>)
+// stream their first rows through memory with pivots from a [row][mode]
+// table that k_tridiag_setup computes once per run by the same arithmetic.
 static constexpr int TRI_THREADS = 64;
 static constexpr int TRI_UNROLL = 12;
-static constexpr int TRI_KEEP = 48;
+static constexpr int TRI_KEEP = 96;
 
 __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_setup(Dims d, PoissonBufs pb)
 {
@@ -270,109 +274,126 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_setup(Dims d, PoissonBu
     if (mode >= nmodes) return;
     const int kx = mode / d.nzp, kz = mode % d.nzp;
     const int n = d.ny - 1;
-    double* tc = pb.tri;
-    double* ti = pb.tri + (long)n * nmodes;
     const double lx = pb.kxx[kx], lz = pb.kzz[kz];
     double den = (pb.dd[0] + lx) + lz;
     if (kx == 0 && kz == 0) den = 3.0 / 2.0 * den;
     double inv = 1.0 / den;
     double c = pb.du[0] * inv;
-    tc[mode] = c;
-    ti[mode] = inv;
+    pb.tri[mode] = inv;
     for (int row = 1; row < n; row++) {
-        const double l = pb.dl[row - 1];
-        den = ((pb.dd[row] + lx) + lz) - l * c;
+        den = ((pb.dd[row] + lx) + lz) - pb.dl[row - 1] * c;
         inv = 1.0 / den;
         c = row < n - 1 ? pb.du[row] * inv : 0.0;
-        tc[(long)row * nmodes + mode] = c;
-        ti[(long)row * nmodes + mode] = inv;
+        pb.tri[(long)row * nmodes + mode] = inv;
     }
 }
 
-// one lane per (mode, real/imaginary part)
+// one lane per (mode, real/imaginary part); modes are kx-major, kz fastest
+template <bool STREAMED>
 __global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
 {
     const long g = (long)blockIdx.x * TRI_THREADS + threadIdx.x;
     const int nmodes = d.nkx * d.nzp;
+    const int n = d.ny - 1;
+    const int nk = n < TRI_KEEP ? n : TRI_KEEP;  // rows n0..n-1 stay in registers
+    const int n0 = STREAMED ? n - nk : 0;
+    // the register rows' matrix entries, read by every lane at the same
+    // address; slots past the last row hold an identity row so that the
+    // unrolled chains run without branches (their values there are not used)
+    __shared__ double sd[TRI_KEEP], sl[TRI_KEEP], su[TRI_KEEP];
+    for (int u = threadIdx.x; u < TRI_KEEP; u += TRI_THREADS) {
+        const int rw = n0 + u;
+        sd[u] = u < nk ? pb.dd[rw] : 1.0;
+        sl[u] = u < nk && rw > 0 ? pb.dl[rw - 1] : 0.0;
+        su[u] = u < nk && rw < n - 1 ? pb.du[rw] : 0.0;
+    }
+    __syncthreads();
     if (g >= 2L * nmodes) return;
     const int mode = (int)(g >> 1);
-    const int n = d.ny - 1;
-    const int nk = n < TRI_KEEP ? n : TRI_KEEP;  // rows n-nk..n-1 stay in registers
-    const int n0 = n - nk;
     const long plane = 2L * nmodes;
     double* ph = reinterpret_cast<double*>(pb.ph) + g;
-    const double* tc = pb.tri + mode;
-    const double* ti = pb.tri + (long)n * nmodes + mode;
-    // forward: row 0 has no lower coupling (l = 0 leaves r * inv exact)
+    const double* ti = STREAMED ? pb.tri + mode : nullptr;  // the table exists for tall systems only
+    double keep[TRI_KEEP], iv[TRI_KEEP];
     double dp = 0.0;
     int row = 0;
-    for (; row + TRI_UNROLL <= n0; row += TRI_UNROLL) {
-        double r[TRI_UNROLL], iv[TRI_UNROLL];
+    if (STREAMED) {
+        // forward over the streamed rows: row 0 has no lower coupling (l = 0 leaves r * inv exact)
+        for (; row + TRI_UNROLL <= n0; row += TRI_UNROLL) {
+            double r[TRI_UNROLL], v[TRI_UNROLL];
 #pragma unroll
-        for (int u = 0; u < TRI_UNROLL; u++) {
-            r[u] = ph[(row + u) * plane];
-            iv[u] = ti[(long)(row + u) * nmodes];
-        }
-#pragma unroll
-        for (int u = 0; u < TRI_UNROLL; u++) {
-            const double l = row + u > 0 ? pb.dl[row + u - 1] : 0.0;
-            dp = (r[u] - l * dp) * iv[u];
-            ph[(row + u) * plane] = dp;
-        }
-    }
-    for (; row < n0; row++) {
-        const double l = row > 0 ? pb.dl[row - 1] : 0.0;
-        dp = (ph[row * plane] - l * dp) * ti[(long)row * nmodes];
-        ph[row * plane] = dp;
-    }
-    double keep[TRI_KEEP];
-    {
-        double r[TRI_KEEP], iv[TRI_KEEP];
-#pragma unroll
-        for (int u = 0; u < TRI_KEEP; u++) {
-            r[u] = u < nk ? ph[(n0 + u) * plane] : 0.0;
-            iv[u] = u < nk ? ti[(long)(n0 + u) * nmodes] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < TRI_KEEP; u++) {
-            if (u < nk) {
-                const double l = n0 + u > 0 ? pb.dl[n0 + u - 1] : 0.0;
-                dp = (r[u] - l * dp) * iv[u];
+            for (int u = 0; u < TRI_UNROLL; u++) {
+                r[u] = ph[(row + u) * plane];
+                v[u] = ti[(long)(row + u) * nmodes];
             }
-            keep[u] = dp;
+#pragma unroll
+            for (int u = 0; u < TRI_UNROLL; u++) {
+                const double l = row + u > 0 ? pb.dl[row + u - 1] : 0.0;
+                dp = (r[u] - l * dp) * v[u];
+                ph[(row + u) * plane] = dp;
+            }
         }
+        for (; row < n0; row++) {
+            const double l = row > 0 ? pb.dl[row - 1] : 0.0;
+            dp = (ph[row * plane] - l * dp) * ti[(long)row * nmodes];
+            ph[row * plane] = dp;
+        }
+        // the register rows with their table pivots (slots past the last row
+        // re-read row n-1 and are not used)
+#pragma unroll
+        for (int u = 0; u < TRI_KEEP; u++) {
+            const long rw = min(n0 + u, n - 1);
+            keep[u] = ph[rw * plane];
+            iv[u] = ti[rw * nmodes];
+        }
+    } else {
+        // right-hand sides first, then the pivot chain while they arrive: the
+        // arithmetic of k_tridiag_setup (on the first row the lower term is 0 * 0)
+#pragma unroll
+        for (int u = 0; u < TRI_KEEP; u++) keep[u] = ph[(long)min(u, n - 1) * plane];
+        const int kx = mode / d.nzp, kz = mode - kx * d.nzp;
+        const double lx = pb.kxx[kx], lz = pb.kzz[kz];
+        double c = 0.0;
+#pragma unroll
+        for (int u = 0; u < TRI_KEEP; u++) {
+            double den = ((sd[u] + lx) + lz) - sl[u] * c;
+            if (u == 0) den = kx == 0 && kz == 0 ? 3.0 / 2.0 * den : den;
+            iv[u] = 1.0 / den;
+            c = su[u] * iv[u];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < TRI_KEEP; u++) {
+        dp = (keep[u] - sl[u] * dp) * iv[u];
+        keep[u] = dp;
     }
     // back substitution: x(n-1) = dp(n-1), x(j) = dp(j) - c'(j) x(j+1)
     double xn = 0.0;
-    {
-        double cc[TRI_KEEP];
 #pragma unroll
-        for (int u = 0; u < TRI_KEEP; u++) cc[u] = u < nk ? tc[(long)(n0 + u) * nmodes] : 0.0;
+    for (int u = TRI_KEEP - 1; u >= 0; u--) {
+        if (u < nk) {
+            xn = u == nk - 1 ? keep[u] : keep[u] - su[u] * iv[u] * xn;
+            ph[(n0 + u) * plane] = xn;
+        }
+    }
+    if (STREAMED) {
+        row = n0 - 1;
+        for (; row - TRI_UNROLL + 1 >= 0; row -= TRI_UNROLL) {
+            double r[TRI_UNROLL], v[TRI_UNROLL];
 #pragma unroll
-        for (int u = TRI_KEEP - 1; u >= 0; u--) {
-            if (u < nk) {
-                xn = u == nk - 1 ? keep[u] : keep[u] - cc[u] * xn;
-                ph[(n0 + u) * plane] = xn;
+            for (int u = 0; u < TRI_UNROLL; u++) {
+                r[u] = ph[(row - u) * plane];
+                v[u] = ti[(long)(row - u) * nmodes];
+            }
+#pragma unroll
+            for (int u = 0; u < TRI_UNROLL; u++) {
+                xn = r[u] - pb.du[row - u] * v[u] * xn;
+                ph[(row - u) * plane] = xn;
             }
         }
-    }
-    row = n0 - 1;
-    for (; row - TRI_UNROLL + 1 >= 0; row -= TRI_UNROLL) {
-        double r[TRI_UNROLL], cc[TRI_UNROLL];
-#pragma unroll
-        for (int u = 0; u < TRI_UNROLL; u++) {
-            r[u] = ph[(row - u) * plane];
-            cc[u] = tc[(long)(row - u) * nmodes];
+        for (; row >= 0; row--) {
+            xn = ph[row * plane] - pb.du[row] * ti[(long)row * nmodes] * xn;
+            ph[row * plane] = xn;
         }
-#pragma unroll
-        for (int u = 0; u < TRI_UNROLL; u++) {
-            xn = r[u] - cc[u] * xn;
-            ph[(row - u) * plane] = xn;
-        }
-    }
-    for (; row >= 0; row--) {
-        xn = ph[row * plane] - tc[(long)row * nmodes] * xn;
-        ph[row * plane] = xn;
     }
 }
 
@@ -473,6 +494,8 @@ void launch_poisson_forward_field(const double* rhs, const Dims& d, PoissonBufs&
     PROF_END(s);
 }
 
+bool tridiag_needs_table(const Dims& d) { return d.ny - 1 > TRI_KEEP; }
+
 void launch_tridiag_setup(const Dims& d, PoissonBufs& pb, hipStream_t s)
 {
     k_tridiag_setup<<<cdiv((long)d.nkx * d.nzp, TRI_THREADS), TRI_THREADS, 0, s>>>(d, pb);
@@ -482,7 +505,10 @@ void launch_poisson_solve(const Dims& d, PoissonBufs& pb, hipStream_t s)
 {
     const long lanes = 2L * d.nkx * d.nzp;
     PROF_BEGIN("k_tridiag", s);
-    k_tridiag<<<cdiv(lanes, TRI_THREADS), TRI_THREADS, 0, s>>>(d, pb);
+    if (d.ny - 1 > TRI_KEEP)
+        k_tridiag<true><<<cdiv(lanes, TRI_THREADS), TRI_THREADS, 0, s>>>(d, pb);
+    else
+        k_tridiag<false><<<cdiv(lanes, TRI_THREADS), TRI_THREADS, 0, s>>>(d, pb);
     PROF_END(s);
 }
 
