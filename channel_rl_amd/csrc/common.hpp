@@ -121,14 +121,19 @@ inline int round_filling_chunk(long tiles, int nk, long slots, int lo, int hi)
 // workgroup id so that each XCD receives a contiguous run of tiles (x
 // fastest, then y, then z) keeps neighbours, and their halos, in one L2.
 // A bijection on 0..n-1 for any n.
-struct TileId { int x, y, z; };
-__device__ inline TileId xcd_tile()
+__device__ inline int xcd_linear()
 {
     const int gx = gridDim.x, gy = gridDim.y;
     const int n = gx * gy * gridDim.z;
     const int h = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
     const int q = n / 8, r = n % 8, x = h % 8;
-    const int l = x * q + (x < r ? x : r) + h / 8;
+    return x * q + (x < r ? x : r) + h / 8;
+}
+struct TileId { int x, y, z; };
+__device__ inline TileId xcd_tile()
+{
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int l = xcd_linear();
     return TileId{l % gx, (l / gx) % gy, l / (gx * gy)};
 }
 

@@ -179,7 +179,9 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_fwd(const T* U, const T* V
     double2* tw = lds;
     double2* a = lds + N;
     const int npairs = (d.ny - 1) * (d.nzp / 2);
-    const int pair0 = blockIdx.x * B;
+    // consecutive blocks share V rows (row j-1 of one is row j of a block
+    // nzp/2B before it): XCD-contiguous order keeps both in one L2
+    const int pair0 = xcd_linear() * B;
     load_twiddles(tw, pb.fx);
     for (int q = threadIdx.x; q < N * B; q += blockDim.x) {
         const int b = quo(q, qN), n = q - b * N;
@@ -257,9 +259,9 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftz(Dims d, PoissonBufs pb)
 //
 // Systems of at most TRI_KEEP rows (k_tridiag<false>) live in registers: the
 // right-hand sides are loaded first, all in flight at once; the pivot chain
-// (one division per row, no memory) runs while they arrive and keeps every
-// 1/den_j; the two sweeps then read registers only and each row is written
-// once.  A wave can hold few loads in flight, so a pivot table in memory
+// (one division per row, no global memory) runs while they arrive and keeps
+// every 1/den_j in LDS; the two sweeps then touch registers and LDS only and
+// each row is written once.  A wave can hold few loads in flight, so a pivot table in memory
 // would cost more than the divisions.  Taller systems (k_tridiag<true>)
 // stream their first rows through memory with pivots from a [row][mode]
 // table that k_tridiag_setup computes once per run by the same arithmetic.
@@ -297,10 +299,18 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
     const int n = d.ny - 1;
     const int nk = n < TRI_KEEP ? n : TRI_KEEP;  // rows n0..n-1 stay in registers
     const int n0 = STREAMED ? n - nk : 0;
+    const int mode = (int)(min(g, 2L * nmodes - 1) >> 1);
+    const int kx = mode / d.nzp, kz = mode - kx * d.nzp;
+    // the mode's wavenumbers before the right-hand sides: loads complete in
+    // issue order, so the pivot chain can then run while the rows arrive
+    const double lx = STREAMED ? 0.0 : pb.kxx[kx], lz = STREAMED ? 0.0 : pb.kzz[kz];
     // the register rows' matrix entries, read by every lane at the same
     // address; slots past the last row hold an identity row so that the
     // unrolled chains run without branches (their values there are not used)
     __shared__ double sd[TRI_KEEP], sl[TRI_KEEP], su[TRI_KEEP];
+    // 1/den of the register rows for the block's modes (both lanes of a mode
+    // write the same value; each lane reads back what it wrote)
+    __shared__ double piv[TRI_KEEP][TRI_THREADS / 2];
     for (int u = threadIdx.x; u < TRI_KEEP; u += TRI_THREADS) {
         const int rw = n0 + u;
         sd[u] = u < nk ? pb.dd[rw] : 1.0;
@@ -309,11 +319,12 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
     }
     __syncthreads();
     if (g >= 2L * nmodes) return;
-    const int mode = (int)(g >> 1);
     const long plane = 2L * nmodes;
     double* ph = reinterpret_cast<double*>(pb.ph) + g;
     const double* ti = STREAMED ? pb.tri + mode : nullptr;  // the table exists for tall systems only
-    double keep[TRI_KEEP], iv[TRI_KEEP];
+    double* pv = &piv[0][threadIdx.x >> 1];
+    constexpr int PS = TRI_THREADS / 2;  // row stride of pv
+    double keep[TRI_KEEP];
     double dp = 0.0;
     int row = 0;
     if (STREAMED) {
@@ -343,27 +354,30 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
         for (int u = 0; u < TRI_KEEP; u++) {
             const long rw = min(n0 + u, n - 1);
             keep[u] = ph[rw * plane];
-            iv[u] = ti[rw * nmodes];
+            pv[u * PS] = ti[rw * nmodes];
         }
     } else {
         // right-hand sides first, then the pivot chain while they arrive: the
         // arithmetic of k_tridiag_setup (on the first row the lower term is 0 * 0)
 #pragma unroll
         for (int u = 0; u < TRI_KEEP; u++) keep[u] = ph[(long)min(u, n - 1) * plane];
-        const int kx = mode / d.nzp, kz = mode - kx * d.nzp;
-        const double lx = pb.kxx[kx], lz = pb.kzz[kz];
+        // keeps the compiler from sinking the loads below the chain
+        asm volatile("" ::: "memory");
         double c = 0.0;
 #pragma unroll
         for (int u = 0; u < TRI_KEEP; u++) {
             double den = ((sd[u] + lx) + lz) - sl[u] * c;
             if (u == 0) den = kx == 0 && kz == 0 ? 3.0 / 2.0 * den : den;
-            iv[u] = 1.0 / den;
-            c = su[u] * iv[u];
+            const double inv = 1.0 / den;
+            pv[u * PS] = inv;
+            c = su[u] * inv;
         }
     }
+    // the sweeps read the pivots back from LDS (no forwarding in registers)
+    asm volatile("" ::: "memory");
 #pragma unroll
     for (int u = 0; u < TRI_KEEP; u++) {
-        dp = (keep[u] - sl[u] * dp) * iv[u];
+        dp = (keep[u] - sl[u] * dp) * pv[u * PS];
         keep[u] = dp;
     }
     // back substitution: x(n-1) = dp(n-1), x(j) = dp(j) - c'(j) x(j+1)
@@ -371,7 +385,7 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
 #pragma unroll
     for (int u = TRI_KEEP - 1; u >= 0; u--) {
         if (u < nk) {
-            xn = u == nk - 1 ? keep[u] : keep[u] - su[u] * iv[u] * xn;
+            xn = u == nk - 1 ? keep[u] : keep[u] - su[u] * pv[u * PS] * xn;
             ph[(n0 + u) * plane] = xn;
         }
     }
@@ -530,9 +544,12 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_project(T* U, T* V, T* W, const double* __restrict__ p, Metric<double> m,
                                                   Dims d)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 2;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
-    const int k = blockIdx.z + 2;
+    // XCD-contiguous tile order: the plane k+1 that a tile reads is the
+    // plane k of a tile that runs on the same XCD, so p is fetched once
+    const TileId t = xcd_tile();
+    const int i = t.x * blockDim.x + threadIdx.x + 2;
+    const int j = t.y * blockDim.y + threadIdx.y + 2;
+    const int k = t.z + 2;
     if (i > d.nxg - 2 || j > d.nyg - 1) return;
     const long q = at(d, i, j, k);
     const double p0 = p[q];
