@@ -257,14 +257,15 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftz(Dims d, PoissonBufs pb)
 // is real, so the elimination's pivots 1/den_j depend on the mode only, and
 // the modified upper diagonal is c'_j = Dyy_j,j+1 / den_j.
 //
-// Systems of at most TRI_KEEP rows (k_tridiag<false>) live in registers: the
-// right-hand sides are loaded first, all in flight at once; the pivot chain
-// (one division per row, no global memory) runs while they arrive and keeps
-// every 1/den_j in LDS; the two sweeps then touch registers and LDS only and
-// each row is written once.  A wave can hold few loads in flight, so a pivot table in memory
-// would cost more than the divisions.  Taller systems (k_tridiag<true>)
-// stream their first rows through memory with pivots from a [row][mode]
-// table that k_tridiag_setup computes once per run by the same arithmetic.
+// Systems of at most TRI_KEEP rows (k_tridiag<false, ..>) live in
+// registers: the right-hand sides are loaded first, all in flight at once;
+// the pivot chain (one division per row, no global memory) runs while they
+// arrive and keeps every 1/den_j in LDS; the two sweeps then touch registers
+// and LDS only and each row is written once.  A wave can hold few loads in
+// flight, so a pivot table in memory would cost more than the divisions.
+// Taller systems (k_tridiag<true, ..>) stream their first rows through memory
+// with pivots from a [row][mode] table that k_tridiag_setup computes once per
+// run by the same arithmetic.
 static constexpr int TRI_THREADS = 64;
 static constexpr int TRI_UNROLL = 12;
 static constexpr int TRI_KEEP = 96;
@@ -291,13 +292,15 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_setup(Dims d, PoissonBu
 }
 
 // one lane per (mode, real/imaginary part); modes are kx-major, kz fastest
-template <bool STREAMED>
+// KEEP register rows: the smallest of 32, 64, 96 that holds the system, or
+// TRI_KEEP with the rows before them STREAMED
+template <bool STREAMED, int KEEP>
 __global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
 {
     const long g = (long)blockIdx.x * TRI_THREADS + threadIdx.x;
     const int nmodes = d.nkx * d.nzp;
     const int n = d.ny - 1;
-    const int nk = n < TRI_KEEP ? n : TRI_KEEP;  // rows n0..n-1 stay in registers
+    const int nk = n < KEEP ? n : KEEP;  // rows n0..n-1 stay in registers
     const int n0 = STREAMED ? n - nk : 0;
     const int mode = (int)(min(g, 2L * nmodes - 1) >> 1);
     const int kx = mode / d.nzp, kz = mode - kx * d.nzp;
@@ -307,11 +310,11 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
     // the register rows' matrix entries, read by every lane at the same
     // address; slots past the last row hold an identity row so that the
     // unrolled chains run without branches (their values there are not used)
-    __shared__ double sd[TRI_KEEP], sl[TRI_KEEP], su[TRI_KEEP];
+    __shared__ double sd[KEEP], sl[KEEP], su[KEEP];
     // 1/den of the register rows for the block's modes (both lanes of a mode
     // write the same value; each lane reads back what it wrote)
-    __shared__ double piv[TRI_KEEP][TRI_THREADS / 2];
-    for (int u = threadIdx.x; u < TRI_KEEP; u += TRI_THREADS) {
+    __shared__ double piv[KEEP][TRI_THREADS / 2];
+    for (int u = threadIdx.x; u < KEEP; u += TRI_THREADS) {
         const int rw = n0 + u;
         sd[u] = u < nk ? pb.dd[rw] : 1.0;
         sl[u] = u < nk && rw > 0 ? pb.dl[rw - 1] : 0.0;
@@ -324,7 +327,7 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
     const double* ti = STREAMED ? pb.tri + mode : nullptr;  // the table exists for tall systems only
     double* pv = &piv[0][threadIdx.x >> 1];
     constexpr int PS = TRI_THREADS / 2;  // row stride of pv
-    double keep[TRI_KEEP];
+    double keep[KEEP];
     double dp = 0.0;
     int row = 0;
     if (STREAMED) {
@@ -351,7 +354,7 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
         // the register rows with their table pivots (slots past the last row
         // re-read row n-1 and are not used)
 #pragma unroll
-        for (int u = 0; u < TRI_KEEP; u++) {
+        for (int u = 0; u < KEEP; u++) {
             const long rw = min(n0 + u, n - 1);
             keep[u] = ph[rw * plane];
             pv[u * PS] = ti[rw * nmodes];
@@ -360,12 +363,12 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
         // right-hand sides first, then the pivot chain while they arrive: the
         // arithmetic of k_tridiag_setup (on the first row the lower term is 0 * 0)
 #pragma unroll
-        for (int u = 0; u < TRI_KEEP; u++) keep[u] = ph[(long)min(u, n - 1) * plane];
+        for (int u = 0; u < KEEP; u++) keep[u] = ph[(long)min(u, n - 1) * plane];
         // keeps the compiler from sinking the loads below the chain
         asm volatile("" ::: "memory");
         double c = 0.0;
 #pragma unroll
-        for (int u = 0; u < TRI_KEEP; u++) {
+        for (int u = 0; u < KEEP; u++) {
             double den = ((sd[u] + lx) + lz) - sl[u] * c;
             if (u == 0) den = kx == 0 && kz == 0 ? 3.0 / 2.0 * den : den;
             const double inv = 1.0 / den;
@@ -376,14 +379,14 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
     // the sweeps read the pivots back from LDS (no forwarding in registers)
     asm volatile("" ::: "memory");
 #pragma unroll
-    for (int u = 0; u < TRI_KEEP; u++) {
+    for (int u = 0; u < KEEP; u++) {
         dp = (keep[u] - sl[u] * dp) * pv[u * PS];
         keep[u] = dp;
     }
     // back substitution: x(n-1) = dp(n-1), x(j) = dp(j) - c'(j) x(j+1)
     double xn = 0.0;
 #pragma unroll
-    for (int u = TRI_KEEP - 1; u >= 0; u--) {
+    for (int u = KEEP - 1; u >= 0; u--) {
         if (u < nk) {
             xn = u == nk - 1 ? keep[u] : keep[u] - su[u] * pv[u * PS] * xn;
             ph[(n0 + u) * plane] = xn;
@@ -519,10 +522,11 @@ void launch_poisson_solve(const Dims& d, PoissonBufs& pb, hipStream_t s)
 {
     const long lanes = 2L * d.nkx * d.nzp;
     PROF_BEGIN("k_tridiag", s);
-    if (d.ny - 1 > TRI_KEEP)
-        k_tridiag<true><<<cdiv(lanes, TRI_THREADS), TRI_THREADS, 0, s>>>(d, pb);
-    else
-        k_tridiag<false><<<cdiv(lanes, TRI_THREADS), TRI_THREADS, 0, s>>>(d, pb);
+    const int n = d.ny - 1, nb = cdiv(lanes, TRI_THREADS);
+    if (n > TRI_KEEP) k_tridiag<true, TRI_KEEP><<<nb, TRI_THREADS, 0, s>>>(d, pb);
+    else if (n > 64) k_tridiag<false, 96><<<nb, TRI_THREADS, 0, s>>>(d, pb);
+    else if (n > 32) k_tridiag<false, 64><<<nb, TRI_THREADS, 0, s>>>(d, pb);
+    else k_tridiag<false, 32><<<nb, TRI_THREADS, 0, s>>>(d, pb);
     PROF_END(s);
 }
 
