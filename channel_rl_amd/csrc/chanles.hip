@@ -179,6 +179,24 @@ bool factor_plan(int n, FftPlan& pl)
     return true;
 }
 
+// the equilibrium wall model's sampling rows: the first centre row jb >= 2
+// whose successor lies at or above y(1) + h_wm (at most nyg-2), and the
+// linear weight of row jb; h_wm = 0 samples row 2 at its own height
+WallRows wall_rows(const chan_ctx* c, double h_wm)
+{
+    const std::vector<double>&y = c->y, &yg = c->yg;
+    const int ny = c->d.ny, nyg = c->d.nyg;
+    WallRows ws{2, 1.0, yg[2] - y[1], y[ny] - yg[nyg - 1]};
+    if (h_wm > 0.0) {
+        const double yt = y[1] + h_wm;
+        while (ws.jb < nyg - 2 && yg[ws.jb + 1] < yt) ws.jb++;
+        ws.wb = (yg[ws.jb + 1] - yt) / (yg[ws.jb + 1] - yg[ws.jb]);
+        ws.hb = h_wm;
+        ws.ht = h_wm;
+    }
+    return ws;
+}
+
 int make_twiddles(chan_ctx* c, FftPlan& pl)
 {
     std::vector<double2> tw(pl.n);
@@ -309,7 +327,7 @@ void enqueue_rk3(chan_ctx* c, int X)
         T** I = (T**)c->state[in[s]];
         T** O = (T**)c->state[out[s]];
         if (c->wall_mode == CHAN_WALL_EQUILIBRIUM) {
-            launch_equilibrium_alpha<T>(I[0], c->alpha, c->md, c->d, c->nu, c->h_wm, c->kappa, c->Bc, c->stream);
+            launch_equilibrium_alpha<T>(I[0], c->alpha, c->d, c->nu, wall_rows(c, c->h_wm), c->kappa, c->Bc, c->stream);
         } else if (c->wall_mode == CHAN_WALL_MLP) {
             const float *np = nullptr, *nn = nullptr;
             int ndev = 0;
@@ -889,11 +907,11 @@ int chan_compute_equilibrium_wall_stress(chan_ctx* c, double h_wm, double kappa,
     int rc = set_device(c);
     if (rc) return rc;
     if (c->prec == CHAN_FP64)
-        launch_equilibrium_alpha<double>((double*)c->state[c->cur][0], c->alpha, c->md, c->d, c->nu, h_wm, kappa, B,
-                                         c->stream);
+        launch_equilibrium_alpha<double>((double*)c->state[c->cur][0], c->alpha, c->d, c->nu, wall_rows(c, h_wm), kappa,
+                                         B, c->stream);
     else
-        launch_equilibrium_alpha<float>((float*)c->state[c->cur][0], c->alpha, c->md, c->d, c->nu, h_wm, kappa, B,
-                                        c->stream);
+        launch_equilibrium_alpha<float>((float*)c->state[c->cur][0], c->alpha, c->d, c->nu, wall_rows(c, h_wm), kappa,
+                                         B, c->stream);
     return check_launch(c);
 }
 
@@ -1061,11 +1079,11 @@ int chan_evaluate_wall_stress(chan_ctx* c)
     void** s = c->state[c->cur];
     if (c->wall_mode == CHAN_WALL_EQUILIBRIUM) {
         if (c->prec == CHAN_FP64)
-            launch_equilibrium_alpha<double>((double*)s[0], c->alpha, c->md, c->d, c->nu, c->h_wm, c->kappa, c->Bc,
-                                             c->stream);
+            launch_equilibrium_alpha<double>((double*)s[0], c->alpha, c->d, c->nu, wall_rows(c, c->h_wm), c->kappa,
+                                         c->Bc, c->stream);
         else
-            launch_equilibrium_alpha<float>((float*)s[0], c->alpha, c->md, c->d, c->nu, c->h_wm, c->kappa, c->Bc,
-                                            c->stream);
+            launch_equilibrium_alpha<float>((float*)s[0], c->alpha, c->d, c->nu, wall_rows(c, c->h_wm), c->kappa,
+                                         c->Bc, c->stream);
     } else if (c->wall_mode == CHAN_WALL_MLP) {
         const float *np = nullptr, *nn = nullptr;
         int ndev = 0;

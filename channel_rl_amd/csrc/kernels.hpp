@@ -62,9 +62,18 @@ template <typename T>
 void launch_wall_model(const T* U, T* nut, const double* alpha_x, const Metric<T>& m, const Dims& d, T nu,
                        hipStream_t s);
 
+// Where the equilibrium wall model samples U: h_wm above each wall,
+// interpolated between centre rows jb and jb+1 with weight wb on row jb
+// (h_wm = 0: row 2 itself, wb = 1); hb is the bottom sampling height, ht the
+// top one.  Depends on the grid only, so the host computes it.
+struct WallRows {
+    int jb;
+    double wb, hb, ht;
+};
+
 template <typename T>
-void launch_equilibrium_alpha(const T* U, double* alpha_x, const Metric<double>& md, const Dims& d, double nu,
-                              double h_wm, double kappa, double B, hipStream_t s);
+void launch_equilibrium_alpha(const T* U, double* alpha_x, const Dims& d, double nu, const WallRows& ws,
+                              double kappa, double B, hipStream_t s);
 
 template <typename T>
 void launch_mass_flow_init(const T* U, const Metric<double>& md, const Dims& d, double* part, int nblk,

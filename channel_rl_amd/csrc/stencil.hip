@@ -173,30 +173,23 @@ void launch_wall_model(const T* U, T* nut, const double* alpha_x, const Metric<T
 
 // one thread per (x face, z centre, wall)
 template <typename T>
-__global__ void k_equilibrium_alpha(const T* U, double* alpha, Metric<double> m, Dims d, double nu, double h_wm,
-                                    double kappa, double B, double yc)
+__global__ void __launch_bounds__(256) k_equilibrium_alpha(const T* U, double* alpha, Dims d, double nu,
+                                                           WallRows ws, double kappa, double B, double yc)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int k = blockIdx.y + 1;
-    const int wall = blockIdx.z;
-    if (i > d.nx) return;
-    const double *y = m.y, *yg = m.yg;
-    int jb = 2;
-    double wb = 1.0, hb = yg[2] - y[1];
-    if (h_wm > 0.0) {
-        double yt = y[1] + h_wm;
-        while (jb < d.nyg - 2 && yg[jb + 1] < yt) jb++;
-        wb = (yg[jb + 1] - yt) / (yg[jb + 1] - yg[jb]);
-        hb = h_wm;
-    }
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 2 * d.nx * d.nzg) return;
+    const int i = q % d.nx + 1, r = q / d.nx;
+    const int k = r % d.nzg + 1, wall = r / d.nzg;
+    const int jb = ws.jb;
+    const double wb = ws.wb;
     double uw, h;
     if (wall == 0) {
         uw = wb * (double)U[at(d, i, jb, k)] + (1.0 - wb) * (double)U[at(d, i, jb + 1, k)];
-        h = hb;
+        h = ws.hb;
     } else {
         const int jt = d.nyg + 1 - jb;
         uw = wb * (double)U[at(d, i, jt, k)] + (1.0 - wb) * (double)U[at(d, i, jt - 1, k)];
-        h = h_wm > 0.0 ? hb : y[d.ny] - yg[d.nyg - 1];
+        h = ws.ht;
     }
     // Newton starts from the cell's previous u_tau (|alpha_x| = u_tau^2): the
     // flow moves little in a step, so a few iterations reach the same root
@@ -206,13 +199,12 @@ __global__ void k_equilibrium_alpha(const T* U, double* alpha, Metric<double> m,
 }
 
 template <typename T>
-void launch_equilibrium_alpha(const T* U, double* alpha_x, const Metric<double>& md, const Dims& d, double nu,
-                              double h_wm, double kappa, double B, hipStream_t s)
+void launch_equilibrium_alpha(const T* U, double* alpha_x, const Dims& d, double nu, const WallRows& ws,
+                              double kappa, double B, hipStream_t s)
 {
-    dim3 blk(64), grd(cdiv(d.nx, 64), d.nzg, 2);
     const double yc = loglaw_crossover(kappa, B);
     PROF_BEGIN("k_equilibrium_alpha", s);
-    k_equilibrium_alpha<T><<<grd, blk, 0, s>>>(U, alpha_x, md, d, nu, h_wm, kappa, B, yc);
+    k_equilibrium_alpha<T><<<cdiv(2L * d.nx * d.nzg, 256), 256, 0, s>>>(U, alpha_x, d, nu, ws, kappa, B, yc);
     PROF_END(s);
 }
 
@@ -560,8 +552,8 @@ void launch_wall_shear(const T* U, const T* nut, const Metric<double>& md, const
                                 int, int, const double*, T, hipStream_t);                                       \
     template void launch_wall_model<T>(const T*, T*, const double*, const Metric<T>&, const Dims&, T,           \
                                        hipStream_t);                                                            \
-    template void launch_equilibrium_alpha<T>(const T*, double*, const Metric<double>&, const Dims&, double,    \
-                                              double, double, double, hipStream_t);                             \
+    template void launch_equilibrium_alpha<T>(const T*, double*, const Dims&, double, const WallRows&,        \
+                                              double, double, hipStream_t);                                     \
     template void launch_mass_flow_init<T>(const T*, const Metric<double>&, const Dims&, double*, int,          \
                                            Scalars*, hipStream_t);                                              \
     template void launch_mass_flow_correct<T>(T*, const Dims&, double*, int, Scalars*, int, hipStream_t);       \
