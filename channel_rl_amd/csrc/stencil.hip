@@ -111,10 +111,11 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_dsm_nut(T* nut, Dims d, DsmBufs<T> b, int dirichlet, int fuse_wm,
                                                   const T* U, const double* alpha, Metric<T> m, T nu)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    if (i > d.nxg || j > d.nyg) return;
+    // one (x, y) plane per blockIdx.y, flattened so that every lane has a cell
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= d.nxg * d.nyg) return;
+    const int jm = q / d.nxg;
+    const int i = q - jm * d.nxg + 1, j = jm + 1, k = blockIdx.y + 1;
     const long p = at(d, i, j, k);
     if (fuse_wm && (j <= 2 || j >= d.nyg - 1)) {
         nut[p] = wm_nut_at<T>(U, alpha, m, d, nu, img_centre(i, d.nxg), k, j <= 2 ? 0 : 1);
@@ -135,9 +136,9 @@ void launch_dsm(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, 
     k_dsm_ratio<<<d.nyg - 2, 256, 0, s>>>(b.part, b.nblk, d.nyg, b.ratio);
     PROF_END(s);
     {
-        dim3 blk(64, 4, 1), grd(cdiv(d.nxg, 64), cdiv(d.nyg, 4), d.nzg);
+        dim3 grd(cdiv((long)d.nxg * d.nyg, 256), d.nzg);
         PROF_BEGIN("k_dsm_nut", s);
-        k_dsm_nut<T><<<grd, blk, 0, s>>>(nut, d, b, dirichlet, fuse_wm, U, alpha_x, m, nu);
+        k_dsm_nut<T><<<grd, 256, 0, s>>>(nut, d, b, dirichlet, fuse_wm, U, alpha_x, m, nu);
         PROF_END(s);
     }
 }
