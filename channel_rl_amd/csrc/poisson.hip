@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftz(Dims d, PoissonBufs pb)
 // with pivots from a [row][mode] table that k_tridiag_setup computes once per
 // run by the same arithmetic.
 static constexpr int TRI_THREADS = 64;
-static constexpr int TRI_UNROLL = 12;
+static constexpr int TRI_GROUP = 8;
 static constexpr int TRI_KEEP = 96;
 
 __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_setup(Dims d, PoissonBufs pb)
@@ -330,26 +330,49 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
     double keep[KEEP];
     double dp = 0.0;
     int row = 0;
-    if (STREAMED) {
-        // forward over the streamed rows: row 0 has no lower coupling (l = 0 leaves r * inv exact)
-        for (; row + TRI_UNROLL <= n0; row += TRI_UNROLL) {
-            double r[TRI_UNROLL], v[TRI_UNROLL];
+    // streamed rows: groups of TRI_GROUP rows with their table pivots and
+    // off-diagonal entries, the next group's loads in flight while one is
+    // computed; every load is unconditional (rows clamped into 0..n0-1, the
+    // extra slots unused) so that no branch makes the compiler drain the
+    // memory counter, which completes in issue order
+    constexpr int G = TRI_GROUP;
+    double ra[G], va[G], ca[G], rb[G], vb[G], cb[G];
+    // dir +1: rows r0.. with l_j (0 on row 0); dir -1: rows r0, r0-1, .. with u_j
+    auto load_rows = [&](int r0, int dir, double (&r)[G], double (&v)[G], double (&c)[G]) {
 #pragma unroll
-            for (int u = 0; u < TRI_UNROLL; u++) {
-                r[u] = ph[(row + u) * plane];
-                v[u] = ti[(long)(row + u) * nmodes];
-            }
+        for (int u = 0; u < G; u++) {
+            const int rw = min(max(r0 + dir * u, 0), n0 - 1);
+            r[u] = ph[(long)rw * plane];
+            v[u] = ti[(long)rw * nmodes];
+            c[u] = dir > 0 ? (rw > 0 ? pb.dl[rw - 1] : 0.0) : pb.du[rw];
+        }
+    };
+    auto forward = [&](int r0, const double (&r)[G], const double (&v)[G], const double (&l)[G]) {
 #pragma unroll
-            for (int u = 0; u < TRI_UNROLL; u++) {
-                const double l = row + u > 0 ? pb.dl[row + u - 1] : 0.0;
-                dp = (r[u] - l * dp) * v[u];
-                ph[(row + u) * plane] = dp;
+        for (int u = 0; u < G; u++) {
+            if (r0 + u < n0) {
+                dp = (r[u] - l[u] * dp) * v[u];
+                ph[(r0 + u) * plane] = dp;
             }
         }
-        for (; row < n0; row++) {
-            const double l = row > 0 ? pb.dl[row - 1] : 0.0;
-            dp = (ph[row * plane] - l * dp) * ti[(long)row * nmodes];
-            ph[row * plane] = dp;
+    };
+    // back substitution, rows r0, r0-1, ...: x(j) = dp(j) - c'(j) x(j+1)
+    auto backward = [&](int r0, const double (&r)[G], const double (&v)[G], const double (&up)[G], double& x) {
+#pragma unroll
+        for (int u = 0; u < G; u++) {
+            if (r0 - u >= 0) {
+                x = r[u] - up[u] * v[u] * x;
+                ph[(r0 - u) * plane] = x;
+            }
+        }
+    };
+    if (STREAMED) {
+        load_rows(0, 1, ra, va, ca);
+        for (; row < n0; row += 2 * G) {
+            load_rows(row + G, 1, rb, vb, cb);
+            forward(row, ra, va, ca);
+            load_rows(row + 2 * G, 1, ra, va, ca);
+            forward(row + G, rb, vb, cb);
         }
         // the register rows with their table pivots (slots past the last row
         // re-read row n-1 and are not used)
@@ -393,23 +416,12 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
         }
     }
     if (STREAMED) {
-        row = n0 - 1;
-        for (; row - TRI_UNROLL + 1 >= 0; row -= TRI_UNROLL) {
-            double r[TRI_UNROLL], v[TRI_UNROLL];
-#pragma unroll
-            for (int u = 0; u < TRI_UNROLL; u++) {
-                r[u] = ph[(row - u) * plane];
-                v[u] = ti[(long)(row - u) * nmodes];
-            }
-#pragma unroll
-            for (int u = 0; u < TRI_UNROLL; u++) {
-                xn = r[u] - pb.du[row - u] * v[u] * xn;
-                ph[(row - u) * plane] = xn;
-            }
-        }
-        for (; row >= 0; row--) {
-            xn = ph[row * plane] - pb.du[row] * ti[(long)row * nmodes] * xn;
-            ph[row * plane] = xn;
+        load_rows(n0 - 1, -1, ra, va, ca);
+        for (row = n0 - 1; row >= 0; row -= 2 * G) {
+            load_rows(row - G, -1, rb, vb, cb);
+            backward(row, ra, va, ca, xn);
+            load_rows(row - 2 * G, -1, ra, va, ca);
+            backward(row - G, rb, vb, cb, xn);
         }
     }
 }
@@ -523,7 +535,7 @@ void launch_poisson_solve(const Dims& d, PoissonBufs& pb, hipStream_t s)
     const long lanes = 2L * d.nkx * d.nzp;
     PROF_BEGIN("k_tridiag", s);
     const int n = d.ny - 1, nb = cdiv(lanes, TRI_THREADS);
-    if (n > TRI_KEEP) k_tridiag<true, TRI_KEEP><<<nb, TRI_THREADS, 0, s>>>(d, pb);
+    if (n > TRI_KEEP) k_tridiag<true, 32><<<nb, TRI_THREADS, 0, s>>>(d, pb);
     else if (n > 64) k_tridiag<false, 96><<<nb, TRI_THREADS, 0, s>>>(d, pb);
     else if (n > 32) k_tridiag<false, 64><<<nb, TRI_THREADS, 0, s>>>(d, pb);
     else k_tridiag<false, 32><<<nb, TRI_THREADS, 0, s>>>(d, pb);
