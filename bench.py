@@ -42,17 +42,18 @@ ALG_FLOPS = {"k_dsm_fused": 519}
 #   k_dsm_nut: reads |S|, writes nu_t (pass B)
 #   k_fftx_fwd: reads U,V,W (divergence), writes p-hat (complex, half spectrum = 1 double/cell)
 #   k_fftz: reads and writes p-hat (1 complex per 2 cells each way: 2 doubles/cell)
-#   k_tridiag: reads p-hat twice, writes twice (forward + back substitution): 4
+#   k_tridiag: reads p-hat once, writes it once (the rows of a system stay in
+#              registers between the sweeps): 2
 #   k_fftx_inv: reads p-hat (1), writes p (1)
 #   k_project: reads p, U,V,W, writes U,V,W: 7
-# A whole step moves 122 values per cell (976 B FP64, 632 B FP32 bulk).
+# A whole step moves 116 values per cell (928 B FP64, 584 B FP32 bulk).
 ALG_VALUES = {
     "k_rhs_rk": (12, 0),
     "k_dsm_fused": (4, 0),
     "k_dsm_nut": (2, 0),
     "k_fftx_fwd": (3, 1),
     "k_fftz": (0, 2),
-    "k_tridiag": (0, 4),
+    "k_tridiag": (0, 2),
     "k_fftx_inv": (0, 2),
     "k_project": (6, 1),
 }
@@ -239,7 +240,7 @@ def main():
                 "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": alg_bytes,
                 "step_ms_profiled": round(tot / args.profile_steps, 3),
                 "copy_kernel_gbs": round(copy_gbs, 1),
-                "step_alg_gbs": round(cells * (976 if prec == cr.FP64 else 632) / (ms_step * 1e-3) / 1e9, 1)}
+                "step_alg_gbs": round(cells * (928 if prec == cr.FP64 else 584) / (ms_step * 1e-3) / 1e9, 1)}
     if dom in ALG_FLOPS:
         # the same kernel against the vector ALU peak of its arithmetic type
         # (no MFMA on this path): its bound is issue and latency, not HBM
