@@ -426,8 +426,8 @@ struct DsmRows {
 };
 
 // z-chunk length filling the last round of blocks (e.g. 192x97x192 FP64,
-// 112 tiles per chunk on 512 slots: 22 planes, 2 rounds, against 16 planes,
-// 3 rounds).
+// 49 tiles per chunk on 256 slots: 39 planes, 5 chunks, one round, against
+// 20 planes, 10 chunks, two rounds).
 template <typename T>
 int dsm_fused_kchunk(const Dims& d, int device)
 {
@@ -439,7 +439,7 @@ int dsm_fused_kchunk(const Dims& d, int device)
         per_cu = 1;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu < 1) ncu = 1;
     const long tiles = (long)cdiv(d.nxg - 2, DSM_TXO) * cdiv(d.nyg - 2, TYO);
-    return round_filling_chunk(tiles, d.nzg - 2, (long)per_cu * ncu, 8, 32);
+    return round_filling_chunk(tiles, d.nzg - 2, (long)per_cu * ncu, 8, 64);
 }
 
 int dsm_fused_nblk(const Dims& d, int kchunk)
