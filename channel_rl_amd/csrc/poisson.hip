@@ -292,8 +292,9 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_setup(Dims d, PoissonBu
 }
 
 // one lane per (mode, real/imaginary part); modes are kx-major, kz fastest
-// KEEP register rows: the smallest of 32, 64, 96 that holds the system, or
-// TRI_KEEP with the rows before them STREAMED
+// KEEP register rows: the smallest of 32, 64, 96 that holds the system; a
+// system of more than TRI_KEEP rows keeps its last 32 in registers (four
+// waves per SIMD) and STREAMS the rows before them
 template <bool STREAMED, int KEEP>
 __global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
 {
