@@ -568,6 +568,14 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
             c->rcp[0][q] = pd;
             c->rcp[1][q] = pf;
         }
+        c->md.rdx = (const double*)c->rcp[0][0]; c->md.rdy = (const double*)c->rcp[0][1];
+        c->md.rdz = (const double*)c->rcp[0][2];
+        c->md.rdxg = (const double*)c->rcp[0][3]; c->md.rdyg = (const double*)c->rcp[0][4];
+        c->md.rdzg = (const double*)c->rcp[0][5];
+        c->mf.rdx = (const float*)c->rcp[1][0]; c->mf.rdy = (const float*)c->rcp[1][1];
+        c->mf.rdz = (const float*)c->rcp[1][2];
+        c->mf.rdxg = (const float*)c->rcp[1][3]; c->mf.rdyg = (const float*)c->rcp[1][4];
+        c->mf.rdzg = (const float*)c->rcp[1][5];
     }
     // partial sums: DSM rows use one slot per (x,z) block of the fused kernel
     c->dsm_kchunk = c->prec == CHAN_FP64 ? dsm_fused_kchunk<double>(d, c->device) : dsm_fused_kchunk<float>(d, c->device);

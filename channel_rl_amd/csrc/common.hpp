@@ -31,6 +31,8 @@ struct Metric {
     const T *x, *y, *z;      // faces
     const T *xg, *yg, *zg;   // centres + ghosts
     const T *w0, *w1;        // weight_y_0/1 (code/initialization.f90:398-400)
+    const T *rdx, *rdy, *rdz;     // reciprocal face spacings 1/(x(i)-x(i-1)), ...
+    const T *rdxg, *rdyg, *rdzg;  // reciprocal centre spacings 1/(xg(i)-xg(i-1)), ...
 };
 
 // Periodic image maps (code/boundary_conditions.f90:60-79; z as its global

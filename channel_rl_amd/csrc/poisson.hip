@@ -152,14 +152,15 @@ __device__ __forceinline__ void load_twiddles(double2* tw, const FftPlan& pl)
     for (int q = threadIdx.x; q < pl.n; q += blockDim.x) tw[q] = pl.tw[q];
 }
 
-// divergence of the staggered velocity at centre (i,j,k), code/projection.f90:45-53
+// divergence of the staggered velocity at centre (i,j,k), code/projection.f90:45-53,
+// with the spacings' reciprocals (within an ulp of the reference's divisions)
 template <typename T>
 __device__ __forceinline__ double div_at(const T* U, const T* V, const T* W, const Metric<double>& m, const Dims& d,
                                          int i, int j, int k)
 {
-    return ((double)U[at(d, i, j, k)] - (double)U[at(d, i - 1, j, k)]) / (m.x[i] - m.x[i - 1]) +
-           ((double)V[at(d, i, j, k)] - (double)V[at(d, i, j - 1, k)]) / (m.y[j] - m.y[j - 1]) +
-           ((double)W[at(d, i, j, k)] - (double)W[at(d, i, j, k - 1)]) / (m.z[k] - m.z[k - 1]);
+    return ((double)U[at(d, i, j, k)] - (double)U[at(d, i - 1, j, k)]) * m.rdx[i] +
+           ((double)V[at(d, i, j, k)] - (double)V[at(d, i, j - 1, k)]) * m.rdy[j] +
+           ((double)W[at(d, i, j, k)] - (double)W[at(d, i, j, k - 1)]) * m.rdz[k];
 }
 
 // pair pr -> (row jr = j-2, first z index kA-2) of the two packed z-rows
@@ -472,7 +473,9 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_inv(Dims d, PoissonBufs pb
     }
     __syncthreads();
     block_fft(a, tw, pb.fx, B, ld, +1);
-    const double scale = (double)((long)d.nxp * d.nzp);
+    // the unnormalised inverse transforms' factor, as a reciprocal (within an
+    // ulp of the division)
+    const double rscale = 1.0 / (double)((long)d.nxp * d.nzp);
     for (int q = threadIdx.x; q < N * B; q += blockDim.x) {
         const int b = quo(q, qN), n = q - b * N;
         const int pr = pair0 + b;
@@ -480,7 +483,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_inv(Dims d, PoissonBufs pb
         int jr, kz;
         pair_rows(d, pr, jr, kz);
         const int j = jr + 2, kA = kz + 2, i = n + 2;
-        const double pa = a[b * ld + n].x / scale, pbv = a[b * ld + n].y / scale;
+        const double pa = a[b * ld + n].x * rscale, pbv = a[b * ld + n].y * rscale;
         pb.p[at(d, i, j, kA)] = pa;
         pb.p[at(d, i, j, kA + 1)] = pbv;
         if (n == 0) {
@@ -570,10 +573,11 @@ __global__ void __launch_bounds__(256) k_project(T* U, T* V, T* W, const double*
     if (i > d.nxg - 2 || j > d.nyg - 1) return;
     const long q = at(d, i, j, k);
     const double p0 = p[q];
-    store_img(U, d, 0, i, j, k, T((double)U[q] - (p[at(d, i + 1, j, k)] - p0) / (m.xg[i + 1] - m.xg[i])));
+    // the spacings' reciprocals: within an ulp of the reference's divisions
+    store_img(U, d, 0, i, j, k, T((double)U[q] - (p[at(d, i + 1, j, k)] - p0) * m.rdxg[i + 1]));
     if (j <= d.ny - 1)
-        store_img(V, d, 1, i, j, k, T((double)V[q] - (p[at(d, i, j + 1, k)] - p0) / (m.yg[j + 1] - m.yg[j])));
-    store_img(W, d, 2, i, j, k, T((double)W[q] - (p[at(d, i, j, k + 1)] - p0) / (m.zg[k + 1] - m.zg[k])));
+        store_img(V, d, 1, i, j, k, T((double)V[q] - (p[at(d, i, j + 1, k)] - p0) * m.rdyg[j + 1]));
+    store_img(W, d, 2, i, j, k, T((double)W[q] - (p[at(d, i, j, k + 1)] - p0) * m.rdzg[k + 1]));
 }
 
 template <typename T>
