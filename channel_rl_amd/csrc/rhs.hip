@@ -249,10 +249,12 @@ __global__ void __launch_bounds__(TX* TY)
         const bool dv = act && (j <= d.ny - 1) && (k <= d.nzg - 1);
         const bool dw = act && (k <= d.nz - 1);
         T uo = T(0), vo = T(0), wo = T(0), fu = T(0), fv = T(0), fw = T(0);
+        // own-cell values read once and the substep-1 F written once: marked
+        // non-temporal so that they do not displace the tiles' halo planes in L2
         if (mode >= 2) {
-            if (du) { uo = Uo[p]; fu = F1u[p]; }
-            if (dv) { vo = Vo[p]; fv = F1v[p]; }
-            if (dw) { wo = Wo[p]; fw = F1w[p]; }
+            if (du) { uo = __builtin_nontemporal_load(&Uo[p]); fu = __builtin_nontemporal_load(&F1u[p]); }
+            if (dv) { vo = __builtin_nontemporal_load(&Vo[p]); fv = __builtin_nontemporal_load(&F1v[p]); }
+            if (dw) { wo = __builtin_nontemporal_load(&Wo[p]); fw = __builtin_nontemporal_load(&F1w[p]); }
         }
         fetch();
         RhsZ<T> gz;
@@ -268,9 +270,9 @@ __global__ void __launch_bounds__(TX* TY)
         } else if (mode == 1) {
             // U = Uo + dt*rk_coef(1,1)*Fu1 with Uo = U at substep 1; the
             // new field is stored with its boundary-condition images
-            if (du) { F1u[p] = ru; store_img(Ou, d, 0, i, j, k, F(0, i, j, k) + dt * c1 * ru); }
-            if (dv) { F1v[p] = rv; store_img(Ov, d, 1, i, j, k, F(1, i, j, k) + dt * c1 * rv); }
-            if (dw) { F1w[p] = rw; store_img(Ow, d, 2, i, j, k, F(2, i, j, k) + dt * c1 * rw); }
+            if (du) { __builtin_nontemporal_store(ru, &F1u[p]); store_img(Ou, d, 0, i, j, k, F(0, i, j, k) + dt * c1 * ru); }
+            if (dv) { __builtin_nontemporal_store(rv, &F1v[p]); store_img(Ov, d, 1, i, j, k, F(1, i, j, k) + dt * c1 * rv); }
+            if (dw) { __builtin_nontemporal_store(rw, &F1w[p]); store_img(Ow, d, 2, i, j, k, F(2, i, j, k) + dt * c1 * rw); }
         } else {
             // U = Uo + dt*(c1*Fu1 + c2*Fu_s); rk_coef(3,2) = 0 so the second stage's F is not kept
             if (du) store_img(Ou, d, 0, i, j, k, uo + dt * (c1 * fu + c2 * ru));
