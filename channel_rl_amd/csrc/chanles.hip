@@ -346,10 +346,12 @@ void enqueue_rk3(chan_ctx* c, int X)
         enqueue_projection<T>(c, out[s]);
     }
     if (c->x_mass_cte == 1) {
+        // the correction's final reduction also advances the time
         T** O = (T**)c->state[Y];
-        launch_mass_flow_correct<T>(O[0], c->d, c->part, c->nblk_red, c->sc, 1, c->stream);
+        launch_mass_flow_correct<T>(O[0], c->d, c->part, c->nblk_red, c->sc, 1, c->stream, 1);
+    } else {
+        k_advance_time<<<1, 1, 0, c->stream>>>(c->sc);
     }
-    k_advance_time<<<1, 1, 0, c->stream>>>(c->sc);
 }
 
 // the captured steps bake in launch shapes and options: drop them on change
@@ -842,9 +844,10 @@ int chan_compute_dPx_for_constant_mass_flow(chan_ctx* c, double* dPdx)
     if ((rc = chan_get_scalars(c, keep))) return rc;
     if (c->prec == CHAN_FP64)
         launch_mass_flow_correct<double>((double*)c->state[c->cur][0], c->d, c->part, c->nblk_red, c->sc, 0,
-                                         c->stream);
+                                         c->stream, 0);
     else
-        launch_mass_flow_correct<float>((float*)c->state[c->cur][0], c->d, c->part, c->nblk_red, c->sc, 0, c->stream);
+        launch_mass_flow_correct<float>((float*)c->state[c->cur][0], c->d, c->part, c->nblk_red, c->sc, 0, c->stream,
+                                        0);
     if ((rc = check_launch(c))) return rc;
     Scalars s0{};
     HIPCHK(c, hipStreamSynchronize(c->stream));

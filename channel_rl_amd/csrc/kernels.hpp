@@ -81,7 +81,7 @@ void launch_mass_flow_init(const T* U, const Metric<double>& md, const Dims& d, 
 
 template <typename T>
 void launch_mass_flow_correct(T* U, const Dims& d, double* part, int nblk, Scalars* sc, int apply,
-                              hipStream_t s);
+                              hipStream_t s, int advance);
 
 template <typename T>
 void launch_dt(const T* U, const T* V, const T* W, const Metric<double>& md, const Dims& d, double* part,

@@ -79,28 +79,26 @@ void launch_bc(T* F, const Dims& d, int kind, hipStream_t s)
 // --------------------------------------------------------------- SGS: DSM
 // Pass A (filters, L_ij, M_ij, plane partial sums, |S|) is dsm_fused.hip.
 
-// <L:M>/<M:M> per row, blocks summed in a fixed order.
-__global__ void k_dsm_ratio(const double* part, int nblk, int nyg, double* ratio)
+// <L:M>/<M:M> per row: one wave per row, the blocks' partials summed in a
+// fixed order (strided per lane, then a butterfly across the wave), so the
+// result does not depend on scheduling; no LDS and no barrier.
+__global__ void __launch_bounds__(64) k_dsm_ratio(const double* part, int nblk, int nyg, double* ratio)
 {
-    __shared__ double red[2][256];
     const int j = blockIdx.x + 2;
     const int t = threadIdx.x;
     double a = 0.0, c = 0.0;
-    for (int q = t; q < nblk; q += blockDim.x) {
+    for (int q = t; q < nblk; q += 64) {
         a += part[((long)(j - 1) * 2 + 0) * nblk + q];
         c += part[((long)(j - 1) * 2 + 1) * nblk + q];
     }
-    red[0][t] = a;
-    red[1][t] = c;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-        if (t < st) { red[0][t] += red[0][t + st]; red[1][t] += red[1][t + st]; }
-        __syncthreads();
+    for (int m = 32; m > 0; m >>= 1) {
+        a += __shfl_xor(a, m, 64);
+        c += __shfl_xor(c, m, 64);
     }
     if (t == 0) {
-        ratio[j - 1] = red[0][0] / red[1][0];
-        ratio[nyg + j - 1] = red[0][0];
-        ratio[2 * nyg + j - 1] = red[1][0];
+        ratio[j - 1] = a / c;
+        ratio[nyg + j - 1] = a;
+        ratio[2 * nyg + j - 1] = c;
     }
 }
 
@@ -133,7 +131,7 @@ void launch_dsm(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, 
 {
     launch_dsm_fused<T>(U, V, W, m, d, b, s);
     PROF_BEGIN("k_dsm_ratio", s);
-    k_dsm_ratio<<<d.nyg - 2, 256, 0, s>>>(b.part, b.nblk, d.nyg, b.ratio);
+    k_dsm_ratio<<<d.nyg - 2, 64, 0, s>>>(b.part, b.nblk, d.nyg, b.ratio);
     PROF_END(s);
     {
         dim3 grd(cdiv((long)d.nxg * d.nyg, 256), d.nzg);
@@ -292,13 +290,14 @@ __global__ void k_dpx_partial(const T* U, Dims d, double* part, int nblk)
     block_sum2(s, c, part, nblk);
 }
 
-__global__ void k_dpx_final(const double* part, int nblk, Scalars* sc)
+__global__ void k_dpx_final(const double* part, int nblk, Scalars* sc, int advance)
 {
     double a, b;
     final_sum2(part, nblk, a, b);
     if (threadIdx.x == 0) {
         sc->last_dPdx = sc->Qflow_x_0 - a / b;
         sc->dPdx = 0.0;  // code/time_integration.f90:92
+        if (advance) sc->t = sc->t + sc->dt;  // the step's time advance, one launch fewer
     }
 }
 
@@ -316,13 +315,14 @@ __global__ void k_shift_u(T* U, Dims d, const Scalars* sc)
 }
 
 template <typename T>
-void launch_mass_flow_correct(T* U, const Dims& d, double* part, int nblk, Scalars* sc, int apply, hipStream_t s)
+void launch_mass_flow_correct(T* U, const Dims& d, double* part, int nblk, Scalars* sc, int apply, hipStream_t s,
+                              int advance)
 {
     PROF_BEGIN("k_dpx_partial", s);
     k_dpx_partial<T><<<nblk, RED_THREADS, 0, s>>>(U, d, part, nblk);
     PROF_END(s);
     PROF_BEGIN("k_dpx_final", s);
-    k_dpx_final<<<1, RED_THREADS, 0, s>>>(part, nblk, sc);
+    k_dpx_final<<<1, RED_THREADS, 0, s>>>(part, nblk, sc, advance);
     PROF_END(s);
     if (apply) {
         // canonical cells only (k up to nzg-2), images written by store_img
@@ -557,7 +557,7 @@ void launch_wall_shear(const T* U, const T* nut, const Metric<double>& md, const
                                               double, double, hipStream_t);                                     \
     template void launch_mass_flow_init<T>(const T*, const Metric<double>&, const Dims&, double*, int,          \
                                            Scalars*, hipStream_t);                                              \
-    template void launch_mass_flow_correct<T>(T*, const Dims&, double*, int, Scalars*, int, hipStream_t);       \
+    template void launch_mass_flow_correct<T>(T*, const Dims&, double*, int, Scalars*, int, hipStream_t, int);  \
     template void launch_dt<T>(const T*, const T*, const T*, const Metric<double>&, const Dims&, double*, int,  \
                                Scalars*, const double*, hipStream_t);                                           \
     template void launch_divergence_check<T>(const T*, const T*, const T*, const Metric<double>&, const Dims&, \
