@@ -293,6 +293,96 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_setup(Dims d, PoissonBu
 }
 
 // one lane per (mode, real/imaginary part); modes are kx-major, kz fastest
+// Systems of at most 2H rows, two-ended: the top half of each system is
+// eliminated downwards and the bottom half upwards at the same time, on
+// neighbouring lanes running one instruction stream, which halves the
+// dependent chain and doubles the waves (four lanes per mode: real and
+// imaginary part, top and bottom).  Step u of a lane is row u (top) or row
+// n-1-u (bottom); with the couplings A (towards the rows already
+// eliminated) and C (towards the rest) both halves run
+//   den = (D + kxx + kzz) - A c',  c' = C / den,  y = (r - A y') / den,
+// the top half over rows 0..hn-1 (hn = ceil(n/2)), the bottom over rows
+// n-1..hn.  The two meet in the 2x2 system of rows hn-1 and hn:
+//   x(hn-1) + c'_top x(hn) = y_top,  x(hn) + c'_bot x(hn-1) = y_bot,
+// which both lanes of a pair solve alike; each half then substitutes back
+// towards its end.  Pivots stay in LDS, rows in registers.
+template <int H>
+__global__ void __launch_bounds__(TRI_THREADS) k_tridiag_tw(Dims d, PoissonBufs pb)
+{
+    const long g = (long)blockIdx.x * TRI_THREADS + threadIdx.x;
+    const int nmodes = d.nkx * d.nzp;
+    const int n = d.ny - 1;
+    const int hn = (n + 1) / 2;
+    const bool bot = (g & 1) != 0;
+    const int hl = bot ? n - hn : hn;  // this lane's rows
+    const int mode = (int)(min(g, 4L * nmodes - 1) >> 2);
+    const int comp = (int)((g >> 1) & 1);
+    const int kx = mode / d.nzp, kz = mode - kx * d.nzp;
+    const double lx = pb.kxx[kx], lz = pb.kzz[kz];
+    __shared__ double sd[2 * H], sl[2 * H], su[2 * H];
+    __shared__ double piv[H][TRI_THREADS];
+    for (int r = threadIdx.x; r < 2 * H; r += TRI_THREADS) {
+        sd[r] = r < n ? pb.dd[r] : 1.0;
+        sl[r] = r < n && r > 0 ? pb.dl[r - 1] : 0.0;
+        su[r] = r < n - 1 ? pb.du[r] : 0.0;
+    }
+    __syncthreads();
+    if (g >= 4L * nmodes) return;
+    const long plane = 2L * nmodes;
+    double* ph = reinterpret_cast<double*>(pb.ph) + 2L * mode + comp;
+    auto row_of = [&](int u) { return bot ? n - 1 - u : u; };
+    // the rows of this half, every load issued first (steps past the half
+    // re-read its last row and are not used)
+    double keep[H];
+#pragma unroll
+    for (int u = 0; u < H; u++) keep[u] = ph[(long)row_of(min(u, hl - 1)) * plane];
+    // keeps the compiler from sinking the loads below the chain
+    asm volatile("" ::: "memory");
+    // pivot chain; row 0's diagonal of the 00 mode scaled by 3/2
+    double c = 0.0;
+    const bool mode00 = kx == 0 && kz == 0;
+#pragma unroll
+    for (int u = 0; u < H; u++) {
+        if (u < hl) {
+            const int r = row_of(u);
+            const double A = bot ? su[r] : sl[r], C = bot ? sl[r] : su[r];
+            double den = ((sd[r] + lx) + lz) - A * c;
+            if (u == 0 && !bot && mode00) den = 3.0 / 2.0 * den;
+            const double inv = 1.0 / den;
+            piv[u][threadIdx.x] = inv;
+            c = C * inv;
+        }
+    }
+    asm volatile("" ::: "memory");
+    double y = 0.0;
+#pragma unroll
+    for (int u = 0; u < H; u++) {
+        if (u < hl) {
+            const int r = row_of(u);
+            const double A = bot ? su[r] : sl[r];
+            y = (keep[u] - A * y) * piv[u][threadIdx.x];
+            keep[u] = y;
+        }
+    }
+    // the two middle rows from the pair's last values (lanes g and g^1)
+    const double co = __shfl_xor(c, 1, 64), yo = __shfl_xor(y, 1, 64);
+    const double ct = bot ? co : c, yt = bot ? yo : y, cb = bot ? c : co, yb = bot ? y : yo;
+    const double xt = (yt - ct * yb) / (1.0 - ct * cb);  // x(hn-1)
+    const double xb = yb - cb * xt;                       // x(hn)
+    double xn = bot ? xb : xt;
+    ph[(long)row_of(hl - 1) * plane] = xn;
+    // back towards the end of the half: x = y - c' x(previous)
+#pragma unroll
+    for (int u = H - 2; u >= 0; u--) {
+        if (u < hl - 1) {
+            const int r = row_of(u);
+            const double C = bot ? sl[r] : su[r];
+            xn = keep[u] - C * piv[u][threadIdx.x] * xn;
+            ph[(long)r * plane] = xn;
+        }
+    }
+}
+
 // KEEP register rows: the smallest of 32, 64, 96 that holds the system; a
 // system of more than TRI_KEEP rows keeps its last 32 in registers (four
 // waves per SIMD) and STREAMS the rows before them
@@ -538,11 +628,11 @@ void launch_poisson_solve(const Dims& d, PoissonBufs& pb, hipStream_t s)
 {
     const long lanes = 2L * d.nkx * d.nzp;
     PROF_BEGIN("k_tridiag", s);
-    const int n = d.ny - 1, nb = cdiv(lanes, TRI_THREADS);
+    const int n = d.ny - 1, nb = cdiv(lanes, TRI_THREADS), nb2 = cdiv(2 * lanes, TRI_THREADS);
     if (n > TRI_KEEP) k_tridiag<true, 32><<<nb, TRI_THREADS, 0, s>>>(d, pb);
-    else if (n > 64) k_tridiag<false, 96><<<nb, TRI_THREADS, 0, s>>>(d, pb);
-    else if (n > 32) k_tridiag<false, 64><<<nb, TRI_THREADS, 0, s>>>(d, pb);
-    else k_tridiag<false, 32><<<nb, TRI_THREADS, 0, s>>>(d, pb);
+    else if (n > 64) k_tridiag_tw<48><<<nb2, TRI_THREADS, 0, s>>>(d, pb);
+    else if (n > 32) k_tridiag_tw<32><<<nb2, TRI_THREADS, 0, s>>>(d, pb);
+    else k_tridiag_tw<16><<<nb2, TRI_THREADS, 0, s>>>(d, pb);
     PROF_END(s);
 }
 
