@@ -258,15 +258,17 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftz(Dims d, PoissonBufs pb)
 // is real, so the elimination's pivots 1/den_j depend on the mode only, and
 // the modified upper diagonal is c'_j = Dyy_j,j+1 / den_j.
 //
-// Systems of at most TRI_KEEP rows (k_tridiag<false, ..>) live in
-// registers: the right-hand sides are loaded first, all in flight at once;
-// the pivot chain (one division per row, no global memory) runs while they
-// arrive and keeps every 1/den_j in LDS; the two sweeps then touch registers
+// Systems of at most TRI_KEEP rows (k_tridiag_tw) live in registers, half a
+// system per lane: the right-hand sides are loaded first, all in flight at
+// once; the pivot chain (one division per row, no global memory) runs while
+// they arrive and keeps every 1/den_j in LDS; the sweeps then touch registers
 // and LDS only and each row is written once.  A wave can hold few loads in
 // flight, so a pivot table in memory would cost more than the divisions.
-// Taller systems (k_tridiag<true, ..>) stream their first rows through memory
-// with pivots from a [row][mode] table that k_tridiag_setup computes once per
-// run by the same arithmetic.
+// Taller systems (k_tridiag<true, 32>) keep their last 32 rows in registers
+// and stream the rows before them through memory, with pivots from a
+// [row][mode] table that k_tridiag_setup computes once per run by the same
+// arithmetic.  (k_tridiag<false, ..>, the one-ended register solve, is the
+// form the two-ended kernel replaced; it is not instantiated.)
 static constexpr int TRI_THREADS = 64;
 static constexpr int TRI_GROUP = 8;
 static constexpr int TRI_KEEP = 96;
@@ -383,9 +385,8 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_tw(Dims d, PoissonBufs 
     }
 }
 
-// KEEP register rows: the smallest of 32, 64, 96 that holds the system; a
-// system of more than TRI_KEEP rows keeps its last 32 in registers (four
-// waves per SIMD) and STREAMS the rows before them
+// KEEP register rows; a system of more than TRI_KEEP rows keeps its last 32
+// in registers (four waves per SIMD) and STREAMS the rows before them
 template <bool STREAMED, int KEEP>
 __global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
 {
