@@ -101,12 +101,13 @@ def test_equilibrium_wall_stress(gpu, n):
         assert rel(got, ref) <= 1e-13, (h, rel(got, ref))
 
 
-@pytest.mark.parametrize("n", SIZES + [(10, 50, 12), (10, 80, 10), (12, 98, 10), (10, 130, 12)])
+@pytest.mark.parametrize("n", SIZES + [(10, 4, 10), (10, 5, 10), (10, 50, 12), (10, 80, 10), (12, 98, 10),
+                                       (10, 130, 12)])
 def test_projection_matches_oracle(gpu, n):
-    """The y solve holds systems of up to 32, 64 or 96 rows in registers
-    (ny - 1 rows: the small sizes, 49, 79); ny = 98 streams one row through
-    memory before the last 96, ny = 130 streams 33 (two unrolled groups of 12
-    and 9 single rows)."""
+    """The y solve of ny - 1 rows: up to 96 two-ended in registers (3 and 4
+    rows, the smallest systems; 49 and 79, odd, where the bottom half is a
+    row shorter; the small sizes), more streamed through memory before the
+    last 32 (97 rows: 65 streamed; 129 rows: 97 streamed, in groups of 8)."""
     case, field = cases.small_case(*n, seed=11)
     o = oracle_for(case)
     o.set_state(*field)
