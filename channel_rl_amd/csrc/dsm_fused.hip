@@ -304,15 +304,18 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     }
     filt_vel(kb - 1, gA, gB);
     level1(kb - 1, Q);
-    for (int c = 0; c < 15; c++) zA[c] = s6 * Q[c];
+    // the 15 filter inputs carry the test filter's weights (1, 4, 1) per
+    // direction, the factor 1/6 of each left for the end (1/216 in all):
+    // each tap set is then one add and one fused multiply-add
+    for (int c = 0; c < 15; c++) zA[c] = Q[c];
     __syncthreads();
     load_plane(kb + 1);
     __syncthreads();
     filt_vel(kb, gB, gA);
     level1(kb, Q);
     for (int c = 0; c < 15; c++) {
-        zA[c] = zA[c] + s23 * Q[c];
-        zB[c] = s6 * Q[c];
+        zA[c] = zA[c] + T(4) * Q[c];
+        zB[c] = Q[c];
     }
     __syncthreads();
     load_plane(kb + 2);
@@ -322,8 +325,9 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     double acc_lm = 0.0, acc_mm = 0.0;
     // y taps of the filter inputs for this column's row (one-sided at rows 2
     // and nyg-1), taken on the output rows 1..TYO only
-    const T wym = (cj == 2) ? T(0) : (cj == d.nyg - 1 ? s13 : s6);
-    const T wyp = (cj == d.nyg - 1) ? T(0) : (cj == 2 ? s13 : s6);
+    // six times the y weights (the one-sided rows' 1/3, 2/3 become 2, 4)
+    const T wym = (cj == 2) ? T(0) : (cj == d.nyg - 1 ? T(2) : T(1));
+    const T wyp = (cj == d.nyg - 1) ? T(0) : (cj == 2 ? T(2) : T(1));
     const int om = cx + CX * (cy - 1), op = cx + CX * (cy + 1);
 
     // one output plane k; za/zb and ga/gb as described above (roles swap)
@@ -333,22 +337,22 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
         level1(k + 1, Q);
         T zown[15];  // z taps of the filter inputs at plane k
         for (int c = 0; c < 15; c++) {
-            zown[c] = za[c] + s6 * Q[c];
-            zb[c] = zb[c] + s23 * Q[c];
-            za[c] = s6 * Q[c];
+            zown[c] = za[c] + Q[c];
+            zb[c] = zb[c] + T(4) * Q[c];
+            za[c] = Q[c];
         }
         for (int c = 0; c < 15; c++) zq[c][tid] = zown[c];
         __syncthreads();
         // phase 2: x-y taps of the 15 filter inputs, every lane (cross-lane moves)
         T tp[15];
         if (cy >= 1 && cy <= TYO) {
-            for (int c = 0; c < 15; c++) tp[c] = wym * zq[c][om] + s23 * zown[c] + wyp * zq[c][op];
+            for (int c = 0; c < 15; c++) tp[c] = wym * zq[c][om] + T(4) * zown[c] + wyp * zq[c][op];
         } else {
             for (int c = 0; c < 15; c++) tp[c] = T(0);
         }
         for (int c = 0; c < 15; c++) {
             const T y = tp[c];
-            tp[c] = s6 * lane_shift<DPP_WAVE_SHR1>(y) + s23 * y + s6 * lane_shift<DPP_WAVE_SHL1>(y);
+            tp[c] = (lane_shift<DPP_WAVE_SHR1>(y) + lane_shift<DPP_WAVE_SHL1>(y)) + T(4) * y;
         }
         if (out_col) {
             // hat(S) from the filtered velocities at the true index (code/subgrid.f90:456)
@@ -373,15 +377,20 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
             acc = acc + T(4) * sh[4] * sh[4];
             acc = acc + T(4) * sh[5] * sh[5];
             const T shm4 = T(4) * sqrt(acc);
+            // tp holds 216 times the filtered inputs: L = s (T_ab - s T_a T_b)
+            // and M = 4|hat S| hat S_ab - s T(|S| S_ab), s = 1/216, the outer
+            // s of L taken once on L:M
+            const T sf = T(1.0 / 216.0);
+            const T sa[3] = {sf * tp[0], sf * tp[1], sf * tp[2]};
             T lm = T(0), mm = T(0);
             for (int q = 0; q < 6; q++) {
-                const T L = tp[3 + q] - tp[pa[q]] * tp[pb[q]];
-                const T M = shm4 * sh[q] - tp[9 + q];
+                const T L = tp[3 + q] - sa[pa[q]] * tp[pb[q]];
+                const T M = shm4 * sh[q] - sf * tp[9 + q];
                 const T wgt = q < 3 ? T(1) : T(2);
                 lm = lm + wgt * (L * M);
                 mm = mm + wgt * (M * M);
             }
-            acc_lm += (double)lm;
+            acc_lm += (double)(sf * lm);
             acc_mm += (double)mm;
         }
         // next level-0 plane into the slot of plane k (last read in phase 1)
