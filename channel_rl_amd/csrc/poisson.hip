@@ -217,38 +217,125 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_fwd(const T* U, const T* V
     }
 }
 
+// First Stockham pass of radix R (span 1) read straight from global memory:
+// butterfly jj of line b takes elements jj + r N/R, consecutive lanes
+// consecutive addresses; the outputs go to LDS where the pass would put
+// them.  All loads are issued before the first one is used.
+template <int R, int SIGN>
+__device__ __forceinline__ void fftz_first_pass(const double2* src, long nrows, double2* a, const double2* tw, int N,
+                                                int B, int ld)
+{
+    constexpr int MB = (FFT_MAXNB / R + FFT_THREADS - 1) / FFT_THREADS;
+    const int NR = N / R, nbf = NR * B;
+    const Quo16 qNR = quo16(NR);
+    double2 v[MB][R];
+#pragma unroll
+    for (int t = 0; t < MB; t++) {
+        const int q = threadIdx.x + t * FFT_THREADS;
+        const int b = quo(q, qNR), jj = q - b * NR;
+        const bool in = q < nbf && b < nrows;
+#pragma unroll
+        for (int r = 0; r < R; r++) v[t][r] = in ? src[(long)b * N + jj + r * NR] : make_double2(0.0, 0.0);
+    }
+    if constexpr (R == 5) __syncthreads();  // the radix-5 DFT reads the twiddle table
+#pragma unroll
+    for (int t = 0; t < MB; t++) {
+        const int q = threadIdx.x + t * FFT_THREADS;
+        if (q < nbf) {
+            const int b = quo(q, qNR), jj = q - b * NR;
+            dft<R>(v[t], tw, N, SIGN);
+#pragma unroll
+            for (int r = 0; r < R; r++) a[b * ld + jj * R + r] = v[t][r];
+        }
+    }
+}
+
+// Last Stockham pass of radix R (span N/R) from LDS straight to global
+// memory: outputs k + r N/R, consecutive lanes consecutive addresses.
+template <int R, int SIGN>
+__device__ __forceinline__ void fftz_last_pass(double2* dst, long nrows, const double2* a, const double2* tw, int N,
+                                               int B, int ld)
+{
+    constexpr int MB = (FFT_MAXNB / R + FFT_THREADS - 1) / FFT_THREADS;
+    const int NR = N / R, nbf = NR * B;
+    const Quo16 qNR = quo16(NR);
+    double2 v[MB][R];
+#pragma unroll
+    for (int t = 0; t < MB; t++) {
+        const int q = threadIdx.x + t * FFT_THREADS;
+        const int b = quo(q, qNR), k = q - b * NR;
+        if (q < nbf) {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[t][r] = a[b * ld + k + r * NR];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < MB; t++) {
+        const int q = threadIdx.x + t * FFT_THREADS;
+        const int b = quo(q, qNR), k = q - b * NR;
+        if (q < nbf && b < nrows) {
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                const double2 w = tw[r * k];
+                v[t][r] = cmul(v[t][r], SIGN < 0 ? w : cconj(w));
+            }
+            dft<R>(v[t], tw, N, SIGN);
+#pragma unroll
+            for (int r = 0; r < R; r++) dst[(long)b * N + k + r * NR] = v[t][r];
+        }
+    }
+}
+
 template <int SIGN>
 __global__ void __launch_bounds__(FFT_THREADS) k_fftz(Dims d, PoissonBufs pb)
 {
     extern __shared__ double2 lds[];
+    const FftPlan& pl = pb.fz;
     const int N = d.nzp, B = pb.bz, ld = N + 1;
-    const Quo16 qN = quo16(N);
     double2* tw = lds;
     double2* a = lds + N;
     const long nlines = (long)(d.ny - 1) * d.nkx;
     const long line0 = (long)blockIdx.x * B;
-    load_twiddles(tw, pb.fz);
-    // every load of the block's lines issued before the first LDS store, so
-    // the HBM round trip is paid once
-    constexpr int MQ = FFT_MAXNB / FFT_THREADS;
-    double2 v[MQ];
-#pragma unroll
-    for (int t = 0; t < MQ; t++) {
-        const int q = threadIdx.x + t * FFT_THREADS;
-        const int b = quo(q, qN), n = q - b * N;
-        v[t] = (q < N * B && line0 + b < nlines) ? pb.ph[(line0 + b) * N + n] : make_double2(0.0, 0.0);
+    const long nrows = nlines - line0;  // lines of this block that exist (>= 1)
+    double2* ln = pb.ph + line0 * N;
+    load_twiddles(tw, pl);
+    if (pl.nr >= 2) {
+        // the first and the last pass run between registers and global
+        // memory, the others in LDS: two LDS sweeps and four barriers fewer
+        // than loading into LDS first and storing from it last
+        const int r0 = pl.r[0];
+        if (r0 == 4) fftz_first_pass<4, SIGN>(ln, nrows, a, tw, N, B, ld);
+        else if (r0 == 2) fftz_first_pass<2, SIGN>(ln, nrows, a, tw, N, B, ld);
+        else if (r0 == 3) fftz_first_pass<3, SIGN>(ln, nrows, a, tw, N, B, ld);
+        else fftz_first_pass<5, SIGN>(ln, nrows, a, tw, N, B, ld);
+        __syncthreads();
+        int Ns = r0;
+        for (int p = 1; p < pl.nr - 1; p++) {
+            const int R = pl.r[p];
+            if (R == 4) fft_pass<4>(a, tw, N, B, ld, Ns, SIGN);
+            else if (R == 2) fft_pass<2>(a, tw, N, B, ld, Ns, SIGN);
+            else if (R == 3) fft_pass<3>(a, tw, N, B, ld, Ns, SIGN);
+            else fft_pass<5>(a, tw, N, B, ld, Ns, SIGN);
+            Ns *= R;
+        }
+        const int rl = pl.r[pl.nr - 1];
+        if (rl == 4) fftz_last_pass<4, SIGN>(ln, nrows, a, tw, N, B, ld);
+        else if (rl == 2) fftz_last_pass<2, SIGN>(ln, nrows, a, tw, N, B, ld);
+        else if (rl == 3) fftz_last_pass<3, SIGN>(ln, nrows, a, tw, N, B, ld);
+        else fftz_last_pass<5, SIGN>(ln, nrows, a, tw, N, B, ld);
+        return;
     }
-#pragma unroll
-    for (int t = 0; t < MQ; t++) {
-        const int q = threadIdx.x + t * FFT_THREADS;
-        const int b = quo(q, qN);
-        if (q < N * B) a[b * ld + q - b * N] = v[t];
-    }
-    __syncthreads();
-    block_fft(a, tw, pb.fz, B, ld, SIGN);
+    // a single pass (N <= 5): through LDS
+    const Quo16 qN = quo16(N);
     for (int q = threadIdx.x; q < N * B; q += blockDim.x) {
         const int b = quo(q, qN), n = q - b * N;
-        if (line0 + b < nlines) pb.ph[(line0 + b) * N + n] = a[b * ld + n];
+        a[b * ld + n] = b < nrows ? ln[(long)b * N + n] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    block_fft(a, tw, pl, B, ld, SIGN);
+    for (int q = threadIdx.x; q < N * B; q += blockDim.x) {
+        const int b = quo(q, qN), n = q - b * N;
+        if (b < nrows) ln[(long)b * N + n] = a[b * ld + n];
     }
 }
 
