@@ -133,11 +133,15 @@ __device__ __forceinline__ void fft_pass(double2* a, const double2* tw, int N, i
 }
 
 // In-place batched Stockham FFT of B transforms of length pl.n held in `a`
-// (element (n,b) at b*ld + n), twiddles tw[t] = exp(-2 pi i t/N) in LDS.
-__device__ void block_fft(double2* a, const double2* tw, const FftPlan& pl, int B, int ld, int sign)
+// (element (n,b) at b*ld + n), twiddles tw[t] = exp(-2 pi i t/N) in LDS:
+// passes p0..p1-1 of the plan (all by default).
+__device__ void block_fft(double2* a, const double2* tw, const FftPlan& pl, int B, int ld, int sign, int p0 = 0,
+                          int p1 = -1)
 {
+    if (p1 < 0) p1 = pl.nr;
     int Ns = 1;
-    for (int p = 0; p < pl.nr; p++) {
+    for (int p = 0; p < p0; p++) Ns *= pl.r[p];
+    for (int p = p0; p < p1; p++) {
         const int R = pl.r[p];
         if (R == 4) fft_pass<4>(a, tw, pl.n, B, ld, Ns, sign);
         else if (R == 2) fft_pass<2>(a, tw, pl.n, B, ld, Ns, sign);
@@ -217,32 +221,48 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_fwd(const T* U, const T* V
     }
 }
 
-// First Stockham pass of radix R (span 1) read straight from global memory:
-// butterfly jj of line b takes elements jj + r N/R, consecutive lanes
-// consecutive addresses; the outputs go to LDS where the pass would put
-// them.  All loads are issued before the first one is used.
-template <int R, int SIGN>
-__device__ __forceinline__ void fftz_first_pass(const double2* src, long nrows, double2* a, const double2* tw, int N,
-                                                int B, int ld)
+// First Stockham pass of radix R (span 1) on values produced in registers:
+// butterfly jj of transform b takes elements n = jj + r N/R, each from
+// elem(valid, b, n) (a global load, or a value computed from loads), and
+// leaves its outputs in LDS where the pass over LDS would put them.  Every
+// load is issued before the first DFT.  BFAST: consecutive lanes take
+// consecutive transforms b (else consecutive jj), whichever makes the
+// elements' addresses consecutive.
+template <int R, int SIGN, bool BFAST, typename F>
+__device__ __forceinline__ void fft_first_pass(double2* a, const double2* tw, int N, int B, int ld, const F& elem)
 {
     constexpr int MB = (FFT_MAXNB / R + FFT_THREADS - 1) / FFT_THREADS;
     const int NR = N / R, nbf = NR * B;
-    const Quo16 qNR = quo16(NR);
+    const Quo16 qd = quo16(BFAST ? B : NR);
     double2 v[MB][R];
+    // lanes past the last butterfly (q >= nbf) load the elements of
+    // butterfly 0 and drop them: with BFAST their jj would pass N/R, their
+    // n pass N
+    auto split = [&](int q, int& b, int& jj) {
+        if (q >= nbf) q = 0;
+        if (BFAST) {
+            jj = quo(q, qd);
+            b = q - jj * B;
+        } else {
+            b = quo(q, qd);
+            jj = q - b * NR;
+        }
+    };
 #pragma unroll
     for (int t = 0; t < MB; t++) {
         const int q = threadIdx.x + t * FFT_THREADS;
-        const int b = quo(q, qNR), jj = q - b * NR;
-        const bool in = q < nbf && b < nrows;
+        int b, jj;
+        split(q, b, jj);
 #pragma unroll
-        for (int r = 0; r < R; r++) v[t][r] = in ? src[(long)b * N + jj + r * NR] : make_double2(0.0, 0.0);
+        for (int r = 0; r < R; r++) v[t][r] = elem(q < nbf, b, jj + r * NR);
     }
     if constexpr (R == 5) __syncthreads();  // the radix-5 DFT reads the twiddle table
 #pragma unroll
     for (int t = 0; t < MB; t++) {
         const int q = threadIdx.x + t * FFT_THREADS;
         if (q < nbf) {
-            const int b = quo(q, qNR), jj = q - b * NR;
+            int b, jj;
+            split(q, b, jj);
             dft<R>(v[t], tw, N, SIGN);
 #pragma unroll
             for (int r = 0; r < R; r++) a[b * ld + jj * R + r] = v[t][r];
@@ -250,11 +270,11 @@ __device__ __forceinline__ void fftz_first_pass(const double2* src, long nrows, 
     }
 }
 
-// Last Stockham pass of radix R (span N/R) from LDS straight to global
-// memory: outputs k + r N/R, consecutive lanes consecutive addresses.
-template <int R, int SIGN>
-__device__ __forceinline__ void fftz_last_pass(double2* dst, long nrows, const double2* a, const double2* tw, int N,
-                                               int B, int ld)
+// Last Stockham pass of radix R (span N/R) from LDS to registers: output n
+// = k + r N/R of transform b goes to store(b, n, value); consecutive lanes
+// take consecutive k.
+template <int R, int SIGN, typename S>
+__device__ __forceinline__ void fft_last_pass(const double2* a, const double2* tw, int N, int B, int ld, const S& store)
 {
     constexpr int MB = (FFT_MAXNB / R + FFT_THREADS - 1) / FFT_THREADS;
     const int NR = N / R, nbf = NR * B;
@@ -273,7 +293,7 @@ __device__ __forceinline__ void fftz_last_pass(double2* dst, long nrows, const d
     for (int t = 0; t < MB; t++) {
         const int q = threadIdx.x + t * FFT_THREADS;
         const int b = quo(q, qNR), k = q - b * NR;
-        if (q < nbf && b < nrows) {
+        if (q < nbf) {
 #pragma unroll
             for (int r = 1; r < R; r++) {
                 const double2 w = tw[r * k];
@@ -281,9 +301,29 @@ __device__ __forceinline__ void fftz_last_pass(double2* dst, long nrows, const d
             }
             dft<R>(v[t], tw, N, SIGN);
 #pragma unroll
-            for (int r = 0; r < R; r++) dst[(long)b * N + k + r * NR] = v[t][r];
+            for (int r = 0; r < R; r++) store(b, k + r * NR, v[t][r]);
         }
     }
+}
+
+// the passes above for the plan's radix R (block-uniform)
+template <int SIGN, bool BFAST, typename F>
+__device__ __forceinline__ void fft_first_pass_r(int R, double2* a, const double2* tw, int N, int B, int ld,
+                                                 const F& elem)
+{
+    if (R == 4) fft_first_pass<4, SIGN, BFAST>(a, tw, N, B, ld, elem);
+    else if (R == 2) fft_first_pass<2, SIGN, BFAST>(a, tw, N, B, ld, elem);
+    else if (R == 3) fft_first_pass<3, SIGN, BFAST>(a, tw, N, B, ld, elem);
+    else fft_first_pass<5, SIGN, BFAST>(a, tw, N, B, ld, elem);
+}
+template <int SIGN, typename S>
+__device__ __forceinline__ void fft_last_pass_r(int R, const double2* a, const double2* tw, int N, int B, int ld,
+                                                const S& store)
+{
+    if (R == 4) fft_last_pass<4, SIGN>(a, tw, N, B, ld, store);
+    else if (R == 2) fft_last_pass<2, SIGN>(a, tw, N, B, ld, store);
+    else if (R == 3) fft_last_pass<3, SIGN>(a, tw, N, B, ld, store);
+    else fft_last_pass<5, SIGN>(a, tw, N, B, ld, store);
 }
 
 template <int SIGN>
@@ -303,26 +343,14 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftz(Dims d, PoissonBufs pb)
         // the first and the last pass run between registers and global
         // memory, the others in LDS: two LDS sweeps and four barriers fewer
         // than loading into LDS first and storing from it last
-        const int r0 = pl.r[0];
-        if (r0 == 4) fftz_first_pass<4, SIGN>(ln, nrows, a, tw, N, B, ld);
-        else if (r0 == 2) fftz_first_pass<2, SIGN>(ln, nrows, a, tw, N, B, ld);
-        else if (r0 == 3) fftz_first_pass<3, SIGN>(ln, nrows, a, tw, N, B, ld);
-        else fftz_first_pass<5, SIGN>(ln, nrows, a, tw, N, B, ld);
+        fft_first_pass_r<SIGN, false>(pl.r[0], a, tw, N, B, ld, [&](bool ok, int b, int n) {
+            return ok && b < nrows ? ln[(long)b * N + n] : make_double2(0.0, 0.0);
+        });
         __syncthreads();
-        int Ns = r0;
-        for (int p = 1; p < pl.nr - 1; p++) {
-            const int R = pl.r[p];
-            if (R == 4) fft_pass<4>(a, tw, N, B, ld, Ns, SIGN);
-            else if (R == 2) fft_pass<2>(a, tw, N, B, ld, Ns, SIGN);
-            else if (R == 3) fft_pass<3>(a, tw, N, B, ld, Ns, SIGN);
-            else fft_pass<5>(a, tw, N, B, ld, Ns, SIGN);
-            Ns *= R;
-        }
-        const int rl = pl.r[pl.nr - 1];
-        if (rl == 4) fftz_last_pass<4, SIGN>(ln, nrows, a, tw, N, B, ld);
-        else if (rl == 2) fftz_last_pass<2, SIGN>(ln, nrows, a, tw, N, B, ld);
-        else if (rl == 3) fftz_last_pass<3, SIGN>(ln, nrows, a, tw, N, B, ld);
-        else fftz_last_pass<5, SIGN>(ln, nrows, a, tw, N, B, ld);
+        block_fft(a, tw, pl, B, ld, SIGN, 1, pl.nr - 1);
+        fft_last_pass_r<SIGN>(pl.r[pl.nr - 1], a, tw, N, B, ld, [&](int b, int n, double2 val) {
+            if (b < nrows) ln[(long)b * N + n] = val;
+        });
         return;
     }
     // a single pass (N <= 5): through LDS
@@ -617,51 +645,39 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_inv(Dims d, PoissonBufs pb
     const int pair0 = blockIdx.x * B;
     const int half = N / 2;
     load_twiddles(tw, pb.fx);
-    // consecutive lanes read consecutive kz of one kx line; all loads are
-    // issued before the first LDS store
-    constexpr int MQ = (FFT_MAXNB / 2 + FFT_THREADS - 1) / FFT_THREADS + 1;
-    double2 va[MQ], vb[MQ];
-#pragma unroll
-    for (int t = 0; t < MQ; t++) {
-        const int q = threadIdx.x + t * FFT_THREADS;
-        const int n = quo(q, qB), b = q - n * B;
-        const int pr = pair0 + b;
-        va[t] = make_double2(0.0, 0.0);
-        vb[t] = va[t];
-        if (q < d.nkx * B && pr < npairs) {
-            int jr, kz;
-            pair_rows(d, pr, jr, kz);
-            const double2* in = pb.ph + ((long)jr * d.nkx + n) * d.nzp + kz;
-            va[t] = in[0];
-            vb[t] = in[1];
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < MQ; t++) {
-        const int q = threadIdx.x + t * FFT_THREADS;
-        if (q >= d.nkx * B) break;
-        const int n = quo(q, qB), b = q - n * B;
-        double2 xa = va[t], xb = vb[t];
-        if (n == 0 || n == half) { xa.y = 0.0; xb.y = 0.0; }
-        a[b * ld + n] = make_double2(xa.x - xb.y, xa.y + xb.x);
-        if (n > 0 && n < half) {
-            const double2 ca = cconj(xa), cb = cconj(xb);
-            a[b * ld + N - n] = make_double2(ca.x - cb.y, ca.y + cb.x);
-        }
-    }
-    __syncthreads();
-    block_fft(a, tw, pb.fx, B, ld, +1);
     // the unnormalised inverse transforms' factor, as a reciprocal (within an
     // ulp of the division)
     const double rscale = 1.0 / (double)((long)d.nxp * d.nzp);
-    for (int q = threadIdx.x; q < N * B; q += blockDim.x) {
-        const int b = quo(q, qN), n = q - b * N;
+    // element n of transform b (two packed z rows): mode n of both rows, or
+    // the conjugate of mode N-n above the Nyquist mode; imaginary parts of
+    // modes 0 and N/2 dropped.  Consecutive lanes read consecutive kz of one
+    // kx line (BFAST); the loads unconditional, from a clamped pair.
+    auto elem = [&](bool ok, int b, int n) {
+        const int pr = min(pair0 + b, npairs - 1);
+        int jr, kz;
+        pair_rows(d, pr, jr, kz);
+        const int mode = n <= half ? n : N - n;
+        const double2* in = pb.ph + ((long)jr * d.nkx + mode) * d.nzp + kz;
+        double2 xa = in[0], xb = in[1];
+        if (n == 0 || n == half) {
+            xa.y = 0.0;
+            xb.y = 0.0;
+        }
+        if (n > half) {
+            xa = cconj(xa);
+            xb = cconj(xb);
+        }
+        const double2 val = make_double2(xa.x - xb.y, xa.y + xb.x);
+        return ok && pair0 + b < npairs ? val : make_double2(0.0, 0.0);
+    };
+    // value n of transform b to P(n+2) of both rows, with the periodic images
+    auto store = [&](int b, int n, double2 val) {
         const int pr = pair0 + b;
-        if (pr >= npairs) continue;
+        if (pr >= npairs) return;
         int jr, kz;
         pair_rows(d, pr, jr, kz);
         const int j = jr + 2, kA = kz + 2, i = n + 2;
-        const double pa = a[b * ld + n].x * rscale, pbv = a[b * ld + n].y * rscale;
+        const double pa = val.x * rscale, pbv = val.y * rscale;
         pb.p[at(d, i, j, kA)] = pa;
         pb.p[at(d, i, j, kA + 1)] = pbv;
         if (n == 0) {
@@ -672,6 +688,24 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_inv(Dims d, PoissonBufs pb
             pb.p[at(d, i, j, d.nzg - 1)] = pa;
             if (n == 0) pb.p[at(d, d.nxg - 1, j, d.nzg - 1)] = pa;
         }
+    };
+    if (pb.fx.nr >= 2) {
+        // first and last pass between registers and global memory
+        fft_first_pass_r<+1, true>(pb.fx.r[0], a, tw, N, B, ld, elem);
+        __syncthreads();
+        block_fft(a, tw, pb.fx, B, ld, +1, 1, pb.fx.nr - 1);
+        fft_last_pass_r<+1>(pb.fx.r[pb.fx.nr - 1], a, tw, N, B, ld, store);
+        return;
+    }
+    for (int q = threadIdx.x; q < N * B; q += blockDim.x) {
+        const int b = quo(q, qN), n = q - b * N;
+        a[b * ld + n] = elem(true, b, n);
+    }
+    __syncthreads();
+    block_fft(a, tw, pb.fx, B, ld, +1);
+    for (int q = threadIdx.x; q < N * B; q += blockDim.x) {
+        const int b = quo(q, qN), n = q - b * N;
+        store(b, n, a[b * ld + n]);
     }
 }
 
