@@ -14,6 +14,19 @@ stores (factor 1).
 
 The traffic table's rows `| kernel | launches | fetch KiB | write KiB | bytes |`
 are what `load_traffic` reads back.
+
+    python -m channel_rl_amd.rocprof_summary sq PASS1/run_counter_collection.csv \
+        PASS2/run_counter_collection.csv
+
+`sq` summarises the SQ counter passes listed in SQ_PASSES (medians per
+dispatch; each counter is a whole-device sum per dispatch).  Per wave:
+VALU and LDS instructions; as fractions of the waves' lifetime
+(SQ_WAVE_CYCLES, in quad-cycles like the wait and active counters): time
+waiting for anything, waiting to issue, and issuing.  Per unit: the SIMDs'
+VALU busy fraction, SQ_ACTIVE_INST_VALU x 4 / (SIMDS x cycles), and the
+LDS's, SQ_LDS_IDX_ACTIVE / (CUS x cycles), with cycles = GRBM_GUI_ACTIVE /
+XCDS (the counter sums the XCDs' clocks); bank-conflict cycles as a
+fraction of SQ_LDS_IDX_ACTIVE.
 """
 import csv
 import collections
@@ -91,6 +104,47 @@ def traffic_table(fetch_path, write_path):
     return "\n".join(out)
 
 
+# the two counter passes (at most 8 SQ and 2 GRBM counters each)
+SQ_PASSES = (
+    "SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_ACTIVE_INST_VALU SQ_ACTIVE_INST_LDS SQ_WAIT_INST_LDS "
+    "SQ_LDS_BANK_CONFLICT SQ_LDS_IDX_ACTIVE SQ_WAIT_ANY GRBM_GUI_ACTIVE",
+    "SQ_WAIT_INST_ANY SQ_ACTIVE_INST_ANY SQ_LDS_DATA_FIFO_FULL SQ_LDS_ADDR_CONFLICT SQ_INSTS_VALU "
+    "SQ_INSTS_LDS SQ_LDS_UNALIGNED_STALL SQ_INSTS_SALU GRBM_GUI_ACTIVE",
+)
+XCDS, CUS, SIMDS = 8, 256, 1024  # MI355X
+
+
+def sq_table(paths):
+    vals = collections.defaultdict(lambda: collections.defaultdict(list))
+    grid = {}
+    for p in paths:
+        for r in csv.DictReader(open(p)):
+            k = short_name(r["Kernel_Name"])
+            vals[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
+            grid[k] = int(r["Grid_Size"])
+    need = ("SQ_WAVE_CYCLES", "SQ_WAIT_ANY", "SQ_WAIT_INST_ANY", "SQ_ACTIVE_INST_ANY", "SQ_ACTIVE_INST_VALU",
+            "SQ_LDS_IDX_ACTIVE", "SQ_LDS_BANK_CONFLICT", "SQ_INSTS_VALU", "SQ_INSTS_LDS", "GRBM_GUI_ACTIVE")
+    out = ["| kernel | waves | VALU inst/wave | LDS inst/wave | waiting | waiting to issue | issuing | "
+           "VALU busy | LDS busy | LDS bank conflicts |",
+           "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    rows = []
+    for k, d in vals.items():
+        if not k.startswith("k_") or any(c not in d for c in need):
+            continue
+        m = {c: statistics.median(v) for c, v in d.items()}
+        waves = grid[k] / 64.0
+        wc = m["SQ_WAVE_CYCLES"]
+        cyc = m["GRBM_GUI_ACTIVE"] / XCDS
+        valu = m["SQ_ACTIVE_INST_VALU"] * 4.0 / (SIMDS * cyc)
+        lds = m["SQ_LDS_IDX_ACTIVE"] / (CUS * cyc)
+        bank = m["SQ_LDS_BANK_CONFLICT"] / m["SQ_LDS_IDX_ACTIVE"] if m["SQ_LDS_IDX_ACTIVE"] else 0.0
+        rows.append((cyc, f"| {k} | {waves:.0f} | {m['SQ_INSTS_VALU'] / waves:.0f} | {m['SQ_INSTS_LDS'] / waves:.0f} | "
+                          f"{m['SQ_WAIT_ANY'] / wc:.2f} | {m['SQ_WAIT_INST_ANY'] / wc:.2f} | "
+                          f"{m['SQ_ACTIVE_INST_ANY'] / wc:.2f} | {valu:.2f} | {lds:.2f} | {bank:.2f} |"))
+    out += [r for _, r in sorted(rows, key=lambda t: -t[0])]
+    return "\n".join(out)
+
+
 def load_traffic(path):
     """Scaled HBM bytes per launch by kernel base name from a traffic table."""
     res = {}
@@ -111,6 +165,8 @@ if __name__ == "__main__":
         print(stats_table(sys.argv[2]))
     elif len(sys.argv) >= 4 and sys.argv[1] == "traffic":
         print(traffic_table(sys.argv[2], sys.argv[3]))
+    elif len(sys.argv) >= 3 and sys.argv[1] == "sq":
+        print(sq_table(sys.argv[2:]))
     else:
         print(__doc__)
         sys.exit(2)
