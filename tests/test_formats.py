@@ -90,3 +90,16 @@ def test_committed_traffic_table_parses():
     p = os.path.join(os.path.dirname(__file__), "..", "profiles", "hbm_traffic_channel_step_192x97x192_fp64.md")
     t = rs.load_traffic(p)
     assert t["k_dsm_fused"] > 0 and t["k_rhs_rk"] > 0
+
+
+def test_rocprof_summary_sq_fractions(tmp_path):
+    # one kernel of 2 waves over 10 cycles per XCD: every derived column by hand
+    from channel_rl_amd import rocprof_summary as rs
+    hdr = "Kernel_Name,Grid_Size,Counter_Name,Counter_Value\n"
+    vals = {"SQ_WAVE_CYCLES": 100, "SQ_WAIT_ANY": 40, "SQ_WAIT_INST_ANY": 20, "SQ_ACTIVE_INST_ANY": 40,
+            "SQ_ACTIVE_INST_VALU": 1280, "SQ_LDS_IDX_ACTIVE": 1280, "SQ_LDS_BANK_CONFLICT": 64,
+            "SQ_INSTS_VALU": 600, "SQ_INSTS_LDS": 30, "GRBM_GUI_ACTIVE": 80}
+    p = tmp_path / "sq.csv"
+    p.write_text(hdr + "".join(f'"void chan::k_dsm_fused<double, 30, 14>(x)",128,{c},{v}\n' for c, v in vals.items()))
+    rows = [r for r in rs.sq_table([str(p)]).splitlines() if r.startswith("| k_")]
+    assert rows == ["| k_dsm_fused<double, 30, 14> | 2 | 300 | 15 | 0.40 | 0.20 | 0.40 | 0.50 | 0.50 | 0.05 |"]
