@@ -422,7 +422,13 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_setup(Dims d, PoissonBu
 // n-1..hn.  The two meet in the 2x2 system of rows hn-1 and hn:
 //   x(hn-1) + c'_top x(hn) = y_top,  x(hn) + c'_bot x(hn-1) = y_bot,
 // which both lanes of a pair solve alike; each half then substitutes back
-// towards its end.  Pivots stay in LDS, rows in registers.
+// towards its end.  Pivots stay in LDS, rows in registers.  The unrolled
+// steps run without branches: a lane's rows are reached by a pointer that
+// moves by one row per step, the matrix entries sit in LDS in each half's
+// step order, and the steps past a half's end (odd n) keep the chain's
+// values by selection and re-store the half's last row with its own value.
+// (An empty asm on each new value keeps the compiler from moving its
+// computation into a branch of the selection.)
 template <int H>
 __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_tw(Dims d, PoissonBufs pb)
 {
@@ -436,50 +442,65 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_tw(Dims d, PoissonBufs 
     const int comp = (int)((g >> 1) & 1);
     const int kx = mode / d.nzp, kz = mode - kx * d.nzp;
     const double lx = pb.kxx[kx], lz = pb.kzz[kz];
-    __shared__ double sd[2 * H], sl[2 * H], su[2 * H];
+    // step u of the top half: row u, couplings (A, C) = (l_u, u_u); of the
+    // bottom half (slot H+u): row n-1-u, (A, C) = (u_r, l_r); an identity
+    // row past a half's end
+    __shared__ double sD[2 * H], sA[2 * H], sC[2 * H];
     __shared__ double piv[H][TRI_THREADS];
-    for (int r = threadIdx.x; r < 2 * H; r += TRI_THREADS) {
-        sd[r] = r < n ? pb.dd[r] : 1.0;
-        sl[r] = r < n && r > 0 ? pb.dl[r - 1] : 0.0;
-        su[r] = r < n - 1 ? pb.du[r] : 0.0;
+    for (int q = threadIdx.x; q < 2 * H; q += TRI_THREADS) {
+        const bool b = q >= H;
+        const int u = b ? q - H : q;
+        const int r = b ? n - 1 - u : u;
+        const bool in = u < (b ? n - hn : hn);
+        const double l = in && r > 0 ? pb.dl[r - 1] : 0.0;
+        const double up = in && r < n - 1 ? pb.du[r] : 0.0;
+        sD[q] = in ? pb.dd[r] : 1.0;
+        sA[q] = b ? up : l;
+        sC[q] = b ? l : up;
     }
     __syncthreads();
     if (g >= 4L * nmodes) return;
+    const double* D = sD + (bot ? H : 0);
+    const double* A = sA + (bot ? H : 0);
+    const double* C = sC + (bot ? H : 0);
     const long plane = 2L * nmodes;
-    double* ph = reinterpret_cast<double*>(pb.ph) + 2L * mode + comp;
-    auto row_of = [&](int u) { return bot ? n - 1 - u : u; };
+    const long step = bot ? -plane : plane;  // one row further into the half
+    double* p0 = reinterpret_cast<double*>(pb.ph) + 2L * mode + comp + (bot ? (long)(n - 1) * plane : 0L);
+    double* plast = p0 + (long)(hl - 1) * step;
     // the rows of this half, every load issued first (steps past the half
     // re-read its last row and are not used)
     double keep[H];
+    {
+        const double* pr = p0;
 #pragma unroll
-    for (int u = 0; u < H; u++) keep[u] = ph[(long)row_of(min(u, hl - 1)) * plane];
+        for (int u = 0; u < H; u++) {
+            keep[u] = *pr;
+            pr = u + 1 < hl ? pr + step : pr;
+        }
+    }
     // keeps the compiler from sinking the loads below the chain
     asm volatile("" ::: "memory");
     // pivot chain; row 0's diagonal of the 00 mode scaled by 3/2
     double c = 0.0;
-    const bool mode00 = kx == 0 && kz == 0;
+    const bool scale0 = !bot && kx == 0 && kz == 0;
 #pragma unroll
     for (int u = 0; u < H; u++) {
-        if (u < hl) {
-            const int r = row_of(u);
-            const double A = bot ? su[r] : sl[r], C = bot ? sl[r] : su[r];
-            double den = ((sd[r] + lx) + lz) - A * c;
-            if (u == 0 && !bot && mode00) den = 3.0 / 2.0 * den;
-            const double inv = 1.0 / den;
-            piv[u][threadIdx.x] = inv;
-            c = C * inv;
-        }
+        double den = ((D[u] + lx) + lz) - A[u] * c;
+        if (u == 0) den = scale0 ? 3.0 / 2.0 * den : den;
+        const double inv = 1.0 / den;
+        piv[u][threadIdx.x] = inv;
+        double cn = C[u] * inv;
+        asm("" : "+v"(cn));
+        c = u < hl ? cn : c;
     }
     asm volatile("" ::: "memory");
     double y = 0.0;
 #pragma unroll
     for (int u = 0; u < H; u++) {
-        if (u < hl) {
-            const int r = row_of(u);
-            const double A = bot ? su[r] : sl[r];
-            y = (keep[u] - A * y) * piv[u][threadIdx.x];
-            keep[u] = y;
-        }
+        double yn = (keep[u] - A[u] * y) * piv[u][threadIdx.x];
+        asm("" : "+v"(yn));
+        y = u < hl ? yn : y;
+        keep[u] = y;
     }
     // the two middle rows from the pair's last values (lanes g and g^1)
     const double co = __shfl_xor(c, 1, 64), yo = __shfl_xor(y, 1, 64);
@@ -487,16 +508,18 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_tw(Dims d, PoissonBufs 
     const double xt = (yt - ct * yb) / (1.0 - ct * cb);  // x(hn-1)
     const double xb = yb - cb * xt;                       // x(hn)
     double xn = bot ? xb : xt;
-    ph[(long)row_of(hl - 1) * plane] = xn;
-    // back towards the end of the half: x = y - c' x(previous)
+    *plast = xn;
+    // back towards the end of the half: x = y - c' x(previous); pw moves
+    // with the valid steps only, from row hl-2 of the half
+    double* pw = p0 + (long)max(hl - 2, 0) * step;
 #pragma unroll
     for (int u = H - 2; u >= 0; u--) {
-        if (u < hl - 1) {
-            const int r = row_of(u);
-            const double C = bot ? sl[r] : su[r];
-            xn = keep[u] - C * piv[u][threadIdx.x] * xn;
-            ph[(long)r * plane] = xn;
-        }
+        double xu = keep[u] - C[u] * piv[u][threadIdx.x] * xn;
+        asm("" : "+v"(xu));
+        const bool in = u < hl - 1;
+        xn = in ? xu : xn;
+        *(in ? pw : plast) = xn;
+        if (u > 0) pw = in ? pw - step : pw;
     }
 }
 
