@@ -156,22 +156,41 @@ __device__ __forceinline__ void load_twiddles(double2* tw, const FftPlan& pl)
     for (int q = threadIdx.x; q < pl.n; q += blockDim.x) tw[q] = pl.tw[q];
 }
 
-// divergence of the staggered velocity at centre (i,j,k), code/projection.f90:45-53,
-// with the spacings' reciprocals (within an ulp of the reference's divisions)
-template <typename T>
-__device__ __forceinline__ double div_at(const T* U, const T* V, const T* W, const Metric<double>& m, const Dims& d,
-                                         int i, int j, int k)
-{
-    return ((double)U[at(d, i, j, k)] - (double)U[at(d, i - 1, j, k)]) * m.rdx[i] +
-           ((double)V[at(d, i, j, k)] - (double)V[at(d, i, j - 1, k)]) * m.rdy[j] +
-           ((double)W[at(d, i, j, k)] - (double)W[at(d, i, j, k - 1)]) * m.rdz[k];
-}
-
 // pair pr -> (row jr = j-2, first z index kA-2) of the two packed z-rows
 __device__ __forceinline__ void pair_rows(const Dims& d, int pr, int& jr, int& kz)
 {
     jr = pr / (d.nzp / 2);
     kz = 2 * (pr % (d.nzp / 2));
+}
+
+// The block's pairs of packed z rows, per pair b < B <= 16: the box index
+// of x centre 1 on the first row (-1 past the last pair), the spectrum
+// offset of its (jr, kx = 0, kz) and the y and z reciprocal spacings, taken
+// once per block instead of per element (a pair index takes an integer
+// division, a box index 64-bit products).  Filled by the first B threads;
+// the caller synchronises.
+struct PairTab {
+    long box[16], spec[16];
+    double rdy[16], rdz0[16], rdz1[16];
+    bool zfirst[16];  // the pair starts at the first unique z centre
+};
+__device__ __forceinline__ void fill_pairs(PairTab& t, const Dims& d, const Metric<double>* m, int pair0, int B,
+                                           int npairs)
+{
+    const int b = threadIdx.x;
+    if (b >= B) return;
+    const int pr = min(pair0 + b, npairs - 1);
+    int jr, kz;
+    pair_rows(d, pr, jr, kz);
+    const int j = jr + 2, kA = kz + 2;
+    t.box[b] = pair0 + b < npairs ? at(d, 1, j, kA) : -1L;
+    t.spec[b] = (long)jr * d.nkx * d.nzp + kz;
+    t.zfirst[b] = kA == 2;
+    if (m) {
+        t.rdy[b] = m->rdy[j];
+        t.rdz0[b] = m->rdz[kA];
+        t.rdz1[b] = m->rdz[kA + 1];
+    }
 }
 
 template <typename T, bool FROM_FIELD>
@@ -187,19 +206,29 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_fwd(const T* U, const T* V
     // consecutive blocks share V rows (row j-1 of one is row j of a block
     // nzp/2B before it): XCD-contiguous order keeps both in one L2
     const int pair0 = xcd_linear() * B;
+    __shared__ PairTab pt;
+    fill_pairs(pt, d, FROM_FIELD ? nullptr : &m, pair0, B, npairs);
     load_twiddles(tw, pb.fx);
+    __syncthreads();
+    const long sx = d.sx, sxy = d.sxy;
     for (int q = threadIdx.x; q < N * B; q += blockDim.x) {
         const int b = quo(q, qN), n = q - b * N;
-        const int pr = pair0 + b;
+        const long base = pt.box[b];
         double2 val = make_double2(0.0, 0.0);
-        if (pr < npairs) {
-            int jr, kz;
-            pair_rows(d, pr, jr, kz);
-            const int j = jr + 2, kA = kz + 2, i = n + 2;
-            if (FROM_FIELD)
-                val = make_double2(rhs[at(d, i, j, kA)], rhs[at(d, i, j, kA + 1)]);
-            else
-                val = make_double2(div_at<T>(U, V, W, m, d, i, j, kA), div_at<T>(U, V, W, m, d, i, j, kA + 1));
+        if (base >= 0) {
+            // the divergence of the staggered velocity at centres (n+2, j, kA)
+            // and (n+2, j, kA+1), code/projection.f90:45-53, with the
+            // spacings' reciprocals (within an ulp of the reference's divisions)
+            const long p = base + n + 1, p1 = p + sxy;
+            if (FROM_FIELD) {
+                val = make_double2(rhs[p], rhs[p1]);
+            } else {
+                const double rx = m.rdx[n + 2], ry = pt.rdy[b];
+                val.x = (((double)U[p] - (double)U[p - 1]) * rx + ((double)V[p] - (double)V[p - sx]) * ry) +
+                        ((double)W[p] - (double)W[p - sxy]) * pt.rdz0[b];
+                val.y = (((double)U[p1] - (double)U[p1 - 1]) * rx + ((double)V[p1] - (double)V[p1 - sx]) * ry) +
+                        ((double)W[p1] - (double)W[p]) * pt.rdz1[b];
+            }
         }
         a[b * ld + n] = val;
     }
@@ -208,14 +237,11 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_fwd(const T* U, const T* V
     // split Z = F[a + i b] into F[a], F[b]; consecutive lanes write consecutive kz
     for (int q = threadIdx.x; q < d.nkx * B; q += blockDim.x) {
         const int kx = quo(q, qB), b = q - kx * B;
-        const int pr = pair0 + b;
-        if (pr >= npairs) continue;
-        int jr, kz;
-        pair_rows(d, pr, jr, kz);
+        if (pt.box[b] < 0) continue;
         const double2 Z = a[b * ld + kx];
         const double2 Zc = cconj(a[b * ld + ((N - kx) % N)]);
         const double2 s = cadd(Z, Zc), df = csub(Z, Zc);
-        double2* out = pb.ph + ((long)jr * d.nkx + kx) * d.nzp + kz;
+        double2* out = pb.ph + pt.spec[b] + (long)kx * d.nzp;
         out[0] = make_double2(0.5 * s.x, 0.5 * s.y);
         out[1] = make_double2(0.5 * df.y, -0.5 * df.x);
     }
@@ -667,7 +693,10 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_inv(Dims d, PoissonBufs pb
     const int npairs = (d.ny - 1) * (d.nzp / 2);
     const int pair0 = blockIdx.x * B;
     const int half = N / 2;
+    __shared__ PairTab pt;
+    fill_pairs(pt, d, nullptr, pair0, B, npairs);
     load_twiddles(tw, pb.fx);
+    __syncthreads();
     // the unnormalised inverse transforms' factor, as a reciprocal (within an
     // ulp of the division)
     const double rscale = 1.0 / (double)((long)d.nxp * d.nzp);
@@ -676,11 +705,8 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_inv(Dims d, PoissonBufs pb
     // modes 0 and N/2 dropped.  Consecutive lanes read consecutive kz of one
     // kx line (BFAST); the loads unconditional, from a clamped pair.
     auto elem = [&](bool ok, int b, int n) {
-        const int pr = min(pair0 + b, npairs - 1);
-        int jr, kz;
-        pair_rows(d, pr, jr, kz);
         const int mode = n <= half ? n : N - n;
-        const double2* in = pb.ph + ((long)jr * d.nkx + mode) * d.nzp + kz;
+        const double2* in = pb.ph + pt.spec[b] + (long)mode * d.nzp;
         double2 xa = in[0], xb = in[1];
         if (n == 0 || n == half) {
             xa.y = 0.0;
@@ -691,25 +717,25 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_inv(Dims d, PoissonBufs pb
             xb = cconj(xb);
         }
         const double2 val = make_double2(xa.x - xb.y, xa.y + xb.x);
-        return ok && pair0 + b < npairs ? val : make_double2(0.0, 0.0);
+        return ok && pt.box[b] >= 0 ? val : make_double2(0.0, 0.0);
     };
     // value n of transform b to P(n+2) of both rows, with the periodic images
+    // P(nxg-1) = P(2) and, for the first z row, P(.., nzg-1) = P(.., 2)
+    const long sxy = d.sxy, zimg = (long)(d.nzg - 3) * d.sxy;
     auto store = [&](int b, int n, double2 val) {
-        const int pr = pair0 + b;
-        if (pr >= npairs) return;
-        int jr, kz;
-        pair_rows(d, pr, jr, kz);
-        const int j = jr + 2, kA = kz + 2, i = n + 2;
+        const long base = pt.box[b];
+        if (base < 0) return;
+        const long p = base + n + 1, pi = base + d.nxg - 2;  // x centres n+2 and nxg-1
         const double pa = val.x * rscale, pbv = val.y * rscale;
-        pb.p[at(d, i, j, kA)] = pa;
-        pb.p[at(d, i, j, kA + 1)] = pbv;
+        pb.p[p] = pa;
+        pb.p[p + sxy] = pbv;
         if (n == 0) {
-            pb.p[at(d, d.nxg - 1, j, kA)] = pa;
-            pb.p[at(d, d.nxg - 1, j, kA + 1)] = pbv;
+            pb.p[pi] = pa;
+            pb.p[pi + sxy] = pbv;
         }
-        if (kA == 2) {
-            pb.p[at(d, i, j, d.nzg - 1)] = pa;
-            if (n == 0) pb.p[at(d, d.nxg - 1, j, d.nzg - 1)] = pa;
+        if (pt.zfirst[b]) {
+            pb.p[p + zimg] = pa;
+            if (n == 0) pb.p[pi + zimg] = pa;
         }
     };
     if (pb.fx.nr >= 2) {
