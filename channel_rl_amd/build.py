@@ -49,20 +49,31 @@ def build_hip(jobs=None):
 
 
 def build_fortran():
-    """iso_c_binding host layer: module + driver program (amdflang)."""
+    """iso_c_binding host layer: the drop-in modules (chanles_gpu, the RL
+    coupling smarties_gpu), the driver program, and the test program that
+    drives every drop-in subroutine (against a recording stand-in of the
+    smarties entry points)."""
     fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
     if not os.path.exists(fc):
         print("amdflang not found; Fortran host layer not built", flush=True)
         return None
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    mod = os.path.join(FSRC, "chanles_mod.f90")
-    drv = os.path.join(FSRC, "channel_driver.f90")
+    obj = lambda name: os.path.join(objdir, name)  # noqa: E731
+    mods = ["-module-dir", objdir, "-I", objdir]
+    _run([fc, "-O2", "-c", os.path.join(FSRC, "chanles_mod.f90"), "-o", obj("chanles_mod.o"), *mods])
+    _run([fc, "-O2", "-c", os.path.join(FSRC, "smarties_gpu.f90"), "-o", obj("smarties_gpu.o"), *mods])
+    _run([fc, "-O2", "-c", os.path.join(FSRC, "channel_driver.f90"), "-o", obj("channel_driver.o"), *mods])
+    link = ["-L" + HERE, "-lchanles", "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN"]
     exe = os.path.join(HERE, "channel_driver")
-    _run([fc, "-O2", "-c", mod, "-o", os.path.join(objdir, "chanles_mod.o"), "-module-dir", objdir])
-    _run([fc, "-O2", "-c", drv, "-o", os.path.join(objdir, "channel_driver.o"), "-module-dir", objdir])
-    _run([fc, "-o", exe, os.path.join(objdir, "channel_driver.o"), os.path.join(objdir, "chanles_mod.o"),
-          "-L" + HERE, "-lchanles", "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN"])
+    _run([fc, "-o", exe, obj("channel_driver.o"), obj("chanles_mod.o"), *link])
+    tdir = os.path.join(ROOT, "tests", "fortran")
+    if os.path.exists(os.path.join(tdir, "check_wrappers.f90")):
+        cc = shutil.which("gcc") or "gcc"
+        _run([cc, "-O2", "-c", os.path.join(tdir, "smarties_double.c"), "-o", obj("smarties_double.o")])
+        _run([fc, "-O2", "-c", os.path.join(tdir, "check_wrappers.f90"), "-o", obj("check_wrappers.o"), *mods])
+        _run([fc, "-o", os.path.join(tdir, "check_wrappers"), obj("check_wrappers.o"), obj("smarties_gpu.o"),
+              obj("chanles_mod.o"), obj("smarties_double.o"), *link])
     return exe
 
 

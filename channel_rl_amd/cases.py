@@ -94,18 +94,27 @@ class Case:
     def field_shapes(self):
         return ((self.nx, self.nyg, self.nzg), (self.nxg, self.ny, self.nzg), (self.nxg, self.nyg, self.nz))
 
-    def initial_field(self, profile, ub, amp, seed):
-        """Mean profile (callable of wall distance) on U plus uniform noise of
-        +-amp*ub on every component; arrays in the reference's shapes."""
+    def initial_field(self, profile, ub, amp, seed, local=False):
+        """Mean profile (callable of wall distance) on U plus uniform noise on
+        every component; arrays in the reference's shapes.  The noise is
+        +-amp*ub everywhere, or with local=True +-amp times the mean profile
+        at each value's own wall distance (a fluctuation of amp relative to
+        the local mean, vanishing at the walls like the mean itself)."""
         yg = centres_with_ghosts(self.y)
         su, sv, sw = self.field_shapes()
         dist = np.minimum(yg, 2.0 - yg)
+        mean = profile(dist)
         U = np.zeros(su, order="F")
-        U[:, :, :] = profile(dist)[None, :, None]
-        U += amp * ub * uniform_noise(seed, 1, su)
-        V = amp * ub * uniform_noise(seed, 2, sv)
-        W = amp * ub * uniform_noise(seed, 3, sw)
-        return U, V, W
+        U[:, :, :] = mean[None, :, None]
+        if local:
+            dv = np.minimum(self.y, 2.0 - self.y)
+            au, av, aw = amp * mean[None, :, None], amp * profile(dv)[None, :, None], amp * mean[None, :, None]
+        else:
+            au = av = aw = amp * ub
+        U += au * uniform_noise(seed, 1, su)
+        V = av * uniform_noise(seed, 2, sv)
+        W = aw * uniform_noise(seed, 3, sw)
+        return np.asfortranarray(U), np.asfortranarray(V), np.asfortranarray(W)
 
 
 def config(name, seed=1234):
@@ -115,7 +124,8 @@ def config(name, seed=1234):
                  laminar parabola + 5% noise, DSM, equilibrium wall model at the
                  first off-wall cell, CFL 0.5, constant mass flow.
     wmles      : 192x97x192 (faces 194 97 194), Re_tau 2000 in the reference's
-                 units (nu, dPdx of input_parameters_2000), log-law + 10% noise,
+                 units (nu, dPdx of input_parameters_2000), log-law + 10% noise
+                 (relative to the local mean, as in `large` and the ensemble),
                  DSM, equilibrium wall model at h_wm = 0.1.
     large      : 384x193x384 (faces 386 193 386), 4pi x 2 x 2pi, Re_tau 5200.
     """
@@ -133,7 +143,7 @@ def config(name, seed=1234):
         c = Case(name, 194, 97, 194, 2 * math.pi, math.pi, tanh_y(97, 1.5), nu=nu, dPdx=dpdx, CFL=0.5,
                  h_wm=0.1, steps=2000)
         ub = bulk_velocity_plus(re_tau) * utau
-        f = c.initial_field(lambda d: utau * loglaw_plus(d * utau / nu), ub, 0.10, seed)
+        f = c.initial_field(lambda d: utau * loglaw_plus(d * utau / nu), ub, 0.10, seed, local=True)
         return c, f
     if name == "large":
         re_tau = 5200.0
@@ -141,7 +151,7 @@ def config(name, seed=1234):
         c = Case(name, 386, 193, 386, 4 * math.pi, 2 * math.pi, tanh_y(193, 1.5), nu=nu, dPdx=1.0, CFL=0.5,
                  h_wm=0.1, steps=500)
         ub = bulk_velocity_plus(re_tau)
-        f = c.initial_field(lambda d: loglaw_plus(d / nu), ub, 0.10, seed)
+        f = c.initial_field(lambda d: loglaw_plus(d / nu), ub, 0.10, seed, local=True)
         return c, f
     raise KeyError(name)
 
@@ -154,7 +164,11 @@ def ensemble(n=64, seed=42, field_seed=1234):
     uniformly from ENSEMBLE_GRIDS and Re_tau log-uniform in [180, 5200] with
     the seeded generator `seed`; nu = 1/Re_tau (u_tau = 1 units), constant
     mass flow, DSM, wall model at h_wm = 0.1, initial field log law + 10%
-    noise (noise seed field_seed + channel index).  Returns [(Case, field)]."""
+    noise relative to the local mean (noise seed field_seed + channel index):
+    noise of 10% of the bulk velocity flips the sign of the wall gradient in
+    the viscous sublayer of the low-Re_tau fine grids, where the wall model's
+    viscosity alpha/(dU/dy) - nu then turns negative and the run diverges
+    within a few steps.  Returns [(Case, field)]."""
     rng = np.random.default_rng(seed)
     out = []
     for q in range(n):
@@ -165,7 +179,7 @@ def ensemble(n=64, seed=42, field_seed=1234):
                  h_wm=0.1, steps=1000)
         c.re_tau = re_tau
         ub = bulk_velocity_plus(re_tau)
-        f = c.initial_field(lambda d, nu=nu: loglaw_plus(d / nu), ub, 0.10, field_seed + q)
+        f = c.initial_field(lambda d, nu=nu: loglaw_plus(d / nu), ub, 0.10, field_seed + q, local=True)
         out.append((c, f))
     return out
 

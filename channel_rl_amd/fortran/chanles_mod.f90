@@ -136,6 +136,31 @@ Module chanles_c
        Import :: c_ptr, c_int
        Type(c_ptr), Value :: ctx
      End Function chan_synchronize
+     Integer(c_int) Function chan_solve_poisson_equation(ctx) Bind(c, name='chan_solve_poisson_equation')
+       Import :: c_ptr, c_int
+       Type(c_ptr), Value :: ctx
+     End Function chan_solve_poisson_equation
+     Integer(c_int) Function chan_compute_equilibrium_wall_stress(ctx, h_wm, kappa, B) &
+          Bind(c, name='chan_compute_equilibrium_wall_stress')
+       Import :: c_ptr, c_int, c_double
+       Type(c_ptr), Value :: ctx
+       Real(c_double), Value :: h_wm, kappa, B
+     End Function chan_compute_equilibrium_wall_stress
+     Integer(c_int) Function chan_rl_observe(ctx, nagents, dPdx_ref, draws, states, rewards) &
+          Bind(c, name='chan_rl_observe')
+       Import :: c_ptr, c_int, c_double
+       Type(c_ptr), Value :: ctx
+       Integer(c_int), Value :: nagents
+       Real(c_double), Value :: dPdx_ref
+       Real(c_double), Intent(In) :: draws(*)
+       Real(c_double), Intent(Out) :: states(*), rewards(*)
+     End Function chan_rl_observe
+     Integer(c_int) Function chan_rl_act(ctx, nagents, factors) Bind(c, name='chan_rl_act')
+       Import :: c_ptr, c_int, c_double
+       Type(c_ptr), Value :: ctx
+       Integer(c_int), Value :: nagents
+       Real(c_double), Intent(In) :: factors(*)
+     End Function chan_rl_act
      Integer(c_size_t) Function c_strlen(s) Bind(c, name='strlen')
        Import :: c_ptr, c_size_t
        Type(c_ptr), Value :: s
@@ -152,7 +177,17 @@ Module chanles_gpu
   ! sizes (code/global.f90:49-61) and the scalars the reference keeps global
   Integer :: nx, ny, nz, nxg, nyg, nzg
   Real(c_double) :: dt = 0d0, t = 0d0, dPdx = 0d0, Qflow_x_0 = 0d0
-  Type(c_ptr) :: ctx = c_null_ptr
+  ! the reference's module arrays that its hot-path subroutines read or
+  ! write besides their arguments (code/global.f90:85-186,
+  ! code/initialization.f90:159,467,476): the eddy viscosity compute_rhs_*
+  ! reads, the wall stress compute_wall_model reads, the pseudo-pressure
+  ! solve_poisson_equation solves in place
+  Real(c_double), Allocatable, Target :: nu_t(:,:,:), alpha_x(:,:,:), rhs_p(:,:,:)
+  ! ctx holds the resident simulation (compute_time_step_RK3 and the calls
+  ! without array arguments act on it); wctx is a work context of the same
+  ! channel on which the wrappers that take arrays compute, so that a call
+  ! between steps never replaces the resident state
+  Type(c_ptr) :: ctx = c_null_ptr, wctx = c_null_ptr
   Real(c_double), Allocatable, Target :: xf(:), yf(:), zf(:)
 
 Contains
@@ -174,6 +209,19 @@ Contains
     Stop 'Error: GPU channel call failed'
   End Subroutine check
 
+  ! check() for a call on another context than ctx
+  Subroutine check_ctx(status, c, what)
+    Integer(c_int), Intent(In) :: status
+    Type(c_ptr), Intent(In) :: c
+    Character(*), Intent(In) :: what
+    Type(c_ptr) :: keep
+    If (status == 0) Return
+    keep = ctx
+    ctx = c
+    Call check(status, what)
+    ctx = keep
+  End Subroutine check_ctx
+
   ! code/initialization.f90:23 initialize (the GPU part): grid and parameters
   Subroutine chanles_initialize(nx_, ny_, nz_, x, y, z, nu, CFL, dPdx_, x_mass_cte, LES_model, &
                                 Dirichlet_nu_t, precision)
@@ -191,13 +239,27 @@ Contains
     cfg%precision = precision; cfg%device = 0
     dPdx = dPdx_
     Call check(chan_create(cfg, ctx), 'chan_create')
+    Call check_ctx(chan_create(cfg, wctx), wctx, 'chan_create (work context)')
+    Allocate(nu_t(nxg,nyg,nzg), alpha_x(nx,2,nzg), rhs_p(2:nxg-1,2:nyg-1,2:nzg))
+    nu_t = 0d0
+    alpha_x = 0d0
+    rhs_p = 0d0
   End Subroutine chanles_initialize
 
   Subroutine chanles_finalize
     If (c_associated(ctx)) Call chan_destroy(ctx)
+    If (c_associated(wctx)) Call chan_destroy(wctx)
     ctx = c_null_ptr
+    wctx = c_null_ptr
     If (Allocated(xf)) Deallocate(xf, yf, zf)
+    If (Allocated(nu_t)) Deallocate(nu_t, alpha_x, rhs_p)
   End Subroutine chanles_finalize
+
+  ! the module's alpha_x as the wall stress of the resident simulation's steps
+  ! (code/wallmodel.f90:30-35 reads the global alpha_x at every substep)
+  Subroutine chanles_set_alpha_x
+    Call check(chan_set_field(ctx, CHAN_ALPHA_X, alpha_x), 'set alpha_x')
+  End Subroutine chanles_set_alpha_x
 
   ! host -> device state, then Qflow_x_0 (code/initialization.f90:442)
   Subroutine chanles_set_state(U, V, W)
@@ -245,58 +307,125 @@ Contains
     Call check(chan_compute_projection_step(ctx), 'compute_projection_step')
   End Subroutine compute_projection_step
 
-  ! code/subgrid.f90:23
+  ! the wrappers below that take arrays compute on the work context wctx
+  Subroutine w_set(which, a, what)
+    Integer(c_int), Intent(In) :: which
+    Real(c_double), Intent(In) :: a(*)
+    Character(*), Intent(In) :: what
+    Call check_ctx(chan_set_field(wctx, which, a), wctx, what)
+  End Subroutine w_set
+
+  Subroutine w_velocity(U_, V_, W_)
+    Real(c_double), Intent(In) :: U_(nx,nyg,nzg), V_(nxg,ny,nzg), W_(nxg,nyg,nz)
+    Call w_set(CHAN_U, U_, 'set U')
+    Call w_set(CHAN_V, V_, 'set V')
+    Call w_set(CHAN_W, W_, 'set W')
+  End Subroutine w_velocity
+
+  ! code/subgrid.f90:23 compute_eddy_viscosity(U_,V_,W_,nu_t_)
   Subroutine compute_eddy_viscosity(U_, V_, W_, nu_t_)
     Real(c_double), Intent(In) :: U_(nx,nyg,nzg), V_(nxg,ny,nzg), W_(nxg,nyg,nz)
     Real(c_double), Intent(Out) :: nu_t_(nxg,nyg,nzg)
-    Call check(chan_set_field(ctx, CHAN_U, U_), 'set U')
-    Call check(chan_set_field(ctx, CHAN_V, V_), 'set V')
-    Call check(chan_set_field(ctx, CHAN_W, W_), 'set W')
-    Call check(chan_compute_eddy_viscosity(ctx), 'compute_eddy_viscosity')
-    Call check(chan_get_field(ctx, CHAN_NU_T, nu_t_), 'get nu_t')
+    Call w_velocity(U_, V_, W_)
+    Call check_ctx(chan_compute_eddy_viscosity(wctx), wctx, 'compute_eddy_viscosity')
+    Call check_ctx(chan_get_field(wctx, CHAN_NU_T, nu_t_), wctx, 'get nu_t')
   End Subroutine compute_eddy_viscosity
 
-  ! code/wallmodel.f90:23 with alpha_x (nx,2,nzg) passed explicitly
-  Subroutine compute_wall_model(U_, V_, W_, nu_t_, alpha_x)
+  ! code/wallmodel.f90:23 compute_wall_model(U_,V_,W_,nu_t_) with the
+  ! module's alpha_x (code/wallmodel.f90:30-35)
+  Subroutine compute_wall_model(U_, V_, W_, nu_t_)
     Real(c_double), Intent(In) :: U_(nx,nyg,nzg), V_(nxg,ny,nzg), W_(nxg,nyg,nz)
     Real(c_double), Intent(InOut) :: nu_t_(nxg,nyg,nzg)
-    Real(c_double), Intent(In) :: alpha_x(nx,2,nzg)
-    Call check(chan_set_field(ctx, CHAN_U, U_), 'set U')
-    Call check(chan_set_field(ctx, CHAN_NU_T, nu_t_), 'set nu_t')
-    Call check(chan_set_field(ctx, CHAN_ALPHA_X, alpha_x), 'set alpha_x')
-    Call check(chan_compute_wall_model(ctx), 'compute_wall_model')
-    Call check(chan_get_field(ctx, CHAN_NU_T, nu_t_), 'get nu_t')
+    Call w_set(CHAN_U, U_, 'set U')
+    Call w_set(CHAN_NU_T, nu_t_, 'set nu_t')
+    Call w_set(CHAN_ALPHA_X, alpha_x, 'set alpha_x')
+    Call check_ctx(chan_compute_wall_model(wctx), wctx, 'compute_wall_model')
+    Call check_ctx(chan_get_field(wctx, CHAN_NU_T, nu_t_), wctx, 'get nu_t')
   End Subroutine compute_wall_model
 
-  ! code/equations.f90:24,174,327 (all three components from one upload)
-  Subroutine compute_rhs(U_, V_, W_, nu_t_, rhs_u, rhs_v, rhs_w)
-    Real(c_double), Intent(In) :: U_(nx,nyg,nzg), V_(nxg,ny,nzg), W_(nxg,nyg,nz), nu_t_(nxg,nyg,nzg)
-    Real(c_double), Intent(Out) :: rhs_u(2:nx-1,2:nyg-1,2:nzg-1)
-    Real(c_double), Intent(Out) :: rhs_v(2:nxg-1,2:ny-1,2:nzg-1)
-    Real(c_double), Intent(Out) :: rhs_w(2:nxg-1,2:nyg-1,2:nz-1)
-    Call check(chan_set_field(ctx, CHAN_U, U_), 'set U')
-    Call check(chan_set_field(ctx, CHAN_V, V_), 'set V')
-    Call check(chan_set_field(ctx, CHAN_W, W_), 'set W')
-    Call check(chan_set_field(ctx, CHAN_NU_T, nu_t_), 'set nu_t')
-    Call check(chan_compute_rhs(ctx), 'compute_rhs')
-    Call check(chan_get_field(ctx, CHAN_RHS_U, rhs_u), 'get rhs_u')
-    Call check(chan_get_field(ctx, CHAN_RHS_V, rhs_v), 'get rhs_v')
-    Call check(chan_get_field(ctx, CHAN_RHS_W, rhs_w), 'get rhs_w')
-  End Subroutine compute_rhs
+  ! the three right-hand sides of U_,V_,W_ with the module's nu_t and dPdx
+  Subroutine w_rhs(U_, V_, W_)
+    Real(c_double), Intent(In) :: U_(nx,nyg,nzg), V_(nxg,ny,nzg), W_(nxg,nyg,nz)
+    Call w_velocity(U_, V_, W_)
+    Call w_set(CHAN_NU_T, nu_t, 'set nu_t')
+    Call check_ctx(chan_set_scalars(wctx, dt, t, dPdx, Qflow_x_0), wctx, 'set_scalars')
+    Call check_ctx(chan_compute_rhs(wctx), wctx, 'compute_rhs')
+  End Subroutine w_rhs
 
-  ! code/mass_flow.f90:113
+  ! code/equations.f90:24 compute_rhs_u(U_,V_,W_,rhs_u)
+  Subroutine compute_rhs_u(U_, V_, W_, rhs_u)
+    Real(c_double), Intent(In) :: U_(nx,nyg,nzg), V_(nxg,ny,nzg), W_(nxg,nyg,nz)
+    Real(c_double), Intent(Out) :: rhs_u(2:nx-1,2:nyg-1,2:nzg-1)
+    Call w_rhs(U_, V_, W_)
+    Call check_ctx(chan_get_field(wctx, CHAN_RHS_U, rhs_u), wctx, 'get rhs_u')
+  End Subroutine compute_rhs_u
+
+  ! code/equations.f90:174 compute_rhs_v(U_,V_,W_,rhs_v)
+  Subroutine compute_rhs_v(U_, V_, W_, rhs_v)
+    Real(c_double), Intent(In) :: U_(nx,nyg,nzg), V_(nxg,ny,nzg), W_(nxg,nyg,nz)
+    Real(c_double), Intent(Out) :: rhs_v(2:nxg-1,2:ny-1,2:nzg-1)
+    Call w_rhs(U_, V_, W_)
+    Call check_ctx(chan_get_field(wctx, CHAN_RHS_V, rhs_v), wctx, 'get rhs_v')
+  End Subroutine compute_rhs_v
+
+  ! code/equations.f90:327 compute_rhs_w(U_,V_,W_,rhs_w)
+  Subroutine compute_rhs_w(U_, V_, W_, rhs_w)
+    Real(c_double), Intent(In) :: U_(nx,nyg,nzg), V_(nxg,ny,nzg), W_(nxg,nyg,nz)
+    Real(c_double), Intent(Out) :: rhs_w(2:nxg-1,2:nyg-1,2:nz-1)
+    Call w_rhs(U_, V_, W_)
+    Call check_ctx(chan_get_field(wctx, CHAN_RHS_W, rhs_w), wctx, 'get rhs_w')
+  End Subroutine compute_rhs_w
+
+  ! code/mass_flow.f90:24 compute_mean_mass_flow_U(U,Qflow)
+  Subroutine compute_mean_mass_flow_U(U_, Qflow)
+    Real(c_double), Intent(In) :: U_(nx,nyg,nzg)
+    Real(c_double), Intent(Out) :: Qflow
+    Call w_set(CHAN_U, U_, 'set U')
+    Call check_ctx(chan_compute_mean_mass_flow_U(wctx, Qflow), wctx, 'compute_mean_mass_flow_U')
+  End Subroutine compute_mean_mass_flow_U
+
+  ! code/mass_flow.f90:113 compute_dPx_for_constant_mass_flow(U,dPdx) with the
+  ! module's Qflow_x_0
   Subroutine compute_dPx_for_constant_mass_flow(U_, dPdx_)
     Real(c_double), Intent(In) :: U_(nx,nyg,nzg)
     Real(c_double), Intent(Out) :: dPdx_
-    Call check(chan_set_field(ctx, CHAN_U, U_), 'set U')
-    Call check(chan_compute_dPx_for_constant_mass_flow(ctx, dPdx_), 'compute_dPx_for_constant_mass_flow')
+    Call w_set(CHAN_U, U_, 'set U')
+    Call check_ctx(chan_set_scalars(wctx, dt, t, dPdx, Qflow_x_0), wctx, 'set_scalars')
+    Call check_ctx(chan_compute_dPx_for_constant_mass_flow(wctx, dPdx_), wctx, &
+                   'compute_dPx_for_constant_mass_flow')
   End Subroutine compute_dPx_for_constant_mass_flow
 
-  ! code/projection.f90:223
-  Subroutine check_divergence(max_divergence, max_abs_divergence)
-    Real(c_double), Intent(Out) :: max_divergence, max_abs_divergence
-    Call check(chan_check_divergence(ctx, max_divergence, max_abs_divergence), 'check_divergence')
+  ! code/projection.f90:74 solve_poisson_equation: the module's rhs_p
+  ! (2:nxg-1,2:nyg-1,2:nzg) in place, with its periodic images in x and z
+  ! (code/projection.f90:133-150); on one domain the extra plane nzg is
+  ! left as it was (code/projection.f90:156-180 receives nothing there)
+  Subroutine solve_poisson_equation
+    Real(c_double), Allocatable :: P(:,:,:)
+    Allocate(P(nxg,nyg,nzg))
+    P = 0d0
+    P(2:nxg-1,2:nyg-1,2:nzg) = rhs_p
+    Call w_set(CHAN_P, P, 'set P')
+    Call check_ctx(chan_solve_poisson_equation(wctx), wctx, 'solve_poisson_equation')
+    Call check_ctx(chan_get_field(wctx, CHAN_P, P), wctx, 'get P')
+    rhs_p(2:nxg-1,2:nyg-1,2:nzg-1) = P(2:nxg-1,2:nyg-1,2:nzg-1)
+    Deallocate(P)
+  End Subroutine solve_poisson_equation
+
+  ! code/projection.f90:223 check_divergence(max_divergence) of the resident
+  ! state: the signed maximum, started at 0 as the reference does
+  Subroutine check_divergence(max_divergence)
+    Real(c_double), Intent(Out) :: max_divergence
+    Real(c_double) :: max_abs
+    Call check(chan_check_divergence(ctx, max_divergence, max_abs), 'check_divergence')
   End Subroutine check_divergence
+
+  ! the largest |divergence| of the resident state (no reference counterpart:
+  ! the round-off check the signed maximum cannot give)
+  Subroutine check_divergence_magnitude(max_abs_divergence)
+    Real(c_double), Intent(Out) :: max_abs_divergence
+    Real(c_double) :: max_div
+    Call check(chan_check_divergence(ctx, max_div, max_abs_divergence), 'check_divergence')
+  End Subroutine check_divergence_magnitude
 
   Subroutine set_equilibrium_wall_model(h_wm)
     Real(c_double), Intent(In) :: h_wm

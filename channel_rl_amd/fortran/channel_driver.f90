@@ -92,7 +92,8 @@ Program channel_driver
      Call compute_dt
      Call compute_time_step_RK3
      If (Mod(istep, nmonitor) == 0 .Or. istep == nsteps) Then
-        Call check_divergence(maxdiv, maxabs)
+        Call check_divergence(maxdiv)
+        Call check_divergence_magnitude(maxabs)
         Call System_clock(c1)
         Write(*,'(A,I8,A,ES12.4,A,ES12.4,A,F10.4)') ' step', istep, '  dt', dt, '  max|div|', maxabs, &
              '  elapsed (s)', Real(c1 - c0, Int64) / Real(crate, Int64)

@@ -31,3 +31,12 @@ def gpu():
     import channel_rl_amd
     channel_rl_amd.load_library()
     return True
+
+
+@pytest.fixture(scope="session")
+def wall_network(gpu):
+    """The learned wall-stress closure trained as the benchmark trains it
+    (10^6 samples of the small channel's runs, 20 epochs): (network, losses)."""
+    from channel_rl_amd import wallnet
+    X, Y = wallnet.solver_samples(1_000_000, seed=7)
+    return wallnet.train(X, Y, epochs=20, batch=1024, lr=1e-3, seed=0)

@@ -68,3 +68,63 @@ def test_ensemble_rejects_bad_members(gpu):
         Ensemble([ch, ch])
     with pytest.raises(Exception):
         Ensemble([])
+
+
+def test_config5_batched_ensemble_matches_members_alone(gpu, wall_network):
+    """The ensemble configuration as specified: 64 channels drawn by seed 42
+    (faces 34 33 34 / 66 49 66 / 98 65 98, Re_tau log-uniform in
+    [180, 5200]), FP32 bulk with FP64 Poisson, the learned wall model in the
+    loop at h_wm = 0.1: 10 batched steps end bit for bit where the same
+    channels end when each is stepped alone."""
+    from channel_rl_amd import WALL_MLP
+    net = wall_network[0]
+    mem = cases.ensemble(64)
+    assert len({(c.nx, c.ny, c.nz) for c, _ in mem}) == 3
+    alone = [make_channel(c, f, precision=FP32_BULK, wall=WALL_MLP, net=net) for c, f in mem]
+    batch = [make_channel(c, f, precision=FP32_BULK, wall=WALL_MLP, net=net) for c, f in mem]
+    ens = Ensemble(batch)
+    for ch in alone:
+        ch.run_steps(10)
+    ens.run_steps(10)
+    ens.synchronize()
+    for q, (a, b) in enumerate(zip(alone, batch)):
+        for x, y in zip(_state(a), _state(b)):
+            assert np.all(np.isfinite(x))
+            assert np.array_equal(x, y), f"member {q}"
+        assert a.wall_model_stats() == b.wall_model_stats()
+    err = np.mean([ch.wall_model_stats()[0] for ch in batch])
+    print(f"ensemble mean wall-stress deviation from the equilibrium law after 10 steps: {err:.4f}")
+    ens.close()
+
+
+def test_config5_one_step_per_grid_against_oracle(gpu):
+    """One batched step of the first member of each grid size of the
+    ensemble draw (its Re_tau and seeded log-law field; FP64, dynamic
+    Smagorinsky, equilibrium wall model at h_wm = 0.1, constant mass flow)
+    against the oracle: 1e-11 relative, as a single channel's step."""
+    import oracle
+    from helpers import rel
+    first = {}
+    for c, f in cases.ensemble(64):
+        first.setdefault((c.nx, c.ny, c.nz), (c, f))
+    assert len(first) == 3
+    chans, oracles = [], []
+    for c, f in first.values():
+        g = make_channel(c, f, precision=FP64, wall=WALL_EQUILIBRIUM)
+        o = oracle.Oracle(c.x, c.y, c.z, **c.kwargs())
+        o.set_state(*g.state())
+        s = g.scalars()
+        o.set_scalars(dt=s["dt"], t=s["t"], dPdx=s["dPdx"], Qflow_x_0=s["Qflow_x_0"])
+        o.set_wall_model(1, h_wm=c.h_wm)
+        chans.append(g)
+        oracles.append(o)
+    ens = Ensemble(chans)
+    ens.run_steps(1)
+    ens.synchronize()
+    for (c, _), g, o in zip(first.values(), chans, oracles):
+        o.compute_dt()
+        o.compute_time_step_RK3()
+        for a, b in zip(g.state(), o.state()):
+            assert rel(a, b) <= 1e-11, ((c.nx, c.ny, c.nz), rel(a, b))
+        assert abs(g.scalars()["t"] - o.scalars()["t"]) <= 1e-15
+    ens.close()

@@ -92,10 +92,9 @@ def test_wall_samples_match_oracle(gpu):
 
 
 @pytest.fixture(scope="module")
-def trained():
-    X, Y = wallnet.solver_samples(1_000_000, seed=7, develop_steps=20)
+def trained(wall_network):
+    net, losses = wall_network
     Xv, Yv = wallnet.solver_samples(20_000, seed=8, develop_steps=80)
-    net, losses = wallnet.train(X, Y, epochs=20, batch=1024, lr=1e-3, seed=0)
     return net, losses, Xv, Yv
 
 
@@ -188,3 +187,27 @@ def test_fallback_gives_the_equilibrium_stress(gpu):
         ref = om.equilibrium_alpha_2d(F, case.nu, 0.41, 5.2, oracle.loglaw_utau)
         got = g.get_field(ALPHA_X).ravel(order="F")
         assert np.max(np.abs(got - ref)) <= 1e-12 * np.abs(ref).max()
+
+
+def test_config4_learned_wall_model_at_wmles_scale(gpu, trained):
+    """The learned-wall-model configuration: 192x97x192 cells (faces 194 97
+    194), Re_tau 2000, h_wm = 0.1, FP64 bulk.  200 equilibrium steps from the
+    seeded log-law field (warm start), then 100 steps with the network in the
+    loop: the network's stress deviates from the equilibrium law's by at most
+    2% on average over the wall cells, and at most 10% of the cells fall
+    back to the equilibrium value."""
+    from channel_rl_amd.ensemble import make_channel
+    net = trained[0]
+    case, field = cases.config("wmles", seed=1234)
+    ch = make_channel(case, field, precision=FP64, wall=WALL_EQUILIBRIUM)
+    ch.run_steps(200)
+    ch.set_wall_network(net, max_rel_err=0.05)
+    ch.set_wall_model(WALL_MLP, h_wm=case.h_wm)
+    stats = []
+    for _ in range(5):
+        ch.run_steps(20)
+        stats.append(ch.wall_model_stats())
+    print("config 4 (error, fallback) every 20 steps:", " ".join(f"({e:.4f}, {f:.4f})" for e, f in stats))
+    err, fb = stats[-1]
+    assert np.all(np.isfinite(ch.get_field(ALPHA_X)))
+    assert err <= 0.02 and fb <= 0.10, (err, fb)

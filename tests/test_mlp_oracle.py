@@ -104,3 +104,128 @@ def test_wall_samples_follow_the_log_law():
     assert np.allclose(X[:, 0], np.arcsinh(h * 0.8 / nu / 10))
     assert np.allclose(X[:, 1:], 0.0)
     assert np.allclose(Y[:, 0], np.log1p(yp * yp)) and np.all(Y[:, 1] == 0.0)
+
+
+# ------------------------------------------------------------------------
+# The forward pass, its gradients and the fallback stress pinned by an
+# independent implementation (torch, float64) and by closed forms.
+
+def _layout():
+    """Offsets of W1 b1 W2 b2 W3 b3 W4 b4 from the documented sizes
+    (W[k][n]: input k, output n): 4x64, 64, 64x64, 64, 64x64, 64, 64x2, 2."""
+    sizes = [(4, 64), (64,), (64, 64), (64,), (64, 64), (64,), (64, 2), (2,)]
+    offs, o = [], 0
+    for s in sizes:
+        offs.append((o, s))
+        o += int(np.prod(s))
+    assert o == 8770
+    return offs
+
+
+def _torch_model(p, torch):
+    lay = _layout()
+    m = torch.nn.Sequential(torch.nn.Linear(4, 64), torch.nn.SiLU(), torch.nn.Linear(64, 64), torch.nn.SiLU(),
+                            torch.nn.Linear(64, 64), torch.nn.SiLU(), torch.nn.Linear(64, 2)).double()
+    lin = [m[0], m[2], m[4], m[6]]
+    with torch.no_grad():
+        for q, layer in enumerate(lin):
+            (ow, sw), (ob, sb) = lay[2 * q], lay[2 * q + 1]
+            W = torch.tensor(p[ow:ow + int(np.prod(sw))].reshape(sw))
+            layer.weight.copy_(W.T)       # Linear computes x @ weight.T
+            layer.bias.copy_(torch.tensor(p[ob:ob + sb[0]]))
+    return m
+
+
+def test_forward_matches_torch_float64():
+    torch = pytest.importorskip("torch")
+    p, norm, X, _ = _random_net(5)
+    m = _torch_model(p, torch)
+    x = (torch.tensor(X.astype(np.float32).astype(np.float64)) - torch.tensor(norm[0:4])) / torch.tensor(norm[4:8])
+    ref = (m(x) * torch.tensor(norm[10:12]) + torch.tensor(norm[8:10])).detach().numpy()
+    got = om.forward(p, norm, X, quant=False)
+    assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_gradients_match_torch_autograd():
+    torch = pytest.importorskip("torch")
+    p, norm, X, T = _random_net(6)
+    m = _torch_model(p, torch)
+    x = (torch.tensor(X.astype(np.float32).astype(np.float64)) - torch.tensor(norm[0:4])) / torch.tensor(norm[4:8])
+    t = (torch.tensor(T.astype(np.float32).astype(np.float64)) - torch.tensor(norm[8:10])) / torch.tensor(norm[10:12])
+    loss = 0.5 * ((m(x) - t) ** 2).sum() / len(X)
+    loss.backward()
+    g, lo = om.gradients(p, norm, X, T, quant=False)
+    assert abs(lo - loss.item()) <= 1e-13 * loss.item()
+    lay = _layout()
+    lin = [m[0], m[2], m[4], m[6]]
+    for q, layer in enumerate(lin):
+        (ow, sw), (ob, sb) = lay[2 * q], lay[2 * q + 1]
+        gw = layer.weight.grad.T.numpy().ravel()
+        gb = layer.bias.grad.numpy()
+        assert np.max(np.abs(g[ow:ow + gw.size] - gw)) <= 1e-12 * np.max(np.abs(gw)), ("W", q + 1)
+        assert np.max(np.abs(g[ob:ob + gb.size] - gb)) <= 1e-12 * np.max(np.abs(gb)), ("b", q + 1)
+
+
+def test_quantised_forward_matches_torch_bfloat16_points():
+    # the BF16 rounding points of the quantised forward (weights, h1..h3),
+    # rounded by torch's own conversion
+    torch = pytest.importorskip("torch")
+    p, norm, X, _ = _random_net(7)
+    lay = _layout()
+
+    def b(a):
+        return a.to(torch.bfloat16).to(torch.float64)
+
+    def W(q):
+        (o, s) = lay[2 * q]
+        return torch.tensor(p[o:o + int(np.prod(s))].reshape(s))
+
+    def bias(q):
+        (o, s) = lay[2 * q + 1]
+        return torch.tensor(p[o:o + s[0]])
+
+    h = (torch.tensor(X.astype(np.float32).astype(np.float64)) - torch.tensor(norm[0:4])) / torch.tensor(norm[4:8])
+    for q in range(3):
+        h = b(torch.nn.functional.silu(h @ b(W(q)) + bias(q)))
+    y = (h @ b(W(3)) + bias(3)) * torch.tensor(norm[10:12]) + torch.tensor(norm[8:10])
+    got = om.forward(p, norm, X, quant=True)
+    assert np.max(np.abs(got - y.numpy())) <= 1e-12 * np.max(np.abs(y.numpy()))
+
+
+def test_quantised_forward_hand_computed():
+    # W2 = W3 = 0, b3 = 1, W4 = 1/2: h3 = bf16(silu(1)) = 187/256 for every
+    # input (silu(1) = 0.7310585786..., BF16 keeps 8 significant bits), so
+    # y_n = 64 * 187/256 * 1/2 + b4_n exactly
+    p, _, X, _ = _random_net(8)
+    lay = _layout()
+    for q in (1, 2):
+        o, s = lay[2 * q]
+        p[o:o + int(np.prod(s))] = 0.0
+    o, s = lay[5]
+    p[o:o + 64] = 1.0
+    o, s = lay[6]
+    p[o:o + 128] = 0.5
+    o, s = lay[7]
+    p[o:o + 2] = [0.25, -3.0]
+    norm = np.array([0, 0, 0, 0, 1, 1, 1, 1, 0, 0, 1, 1], float)
+    got = om.forward(p, norm, X, quant=True)
+    assert np.all(got[:, 0] == 64 * 187 / 256 * 0.5 + 0.25)
+    assert np.all(got[:, 1] == 64 * 187 / 256 * 0.5 - 3.0)
+
+
+def test_fallback_stress_closed_form():
+    # equilibrium_alpha_2d: a velocity built by the forward log law at a
+    # known u_tau, along any wall-parallel direction, gives tau_wx =
+    # u_tau^2 cos(theta); in the viscous sublayer u_tau^2 = nu |u| / h
+    import oracle
+    nu, kappa, B = 1e-4, 0.41, 5.2
+    F, ref = [], []
+    for ut, h, th in ((0.05, 0.1, 0.0), (0.03, 0.2, 0.7), (1.0, 0.05, -2.0), (0.2, 0.01, 3.0)):
+        up = ut * (np.log(h * ut / nu) / kappa + B)
+        F.append((up * np.cos(th), up * np.sin(th), h))
+        ref.append(ut * ut * np.cos(th))
+    h, up = 1e-3, 0.02              # y+ = h u_tau / nu = sqrt(h up / nu) = 0.45: linear law
+    F.append((up, 0.0, h))
+    ref.append(nu * up / h)
+    got = om.equilibrium_alpha_2d(np.array(F), nu, kappa, B, oracle.loglaw_utau)
+    assert np.allclose(got, ref, rtol=1e-13, atol=0)
