@@ -37,6 +37,7 @@ EXPORTS = [
     "chan_ensemble_create", "chan_ensemble_destroy", "chan_ensemble_last_error", "chan_ensemble_size",
     "chan_ensemble_run_steps", "chan_ensemble_synchronize", "chan_wall_shear", "chan_rl_observe", "chan_rl_act",
     "chan_compute_pressure",
+    "chan_compute_pressure_noalloc",
 ]
 KERNEL_RHS = 0
 WALL_MLP = 2
@@ -129,6 +130,7 @@ def load_library(path=LIB_PATH):
     L.chan_rl_observe.argtypes = [vp, ctypes.c_int, ctypes.c_double, _D, _D, _D]
     L.chan_rl_act.argtypes = [vp, ctypes.c_int, _D]
     L.chan_compute_pressure.argtypes = [vp]
+    L.chan_compute_pressure_noalloc.argtypes = [vp]
     _LIB = L
     return L
 

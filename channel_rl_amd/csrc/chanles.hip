@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -71,6 +72,14 @@ struct Shape { int n1, n2, n3, o; };
 
 }  // namespace
 
+// Device memory of chan_compute_pressure_noalloc: the wall data of the true
+// pressure's Neumann rows (two nxg x nzg planes) for callers that cannot
+// allocate.  One copy per GPU, held until the process ends; its address on a
+// device comes from hipGetSymbolAddress with that device current (the name
+// itself is a host-side stand-in).
+static constexpr long NOALLOC_SCRATCH = 1L << 21;  // doubles, 16 MiB
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH];
+
 struct chan_ctx {
     std::string err;
     int prec = CHAN_FP64;
@@ -114,6 +123,7 @@ struct chan_ctx {
     float* wm_samples = nullptr;  // 2*nx*nzg x (4 + 2) floats, on demand
     long generation = 0;          // bumped whenever captured steps go stale
     double* pbc = nullptr;        // wall data of the true pressure, 2 planes nxg x nzg
+    double* scratch = nullptr;    // this device's copy of `scratch` (no-alloc entry point)
     Prof prof;
     std::vector<std::pair<std::string, std::pair<double, long>>> prof_out;
 };
@@ -406,6 +416,28 @@ int set_device(chan_ctx* c)
     return CHAN_OK;
 }
 
+// the address of this device's copy of `scratch`, looked up once per device
+// with that device current (the caller has made it current)
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[SCRATCH_DEVICES] = {};
+
+int scratch_address(chan_ctx* c)
+{
+    if (c->device < 0 || c->device >= SCRATCH_DEVICES)
+        return fail(c, CHAN_EINVAL, "device ordinal beyond the no-alloc scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[c->device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(c, CHAN_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[c->device] = static_cast<double*>(p);
+    }
+    c->scratch = g_scratch[c->device];
+    return CHAN_OK;
+}
+
 }  // namespace
 
 // ================================================================ C ABI
@@ -534,6 +566,7 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
     int rc = set_device(c);
     if (rc) return rc;
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    if ((rc = scratch_address(c))) return rc;
 
     // metrics
     if ((rc = upload(c, c->x, &c->md.x)) || (rc = upload(c, c->y, &c->md.y)) || (rc = upload(c, c->z, &c->md.z)) ||
@@ -782,14 +815,13 @@ int chan_solve_poisson_equation(chan_ctx* c)
 }
 
 extern "C++" template <typename T>
-void enqueue_pressure(chan_ctx* c)
+void enqueue_pressure(chan_ctx* c, double* bc1)
 {
     T** S = (T**)c->state[c->cur];
     T** R = (T**)c->rhs;
     launch_rhs_rk<T>(S[0], S[1], S[2], (const T*)c->nut, rhs_tabs<T>(c), c->d, T(c->nu), c->sc, 0, nullptr, nullptr,
                      nullptr, nullptr, nullptr, nullptr, R[0], R[1], R[2], 0.0, 0.0, c->rhs_shape, c->stream);
-    double* bc1 = c->pbc;
-    double* bc2 = c->pbc + (long)c->d.nxg * c->d.nzg;
+    double* bc2 = bc1 + (long)c->d.nxg * c->d.nzg;
     launch_pressure_rhs<T>(S[1], (const T*)c->nut, R[0], R[1], R[2], c->md, c->d, c->nu, bc1, bc2, c->pb.p,
                            c->stream);
     launch_poisson_forward_field(c->pb.p, c->d, c->pb, c->stream);
@@ -804,7 +836,24 @@ int chan_compute_pressure(chan_ctx* c)
     int rc = set_device(c);
     if (rc) return rc;
     if (!c->pbc && (rc = dalloc(c, (void**)&c->pbc, (size_t)2 * c->d.nxg * c->d.nzg * sizeof(double)))) return rc;
-    DISPATCH(c, enqueue_pressure, c);
+    DISPATCH(c, enqueue_pressure, c, c->pbc);
+    return check_launch(c);
+}
+
+int chan_compute_pressure_noalloc(chan_ctx* c)
+{
+    if (!c) return CHAN_EINVAL;
+    int rc = set_device(c);
+    if (rc) return rc;
+    if (!c->scratch) return fail(c, CHAN_ESTATE, "chan_compute_pressure_noalloc: no scratch address on this device");
+    const long need = 2L * c->d.nxg * c->d.nzg;
+    if (need > NOALLOC_SCRATCH)
+        return fail(c, CHAN_EINVAL, "chan_compute_pressure_noalloc: the wall data need 2*nxg*nzg = " +
+                                        std::to_string(need) + " doubles, more than the " +
+                                        std::to_string(NOALLOC_SCRATCH) + " of the library's scratch");
+    // every wall-data entry the kernels read is written earlier in the same
+    // call: nothing needs clearing
+    DISPATCH(c, enqueue_pressure, c, c->scratch);
     return check_launch(c);
 }
 

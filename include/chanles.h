@@ -96,6 +96,16 @@ int chan_solve_poisson_equation(chan_ctx* ctx);
  * every periodic image and wall ghost row set.  The next projection
  * overwrites CHAN_P with the pseudo-pressure. */
 int chan_compute_pressure(chan_ctx* ctx);
+/* chan_compute_pressure without any allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No reference counterpart.  The wall data
+ * that chan_compute_pressure allocates on its first call live instead in one
+ * library-wide device array of 2^21 doubles (16 MiB) per GPU, held until the
+ * process ends: a channel needs 2*nxg*nzg of them (CHAN_EINVAL beyond).  All
+ * calls on one GPU share that array, so calls that overlap on one GPU (from
+ * any contexts or threads) race; the results of a call do not depend on what
+ * an earlier call left there. */
+int chan_compute_pressure_noalloc(chan_ctx* ctx);
 /* code/mass_flow.f90:24 compute_mean_mass_flow_U(U,Q) */
 int chan_compute_mean_mass_flow_U(chan_ctx* ctx, double* Q);
 /* code/initialization.f90:442: Qflow_x_0 = compute_mean_mass_flow_U(U) */
