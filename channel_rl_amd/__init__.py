@@ -37,7 +37,7 @@ EXPORTS = [
     "chan_ensemble_create", "chan_ensemble_destroy", "chan_ensemble_last_error", "chan_ensemble_size",
     "chan_ensemble_run_steps", "chan_ensemble_synchronize", "chan_wall_shear", "chan_rl_observe", "chan_rl_act",
     "chan_compute_pressure",
-    "chan_compute_pressure_noalloc",
+    "chan_compute_pressure_noalloc", "chan_ensemble_set_grouping", "chan_ensemble_group_count",
 ]
 KERNEL_RHS = 0
 WALL_MLP = 2
@@ -125,6 +125,8 @@ def load_library(path=LIB_PATH):
     L.chan_ensemble_last_error.restype = ctypes.c_char_p
     L.chan_ensemble_size.argtypes = [vp]
     L.chan_ensemble_run_steps.argtypes = [vp, ctypes.c_int]
+    L.chan_ensemble_set_grouping.argtypes = [vp, ctypes.c_int]
+    L.chan_ensemble_group_count.argtypes = [vp]
     L.chan_ensemble_synchronize.argtypes = [vp]
     L.chan_wall_shear.argtypes = [vp, _D]
     L.chan_rl_observe.argtypes = [vp, ctypes.c_int, ctypes.c_double, _D, _D, _D]
@@ -183,6 +185,18 @@ class Ensemble:
 
     def run_steps(self, n):
         self._chk(load_library().chan_ensemble_run_steps(self._h, int(n)), "run_steps")
+
+    def set_grouping(self, on):
+        """Grouped launches for members of one shape (on, the default), or
+        every member's own kernels (off)."""
+        self._chk(load_library().chan_ensemble_set_grouping(self._h, 1 if on else 0), "set_grouping")
+
+    def group_count(self):
+        """Groups and lone members the members' current settings give."""
+        n = load_library().chan_ensemble_group_count(self._h)
+        if n < 0:
+            self._chk(n, "group_count")
+        return n
 
     def synchronize(self):
         self._chk(load_library().chan_ensemble_synchronize(self._h), "synchronize")

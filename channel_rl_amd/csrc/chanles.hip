@@ -3,6 +3,7 @@
 // points, including the graph-captured RK3 step.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -13,6 +14,7 @@
 #include "../../include/chanles.h"
 #include "kernels.hpp"
 #include "prof.hpp"
+#include "wallmodel.hpp"
 
 using namespace chan;
 
@@ -615,7 +617,8 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
     // partial sums: DSM rows use one slot per (x,z) block of the fused kernel
     c->dsm_kchunk = c->prec == CHAN_FP64 ? dsm_fused_kchunk<double>(d, c->device) : dsm_fused_kchunk<float>(d, c->device);
     c->nblk_dsm = dsm_fused_nblk(d, c->dsm_kchunk);
-    c->nblk_red = 1024;
+    // reduction blocks: about eight (y, z) rows each, at most 1024
+    c->nblk_red = (int)std::max(64L, std::min(1024L, (long)(d.nyg - 2) * (d.nzg - 2) / 8));
     size_t npart = std::max((size_t)(d.nyg + 1) * 2 * c->nblk_dsm, (size_t)3 * c->nblk_red);
     if ((rc = dalloc(c, (void**)&c->part, npart * sizeof(double)))) return rc;
     if ((rc = dalloc(c, (void**)&c->ratio, (size_t)3 * d.nyg * sizeof(double)))) return rc;
@@ -1330,10 +1333,14 @@ int chan_synchronize(chan_ctx* c)
 
 // ------------------------------------------------------------------ ensemble
 // Many independent channels (any sizes and precisions, one device) advanced
-// together: one step of every member is captured into one graph, each member
-// on its own stream between a fork and a join, so the members' kernels are
-// independent graph branches and one graph launch runs the whole ensemble
-// step.  Three graphs cover the members' state rotation.
+// together.  Members of one shape and the same step settings form a group:
+// its step is one launch per stage for all of its members (the kernels'
+// grouped form, each member's arguments in a device table), so a group of
+// 30 small channels costs the launches of one channel.  Members that fit no
+// group (another SGS model, a shape of their own) run their own kernels.
+// One ensemble step (every group and lone member a branch between a fork
+// and a join) is captured into a graph; three graphs cover the rotation of
+// the members' state buffers, each with the argument tables of its phase.
 struct chan_ensemble {
     std::string err;
     std::vector<chan_ctx*> ch;
@@ -1342,9 +1349,12 @@ struct chan_ensemble {
     hipEvent_t fork = nullptr, done = nullptr;
     std::vector<hipEvent_t> join;
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+    void* tabmem[3] = {nullptr, nullptr, nullptr};  // argument tables of each graph
     std::vector<int> cur0;    // members' rotation when the graphs were captured
     std::vector<long> gen0;   // members' generation then
     long step = 0;            // steps since the capture base
+    int grouping = 1;         // 0: every member runs its own kernels
+    std::vector<std::vector<int>> groups;  // members by group (a lone member: a group of one)
 };
 
 static int efail(chan_ensemble* e, int code, const std::string& msg)
@@ -1355,12 +1365,229 @@ static int efail(chan_ensemble* e, int code, const std::string& msg)
 
 static void ens_drop(chan_ensemble* e)
 {
-    for (int q = 0; q < 3; q++)
+    for (int q = 0; q < 3; q++) {
         if (e->graph[q]) { (void)hipGraphExecDestroy(e->graph[q]); e->graph[q] = nullptr; }
+        if (e->tabmem[q]) { (void)hipFree(e->tabmem[q]); e->tabmem[q] = nullptr; }
+    }
     e->cur0.clear();
     e->gen0.clear();
     e->step = 0;
 }
+
+extern "C++" {
+namespace {
+
+// Argument tables of one graph, built in two passes over the same enqueue
+// code: the first collects every stage's table in host memory (nothing is
+// launched), the second, after the tables are in device memory, hands out
+// their device addresses in the same order and launches.
+struct TabSink {
+    bool launch = false;
+    std::vector<unsigned char> host;
+    std::vector<size_t> offs;
+    size_t next = 0;
+    unsigned char* dev = nullptr;
+    template <typename A>
+    const A* put(const std::vector<A>& v)
+    {
+        if (!launch) {
+            const size_t o = (host.size() + 255) & ~(size_t)255;
+            host.resize(o + v.size() * sizeof(A));
+            std::memcpy(host.data() + o, v.data(), v.size() * sizeof(A));
+            offs.push_back(o);
+            return nullptr;
+        }
+        return reinterpret_cast<const A*>(dev + offs[next++]);
+    }
+};
+
+// the step settings that decide a channel's kernels and launch shapes
+bool same_group(const chan_ctx* a, const chan_ctx* b)
+{
+    return a->prec == b->prec && a->d.nx == b->d.nx && a->d.ny == b->d.ny && a->d.nz == b->d.nz &&
+           a->LES_model == 2 && b->LES_model == 2 && a->wall_mode == b->wall_mode && a->x_mass_cte == b->x_mass_cte &&
+           a->rhs_shape == b->rhs_shape && a->dsm_kchunk == b->dsm_kchunk && a->nblk_dsm == b->nblk_dsm &&
+           a->nblk_red == b->nblk_red && a->pb.bx == b->pb.bx && a->pb.bz == b->pb.bz;
+}
+
+template <typename T>
+__global__ void k_advance_time_g(const MassArgs<T>* tab) { tab[blockIdx.x].sc->t = tab[blockIdx.x].sc->t + tab[blockIdx.x].sc->dt; }
+
+// compute_dt + compute_time_step_RK3 of every member of a group, one launch
+// per stage (the arithmetic of enqueue_dt + enqueue_rk3 per member)
+template <typename T>
+int enqueue_group_step(chan_ensemble* e, const std::vector<int>& idx, TabSink& sink, hipStream_t s)
+{
+    const int n = (int)idx.size();
+    chan_ctx* c0 = e->ch[idx[0]];
+    const Dims& d = c0->d;
+    std::vector<chan_ctx*> cs(n);
+    std::vector<int> X(n);
+    for (int q = 0; q < n; q++) {
+        cs[q] = e->ch[idx[q]];
+        X[q] = cs[q]->cur;
+    }
+    auto st = [&](int q, int buf) { return (T**)cs[q]->state[buf]; };
+    {
+        std::vector<DtArgs<T>> a(n);
+        for (int q = 0; q < n; q++) {
+            T** u = st(q, X[q]);
+            a[q] = DtArgs<T>{u[0], u[1], u[2], cs[q]->md, cs[q]->part, cs[q]->sc, cs[q]->mins3};
+        }
+        const DtArgs<T>* t = sink.put(a);
+        if (sink.launch) run_dt<T>(a[0], t, n, d, c0->nblk_red, s);
+    }
+    for (int sub = 0; sub < 3; sub++) {
+        auto inb = [&](int q) { return sub == 0 ? X[q] : (sub == 1 ? (X[q] + 1) % 3 : (X[q] + 2) % 3); };
+        auto outb = [&](int q) { return sub == 1 ? (X[q] + 2) % 3 : (X[q] + 1) % 3; };
+        // wall stress into alpha_x
+        if (c0->wall_mode == CHAN_WALL_EQUILIBRIUM) {
+            std::vector<EqAlphaArgs<T>> a(n);
+            for (int q = 0; q < n; q++) {
+                chan_ctx* c = cs[q];
+                a[q] = EqAlphaArgs<T>{st(q, inb(q))[0], c->alpha, c->nu, wall_rows(c, c->h_wm), c->kappa, c->Bc,
+                                      loglaw_crossover(c->kappa, c->Bc)};
+            }
+            const EqAlphaArgs<T>* t = sink.put(a);
+            if (sink.launch) run_equilibrium_alpha<T>(a[0], t, n, d, s);
+        } else if (c0->wall_mode == CHAN_WALL_MLP) {
+            std::vector<WmArgs<T>> a(n);
+            for (int q = 0; q < n; q++) {
+                chan_ctx* c = cs[q];
+                const float *np = nullptr, *nn = nullptr;
+                int ndev = 0;
+                if (!mlp_device_params(c->net, &np, &nn, &ndev)) return efail(e, CHAN_ESTATE, "a member's wall network is gone");
+                T** u = st(q, inb(q));
+                a[q] = WmArgs<T>{np, nn, u[0], u[2], c->alpha, c->sc, c->md, c->nu, c->h_wm, c->kappa, c->Bc,
+                                 loglaw_crossover(c->kappa, c->Bc), c->wm_max_err, c->wm_stats, c->x_mass_cte};
+            }
+            const WmArgs<T>* t = sink.put(a);
+            if (sink.launch) run_wm_mlp<T>(a[0], t, n, d, s);
+        }
+        // dynamic Smagorinsky (pass A, plane ratios, pass B with the wall rows)
+        {
+            std::vector<DsmArgs<T>> a(n);
+            std::vector<DsmRatioArgs> r(n);
+            std::vector<DsmNutArgs<T>> v(n);
+            for (int q = 0; q < n; q++) {
+                chan_ctx* c = cs[q];
+                T** u = st(q, inb(q));
+                const DsmBufs<T> b = dsm_bufs<T>(c);
+                a[q] = DsmArgs<T>{u[0], u[1], u[2], b};
+                r[q] = DsmRatioArgs{b.part, b.ratio};
+                v[q] = DsmNutArgs<T>{(T*)c->nut, b.smag, b.ratio, u[0], c->alpha, metric<T>(c), T(c->nu),
+                                     c->Dirichlet_nu_t == 1};
+            }
+            const DsmArgs<T>* ta = sink.put(a);
+            const DsmRatioArgs* tr = sink.put(r);
+            const DsmNutArgs<T>* tv = sink.put(v);
+            if (sink.launch) {
+                run_dsm_fused<T>(a[0], ta, n, d, c0->dsm_kchunk, s);
+                run_dsm_ratio(r[0], tr, n, d, c0->nblk_dsm, s);
+                run_dsm_nut<T>(v[0], tv, n, d, 1, s);
+            }
+        }
+        // RHS + RK combine
+        {
+            double c1, c2 = 0.0;
+            if (sub == 0) c1 = c0->rk_coef[0][0];
+            else if (sub == 1) { c1 = c0->rk_coef[1][0]; c2 = c0->rk_coef[1][1]; }
+            else { c1 = c0->rk_coef[2][0]; c2 = c0->rk_coef[2][2]; }
+            std::vector<RhsArgs<T>> a(n);
+            for (int q = 0; q < n; q++) {
+                chan_ctx* c = cs[q];
+                T** I = st(q, inb(q));
+                T** O = st(q, outb(q));
+                T** Xs = st(q, X[q]);
+                a[q] = RhsArgs<T>{I[0], I[1], I[2], (const T*)c->nut, rhs_tabs<T>(c), T(c->nu), c->sc, Xs[0], Xs[1],
+                                  Xs[2], (T*)c->F1[0], (T*)c->F1[1], (T*)c->F1[2], O[0], O[1], O[2]};
+            }
+            const RhsArgs<T>* t = sink.put(a);
+            if (sink.launch) run_rhs_rk<T>(a[0], t, n, d, sub + 1, c1, c2, c0->rhs_shape, s);
+        }
+        // projection of the new state
+        {
+            std::vector<FwdArgs<T>> f(n);
+            std::vector<PoissonBufs> pbv(n);
+            std::vector<ProjArgs<T>> pr(n);
+            for (int q = 0; q < n; q++) {
+                chan_ctx* c = cs[q];
+                T** O = st(q, outb(q));
+                f[q] = FwdArgs<T>{O[0], O[1], O[2], c->md, c->pb};
+                pbv[q] = c->pb;
+                pr[q] = ProjArgs<T>{O[0], O[1], O[2], c->pb.p, c->md};
+            }
+            const FwdArgs<T>* tf = sink.put(f);
+            const PoissonBufs* tp = sink.put(pbv);
+            const ProjArgs<T>* tr = sink.put(pr);
+            if (sink.launch) {
+                run_poisson_forward<T>(f[0], tf, tp, n, d, c0->pb, s);
+                run_poisson_solve(c0->pb, tp, n, d, s);
+                run_poisson_inverse(c0->pb, tp, n, d, s);
+                run_project<T>(pr[0], tr, n, d, s);
+            }
+        }
+    }
+    // constant mass flow (its final reduction also advances the time), or the time advance alone
+    {
+        std::vector<MassArgs<T>> a(n);
+        for (int q = 0; q < n; q++) a[q] = MassArgs<T>{st(q, (X[q] + 1) % 3)[0], cs[q]->part, cs[q]->sc};
+        const MassArgs<T>* t = sink.put(a);
+        if (sink.launch) {
+            if (c0->x_mass_cte == 1) run_mass_flow_correct<T>(a[0], t, n, d, c0->nblk_red, 1, 1, s);
+            else k_advance_time_g<T><<<n, 1, 0, s>>>(t);
+        }
+    }
+    return CHAN_OK;
+}
+
+// one ensemble step of every group and lone member (sink: see TabSink)
+int enqueue_ensemble_step(chan_ensemble* e, TabSink& sink, hipError_t& h)
+{
+    for (const std::vector<int>& g : e->groups) {
+        chan_ctx* c0 = e->ch[g[0]];
+        if (sink.launch) {
+            h = hipStreamWaitEvent(c0->stream, e->fork, 0);
+            if (h != hipSuccess) return CHAN_EHIP;
+        }
+        if (g.size() > 1) {
+            const int rc = c0->prec == CHAN_FP64 ? enqueue_group_step<double>(e, g, sink, c0->stream)
+                                                 : enqueue_group_step<float>(e, g, sink, c0->stream);
+            if (rc) return rc;
+        } else if (sink.launch) {
+            DISPATCH(c0, enqueue_dt, c0);
+            DISPATCH(c0, enqueue_rk3, c0, c0->cur);
+        }
+        if (sink.launch) {
+            h = hipEventRecord(e->join[g[0]], c0->stream);
+            if (h == hipSuccess) h = hipStreamWaitEvent(e->stream, e->join[g[0]], 0);
+            if (h != hipSuccess) return CHAN_EHIP;
+        }
+    }
+    return CHAN_OK;
+}
+
+void ens_form_groups(chan_ensemble* e)
+{
+    e->groups.clear();
+    const int n = (int)e->ch.size();
+    std::vector<int> taken(n, 0);
+    for (int q = 0; q < n; q++) {
+        if (taken[q]) continue;
+        std::vector<int> g{q};
+        taken[q] = 1;
+        if (e->grouping)
+            for (int r = q + 1; r < n; r++)
+                if (!taken[r] && same_group(e->ch[q], e->ch[r])) {
+                    g.push_back(r);
+                    taken[r] = 1;
+                }
+        e->groups.push_back(g);
+    }
+}
+
+}  // namespace
+}  // extern "C++"
 
 int chan_ensemble_create(chan_ctx* const* members, int n, chan_ensemble** out)
 {
@@ -1405,6 +1632,24 @@ const char* chan_ensemble_last_error(const chan_ensemble* e) { return e ? e->err
 
 int chan_ensemble_size(const chan_ensemble* e) { return e ? (int)e->ch.size() : 0; }
 
+int chan_ensemble_set_grouping(chan_ensemble* e, int on)
+{
+    if (!e) return CHAN_EINVAL;
+    hipError_t h = hipSetDevice(e->device);
+    if (h == hipSuccess) h = hipStreamSynchronize(e->stream);
+    if (h != hipSuccess) return efail(e, CHAN_EHIP, std::string("ensemble grouping: ") + hipGetErrorString(h));
+    e->grouping = on ? 1 : 0;
+    ens_drop(e);
+    return CHAN_OK;
+}
+
+int chan_ensemble_group_count(chan_ensemble* e)
+{
+    if (!e) return CHAN_EINVAL;
+    ens_form_groups(e);
+    return (int)e->groups.size();
+}
+
 int chan_ensemble_run_steps(chan_ensemble* e, int nsteps)
 {
     if (!e || nsteps < 0) return CHAN_EINVAL;
@@ -1417,8 +1662,12 @@ int chan_ensemble_run_steps(chan_ensemble* e, int nsteps)
     for (int q = 0; q < n && !stale; q++)
         stale = e->ch[q]->generation != e->gen0[q] || e->ch[q]->cur != (e->cur0[q] + e->step) % 3;
     if (stale) {
+        // a table may still be read by a step in flight
+        h = hipStreamSynchronize(e->stream);
+        if (h != hipSuccess) return efail(e, CHAN_EHIP, std::string("ensemble sync: ") + hipGetErrorString(h));
         ens_drop(e);
         for (chan_ctx* c : e->ch) { e->cur0.push_back(c->cur); e->gen0.push_back(c->generation); }
+        ens_form_groups(e);
     }
     // order after whatever the members' own streams hold
     for (int q = 0; q < n; q++) {
@@ -1429,17 +1678,28 @@ int chan_ensemble_run_steps(chan_ensemble* e, int nsteps)
     for (int it = 0; it < nsteps; it++) {
         const int o = (int)(e->step % 3);
         if (!e->graph[o]) {
+            // pass 1: the argument tables of this phase, into device memory
+            TabSink sink;
+            int rc = enqueue_ensemble_step(e, sink, h);
+            if (rc) { ens_drop(e); return rc == CHAN_EHIP ? efail(e, rc, hipGetErrorString(h)) : rc; }
+            if (!sink.host.empty()) {
+                h = hipMalloc(&e->tabmem[o], sink.host.size());
+                if (h == hipSuccess)
+                    h = hipMemcpy(e->tabmem[o], sink.host.data(), sink.host.size(), hipMemcpyHostToDevice);
+                if (h != hipSuccess) {
+                    ens_drop(e);
+                    return efail(e, CHAN_EHIP, std::string("ensemble tables: ") + hipGetErrorString(h));
+                }
+            }
+            // pass 2: the launches, captured
+            sink.launch = true;
+            sink.dev = (unsigned char*)e->tabmem[o];
             hipGraph_t g;
             h = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
             if (h == hipSuccess) h = hipEventRecord(e->fork, e->stream);
-            for (int q = 0; q < n && h == hipSuccess; q++) {
-                chan_ctx* c = e->ch[q];
-                h = hipStreamWaitEvent(c->stream, e->fork, 0);
-                if (h != hipSuccess) break;
-                DISPATCH(c, enqueue_dt, c);
-                DISPATCH(c, enqueue_rk3, c, c->cur);
-                h = hipEventRecord(e->join[q], c->stream);
-                if (h == hipSuccess) h = hipStreamWaitEvent(e->stream, e->join[q], 0);
+            if (h == hipSuccess) {
+                rc = enqueue_ensemble_step(e, sink, h);
+                if (rc && h == hipSuccess) h = hipErrorUnknown;
             }
             hipError_t h2 = hipStreamEndCapture(e->stream, &g);
             if (h == hipSuccess) h = h2;
@@ -1449,7 +1709,8 @@ int chan_ensemble_run_steps(chan_ensemble* e, int nsteps)
             }
             if (h != hipSuccess) {
                 ens_drop(e);
-                return efail(e, CHAN_EHIP, std::string("ensemble capture: ") + hipGetErrorString(h));
+                return efail(e, CHAN_EHIP, std::string("ensemble capture: ") + hipGetErrorString(h) +
+                                               (e->err.empty() ? "" : " (" + e->err + ")"));
             }
         }
         h = hipGraphLaunch(e->graph[o], e->stream);

@@ -139,4 +139,61 @@ __device__ inline TileId xcd_tile()
     return TileId{l % gx, (l / gx) % gy, l / (gx * gy)};
 }
 
+// Grouped launches.  A group is several channels of one shape, each with its
+// own buffers and scalars, advanced by one launch per stage: the launch's
+// grid stacks the members' grids along z (gz planes of blocks each) and
+// tab[m] holds member m's arguments in device memory.  A launch for one
+// channel passes its arguments by value, tab = nullptr and gz = gridDim.z;
+// its member is 0.  GBlk is a block's position within its member's grid.
+struct GBlk {
+    int x, y, z;     // block within the member's grid
+    int gx, gy, gz;  // the member's grid
+    int m;           // member
+};
+
+// hardware order: member = blockIdx.z / gz
+__device__ inline GBlk gblk(int gz)
+{
+    const int z = blockIdx.z;
+    return GBlk{(int)blockIdx.x, (int)blockIdx.y, z % gz, (int)gridDim.x, (int)gridDim.y, gz, z / gz};
+}
+
+// XCD-contiguous order (xcd_linear) over the whole launch, then split into
+// member and tile (x fastest, then y, then z): every member's tiles stay
+// contiguous, so a small member runs on one or two XCDs and its halos stay
+// in their L2s.  For one channel this is xcd_tile().
+__device__ inline GBlk gblk_xcd(int gz)
+{
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int l = xcd_linear();
+    const int per = gx * gy * gz;
+    const int m = l / per, t = l - m * per;
+    return GBlk{t % gx, (t / gx) % gy, t / (gx * gy), gx, gy, gz, m};
+}
+
+// a member's arguments: its table entry in a grouped launch, else the launch's
+template <typename A>
+__device__ __forceinline__ A member_args(const A& one, const A* __restrict__ tab, int m)
+{
+    return tab ? tab[m] : one;
+}
+// the same chosen at compile time, for kernels instantiated per form: the
+// one-channel form keeps its arguments in the kernel-argument segment, from
+// which the compiler reloads them where they are used instead of holding
+// them in registers across the kernel
+template <bool G, typename A>
+__device__ __forceinline__ A member_args(const A& one, const A* __restrict__ tab, int m)
+{
+    if constexpr (G) return tab[m];
+    else return one;
+}
+// by reference, for arguments a kernel indexes at run time (an array inside
+// the struct): a copy would live in scratch memory
+template <bool G, typename A>
+__device__ __forceinline__ const A& member_ref(const A& one, const A* __restrict__ tab, int m)
+{
+    if constexpr (G) return tab[m];
+    else return one;
+}
+
 }  // namespace chan

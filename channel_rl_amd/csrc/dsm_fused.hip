@@ -73,10 +73,9 @@ __device__ __forceinline__ T strain_offdiag(const FA& A, const FB& B, T ra0, T r
 
 // Two blocks fit a CU's LDS; the register budget is held to the same two
 // waves per SIMD.
-template <typename T, int TXO, int TYO>
+template <typename T, int TXO, int TYO, bool G>
 __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_waves_per_eu(2)))
-    k_dsm_fused(const T* __restrict__ U, const T* __restrict__ V, const T* __restrict__ W, Dims d, DsmBufs<T> b,
-                int kchunk)
+    k_dsm_fused(DsmArgs<T> one, const DsmArgs<T>* tab, Dims d, int kchunk)
 {
     // The model's sums are reordered against the reference anyway (separable
     // filter, per-row partials); contracting a*b+c into FMAs here saves a
@@ -91,7 +90,13 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
 
     const int tid = threadIdx.x;
     const int cx = tid % CX, cy = tid / CX;
-    const TileId tile = xcd_tile();
+    const GBlk gb = gblk_xcd((d.nzg - 2 + kchunk - 1) / kchunk);
+    const DsmArgs<T> arg = member_args<G>(one, tab, gb.m);
+    const T* __restrict__ U = arg.U;
+    const T* __restrict__ V = arg.V;
+    const T* __restrict__ W = arg.W;
+    const DsmBufs<T> b = arg.b;
+    const TileId tile{gb.x, gb.y, gb.z};
     const int i0 = tile.x * TXO + 2, j0 = tile.y * TYO + 2;
     const int kb = tile.z * kchunk + 2;
     const int ke = min(kb + kchunk - 1, d.nzg - 1);
@@ -417,7 +422,7 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
                 sl += red[c + CX * cyy];
                 sm2 += red[NC + c + CX * cyy];
             }
-            const int blk = tile.z * gridDim.x + tile.x;
+            const int blk = tile.z * gb.gx + tile.x;
             b.part[((long)(j - 1) * 2 + 0) * b.nblk + blk] = sl;
             b.part[((long)(j - 1) * 2 + 1) * b.nblk + blk] = sm2;
         }
@@ -442,7 +447,7 @@ int dsm_fused_kchunk(const Dims& d, int device)
 {
     int per_cu = 0, ncu = 0;
     constexpr int TYO = DsmRows<T>::TYO;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dsm_fused<T, DSM_TXO, TYO>, (DSM_TXO + 2) * (TYO + 2),
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dsm_fused<T, DSM_TXO, TYO, false>, (DSM_TXO + 2) * (TYO + 2),
                                                      0) != hipSuccess ||
         per_cu < 1)
         per_cu = 1;
@@ -457,16 +462,23 @@ int dsm_fused_nblk(const Dims& d, int kchunk)
 }
 
 template <typename T>
+void run_dsm_fused(const DsmArgs<T>& one, const DsmArgs<T>* tab, int nmem, const Dims& d, int kchunk, hipStream_t s)
+{
+    constexpr int TYO = DsmRows<T>::TYO;
+    dim3 blk((DSM_TXO + 2) * (TYO + 2));
+    dim3 grd(cdiv(d.nxg - 2, DSM_TXO), cdiv(d.nyg - 2, TYO), (unsigned)cdiv(d.nzg - 2, kchunk) * nmem);
+    PROF_BEGIN("k_dsm_fused", s);
+    if (tab) k_dsm_fused<T, DSM_TXO, TYO, true><<<grd, blk, 0, s>>>(one, tab, d, kchunk);
+    else k_dsm_fused<T, DSM_TXO, TYO, false><<<grd, blk, 0, s>>>(one, tab, d, kchunk);
+    PROF_END(s);
+}
+
+template <typename T>
 void launch_dsm_fused(const T* U, const T* V, const T* W, const Metric<T>& m, const Dims& d, DsmBufs<T>& b,
                       hipStream_t s)
 {
     (void)m;
-    constexpr int TYO = DsmRows<T>::TYO;
-    dim3 blk((DSM_TXO + 2) * (TYO + 2));
-    dim3 grd(cdiv(d.nxg - 2, DSM_TXO), cdiv(d.nyg - 2, TYO), cdiv(d.nzg - 2, b.kchunk));
-    PROF_BEGIN("k_dsm_fused", s);
-    k_dsm_fused<T, DSM_TXO, TYO><<<grd, blk, 0, s>>>(U, V, W, d, b, b.kchunk);
-    PROF_END(s);
+    run_dsm_fused<T>(DsmArgs<T>{U, V, W, b}, nullptr, 1, d, b.kchunk, s);
 }
 
 template int dsm_fused_kchunk<double>(const Dims&, int);
@@ -475,5 +487,8 @@ template void launch_dsm_fused<double>(const double*, const double*, const doubl
                                        const Dims&, DsmBufs<double>&, hipStream_t);
 template void launch_dsm_fused<float>(const float*, const float*, const float*, const Metric<float>&, const Dims&,
                                       DsmBufs<float>&, hipStream_t);
+template void run_dsm_fused<double>(const DsmArgs<double>&, const DsmArgs<double>*, int, const Dims&, int,
+                                    hipStream_t);
+template void run_dsm_fused<float>(const DsmArgs<float>&, const DsmArgs<float>*, int, const Dims&, int, hipStream_t);
 
 }  // namespace chan

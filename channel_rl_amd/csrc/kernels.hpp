@@ -75,6 +75,95 @@ template <typename T>
 void launch_equilibrium_alpha(const T* U, double* alpha_x, const Dims& d, double nu, const WallRows& ws,
                               double kappa, double B, hipStream_t s);
 
+// ------------------------------------------------------ grouped launches
+// One launch per stage for a group of same-shape channels: each run_*
+// launcher takes one channel's arguments by value (tab = nullptr, nmem = 1)
+// or a device table of nmem members' arguments; the other parameters are
+// the group's common shape and launch settings.
+struct DsmRatioArgs {
+    const double* part;
+    double* ratio;
+};
+template <typename T>
+struct DsmNutArgs {
+    T* nut;
+    const T* smag;
+    const double* ratio;
+    const T* U;
+    const double* alpha;
+    Metric<T> m;
+    T nu;
+    int dirichlet;
+};
+template <typename T>
+struct EqAlphaArgs {
+    const T* U;
+    double* alpha;
+    double nu;
+    WallRows ws;
+    double kappa, B, yc;
+};
+template <typename T>
+struct MassArgs {
+    T* U;
+    double* part;
+    Scalars* sc;
+};
+template <typename T>
+struct DtArgs {
+    const T *U, *V, *W;
+    Metric<double> m;
+    double* part;
+    Scalars* sc;
+    const double* mins3;
+};
+template <typename T>
+struct DsmArgs {
+    const T *U, *V, *W;
+    DsmBufs<T> b;
+};
+template <typename T>
+struct RhsArgs {
+    const T *U, *V, *W, *nt;
+    RhsTabs<T> tb;
+    T nu;
+    const Scalars* sc;
+    const T *Uo, *Vo, *Wo;
+    T *F1u, *F1v, *F1w, *Ou, *Ov, *Ow;
+};
+template <typename T>
+struct WmArgs {
+    const float *p, *norm;  // the network's parameters and standardisation
+    const T *U, *W;
+    double* alpha;
+    const Scalars* sc;
+    Metric<double> m;
+    double nu, h_wm, kappa, B, yc, max_err;
+    double* stats;
+    int x_mass_cte;
+};
+
+void run_dsm_ratio(const DsmRatioArgs& one, const DsmRatioArgs* tab, int nmem, const Dims& d, int nblk,
+                   hipStream_t s);
+template <typename T>
+void run_dsm_nut(const DsmNutArgs<T>& one, const DsmNutArgs<T>* tab, int nmem, const Dims& d, int fuse_wm,
+                 hipStream_t s);
+template <typename T>
+void run_equilibrium_alpha(const EqAlphaArgs<T>& one, const EqAlphaArgs<T>* tab, int nmem, const Dims& d,
+                           hipStream_t s);
+template <typename T>
+void run_mass_flow_correct(const MassArgs<T>& one, const MassArgs<T>* tab, int nmem, const Dims& d, int nblk,
+                           int apply, int advance, hipStream_t s);
+template <typename T>
+void run_dt(const DtArgs<T>& one, const DtArgs<T>* tab, int nmem, const Dims& d, int nblk, hipStream_t s);
+template <typename T>
+void run_dsm_fused(const DsmArgs<T>& one, const DsmArgs<T>* tab, int nmem, const Dims& d, int kchunk, hipStream_t s);
+template <typename T>
+void run_rhs_rk(const RhsArgs<T>& one, const RhsArgs<T>* tab, int nmem, const Dims& d, int mode, double c1,
+                double c2, int shape, hipStream_t s);
+template <typename T>
+void run_wm_mlp(const WmArgs<T>& one, const WmArgs<T>* tab, int nmem, const Dims& d, hipStream_t s);
+
 template <typename T>
 void launch_mass_flow_init(const T* U, const Metric<double>& md, const Dims& d, double* part, int nblk,
                            Scalars* sc, hipStream_t s);
@@ -136,6 +225,30 @@ struct PoissonBufs {
     int bx, bz;         // transforms per block in the x and z passes
     double* tri;        // pivots 1/den_j of the y elimination, [row][mode] (tall systems only)
 };
+
+template <typename T>
+struct FwdArgs {
+    const T *U, *V, *W;
+    Metric<double> m;
+    PoissonBufs pb;
+};
+template <typename T>
+struct ProjArgs {
+    T *U, *V, *W;
+    const double* p;
+    Metric<double> m;
+};
+// grouped forms of the projection's stages.  pb0 (forward) and `one`
+// (solve, inverse) are any member's buffers in a grouped launch: the group's
+// transform plans and batch sizes come from there, the members' pointers
+// from tab (tab_pb for the z pass of the forward transform).
+template <typename T>
+void run_poisson_forward(const FwdArgs<T>& one, const FwdArgs<T>* tab, const PoissonBufs* tab_pb, int nmem,
+                         const Dims& d, const PoissonBufs& pb0, hipStream_t s);
+void run_poisson_solve(const PoissonBufs& one, const PoissonBufs* tab, int nmem, const Dims& d, hipStream_t s);
+void run_poisson_inverse(const PoissonBufs& one, const PoissonBufs* tab, int nmem, const Dims& d, hipStream_t s);
+template <typename T>
+void run_project(const ProjArgs<T>& one, const ProjArgs<T>* tab, int nmem, const Dims& d, hipStream_t s);
 
 template <typename T>
 void launch_poisson_forward(const T* U, const T* V, const T* W, const Metric<double>& md, const Dims& d,

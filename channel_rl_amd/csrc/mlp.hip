@@ -366,12 +366,20 @@ __global__ void k_wm_samples(const T* U, const T* W, const double* alpha, const 
 // equilibrium value.  Writes
 // alpha_x and, per 16-cell group, the sum of relative errors and the number
 // of fallbacks.
-template <typename T>
-__global__ void __launch_bounds__(256)
-    k_wm_mlp(const float* __restrict__ p, const float* __restrict__ norm, const T* U, const T* W, double* alpha,
-             const Scalars* sc, Metric<double> m, Dims d, double nu, double h_wm, double kappa, double B, double yc,
-             int x_mass_cte, double max_err, double* stats)
+template <typename T, bool G>
+__global__ void __launch_bounds__(256) k_wm_mlp(WmArgs<T> one, const WmArgs<T>* tab, Dims d)
 {
+    const GBlk gb = gblk(1);
+    const WmArgs<T> arg = member_args<G>(one, tab, gb.m);
+    const float* __restrict__ p = arg.p;
+    const float* __restrict__ norm = arg.norm;
+    const T *U = arg.U, *W = arg.W;
+    double* alpha = arg.alpha;
+    const Scalars* sc = arg.sc;
+    const Metric<double>& m = arg.m;
+    const double nu = arg.nu, h_wm = arg.h_wm, kappa = arg.kappa, B = arg.B, yc = arg.yc, max_err = arg.max_err;
+    double* stats = arg.stats;
+    const int x_mass_cte = arg.x_mass_cte;
     __shared__ MlpLds s;
     mlp_load_lds(s, p, norm);
     __syncthreads();
@@ -379,7 +387,7 @@ __global__ void __launch_bounds__(256)
     const long ncell = 2L * d.nx * d.nzg;
     const long ngroups = (ncell + 15) / 16;
     const double dpdx = wm_driving_dpdx(sc, x_mass_cte);
-    for (long grp = (long)blockIdx.x * 4 + wave; grp < ngroups; grp += (long)gridDim.x * 4) {
+    for (long grp = (long)gb.x * 4 + wave; grp < ngroups; grp += (long)gb.gx * 4) {
         lds_reload_fence();
         const long c = grp * 16 + lane % 16;
         const bool ok = c < ncell;
@@ -448,16 +456,24 @@ void launch_wm_samples(const T* U, const T* W, const double* alpha, const Scalar
 long wm_mlp_groups(const Dims& d) { return (2L * d.nx * d.nzg + 15) / 16; }
 
 template <typename T>
-void launch_wm_mlp(const float* p, const float* norm, const T* U, const T* W, double* alpha, const Scalars* sc,
-                   const Metric<double>& m, const Dims& d, double nu, double h_wm, double kappa, double B,
-                   int x_mass_cte, double max_err, double* stats, hipStream_t s)
+void run_wm_mlp(const WmArgs<T>& one, const WmArgs<T>* tab, int nmem, const Dims& d, hipStream_t s)
 {
     const long groups = wm_mlp_groups(d);
     const int blocks = (int)std::max(1L, std::min(2048L, (groups + 3) / 4));
     PROF_BEGIN("k_wm_mlp", s);
-    k_wm_mlp<T><<<blocks, 256, 0, s>>>(p, norm, U, W, alpha, sc, m, d, nu, h_wm, kappa, B,
-                                       loglaw_crossover(kappa, B), x_mass_cte, max_err, stats);
+    if (tab) k_wm_mlp<T, true><<<dim3(blocks, 1, nmem), 256, 0, s>>>(one, tab, d);
+    else k_wm_mlp<T, false><<<dim3(blocks, 1, 1), 256, 0, s>>>(one, nullptr, d);
     PROF_END(s);
+}
+
+template <typename T>
+void launch_wm_mlp(const float* p, const float* norm, const T* U, const T* W, double* alpha, const Scalars* sc,
+                   const Metric<double>& m, const Dims& d, double nu, double h_wm, double kappa, double B,
+                   int x_mass_cte, double max_err, double* stats, hipStream_t s)
+{
+    run_wm_mlp<T>(WmArgs<T>{p, norm, U, W, alpha, sc, m, nu, h_wm, kappa, B, loglaw_crossover(kappa, B), max_err, stats,
+                            x_mass_cte},
+                  nullptr, 1, d, s);
 }
 
 template void launch_wm_samples<double>(const double*, const double*, const double*, const Scalars*,
@@ -466,6 +482,8 @@ template void launch_wm_samples<double>(const double*, const double*, const doub
 template void launch_wm_samples<float>(const float*, const float*, const double*, const Scalars*,
                                        const Metric<double>&, const Dims&, double, double, double, double, int, float*,
                                        float*, double*, hipStream_t);
+template void run_wm_mlp<double>(const WmArgs<double>&, const WmArgs<double>*, int, const Dims&, hipStream_t);
+template void run_wm_mlp<float>(const WmArgs<float>&, const WmArgs<float>*, int, const Dims&, hipStream_t);
 template void launch_wm_mlp<double>(const float*, const float*, const double*, const double*, double*,
                                     const Scalars*, const Metric<double>&, const Dims&, double, double, double,
                                     double, int, double, double*, hipStream_t);

@@ -151,9 +151,9 @@ __device__ void block_fft(double2* a, const double2* tw, const FftPlan& pl, int 
     }
 }
 
-__device__ __forceinline__ void load_twiddles(double2* tw, const FftPlan& pl)
+__device__ __forceinline__ void load_twiddles(double2* tw, const double2* src, int n)
 {
-    for (int q = threadIdx.x; q < pl.n; q += blockDim.x) tw[q] = pl.tw[q];
+    for (int q = threadIdx.x; q < n; q += blockDim.x) tw[q] = src[q];
 }
 
 // pair pr -> (row jr = j-2, first z index kA-2) of the two packed z-rows
@@ -193,22 +193,30 @@ __device__ __forceinline__ void fill_pairs(PairTab& t, const Dims& d, const Metr
     }
 }
 
-template <typename T, bool FROM_FIELD>
-__global__ void __launch_bounds__(FFT_THREADS) k_fftx_fwd(const T* U, const T* V, const T* W, const double* rhs,
-                                                          Metric<double> m, Dims d, PoissonBufs pb)
+template <typename T, bool FROM_FIELD, bool G>
+__global__ void __launch_bounds__(FFT_THREADS) k_fftx_fwd(FwdArgs<T> one, const FwdArgs<T>* tab, const double* rhs,
+                                                          Dims d)
 {
+    // consecutive blocks share V rows (row j-1 of one is row j of a block
+    // nzp/2B before it): XCD-contiguous order keeps both in one L2
+    const GBlk gb = gblk_xcd(1);
+    // the group's plans and batch sizes from the launch's arguments, this
+    // member's buffers from its table entry
+    const FwdArgs<T> arg = member_args<G>(one, tab, gb.m);
+    const T *U = arg.U, *V = arg.V, *W = arg.W;
+    const Metric<double>& m = arg.m;
+    const PoissonBufs& pb = one.pb;
+    double2* const ph = arg.pb.ph;
     extern __shared__ double2 lds[];
     const int N = d.nxp, B = pb.bx, ld = N + 1;
     const Quo16 qN = quo16(N), qB = quo16(B);
     double2* tw = lds;
     double2* a = lds + N;
     const int npairs = (d.ny - 1) * (d.nzp / 2);
-    // consecutive blocks share V rows (row j-1 of one is row j of a block
-    // nzp/2B before it): XCD-contiguous order keeps both in one L2
-    const int pair0 = xcd_linear() * B;
+    const int pair0 = gb.x * B;
     __shared__ PairTab pt;
     fill_pairs(pt, d, FROM_FIELD ? nullptr : &m, pair0, B, npairs);
-    load_twiddles(tw, pb.fx);
+    load_twiddles(tw, arg.pb.fx.tw, N);
     __syncthreads();
     const long sx = d.sx, sxy = d.sxy;
     for (int q = threadIdx.x; q < N * B; q += blockDim.x) {
@@ -241,7 +249,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_fwd(const T* U, const T* V
         const double2 Z = a[b * ld + kx];
         const double2 Zc = cconj(a[b * ld + ((N - kx) % N)]);
         const double2 s = cadd(Z, Zc), df = csub(Z, Zc);
-        double2* out = pb.ph + pt.spec[b] + (long)kx * d.nzp;
+        double2* out = ph + pt.spec[b] + (long)kx * d.nzp;
         out[0] = make_double2(0.5 * s.x, 0.5 * s.y);
         out[1] = make_double2(0.5 * df.y, -0.5 * df.x);
     }
@@ -352,19 +360,22 @@ __device__ __forceinline__ void fft_last_pass_r(int R, const double2* a, const d
     else fft_last_pass<5, SIGN>(a, tw, N, B, ld, store);
 }
 
-template <int SIGN>
-__global__ void __launch_bounds__(FFT_THREADS) k_fftz(Dims d, PoissonBufs pb)
+template <int SIGN, bool G>
+__global__ void __launch_bounds__(FFT_THREADS) k_fftz(PoissonBufs one, const PoissonBufs* tab, Dims d)
 {
+    const GBlk gb = gblk(1);
+    const PoissonBufs& pb = one;  // the group's plans and batch sizes
+    const PoissonBufs mb = member_args<G>(one, tab, gb.m);  // this member's buffers
     extern __shared__ double2 lds[];
     const FftPlan& pl = pb.fz;
     const int N = d.nzp, B = pb.bz, ld = N + 1;
     double2* tw = lds;
     double2* a = lds + N;
     const long nlines = (long)(d.ny - 1) * d.nkx;
-    const long line0 = (long)blockIdx.x * B;
+    const long line0 = (long)gb.x * B;
     const long nrows = nlines - line0;  // lines of this block that exist (>= 1)
-    double2* ln = pb.ph + line0 * N;
-    load_twiddles(tw, pl);
+    double2* ln = mb.ph + line0 * N;
+    load_twiddles(tw, mb.fz.tw, N);
     if (pl.nr >= 2) {
         // the first and the last pass run between registers and global
         // memory, the others in LDS: two LDS sweeps and four barriers fewer
@@ -455,10 +466,12 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_setup(Dims d, PoissonBu
 // values by selection and re-store the half's last row with its own value.
 // (An empty asm on each new value keeps the compiler from moving its
 // computation into a branch of the selection.)
-template <int H>
-__global__ void __launch_bounds__(TRI_THREADS) k_tridiag_tw(Dims d, PoissonBufs pb)
+template <int H, bool G>
+__global__ void __launch_bounds__(TRI_THREADS) k_tridiag_tw(PoissonBufs one, const PoissonBufs* tab, Dims d)
 {
-    const long g = (long)blockIdx.x * TRI_THREADS + threadIdx.x;
+    const GBlk gb = gblk(1);
+    const PoissonBufs pb = member_args<G>(one, tab, gb.m);
+    const long g = (long)gb.x * TRI_THREADS + threadIdx.x;
     const int nmodes = d.nkx * d.nzp;
     const int n = d.ny - 1;
     const int hn = (n + 1) / 2;
@@ -551,10 +564,12 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_tw(Dims d, PoissonBufs 
 
 // KEEP register rows; a system of more than TRI_KEEP rows keeps its last 32
 // in registers (four waves per SIMD) and STREAMS the rows before them
-template <bool STREAMED, int KEEP>
-__global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
+template <bool STREAMED, int KEEP, bool GRP>
+__global__ void __launch_bounds__(TRI_THREADS) k_tridiag(PoissonBufs one, const PoissonBufs* tab, Dims d)
 {
-    const long g = (long)blockIdx.x * TRI_THREADS + threadIdx.x;
+    const GBlk gb = gblk(1);
+    const PoissonBufs pb = member_args<GRP>(one, tab, gb.m);
+    const long g = (long)gb.x * TRI_THREADS + threadIdx.x;
     const int nmodes = d.nkx * d.nzp;
     const int n = d.ny - 1;
     const int nk = n < KEEP ? n : KEEP;  // rows n0..n-1 stay in registers
@@ -683,19 +698,24 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag(Dims d, PoissonBufs pb)
     }
 }
 
-__global__ void __launch_bounds__(FFT_THREADS) k_fftx_inv(Dims d, PoissonBufs pb)
+template <bool G>
+__global__ void __launch_bounds__(FFT_THREADS) k_fftx_inv(PoissonBufs one, const PoissonBufs* tab, Dims d)
 {
+    const GBlk gb = gblk(1);
+    const PoissonBufs& pb = one;  // the group's plans and batch sizes
+    const PoissonBufs mb = member_args<G>(one, tab, gb.m);  // this member's buffers
+    double* const P = mb.p;
     extern __shared__ double2 lds[];
     const int N = d.nxp, B = pb.bx, ld = N + 1;
-    const Quo16 qN = quo16(N), qB = quo16(B);
+    const Quo16 qN = quo16(N);
     double2* tw = lds;
     double2* a = lds + N;
     const int npairs = (d.ny - 1) * (d.nzp / 2);
-    const int pair0 = blockIdx.x * B;
+    const int pair0 = gb.x * B;
     const int half = N / 2;
     __shared__ PairTab pt;
     fill_pairs(pt, d, nullptr, pair0, B, npairs);
-    load_twiddles(tw, pb.fx);
+    load_twiddles(tw, mb.fx.tw, N);
     __syncthreads();
     // the unnormalised inverse transforms' factor, as a reciprocal (within an
     // ulp of the division)
@@ -706,7 +726,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_inv(Dims d, PoissonBufs pb
     // kx line (BFAST); the loads unconditional, from a clamped pair.
     auto elem = [&](bool ok, int b, int n) {
         const int mode = n <= half ? n : N - n;
-        const double2* in = pb.ph + pt.spec[b] + (long)mode * d.nzp;
+        const double2* in = mb.ph + pt.spec[b] + (long)mode * d.nzp;
         double2 xa = in[0], xb = in[1];
         if (n == 0 || n == half) {
             xa.y = 0.0;
@@ -727,15 +747,15 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_inv(Dims d, PoissonBufs pb
         if (base < 0) return;
         const long p = base + n + 1, pi = base + d.nxg - 2;  // x centres n+2 and nxg-1
         const double pa = val.x * rscale, pbv = val.y * rscale;
-        pb.p[p] = pa;
-        pb.p[p + sxy] = pbv;
+        P[p] = pa;
+        P[p + sxy] = pbv;
         if (n == 0) {
-            pb.p[pi] = pa;
-            pb.p[pi + sxy] = pbv;
+            P[pi] = pa;
+            P[pi + sxy] = pbv;
         }
         if (pt.zfirst[b]) {
-            pb.p[p + zimg] = pa;
-            if (n == 0) pb.p[pi + zimg] = pa;
+            P[p + zimg] = pa;
+            if (n == 0) P[pi + zimg] = pa;
         }
     };
     if (pb.fx.nr >= 2) {
@@ -761,30 +781,46 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftx_inv(Dims d, PoissonBufs pb
 static size_t fft_lds(int N, int B) { return (size_t)(N + (size_t)(N + 1) * B) * sizeof(double2); }
 
 template <typename T>
-void launch_poisson_forward(const T* U, const T* V, const T* W, const Metric<double>& md, const Dims& d,
-                            PoissonBufs& pb, hipStream_t s)
+void run_poisson_forward(const FwdArgs<T>& one, const FwdArgs<T>* tab, const PoissonBufs* tab_pb, int nmem,
+                         const Dims& d, const PoissonBufs& pb0, hipStream_t s)
 {
     const int npairs = (d.ny - 1) * (d.nzp / 2);
     PROF_BEGIN("k_fftx_fwd", s);
-    k_fftx_fwd<T, false><<<cdiv(npairs, pb.bx), FFT_THREADS, fft_lds(d.nxp, pb.bx), s>>>(U, V, W, nullptr, md, d,
-                                                                                          pb);
+    if (tab) {
+        FwdArgs<T> rep{};
+        rep.pb = pb0;
+        k_fftx_fwd<T, false, true><<<dim3(cdiv(npairs, pb0.bx), 1, nmem), FFT_THREADS, fft_lds(d.nxp, pb0.bx), s>>>(
+            rep, tab, nullptr, d);
+    }
+    else
+        k_fftx_fwd<T, false, false><<<dim3(cdiv(npairs, pb0.bx), 1, 1), FFT_THREADS, fft_lds(d.nxp, pb0.bx), s>>>(
+            one, nullptr, nullptr, d);
     PROF_END(s);
+    const dim3 gz(cdiv((long)(d.ny - 1) * d.nkx, pb0.bz), 1, nmem);
     PROF_BEGIN("k_fftz", s);
-    k_fftz<-1><<<cdiv((long)(d.ny - 1) * d.nkx, pb.bz), FFT_THREADS, fft_lds(d.nzp, pb.bz), s>>>(d, pb);
+    if (tab) k_fftz<-1, true><<<gz, FFT_THREADS, fft_lds(d.nzp, pb0.bz), s>>>(pb0, tab_pb, d);
+    else k_fftz<-1, false><<<gz, FFT_THREADS, fft_lds(d.nzp, pb0.bz), s>>>(one.pb, nullptr, d);
     PROF_END(s);
+}
+
+template <typename T>
+void launch_poisson_forward(const T* U, const T* V, const T* W, const Metric<double>& md, const Dims& d,
+                            PoissonBufs& pb, hipStream_t s)
+{
+    run_poisson_forward<T>(FwdArgs<T>{U, V, W, md, pb}, nullptr, nullptr, 1, d, pb, s);
 }
 
 void launch_poisson_forward_field(const double* rhs, const Dims& d, PoissonBufs& pb, hipStream_t s)
 {
     const int npairs = (d.ny - 1) * (d.nzp / 2);
-    Metric<double> none{};
+    FwdArgs<double> one{nullptr, nullptr, nullptr, Metric<double>{}, pb};
     PROF_BEGIN("k_fftx_fwd", s);
-    k_fftx_fwd<double, true><<<cdiv(npairs, pb.bx), FFT_THREADS, fft_lds(d.nxp, pb.bx), s>>>(nullptr, nullptr,
-                                                                                              nullptr, rhs, none, d,
-                                                                                              pb);
+    k_fftx_fwd<double, true, false><<<cdiv(npairs, pb.bx), FFT_THREADS, fft_lds(d.nxp, pb.bx), s>>>(one, nullptr, rhs,
+                                                                                                  d);
     PROF_END(s);
     PROF_BEGIN("k_fftz", s);
-    k_fftz<-1><<<cdiv((long)(d.ny - 1) * d.nkx, pb.bz), FFT_THREADS, fft_lds(d.nzp, pb.bz), s>>>(d, pb);
+    k_fftz<-1, false><<<cdiv((long)(d.ny - 1) * d.nkx, pb.bz), FFT_THREADS, fft_lds(d.nzp, pb.bz), s>>>(pb, nullptr,
+                                                                                                      d);
     PROF_END(s);
 }
 
@@ -795,39 +831,64 @@ void launch_tridiag_setup(const Dims& d, PoissonBufs& pb, hipStream_t s)
     k_tridiag_setup<<<cdiv((long)d.nkx * d.nzp, TRI_THREADS), TRI_THREADS, 0, s>>>(d, pb);
 }
 
-void launch_poisson_solve(const Dims& d, PoissonBufs& pb, hipStream_t s)
+void run_poisson_solve(const PoissonBufs& one, const PoissonBufs* tab, int nmem, const Dims& d, hipStream_t s)
 {
     const long lanes = 2L * d.nkx * d.nzp;
     PROF_BEGIN("k_tridiag", s);
-    const int n = d.ny - 1, nb = cdiv(lanes, TRI_THREADS), nb2 = cdiv(2 * lanes, TRI_THREADS);
-    if (n > TRI_KEEP) k_tridiag<true, 32><<<nb, TRI_THREADS, 0, s>>>(d, pb);
-    else if (n > 64) k_tridiag_tw<48><<<nb2, TRI_THREADS, 0, s>>>(d, pb);
-    else if (n > 32) k_tridiag_tw<32><<<nb2, TRI_THREADS, 0, s>>>(d, pb);
-    else k_tridiag_tw<16><<<nb2, TRI_THREADS, 0, s>>>(d, pb);
+    const int n = d.ny - 1;
+    const dim3 g1(cdiv(lanes, TRI_THREADS), 1, nmem), g2(cdiv(2 * lanes, TRI_THREADS), 1, nmem);
+    if (n > TRI_KEEP) {
+        if (tab) k_tridiag<true, 32, true><<<g1, TRI_THREADS, 0, s>>>(one, tab, d);
+        else k_tridiag<true, 32, false><<<g1, TRI_THREADS, 0, s>>>(one, nullptr, d);
+    } else if (n > 64) {
+        if (tab) k_tridiag_tw<48, true><<<g2, TRI_THREADS, 0, s>>>(one, tab, d);
+        else k_tridiag_tw<48, false><<<g2, TRI_THREADS, 0, s>>>(one, nullptr, d);
+    } else if (n > 32) {
+        if (tab) k_tridiag_tw<32, true><<<g2, TRI_THREADS, 0, s>>>(one, tab, d);
+        else k_tridiag_tw<32, false><<<g2, TRI_THREADS, 0, s>>>(one, nullptr, d);
+    } else {
+        if (tab) k_tridiag_tw<16, true><<<g2, TRI_THREADS, 0, s>>>(one, tab, d);
+        else k_tridiag_tw<16, false><<<g2, TRI_THREADS, 0, s>>>(one, nullptr, d);
+    }
+    PROF_END(s);
+}
+
+void launch_poisson_solve(const Dims& d, PoissonBufs& pb, hipStream_t s) { run_poisson_solve(pb, nullptr, 1, d, s); }
+
+void run_poisson_inverse(const PoissonBufs& one, const PoissonBufs* tab, int nmem, const Dims& d, hipStream_t s)
+{
+    PROF_BEGIN("k_fftz", s);
+    const dim3 gz(cdiv((long)(d.ny - 1) * d.nkx, one.bz), 1, nmem);
+    if (tab) k_fftz<+1, true><<<gz, FFT_THREADS, fft_lds(d.nzp, one.bz), s>>>(one, tab, d);
+    else k_fftz<+1, false><<<gz, FFT_THREADS, fft_lds(d.nzp, one.bz), s>>>(one, nullptr, d);
+    PROF_END(s);
+    const int npairs = (d.ny - 1) * (d.nzp / 2);
+    const dim3 gx(cdiv(npairs, one.bx), 1, nmem);
+    PROF_BEGIN("k_fftx_inv", s);
+    if (tab) k_fftx_inv<true><<<gx, FFT_THREADS, fft_lds(d.nxp, one.bx), s>>>(one, tab, d);
+    else k_fftx_inv<false><<<gx, FFT_THREADS, fft_lds(d.nxp, one.bx), s>>>(one, nullptr, d);
     PROF_END(s);
 }
 
 void launch_poisson_inverse(const Dims& d, PoissonBufs& pb, hipStream_t s)
 {
-    PROF_BEGIN("k_fftz", s);
-    k_fftz<+1><<<cdiv((long)(d.ny - 1) * d.nkx, pb.bz), FFT_THREADS, fft_lds(d.nzp, pb.bz), s>>>(d, pb);
-    PROF_END(s);
-    const int npairs = (d.ny - 1) * (d.nzp / 2);
-    PROF_BEGIN("k_fftx_inv", s);
-    k_fftx_inv<<<cdiv(npairs, pb.bx), FFT_THREADS, fft_lds(d.nxp, pb.bx), s>>>(d, pb);
-    PROF_END(s);
+    run_poisson_inverse(pb, nullptr, 1, d, s);
 }
 
 // code/projection.f90:192-214: u -= grad p on the canonical cells, each
 // stored with its boundary-condition images (the reference applies
 // apply_boundary_conditions right after, code/time_integration.f90:49).
-template <typename T>
-__global__ void __launch_bounds__(256) k_project(T* U, T* V, T* W, const double* __restrict__ p, Metric<double> m,
-                                                  Dims d)
+template <typename T, bool G>
+__global__ void __launch_bounds__(256) k_project(ProjArgs<T> one, const ProjArgs<T>* tab, Dims d)
 {
     // XCD-contiguous tile order: the plane k+1 that a tile reads is the
     // plane k of a tile that runs on the same XCD, so p is fetched once
-    const TileId t = xcd_tile();
+    const GBlk gb = gblk_xcd(d.nzg - 3);
+    const ProjArgs<T> arg = member_args<G>(one, tab, gb.m);
+    T *U = arg.U, *V = arg.V, *W = arg.W;
+    const double* __restrict__ p = arg.p;
+    const Metric<double>& m = arg.m;
+    const TileId t{gb.x, gb.y, gb.z};
     const int i = t.x * blockDim.x + threadIdx.x + 2;
     const int j = t.y * blockDim.y + threadIdx.y + 2;
     const int k = t.z + 2;
@@ -842,18 +903,31 @@ __global__ void __launch_bounds__(256) k_project(T* U, T* V, T* W, const double*
 }
 
 template <typename T>
+void run_project(const ProjArgs<T>& one, const ProjArgs<T>* tab, int nmem, const Dims& d, hipStream_t s)
+{
+    dim3 blk(64, 4, 1), grd(cdiv(d.nxg - 3, 64), cdiv(d.nyg - 2, 4), (unsigned)(d.nzg - 3) * nmem);
+    PROF_BEGIN("k_project", s);
+    if (tab) k_project<T, true><<<grd, blk, 0, s>>>(one, tab, d);
+    else k_project<T, false><<<grd, blk, 0, s>>>(one, nullptr, d);
+    PROF_END(s);
+}
+
+template <typename T>
 void launch_project(T* U, T* V, T* W, const double* p, const Metric<double>& md, const Dims& d, hipStream_t s)
 {
-    dim3 blk(64, 4, 1), grd(cdiv(d.nxg - 3, 64), cdiv(d.nyg - 2, 4), d.nzg - 3);
-    PROF_BEGIN("k_project", s);
-    k_project<T><<<grd, blk, 0, s>>>(U, V, W, p, md, d);
-    PROF_END(s);
+    run_project<T>(ProjArgs<T>{U, V, W, p, md}, nullptr, 1, d, s);
 }
 
 template void launch_poisson_forward<double>(const double*, const double*, const double*, const Metric<double>&,
                                              const Dims&, PoissonBufs&, hipStream_t);
 template void launch_poisson_forward<float>(const float*, const float*, const float*, const Metric<double>&,
                                             const Dims&, PoissonBufs&, hipStream_t);
+template void run_poisson_forward<double>(const FwdArgs<double>&, const FwdArgs<double>*, const PoissonBufs*, int,
+                                          const Dims&, const PoissonBufs&, hipStream_t);
+template void run_poisson_forward<float>(const FwdArgs<float>&, const FwdArgs<float>*, const PoissonBufs*, int,
+                                         const Dims&, const PoissonBufs&, hipStream_t);
+template void run_project<double>(const ProjArgs<double>&, const ProjArgs<double>*, int, const Dims&, hipStream_t);
+template void run_project<float>(const ProjArgs<float>&, const ProjArgs<float>*, int, const Dims&, hipStream_t);
 template void launch_project<double>(double*, double*, double*, const double*, const Metric<double>&, const Dims&,
                                      hipStream_t);
 template void launch_project<float>(float*, float*, float*, const double*, const Metric<double>&, const Dims&,

@@ -163,11 +163,12 @@ __device__ __forceinline__ T rhs_w_gen(const A& F, const RhsXY<T>& g, const RhsZ
 // stencil as well.
 static constexpr int RHS_KMAX = 32;  // longest z chunk a block marches
 
-template <typename T, int TX, int TY>
+// Register budget: three waves per SIMD (168 VGPRs) in FP64 (two for the
+// 512-thread tile, which one block per CU holds by its LDS), four in FP32.
+template <typename T, int TX, int TY, bool G>
 __global__ void __launch_bounds__(TX* TY)
-    k_rhs_rk(const T* __restrict__ U, const T* __restrict__ V, const T* __restrict__ W, const T* __restrict__ nt,
-             RhsTabs<T> tb, Dims d, T nu, const Scalars* sc, int mode, const T* Uo, const T* Vo, const T* Wo, T* F1u,
-             T* F1v, T* F1w, T* Ou, T* Ov, T* Ow, T c1, T c2, int kchunk)
+    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (TX * TY > 256 ? 2 : 3) : 4)))
+    k_rhs_rk(RhsArgs<T> one, const RhsArgs<T>* tab, Dims d, int mode, T c1, T c2, int kchunk)
 {
     constexpr int PX = TX + 2, PY = TY + 2, NP = PX * PY, NT = TX * TY, NL = (NP + NT - 1) / NT;
     __shared__ T sh[4][4][NP];  // [field][slot][position]
@@ -177,10 +178,21 @@ __global__ void __launch_bounds__(TX* TY)
     constexpr int KZ = RHS_KMAX + 2;
     __shared__ T zt[2][KZ];  // 1/(z(k)-z(k-1)), 1/(zg(k)-zg(k-1)) at k = kb + q
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-    const TileId tile = xcd_tile();
-    const int i0 = tile.x * TX + 2, j0 = tile.y * TY + 2;
     const int imax = mode == 0 ? d.nxg - 1 : d.nxg - 2;
     const int kmax = mode == 0 ? d.nzg - 1 : d.nzg - 2;
+    const GBlk gb = gblk_xcd((kmax - 1 + kchunk - 1) / kchunk);
+    const RhsArgs<T> arg = member_args<G>(one, tab, gb.m);
+    const T* __restrict__ U = arg.U;
+    const T* __restrict__ V = arg.V;
+    const T* __restrict__ W = arg.W;
+    const T* __restrict__ nt = arg.nt;
+    const RhsTabs<T>& tb = arg.tb;
+    const T nu = arg.nu;
+    const Scalars* sc = arg.sc;
+    const T *Uo = arg.Uo, *Vo = arg.Vo, *Wo = arg.Wo;
+    T *F1u = arg.F1u, *F1v = arg.F1v, *F1w = arg.F1w, *Ou = arg.Ou, *Ov = arg.Ov, *Ow = arg.Ow;
+    const TileId tile{gb.x, gb.y, gb.z};
+    const int i0 = tile.x * TX + 2, j0 = tile.y * TY + 2;
     const int kb = tile.z * kchunk + 2;
     const int ke = min(kb + kchunk - 1, kmax);
     const int i = i0 + tx, j = j0 + ty;
@@ -308,7 +320,7 @@ template <typename T, int TX, int TY>
 int rhs_round_chunk(int ni, int nj, int nk)
 {
     int per_cu = 0, ncu = 0, dev = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rhs_rk<T, TX, TY>, TX * TY, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rhs_rk<T, TX, TY, false>, TX * TY, 0) != hipSuccess ||
         per_cu < 1)
         per_cu = 1;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -319,9 +331,8 @@ int rhs_round_chunk(int ni, int nj, int nk)
 int rhs_shape_count() { return (int)(sizeof(kRhsShapes) / sizeof(kRhsShapes[0])); }
 
 template <typename T>
-void launch_rhs_rk(const T* U, const T* V, const T* W, const T* nut, const RhsTabs<T>& m, const Dims& d, T nu,
-                   const Scalars* sc, int mode, const T* Uo, const T* Vo, const T* Wo, T* F1u, T* F1v, T* F1w,
-                   T* Ou, T* Ov, T* Ow, double c1, double c2, int shape, hipStream_t s)
+void run_rhs_rk(const RhsArgs<T>& one, const RhsArgs<T>* tab, int nmem, const Dims& d, int mode, double c1,
+                double c2, int shape, hipStream_t s)
 {
     const RhsShape sh = kRhsShapes[(shape >= 0 && shape < rhs_shape_count()) ? shape : 0];
     const int ni = (mode == 0 ? d.nxg - 1 : d.nxg - 2) - 1;
@@ -331,9 +342,9 @@ void launch_rhs_rk(const T* U, const T* V, const T* W, const T* nut, const RhsTa
 #define RHS_GO(TX, TY)                                                                                              \
     do {                                                                                                            \
         const int kc = std::min(RHS_KMAX, sh.kchunk > 0 ? sh.kchunk : rhs_round_chunk<T, TX, TY>(ni, d.nyg - 2, nk)); \
-        dim3 grd(cdiv(ni, TX), cdiv(d.nyg - 2, TY), cdiv(nk, kc));                                                  \
-        k_rhs_rk<T, TX, TY><<<grd, blk, 0, s>>>(U, V, W, nut, m, d, nu, sc, mode, Uo, Vo, Wo, F1u, F1v, F1w, Ou, Ov, \
-                                                Ow, T(c1), T(c2), kc);                                              \
+        dim3 grd(cdiv(ni, TX), cdiv(d.nyg - 2, TY), (unsigned)cdiv(nk, kc) * nmem);                                 \
+        if (tab) k_rhs_rk<T, TX, TY, true><<<grd, blk, 0, s>>>(one, tab, d, mode, T(c1), T(c2), kc);                \
+        else k_rhs_rk<T, TX, TY, false><<<grd, blk, 0, s>>>(one, tab, d, mode, T(c1), T(c2), kc);                   \
     } while (0)
     if (sh.tx == 64 && sh.ty == 4) RHS_GO(64, 4);
     else if (sh.tx == 64 && sh.ty == 8) RHS_GO(64, 8);
@@ -342,10 +353,23 @@ void launch_rhs_rk(const T* U, const T* V, const T* W, const T* nut, const RhsTa
     PROF_END(s);
 }
 
+template <typename T>
+void launch_rhs_rk(const T* U, const T* V, const T* W, const T* nut, const RhsTabs<T>& m, const Dims& d, T nu,
+                   const Scalars* sc, int mode, const T* Uo, const T* Vo, const T* Wo, T* F1u, T* F1v, T* F1w,
+                   T* Ou, T* Ov, T* Ow, double c1, double c2, int shape, hipStream_t s)
+{
+    run_rhs_rk<T>(RhsArgs<T>{U, V, W, nut, m, nu, sc, Uo, Vo, Wo, F1u, F1v, F1w, Ou, Ov, Ow}, nullptr, 1, d, mode, c1,
+                  c2, shape, s);
+}
+
 template void launch_rhs_rk<double>(const double*, const double*, const double*, const double*,
                                     const RhsTabs<double>&, const Dims&, double, const Scalars*, int, const double*,
                                     const double*, const double*, double*, double*, double*, double*, double*,
                                     double*, double, double, int, hipStream_t);
+template void run_rhs_rk<double>(const RhsArgs<double>&, const RhsArgs<double>*, int, const Dims&, int, double,
+                                 double, int, hipStream_t);
+template void run_rhs_rk<float>(const RhsArgs<float>&, const RhsArgs<float>*, int, const Dims&, int, double, double,
+                                int, hipStream_t);
 template void launch_rhs_rk<float>(const float*, const float*, const float*, const float*, const RhsTabs<float>&,
                                    const Dims&, float, const Scalars*, int, const float*, const float*, const float*,
                                    float*, float*, float*, float*, float*, float*, double, double, int, hipStream_t);

@@ -82,9 +82,13 @@ void launch_bc(T* F, const Dims& d, int kind, hipStream_t s)
 // <L:M>/<M:M> per row: one wave per row, the blocks' partials summed in a
 // fixed order (strided per lane, then a butterfly across the wave), so the
 // result does not depend on scheduling; no LDS and no barrier.
-__global__ void __launch_bounds__(64) k_dsm_ratio(const double* part, int nblk, int nyg, double* ratio)
+__global__ void __launch_bounds__(64) k_dsm_ratio(DsmRatioArgs one, const DsmRatioArgs* tab, int nblk, int nyg)
 {
-    const int j = blockIdx.x + 2;
+    const GBlk g = gblk(1);
+    const DsmRatioArgs ar = member_args(one, tab, g.m);
+    const double* part = ar.part;
+    double* ratio = ar.ratio;
+    const int j = g.x + 2;
     const int t = threadIdx.x;
     double a = 0.0, c = 0.0;
     for (int q = t; q < nblk; q += 64) {
@@ -106,23 +110,48 @@ __global__ void __launch_bounds__(64) k_dsm_ratio(const double* part, int nblk, 
 // Dirichlet_nu_t; with fuse_wm the wall rows come from the wall model
 // (code/wallmodel.f90:39-42), which the reference applies right after.
 template <typename T>
-__global__ void __launch_bounds__(256) k_dsm_nut(T* nut, Dims d, DsmBufs<T> b, int dirichlet, int fuse_wm,
-                                                  const T* U, const double* alpha, Metric<T> m, T nu)
+__global__ void __launch_bounds__(256) k_dsm_nut(DsmNutArgs<T> one, const DsmNutArgs<T>* tab, Dims d, int fuse_wm)
 {
-    // one (x, y) plane per blockIdx.y, flattened so that every lane has a cell
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const GBlk g = gblk(d.nzg);
+    const DsmNutArgs<T> a = member_args(one, tab, g.m);
+    T* nut = a.nut;
+    const T* U = a.U;
+    const double* alpha = a.alpha;
+    const Metric<T>& m = a.m;
+    const T nu = a.nu;
+    const int dirichlet = a.dirichlet;
+    // one (x, y) plane per block z, flattened so that every lane has a cell
+    const int q = g.x * blockDim.x + threadIdx.x;
     if (q >= d.nxg * d.nyg) return;
     const int jm = q / d.nxg;
-    const int i = q - jm * d.nxg + 1, j = jm + 1, k = blockIdx.y + 1;
+    const int i = q - jm * d.nxg + 1, j = jm + 1, k = g.z + 1;
     const long p = at(d, i, j, k);
     if (fuse_wm && (j <= 2 || j >= d.nyg - 1)) {
         nut[p] = wm_nut_at<T>(U, alpha, m, d, nu, img_centre(i, d.nxg), k, j <= 2 ? 0 : 1);
         return;
     }
     const int jj = (j == 1) ? 2 : (j == d.nyg ? d.nyg - 1 : j);
-    const T r = T(b.ratio[jj - 1]);
-    const T v = fmax(T(-0.5) * r * b.smag[at(d, img_centre(i, d.nxg), jj, img_centre(k, d.nzg))], T(0));
+    const T r = T(a.ratio[jj - 1]);
+    const T v = fmax(T(-0.5) * r * a.smag[at(d, img_centre(i, d.nxg), jj, img_centre(k, d.nzg))], T(0));
     nut[p] = (jj != j && dirichlet) ? -v : v;
+}
+
+void run_dsm_ratio(const DsmRatioArgs& one, const DsmRatioArgs* tab, int nmem, const Dims& d, int nblk,
+                   hipStream_t s)
+{
+    PROF_BEGIN("k_dsm_ratio", s);
+    k_dsm_ratio<<<dim3(d.nyg - 2, 1, nmem), 64, 0, s>>>(one, tab, nblk, d.nyg);
+    PROF_END(s);
+}
+
+template <typename T>
+void run_dsm_nut(const DsmNutArgs<T>& one, const DsmNutArgs<T>* tab, int nmem, const Dims& d, int fuse_wm,
+                 hipStream_t s)
+{
+    dim3 grd(cdiv((long)d.nxg * d.nyg, 256), 1, (unsigned)d.nzg * nmem);
+    PROF_BEGIN("k_dsm_nut", s);
+    k_dsm_nut<T><<<grd, 256, 0, s>>>(one, tab, d, fuse_wm);
+    PROF_END(s);
 }
 
 template <typename T>
@@ -130,15 +159,8 @@ void launch_dsm(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, 
                 int dirichlet, int fuse_wm, const double* alpha_x, T nu, hipStream_t s)
 {
     launch_dsm_fused<T>(U, V, W, m, d, b, s);
-    PROF_BEGIN("k_dsm_ratio", s);
-    k_dsm_ratio<<<d.nyg - 2, 64, 0, s>>>(b.part, b.nblk, d.nyg, b.ratio);
-    PROF_END(s);
-    {
-        dim3 grd(cdiv((long)d.nxg * d.nyg, 256), d.nzg);
-        PROF_BEGIN("k_dsm_nut", s);
-        k_dsm_nut<T><<<grd, 256, 0, s>>>(nut, d, b, dirichlet, fuse_wm, U, alpha_x, m, nu);
-        PROF_END(s);
-    }
+    run_dsm_ratio(DsmRatioArgs{b.part, b.ratio}, nullptr, 1, d, b.nblk, s);
+    run_dsm_nut<T>(DsmNutArgs<T>{nut, b.smag, b.ratio, U, alpha_x, m, nu, dirichlet}, nullptr, 1, d, fuse_wm, s);
 }
 
 // code/wallmodel.f90:23-46 on its own: wall rows from alpha_x, then the
@@ -172,10 +194,15 @@ void launch_wall_model(const T* U, T* nut, const double* alpha_x, const Metric<T
 
 // one thread per (x face, z centre, wall)
 template <typename T>
-__global__ void __launch_bounds__(256) k_equilibrium_alpha(const T* U, double* alpha, Dims d, double nu,
-                                                           WallRows ws, double kappa, double B, double yc)
+__global__ void __launch_bounds__(256) k_equilibrium_alpha(EqAlphaArgs<T> one, const EqAlphaArgs<T>* tab, Dims d)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const GBlk g = gblk(1);
+    const EqAlphaArgs<T> a = member_args(one, tab, g.m);
+    const T* U = a.U;
+    double* alpha = a.alpha;
+    const double nu = a.nu, kappa = a.kappa, B = a.B, yc = a.yc;
+    const WallRows& ws = a.ws;
+    const int q = g.x * blockDim.x + threadIdx.x;
     if (q >= 2 * d.nx * d.nzg) return;
     const int i = q % d.nx + 1, r = q / d.nx;
     const int k = r % d.nzg + 1, wall = r / d.nzg;
@@ -201,16 +228,23 @@ template <typename T>
 void launch_equilibrium_alpha(const T* U, double* alpha_x, const Dims& d, double nu, const WallRows& ws,
                               double kappa, double B, hipStream_t s)
 {
-    const double yc = loglaw_crossover(kappa, B);
+    run_equilibrium_alpha<T>(EqAlphaArgs<T>{U, alpha_x, nu, ws, kappa, B, loglaw_crossover(kappa, B)}, nullptr, 1, d,
+                             s);
+}
+
+template <typename T>
+void run_equilibrium_alpha(const EqAlphaArgs<T>& one, const EqAlphaArgs<T>* tab, int nmem, const Dims& d,
+                           hipStream_t s)
+{
     PROF_BEGIN("k_equilibrium_alpha", s);
-    k_equilibrium_alpha<T><<<cdiv(2L * d.nx * d.nzg, 256), 256, 0, s>>>(U, alpha_x, d, nu, ws, kappa, B, yc);
+    k_equilibrium_alpha<T><<<dim3(cdiv(2L * d.nx * d.nzg, 256), 1, nmem), 256, 0, s>>>(one, tab, d);
     PROF_END(s);
 }
 
 // ------------------------------------------------------------- reductions
 
 // block sum of two doubles into part[blk], part[nblk + blk]
-__device__ __forceinline__ void block_sum2(double a, double b, double* part, int nblk)
+__device__ __forceinline__ void block_sum2(double a, double b, double* part, int nblk, int blk)
 {
     __shared__ double red[2][RED_THREADS];
     const int t = threadIdx.x;
@@ -220,7 +254,7 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* part, int
         if (t < st) { red[0][t] += red[0][t + st]; red[1][t] += red[1][t + st]; }
         __syncthreads();
     }
-    if (t == 0) { part[blockIdx.x] = red[0][0]; part[nblk + blockIdx.x] = red[1][0]; }
+    if (t == 0) { part[blk] = red[0][0]; part[nblk + blk] = red[1][0]; }
 }
 
 __device__ __forceinline__ void final_sum2(const double* part, int nblk, double& a, double& b)
@@ -253,7 +287,7 @@ __global__ void k_massflow_u(const T* U, Metric<double> m, Dims d, double* part,
         q += ((double)U[at(d, i, j, k)] + (double)U[at(d, i, j - 1, k)]) * 0.5 * dy;
         nrm += (1.0 + 1.0) * 0.5 * dy;
     }
-    block_sum2(q, nrm, part, nblk);
+    block_sum2(q, nrm, part, nblk, blockIdx.x);
 }
 
 __global__ void k_massflow_u_final(const double* part, int nblk, Scalars* sc)
@@ -278,22 +312,29 @@ void launch_mass_flow_init(const T* U, const Metric<double>& md, const Dims& d, 
 // code/mass_flow.f90:113-166: mean of 0.5(U(:,3,:)+U(:,nyg-2,:)) over the
 // unique x faces and z centres; dPdx = Qflow_x_0 - mean.
 template <typename T>
-__global__ void k_dpx_partial(const T* U, Dims d, double* part, int nblk)
+__global__ void k_dpx_partial(MassArgs<T> one, const MassArgs<T>* tab, Dims d, int nblk)
 {
+    const GBlk g = gblk(1);
+    const MassArgs<T> ar = member_args(one, tab, g.m);
+    const T* U = ar.U;
     const long n1 = d.nx - 2, n3 = d.nzp, tot = n1 * n3;
     double s = 0.0, c = 0.0;
-    for (long t = (long)blockIdx.x * RED_THREADS + threadIdx.x; t < tot; t += (long)nblk * RED_THREADS) {
+    for (long t = (long)g.x * RED_THREADS + threadIdx.x; t < tot; t += (long)nblk * RED_THREADS) {
         const int i = (int)(t % n1) + 2, k = (int)(t / n1) + 2;
         s += 0.5 * ((double)U[at(d, i, 3, k)] + (double)U[at(d, i, d.nyg - 2, k)]);
         c += 1.0;
     }
-    block_sum2(s, c, part, nblk);
+    block_sum2(s, c, ar.part, nblk, g.x);
 }
 
-__global__ void k_dpx_final(const double* part, int nblk, Scalars* sc, int advance)
+template <typename T>
+__global__ void k_dpx_final(MassArgs<T> one, const MassArgs<T>* tab, int nblk, int advance)
 {
+    const GBlk g = gblk(1);
+    const MassArgs<T> ar = member_args(one, tab, g.m);
+    Scalars* sc = ar.sc;
     double a, b;
-    final_sum2(part, nblk, a, b);
+    final_sum2(ar.part, nblk, a, b);
     if (threadIdx.x == 0) {
         sc->last_dPdx = sc->Qflow_x_0 - a / b;
         sc->dPdx = 0.0;  // code/time_integration.f90:92
@@ -303,11 +344,15 @@ __global__ void k_dpx_final(const double* part, int nblk, Scalars* sc, int advan
 
 // U(2:nx-1,2:nyg-1,2:nzg-1) += dPdx (code/time_integration.f90:91)
 template <typename T>
-__global__ void k_shift_u(T* U, Dims d, const Scalars* sc)
+__global__ void k_shift_u(MassArgs<T> one, const MassArgs<T>* tab, Dims d)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 2;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
-    const int k = blockIdx.z + 2;
+    const GBlk g = gblk(d.nzg - 3);
+    const MassArgs<T> ar = member_args(one, tab, g.m);
+    T* U = ar.U;
+    const Scalars* sc = ar.sc;
+    const int i = g.x * blockDim.x + threadIdx.x + 2;
+    const int j = g.y * blockDim.y + threadIdx.y + 2;
+    const int k = g.z + 2;
     if (i > d.nx - 1 || j > d.nyg - 1) return;
     const T s = T(sc->last_dPdx);
     const long p = at(d, i, j, k);
@@ -315,27 +360,34 @@ __global__ void k_shift_u(T* U, Dims d, const Scalars* sc)
 }
 
 template <typename T>
-void launch_mass_flow_correct(T* U, const Dims& d, double* part, int nblk, Scalars* sc, int apply, hipStream_t s,
-                              int advance)
+void run_mass_flow_correct(const MassArgs<T>& one, const MassArgs<T>* tab, int nmem, const Dims& d, int nblk,
+                           int apply, int advance, hipStream_t s)
 {
     PROF_BEGIN("k_dpx_partial", s);
-    k_dpx_partial<T><<<nblk, RED_THREADS, 0, s>>>(U, d, part, nblk);
+    k_dpx_partial<T><<<dim3(nblk, 1, nmem), RED_THREADS, 0, s>>>(one, tab, d, nblk);
     PROF_END(s);
     PROF_BEGIN("k_dpx_final", s);
-    k_dpx_final<<<1, RED_THREADS, 0, s>>>(part, nblk, sc, advance);
+    k_dpx_final<T><<<dim3(1, 1, nmem), RED_THREADS, 0, s>>>(one, tab, nblk, advance);
     PROF_END(s);
     if (apply) {
         // canonical cells only (k up to nzg-2), images written by store_img
-        dim3 blk(64, 4, 1), grd(cdiv(d.nx - 2, 64), cdiv(d.nyg - 2, 4), d.nzg - 3);
+        dim3 blk(64, 4, 1), grd(cdiv(d.nx - 2, 64), cdiv(d.nyg - 2, 4), (unsigned)(d.nzg - 3) * nmem);
         PROF_BEGIN("k_shift_u", s);
-        k_shift_u<T><<<grd, blk, 0, s>>>(U, d, sc);
+        k_shift_u<T><<<grd, blk, 0, s>>>(one, tab, d);
         PROF_END(s);
     }
 }
 
+template <typename T>
+void launch_mass_flow_correct(T* U, const Dims& d, double* part, int nblk, Scalars* sc, int apply, hipStream_t s,
+                              int advance)
+{
+    run_mass_flow_correct<T>(MassArgs<T>{U, part, sc}, nullptr, 1, d, nblk, apply, advance, s);
+}
+
 // code/time_integration.f90:102-150 (max of dx/|u| as the reference; max is
 // order independent, so the result is exact).
-__device__ __forceinline__ void block_max3(double a, double b, double c, double* part, int nblk)
+__device__ __forceinline__ void block_max3(double a, double b, double c, double* part, int nblk, int blk)
 {
     __shared__ double red[3][RED_THREADS];
     const int t = threadIdx.x;
@@ -350,9 +402,9 @@ __device__ __forceinline__ void block_max3(double a, double b, double c, double*
         __syncthreads();
     }
     if (t == 0) {
-        part[blockIdx.x] = red[0][0];
-        part[nblk + blockIdx.x] = red[1][0];
-        part[2 * nblk + blockIdx.x] = red[2][0];
+        part[blk] = red[0][0];
+        part[nblk + blk] = red[1][0];
+        part[2 * nblk + blk] = red[2][0];
     }
 }
 
@@ -360,11 +412,15 @@ __device__ __forceinline__ void block_max3(double a, double b, double c, double*
 // and no per-element index division is needed; the maximum does not depend
 // on the order, so the result is the same as any other traversal.
 template <typename T>
-__global__ void k_dt_partial(const T* U, const T* V, const T* W, Metric<double> m, Dims d, double* part, int nblk)
+__global__ void k_dt_partial(DtArgs<T> one, const DtArgs<T>* tab, Dims d, int nblk)
 {
+    const GBlk g = gblk(1);
+    const DtArgs<T> a = member_args(one, tab, g.m);
+    const T *U = a.U, *V = a.V, *W = a.W;
+    const Metric<double>& m = a.m;
     const int n2 = d.nyg - 2, rows = n2 * (d.nzg - 2);
     double lu = 0.0, lv = 0.0, lw = 0.0;
-    for (int r = blockIdx.x; r < rows; r += nblk) {
+    for (int r = g.x; r < rows; r += nblk) {
         const int j = r % n2 + 2, k = r / n2 + 2;
         const double dyj = m.yg[j + 1] - m.yg[j], dzk = m.zg[k + 1] - m.zg[k];
         const long base = at(d, 1, j, k);
@@ -374,11 +430,17 @@ __global__ void k_dt_partial(const T* U, const T* V, const T* W, Metric<double> 
             lw = fmax(lw, dzk / fabs((double)W[base + i - 1]));
         }
     }
-    block_max3(lu, lv, lw, part, nblk);
+    block_max3(lu, lv, lw, a.part, nblk, g.x);
 }
 
-__global__ void k_dt_final(const double* part, int nblk, Scalars* sc, const double* mins3)
+template <typename T>
+__global__ void k_dt_final(DtArgs<T> one, const DtArgs<T>* tab, int nblk)
 {
+    const GBlk g = gblk(1);
+    const DtArgs<T> ar = member_args(one, tab, g.m);
+    const double* part = ar.part;
+    Scalars* sc = ar.sc;
+    const double* mins3 = ar.mins3;
     __shared__ double red[3][RED_THREADS];
     const int t = threadIdx.x;
     double a = 0.0, b = 0.0, c = 0.0;
@@ -408,15 +470,21 @@ __global__ void k_dt_final(const double* part, int nblk, Scalars* sc, const doub
 }
 
 template <typename T>
+void run_dt(const DtArgs<T>& one, const DtArgs<T>* tab, int nmem, const Dims& d, int nblk, hipStream_t s)
+{
+    PROF_BEGIN("k_dt_partial", s);
+    k_dt_partial<T><<<dim3(nblk, 1, nmem), RED_THREADS, 0, s>>>(one, tab, d, nblk);
+    PROF_END(s);
+    PROF_BEGIN("k_dt_final", s);
+    k_dt_final<T><<<dim3(1, 1, nmem), RED_THREADS, 0, s>>>(one, tab, nblk);
+    PROF_END(s);
+}
+
+template <typename T>
 void launch_dt(const T* U, const T* V, const T* W, const Metric<double>& md, const Dims& d, double* part, int nblk,
                Scalars* sc, const double* mins3, hipStream_t s)
 {
-    PROF_BEGIN("k_dt_partial", s);
-    k_dt_partial<T><<<nblk, RED_THREADS, 0, s>>>(U, V, W, md, d, part, nblk);
-    PROF_END(s);
-    PROF_BEGIN("k_dt_final", s);
-    k_dt_final<<<1, RED_THREADS, 0, s>>>(part, nblk, sc, mins3);
-    PROF_END(s);
+    run_dt<T>(DtArgs<T>{U, V, W, md, part, sc, mins3}, nullptr, 1, d, nblk, s);
 }
 
 // code/projection.f90:223-245 (signed max) plus max |div|
@@ -435,7 +503,7 @@ __global__ void k_div_partial(const T* U, const T* V, const T* W, Metric<double>
         mx = fmax(mx, v);
         ma = fmax(ma, fabs(v));
     }
-    block_max3(mx, ma, 0.0, part, nblk);
+    block_max3(mx, ma, 0.0, part, nblk, blockIdx.x);
 }
 
 __global__ void k_div_final(const double* part, int nblk, Scalars* sc)
@@ -526,7 +594,7 @@ __global__ void k_wall_shear_partial(const T* U, const T* nut, Metric<double> m,
         sb += ((double)U[at(d, i, 2, k)] - (double)U[at(d, i, 1, k)]) / (m.yg[2] - m.yg[1]) * (nu + nb);
         st += ((double)U[at(d, i, jt, k)] - (double)U[at(d, i, jt + 1, k)]) / (m.yg[jt + 1] - m.yg[jt]) * (nu + nt);
     }
-    block_sum2(sb, st, part, nblk);
+    block_sum2(sb, st, part, nblk, blockIdx.x);
 }
 
 __global__ void k_wall_shear_final(const double* part, int nblk, double ncell, double* out)
@@ -565,7 +633,14 @@ void launch_wall_shear(const T* U, const T* nut, const Metric<double>& md, const
     template void launch_plane_stats<T>(const T*, const T*, const T*, const Metric<double>&, const Dims&,       \
                                         double*, hipStream_t);                                                  \
     template void launch_wall_shear<T>(const T*, const T*, const Metric<double>&, const Dims&, double, double*, \
-                                       int, double*, hipStream_t);
+                                       int, double*, hipStream_t);                                              \
+    template void run_dsm_nut<T>(const DsmNutArgs<T>&, const DsmNutArgs<T>*, int, const Dims&, int,             \
+                                 hipStream_t);                                                                  \
+    template void run_equilibrium_alpha<T>(const EqAlphaArgs<T>&, const EqAlphaArgs<T>*, int, const Dims&,      \
+                                           hipStream_t);                                                        \
+    template void run_mass_flow_correct<T>(const MassArgs<T>&, const MassArgs<T>*, int, const Dims&, int, int,  \
+                                           int, hipStream_t);                                                   \
+    template void run_dt<T>(const DtArgs<T>&, const DtArgs<T>*, int, const Dims&, int, hipStream_t);
 
 CHAN_INST(double)
 CHAN_INST(float)

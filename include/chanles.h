@@ -234,20 +234,29 @@ int chan_wall_model_stats(chan_ctx* ctx, double out2[2]);
 /* ---------------------------------------------------------------------
  * Ensembles: many independent channels (any sizes, precisions and wall
  * models; one device) advanced together.  No reference counterpart (the
- * reference runs one channel per MPI job).  One step of every member is
- * captured into one graph in which each member's kernels form an independent
- * branch; chan_ensemble_run_steps launches that graph once per step
- * (compute_dt + compute_time_step_RK3 of every member).  Members stay usable
- * on their own: the ensemble orders its steps after the members' pending
- * work and the members' later work after its steps.  Changing a member's
- * settings (wall model, tile shape) or stepping it alone is detected and the
- * graphs are captured again.  The ensemble does not own the members; destroy
- * it before them. */
+ * reference runs one channel per MPI job).  Members of one shape and the
+ * same step settings (precision, dynamic Smagorinsky, wall-model kind,
+ * mass-flow mode, tile shapes) form a group whose step is one launch per
+ * stage for all of its members, each member's arguments in a device table;
+ * other members run their own kernels.  One step of every group and lone
+ * member is captured into one graph of independent branches;
+ * chan_ensemble_run_steps launches that graph once per step (compute_dt +
+ * compute_time_step_RK3 of every member).  Results are bit for bit those of
+ * the members stepped alone.  Members stay usable on their own: the ensemble
+ * orders its steps after the members' pending work and the members' later
+ * work after its steps.  Changing a member's settings (wall model, tile
+ * shape) or stepping it alone is detected and the graphs are captured again.
+ * The ensemble does not own the members; destroy it before them. */
 int  chan_ensemble_create(chan_ctx* const* members, int n, chan_ensemble** out);
 void chan_ensemble_destroy(chan_ensemble* ens);
 const char* chan_ensemble_last_error(const chan_ensemble* ens);
 int chan_ensemble_size(const chan_ensemble* ens);
 int chan_ensemble_run_steps(chan_ensemble* ens, int nsteps);
+/* grouping on (1, the default) or off (0: every member runs its own
+ * kernels, one branch per member); the number of groups and lone members
+ * the current settings give */
+int chan_ensemble_set_grouping(chan_ensemble* ens, int on);
+int chan_ensemble_group_count(chan_ensemble* ens);
 int chan_ensemble_synchronize(chan_ensemble* ens);
 
 /* Tile shapes and the autotuner.  No reference counterpart (the reference's

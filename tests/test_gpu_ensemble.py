@@ -128,3 +128,47 @@ def test_config5_one_step_per_grid_against_oracle(gpu):
             assert rel(a, b) <= 1e-11, ((c.nx, c.ny, c.nz), rel(a, b))
         assert abs(g.scalars()["t"] - o.scalars()["t"]) <= 1e-15
     ens.close()
+
+
+def _grouped_members():
+    """Four FP64 channels of one shape (different Re_tau), three FP32-bulk
+    channels of another, one AMD channel: two groups and one lone member."""
+    out = []
+    for q in range(4):
+        case, field = cases.small_case(18, 17, 18, seed=10 + q)
+        case.nu = case.nu / (1.0 + q)
+        out.append((case, field, FP64))
+    for q in range(3):
+        case, field = cases.small_case(26, 17, 18, seed=20 + q)
+        out.append((case, field, FP32_BULK))
+    case, field = cases.small_case(18, 17, 18, seed=30, LES_model=3)
+    out.append((case, field, FP64))
+    return out
+
+
+def test_grouped_ensemble_matches_members_alone(gpu):
+    """Grouped launches (one launch per stage for all members of a shape):
+    after 7 steps, bit for bit the members stepped alone and the same
+    ensemble without grouping (every member its own kernels)."""
+    mem = _grouped_members()
+    alone = [make_channel(c, f, precision=p, wall=WALL_EQUILIBRIUM) for c, f, p in mem]
+    grouped = [make_channel(c, f, precision=p, wall=WALL_EQUILIBRIUM) for c, f, p in mem]
+    single = [make_channel(c, f, precision=p, wall=WALL_EQUILIBRIUM) for c, f, p in mem]
+    for ch in alone + grouped + single:
+        ch.set_wall_model(WALL_EQUILIBRIUM, h_wm=0.1)
+    eg, es = Ensemble(grouped), Ensemble(single)
+    es.set_grouping(False)
+    assert eg.group_count() == 3 and es.group_count() == len(mem)
+    for ch in alone:
+        ch.run_steps(7)
+    eg.run_steps(7)
+    es.run_steps(7)
+    eg.synchronize()
+    es.synchronize()
+    for q, (a, b, c) in enumerate(zip(alone, grouped, single)):
+        for x, y, z in zip(_state(a), _state(b), _state(c)):
+            assert np.all(np.isfinite(x))
+            assert np.array_equal(x, y), f"member {q} grouped"
+            assert np.array_equal(x, z), f"member {q} ungrouped"
+    eg.close()
+    es.close()
