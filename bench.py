@@ -69,12 +69,17 @@ def parse():
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=20,
                     help="at most this many oracle steps for the CPU baseline (0: skip)")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0,
-                    help="stop the CPU baseline after the step that passes this many seconds")
+    ap.add_argument("--cpu-seconds", type=float, default=5.0,
+                    help="stop each CPU baseline run after the step that passes this many seconds")
+    ap.add_argument("--cpu-runs", type=int, default=3, help="runs of the CPU baseline sample")
     ap.add_argument("--ensemble", type=int, default=64, help="channels of the batched ensemble (0: skip)")
-    ap.add_argument("--ensemble-steps", type=int, default=10)
-    ap.add_argument("--profile-compare", type=int, default=2000,
-                    help="steps of the FP32-bulk vs FP64 mean-profile comparison (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=1000,
+                    help="steps of the batched ensemble (its wall-stress error is read after them)")
+    ap.add_argument("--ensemble-timed", type=int, default=100,
+                    help="steps timed for each form of the ensemble")
+    ap.add_argument("--profile-compare", type=int, default=20000,
+                    help="steps of the FP32-bulk vs FP64 mean-profile comparison (0: skip); the second "
+                         "half is averaged")
     return ap.parse_args()
 
 
@@ -82,42 +87,68 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _fluct_diff(a, b):
+    """rms of the difference of two channels' U fluctuations (about their
+    plane means) relative to the rms of the first's: ~1.4 once two runs are
+    decorrelated, 0 when identical."""
+    import channel_rl_amd as cr
+    Ua, Ub = a.get_field(cr.U), b.get_field(cr.U)
+    fa = Ua - Ua.mean(axis=(0, 2), keepdims=True)
+    fb = Ub - Ub.mean(axis=(0, 2), keepdims=True)
+    return float(np.sqrt(np.mean((fa - fb) ** 2)) / np.sqrt(np.mean(fa ** 2)))
+
+
 def compare_fp32_profiles(case, field, args, cr):
     """The FP32 bulk against FP64 over a long run of this channel: both from
-    the same initial field, plane statistics averaged every 10 steps over the
-    second half; relative differences of the mean U+ profile, <u'v'>+ and
-    u_tau, and the FP32 run's ms/step."""
+    the same initial field, the first half discarded, plane statistics
+    averaged every 50 steps over the second half; how far the two
+    trajectories have separated when the averaging starts (fluctuation rms
+    of the difference over that of one run: ~1.4 for independent fields),
+    the relative differences of the mean U+ profile, <u'v'>+ and u_tau, and
+    ms/step of each precision."""
     from channel_rl_amd import statistics as S
     from channel_rl_amd.ensemble import make_channel
     n = args.profile_compare
-    res, ms = {}, {}
-    for prec in (cr.FP64, cr.FP32_BULK):
-        c = make_channel(case, field, precision=prec, wall=cr.WALL_EQUILIBRIUM)
+    half = n // 2
+    chans = {p: make_channel(case, field, precision=p, wall=cr.WALL_EQUILIBRIUM) for p in (cr.FP64, cr.FP32_BULK)}
+    ms = {}
+    for p, c in chans.items():
         t0 = time.perf_counter()
-        c.run_steps(n // 2)
-        a = S.ProfileAverager(c)
-        a.run(n - n // 2, 10)
+        c.run_steps(half)
         c.synchronize()
-        ms[prec] = 1e3 * (time.perf_counter() - t0) / n
-        res[prec] = a.result()
+        ms[p] = 1e3 * (time.perf_counter() - t0) / half
+    sep = _fluct_diff(chans[cr.FP64], chans[cr.FP32_BULK])
+    res = {}
+    for p, c in chans.items():
+        a = S.ProfileAverager(c)
+        a.run(n - half, 50)
+        res[p] = a.result()
+    sep_end = _fluct_diff(chans[cr.FP64], chans[cr.FP32_BULK])
+    for c in chans.values():
         c.close()
     d = S.profile_difference(res[cr.FP64], res[cr.FP32_BULK])
-    out = {"steps": n, "Uplus_rel_diff": d["Uplus"], "uvplus_rel_diff": d["uvplus"], "u_tau_rel_diff": d["u_tau"],
+    out = {"steps": n, "averaged_steps": n - half, "samples": int(res[cr.FP64]["samples"]),
+           "fluct_separation_at_average_start": round(sep, 3), "fluct_separation_at_end": round(sep_end, 3),
+           "Uplus_rel_diff": d["Uplus"], "uvplus_rel_diff": d["uvplus"], "u_tau_rel_diff": d["u_tau"],
            "Re_tau_fp64": round(float(res[cr.FP64]["Re_tau"]), 2),
-           "fp32_ms_per_step_with_stats": round(ms[cr.FP32_BULK], 4),
-           "fp64_ms_per_step_with_stats": round(ms[cr.FP64], 4)}
-    log(f"[bench] FP32 bulk vs FP64 over {n} steps: U+ {d['Uplus']:.2e}, <u'v'>+ {d['uvplus']:.2e}, "
-        f"u_tau {d['u_tau']:.2e}")
+           "Re_tau_fp32": round(float(res[cr.FP32_BULK]["Re_tau"]), 2),
+           "fp32_ms_per_step": round(ms[cr.FP32_BULK], 4), "fp64_ms_per_step": round(ms[cr.FP64], 4)}
+    log(f"[bench] FP32 bulk vs FP64, {n} steps (second half averaged, separation {sep:.2f} -> {sep_end:.2f}): "
+        f"U+ {d['Uplus']:.2e}, <u'v'>+ {d['uvplus']:.2e}, u_tau {d['u_tau']:.2e}")
     return out
 
 
 def measure_batched(ch, case, args, cr, cases):
     """The learned wall model and the batched ensemble: train the network on
     data of the small channel (10^6 samples, 20 epochs), run this channel with
-    it, then the ensemble's channels plain (FP64, equilibrium wall model, each
-    channel's own captured step, one channel after the other) and batched
-    (FP32 bulk + FP64 Poisson, the network in the loop, one graph launch per
-    ensemble step)."""
+    it, then the ensemble's 64 channels four ways, each timed over the same
+    steps: plain (FP64, equilibrium wall model, each channel's own captured
+    step, one channel after the other: the single-channel form), plain with
+    the batched form's settings (FP32 bulk + FP64 Poisson, the network in the
+    loop, one after the other), batched with every member's own kernels (one
+    graph of per-member branches), and batched with grouped launches (one
+    launch per stage for all channels of a shape).  The grouped ensemble then
+    runs to the configured step count and reports its wall-stress error."""
     from channel_rl_amd import ensemble as E
     from channel_rl_amd import wallnet
     t0 = time.perf_counter()
@@ -143,36 +174,120 @@ def measure_batched(ch, case, args, cr, cases):
     log(f"[bench] {case.name} with the network: {ms_mlp:.3f} ms/step, error {err1:.4f}, fallback {fb1:.4f}")
     mem = cases.ensemble(args.ensemble)
     ncell = sum(c.cells for c, _ in mem)
-    k = args.ensemble_steps
-    plain = [E.make_channel(c, f, precision=cr.FP64, wall=cr.WALL_EQUILIBRIUM) for c, f in mem]
-    tp = E.time_sequential(plain, k)
-    for c in plain:
-        c.close()
+    k = args.ensemble_timed
+    times = {}
+    for name, prec, wall in (("plain_fp64_equilibrium", cr.FP64, cr.WALL_EQUILIBRIUM),
+                             ("plain_fp32_network", cr.FP32_BULK, cr.WALL_MLP)):
+        chans = [E.make_channel(c, f, precision=prec, wall=wall, net=net) for c, f in mem]
+        times[name] = E.time_sequential(chans, k)
+        for c in chans:
+            c.close()
+        log(f"[bench] ensemble {name}: {len(mem) * k / times[name]:.1f} channel-steps/s")
     chans = [E.make_channel(c, f, precision=cr.FP32_BULK, wall=cr.WALL_MLP, net=net) for c, f in mem]
     ens = cr.Ensemble(chans)
-    tb = E.time_batched(ens, k)
+    ens.set_grouping(False)
+    times["batched_member_kernels"] = E.time_batched(ens, k)
+    log(f"[bench] ensemble batched_member_kernels: {len(mem) * k / times['batched_member_kernels']:.1f} "
+        "channel-steps/s")
+    ens.set_grouping(True)
+    groups = ens.group_count()
+    times["batched_grouped"] = E.time_batched(ens, k)
+    log(f"[bench] ensemble batched_grouped ({groups} groups): {len(mem) * k / times['batched_grouped']:.1f} "
+        "channel-steps/s")
+    done = 2 * (k + 1)
+    if args.ensemble_steps > done:
+        ens.run_steps(args.ensemble_steps - done)
+        ens.synchronize()
+        done = args.ensemble_steps
     werr, wfb = E.wall_error(chans)
+    finite = all(np.all(np.isfinite(c.get_field(cr.U))) for c in chans)
     ens.close()
     for c in chans:
         c.close()
+    cps = {name: round(len(mem) * k / t, 1) for name, t in times.items()}
+    tg = times["batched_grouped"]
     out = {
         "workload": f"{len(mem)} channels, faces drawn from 34 33 34 / 66 49 66 / 98 65 98, Re_tau 180..5200, "
                     f"{ncell} cells, wall model at h_wm = 0.1",
-        "steps": k,
-        "plain_channel_steps_per_s": round(len(mem) * k / tp, 1),
-        "batched_channel_steps_per_s": round(len(mem) * k / tb, 1),
-        "speedup": round(tp / tb, 3),
-        "batched_mcells_per_s": round(ncell * k / tb / 1e6, 1),
+        "timed_steps": k,
+        "channel_steps_per_s": cps,
+        "groups": groups,
+        # the batched + learned-wall-model form against the plain single-channel form
+        "speedup": round(times["plain_fp64_equilibrium"] / tg, 3),
+        # grouped batching alone, same precision and wall model on both sides
+        "batching_speedup_same_settings": round(times["plain_fp32_network"] / tg, 3),
+        "grouping_speedup_over_member_kernels": round(times["batched_member_kernels"] / tg, 3),
+        "batched_mcells_per_s": round(ncell * k / tg / 1e6, 1),
+        "batched_alg_hbm_frac": round(ncell * 584 * k / tg / (HBM_PEAK_GBS * 1e9), 4),
+        "steps_run": done,
+        "all_finite": bool(finite),
         "ensemble_wall_stress_error": round(werr, 4),
         "ensemble_fallback_fraction": round(wfb, 4),
         "network": {"samples": len(X), "epochs": 20, "train_s": round(t2 - t1, 2), "final_loss": float(losses[-1]),
                     "validation_error": round(val, 4)},
         "single_channel_mlp_ms_per_step": round(ms_mlp, 4),
         "single_channel_mlp_error": round(err1, 4),
+        "single_channel_mlp_fallback": round(fb1, 4),
     }
-    log(f"[bench] ensemble: plain {out['plain_channel_steps_per_s']} channel-steps/s, batched "
-        f"{out['batched_channel_steps_per_s']} channel-steps/s, wall error {werr:.4f}, fallback {wfb:.4f}")
+    log(f"[bench] ensemble after {done} steps: wall error {werr:.4f}, fallback {wfb:.4f}, finite {finite}")
     return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(case, ch, q0, args):
+    """The oracle (the reference's algorithm restated in C, OpenMP over its
+    loops with disjoint outputs) on this host's cores, FP64, on a bounded
+    sample of the same workload: args.cpu_runs runs of whole RK3 steps
+    (compute_dt + compute_time_step_RK3) from the GPU's current state, each
+    until args.cpu_seconds have passed (at most args.cpu_steps steps); the
+    mean and spread over the runs, and ms per step of each stage."""
+    import oracle
+    flags = oracle.use_fast_build()
+    U, V, W = ch.state()
+    nthr = oracle.num_threads()
+    log(f"[bench] cpu baseline: oracle ({flags}) on {nthr} OpenMP thread(s), nproc {os.cpu_count()}, "
+        f"{cpu_model()}")
+    rates, steps_all = [], 0
+    st = np.zeros(len(oracle.STAGE_NAMES))
+    for r in range(args.cpu_runs):
+        o = oracle.Oracle(case.x, case.y, case.z, **case.kwargs())
+        o.set_state(U, V, W)
+        o.set_scalars(dt=0.0, t=0.0, dPdx=0.0, Qflow_x_0=q0)
+        o.set_wall_model(1, h_wm=case.h_wm)
+        o.compute_dt()
+        o.compute_time_step_RK3()   # warm-up step, untimed
+        st0 = oracle.stage_times()
+        c0 = time.perf_counter()
+        done = 0
+        while done < args.cpu_steps:
+            o.compute_dt()
+            o.compute_time_step_RK3()
+            done += 1
+            if time.perf_counter() - c0 >= args.cpu_seconds:
+                break
+        sec = time.perf_counter() - c0
+        st += oracle.stage_times() - st0
+        rates.append(case.cells * done / sec / 1e6)
+        steps_all += done
+        log(f"[bench] cpu run {r + 1}: {done} step(s) in {sec:.1f} s, {rates[-1]:.4f} Mcells/s")
+        del o
+    stages = {name: round(float(1e3 * t / steps_all), 2) for name, t in zip(oracle.STAGE_NAMES, st) if t > 0}
+    return {"value": round(float(np.mean(rates)), 4), "unit": "Mcells/s", "cores": nthr, "kind": "port",
+            "runs": [round(x, 4) for x in rates], "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "flags": flags, "ms_per_step_by_stage": stages,
+            "sample": f"{args.cpu_runs} run(s), {steps_all} RK3 step(s) in all, of the same {case.nx}x{case.ny}x"
+                      f"{case.nz} channel from the GPU run's state (after one untimed step each), FP64 C oracle "
+                      f"(OpenMP over loops with disjoint outputs), {nthr} thread(s), mean over the runs"}
 
 
 def main():
@@ -259,30 +374,8 @@ def main():
 
     cpu = None
     if args.cpu_steps > 0:
-        import oracle
-        o = oracle.Oracle(case.x, case.y, case.z, **case.kwargs())
-        U, V, W = ch.state()
-        o.set_state(U, V, W)
-        o.set_scalars(dt=0.0, t=0.0, dPdx=0.0, Qflow_x_0=q0)
-        o.set_wall_model(1, h_wm=case.h_wm)
-        nthr = oracle.num_threads()
-        log(f"[bench] cpu baseline: oracle on {nthr} OpenMP thread(s), up to {args.cpu_steps} step(s) "
-            f"or {args.cpu_seconds} s at the same size")
-        c0 = time.perf_counter()
-        done = 0
-        while done < args.cpu_steps:
-            o.compute_dt()
-            o.compute_time_step_RK3()
-            done += 1
-            log(f"[bench] cpu step {done}: {time.perf_counter() - c0:.1f} s")
-            if time.perf_counter() - c0 >= args.cpu_seconds:
-                break
-        c1 = time.perf_counter()
-        cpu = {"value": round(cells * done / (c1 - c0) / 1e6, 4), "unit": "Mcells/s", "cores": nthr,
-               "kind": "port", "sample": f"{done} RK3 step(s) of the same {case.nx}x{case.ny}x{case.nz} "
-               f"channel, FP64 C oracle (OpenMP over loops with disjoint outputs, -O2), {nthr} thread(s), "
-               f"{c1 - c0:.1f} s"}
-        log(f"[bench] cpu: {cpu['value']} Mcells/s")
+        cpu = cpu_baseline(case, ch, q0, args)
+        log(f"[bench] cpu: {cpu['value']} Mcells/s ({cpu['runs']})")
 
     out = {
         "metric": "channel cell-RK3-steps/s (Mcells/s)",
@@ -298,8 +391,9 @@ def main():
         "dtype": "f64" if prec == cr.FP64 else "f32-bulk/f64-poisson",
         "data": "synthetic",
         "config": {"workload": f"channel LES RK3 step, {case.nx - 2}x{case.ny}x{case.nz - 2} "
-                               f"(faces {case.nx} {case.ny} {case.nz}), Re_tau 2000, dynamic Smagorinsky, "
-                               "equilibrium wall model, constant mass flow",
+                               f"(faces {case.nx} {case.ny} {case.nz}), Re_tau {case.re_tau:.0f}, dynamic "
+                               f"Smagorinsky, equilibrium wall model at h_wm = {case.h_wm}, constant mass flow, "
+                               f"{args.precision} bulk",
                    "cells": cells, "parallelism": "single GPU", "max_abs_div": maxdiv},
         "roofline": roofline,
         "cpu_baseline": cpu,

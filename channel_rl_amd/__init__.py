@@ -38,6 +38,7 @@ EXPORTS = [
     "chan_ensemble_run_steps", "chan_ensemble_synchronize", "chan_wall_shear", "chan_rl_observe", "chan_rl_act",
     "chan_compute_pressure",
     "chan_compute_pressure_noalloc", "chan_ensemble_set_grouping", "chan_ensemble_group_count",
+    "chan_set_wall_viscosity_limit",
 ]
 KERNEL_RHS = 0
 WALL_MLP = 2
@@ -126,6 +127,7 @@ def load_library(path=LIB_PATH):
     L.chan_ensemble_size.argtypes = [vp]
     L.chan_ensemble_run_steps.argtypes = [vp, ctypes.c_int]
     L.chan_ensemble_set_grouping.argtypes = [vp, ctypes.c_int]
+    L.chan_set_wall_viscosity_limit.argtypes = [vp, ctypes.c_double]
     L.chan_ensemble_group_count.argtypes = [vp]
     L.chan_ensemble_synchronize.argtypes = [vp]
     L.chan_wall_shear.argtypes = [vp, _D]
@@ -386,6 +388,10 @@ class Channel:
 
     def set_wall_model(self, mode, h_wm=0.0, kappa=0.41, B=5.2):
         self._chk(load_library().chan_set_wall_model(self._h, mode, h_wm, kappa, B), "set_wall_model")
+
+    def set_wall_viscosity_limit(self, limit):
+        """Hold the wall model's total viscosity in [0, limit dy^2/dt] (0: off)."""
+        self._chk(load_library().chan_set_wall_viscosity_limit(self._h, float(limit)), "set_wall_viscosity_limit")
 
     def compute_equilibrium_wall_stress(self, h_wm=0.0, kappa=0.41, B=5.2):
         self._chk(load_library().chan_compute_equilibrium_wall_stress(self._h, h_wm, kappa, B),

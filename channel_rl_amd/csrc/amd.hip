@@ -127,7 +127,7 @@ __global__ void __launch_bounds__(256) k_amd(const T* U, const T* V, const T* W,
     if (i > d.nxg || j > d.nyg) return;
     const long p = at(d, i, j, k);
     if (fuse_wm && (j <= 2 || j >= d.nyg - 1)) {
-        nut[p] = wm_nut_at<T>(U, alpha, m, d, nu, img_centre(i, d.nxg), k, j <= 2 ? 0 : 1);
+        nut[p] = wm_nut_at<T>(U, alpha, m, d, nu, img_centre(i, d.nxg), k, j <= 2 ? 0 : 1, sc);
         return;
     }
     const int jj = (j == 1) ? 2 : (j == d.nyg ? d.nyg - 1 : j);

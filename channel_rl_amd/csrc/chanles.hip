@@ -307,13 +307,13 @@ void enqueue_eddy_viscosity(chan_ctx* c, int buf, int fuse_wm)
     if (c->LES_model == 2) {
         DsmBufs<T> b = dsm_bufs<T>(c);
         launch_dsm<T>(s[0], s[1], s[2], (T*)c->nut, metric<T>(c), c->d, b, c->Dirichlet_nu_t == 1, fuse_wm,
-                      c->alpha, nu, c->stream);
+                      c->alpha, nu, c->sc, c->stream);
     } else if (c->LES_model == 3) {
         launch_amd<T>(s[0], s[1], s[2], (T*)c->nut, metric<T>(c), c->d, c->Dirichlet_nu_t, c->sc, fuse_wm, c->alpha,
                       nu, c->stream);
     } else {
         k_fill<T><<<cdiv(c->d.nbox, 256), 256, 0, c->stream>>>((T*)c->nut, c->d.nbox, T(0));
-        if (fuse_wm) launch_wall_model<T>(s[0], (T*)c->nut, c->alpha, metric<T>(c), c->d, nu, c->stream);
+        if (fuse_wm) launch_wall_model<T>(s[0], (T*)c->nut, c->alpha, metric<T>(c), c->d, nu, c->sc, c->stream);
     }
 }
 
@@ -357,10 +357,10 @@ void enqueue_rk3(chan_ctx* c, int X)
                          c->rhs_shape, c->stream);
         enqueue_projection<T>(c, out[s]);
     }
-    if (c->x_mass_cte == 1) {
+    if (c->x_mass_cte == 1 || c->x_mass_cte == 2) {
         // the correction's final reduction also advances the time
         T** O = (T**)c->state[Y];
-        launch_mass_flow_correct<T>(O[0], c->d, c->part, c->nblk_red, c->sc, 1, c->stream, 1);
+        launch_mass_flow_correct<T>(O[0], c->md, c->d, c->part, c->nblk_red, c->sc, c->x_mass_cte, 1, c->stream, 1);
     } else {
         k_advance_time<<<1, 1, 0, c->stream>>>(c->sc);
     }
@@ -463,6 +463,8 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
     if (!cfg->x || !cfg->y || !cfg->z) return fail(c, CHAN_EINVAL, "Error: missing grid coordinates");
     if (cfg->precision != CHAN_FP64 && cfg->precision != CHAN_FP32_BULK)
         return fail(c, CHAN_EINVAL, "Error: invalid precision");
+    if (cfg->x_mass_cte < 0 || cfg->x_mass_cte > 2)
+        return fail(c, CHAN_EINVAL, "Error: x_mass_cte must be 0, 1 (the reference's functional) or 2 (bulk flow)");
     if (cfg->LES_model != 0 && cfg->LES_model != 2 && cfg->LES_model != 3)
         return fail(c, CHAN_EINVAL, "Error: LES_model must be 0 (none), 2 (dynamic Smagorinsky) or 3 (AMD)");
     if (cfg->Dirichlet_nu_t < 0 || cfg->Dirichlet_nu_t > 2 || (cfg->Dirichlet_nu_t == 2 && cfg->LES_model != 3))
@@ -770,10 +772,10 @@ int chan_compute_wall_model(chan_ctx* c)
     if (rc) return rc;
     if (c->prec == CHAN_FP64)
         launch_wall_model<double>((double*)c->state[c->cur][0], (double*)c->nut, c->alpha, c->md, c->d, c->nu,
-                                  c->stream);
+                                  c->sc, c->stream);
     else
         launch_wall_model<float>((float*)c->state[c->cur][0], (float*)c->nut, c->alpha, c->mf, c->d, (float)c->nu,
-                                 c->stream);
+                                 c->sc, c->stream);
     return check_launch(c);
 }
 
@@ -894,12 +896,13 @@ int chan_compute_dPx_for_constant_mass_flow(chan_ctx* c, double* dPdx)
     if (rc) return rc;
     double keep[4];
     if ((rc = chan_get_scalars(c, keep))) return rc;
+    const int mode = c->x_mass_cte == 2 ? 2 : 1;
     if (c->prec == CHAN_FP64)
-        launch_mass_flow_correct<double>((double*)c->state[c->cur][0], c->d, c->part, c->nblk_red, c->sc, 0,
-                                         c->stream, 0);
+        launch_mass_flow_correct<double>((double*)c->state[c->cur][0], c->md, c->d, c->part, c->nblk_red, c->sc, mode,
+                                         0, c->stream, 0);
     else
-        launch_mass_flow_correct<float>((float*)c->state[c->cur][0], c->d, c->part, c->nblk_red, c->sc, 0, c->stream,
-                                        0);
+        launch_mass_flow_correct<float>((float*)c->state[c->cur][0], c->md, c->d, c->part, c->nblk_red, c->sc, mode, 0,
+                                        c->stream, 0);
     if ((rc = check_launch(c))) return rc;
     Scalars s0{};
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -961,6 +964,20 @@ int chan_set_wall_model(chan_ctx* c, int mode, double h_wm, double kappa, double
     if (!(kappa > 0)) return fail(c, CHAN_EINVAL, "kappa must be positive");
     c->wall_mode = mode; c->h_wm = h_wm; c->kappa = kappa; c->Bc = B;
     drop_graphs(c);
+    return CHAN_OK;
+}
+
+int chan_set_wall_viscosity_limit(chan_ctx* c, double limit)
+{
+    if (!c) return CHAN_EINVAL;
+    if (!(limit >= 0.0)) return fail(c, CHAN_EINVAL, "the wall viscosity limit must be >= 0");
+    int rc = set_device(c);
+    if (rc) return rc;
+    Scalars s0{};
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(&s0, c->sc, sizeof(s0), hipMemcpyDeviceToHost));
+    s0.wm_nu_limit = limit;
+    HIPCHK(c, hipMemcpy(c->sc, &s0, sizeof(s0), hipMemcpyHostToDevice));
     return CHAN_OK;
 }
 
@@ -1476,7 +1493,7 @@ int enqueue_group_step(chan_ensemble* e, const std::vector<int>& idx, TabSink& s
                 a[q] = DsmArgs<T>{u[0], u[1], u[2], b};
                 r[q] = DsmRatioArgs{b.part, b.ratio};
                 v[q] = DsmNutArgs<T>{(T*)c->nut, b.smag, b.ratio, u[0], c->alpha, metric<T>(c), T(c->nu),
-                                     c->Dirichlet_nu_t == 1};
+                                     c->Dirichlet_nu_t == 1, c->sc};
             }
             const DsmArgs<T>* ta = sink.put(a);
             const DsmRatioArgs* tr = sink.put(r);
@@ -1531,10 +1548,11 @@ int enqueue_group_step(chan_ensemble* e, const std::vector<int>& idx, TabSink& s
     // constant mass flow (its final reduction also advances the time), or the time advance alone
     {
         std::vector<MassArgs<T>> a(n);
-        for (int q = 0; q < n; q++) a[q] = MassArgs<T>{st(q, (X[q] + 1) % 3)[0], cs[q]->part, cs[q]->sc};
+        for (int q = 0; q < n; q++) a[q] = MassArgs<T>{st(q, (X[q] + 1) % 3)[0], cs[q]->part, cs[q]->sc, cs[q]->md};
         const MassArgs<T>* t = sink.put(a);
         if (sink.launch) {
-            if (c0->x_mass_cte == 1) run_mass_flow_correct<T>(a[0], t, n, d, c0->nblk_red, 1, 1, s);
+            if (c0->x_mass_cte == 1 || c0->x_mass_cte == 2)
+                run_mass_flow_correct<T>(a[0], t, n, d, c0->nblk_red, c0->x_mass_cte, 1, 1, s);
             else k_advance_time_g<T><<<n, 1, 0, s>>>(t);
         }
     }

@@ -90,7 +90,8 @@ __device__ __forceinline__ void store_img(T* F, const Dims& d, int kind, int i, 
 // from here, so a captured step needs no host round trip).
 struct Scalars {
     double dt, t, dPdx, Qflow_x_0;
-    double cfl, nu, dt_max, pad;
+    double cfl, nu, dt_max;
+    double wm_nu_limit;  // > 0: the wall model's viscosity limit (see wm_nut_at); 0: none
     double maxdiv, maxabsdiv, last_dPdx, pad2;
 };
 

@@ -42,7 +42,8 @@ int rhs_shape_count();
 
 template <typename T>
 void launch_dsm(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, const Dims& d,
-                DsmBufs<T>& b, int dirichlet, int fuse_wm, const double* alpha_x, T nu, hipStream_t s);
+                DsmBufs<T>& b, int dirichlet, int fuse_wm, const double* alpha_x, T nu, const Scalars* sc,
+                hipStream_t s);
 
 // AMD eddy viscosity (amd.hip), dirichlet = Dirichlet_nu_t (0, 1, 2)
 template <typename T>
@@ -60,7 +61,7 @@ void launch_dsm_fused(const T* U, const T* V, const T* W, const Metric<T>& m, co
 
 template <typename T>
 void launch_wall_model(const T* U, T* nut, const double* alpha_x, const Metric<T>& m, const Dims& d, T nu,
-                       hipStream_t s);
+                       const Scalars* sc, hipStream_t s);
 
 // Where the equilibrium wall model samples U: h_wm above each wall,
 // interpolated between centre rows jb and jb+1 with weight wb on row jb
@@ -94,6 +95,7 @@ struct DsmNutArgs {
     Metric<T> m;
     T nu;
     int dirichlet;
+    const Scalars* sc;
 };
 template <typename T>
 struct EqAlphaArgs {
@@ -108,6 +110,7 @@ struct MassArgs {
     T* U;
     double* part;
     Scalars* sc;
+    Metric<double> m;
 };
 template <typename T>
 struct DtArgs {
@@ -151,9 +154,11 @@ void run_dsm_nut(const DsmNutArgs<T>& one, const DsmNutArgs<T>* tab, int nmem, c
 template <typename T>
 void run_equilibrium_alpha(const EqAlphaArgs<T>& one, const EqAlphaArgs<T>* tab, int nmem, const Dims& d,
                            hipStream_t s);
+// mode 1: the reference's functional (code/mass_flow.f90:113-166); mode 2:
+// the bulk mean of code/mass_flow.f90:24-62 (constant bulk flow)
 template <typename T>
 void run_mass_flow_correct(const MassArgs<T>& one, const MassArgs<T>* tab, int nmem, const Dims& d, int nblk,
-                           int apply, int advance, hipStream_t s);
+                           int mode, int apply, int advance, hipStream_t s);
 template <typename T>
 void run_dt(const DtArgs<T>& one, const DtArgs<T>* tab, int nmem, const Dims& d, int nblk, hipStream_t s);
 template <typename T>
@@ -169,8 +174,8 @@ void launch_mass_flow_init(const T* U, const Metric<double>& md, const Dims& d, 
                            Scalars* sc, hipStream_t s);
 
 template <typename T>
-void launch_mass_flow_correct(T* U, const Dims& d, double* part, int nblk, Scalars* sc, int apply,
-                              hipStream_t s, int advance);
+void launch_mass_flow_correct(T* U, const Metric<double>& md, const Dims& d, double* part, int nblk, Scalars* sc,
+                              int mode, int apply, hipStream_t s, int advance);
 
 template <typename T>
 void launch_dt(const T* U, const T* V, const T* W, const Metric<double>& md, const Dims& d, double* part,

@@ -334,7 +334,7 @@ __device__ __forceinline__ double wm_driving_dpdx(const Scalars* sc, int x_mass_
 {
     // constant mass flow: the per-step correction U += last_dPdx acts as
     // last_dPdx / dt; otherwise the imposed dPdx
-    if (x_mass_cte == 1) return sc->dt > 0.0 ? sc->last_dPdx / sc->dt : 0.0;
+    if (x_mass_cte == 1 || x_mass_cte == 2) return sc->dt > 0.0 ? sc->last_dPdx / sc->dt : 0.0;
     return sc->dPdx;
 }
 

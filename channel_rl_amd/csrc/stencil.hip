@@ -127,7 +127,7 @@ __global__ void __launch_bounds__(256) k_dsm_nut(DsmNutArgs<T> one, const DsmNut
     const int i = q - jm * d.nxg + 1, j = jm + 1, k = g.z + 1;
     const long p = at(d, i, j, k);
     if (fuse_wm && (j <= 2 || j >= d.nyg - 1)) {
-        nut[p] = wm_nut_at<T>(U, alpha, m, d, nu, img_centre(i, d.nxg), k, j <= 2 ? 0 : 1);
+        nut[p] = wm_nut_at<T>(U, alpha, m, d, nu, img_centre(i, d.nxg), k, j <= 2 ? 0 : 1, a.sc);
         return;
     }
     const int jj = (j == 1) ? 2 : (j == d.nyg ? d.nyg - 1 : j);
@@ -156,17 +156,17 @@ void run_dsm_nut(const DsmNutArgs<T>& one, const DsmNutArgs<T>* tab, int nmem, c
 
 template <typename T>
 void launch_dsm(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, const Dims& d, DsmBufs<T>& b,
-                int dirichlet, int fuse_wm, const double* alpha_x, T nu, hipStream_t s)
+                int dirichlet, int fuse_wm, const double* alpha_x, T nu, const Scalars* sc, hipStream_t s)
 {
     launch_dsm_fused<T>(U, V, W, m, d, b, s);
     run_dsm_ratio(DsmRatioArgs{b.part, b.ratio}, nullptr, 1, d, b.nblk, s);
-    run_dsm_nut<T>(DsmNutArgs<T>{nut, b.smag, b.ratio, U, alpha_x, m, nu, dirichlet}, nullptr, 1, d, fuse_wm, s);
+    run_dsm_nut<T>(DsmNutArgs<T>{nut, b.smag, b.ratio, U, alpha_x, m, nu, dirichlet, sc}, nullptr, 1, d, fuse_wm, s);
 }
 
 // code/wallmodel.f90:23-46 on its own: wall rows from alpha_x, then the
 // x-periodic fill of every other row of nu_t.
 template <typename T>
-__global__ void k_wall_model(const T* U, T* nut, const double* alpha, Metric<T> m, Dims d, T nu)
+__global__ void k_wall_model(const T* U, T* nut, const double* alpha, Metric<T> m, Dims d, T nu, const Scalars* sc)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
@@ -174,7 +174,7 @@ __global__ void k_wall_model(const T* U, T* nut, const double* alpha, Metric<T> 
     if (i > d.nxg || j > d.nyg) return;
     const bool wallrow = (j <= 2 || j >= d.nyg - 1);
     if (wallrow) {
-        nut[at(d, i, j, k)] = wm_nut_at<T>(U, alpha, m, d, nu, img_centre(i, d.nxg), k, j <= 2 ? 0 : 1);
+        nut[at(d, i, j, k)] = wm_nut_at<T>(U, alpha, m, d, nu, img_centre(i, d.nxg), k, j <= 2 ? 0 : 1, sc);
     } else if (i == 1 || i >= d.nxg - 1) {
         nut[at(d, i, j, k)] = nut[at(d, img_centre(i, d.nxg), j, k)];
     }
@@ -182,11 +182,11 @@ __global__ void k_wall_model(const T* U, T* nut, const double* alpha, Metric<T> 
 
 template <typename T>
 void launch_wall_model(const T* U, T* nut, const double* alpha_x, const Metric<T>& m, const Dims& d, T nu,
-                       hipStream_t s)
+                       const Scalars* sc, hipStream_t s)
 {
     dim3 blk(64, 4, 1), grd(cdiv(d.nxg, 64), cdiv(d.nyg, 4), d.nzg);
     PROF_BEGIN("k_wall_model", s);
-    k_wall_model<T><<<grd, blk, 0, s>>>(U, nut, alpha_x, m, d, nu);
+    k_wall_model<T><<<grd, blk, 0, s>>>(U, nut, alpha_x, m, d, nu, sc);
     PROF_END(s);
 }
 
@@ -359,12 +359,39 @@ __global__ void k_shift_u(MassArgs<T> one, const MassArgs<T>* tab, Dims d)
     store_img(U, d, 0, i, j, k, U[p] + s);
 }
 
+// Constant bulk flow (mass-flow mode 2; no reference counterpart: the
+// reference's functional above holds the mean of two near-wall rows, under
+// which the bulk of a developing flow drifts): the shift that restores the
+// trapezoid bulk mean of code/mass_flow.f90:24-62, taken over every interior
+// row, to Qflow_x_0.  The shift moves that mean by exactly itself.
+template <typename T>
+__global__ void k_bulk_partial(MassArgs<T> one, const MassArgs<T>* tab, Dims d, int nblk)
+{
+    const GBlk g = gblk(1);
+    const MassArgs<T> ar = member_args(one, tab, g.m);
+    const T* U = ar.U;
+    const double* yg = ar.m.yg;
+    const long n1 = d.nx - 2, n2 = d.nyg - 3, n3 = d.nzp;
+    const long tot = n1 * n2 * n3;
+    double q = 0.0, nrm = 0.0;
+    for (long t = (long)g.x * RED_THREADS + threadIdx.x; t < tot; t += (long)nblk * RED_THREADS) {
+        const int i = (int)(t % n1) + 2;
+        const int j = (int)((t / n1) % n2) + 3;
+        const int k = (int)(t / (n1 * n2)) + 2;
+        const double dy = yg[j] - yg[j - 1];
+        q += ((double)U[at(d, i, j, k)] + (double)U[at(d, i, j - 1, k)]) * 0.5 * dy;
+        nrm += (1.0 + 1.0) * 0.5 * dy;
+    }
+    block_sum2(q, nrm, ar.part, nblk, g.x);
+}
+
 template <typename T>
 void run_mass_flow_correct(const MassArgs<T>& one, const MassArgs<T>* tab, int nmem, const Dims& d, int nblk,
-                           int apply, int advance, hipStream_t s)
+                           int mode, int apply, int advance, hipStream_t s)
 {
     PROF_BEGIN("k_dpx_partial", s);
-    k_dpx_partial<T><<<dim3(nblk, 1, nmem), RED_THREADS, 0, s>>>(one, tab, d, nblk);
+    if (mode == 2) k_bulk_partial<T><<<dim3(nblk, 1, nmem), RED_THREADS, 0, s>>>(one, tab, d, nblk);
+    else k_dpx_partial<T><<<dim3(nblk, 1, nmem), RED_THREADS, 0, s>>>(one, tab, d, nblk);
     PROF_END(s);
     PROF_BEGIN("k_dpx_final", s);
     k_dpx_final<T><<<dim3(1, 1, nmem), RED_THREADS, 0, s>>>(one, tab, nblk, advance);
@@ -379,10 +406,10 @@ void run_mass_flow_correct(const MassArgs<T>& one, const MassArgs<T>* tab, int n
 }
 
 template <typename T>
-void launch_mass_flow_correct(T* U, const Dims& d, double* part, int nblk, Scalars* sc, int apply, hipStream_t s,
-                              int advance)
+void launch_mass_flow_correct(T* U, const Metric<double>& md, const Dims& d, double* part, int nblk, Scalars* sc,
+                              int mode, int apply, hipStream_t s, int advance)
 {
-    run_mass_flow_correct<T>(MassArgs<T>{U, part, sc}, nullptr, 1, d, nblk, apply, advance, s);
+    run_mass_flow_correct<T>(MassArgs<T>{U, part, sc, md}, nullptr, 1, d, nblk, mode, apply, advance, s);
 }
 
 // code/time_integration.f90:102-150 (max of dx/|u| as the reference; max is
@@ -618,14 +645,15 @@ void launch_wall_shear(const T* U, const T* nut, const Metric<double>& md, const
 #define CHAN_INST(T)                                                                                            \
     template void launch_bc<T>(T*, const Dims&, int, hipStream_t);                                              \
     template void launch_dsm<T>(const T*, const T*, const T*, T*, const Metric<T>&, const Dims&, DsmBufs<T>&,   \
-                                int, int, const double*, T, hipStream_t);                                       \
+                                int, int, const double*, T, const Scalars*, hipStream_t);                       \
     template void launch_wall_model<T>(const T*, T*, const double*, const Metric<T>&, const Dims&, T,           \
-                                       hipStream_t);                                                            \
+                                       const Scalars*, hipStream_t);                                            \
     template void launch_equilibrium_alpha<T>(const T*, double*, const Dims&, double, const WallRows&,        \
                                               double, double, hipStream_t);                                     \
     template void launch_mass_flow_init<T>(const T*, const Metric<double>&, const Dims&, double*, int,          \
                                            Scalars*, hipStream_t);                                              \
-    template void launch_mass_flow_correct<T>(T*, const Dims&, double*, int, Scalars*, int, hipStream_t, int);  \
+    template void launch_mass_flow_correct<T>(T*, const Metric<double>&, const Dims&, double*, int, Scalars*,   \
+                                              int, int, hipStream_t, int);                                      \
     template void launch_dt<T>(const T*, const T*, const T*, const Metric<double>&, const Dims&, double*, int,  \
                                Scalars*, const double*, hipStream_t);                                           \
     template void launch_divergence_check<T>(const T*, const T*, const T*, const Metric<double>&, const Dims&, \
@@ -639,7 +667,7 @@ void launch_wall_shear(const T* U, const T* nut, const Metric<double>& md, const
     template void run_equilibrium_alpha<T>(const EqAlphaArgs<T>&, const EqAlphaArgs<T>*, int, const Dims&,      \
                                            hipStream_t);                                                        \
     template void run_mass_flow_correct<T>(const MassArgs<T>&, const MassArgs<T>*, int, const Dims&, int, int,  \
-                                           int, hipStream_t);                                                   \
+                                           int, int, hipStream_t);                                              \
     template void run_dt<T>(const DtArgs<T>&, const DtArgs<T>*, int, const Dims&, int, hipStream_t);
 
 CHAN_INST(double)

@@ -35,16 +35,26 @@ __device__ inline double loglaw_utau(double u, double h, double nu, double kappa
 
 // code/wallmodel.f90:32-37: wall-model eddy viscosity at a centre from the
 // two adjacent x faces; wall = 0 bottom (rows 1,2), 1 top (rows nyg-1,nyg).
+// With sc->wm_nu_limit = L > 0 (no reference counterpart) each face's total
+// viscosity alpha/(dU/dy) is held in [0, L dy^2/dt], dy = yg(2)-yg(1): the
+// formula asks for an unbounded viscosity where the resolved wall gradient
+// vanishes and a negative one where it reverses, either of which breaks the
+// explicit step (its diffusion number passes the stability bound); L = 0
+// keeps the reference's formula as it is.
 template <typename T>
 __device__ inline T wm_nut_at(const T* U, const double* alpha, const Metric<T>& m, const Dims& d, T nu,
-                                       int ic, int k, int wall)
+                                       int ic, int k, int wall, const Scalars* sc)
 {
     const T dyw = m.yg[2] - m.yg[1];
+    const double lim = sc ? sc->wm_nu_limit : 0.0;
+    const T numax = lim > 0.0 ? T(lim * (double)dyw * (double)dyw / sc->dt) : T(0);
     auto tau = [&](int f) {
         const T a = T(alpha[(f - 1) + (long)d.nx * (wall + 2L * (k - 1))]);
         const T du = wall == 0 ? (U[at(d, f, 2, k)] - U[at(d, f, 1, k)])
                                : (U[at(d, f, d.nyg - 1, k)] - U[at(d, f, d.nyg, k)]);
-        return a / (du / dyw) - nu;
+        T tot = a / (du / dyw);
+        if (lim > 0.0) tot = (tot >= T(0)) ? (tot <= numax ? tot : numax) : T(0);
+        return tot - nu;
     };
     return T(0.5) * (tau(ic - 1) + tau(ic));
 }

@@ -19,6 +19,8 @@ def make_channel(case, field, precision=FP64, wall=WALL_EQUILIBRIUM, net=None, m
     if net is not None:
         ch.set_wall_network(net, max_rel_err)
     ch.set_wall_model(wall, h_wm=case.h_wm)
+    if getattr(case, "wm_nu_limit", 0.0) > 0.0:
+        ch.set_wall_viscosity_limit(case.wm_nu_limit)
     return ch
 
 

@@ -52,7 +52,10 @@ typedef struct {
     double nu;                 /* viscosity                                */
     double CFL;                /* CFL, or fixed dt = -CFL when negative    */
     double dPdx;               /* pressure gradient                        */
-    int x_mass_cte;            /* 1: constant mass flow in x               */
+    int x_mass_cte;            /* 1: constant mass flow in x by the
+                                  reference's functional (the mean of U on
+                                  rows 3 and nyg-2); 2: constant bulk flow
+                                  (no reference counterpart); 0: none      */
     int LES_model;             /* 0 none, 2 dynamic Smagorinsky, 3 AMD     */
     int Dirichlet_nu_t;        /* 1 Dirichlet, 0 Neumann ghost for nu_t,
                                   2 Dirichlet with wall offset (AMD)       */
@@ -125,6 +128,13 @@ int chan_check_divergence(chan_ctx* ctx, double* max_div, double* max_abs_div);
  * (h_wm <= 0: first off-wall cell) before each substep's wall model.  No
  * reference counterpart: the log-law closure is new. */
 int chan_set_wall_model(chan_ctx* ctx, int mode, double h_wm, double kappa, double B);
+/* Limit on the wall model's viscosity (no reference counterpart; default 0:
+ * the reference's formula unchanged).  With limit > 0 the total viscosity
+ * alpha_x / (dU/dy) that code/wallmodel.f90:30-35 sets at each wall face is
+ * held in [0, limit (yg(2)-yg(1))^2 / dt]: where the resolved wall gradient
+ * vanishes or reverses the formula asks for an unbounded or negative
+ * viscosity, and the explicit step then diverges. */
+int chan_set_wall_viscosity_limit(chan_ctx* ctx, double limit);
 /* evaluate the equilibrium wall stress into alpha_x once */
 int chan_compute_equilibrium_wall_stress(chan_ctx* ctx, double h_wm, double kappa, double B);
 

@@ -19,10 +19,53 @@ _LIB = None
 _D = ctypes.POINTER(ctypes.c_double)
 
 
+# -O3 -march=x86-64-v3 build (oracle/build.sh) for the CPU baseline, and the
+# instruction-set features it needs from the host
+FAST_FLAGS = "-O3 -march=x86-64-v3 -fopenmp -ffp-contract=off"
+BASE_FLAGS = "-O2 -fopenmp -ffp-contract=off"
+_V3 = ("avx", "avx2", "bmi1", "bmi2", "fma", "f16c", "movbe", "abm", "xsave")
+_LIBNAME = "liborc.so"
+
+
+def host_supports_fast_build():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    have = set(line.split(":", 1)[1].split())
+                    return all(x in have for x in _V3)
+    except OSError:
+        pass
+    return False
+
+
+def use_fast_build():
+    """Load oracle/liborc_fast.so instead of liborc.so when the host supports
+    it and no Oracle has been made yet; returns the flags of the build in use."""
+    global _LIBNAME
+    if _LIB is not None:
+        return FAST_FLAGS if _LIBNAME == "liborc_fast.so" else BASE_FLAGS
+    if host_supports_fast_build() and os.path.exists(os.path.join(_HERE, "liborc_fast.so")):
+        _LIBNAME = "liborc_fast.so"
+        return FAST_FLAGS
+    return BASE_FLAGS
+
+
+def stage_times(reset=False):
+    """Accumulated wall seconds per stage (see orc_stage_times)."""
+    out = np.zeros(9)
+    lib().orc_stage_times(_p(out), 1 if reset else 0)
+    return out
+
+
+STAGE_NAMES = ("equilibrium_wall_stress", "eddy_viscosity", "wall_model", "rhs", "rk_combine",
+               "boundary_conditions", "projection", "mass_flow", "compute_dt")
+
+
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liborc.so")
+        path = os.path.join(_HERE, _LIBNAME)
         if not os.path.exists(path):
             raise RuntimeError("oracle/liborc.so is not built (run oracle/build.sh)")
         L = ctypes.CDLL(path)
@@ -68,6 +111,7 @@ def lib():
         L.orc_compute_pressure.argtypes = [vp, _D]
         L.orc_set_wall_model.argtypes = [vp, ctypes.c_int] + [ctypes.c_double] * 3
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_stage_times.argtypes = [_D, ctypes.c_int]
         _LIB = L
     return _LIB
 

@@ -18,6 +18,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -35,6 +36,36 @@ typedef double complex cplx;
 #define PAR_FOR
 #define PAR_FOR2
 #endif
+
+/* Wall time per stage of the RK3 step and of compute_dt, accumulated over
+ * calls (for the CPU baseline's per-stage report; not part of the checks):
+ * 0 equilibrium wall stress, 1 eddy viscosity, 2 wall model, 3 right-hand
+ * sides, 4 RK combine, 5 boundary conditions, 6 projection, 7 mass-flow
+ * correction, 8 compute_dt. */
+static double g_stage[ORC_NSTAGE];
+static double wall_now(void)
+{
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+#endif
+}
+void orc_stage_times(double* out, int reset)
+{
+    for (int q = 0; q < ORC_NSTAGE; q++) {
+        if (out) out[q] = g_stage[q];
+        if (reset) g_stage[q] = 0.0;
+    }
+}
+#define STAGE(q, stmt)                         \
+    do {                                       \
+        const double t0_ = wall_now();         \
+        stmt;                                  \
+        g_stage[q] += wall_now() - t0_;        \
+    } while (0)
 
 int orc_num_threads(void)
 {
@@ -1340,7 +1371,16 @@ double orc_compute_dPx_for_constant_mass_flow(const orc* o, const double* U)
 /* ----------------------------------------------------------------- dt */
 
 /* code/time_integration.f90:102-150 (Max of dx/|u| as the reference) */
+static double compute_dt_core(orc* o);
 double orc_compute_dt(orc* o)
+{
+    const double t0 = wall_now();
+    const double dt = compute_dt_core(o);
+    g_stage[8] += wall_now() - t0;
+    return dt;
+}
+
+static double compute_dt_core(orc* o)
 {
     const double *xg = o->xg, *yg = o->yg, *zg = o->zg;
     double lU = 0.0, lV = 0.0, lW = 0.0;
@@ -1380,13 +1420,16 @@ void orc_compute_time_step_RK3(orc* o)
     memcpy(o->Wo, o->W, nW(o) * sizeof(double));
     const double dt = o->dt;
     for (int s = 0; s < 3; s++) {
-        if (o->wall_mode == 1) orc_equilibrium_alpha(o, o->U, o->h_wm, o->kappa, o->Bc);
-        orc_compute_eddy_viscosity(o, o->U, o->V, o->W, o->nu_t);
-        orc_compute_wall_model(o, o->U, o->nu_t);
-        orc_compute_rhs_u(o, o->U, o->V, o->W, o->Fu[s]);
-        orc_compute_rhs_v(o, o->U, o->V, o->W, o->Fv[s]);
-        orc_compute_rhs_w(o, o->U, o->V, o->W, o->Fw[s]);
+        if (o->wall_mode == 1) STAGE(0, orc_equilibrium_alpha(o, o->U, o->h_wm, o->kappa, o->Bc));
+        STAGE(1, orc_compute_eddy_viscosity(o, o->U, o->V, o->W, o->nu_t));
+        STAGE(2, orc_compute_wall_model(o, o->U, o->nu_t));
+        STAGE(3, {
+            orc_compute_rhs_u(o, o->U, o->V, o->W, o->Fu[s]);
+            orc_compute_rhs_v(o, o->U, o->V, o->W, o->Fv[s]);
+            orc_compute_rhs_w(o, o->U, o->V, o->W, o->Fw[s]);
+        });
         const double* c = o->rk_coef[s];
+        const double t_rk = wall_now();
         PAR_FOR
         for (int k = 2; k <= o->nzg - 1; k++)
             for (int j = 2; j <= o->nyg - 1; j++)
@@ -1414,13 +1457,18 @@ void orc_compute_time_step_RK3(orc* o)
                     else if (s == 1) o->W[q] = o->Wo[q] + dt * (c[0] * o->Fw[0][q] + c[1] * o->Fw[1][q]);
                     else o->W[q] = o->Wo[q] + dt * (c[0] * o->Fw[0][q] + c[1] * o->Fw[1][q] + c[2] * o->Fw[2][q]);
                 }
+        g_stage[4] += wall_now() - t_rk;
         o->t = to + o->rk_t[s] * dt;
-        orc_apply_boundary_conditions(o);
-        orc_compute_projection_step(o);
-        orc_apply_boundary_conditions(o);
+        STAGE(5, orc_apply_boundary_conditions(o));
+        STAGE(6, orc_compute_projection_step(o));
+        STAGE(5, orc_apply_boundary_conditions(o));
     }
-    if (o->x_mass_cte == 1) {
-        o->dPdx = orc_compute_dPx_for_constant_mass_flow(o, o->U);
+    const double t_mf = wall_now();
+    if (o->x_mass_cte == 1 || o->x_mass_cte == 2) {
+        /* mode 2, constant bulk flow (no reference counterpart): the shift
+         * that restores the bulk mean of code/mass_flow.f90:24-62 */
+        o->dPdx = o->x_mass_cte == 1 ? orc_compute_dPx_for_constant_mass_flow(o, o->U)
+                                     : o->Qflow_x_0 - orc_compute_mean_mass_flow_U(o, o->U);
         PAR_FOR
         for (int k = 2; k <= o->nzg - 1; k++)
             for (int j = 2; j <= o->nyg - 1; j++)
@@ -1428,6 +1476,7 @@ void orc_compute_time_step_RK3(orc* o)
         o->dPdx = 0.0;
         orc_apply_boundary_conditions(o);
     }
+    g_stage[7] += wall_now() - t_mf;
 }
 
 /* ---------------------------------------------- equilibrium wall stress */
@@ -1470,6 +1519,8 @@ void orc_equilibrium_alpha(orc* o, const double* U, double h_wm, double kappa, d
         wb = (yg[jb + 1] - yt) / (yg[jb + 1] - yg[jb]);
         hb = h_wm;
     }
+    /* every (k, i) writes its own two entries */
+    PAR_FOR
     for (int k = 1; k <= o->nzg; k++)
         for (int i = 1; i <= nx; i++) {
             double ub = wb * AU(U, i, jb, k) + (1.0 - wb) * AU(U, i, jb + 1, k);

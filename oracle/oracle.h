@@ -79,6 +79,10 @@ void   orc_plane_stats(const orc* o, double* out /* 7 x ny */);
 /* OpenMP threads of the loops with disjoint outputs (1 without -fopenmp);
  * results are bit-identical for any count */
 int    orc_num_threads(void);
+/* accumulated wall time per stage (test infrastructure: the CPU baseline's
+ * per-stage report), ORC_NSTAGE values; reset != 0 clears them */
+#define ORC_NSTAGE 9
+void   orc_stage_times(double* out, int reset);
 
 #ifdef __cplusplus
 }

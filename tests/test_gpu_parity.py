@@ -327,3 +327,57 @@ def test_rk3_hundred_steps_config1(gpu):
     assert abs(sg["t"] - so["t"]) <= 1e-12 * so["t"] and abs(sg["dt"] - so["dt"]) <= 1e-12 * so["dt"]
     _, ma = g.check_divergence()
     assert ma <= 1e-10, ma
+
+
+def test_rk3_constant_bulk_flow_matches_oracle(gpu):
+    """Mass-flow mode 2 (constant bulk flow, no reference counterpart): five
+    steps against the oracle at 1e-11, and the bulk mean of
+    code/mass_flow.f90:24-62 held at Qflow_x_0 to round-off."""
+    case, o, g = run_pair(34, 33, 34, 5, x_mass_cte=2)
+    for a, b in zip(g.state(), o.state()):
+        assert rel(a, b) <= 1e-11, rel(a, b)
+    q0 = g.scalars()["Qflow_x_0"]
+    assert abs(g.compute_mean_mass_flow_U() - q0) <= 1e-13 * abs(q0)
+    assert abs(o.compute_mean_mass_flow_U(o.state()[0]) - q0) <= 1e-13 * abs(q0)
+
+
+def test_wall_viscosity_limit(gpu):
+    """chan_set_wall_viscosity_limit (no reference counterpart): with limit L
+    each wall face's total viscosity alpha/(dU/dy) is held in
+    [0, L (yg(2)-yg(1))^2 / dt] before the x average of
+    code/wallmodel.f90:37; L = 0 is the reference's formula (the oracle's)."""
+    case, o, g, (U, V, W) = make_pair(18, 17, 18)
+    rng = np.random.default_rng(8)
+    U = np.array(U, order="F")
+    # reversed, vanishing and ordinary wall gradients at both walls
+    U[:, 1, :] *= rng.choice([-1.0, 1e-6, 1.0], size=(case.nx, case.nzg))
+    U[:, 0, :] = -U[:, 1, :]
+    U[:, -2, :] *= rng.choice([-1.0, 1e-6, 1.0], size=(case.nx, case.nzg))
+    U[:, -1, :] = -U[:, -2, :]
+    alpha = np.asfortranarray(rng.uniform(0.5, 2.0, (case.nx, 2, case.nzg)))
+    nut = random_nut(case, 5)
+    g.set_state(U, V, W)
+    g.set_field(ALPHA_X, alpha)
+    dt, lim = 1e-3, 0.25
+    g.set_scalars(dt=dt)
+    g.set_field(NU_T, nut)
+    g.compute_wall_model()
+    o.set(ALPHA_X, alpha)
+    assert rel(g.get_field(NU_T), o.compute_wall_model(U, nut)) <= 1e-14   # limit 0: the reference
+    g.set_wall_viscosity_limit(lim)
+    g.set_field(NU_T, nut)
+    g.compute_wall_model()
+    got = g.get_field(NU_T)
+    yg = o.grid()["yg"]
+    dyw = yg[1] - yg[0]
+    numax = lim * dyw * dyw / dt
+    tot = np.empty((case.nx, 2, case.nzg))
+    tot[:, 0, :] = alpha[:, 0, :] / ((U[:, 1, :] - U[:, 0, :]) / dyw)
+    tot[:, 1, :] = alpha[:, 1, :] / ((U[:, -2, :] - U[:, -1, :]) / dyw)
+    tot = np.where(tot >= 0.0, np.minimum(tot, numax), 0.0) - case.nu
+    cen = 0.5 * (tot[:-1] + tot[1:])                 # x centres 2..nxg-1
+    # centres 1..nxg from their canonical cells: 1 <- nxg-2, nxg-1 <- 2, nxg <- 3
+    ref = np.concatenate([cen[-2:-1], cen[:-1], cen[:2]])
+    for row, wall in ((0, 0), (1, 0), (-2, 1), (-1, 1)):
+        assert np.max(np.abs(got[:, row, :] - ref[:, wall, :])) <= 1e-14 * numax
+    assert np.array_equal(got[:, 2:-2, :], g.get_field(NU_T)[:, 2:-2, :])
