@@ -175,15 +175,17 @@ def measure_batched(ch, case, args, cr, cases):
     mem = cases.ensemble(args.ensemble)
     ncell = sum(c.cells for c, _ in mem)
     k = args.ensemble_timed
-    times = {}
+    times, tuned = {}, {}
     for name, prec, wall in (("plain_fp64_equilibrium", cr.FP64, cr.WALL_EQUILIBRIUM),
                              ("plain_fp32_network", cr.FP32_BULK, cr.WALL_MLP)):
         chans = [E.make_channel(c, f, precision=prec, wall=wall, net=net) for c, f in mem]
+        tuned[name] = E.tune_members(chans)
         times[name] = E.time_sequential(chans, k)
         for c in chans:
             c.close()
         log(f"[bench] ensemble {name}: {len(mem) * k / times[name]:.1f} channel-steps/s")
     chans = [E.make_channel(c, f, precision=cr.FP32_BULK, wall=cr.WALL_MLP, net=net) for c, f in mem]
+    tuned["batched"] = E.tune_members(chans)
     ens = cr.Ensemble(chans)
     ens.set_grouping(False)
     times["batched_member_kernels"] = E.time_batched(ens, k)
@@ -212,6 +214,8 @@ def measure_batched(ch, case, args, cr, cases):
         "timed_steps": k,
         "channel_steps_per_s": cps,
         "groups": groups,
+        "tile_shapes": {name: {f"{k[0]} {k[1]} {k[2]}": {"rhs": v[0], "dsm": v[1]} for k, v in t.items()}
+                        for name, t in tuned.items()},
         # the batched + learned-wall-model form against the plain single-channel form
         "speedup": round(times["plain_fp64_equilibrium"] / tg, 3),
         # grouped batching alone, same precision and wall model on both sides
@@ -312,7 +316,10 @@ def main():
     ch.set_wall_model(cr.WALL_EQUILIBRIUM, h_wm=case.h_wm)
     cells = case.cells
     tune = ch.autotune(reps=5)
-    log(f"[bench] autotuned RHS tile shape {ch.tile_shape()}: " + " ".join(f"{1e3 * t:.1f}" for t in tune) + " us")
+    shapes = {"rhs": ch.tile_shape(cr.KERNEL_RHS), "dsm": ch.tile_shape(cr.KERNEL_DSM),
+              "rhs_us": [round(1e3 * t, 1) for t in tune["rhs"]], "dsm_us": [round(1e3 * t, 1) for t in tune["dsm"]]}
+    log(f"[bench] autotuned tile shapes: RHS {shapes['rhs']} ({' '.join(map(str, shapes['rhs_us']))} us), "
+        f"DSM {shapes['dsm']} ({' '.join(map(str, shapes['dsm_us']))} us)")
     log(f"[bench] {case.name} faces {case.nx} {case.ny} {case.nz}, {cells} cells, {args.precision}")
 
     ch.run_steps(args.warmup)
@@ -395,6 +402,7 @@ def main():
                                f"Smagorinsky, equilibrium wall model at h_wm = {case.h_wm}, constant mass flow, "
                                f"{args.precision} bulk",
                    "cells": cells, "parallelism": "single GPU", "max_abs_div": maxdiv},
+        "tile_shapes": shapes,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,

@@ -41,6 +41,7 @@ EXPORTS = [
     "chan_set_wall_viscosity_limit",
 ]
 KERNEL_RHS = 0
+KERNEL_DSM = 1
 WALL_MLP = 2
 MLP_NPARAM, MLP_NNORM = 8770, 12
 
@@ -488,10 +489,11 @@ class Channel:
         self._chk(load_library().chan_set_tile_shape(self._h, kernel, int(shape)), "set_tile_shape")
 
     def autotune(self, reps=10):
-        """Time every RHS tile shape on this channel and keep the fastest;
-        returns the average ms per launch of each shape."""
+        """Time every tile shape of the RHS and the DSM pass-A kernels on this
+        channel and keep the fastest of each; returns {"rhs": ms per launch of
+        each RHS shape, "dsm": the same for the DSM shapes}."""
         L = load_library()
-        n = L.chan_tile_shape_count(KERNEL_RHS)
-        t = np.zeros(n)
-        self._chk(L.chan_autotune(self._h, int(reps), _p(t), n), "autotune")
-        return t
+        nr, nd = L.chan_tile_shape_count(KERNEL_RHS), L.chan_tile_shape_count(KERNEL_DSM)
+        t = np.zeros(nr + nd)
+        self._chk(L.chan_autotune(self._h, int(reps), _p(t), nr + nd), "autotune")
+        return {"rhs": t[:nr], "dsm": t[nr:]}

@@ -108,7 +108,7 @@ struct chan_ctx {
     void* dsm[13] = {};
     const void* rcp[2][6] = {};  // reciprocal spacings [double, float][dx, dy, dz, dxg, dyg, dzg]
     double* part = nullptr;
-    int nblk_dsm = 0, nblk_red = 0, dsm_kchunk = 16;
+    int nblk_dsm = 0, nblk_red = 0, dsm_kchunk = 16, dsm_shape = 0;
     double* ratio = nullptr;
     double* alpha = nullptr;
     double* stage = nullptr;
@@ -263,6 +263,7 @@ DsmBufs<T> dsm_bufs(chan_ctx* c)
     b.ratio = c->ratio;
     b.nblk = c->nblk_dsm;
     b.kchunk = c->dsm_kchunk;
+    b.shape = c->dsm_shape;
     const int pi = sizeof(T) == sizeof(double) ? 0 : 1;
     b.rdx = (const T*)c->rcp[pi][0]; b.rdy = (const T*)c->rcp[pi][1]; b.rdz = (const T*)c->rcp[pi][2];
     b.rdxg = (const T*)c->rcp[pi][3]; b.rdyg = (const T*)c->rcp[pi][4]; b.rdzg = (const T*)c->rcp[pi][5];
@@ -416,6 +417,19 @@ int set_device(chan_ctx* c)
 {
     HIPCHK(c, hipSetDevice(c->device));
     return CHAN_OK;
+}
+
+// the DSM pass A tile shape q with its z chunk and partial-sum block count
+void set_dsm_shape(chan_ctx* c, int q)
+{
+    c->dsm_shape = q;
+    if (c->prec == CHAN_FP64) {
+        c->dsm_kchunk = dsm_fused_kchunk<double>(c->d, c->device, q);
+        c->nblk_dsm = dsm_fused_nblk<double>(c->d, c->dsm_kchunk, q);
+    } else {
+        c->dsm_kchunk = dsm_fused_kchunk<float>(c->d, c->device, q);
+        c->nblk_dsm = dsm_fused_nblk<float>(c->d, c->dsm_kchunk, q);
+    }
 }
 
 // the address of this device's copy of `scratch`, looked up once per device
@@ -616,12 +630,16 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
         c->mf.rdxg = (const float*)c->rcp[1][3]; c->mf.rdyg = (const float*)c->rcp[1][4];
         c->mf.rdzg = (const float*)c->rcp[1][5];
     }
-    // partial sums: DSM rows use one slot per (x,z) block of the fused kernel
-    c->dsm_kchunk = c->prec == CHAN_FP64 ? dsm_fused_kchunk<double>(d, c->device) : dsm_fused_kchunk<float>(d, c->device);
-    c->nblk_dsm = dsm_fused_nblk(d, c->dsm_kchunk);
+    // partial sums: DSM rows use one slot per (x,z) block of the fused
+    // kernel; room for the largest count any tile shape gives
+    size_t nblk_max = 0;
+    for (int q = dsm_shape_count() - 1; q >= 0; q--) {
+        set_dsm_shape(c, q);
+        nblk_max = std::max(nblk_max, (size_t)c->nblk_dsm);
+    }
     // reduction blocks: about eight (y, z) rows each, at most 1024
     c->nblk_red = (int)std::max(64L, std::min(1024L, (long)(d.nyg - 2) * (d.nzg - 2) / 8));
-    size_t npart = std::max((size_t)(d.nyg + 1) * 2 * c->nblk_dsm, (size_t)3 * c->nblk_red);
+    size_t npart = std::max((size_t)(d.nyg + 1) * 2 * nblk_max, (size_t)3 * c->nblk_red);
     if ((rc = dalloc(c, (void**)&c->part, npart * sizeof(double)))) return rc;
     if ((rc = dalloc(c, (void**)&c->ratio, (size_t)3 * d.nyg * sizeof(double)))) return rc;
     if ((rc = dalloc(c, (void**)&c->alpha, (size_t)d.nx * 2 * d.nzg * sizeof(double)))) return rc;
@@ -1214,23 +1232,26 @@ int chan_wall_model_stats(chan_ctx* c, double out2[2])
 
 int chan_tile_shape_count(int kernel)
 {
-    return kernel == CHAN_KERNEL_RHS ? rhs_shape_count() : 0;
+    return kernel == CHAN_KERNEL_RHS ? rhs_shape_count() : (kernel == CHAN_KERNEL_DSM ? dsm_shape_count() : 0);
 }
 
 int chan_set_tile_shape(chan_ctx* c, int kernel, int shape)
 {
     if (!c) return CHAN_EINVAL;
-    if (kernel != CHAN_KERNEL_RHS) return fail(c, CHAN_EINVAL, "unknown kernel for tile shapes");
-    if (shape < 0 || shape >= rhs_shape_count()) return fail(c, CHAN_EINVAL, "tile shape out of range");
-    c->rhs_shape = shape;
+    if (kernel != CHAN_KERNEL_RHS && kernel != CHAN_KERNEL_DSM) return fail(c, CHAN_EINVAL, "unknown kernel for tile shapes");
+    if (shape < 0 || shape >= chan_tile_shape_count(kernel)) return fail(c, CHAN_EINVAL, "tile shape out of range");
+    if (kernel == CHAN_KERNEL_RHS) c->rhs_shape = shape;
+    else set_dsm_shape(c, shape);
     drop_graphs(c);
     return CHAN_OK;
 }
 
 int chan_get_tile_shape(const chan_ctx* c, int kernel)
 {
-    if (!c || kernel != CHAN_KERNEL_RHS) return CHAN_EINVAL;
-    return c->rhs_shape;
+    if (!c) return CHAN_EINVAL;
+    if (kernel == CHAN_KERNEL_RHS) return c->rhs_shape;
+    if (kernel == CHAN_KERNEL_DSM) return c->dsm_shape;
+    return CHAN_EINVAL;
 }
 
 extern "C++" {
@@ -1244,48 +1265,67 @@ void tune_rhs(chan_ctx* c)
                      (T*)c->F1[0], (T*)c->F1[1], (T*)c->F1[2], R[0], R[1], R[2], c->rk_coef[1][0], c->rk_coef[1][1],
                      c->rhs_shape, c->stream);
 }
-int tune_rhs_launch(chan_ctx* c)
+template <typename T>
+void tune_dsm(chan_ctx* c)
 {
-    if (c->prec == CHAN_FP64) tune_rhs<double>(c);
-    else tune_rhs<float>(c);
+    T** I = (T**)c->state[c->cur];
+    DsmBufs<T> b = dsm_bufs<T>(c);
+    launch_dsm_fused<T>(I[0], I[1], I[2], metric<T>(c), c->d, b, c->stream);
+}
+int tune_launch(chan_ctx* c, int kernel)
+{
+    if (kernel == CHAN_KERNEL_RHS) {
+        if (c->prec == CHAN_FP64) tune_rhs<double>(c);
+        else tune_rhs<float>(c);
+    } else {
+        if (c->prec == CHAN_FP64) tune_dsm<double>(c);
+        else tune_dsm<float>(c);
+    }
     return check_launch(c);
 }
 }  // namespace
 }
 
+// every shape of each tuned kernel: one untimed launch, then reps timed ones
+// (the RHS kernel's substep-2 form writes into the rhs buffers and the DSM
+// pass A only |S| and its partial sums, so the state is unchanged); the
+// fastest shape of each kernel is kept
 int chan_autotune(chan_ctx* c, int reps, double* times_ms, int max_out)
 {
     if (!c || reps < 1) return CHAN_EINVAL;
     int rc = set_device(c);
     if (rc) return rc;
-    const int n = rhs_shape_count();
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
-    int best = c->rhs_shape;
-    double best_ms = 1e300;
-    const int keep = c->rhs_shape;
-    for (int q = 0; q < n && rc == CHAN_OK; q++) {
-        c->rhs_shape = q;
-        // one untimed launch, then reps timed ones, of the substep-2 form of
-        // the kernel (reads the state, Uo = the state, and F1; its output goes
-        // to the rhs buffers, so the state is unchanged)
-        float ms = 0.f;
-        for (int r = 0; r <= reps && rc == CHAN_OK; r++) {
-            if (r == 1) (void)hipEventRecord(e0, c->stream);
-            rc = tune_rhs_launch(c);
+    int out = 0;
+    for (int kernel : {CHAN_KERNEL_RHS, CHAN_KERNEL_DSM}) {
+        const int n = chan_tile_shape_count(kernel);
+        const int keep = chan_get_tile_shape(c, kernel);
+        int best = keep;
+        double best_ms = 1e300;
+        for (int q = 0; q < n && rc == CHAN_OK; q++, out++) {
+            if (kernel == CHAN_KERNEL_RHS) c->rhs_shape = q;
+            else set_dsm_shape(c, q);
+            float ms = 0.f;
+            for (int r = 0; r <= reps && rc == CHAN_OK; r++) {
+                if (r == 1) (void)hipEventRecord(e0, c->stream);
+                rc = tune_launch(c, kernel);
+            }
+            (void)hipEventRecord(e1, c->stream);
+            hipError_t e = hipEventSynchronize(e1);
+            if (!rc && e != hipSuccess) rc = fail(c, CHAN_EHIP, std::string("autotune: ") + hipGetErrorString(e));
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            const double avg = (double)ms / reps;
+            if (times_ms && out < max_out) times_ms[out] = avg;
+            if (avg < best_ms) { best_ms = avg; best = q; }
         }
-        (void)hipEventRecord(e1, c->stream);
-        hipError_t e = hipEventSynchronize(e1);
-        if (!rc && e != hipSuccess) rc = fail(c, CHAN_EHIP, std::string("autotune: ") + hipGetErrorString(e));
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        const double avg = (double)ms / reps;
-        if (times_ms && q < max_out) times_ms[q] = avg;
-        if (avg < best_ms) { best_ms = avg; best = q; }
+        const int pick = rc == CHAN_OK ? best : keep;
+        if (kernel == CHAN_KERNEL_RHS) c->rhs_shape = pick;
+        else set_dsm_shape(c, pick);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    c->rhs_shape = rc == CHAN_OK ? best : keep;
     drop_graphs(c);
     return rc;
 }
@@ -1423,7 +1463,8 @@ bool same_group(const chan_ctx* a, const chan_ctx* b)
 {
     return a->prec == b->prec && a->d.nx == b->d.nx && a->d.ny == b->d.ny && a->d.nz == b->d.nz &&
            a->LES_model == 2 && b->LES_model == 2 && a->wall_mode == b->wall_mode && a->x_mass_cte == b->x_mass_cte &&
-           a->rhs_shape == b->rhs_shape && a->dsm_kchunk == b->dsm_kchunk && a->nblk_dsm == b->nblk_dsm &&
+           a->rhs_shape == b->rhs_shape && a->dsm_shape == b->dsm_shape && a->dsm_kchunk == b->dsm_kchunk &&
+           a->nblk_dsm == b->nblk_dsm &&
            a->nblk_red == b->nblk_red && a->pb.bx == b->pb.bx && a->pb.bz == b->pb.bz;
 }
 
@@ -1499,7 +1540,7 @@ int enqueue_group_step(chan_ensemble* e, const std::vector<int>& idx, TabSink& s
             const DsmRatioArgs* tr = sink.put(r);
             const DsmNutArgs<T>* tv = sink.put(v);
             if (sink.launch) {
-                run_dsm_fused<T>(a[0], ta, n, d, c0->dsm_kchunk, s);
+                run_dsm_fused<T>(a[0], ta, n, d, c0->dsm_kchunk, c0->dsm_shape, s);
                 run_dsm_ratio(r[0], tr, n, d, c0->nblk_dsm, s);
                 run_dsm_nut<T>(v[0], tv, n, d, 1, s);
             }

@@ -429,47 +429,70 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     }
 }
 
-// Tile: 30 x-columns (a row of 32 columns is half a wave); y rows by type:
-// FP64 30x14 outputs (512 threads, 144 KB LDS, one block of 8 waves per CU:
-// less halo work per output than 30x6 at the same occupancy), FP32 30x6
-// (256 threads, 40 KB: three blocks per CU, as its registers allow).
-static constexpr int DSM_TXO = 30;
+// Tile shapes, (x outputs, y outputs): a row of TXO + 2 columns divides the
+// wave.  FP64 blocks have 512 threads (144 KB of LDS, one block of 8 waves
+// per CU), FP32 blocks 256 (40 KB).  Shape 0 has 30 outputs per row (32
+// columns, half a wave): the least halo per output on long rows; shape 1 has
+// 14 (16 columns) and twice the rows: the same halo per output, and less of
+// the last x tile left empty on short rows (33, 65 or 97 cells: 60, 90, 120
+// columns of 30 against 42, 70, 98 of 14).
+struct DsmShape { int txo, tyo; };
 template <typename T>
-struct DsmRows {
-    static constexpr int TYO = sizeof(T) == 8 ? 14 : 6;
-};
+constexpr DsmShape dsm_shape(int q)
+{
+    return sizeof(T) == 8 ? (q == 0 ? DsmShape{30, 14} : DsmShape{14, 30})
+                          : (q == 0 ? DsmShape{30, 6} : DsmShape{14, 14});
+}
+int dsm_shape_count() { return 2; }
+
+template <typename T, int TXO, int TYO>
+int dsm_per_cu()
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dsm_fused<T, TXO, TYO, false>, (TXO + 2) * (TYO + 2),
+                                                     0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    return per_cu;
+}
 
 // z-chunk length filling the last round of blocks (e.g. 192x97x192 FP64,
 // 49 tiles per chunk on 256 slots: 39 planes, 5 chunks, one round, against
 // 20 planes, 10 chunks, two rounds).
 template <typename T>
-int dsm_fused_kchunk(const Dims& d, int device)
+int dsm_fused_kchunk(const Dims& d, int device, int shape)
 {
-    int per_cu = 0, ncu = 0;
-    constexpr int TYO = DsmRows<T>::TYO;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_dsm_fused<T, DSM_TXO, TYO, false>, (DSM_TXO + 2) * (TYO + 2),
-                                                     0) != hipSuccess ||
-        per_cu < 1)
-        per_cu = 1;
+    int ncu = 0;
+    const DsmShape sh = dsm_shape<T>(shape);
+    const int per_cu = shape == 0 ? dsm_per_cu<T, dsm_shape<T>(0).txo, dsm_shape<T>(0).tyo>()
+                                  : dsm_per_cu<T, dsm_shape<T>(1).txo, dsm_shape<T>(1).tyo>();
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu < 1) ncu = 1;
-    const long tiles = (long)cdiv(d.nxg - 2, DSM_TXO) * cdiv(d.nyg - 2, TYO);
+    const long tiles = (long)cdiv(d.nxg - 2, sh.txo) * cdiv(d.nyg - 2, sh.tyo);
     return round_filling_chunk(tiles, d.nzg - 2, (long)per_cu * ncu, 8, 64);
 }
 
-int dsm_fused_nblk(const Dims& d, int kchunk)
+template <typename T>
+int dsm_fused_nblk(const Dims& d, int kchunk, int shape)
 {
-    return cdiv(d.nxg - 2, DSM_TXO) * cdiv(d.nzg - 2, kchunk);
+    return cdiv(d.nxg - 2, dsm_shape<T>(shape).txo) * cdiv(d.nzg - 2, kchunk);
 }
 
 template <typename T>
-void run_dsm_fused(const DsmArgs<T>& one, const DsmArgs<T>* tab, int nmem, const Dims& d, int kchunk, hipStream_t s)
+void run_dsm_fused(const DsmArgs<T>& one, const DsmArgs<T>* tab, int nmem, const Dims& d, int kchunk, int shape,
+                   hipStream_t s)
 {
-    constexpr int TYO = DsmRows<T>::TYO;
-    dim3 blk((DSM_TXO + 2) * (TYO + 2));
-    dim3 grd(cdiv(d.nxg - 2, DSM_TXO), cdiv(d.nyg - 2, TYO), (unsigned)cdiv(d.nzg - 2, kchunk) * nmem);
     PROF_BEGIN("k_dsm_fused", s);
-    if (tab) k_dsm_fused<T, DSM_TXO, TYO, true><<<grd, blk, 0, s>>>(one, tab, d, kchunk);
-    else k_dsm_fused<T, DSM_TXO, TYO, false><<<grd, blk, 0, s>>>(one, tab, d, kchunk);
+#define DSM_GO(Q)                                                                                                   \
+    do {                                                                                                            \
+        constexpr DsmShape sh = dsm_shape<T>(Q);                                                                    \
+        dim3 blk((sh.txo + 2) * (sh.tyo + 2));                                                                      \
+        dim3 grd(cdiv(d.nxg - 2, sh.txo), cdiv(d.nyg - 2, sh.tyo), (unsigned)cdiv(d.nzg - 2, kchunk) * nmem);       \
+        if (tab) k_dsm_fused<T, sh.txo, sh.tyo, true><<<grd, blk, 0, s>>>(one, tab, d, kchunk);                     \
+        else k_dsm_fused<T, sh.txo, sh.tyo, false><<<grd, blk, 0, s>>>(one, tab, d, kchunk);                        \
+    } while (0)
+    if (shape == 1) DSM_GO(1);
+    else DSM_GO(0);
+#undef DSM_GO
     PROF_END(s);
 }
 
@@ -478,17 +501,20 @@ void launch_dsm_fused(const T* U, const T* V, const T* W, const Metric<T>& m, co
                       hipStream_t s)
 {
     (void)m;
-    run_dsm_fused<T>(DsmArgs<T>{U, V, W, b}, nullptr, 1, d, b.kchunk, s);
+    run_dsm_fused<T>(DsmArgs<T>{U, V, W, b}, nullptr, 1, d, b.kchunk, b.shape, s);
 }
 
-template int dsm_fused_kchunk<double>(const Dims&, int);
-template int dsm_fused_kchunk<float>(const Dims&, int);
+template int dsm_fused_kchunk<double>(const Dims&, int, int);
+template int dsm_fused_kchunk<float>(const Dims&, int, int);
+template int dsm_fused_nblk<double>(const Dims&, int, int);
+template int dsm_fused_nblk<float>(const Dims&, int, int);
 template void launch_dsm_fused<double>(const double*, const double*, const double*, const Metric<double>&,
                                        const Dims&, DsmBufs<double>&, hipStream_t);
 template void launch_dsm_fused<float>(const float*, const float*, const float*, const Metric<float>&, const Dims&,
                                       DsmBufs<float>&, hipStream_t);
-template void run_dsm_fused<double>(const DsmArgs<double>&, const DsmArgs<double>*, int, const Dims&, int,
+template void run_dsm_fused<double>(const DsmArgs<double>&, const DsmArgs<double>*, int, const Dims&, int, int,
                                     hipStream_t);
-template void run_dsm_fused<float>(const DsmArgs<float>&, const DsmArgs<float>*, int, const Dims&, int, hipStream_t);
+template void run_dsm_fused<float>(const DsmArgs<float>&, const DsmArgs<float>*, int, const Dims&, int, int,
+                                   hipStream_t);
 
 }  // namespace chan

@@ -17,6 +17,7 @@ struct DsmBufs {
     double* ratio; // <LM>/<MM> per row (index j-1); also LM, MM at +nyg, +2nyg
     int nblk;      // partial-sum blocks per row
     int kchunk;    // z planes per block of the fused pass A
+    int shape;     // tile shape of the fused pass A (dsm_shape_count)
     // reciprocal spacings, 1-based: 1/(x(i)-x(i-1)), 1/(xg(i)-xg(i-1)), ...
     const T *rdx, *rdy, *rdz, *rdxg, *rdyg, *rdzg;
 };
@@ -53,8 +54,10 @@ void launch_amd(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, 
 // fused pass A of the dynamic Smagorinsky model (dsm_fused.hip): z-chunk
 // length for this grid and device, and the partial-sum blocks per row it gives
 template <typename T>
-int dsm_fused_kchunk(const Dims& d, int device);
-int dsm_fused_nblk(const Dims& d, int kchunk);
+int dsm_fused_kchunk(const Dims& d, int device, int shape);
+template <typename T>
+int dsm_fused_nblk(const Dims& d, int kchunk, int shape);
+int dsm_shape_count();
 template <typename T>
 void launch_dsm_fused(const T* U, const T* V, const T* W, const Metric<T>& m, const Dims& d, DsmBufs<T>& b,
                       hipStream_t s);
@@ -162,7 +165,8 @@ void run_mass_flow_correct(const MassArgs<T>& one, const MassArgs<T>* tab, int n
 template <typename T>
 void run_dt(const DtArgs<T>& one, const DtArgs<T>* tab, int nmem, const Dims& d, int nblk, hipStream_t s);
 template <typename T>
-void run_dsm_fused(const DsmArgs<T>& one, const DsmArgs<T>* tab, int nmem, const Dims& d, int kchunk, hipStream_t s);
+void run_dsm_fused(const DsmArgs<T>& one, const DsmArgs<T>* tab, int nmem, const Dims& d, int kchunk, int shape,
+                   hipStream_t s);
 template <typename T>
 void run_rhs_rk(const RhsArgs<T>& one, const RhsArgs<T>* tab, int nmem, const Dims& d, int mode, double c1,
                 double c2, int shape, hipStream_t s);

@@ -24,6 +24,24 @@ def make_channel(case, field, precision=FP64, wall=WALL_EQUILIBRIUM, net=None, m
     return ch
 
 
+def tune_members(channels, reps=3):
+    """The autotuner on one channel of each shape and precision, its tile
+    shapes then set on the others of that kind (channels of one shape keep
+    one set of shapes, so they still form one group).  Returns {(nx, ny, nz,
+    precision): (rhs shape, dsm shape)}."""
+    from . import KERNEL_DSM, KERNEL_RHS
+    picked = {}
+    for ch in channels:
+        key = (ch.nx, ch.ny, ch.nz, ch.precision)
+        if key not in picked:
+            ch.autotune(reps=reps)
+            picked[key] = (ch.tile_shape(KERNEL_RHS), ch.tile_shape(KERNEL_DSM))
+        else:
+            ch.set_tile_shape(picked[key][0], KERNEL_RHS)
+            ch.set_tile_shape(picked[key][1], KERNEL_DSM)
+    return picked
+
+
 def time_sequential(channels, steps):
     """Seconds for `steps` steps of every channel, one channel after the other
     (each with its own captured step)."""

@@ -271,12 +271,16 @@ int chan_ensemble_synchronize(chan_ensemble* ens);
 
 /* Tile shapes and the autotuner.  No reference counterpart (the reference's
  * loops have no tiles).  CHAN_KERNEL_RHS selects the fused RHS + RK kernel,
- * whose shapes differ in (x, y) tile and z-chunk length.  chan_autotune times
- * every shape on this channel (reps launches each of the kernel's substep-2
- * form, writing into the rhs buffers: the state is not changed), keeps the
- * fastest for the step and, when times_ms is not NULL, writes the average ms
- * per launch of the first max_out shapes. */
-enum { CHAN_KERNEL_RHS = 0 };
+ * whose shapes differ in (x, y) tile and z-chunk length; CHAN_KERNEL_DSM the
+ * dynamic Smagorinsky pass A (x, y tile; its z chunk follows the grid).
+ * chan_autotune times every shape of both kernels on this channel (reps
+ * launches each: the RHS kernel's substep-2 form writing into the rhs
+ * buffers, the DSM pass A writing |S| and its partial sums; the state is not
+ * changed), keeps the fastest of each for the step and, when times_ms is not
+ * NULL, writes the average ms per launch of the RHS shapes followed by the
+ * DSM shapes, at most max_out values.  DSM shapes change the order of the
+ * plane sums (results agree to rounding, not bit for bit). */
+enum { CHAN_KERNEL_RHS = 0, CHAN_KERNEL_DSM = 1 };
 int chan_tile_shape_count(int kernel);
 int chan_set_tile_shape(chan_ctx* ctx, int kernel, int shape);
 int chan_get_tile_shape(const chan_ctx* ctx, int kernel);  /* shape index, or < 0 */

@@ -207,11 +207,42 @@ def test_rhs_tile_shapes_agree_bitwise(gpu):
     g = Channel(case.x, case.y, case.z, precision=FP64, **case.kwargs())
     g.set_state(U, V, W)
     before = g.state()
-    t = g.autotune(reps=2)
+    t = g.autotune(reps=2)["rhs"]
     assert t.shape == (n,) and np.all(t > 0)
     assert 0 <= g.tile_shape() < n and t[g.tile_shape()] == t.min()
     for a, b in zip(g.state(), before):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", [FP64, "fp32"])
+def test_dsm_tile_shapes_match_oracle(gpu, precision):
+    """Every tile shape of the DSM pass A (30 or 14 outputs per row) gives
+    the oracle's eddy viscosity (FP64: 1e-12; FP32 bulk: 1e-3 of its
+    maximum, the FP32 rounding of the inputs through the differences of
+    filtered products in L_ij and M_ij: 1.5e-4 measured), on grids whose rows
+    fill the tiles unevenly; the autotuner
+    picks the fastest DSM shape as well and leaves the state alone."""
+    from channel_rl_amd import FP32_BULK, KERNEL_DSM, load_library
+    prec = FP64 if precision == FP64 else FP32_BULK
+    tol = 1e-12 if prec == FP64 else 1e-3
+    n = load_library().chan_tile_shape_count(KERNEL_DSM)
+    assert n == 2
+    for dims in ((34, 33, 34), (22, 13, 32)):
+        case, o, g, (U, V, W) = make_pair(*dims)
+        ref = o.compute_eddy_viscosity(U, V, W)
+        for shape in range(n):
+            g = Channel(case.x, case.y, case.z, precision=prec, **case.kwargs())
+            g.set_state(U, V, W)
+            g.set_tile_shape(shape, KERNEL_DSM)
+            assert g.tile_shape(KERNEL_DSM) == shape
+            g.compute_eddy_viscosity()
+            got = g.get_field(NU_T)
+            assert rel(got, ref) <= tol, (dims, shape, rel(got, ref))
+        before = g.state()
+        t = g.autotune(reps=2)["dsm"]
+        assert t.shape == (n,) and np.all(t > 0) and t[g.tile_shape(KERNEL_DSM)] == t.min()
+        for a, b in zip(g.state(), before):
+            assert np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("n,dirichlet", [((10, 9, 10), 1), ((18, 17, 18), 0), ((34, 33, 34), 2)])
