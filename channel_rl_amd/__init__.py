@@ -105,7 +105,6 @@ def load_library(path=LIB_PATH):
     _F = ctypes.POINTER(ctypes.c_float)
     L.chan_mlp_create.argtypes = [ctypes.c_int, ctypes.c_ulonglong, ctypes.POINTER(vp)]
     L.chan_mlp_destroy.argtypes = [vp]
-    L.chan_mlp_destroy.restype = None
     L.chan_mlp_last_error.argtypes = [vp]
     L.chan_mlp_last_error.restype = ctypes.c_char_p
     L.chan_mlp_get_params.argtypes = [vp, _F, _F]
@@ -222,12 +221,18 @@ class WallNetwork:
             raise ChanError(f"chan_mlp_create: {msg} (status {rc})")
 
     def close(self):
+        """Release the network; refused (ChanError) while a channel still uses
+        it as its wall model."""
         if getattr(self, "_h", None):
-            load_library().chan_mlp_destroy(self._h)
+            self._chk(load_library().chan_mlp_destroy(self._h), "close")
             self._h = None
 
     def __del__(self):
-        self.close()
+        # channels hold a reference while attached, so none is left here
+        try:
+            self.close()
+        except ChanError:
+            pass
 
     def _chk(self, rc, what):
         if rc != 0:
@@ -301,6 +306,7 @@ class Channel:
         if getattr(self, "_h", None):
             load_library().chan_destroy(self._h)
             self._h = None
+        self._net = None
 
     def __del__(self):
         self.close()

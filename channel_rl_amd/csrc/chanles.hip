@@ -342,10 +342,10 @@ void enqueue_rk3(chan_ctx* c, int X)
         if (c->wall_mode == CHAN_WALL_EQUILIBRIUM) {
             launch_equilibrium_alpha<T>(I[0], c->alpha, c->d, c->nu, wall_rows(c, c->h_wm), c->kappa, c->Bc, c->stream);
         } else if (c->wall_mode == CHAN_WALL_MLP) {
-            const float *np = nullptr, *nn = nullptr;
+            const void* net = nullptr;
             int ndev = 0;
-            if (mlp_device_params(c->net, &np, &nn, &ndev))
-                launch_wm_mlp<T>(np, nn, I[0], I[2], c->alpha, c->sc, c->md, c->d, c->nu, c->h_wm, c->kappa, c->Bc,
+            if (mlp_device_net(c->net, &net, &ndev))
+                launch_wm_mlp<T>(net, I[0], I[2], c->alpha, c->sc, c->md, c->d, c->nu, c->h_wm, c->kappa, c->Bc,
                                  c->x_mass_cte, c->wm_max_err, c->wm_stats, c->stream);
         }
         enqueue_eddy_viscosity<T>(c, in[s], 1);
@@ -416,6 +416,13 @@ void enqueue_dt(chan_ctx* c)
 int set_device(chan_ctx* c)
 {
     HIPCHK(c, hipSetDevice(c->device));
+    return CHAN_OK;
+}
+
+// the channel's next launches after the last change of its wall network
+int net_order(chan_ctx* c)
+{
+    if (c->wall_mode == CHAN_WALL_MLP && c->net) HIPCHK(c, hipStreamWaitEvent(c->stream, mlp_ready(c->net), 0));
     return CHAN_OK;
 }
 
@@ -686,6 +693,7 @@ void chan_destroy(chan_ctx* c)
     for (int q = 0; q < 3; q++)
         if (c->graph[q]) (void)hipGraphExecDestroy(c->graph[q]);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->net) mlp_detach(c->net, c->stream);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -949,6 +957,7 @@ int chan_compute_time_step_RK3(chan_ctx* c)
     if (!c) return CHAN_EINVAL;
     int rc = set_device(c);
     if (rc) return rc;
+    if ((rc = net_order(c))) return rc;
     return DISPATCH(c, rk3_step, c);
 }
 
@@ -1183,14 +1192,15 @@ int chan_evaluate_wall_stress(chan_ctx* c)
             launch_equilibrium_alpha<float>((float*)s[0], c->alpha, c->d, c->nu, wall_rows(c, c->h_wm), c->kappa,
                                          c->Bc, c->stream);
     } else if (c->wall_mode == CHAN_WALL_MLP) {
-        const float *np = nullptr, *nn = nullptr;
+        const void* net = nullptr;
         int ndev = 0;
-        if (!mlp_device_params(c->net, &np, &nn, &ndev)) return fail(c, CHAN_ESTATE, "no wall network set");
+        if (!mlp_device_net(c->net, &net, &ndev)) return fail(c, CHAN_ESTATE, "no wall network set");
+        if ((rc = net_order(c))) return rc;
         if (c->prec == CHAN_FP64)
-            launch_wm_mlp<double>(np, nn, (double*)s[0], (double*)s[2], c->alpha, c->sc, c->md, c->d, c->nu,
+            launch_wm_mlp<double>(net, (double*)s[0], (double*)s[2], c->alpha, c->sc, c->md, c->d, c->nu,
                                   c->h_wm, c->kappa, c->Bc, c->x_mass_cte, c->wm_max_err, c->wm_stats, c->stream);
         else
-            launch_wm_mlp<float>(np, nn, (float*)s[0], (float*)s[2], c->alpha, c->sc, c->md, c->d, c->nu, c->h_wm,
+            launch_wm_mlp<float>(net, (float*)s[0], (float*)s[2], c->alpha, c->sc, c->md, c->d, c->nu, c->h_wm,
                                  c->kappa, c->Bc, c->x_mass_cte, c->wm_max_err, c->wm_stats, c->stream);
     }
     return check_launch(c);
@@ -1201,12 +1211,18 @@ int chan_set_wall_network(chan_ctx* c, chan_mlp* net, double max_rel_err)
     if (!c) return CHAN_EINVAL;
     if (!(max_rel_err >= 0.0)) return fail(c, CHAN_EINVAL, "max_rel_err must be >= 0");
     if (net) {
-        const float *p, *n;
+        const void* img = nullptr;
         int dev = -1;
-        if (!mlp_device_params(net, &p, &n, &dev)) return fail(c, CHAN_EINVAL, "invalid network");
+        if (!mlp_device_net(net, &img, &dev)) return fail(c, CHAN_EINVAL, "invalid network");
         if (dev != c->device) return fail(c, CHAN_EINVAL, "the network lives on another device");
     }
     if (!net && c->wall_mode == CHAN_WALL_MLP) return fail(c, CHAN_ESTATE, "the MLP wall model is in use");
+    if (net != c->net) {
+        int rc = set_device(c);
+        if (rc) return rc;
+        if (net) HIPCHK(c, mlp_attach(net, c->stream));
+        if (c->net) mlp_detach(c->net, c->stream);
+    }
     c->net = net;
     c->wm_max_err = max_rel_err;
     drop_graphs(c);
@@ -1512,11 +1528,11 @@ int enqueue_group_step(chan_ensemble* e, const std::vector<int>& idx, TabSink& s
             std::vector<WmArgs<T>> a(n);
             for (int q = 0; q < n; q++) {
                 chan_ctx* c = cs[q];
-                const float *np = nullptr, *nn = nullptr;
+                const void* net = nullptr;
                 int ndev = 0;
-                if (!mlp_device_params(c->net, &np, &nn, &ndev)) return efail(e, CHAN_ESTATE, "a member's wall network is gone");
+                if (!mlp_device_net(c->net, &net, &ndev)) return efail(e, CHAN_ESTATE, "a member's wall network is gone");
                 T** u = st(q, inb(q));
-                a[q] = WmArgs<T>{np, nn, u[0], u[2], c->alpha, c->sc, c->md, c->nu, c->h_wm, c->kappa, c->Bc,
+                a[q] = WmArgs<T>{net, u[0], u[2], c->alpha, c->sc, c->md, c->nu, c->h_wm, c->kappa, c->Bc,
                                  loglaw_crossover(c->kappa, c->Bc), c->wm_max_err, c->wm_stats, c->x_mass_cte};
             }
             const WmArgs<T>* t = sink.put(a);
@@ -1728,10 +1744,13 @@ int chan_ensemble_run_steps(chan_ensemble* e, int nsteps)
         for (chan_ctx* c : e->ch) { e->cur0.push_back(c->cur); e->gen0.push_back(c->generation); }
         ens_form_groups(e);
     }
-    // order after whatever the members' own streams hold
+    // order after whatever the members' own streams hold and after the last
+    // change of their wall networks
     for (int q = 0; q < n; q++) {
         h = hipEventRecord(e->join[q], e->ch[q]->stream);
         if (h == hipSuccess) h = hipStreamWaitEvent(e->stream, e->join[q], 0);
+        if (h == hipSuccess && e->ch[q]->wall_mode == CHAN_WALL_MLP && e->ch[q]->net)
+            h = hipStreamWaitEvent(e->stream, mlp_ready(e->ch[q]->net), 0);
         if (h != hipSuccess) return efail(e, CHAN_EHIP, std::string("ensemble order: ") + hipGetErrorString(h));
     }
     for (int it = 0; it < nsteps; it++) {

@@ -139,7 +139,7 @@ struct RhsArgs {
 };
 template <typename T>
 struct WmArgs {
-    const float *p, *norm;  // the network's parameters and standardisation
+    const void* net;  // the network's inference image (mlp_device_net)
     const T *U, *W;
     double* alpha;
     const Scalars* sc;
@@ -272,17 +272,23 @@ void launch_poisson_inverse(const Dims& d, PoissonBufs& pb, hipStream_t s);
 // forward x pass of a given centre field (for the standalone Poisson solve)
 void launch_poisson_forward_field(const double* rhs, const Dims& d, PoissonBufs& pb, hipStream_t s);
 
-// the learned wall model (mlp.hip); mlp_device_params gives a network's
-// device parameters, standardisation constants and device
-bool mlp_device_params(const ::chan_mlp* net, const float** p, const float** norm, int* device);
-void launch_mlp_infer(const float* p, const float* norm, const float* X, float* Y, int n, hipStream_t s);
+// the learned wall model (mlp.hip).  mlp_device_net gives a network's
+// inference image and device.  A channel that reads the network attaches
+// its stream (mlp_attach, mlp_detach on release) and, before each launch
+// that reads it, waits on mlp_ready: parameter changes then never overlap
+// its steps.
+bool mlp_device_net(const ::chan_mlp* net, const void** img, int* device);
+hipEvent_t mlp_ready(const ::chan_mlp* net);
+hipError_t mlp_attach(::chan_mlp* net, hipStream_t s);
+void mlp_detach(::chan_mlp* net, hipStream_t s);
+void launch_mlp_infer(const void* img, const float* X, float* Y, int n, hipStream_t s);
 template <typename T>
 void launch_wm_samples(const T* U, const T* W, const double* alpha, const Scalars* sc, const Metric<double>& m,
                        const Dims& d, double nu, double h_wm, double kappa, double B, int x_mass_cte, float* X, float* Y,
                        double* Tq, hipStream_t s);
 long wm_mlp_groups(const Dims& d);
 template <typename T>
-void launch_wm_mlp(const float* p, const float* norm, const T* U, const T* W, double* alpha, const Scalars* sc,
+void launch_wm_mlp(const void* net, const T* U, const T* W, double* alpha, const Scalars* sc,
                    const Metric<double>& m, const Dims& d, double nu, double h_wm, double kappa, double B,
                    int x_mass_cte, double max_err, double* stats, hipStream_t s);
 

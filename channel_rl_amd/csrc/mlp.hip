@@ -19,6 +19,7 @@
 // are the B operand of the next without any data movement.
 #include <hip/hip_runtime.h>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "chanles.h"
@@ -38,23 +39,49 @@ __device__ __forceinline__ float dsilu(float z)
     const float s = 1.0f / (1.0f + expf(-z));
     return s * (1.0f + z * (1.0f - s));
 }
+// SiLU from the hardware's base-2 exponential and reciprocal (v_exp_f32,
+// v_rcp_f32; about 1 ulp each, plus the rounding of -z log2(e), a relative
+// 1e-6 for |z| < 20): five instructions where the library's correctly
+// rounded expf and division take about twenty.  For the wall model in the
+// step only: the activations are rounded to BF16 (8 bits) before the next
+// layer, and the ~5x larger error here moves a few more of them across a
+// rounding boundary than the library form does, so the network functions
+// that are compared with the FP64 oracle sample by sample (inference,
+// training) keep the library form.  z -> -inf gives exp2 = inf and 0.
+__device__ __forceinline__ float silu_fast(float z)
+{
+    return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+}
+template <bool FAST>
+__device__ __forceinline__ float silu_sel(float z)
+{
+    return FAST ? silu_fast(z) : silu(z);
+}
 __device__ __forceinline__ float bfr(float x) { return (float)(__bf16)x; }
 
-// LDS image of the network for one block: MFMA A operands pre-permuted so
-// that each lane reads its 8 values with one 16-byte load.
-//   fw[L][m][q][lane]: forward operand of hidden layer L (0: W2, 1: W3),
-//                      A[n = 16m + l%16][k] = W[k][n]
-//   bw[L][m][q][lane]: backward operand, A[k = 16m + l%16][n] = W[k][n]
-struct MlpLds {
+// The network as the inference kernels read it: the forward MFMA A operands
+// pre-permuted so that each lane reads its 8 values with one 16-byte load,
+//   fw[L][m][q][lane]: hidden layer L (0: W2, 1: W3), A[n = 16m + l%16][k] = W[k][n],
+// the input and output layers, the biases and the standardisation.  Built in
+// global memory once per parameter change (k_mlp_image); inference blocks
+// copy it into LDS with 16-byte loads.
+struct alignas(16) MlpNet {
     bf16x8 fw[2][4][2][64];
-    bf16x8 bw[2][4][2][64];
     float w1[4 * 64], b1[64], b2[64], b3[64], w4[64 * 2], b4[2];
     float xm[4], xs[4], ym[2], ys[2];
+};
+static_assert(sizeof(MlpNet) % 16 == 0, "the image is copied in 16-byte pieces");
+// training also holds the backward operands, A[k = 16m + l%16][n] = W[k][n]
+struct MlpLds : MlpNet {
+    bf16x8 bw[2][4][2][64];
 };
 
 __device__ __forceinline__ int perm_feature(int q, int g, int e) { return 16 * (2 * q + (e >> 2)) + 4 * g + (e & 3); }
 
-__device__ void mlp_load_lds(MlpLds& s, const float* __restrict__ p, const float* __restrict__ norm)
+// the network from its FP32 parameters p and standardisation norm, by the
+// threads of one block (the backward operands when bw is given)
+__device__ void mlp_fill(MlpNet& s, bf16x8 (*bw)[4][2][64], const float* __restrict__ p,
+                         const float* __restrict__ norm)
 {
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int idx = tid; idx < 2 * 4 * 2 * 64; idx += nt) {
@@ -68,7 +95,7 @@ __device__ void mlp_load_lds(MlpLds& s, const float* __restrict__ p, const float
             b[e] = (__bf16)W[row * 64 + k];   // W[row][k]
         }
         s.fw[L][m][q][lane] = f;
-        s.bw[L][m][q][lane] = b;
+        if (bw) bw[L][m][q][lane] = b;
     }
     for (int idx = tid; idx < 256; idx += nt) s.w1[idx] = bfr(p[MLP_W1 + idx]);
     for (int idx = tid; idx < 128; idx += nt) s.w4[idx] = bfr(p[MLP_W4 + idx]);
@@ -80,6 +107,26 @@ __device__ void mlp_load_lds(MlpLds& s, const float* __restrict__ p, const float
     if (tid < 2) s.b4[tid] = p[MLP_B4 + tid];
     if (tid < 4) { s.xm[tid] = norm[tid]; s.xs[tid] = norm[4 + tid]; }
     if (tid < 2) { s.ym[tid] = norm[8 + tid]; s.ys[tid] = norm[10 + tid]; }
+}
+
+__device__ void mlp_load_lds(MlpLds& s, const float* __restrict__ p, const float* __restrict__ norm)
+{
+    mlp_fill(s, s.bw, p, norm);
+}
+
+// one block: the inference image of the parameters
+__global__ void __launch_bounds__(256) k_mlp_image(const float* __restrict__ p, const float* __restrict__ norm,
+                                                   MlpNet* __restrict__ img)
+{
+    mlp_fill(*img, nullptr, p, norm);
+}
+
+// the image into a block's LDS
+__device__ void mlp_copy_image(MlpNet& s, const MlpNet* __restrict__ img)
+{
+    const uint4* __restrict__ src = reinterpret_cast<const uint4*>(img);
+    uint4* dst = reinterpret_cast<uint4*>(&s);
+    for (int q = threadIdx.x; q < (int)(sizeof(MlpNet) / 16); q += blockDim.x) dst[q] = src[q];
 }
 
 // Compiler-only fence at the top of a grid-stride loop over sample groups:
@@ -108,7 +155,8 @@ __device__ __forceinline__ void mfma_layer(const bf16x8 (*A)[2][64], const Act& 
 // Forward pass of one 16-sample group.  x: standardised features of this
 // lane's sample.  z1..z3: pre-activations; h3q: last hidden layer after BF16
 // rounding; y: standardised outputs (every lane of a sample holds them).
-__device__ __forceinline__ void mlp_forward(const MlpLds& s, const float x[4], Act& z1, Act& z2, Act& z3, Act& h3q,
+template <bool FAST = false>
+__device__ __forceinline__ void mlp_forward(const MlpNet& s, const float x[4], Act& z1, Act& z2, Act& z3, Act& h3q,
                                             float y[2])
 {
     const int lane = threadIdx.x % 64, g = lane / 16;
@@ -119,20 +167,20 @@ __device__ __forceinline__ void mlp_forward(const MlpLds& s, const float x[4], A
             float z = s.b1[n];
             for (int k = 0; k < 4; k++) z += x[k] * s.w1[k * 64 + n];
             z1.a[t * 4 + r] = z;
-            h.a[t * 4 + r] = silu(z);
+            h.a[t * 4 + r] = silu_sel<FAST>(z);
         }
     mfma_layer(s.fw[0], h, z2, lane);
     for (int q = 0; q < 16; q++) {
         const int n = 16 * (q / 4) + 4 * g + (q % 4);
         z2.a[q] += s.b2[n];
-        h.a[q] = silu(z2.a[q]);
+        h.a[q] = silu_sel<FAST>(z2.a[q]);
     }
     mfma_layer(s.fw[1], h, z3, lane);
     float y0 = 0.f, y1 = 0.f;
     for (int q = 0; q < 16; q++) {
         const int n = 16 * (q / 4) + 4 * g + (q % 4);
         z3.a[q] += s.b3[n];
-        h3q.a[q] = bfr(silu(z3.a[q]));
+        h3q.a[q] = bfr(silu_sel<FAST>(z3.a[q]));
         y0 += h3q.a[q] * s.w4[n * 2 + 0];
         y1 += h3q.a[q] * s.w4[n * 2 + 1];
     }
@@ -147,11 +195,11 @@ __device__ __forceinline__ void mlp_forward(const MlpLds& s, const float x[4], A
 
 // --------------------------------------------------------------- inference
 // raw features in, raw outputs out (standardisation inside)
-__global__ void __launch_bounds__(256) k_mlp_infer(const float* __restrict__ p, const float* __restrict__ norm,
-                                                   const float* __restrict__ X, float* __restrict__ Y, int n)
+__global__ void __launch_bounds__(256) k_mlp_infer(const MlpNet* __restrict__ img, const float* __restrict__ X,
+                                                   float* __restrict__ Y, int n)
 {
-    __shared__ MlpLds s;
-    mlp_load_lds(s, p, norm);
+    __shared__ MlpNet s;
+    mlp_copy_image(s, img);
     __syncthreads();
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const int ngroups = (n + 15) / 16;
@@ -244,8 +292,12 @@ __global__ void __launch_bounds__(256) k_mlp_fwd_bwd(const float* __restrict__ p
 
 // Weight gradients.  Blocks 0..7: one wave per 16x16 tile of dW2 / dW3 =
 // sum_s H_{L-1}[k][s] dZ_L[n][s] on the matrix cores (operands rounded to
-// BF16, K = 32 samples per instruction).  Block 8: the small parameters
-// (W1, b1..b3, W4, b4) by plain sums.  Every sum runs in a fixed order.
+// BF16, K = 32 samples per instruction).  Blocks 8..: one wave per small
+// parameter (W1, b1..b3, W4, b4; 578 of them), its lanes striding over the
+// samples (coalesced rows of the feature-major buffers), then a butterfly
+// over the lanes.  Every sum runs in a fixed order.
+constexpr int MLP_SMALL = 578;
+constexpr int MLP_WGRAD_BLOCKS = 8 + (MLP_SMALL + 3) / 4;
 __global__ void __launch_bounds__(256) k_mlp_wgrad(TrainBufs tb, int B, float* __restrict__ grad)
 {
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64, g = lane / 16;
@@ -270,26 +322,36 @@ __global__ void __launch_bounds__(256) k_mlp_wgrad(TrainBufs tb, int B, float* _
         return;
     }
     // small parameters: 256 (W1) + 64 x 3 (biases) + 128 (W4) + 2 (b4)
-    for (int q = threadIdx.x; q < 578; q += blockDim.x) {
-        float sum = 0.f;
-        if (q < 256) {
-            const int k = q / 64, n = q % 64;
-            for (int s = 0; s < B; s++) sum += tb.Xt[(long)k * B + s] * tb.DZ[0][(long)n * B + s];
-            grad[MLP_W1 + q] = sum;
-        } else if (q < 448) {
-            const int L = (q - 256) / 64, n = (q - 256) % 64;
-            for (int s = 0; s < B; s++) sum += tb.DZ[L][(long)n * B + s];
-            grad[(L == 0 ? MLP_B1 : (L == 1 ? MLP_B2 : MLP_B3)) + n] = sum;
-        } else if (q < 576) {
-            const int n = (q - 448) / 2, m = (q - 448) % 2;
-            for (int s = 0; s < B; s++) sum += tb.H[2][(long)n * B + s] * tb.DY[(long)m * B + s];
-            grad[MLP_W4 + q - 448] = sum;
-        } else {
-            const int m = q - 576;
-            for (int s = 0; s < B; s++) sum += tb.DY[(long)m * B + s];
-            grad[MLP_B4 + m] = sum;
-        }
+    const int q = (blockIdx.x - 8) * 4 + wave;
+    if (q >= MLP_SMALL) return;
+    const float *A = nullptr, *Bv = nullptr;  // the sum of A[s] * Bv[s] (Bv null: of A[s])
+    float* out;
+    if (q < 256) {
+        const int k = q / 64, n = q % 64;
+        A = tb.DZ[0] + (long)n * B;
+        Bv = tb.Xt + (long)k * B;
+        out = grad + MLP_W1 + q;
+    } else if (q < 448) {
+        const int L = (q - 256) / 64, n = (q - 256) % 64;
+        A = tb.DZ[L] + (long)n * B;
+        out = grad + (L == 0 ? MLP_B1 : (L == 1 ? MLP_B2 : MLP_B3)) + n;
+    } else if (q < 576) {
+        const int n = (q - 448) / 2, m = (q - 448) % 2;
+        A = tb.H[2] + (long)n * B;
+        Bv = tb.DY + (long)m * B;
+        out = grad + MLP_W4 + q - 448;
+    } else {
+        A = tb.DY + (long)(q - 576) * B;
+        out = grad + MLP_B4 + q - 576;
     }
+    float sum = 0.f;
+    if (Bv) {
+        for (int s = lane; s < B; s += 64) sum += A[s] * Bv[s];
+    } else {
+        for (int s = lane; s < B; s += 64) sum += A[s];
+    }
+    for (int o = 32; o >= 1; o >>= 1) sum += __shfl_xor(sum, o);
+    if (lane == 0) *out = sum;
 }
 
 // Adam (Kingma & Ba 2015) on the FP32 master parameters; c1 = 1/(1-b1^t),
@@ -363,16 +425,18 @@ __global__ void k_wm_samples(const T* U, const T* W, const double* alpha, const 
 // The MLP wall model of one substep: per wall cell, the network's tau_wx
 // unless its stress vector (tau_par, tau_perp) differs from the equilibrium
 // log law's (u_tau^2, 0) by more than max_err relative to u_tau^2; then the
-// equilibrium value.  Writes
-// alpha_x and, per 16-cell group, the sum of relative errors and the number
-// of fallbacks.
+// equilibrium value.  Writes alpha_x and, per 16-cell group, the sum of
+// relative errors and the number of fallbacks.
+// A wave takes 64 cells at a time, one per lane: each lane samples the flow
+// and forms the features of its own cell, the network runs on the four
+// 16-cell groups in turn (an MFMA pass takes 16 samples, the features of
+// group q broadcast from lanes 16q..16q+15), and each lane keeps the outputs
+// of its own cell for the FP64 closure that follows.
 template <typename T, bool G>
-__global__ void __launch_bounds__(256) k_wm_mlp(WmArgs<T> one, const WmArgs<T>* tab, Dims d)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_wm_mlp(WmArgs<T> one, const WmArgs<T>* tab, Dims d)
 {
     const GBlk gb = gblk(1);
     const WmArgs<T> arg = member_args<G>(one, tab, gb.m);
-    const float* __restrict__ p = arg.p;
-    const float* __restrict__ norm = arg.norm;
     const T *U = arg.U, *W = arg.W;
     double* alpha = arg.alpha;
     const Scalars* sc = arg.sc;
@@ -380,37 +444,46 @@ __global__ void __launch_bounds__(256) k_wm_mlp(WmArgs<T> one, const WmArgs<T>* 
     const double nu = arg.nu, h_wm = arg.h_wm, kappa = arg.kappa, B = arg.B, yc = arg.yc, max_err = arg.max_err;
     double* stats = arg.stats;
     const int x_mass_cte = arg.x_mass_cte;
-    __shared__ MlpLds s;
-    mlp_load_lds(s, p, norm);
+    __shared__ MlpNet s;
+    mlp_copy_image(s, static_cast<const MlpNet*>(arg.net));
     __syncthreads();
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const long ncell = 2L * d.nx * d.nzg;
     const long ngroups = (ncell + 15) / 16;
+    const long nquad = (ncell + 63) / 64;
     const double dpdx = wm_driving_dpdx(sc, x_mass_cte);
-    for (long grp = (long)gb.x * 4 + wave; grp < ngroups; grp += (long)gb.gx * 4) {
+    for (long qd = (long)gb.x * 4 + wave; qd < nquad; qd += (long)gb.gx * 4) {
         lds_reload_fence();
-        const long c = grp * 16 + lane % 16;
+        const long c = qd * 64 + lane;
         const bool ok = c < ncell;
-        float x[4] = {0.f, 0.f, 0.f, 0.f};
+        float xo[4] = {0.f, 0.f, 0.f, 0.f};
         WallSample ws{0.0, 0.0, 0.0, 1.0};
         if (ok) {
             const int i = (int)(c % d.nx) + 1, wall = (int)((c / d.nx) % 2), k = (int)(c / (2L * d.nx)) + 1;
             ws = wall_sample(U, W, m, d, i, k, wall, h_wm);
             float f[4];
             wm_features(ws, nu, alpha[c], dpdx, f);
-            for (int q = 0; q < 4; q++) x[q] = (f[q] - s.xm[q]) / s.xs[q];
+            for (int q = 0; q < 4; q++) xo[q] = (f[q] - s.xm[q]) / s.xs[q];
         }
-        Act z1, z2, z3, h3q;
-        float y[2];
-        mlp_forward(s, x, z1, z2, z3, h3q, y);
+        float ys[2] = {0.f, 0.f};
+#pragma unroll 1
+        for (int q = 0; q < 4; q++) {
+            lds_reload_fence();
+            float x[4];
+            for (int e = 0; e < 4; e++) x[e] = __shfl(xo[e], 16 * q + lane % 16);
+            Act z1, z2, z3, h3q;
+            float y[2];
+            mlp_forward<true>(s, x, z1, z2, z3, h3q, y);
+            if (lane / 16 == q) { ys[0] = y[0]; ys[1] = y[1]; }
+        }
         double err = 0.0, fb = 0.0;
-        if (ok && lane < 16) {
+        if (ok) {
             const double up = sqrt(ws.u * ws.u + ws.w * ws.w);
             const double cs = up > 0.0 ? ws.u / up : 1.0, sn = up > 0.0 ? ws.w / up : 0.0;
             const double sc2 = nu * nu / (ws.h * ws.h);
-            const double o0 = (double)(y[0] * s.ys[0] + s.ym[0]);
+            const double o0 = (double)(ys[0] * s.ys[0] + s.ym[0]);
             const double tpar = sc2 * expm1(o0);
-            const double tper = sc2 * exp(o0) * (double)(y[1] * s.ys[1] + s.ym[1]);
+            const double tper = sc2 * exp(o0) * (double)(ys[1] * s.ys[1] + s.ym[1]);
             const double tx = tpar * cs - tper * sn;
             // the network's own u_tau starts the equilibrium Newton solve
             const double ut = loglaw_utau(up, ws.h, nu, kappa, B, yc, tpar > 0.0 ? sqrt(tpar) : 0.0);
@@ -422,11 +495,13 @@ __global__ void __launch_bounds__(256) k_wm_mlp(WmArgs<T> one, const WmArgs<T>* 
             fb = use_eq ? 1.0 : 0.0;
             alpha[c] = use_eq ? tx_eq : tx;
         }
+        // sums over each 16-cell group, in a fixed order
         for (int o = 8; o >= 1; o >>= 1) {
             err += __shfl_xor(err, o);
             fb += __shfl_xor(fb, o);
         }
-        if (lane == 0) {
+        const long grp = qd * 4 + lane / 16;
+        if (lane % 16 == 0 && grp < ngroups) {
             stats[grp] = err;
             stats[ngroups + grp] = fb;
         }
@@ -434,12 +509,12 @@ __global__ void __launch_bounds__(256) k_wm_mlp(WmArgs<T> one, const WmArgs<T>* 
 }
 
 // ------------------------------------------------------------- launchers
-void launch_mlp_infer(const float* p, const float* norm, const float* X, float* Y, int n, hipStream_t s)
+void launch_mlp_infer(const void* img, const float* X, float* Y, int n, hipStream_t s)
 {
     const int groups = (n + 15) / 16;
     const int blocks = std::max(1, std::min(1024, (groups + 3) / 4));
     PROF_BEGIN("k_mlp_infer", s);
-    k_mlp_infer<<<blocks, 256, 0, s>>>(p, norm, X, Y, n);
+    k_mlp_infer<<<blocks, 256, 0, s>>>(static_cast<const MlpNet*>(img), X, Y, n);
     PROF_END(s);
 }
 
@@ -455,11 +530,14 @@ void launch_wm_samples(const T* U, const T* W, const double* alpha, const Scalar
 
 long wm_mlp_groups(const Dims& d) { return (2L * d.nx * d.nzg + 15) / 16; }
 
+// blocks of 4 waves, one 64-cell set per wave: a set's forward passes take
+// several microseconds of dependent instructions, so waves rather than
+// loop trips carry the cells (the 18.8 KB image copy per block comes from L2)
 template <typename T>
 void run_wm_mlp(const WmArgs<T>& one, const WmArgs<T>* tab, int nmem, const Dims& d, hipStream_t s)
 {
-    const long groups = wm_mlp_groups(d);
-    const int blocks = (int)std::max(1L, std::min(2048L, (groups + 3) / 4));
+    const long quads = (2L * d.nx * d.nzg + 63) / 64;
+    const int blocks = (int)std::max(1L, std::min(2048L, (quads + 3) / 4));
     PROF_BEGIN("k_wm_mlp", s);
     if (tab) k_wm_mlp<T, true><<<dim3(blocks, 1, nmem), 256, 0, s>>>(one, tab, d);
     else k_wm_mlp<T, false><<<dim3(blocks, 1, 1), 256, 0, s>>>(one, nullptr, d);
@@ -467,11 +545,11 @@ void run_wm_mlp(const WmArgs<T>& one, const WmArgs<T>* tab, int nmem, const Dims
 }
 
 template <typename T>
-void launch_wm_mlp(const float* p, const float* norm, const T* U, const T* W, double* alpha, const Scalars* sc,
+void launch_wm_mlp(const void* net, const T* U, const T* W, double* alpha, const Scalars* sc,
                    const Metric<double>& m, const Dims& d, double nu, double h_wm, double kappa, double B,
                    int x_mass_cte, double max_err, double* stats, hipStream_t s)
 {
-    run_wm_mlp<T>(WmArgs<T>{p, norm, U, W, alpha, sc, m, nu, h_wm, kappa, B, loglaw_crossover(kappa, B), max_err, stats,
+    run_wm_mlp<T>(WmArgs<T>{net, U, W, alpha, sc, m, nu, h_wm, kappa, B, loglaw_crossover(kappa, B), max_err, stats,
                             x_mass_cte},
                   nullptr, 1, d, s);
 }
@@ -484,10 +562,10 @@ template void launch_wm_samples<float>(const float*, const float*, const double*
                                        float*, double*, hipStream_t);
 template void run_wm_mlp<double>(const WmArgs<double>&, const WmArgs<double>*, int, const Dims&, hipStream_t);
 template void run_wm_mlp<float>(const WmArgs<float>&, const WmArgs<float>*, int, const Dims&, hipStream_t);
-template void launch_wm_mlp<double>(const float*, const float*, const double*, const double*, double*,
-                                    const Scalars*, const Metric<double>&, const Dims&, double, double, double,
-                                    double, int, double, double*, hipStream_t);
-template void launch_wm_mlp<float>(const float*, const float*, const float*, const float*, double*, const Scalars*,
+template void launch_wm_mlp<double>(const void*, const double*, const double*, double*, const Scalars*,
+                                    const Metric<double>&, const Dims&, double, double, double, double, int, double,
+                                    double*, hipStream_t);
+template void launch_wm_mlp<float>(const void*, const float*, const float*, double*, const Scalars*,
                                    const Metric<double>&, const Dims&, double, double, double, double, int, double,
                                    double*, hipStream_t);
 
@@ -496,11 +574,25 @@ template void launch_wm_mlp<float>(const float*, const float*, const float*, con
 // ------------------------------------------------------------------ C ABI
 using namespace chan;
 
+// A network and the streams that read it.  Channels that use it for their
+// wall model attach their stream.  Every parameter change is first ordered
+// after the work already queued on those streams (an event recorded there,
+// waited on by the network's stream), and ends with the rebuilt inference
+// image and the `ready` event, which channels wait on before their next
+// launch: a change never overlaps a step that reads the network.
 struct chan_mlp {
     std::string err;
     int device = 0;
     hipStream_t stream = nullptr;
     float *p = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr, *norm = nullptr;
+    MlpNet* img = nullptr;
+    hipEvent_t ready = nullptr;
+    std::mutex mu;
+    struct User {
+        hipStream_t s;
+        hipEvent_t ev;
+    };
+    std::vector<User> users;
     int adam_t = 0;
     // training buffers (grown on demand)
     int cap_B = 0;
@@ -512,13 +604,36 @@ struct chan_mlp {
 };
 
 namespace chan {
-bool mlp_device_params(const chan_mlp* net, const float** p, const float** norm, int* device)
+bool mlp_device_net(const chan_mlp* net, const void** img, int* device)
 {
-    if (!net || !net->p) return false;
-    *p = net->p;
-    *norm = net->norm;
+    if (!net || !net->img) return false;
+    *img = net->img;
     *device = net->device;
     return true;
+}
+
+hipEvent_t mlp_ready(const chan_mlp* net) { return net ? net->ready : nullptr; }
+
+hipError_t mlp_attach(chan_mlp* net, hipStream_t s)
+{
+    std::lock_guard<std::mutex> lk(net->mu);
+    for (const auto& u : net->users)
+        if (u.s == s) return hipSuccess;
+    hipEvent_t ev = nullptr;
+    hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e == hipSuccess) net->users.push_back({s, ev});
+    return e;
+}
+
+void mlp_detach(chan_mlp* net, hipStream_t s)
+{
+    std::lock_guard<std::mutex> lk(net->mu);
+    for (size_t q = 0; q < net->users.size(); q++)
+        if (net->users[q].s == s) {
+            (void)hipEventDestroy(net->users[q].ev);
+            net->users.erase(net->users.begin() + q);
+            return;
+        }
 }
 }  // namespace chan
 
@@ -581,13 +696,33 @@ int mset_device(chan_mlp* c)
     return CHAN_OK;
 }
 
+// order the network's stream after the work queued on its users' streams
+int begin_update(chan_mlp* c)
+{
+    std::lock_guard<std::mutex> lk(c->mu);
+    for (const auto& u : c->users) {
+        MCHK(c, hipEventRecord(u.ev, u.s));
+        MCHK(c, hipStreamWaitEvent(c->stream, u.ev, 0));
+    }
+    return CHAN_OK;
+}
+
+// the inference image of the new parameters, then the ready event
+int end_update(chan_mlp* c)
+{
+    k_mlp_image<<<1, 256, 0, c->stream>>>(c->p, c->norm, c->img);
+    MCHK(c, hipGetLastError());
+    MCHK(c, hipEventRecord(c->ready, c->stream));
+    return CHAN_OK;
+}
+
 // one optimiser step on the batch idx[0..nvalid) (already on the device)
 int train_batch(chan_mlp* c, int nvalid, int B, float lr)
 {
     const int groups = B / 16;
     const int blocks = std::max(1, (groups + 3) / 4);
     k_mlp_fwd_bwd<<<blocks, 256, 0, c->stream>>>(c->p, c->norm, c->X, c->Y, c->idx, nvalid, B, c->tb);
-    k_mlp_wgrad<<<9, 256, 0, c->stream>>>(c->tb, B, c->grad);
+    k_mlp_wgrad<<<MLP_WGRAD_BLOCKS, 256, 0, c->stream>>>(c->tb, B, c->grad);
     c->adam_t++;
     const double b1 = 0.9, b2 = 0.999;
     const float c1 = (float)(1.0 / (1.0 - std::pow(b1, c->adam_t)));
@@ -617,6 +752,8 @@ int chan_mlp_create(int device, unsigned long long seed, chan_mlp** out)
     MCHK(c, hipMalloc(&c->v, MLP_NPARAM * sizeof(float)));
     MCHK(c, hipMalloc(&c->grad, MLP_NPARAM * sizeof(float)));
     MCHK(c, hipMalloc(&c->norm, MLP_NNORM * sizeof(float)));
+    MCHK(c, hipMalloc(&c->img, sizeof(MlpNet)));
+    MCHK(c, hipEventCreateWithFlags(&c->ready, hipEventDisableTiming));
     // He-uniform weights (bound sqrt(6 / fan_in)), zero biases, identity
     // standardisation; seeded counter RNG
     std::vector<float> p(MLP_NPARAM, 0.0f);
@@ -635,20 +772,32 @@ int chan_mlp_create(int device, unsigned long long seed, chan_mlp** out)
     MCHK(c, hipMemset(c->v, 0, MLP_NPARAM * sizeof(float)));
     MCHK(c, hipMemset(c->grad, 0, MLP_NPARAM * sizeof(float)));
     MCHK(c, hipMemcpy(c->norm, norm, sizeof(norm), hipMemcpyHostToDevice));
+    int rc2 = end_update(c);
+    if (rc2) return rc2;
+    MCHK(c, hipStreamSynchronize(c->stream));
     return CHAN_OK;
 }
 
-void chan_mlp_destroy(chan_mlp* c)
+int chan_mlp_destroy(chan_mlp* c)
 {
-    if (!c) return;
+    if (!c) return CHAN_OK;
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (!c->users.empty())
+            return mfail(c, CHAN_ESTATE, "the network is the wall model of " + std::to_string(c->users.size()) +
+                                             " channel(s); detach it first (chan_set_wall_network(c, NULL, ...))");
+    }
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     float* bufs[] = {c->p, c->m, c->v, c->grad, c->norm, c->tbuf, c->X, c->Y};
     for (float* b : bufs)
         if (b) (void)hipFree(b);
+    if (c->img) (void)hipFree(c->img);
     if (c->idx) (void)hipFree(c->idx);
+    if (c->ready) (void)hipEventDestroy(c->ready);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+    return CHAN_OK;
 }
 
 const char* chan_mlp_last_error(const chan_mlp* c) { return c ? c->err.c_str() : "null network"; }
@@ -667,22 +816,25 @@ int chan_mlp_get_params(chan_mlp* c, float* params, float* norm)
 int chan_mlp_set_params(chan_mlp* c, const float* params, const float* norm)
 {
     if (!c) return CHAN_EINVAL;
-    int rc = mset_device(c);
-    if (rc) return rc;
-    MCHK(c, hipStreamSynchronize(c->stream));
     if (norm) {
         for (int q = 4; q < 8; q++)
             if (!(norm[q] > 0.0f)) return mfail(c, CHAN_EINVAL, "feature scales must be positive");
         for (int q = 10; q < 12; q++)
             if (!(norm[q] > 0.0f)) return mfail(c, CHAN_EINVAL, "output scales must be positive");
-        MCHK(c, hipMemcpy(c->norm, norm, MLP_NNORM * sizeof(float), hipMemcpyHostToDevice));
     }
+    int rc = mset_device(c);
+    if (rc) return rc;
+    if ((rc = begin_update(c))) return rc;
+    if (norm) MCHK(c, hipMemcpyAsync(c->norm, norm, MLP_NNORM * sizeof(float), hipMemcpyHostToDevice, c->stream));
     if (params) {
-        MCHK(c, hipMemcpy(c->p, params, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
-        MCHK(c, hipMemset(c->m, 0, MLP_NPARAM * sizeof(float)));
-        MCHK(c, hipMemset(c->v, 0, MLP_NPARAM * sizeof(float)));
+        MCHK(c, hipMemcpyAsync(c->p, params, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        MCHK(c, hipMemsetAsync(c->m, 0, MLP_NPARAM * sizeof(float), c->stream));
+        MCHK(c, hipMemsetAsync(c->v, 0, MLP_NPARAM * sizeof(float), c->stream));
         c->adam_t = 0;
     }
+    if ((rc = end_update(c))) return rc;
+    // the host arrays may be released on return
+    MCHK(c, hipStreamSynchronize(c->stream));
     return CHAN_OK;
 }
 
@@ -694,7 +846,7 @@ int chan_mlp_forward(chan_mlp* c, const float* X, int n, float* Y)
     if (rc) return rc;
     if ((rc = ensure_train(c, 32, n))) return rc;
     MCHK(c, hipMemcpyAsync(c->X, X, (size_t)n * 4 * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    launch_mlp_infer(c->p, c->norm, c->X, c->Y, n, c->stream);
+    launch_mlp_infer(c->img, c->X, c->Y, n, c->stream);
     MCHK(c, hipGetLastError());
     MCHK(c, hipMemcpyAsync(Y, c->Y, (size_t)n * 2 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     MCHK(c, hipStreamSynchronize(c->stream));
@@ -716,7 +868,7 @@ int chan_mlp_gradients(chan_mlp* c, const float* X, const float* Y, int n, float
     const int groups = B / 16;
     k_mlp_fwd_bwd<<<std::max(1, (groups + 3) / 4), 256, 0, c->stream>>>(c->p, c->norm, c->X, c->Y, c->idx, n, B,
                                                                         c->tb);
-    k_mlp_wgrad<<<9, 256, 0, c->stream>>>(c->tb, B, c->grad);
+    k_mlp_wgrad<<<MLP_WGRAD_BLOCKS, 256, 0, c->stream>>>(c->tb, B, c->grad);
     MCHK(c, hipGetLastError());
     std::vector<float> lp(groups);
     MCHK(c, hipMemcpyAsync(grad, c->grad, MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost, c->stream));
@@ -762,6 +914,7 @@ int chan_mlp_train(chan_mlp* c, const float* X, const float* Y, int n, int epoch
     int rc = mset_device(c);
     if (rc) return rc;
     if ((rc = ensure_train(c, batch, n))) return rc;
+    if ((rc = begin_update(c))) return rc;
     MCHK(c, hipMemcpyAsync(c->X, X, (size_t)n * 4 * sizeof(float), hipMemcpyHostToDevice, c->stream));
     MCHK(c, hipMemcpyAsync(c->Y, Y, (size_t)n * 2 * sizeof(float), hipMemcpyHostToDevice, c->stream));
     std::vector<int> perm(n);
@@ -813,10 +966,12 @@ int chan_mlp_train(chan_mlp* c, const float* X, const float* Y, int n, int epoch
         }
     }
     c->tb.loss = keep_loss;
+    // the image of whatever the parameters now are, also after a failure
+    const int rc_img = end_update(c);
     (void)hipStreamSynchronize(c->stream);
     (void)hipFree(dperm);
     (void)hipFree(dloss);
-    return rc;
+    return rc ? rc : rc_img;
 }
 
 }  // extern "C"

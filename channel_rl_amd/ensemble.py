@@ -71,3 +71,43 @@ def wall_error(channels):
     equilibrium stress and of the fallback fraction (last evaluation)."""
     st = np.array([ch.wall_model_stats() for ch in channels])
     return float(st[:, 0].mean()), float(st[:, 1].mean())
+
+
+def main(argv=None):
+    """Run and time the grouped ensemble of the 64-channel draw (or of its
+    members with a given number of x faces), FP32 bulk with the learned wall
+    model (trained on 2x10^5 samples, 4 epochs), for profiling:
+
+        python -m channel_rl_amd.ensemble --faces 98 --steps 30
+    """
+    import argparse
+    import sys
+
+    from . import wallnet
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--faces", type=int, default=0, help="only the members with this many x faces (0: all)")
+    ap.add_argument("--steps", type=int, default=30)
+    a = ap.parse_args(argv)
+    X, Y = wallnet.solver_samples(200_000, seed=7)
+    net, _ = wallnet.train(X, Y, epochs=4)
+    mem = [(c, f) for c, f in cases.ensemble(a.members) if a.faces in (0, c.nx)]
+    if not mem:
+        print(f"no member with {a.faces} x faces", file=sys.stderr)
+        return 2
+    chans = [make_channel(c, f, precision=FP32_BULK, wall=WALL_MLP, net=net) for c, f in mem]
+    tune_members(chans)
+    ens = Ensemble(chans)
+    ens.run_steps(10)
+    ens.synchronize()
+    dt = time_batched(ens, a.steps)
+    print(f"{len(mem)} members in {ens.group_count()} groups: {1e3 * dt / a.steps:.3f} ms per step, "
+          f"{sum(c.cells for c, _ in mem) * a.steps / dt / 1e6:.1f} Mcells/s", flush=True)
+    ens.close()
+    for c in chans:
+        c.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -18,6 +18,13 @@ are what `load_traffic` reads back.
     python -m channel_rl_amd.rocprof_summary sq PASS1/run_counter_collection.csv \
         PASS2/run_counter_collection.csv
 
+    python -m channel_rl_amd.rocprof_summary stages DIR/run_kernel_stats.csv \
+        FETCH_DIR/run_counter_collection.csv WRITE_DIR/run_counter_collection.csv
+
+`stages` joins the two: per kernel, its average duration from the kernel
+trace, its scaled HBM bytes per launch, the rate those bytes move at and
+that rate as a fraction of the 8 TB/s HBM peak.
+
 `sq` summarises the SQ counter passes listed in SQ_PASSES (medians per
 dispatch; each counter is a whole-device sum per dispatch).  Per wave:
 VALU and LDS instructions; as fractions of the waves' lifetime
@@ -104,6 +111,23 @@ def traffic_table(fetch_path, write_path):
     return "\n".join(out)
 
 
+HBM_PEAK_GBS = 8000.0
+
+
+def stages_table(stats_path, fetch_path, write_path):
+    res, sf, sw = traffic(fetch_path, write_path)
+    out = [f"copy-kernel scale factors: FETCH_SIZE x {sf:.3f}, WRITE_SIZE x {sw:.3f}", "",
+           "| kernel | calls | average us | % of time | HBM bytes/launch | GB/s | of HBM peak |",
+           "|---|---:|---:|---:|---:|---:|---:|"]
+    for n, c, tot, avg, pct in read_stats(stats_path):
+        if n not in res or base_name(n) == COPY_KERNEL:
+            continue
+        b = res[n][3]
+        gbs = b / avg  # bytes per ns = GB/s
+        out.append(f"| {n} | {c} | {avg / 1e3:.2f} | {pct:.2f} | {b:.0f} | {gbs:.0f} | {gbs / HBM_PEAK_GBS:.3f} |")
+    return "\n".join(out)
+
+
 # the two counter passes (at most 8 SQ and 2 GRBM counters each)
 SQ_PASSES = (
     "SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_ACTIVE_INST_VALU SQ_ACTIVE_INST_LDS SQ_WAIT_INST_LDS "
@@ -146,17 +170,22 @@ def sq_table(paths):
 
 
 def load_traffic(path):
-    """Scaled HBM bytes per launch by kernel base name from a traffic table."""
-    res = {}
+    """Scaled HBM bytes per launch by kernel base name from a traffic table;
+    of several instances of a kernel (tile shapes timed by the autotuner),
+    the one launched most often: the one the step runs."""
+    res, most = {}, {}
     if not os.path.exists(path):
         return res
     for line in open(path):
         p = [s.strip() for s in line.strip().strip("|").split("|")]
         if len(p) == 5 and p[0].startswith("k_"):
             try:
-                res[base_name(p[0])] = float(p[4])
+                n, b = int(p[1]), float(p[4])
             except ValueError:
-                pass
+                continue
+            k = base_name(p[0])
+            if n > most.get(k, -1):
+                most[k], res[k] = n, b
     return res
 
 
@@ -165,6 +194,8 @@ if __name__ == "__main__":
         print(stats_table(sys.argv[2]))
     elif len(sys.argv) >= 4 and sys.argv[1] == "traffic":
         print(traffic_table(sys.argv[2], sys.argv[3]))
+    elif len(sys.argv) >= 5 and sys.argv[1] == "stages":
+        print(stages_table(sys.argv[2], sys.argv[3], sys.argv[4]))
     elif len(sys.argv) >= 3 and sys.argv[1] == "sq":
         print(sq_table(sys.argv[2:]))
     else:

@@ -205,9 +205,16 @@ int chan_stream_copy(chan_ctx* ctx, long nbytes, int reps, double* ms_per_copy);
  * W3[64][64] b3[64] W4[64][2] b4[2] (W[k][n], input k, output n); and 12
  * standardisation constants: feature means and scales (4 + 4), output means
  * and scales (2 + 2).  Features/outputs passed to the calls below are raw
- * (the network standardises them).  X is n x 4, Y is n x 2, row-major. */
-int  chan_mlp_create(int device, unsigned long long seed, chan_mlp** out);
-void chan_mlp_destroy(chan_mlp* net);
+ * (the network standardises them).  X is n x 4, Y is n x 2, row-major.
+ *
+ * Channels that use a network as their wall model (chan_set_wall_network)
+ * are its users: chan_mlp_set_params, chan_mlp_fit_normalization and
+ * chan_mlp_train first wait for the work already queued on the users'
+ * streams, and the users' next steps wait for the change to finish, so a
+ * change never overlaps a step that reads the network.  chan_mlp_destroy
+ * refuses (CHAN_ESTATE, nothing released) while the network has users. */
+int chan_mlp_create(int device, unsigned long long seed, chan_mlp** out);
+int chan_mlp_destroy(chan_mlp* net);
 const char* chan_mlp_last_error(const chan_mlp* net);
 int chan_mlp_get_params(chan_mlp* net, float* params /* 8770 or NULL */, float* norm /* 12 or NULL */);
 int chan_mlp_set_params(chan_mlp* net, const float* params, const float* norm);
@@ -229,8 +236,10 @@ int chan_mlp_train(chan_mlp* net, const float* X, const float* Y, int n, int epo
  * feature uses the context's alpha_x. */
 int chan_wall_samples(chan_ctx* ctx, double h_wm, double nu, double kappa, double B, float* X, float* Y,
                       double* tau);
-/* the network used by CHAN_WALL_MLP (not owned; same device as ctx; do not
- * train it while a step that uses it runs) and the largest relative deviation
+/* the network used by CHAN_WALL_MLP (not owned; same device as ctx; NULL
+ * detaches the current one unless CHAN_WALL_MLP is selected; the channel is
+ * a user of the network until it is detached or destroyed) and the largest
+ * relative deviation
  * |(tau_par, tau_perp) - (u_tau^2, 0)| / u_tau^2 from the equilibrium stress
  * it may have before a cell falls back to the equilibrium value */
 int chan_set_wall_network(chan_ctx* ctx, chan_mlp* net, double max_rel_err);

@@ -91,6 +91,72 @@ def test_wall_samples_match_oracle(gpu):
         assert np.max(np.abs(Y - Yr)) <= 1e-5 * max(1.0, np.abs(Yr).max())
 
 
+def test_network_wall_stress_matches_oracle_per_cell(gpu, wall_network):
+    # every wall cell takes the network's stress (no fallback): the sampled
+    # flow, the features, the forward pass and the rotation back to tau_wx
+    # against the oracle's.  The forward pass agrees to 2e-3 on standardised
+    # outputs (FP32 against FP64 accumulation), which moves tau_par by a few
+    # 1e-3 relative: 1e-2 of the largest stress per cell, 3e-3 on average.
+    net = wall_network[0]
+    p, norm = net.params()
+    case, field = cases.small_case(18, 17, 18, seed=9, x_mass_cte=0)
+    U, V, W = projected_state(case, field)
+    g = Channel(case.x, case.y, case.z, precision=FP64, **case.kwargs())
+    g.set_state(U, V, W)
+    rng = np.random.default_rng(6)
+    alpha = np.asfortranarray(rng.uniform(0.0, 2e-2, (case.nx, 2, case.nzg)))
+    g.set_scalars(dt=1e-3, t=0.0, dPdx=0.37, Qflow_x_0=0.0)
+    g.set_wall_network(net, max_rel_err=1e30)
+    y, yg = _grid1(case)
+    for h in (0.0, 0.2):
+        g.set_field(ALPHA_X, alpha)
+        g.set_wall_model(WALL_MLP, h_wm=h)
+        g.evaluate_wall_stress()
+        assert g.wall_model_stats()[1] == 0.0
+        got = g.get_field(ALPHA_X).ravel(order="F")
+        X, _, F = om.wall_samples(U, W, alpha, y, yg, case.nu, h, 0.41, 5.2, 0.37, oracle.loglaw_utau, flow=True)
+        o = om.forward(p.astype(np.float64), norm.astype(np.float64), X.astype(np.float32).astype(np.float64))
+        u, w, hh = F[:, 0], F[:, 1], F[:, 2]
+        up = np.hypot(u, w)
+        cs = np.where(up > 0, u / np.where(up > 0, up, 1.0), 1.0)
+        sn = np.where(up > 0, w / np.where(up > 0, up, 1.0), 0.0)
+        sc2 = case.nu ** 2 / hh ** 2
+        tpar = sc2 * np.expm1(o[:, 0])
+        tper = sc2 * np.exp(o[:, 0]) * o[:, 1]
+        ref = tpar * cs - tper * sn
+        d = np.abs(got - ref) / np.abs(ref).max()
+        assert d.max() <= 1e-2 and d.mean() <= 3e-3, (h, d.max(), d.mean())
+
+
+def test_network_changes_wait_for_queued_steps(gpu, wall_network):
+    # parameters changed right after queuing steps that read the network:
+    # the change waits for those steps and the steps after it see the new
+    # parameters, as when the host waits in between; a network in use
+    # cannot be released
+    from channel_rl_amd import U as FU, ChanError
+    p, norm = wall_network[0].params()
+    p2 = p.copy()
+    p2[om.B4] += 0.05
+    fields = []
+    for wait in (False, True):
+        net = WallNetwork(seed=0)
+        net.set_params(p, norm)
+        g = _small_channel(5, WALL_EQUILIBRIUM, net, steps=20)
+        g.set_wall_model(WALL_MLP, h_wm=0.0)
+        g.run_steps(10)
+        if wait:
+            g.synchronize()
+        net.set_params(p2, norm)
+        g.run_steps(10)
+        fields.append(g.get_field(FU))
+        with pytest.raises(ChanError):
+            net.close()
+        g.close()
+        net.close()
+    assert np.all(np.isfinite(fields[0]))
+    assert np.array_equal(fields[0], fields[1])
+
+
 @pytest.fixture(scope="module")
 def trained(wall_network):
     net, losses = wall_network
