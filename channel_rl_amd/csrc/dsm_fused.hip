@@ -71,8 +71,8 @@ __device__ __forceinline__ T strain_offdiag(const FA& A, const FB& B, T ra0, T r
     return T(0.125) * ((rb0 * sA0 + rb1 * sA1) + (ra0 * sB0 + ra1 * sB1));
 }
 
-// Two blocks fit a CU's LDS; the register budget is held to the same two
-// waves per SIMD.
+// The register budget is held to two waves per SIMD (the FP64 kernels need
+// it; the FP32 ones fit in fewer registers and run three or four).
 template <typename T, int TXO, int TYO, bool G>
 __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_waves_per_eu(2)))
     k_dsm_fused(DsmArgs<T> one, const DsmArgs<T>* tab, Dims d, int kchunk)
@@ -430,20 +430,24 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
 }
 
 // Tile shapes, (x outputs, y outputs): a row of TXO + 2 columns divides the
-// wave.  FP64 blocks have 512 threads (144 KB of LDS, one block of 8 waves
-// per CU), FP32 blocks 256 (40 KB).  Shape 0 has 30 outputs per row (32
-// columns, half a wave): the least halo per output on long rows; shape 1 has
-// 14 (16 columns) and twice the rows: the same halo per output, and less of
-// the last x tile left empty on short rows (33, 65 or 97 cells: 60, 90, 120
-// columns of 30 against 42, 70, 98 of 14).
+// wave.  Shape 0 has 30 outputs per row (32 columns, half a wave): the least
+// halo per output on long rows; shape 1 has 14 (16 columns) and twice the
+// rows: the same halo per output, and less of the last x tile left empty on
+// short rows (33, 65 or 97 cells: 60, 90, 120 columns of 30 against 42, 70,
+// 98 of 14).  FP64: shapes 0 and 1 are 512-thread blocks (144 KB of LDS, one
+// block of 8 waves per CU, whose two barriers per plane stall the whole CU);
+// shape 2, 14 x 14 outputs in 256 threads (72 KB), puts two blocks of 4
+// waves on a CU, whose barriers overlap, for 7% more halo columns.  FP32:
+// shapes 0 and 1 are 256-thread blocks (40 KB), shape 2 is 30 x 14 outputs
+// in 512 threads (72 KB), the least halo per output.
 struct DsmShape { int txo, tyo; };
 template <typename T>
 constexpr DsmShape dsm_shape(int q)
 {
-    return sizeof(T) == 8 ? (q == 0 ? DsmShape{30, 14} : DsmShape{14, 30})
-                          : (q == 0 ? DsmShape{30, 6} : DsmShape{14, 14});
+    return sizeof(T) == 8 ? (q == 0 ? DsmShape{30, 14} : (q == 1 ? DsmShape{14, 30} : DsmShape{14, 14}))
+                          : (q == 0 ? DsmShape{30, 6} : (q == 1 ? DsmShape{14, 14} : DsmShape{30, 14}));
 }
-int dsm_shape_count() { return 2; }
+int dsm_shape_count() { return 3; }
 
 template <typename T, int TXO, int TYO>
 int dsm_per_cu()
@@ -464,8 +468,9 @@ int dsm_fused_kchunk(const Dims& d, int device, int shape)
 {
     int ncu = 0;
     const DsmShape sh = dsm_shape<T>(shape);
-    const int per_cu = shape == 0 ? dsm_per_cu<T, dsm_shape<T>(0).txo, dsm_shape<T>(0).tyo>()
-                                  : dsm_per_cu<T, dsm_shape<T>(1).txo, dsm_shape<T>(1).tyo>();
+    const int per_cu = shape == 0   ? dsm_per_cu<T, dsm_shape<T>(0).txo, dsm_shape<T>(0).tyo>()
+                       : shape == 1 ? dsm_per_cu<T, dsm_shape<T>(1).txo, dsm_shape<T>(1).tyo>()
+                                    : dsm_per_cu<T, dsm_shape<T>(2).txo, dsm_shape<T>(2).tyo>();
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu < 1) ncu = 1;
     const long tiles = (long)cdiv(d.nxg - 2, sh.txo) * cdiv(d.nyg - 2, sh.tyo);
     return round_filling_chunk(tiles, d.nzg - 2, (long)per_cu * ncu, 8, 64);
@@ -490,7 +495,8 @@ void run_dsm_fused(const DsmArgs<T>& one, const DsmArgs<T>* tab, int nmem, const
         if (tab) k_dsm_fused<T, sh.txo, sh.tyo, true><<<grd, blk, 0, s>>>(one, tab, d, kchunk);                     \
         else k_dsm_fused<T, sh.txo, sh.tyo, false><<<grd, blk, 0, s>>>(one, tab, d, kchunk);                        \
     } while (0)
-    if (shape == 1) DSM_GO(1);
+    if (shape == 2) DSM_GO(2);
+    else if (shape == 1) DSM_GO(1);
     else DSM_GO(0);
 #undef DSM_GO
     PROF_END(s);
