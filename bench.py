@@ -56,6 +56,8 @@ ALG_VALUES = {
     "k_tridiag": (0, 2),
     "k_fftx_inv": (0, 2),
     "k_project": (6, 1),
+    # the AMD eddy viscosity (LES model 3): reads U,V,W, writes nu_t
+    "k_amd": (4, 0),
 }
 
 
@@ -66,6 +68,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="wmles", choices=["small", "wmles", "large"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--les", default="dsm", choices=["dsm", "amd"],
+                    help="subgrid model: dynamic Smagorinsky (the workload's) or AMD (LES model 3)")
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=20,
                     help="at most this many oracle steps for the CPU baseline (0: skip)")
@@ -305,6 +309,8 @@ def main():
 
     prec = cr.FP64 if args.precision == "fp64" else cr.FP32_BULK
     case, field = cases.config(args.config, seed=1234)
+    if args.les == "amd":
+        case.LES_model = 3
     ch = cr.Channel(case.x, case.y, case.z, precision=prec, **case.kwargs())
     # initial field: boundary conditions, projection to div = 0, initial bulk flow
     ch.set_state(*field)
@@ -352,8 +358,25 @@ def main():
     # HBM bytes per launch of the same kernel from the two rocprofv3 counter
     # passes over this workload, scaled by the copy kernel (null if absent)
     from channel_rl_amd import rocprof_summary
-    tag = f"{case.nx - 2}x{case.ny}x{case.nz - 2}_{args.precision}"
-    traffic = rocprof_summary.load_traffic(os.path.join(ROOT, "profiles", f"hbm_traffic_channel_step_{tag}.md")).get(dom)
+    tag = f"{case.nx - 2}x{case.ny}x{case.nz - 2}_{args.precision}" + ("_amd" if args.les == "amd" else "")
+    traffic_all = rocprof_summary.load_traffic(os.path.join(ROOT, "profiles", f"hbm_traffic_channel_step_{tag}.md"))
+    traffic = traffic_all.get(dom)
+    # every stage of the step: HIP-event time per launch, algorithmic bytes
+    # (where counted) and the HBM bytes of the counter passes, each as a
+    # rate and as a fraction of the HBM peak
+    stages = {}
+    for name, (ms, n) in ranked:
+        us = 1e3 * ms / n
+        st = {"us_per_launch": round(us, 2), "launches_per_step": round(n / args.profile_steps, 2)}
+        if name in ALG_VALUES:
+            b0, b1 = ALG_VALUES[name]
+            ab = cells * (b0 * bulk_b + b1 * 8)
+            st["alg_bytes"] = ab
+            st["alg_hbm_frac"] = round(ab / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
+        if name in traffic_all:
+            st["hbm_bytes"] = int(traffic_all[name])
+            st["hbm_frac"] = round(traffic_all[name] / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
+        stages[name] = st
     copy_gbs, copy_ms = ch.stream_copy(1 << 30, reps=10)
     log(f"[bench] stream copy 1 GiB: {copy_gbs:.0f} GB/s (read + write), {copy_ms:.3f} ms")
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -398,12 +421,13 @@ def main():
         "dtype": "f64" if prec == cr.FP64 else "f32-bulk/f64-poisson",
         "data": "synthetic",
         "config": {"workload": f"channel LES RK3 step, {case.nx - 2}x{case.ny}x{case.nz - 2} "
-                               f"(faces {case.nx} {case.ny} {case.nz}), Re_tau {case.re_tau:.0f}, dynamic "
-                               f"Smagorinsky, equilibrium wall model at h_wm = {case.h_wm}, constant mass flow, "
+                               f"(faces {case.nx} {case.ny} {case.nz}), Re_tau {case.re_tau:.0f}, "
+                               f"{'AMD' if args.les == 'amd' else 'dynamic Smagorinsky'}, equilibrium wall model at h_wm = {case.h_wm}, constant mass flow, "
                                f"{args.precision} bulk",
                    "cells": cells, "parallelism": "single GPU", "max_abs_div": maxdiv},
         "tile_shapes": shapes,
         "roofline": roofline,
+        "stages": stages,
         "cpu_baseline": cpu,
         "batched": batched,
         "fp32_bulk": fp32,

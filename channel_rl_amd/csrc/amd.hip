@@ -3,44 +3,56 @@
 //   nu_t = max(-C N / D, 0), C = 0.3,
 //   N = sum_ijk (dh_k u_i)(dh_k u_j) S_ij   (dh: undivided difference)
 //   D = sum_lm (d_l u_m)^2
-// One thread per box cell: the unique interior cells compute the model from
-// U, V, W in L1/L2 (the reads of neighbouring threads overlap), image cells
-// copy their canonical cell's value and the wall ghost rows follow
+// A block owns a TX x TY tile of (x,y) centres and marches in z; U, V, W of
+// planes k-1, k, k+1 over the tile plus a one-cell halo sit in a four-slot
+// LDS ring (the next plane is fetched into registers while one is
+// computed), so each value is read from HBM once per tile and the ~80
+// stencil operands of a centre come from LDS.  Each unique centre writes
+// its periodic images and, on the wall rows, the ghost rows per
 // Dirichlet_nu_t, so no boundary pass follows.  The staggered differences
 // reach the centre through the reference's face-to-centre averages in the
 // order of its interpolate calls (code/interpolation.f90:24-95), and the
-// terms are summed in the order of its statements.
+// terms are summed in the order of its statements; divisions by spacings
+// are multiplications by their reciprocals (an ulp per factor).
+#include <algorithm>
 #include "kernels.hpp"
 #include "prof.hpp"
 #include "wallmodel.hpp"
 
 namespace chan {
 
+// reciprocal spacings at the centre (i, j) and per plane k:
+// rdx = 1/(x(i)-x(i-1)), rdxg0/1 = 1/(xg(i)-xg(i-1)), 1/(xg(i+1)-xg(i)), ...
 template <typename T>
-__device__ __forceinline__ T amd_at(const T* U, const T* V, const T* W, const Metric<T>& m, const Dims& d, int i,
-                                    int j, int k)
+struct AmdXY { T rdx, rdxg0, rdxg1, rdy, rdyg0, rdyg1; };
+template <typename T>
+struct AmdZ { T rdz, rdzg0, rdzg1; };
+
+// the model at centre (i, j, k) from F(f, a, b, c) (0 U, 1 V, 2 W)
+template <typename T, typename A>
+__device__ __forceinline__ T amd_gen(const A& F, const AmdXY<T>& g, const AmdZ<T>& gz, int i, int j, int k)
 {
-    const T *x = m.x, *y = m.y, *z = m.z, *xg = m.xg, *yg = m.yg, *zg = m.zg;
     const T h = T(0.5), qq = T(0.25);
-    auto u = [&](int a, int b, int c) { return U[at(d, a, b, c)]; };
-    auto v = [&](int a, int b, int c) { return V[at(d, a, b, c)]; };
-    auto w = [&](int a, int b, int c) { return W[at(d, a, b, c)]; };
+    auto u = [&](int a, int b, int c) { return F(0, a, b, c); };
+    auto v = [&](int a, int b, int c) { return F(1, a, b, c); };
+    auto w = [&](int a, int b, int c) { return F(2, a, b, c); };
     auto ax = [&](T a, T b) { return h * (a + b); };
+    // reciprocal centre spacings by offset: xg(i)-xg(i-1) at a = i, xg(i+1)-xg(i) at a = i+1
+    auto rxg = [&](int a) { return a == i ? g.rdxg0 : g.rdxg1; };
+    auto ryg = [&](int b) { return b == j ? g.rdyg0 : g.rdyg1; };
+    auto rzg = [&](int c) { return c == k ? gz.rdzg0 : gz.rdzg1; };
     // rate of strain at the centre (code/subgrid.f90:1027-1059)
-    const T S11 = (u(i, j, k) - u(i - 1, j, k)) / (x[i] - x[i - 1]);
-    const T S22 = (v(i, j, k) - v(i, j - 1, k)) / (y[j] - y[j - 1]);
-    const T S33 = (w(i, j, k) - w(i, j, k - 1)) / (z[k] - z[k - 1]);
+    const T S11 = (u(i, j, k) - u(i - 1, j, k)) * g.rdx;
+    const T S22 = (v(i, j, k) - v(i, j - 1, k)) * g.rdy;
+    const T S33 = (w(i, j, k) - w(i, j, k - 1)) * gz.rdz;
     auto e4 = [&](int a, int b) {
-        return h * ((u(a - 1, b, k) - u(a - 1, b - 1, k)) / (yg[b] - yg[b - 1]) +
-                    (v(a, b - 1, k) - v(a - 1, b - 1, k)) / (xg[a] - xg[a - 1]));
+        return h * ((u(a - 1, b, k) - u(a - 1, b - 1, k)) * ryg(b) + (v(a, b - 1, k) - v(a - 1, b - 1, k)) * rxg(a));
     };
     auto e5 = [&](int a, int c) {
-        return h * ((u(a - 1, j, c) - u(a - 1, j, c - 1)) / (zg[c] - zg[c - 1]) +
-                    (w(a, j, c - 1) - w(a - 1, j, c - 1)) / (xg[a] - xg[a - 1]));
+        return h * ((u(a - 1, j, c) - u(a - 1, j, c - 1)) * rzg(c) + (w(a, j, c - 1) - w(a - 1, j, c - 1)) * rxg(a));
     };
     auto e6 = [&](int b, int c) {
-        return h * ((v(i, b - 1, c) - v(i, b - 1, c - 1)) / (zg[c] - zg[c - 1]) +
-                    (w(i, b, c - 1) - w(i, b - 1, c - 1)) / (yg[b] - yg[b - 1]));
+        return h * ((v(i, b - 1, c) - v(i, b - 1, c - 1)) * rzg(c) + (w(i, b, c - 1) - w(i, b - 1, c - 1)) * ryg(b));
     };
     const T S12 = qq * (e4(i, j) + e4(i + 1, j) + e4(i, j + 1) + e4(i + 1, j + 1));
     const T S13 = qq * (e5(i, k) + e5(i + 1, k) + e5(i, k + 1) + e5(i + 1, k + 1));
@@ -64,15 +76,13 @@ __device__ __forceinline__ T amd_at(const T* U, const T* V, const T* W, const Me
     const T Wzh = dWz(i, j, k);
     const T Wxh = ax(ax(dWx(i, j, k - 1), dWx(i, j, k)), ax(dWx(i + 1, j, k - 1), dWx(i + 1, j, k)));
     const T Wyh = ax(ax(dWy(i, j, k - 1), dWy(i, j, k)), ax(dWy(i, j + 1, k - 1), dWy(i, j + 1, k)));
-    const T dyg0 = yg[j] - yg[j - 1], dyg1 = yg[j + 1] - yg[j];
-    const T dzg0 = zg[k] - zg[k - 1], dzg1 = zg[k + 1] - zg[k];
-    const T dxg0 = xg[i] - xg[i - 1], dxg1 = xg[i + 1] - xg[i];
-    const T Uyd = ax(ax(dUy(i - 1, j, k) / dyg0, dUy(i, j, k) / dyg0), ax(dUy(i - 1, j + 1, k) / dyg1, dUy(i, j + 1, k) / dyg1));
-    const T Uzd = ax(ax(dUz(i - 1, j, k) / dzg0, dUz(i, j, k) / dzg0), ax(dUz(i - 1, j, k + 1) / dzg1, dUz(i, j, k + 1) / dzg1));
-    const T Vxd = ax(ax(dVx(i, j - 1, k) / dxg0, dVx(i, j, k) / dxg0), ax(dVx(i + 1, j - 1, k) / dxg1, dVx(i + 1, j, k) / dxg1));
-    const T Vzd = ax(ax(dVz(i, j - 1, k) / dzg0, dVz(i, j, k) / dzg0), ax(dVz(i, j - 1, k + 1) / dzg1, dVz(i, j, k + 1) / dzg1));
-    const T Wxd = ax(ax(dWx(i, j, k - 1) / dxg0, dWx(i, j, k) / dxg0), ax(dWx(i + 1, j, k - 1) / dxg1, dWx(i + 1, j, k) / dxg1));
-    const T Wyd = ax(ax(dWy(i, j, k - 1) / dyg0, dWy(i, j, k) / dyg0), ax(dWy(i, j + 1, k - 1) / dyg1, dWy(i, j + 1, k) / dyg1));
+    const T ry0 = g.rdyg0, ry1 = g.rdyg1, rz0 = gz.rdzg0, rz1 = gz.rdzg1, rx0 = g.rdxg0, rx1 = g.rdxg1;
+    const T Uyd = ax(ax(dUy(i - 1, j, k) * ry0, dUy(i, j, k) * ry0), ax(dUy(i - 1, j + 1, k) * ry1, dUy(i, j + 1, k) * ry1));
+    const T Uzd = ax(ax(dUz(i - 1, j, k) * rz0, dUz(i, j, k) * rz0), ax(dUz(i - 1, j, k + 1) * rz1, dUz(i, j, k + 1) * rz1));
+    const T Vxd = ax(ax(dVx(i, j - 1, k) * rx0, dVx(i, j, k) * rx0), ax(dVx(i + 1, j - 1, k) * rx1, dVx(i + 1, j, k) * rx1));
+    const T Vzd = ax(ax(dVz(i, j - 1, k) * rz0, dVz(i, j, k) * rz0), ax(dVz(i, j - 1, k + 1) * rz1, dVz(i, j, k + 1) * rz1));
+    const T Wxd = ax(ax(dWx(i, j, k - 1) * rx0, dWx(i, j, k) * rx0), ax(dWx(i + 1, j, k - 1) * rx1, dWx(i + 1, j, k) * rx1));
+    const T Wyd = ax(ax(dWy(i, j, k - 1) * ry0, dWy(i, j, k) * ry0), ax(dWy(i, j + 1, k - 1) * ry1, dWy(i, j + 1, k) * ry1));
     // cross terms formed before the last average (code/subgrid.f90:770-779,803-812,836-845)
     const T z12 = ax(ax(dUz(i - 1, j, k), dUz(i, j, k)) * ax(dVz(i, j - 1, k), dVz(i, j, k)),
                      ax(dUz(i - 1, j, k + 1), dUz(i, j, k + 1)) * ax(dVz(i, j - 1, k + 1), dVz(i, j, k + 1)));
@@ -99,55 +109,152 @@ __device__ __forceinline__ T amd_at(const T* U, const T* V, const T* W, const Me
     N = N + two * (Wyh * Vyh) * S23;
     N = N + two * (Vzh * Wzh) * S23;
     N = N + two * x23 * S23;
-    const T dx = x[i] - x[i - 1], dy = y[j] - y[j - 1], dz = z[k] - z[k - 1];
-    T D = Uxh * Uxh / (dx * dx);
+    T D = Uxh * Uxh * (g.rdx * g.rdx);
     D = D + Uyd * Uyd;
     D = D + Uzd * Uzd;
-    D = D + Vyh * Vyh / (dy * dy);
+    D = D + Vyh * Vyh * (g.rdy * g.rdy);
     D = D + Vxd * Vxd;
     D = D + Vzd * Vzd;
-    D = D + (Wzh * Wzh) / (dz * dz);
+    D = D + (Wzh * Wzh) * (gz.rdz * gz.rdz);
     D = D + Wxd * Wxd;
     D = D + Wyd * Wyd;
     // clipping; 0/0 (uniform neighbourhood) is NaN and clips to 0
     return fmax(-T(0.3) * N / D, T(0));
 }
 
-// every box cell: interior from the model at its canonical image, wall ghost
-// rows per Dirichlet_nu_t (1 mirror, 0 copy, 2 mirror plus the offset
-// 0.38 sqrt(dPdx) (y(2)-y(1)), code/subgrid.f90:864-871); with fuse_wm the
-// wall rows come from the wall model instead
-template <typename T>
-__global__ void __launch_bounds__(256) k_amd(const T* U, const T* V, const T* W, T* nut, Metric<T> m, Dims d,
-                                             int dirichlet, const Scalars* sc, int fuse_wm, const double* alpha, T nu)
+// The unique centres i = 2..nxg-2, j = 2..nyg-1, k = 2..nzg-2 from the LDS
+// ring; each writes its periodic images (x: 1 <- nxg-2, nxg-1 <- 2,
+// nxg <- 3; z alike) and rows 2 and nyg-1 the wall ghost rows 1 and nyg per
+// Dirichlet_nu_t (1 mirror, 0 copy, 2 mirror plus the offset
+// 0.38 sqrt(dPdx) (y(2)-y(1)), code/subgrid.f90:864-871).  With fuse_wm the
+// four wall rows are left to k_amd_wall_rows.
+static constexpr int AMD_KMAX = 32;
+template <typename T, int TX, int TY>
+__global__ void __launch_bounds__(TX* TY) k_amd(const T* __restrict__ U, const T* __restrict__ V,
+                                                const T* __restrict__ W, T* __restrict__ nut, Metric<T> m, Dims d,
+                                                int dirichlet, const Scalars* sc, int fuse_wm, int kchunk)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    if (i > d.nxg || j > d.nyg) return;
-    const long p = at(d, i, j, k);
-    if (fuse_wm && (j <= 2 || j >= d.nyg - 1)) {
-        nut[p] = wm_nut_at<T>(U, alpha, m, d, nu, img_centre(i, d.nxg), k, j <= 2 ? 0 : 1, sc);
-        return;
+    constexpr int PX = TX + 2, PY = TY + 2, NP = PX * PY, NT = TX * TY, NL = (NP + NT - 1) / NT;
+    __shared__ T sh[3][4][NP];  // [field][slot][position]
+    __shared__ T zt[3][AMD_KMAX];  // rdz(k), rdzg(k), rdzg(k+1) at k = kb + q
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+    const GBlk gb = gblk_xcd((d.nzg - 3 + kchunk - 1) / kchunk);
+    const int i0 = gb.x * TX + 2, j0 = gb.y * TY + 2;
+    const int kb = gb.z * kchunk + 2;
+    const int ke = min(kb + kchunk - 1, d.nzg - 2);
+    const int i = i0 + tx, j = j0 + ty;
+    const T* src[3] = {U, V, W};
+    // this thread's share of a halo plane: LDS index and box offset (-1: outside the box)
+    int lq[NL];
+    long lo[NL];
+#pragma unroll
+    for (int r = 0; r < NL; r++) {
+        const int q = tid + r * NT;
+        const int ii = i0 - 1 + q % PX, jj = j0 - 1 + q / PX;
+        lq[r] = q < NP ? q : -1;
+        lo[r] = (q < NP && ii <= d.nxg && jj <= d.nyg) ? at(d, ii, jj, 1) : -1;
     }
-    const int jj = (j == 1) ? 2 : (j == d.nyg ? d.nyg - 1 : j);
-    const T v = amd_at<T>(U, V, W, m, d, img_centre(i, d.nxg), jj, img_centre(k, d.nzg));
-    T out = v;
-    if (jj != j) {
-        if (dirichlet == 1) out = -v;
-        else if (dirichlet == 2) out = -v + T(2.0 * 0.38 * sqrt(sc->dPdx) * (double)(m.y[2] - m.y[1]) / 2.0);
+    T pre[NL][3];
+    auto gload = [&](int p) {
+        const long po = (long)(p - 1) * d.sxy;
+#pragma unroll
+        for (int r = 0; r < NL; r++)
+#pragma unroll
+            for (int f = 0; f < 3; f++) pre[r][f] = lo[r] >= 0 ? src[f][lo[r] + po] : T(0);
+    };
+    auto slot = [&](int p) { return (p - kb + 4) & 3; };
+    auto sstore = [&](int p) {
+        const int s = slot(p);
+#pragma unroll
+        for (int r = 0; r < NL; r++)
+            if (lq[r] >= 0)
+#pragma unroll
+                for (int f = 0; f < 3; f++) sh[f][s][lq[r]] = pre[r][f];
+    };
+    auto F = [&](int f, int a, int b, int c) -> T { return sh[f][slot(c)][(a - i0 + 1) + PX * (b - j0 + 1)]; };
+    AmdXY<T> g;
+    {
+        const int ic = min(i, d.nxg - 2), jc = min(j, d.nyg - 1);
+        g.rdx = m.rdx[ic]; g.rdxg0 = m.rdxg[ic]; g.rdxg1 = m.rdxg[ic + 1];
+        g.rdy = m.rdy[jc]; g.rdyg0 = m.rdyg[jc]; g.rdyg1 = m.rdyg[jc + 1];
     }
-    nut[p] = out;
+    for (int q = tid; q < AMD_KMAX; q += NT) {
+        const int kk = min(kb + q, d.nzg - 1);
+        zt[0][q] = m.rdz[kk];
+        zt[1][q] = m.rdzg[kk];
+        zt[2][q] = m.rdzg[kk + 1];
+    }
+    const bool act = i <= d.nxg - 2 && j <= d.nyg - 1 && !(fuse_wm && (j == 2 || j == d.nyg - 1));
+    const bool ghost = !fuse_wm && (j == 2 || j == d.nyg - 1);
+    const T off = dirichlet == 2 ? T(2.0 * 0.38 * sqrt(sc->dPdx) * (double)(m.y[2] - m.y[1]) / 2.0) : T(0);
+    int xs[3], nxs = 1;
+    xs[0] = i;
+    if (i == d.nxg - 2) xs[nxs++] = 1;
+    if (i == 2) xs[nxs++] = d.nxg - 1;
+    if (i == 3) xs[nxs++] = d.nxg;
+    gload(kb - 1);
+    sstore(kb - 1);
+    gload(kb);
+    sstore(kb);
+    gload(kb + 1);
+    __syncthreads();
+    for (int k = kb; k <= ke; k++) {
+        // plane k+1 into the slot of plane k-3 (free), then the next fetch
+        sstore(k + 1);
+        if (k + 2 <= ke + 1) gload(k + 2);
+        __syncthreads();
+        if (act) {
+            const AmdZ<T> gz{zt[0][k - kb], zt[1][k - kb], zt[2][k - kb]};
+            const T val = amd_gen<T>(F, g, gz, i, j, k);
+            int zs[3], nzs = 1;
+            zs[0] = k;
+            if (k == d.nzg - 2) zs[nzs++] = 1;
+            if (k == 2) zs[nzs++] = d.nzg - 1;
+            if (k == 3) zs[nzs++] = d.nzg;
+            const int jg = j == 2 ? 1 : d.nyg;
+            const T gv = dirichlet == 1 ? -val : (dirichlet == 2 ? -val + off : val);
+            for (int a = 0; a < nxs; a++)
+                for (int c = 0; c < nzs; c++) {
+                    nut[at(d, xs[a], j, zs[c])] = val;
+                    if (ghost) nut[at(d, xs[a], jg, zs[c])] = gv;
+                }
+        }
+    }
+}
+
+// the four wall rows from the wall model, every box cell (i, k) of them
+// (code/wallmodel.f90:23-46 at x face pairs around the canonical centre)
+template <typename T>
+__global__ void __launch_bounds__(256) k_amd_wall_rows(const T* U, T* nut, const double* alpha, Metric<T> m, Dims d,
+                                                       T nu, const Scalars* sc)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 4L * d.nxg * d.nzg) return;
+    const int i = (int)(q % d.nxg) + 1, r = (int)((q / d.nxg) % 4), k = (int)(q / (4L * d.nxg)) + 1;
+    const int j = r < 2 ? r + 1 : d.nyg - 3 + r;
+    nut[at(d, i, j, k)] = wm_nut_at<T>(U, alpha, m, d, nu, img_centre(i, d.nxg), k, j <= 2 ? 0 : 1, sc);
 }
 
 template <typename T>
 void launch_amd(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, const Dims& d, int dirichlet,
                 const Scalars* sc, int fuse_wm, const double* alpha, T nu, hipStream_t s)
 {
-    dim3 blk(64, 4, 1), grd(cdiv(d.nxg, 64), cdiv(d.nyg, 4), d.nzg);
+    constexpr int TX = 32, TY = 8;
+    const int nk = d.nzg - 3;
+    // z chunks of at most AMD_KMAX planes, enough blocks for two rounds on 256 CUs
+    const long tiles = (long)cdiv(d.nxg - 3, TX) * cdiv(d.nyg - 2, TY);
+    int kc = (int)std::min<long>(AMD_KMAX, std::max<long>(4, (tiles * nk + 2047) / 2048));
+    kc = std::min(kc, nk);
+    dim3 blk(TX, TY, 1), grd(cdiv(d.nxg - 3, TX), cdiv(d.nyg - 2, TY), (unsigned)cdiv(nk, kc));
     PROF_BEGIN("k_amd", s);
-    k_amd<T><<<grd, blk, 0, s>>>(U, V, W, nut, m, d, dirichlet, sc, fuse_wm, alpha, nu);
+    k_amd<T, TX, TY><<<grd, blk, 0, s>>>(U, V, W, nut, m, d, dirichlet, sc, fuse_wm, kc);
     PROF_END(s);
+    if (fuse_wm) {
+        const long n = 4L * d.nxg * d.nzg;
+        PROF_BEGIN("k_amd_wall_rows", s);
+        k_amd_wall_rows<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(U, nut, alpha, m, d, nu, sc);
+        PROF_END(s);
+    }
 }
 
 template void launch_amd<double>(const double*, const double*, const double*, double*, const Metric<double>&,

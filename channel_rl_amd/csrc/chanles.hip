@@ -1290,6 +1290,8 @@ void tune_dsm(chan_ctx* c)
 }
 int tune_launch(chan_ctx* c, int kernel)
 {
+    if (kernel == CHAN_KERNEL_DSM && c->LES_model != 2)
+        return fail(c, CHAN_ESTATE, "autotune: the DSM kernel runs only with LES_model 2");
     if (kernel == CHAN_KERNEL_RHS) {
         if (c->prec == CHAN_FP64) tune_rhs<double>(c);
         else tune_rhs<float>(c);
@@ -1317,6 +1319,12 @@ int chan_autotune(chan_ctx* c, int reps, double* times_ms, int max_out)
     int out = 0;
     for (int kernel : {CHAN_KERNEL_RHS, CHAN_KERNEL_DSM}) {
         const int n = chan_tile_shape_count(kernel);
+        if (kernel == CHAN_KERNEL_DSM && c->LES_model != 2) {
+            // no dynamic Smagorinsky, no |S| buffer: nothing to time
+            for (int q = 0; q < n; q++, out++)
+                if (times_ms && out < max_out) times_ms[out] = 0.0;
+            continue;
+        }
         const int keep = chan_get_tile_shape(c, kernel);
         int best = keep;
         double best_ms = 1e300;

@@ -287,7 +287,9 @@ int chan_ensemble_synchronize(chan_ensemble* ens);
  * buffers, the DSM pass A writing |S| and its partial sums; the state is not
  * changed), keeps the fastest of each for the step and, when times_ms is not
  * NULL, writes the average ms per launch of the RHS shapes followed by the
- * DSM shapes, at most max_out values.  DSM shapes change the order of the
+ * DSM shapes, at most max_out values.  A channel whose LES model is not the
+ * dynamic Smagorinsky (LES_model != 2) has no DSM pass: its DSM shapes are
+ * not timed and their times are 0.  DSM shapes change the order of the
  * plane sums (results agree to rounding, not bit for bit). */
 enum { CHAN_KERNEL_RHS = 0, CHAN_KERNEL_DSM = 1 };
 int chan_tile_shape_count(int kernel);

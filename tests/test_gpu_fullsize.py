@@ -67,6 +67,27 @@ def test_fullsize_stages_match_oracle(gpu, wmles):
         assert rel(got_r[q], ref_q) <= 1e-14, (q, rel(got_r[q], ref_q))
 
 
+@pytest.mark.parametrize("dirichlet", [1, 2])
+def test_fullsize_amd_matches_oracle(gpu, wmles, dirichlet):
+    """The AMD eddy viscosity (LES model 3, its LDS-tiled z march with
+    several chunks per column and the ghost rows of two wall conditions) on
+    the 192x97x192 state against the oracle, 1e-12 as on the small grids."""
+    import copy
+    case, _, (U, V, W) = wmles
+    case = copy.copy(case)
+    case.LES_model, case.Dirichlet_nu_t = 3, dirichlet
+    o = oracle_for(case)
+    o.set_scalars(dPdx=0.37)
+    ref = o.compute_eddy_viscosity(U, V, W)
+    g = Channel(case.x, case.y, case.z, precision=FP64, **case.kwargs())
+    g.set_state(U, V, W)
+    g.set_scalars(dt=0.0, t=0.0, dPdx=0.37, Qflow_x_0=0.0)
+    g.compute_eddy_viscosity()
+    got = g.get_field(NU_T)
+    assert np.all(np.isfinite(got))
+    assert rel(got, ref) <= 1e-12, rel(got, ref)
+
+
 def test_fullsize_projection_matches_oracle_and_is_idempotent(gpu, wmles):
     case, field, _ = wmles
     o = oracle_for(case)

@@ -262,6 +262,20 @@ def test_amd_matches_oracle(gpu, n, dirichlet):
     assert rel(got, ref) <= 1e-12, rel(got, ref)
 
 
+def test_autotune_without_the_dynamic_model(gpu):
+    """A channel with the AMD model (LES_model 3) has no DSM pass and no |S|
+    buffer: the autotuner times the RHS shapes only, reports 0 for the DSM
+    shapes and leaves the state and the step as they were."""
+    from channel_rl_amd import KERNEL_DSM, load_library
+    case, o, g, (U, V, W) = make_pair(18, 17, 18, LES_model=3)
+    before = g.state()
+    t = g.autotune(reps=1)
+    assert np.all(t["rhs"] > 0)
+    assert t["dsm"].shape == (load_library().chan_tile_shape_count(KERNEL_DSM),) and np.all(t["dsm"] == 0)
+    for a, b in zip(g.state(), before):
+        assert np.array_equal(a, b)
+
+
 def test_rk3_with_amd(gpu):
     case, o, g = run_pair(18, 17, 18, 3, LES_model=3)
     for a, b in zip(g.state(), o.state()):
