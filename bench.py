@@ -373,9 +373,11 @@ def main():
             ab = cells * (b0 * bulk_b + b1 * 8)
             st["alg_bytes"] = ab
             st["alg_hbm_frac"] = round(ab / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
-        if name in traffic_all:
-            st["hbm_bytes"] = int(traffic_all[name])
-            st["hbm_frac"] = round(traffic_all[name] / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
+        # the profiles name the tridiagonal kernels by their form (k_tridiag_tw: rows in registers)
+        tb = traffic_all.get(name, traffic_all.get(name + "_tw"))
+        if tb is not None:
+            st["hbm_bytes"] = int(tb)
+            st["hbm_frac"] = round(tb / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
         stages[name] = st
     copy_gbs, copy_ms = ch.stream_copy(1 << 30, reps=10)
     log(f"[bench] stream copy 1 GiB: {copy_gbs:.0f} GB/s (read + write), {copy_ms:.3f} ms")

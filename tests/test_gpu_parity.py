@@ -181,18 +181,21 @@ def test_rk3_ten_steps_small_config(gpu):
     assert ma <= 1e-10, ma
 
 
-def test_rhs_tile_shapes_agree_bitwise(gpu):
-    # every tile shape of the fused RHS + RK kernel does the same arithmetic
-    # per cell: one RK3 step from the same state must agree bit for bit; the
-    # autotuner leaves the state alone and picks one of the shapes
-    from channel_rl_amd import load_library, KERNEL_RHS
+@pytest.mark.parametrize("precision", [FP64, "fp32"])
+def test_rhs_tile_shapes_agree_bitwise(gpu, precision):
+    # every tile shape of the fused RHS + RK kernel (tile, z chunk) does the
+    # same arithmetic per cell: one RK3 step from the same state must agree
+    # bit for bit; the autotuner leaves the state alone and picks one of the
+    # shapes
+    from channel_rl_amd import FP32_BULK, load_library, KERNEL_RHS
+    prec = FP64 if precision == FP64 else FP32_BULK
     n = load_library().chan_tile_shape_count(KERNEL_RHS)
     assert n >= 2
     case, field = cases.small_case(34, 33, 34, seed=11)
     U, V, W = projected_state(case, field)
     out = []
     for shape in range(n):
-        g = Channel(case.x, case.y, case.z, precision=FP64, **case.kwargs())
+        g = Channel(case.x, case.y, case.z, precision=prec, **case.kwargs())
         g.set_state(U, V, W)
         g.set_scalars(dt=0.0, t=0.0, dPdx=case.dPdx, Qflow_x_0=g.compute_mean_mass_flow_U())
         g.set_wall_model(1, h_wm=case.h_wm)
