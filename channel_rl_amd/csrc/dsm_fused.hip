@@ -74,7 +74,7 @@ __device__ __forceinline__ T strain_offdiag(const FA& A, const FB& B, T ra0, T r
 // The register budget is held to two waves per SIMD (the FP64 kernels need
 // it; the FP32 ones fit in fewer registers and run three or four).
 template <typename T, int TXO, int TYO, bool G>
-__global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 4)))
     k_dsm_fused(DsmArgs<T> one, const DsmArgs<T>* tab, Dims d, int kchunk)
 {
     // The model's sums are reordered against the reference anyway (separable
@@ -247,11 +247,14 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
 
     // level 1 of this column at plane p: the filter inputs Q = a (3),
     // a_a a_b (6), |S| S_ij (6); |S| of the output planes to global memory
+    // Rows outside 2..nyg-1 (the wall ghost rows and beyond) compute on the
+    // zeros loaded there and finite metric factors: their inputs enter the
+    // y taps of rows 2 and nyg-1 only with weight 0 and the other taps not
+    // at all, so no branch is needed around them.
     auto level1 = [&](int p, T Q[15]) {
         const int pc = wz(p);
-        T a[3] = {T(0), T(0), T(0)}, ss[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-        T sm = T(0);
-        if (cj >= 2 && cj <= d.nyg - 1) {
+        T a[3], ss[6], sm;
+        {
             // code/subgrid.f90:394-396 centre velocities
             a[0] = h * (L0(0, p, -1, 0) + L0(0, p, 0, 0));
             a[1] = h * (L0(1, p, 0, -1) + L0(1, p, 0, 0));
@@ -333,7 +336,7 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     // six times the y weights (the one-sided rows' 1/3, 2/3 become 2, 4)
     const T wym = (cj == 2) ? T(0) : (cj == d.nyg - 1 ? T(2) : T(1));
     const T wyp = (cj == d.nyg - 1) ? T(0) : (cj == 2 ? T(2) : T(1));
-    const int om = cx + CX * (cy - 1), op = cx + CX * (cy + 1);
+    const int om = cx + CX * max(cy - 1, 0), op = cx + CX * min(cy + 1, Tl::CY - 1);
 
     // one output plane k; za/zb and ga/gb as described above (roles swap)
     auto plane = [&](int k, T (&za)[15], T (&zb)[15], T (&ga)[2][3], T (&gb)[2][3]) {
@@ -350,11 +353,9 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
         __syncthreads();
         // phase 2: x-y taps of the 15 filter inputs, every lane (cross-lane moves)
         T tp[15];
-        if (cy >= 1 && cy <= TYO) {
-            for (int c = 0; c < 15; c++) tp[c] = wym * zq[c][om] + T(4) * zown[c] + wyp * zq[c][op];
-        } else {
-            for (int c = 0; c < 15; c++) tp[c] = T(0);
-        }
+        // on the halo rows (cy 0 and CY-1, clamped indices) the taps are
+        // taken as well and never used: no branch
+        for (int c = 0; c < 15; c++) tp[c] = wym * zq[c][om] + T(4) * zown[c] + wyp * zq[c][op];
         for (int c = 0; c < 15; c++) {
             const T y = tp[c];
             tp[c] = (lane_shift<DPP_WAVE_SHR1>(y) + lane_shift<DPP_WAVE_SHL1>(y)) + T(4) * y;
