@@ -587,6 +587,8 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
         return fail(c, CHAN_EINVAL, "Error: nx-2 and nz-2 must factor into 2, 3 and 5");
     // one block holds a whole line (and its batch) in LDS
     if (d.nxp > 2048 || d.nzp > 2048) return fail(c, CHAN_EINVAL, "Error: nx-2 and nz-2 must be at most 2048");
+    // the reductions flatten a box's cells into 32-bit indices
+    if (d.nbox > 0x7fffffffL) return fail(c, CHAN_EINVAL, "Error: the grid has more than 2^31 box cells");
 
     int rc = set_device(c);
     if (rc) return rc;

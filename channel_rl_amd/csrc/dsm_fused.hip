@@ -27,6 +27,9 @@
 
 namespace chan {
 
+// longest z chunk a block marches (round_filling_chunk's upper bound below)
+constexpr int DSM_KMAX = 64;
+
 template <int TXO, int TYO>
 struct DsmTile {
     static constexpr int CX = TXO + 2, CY = TYO + 2, NC = CX * CY;  // columns
@@ -108,14 +111,23 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     auto wz = [&](int q) { return q < 2 ? q + nzp : (q > nzp + 1 ? q - nzp : q); };
     const T h = T(0.5);
     const T s6 = T(1.0 / 6.0), s23 = T(2.0 / 3.0), s13 = T(1.0 / 3.0);
-    const T *rdz = b.rdz, *rdzg = b.rdzg;
-    // x and y reciprocal spacings of the block's columns and rows, in LDS:
-    // read per plane from here rather than from global memory, whose loads
-    // would wait behind the velocity prefetch (memory counters drain in
-    // issue order).  Level 1 uses the canonical x image, the outputs the
-    // true index (code/subgrid.f90:456); z factors are uniform per plane.
+    // x, y and z reciprocal spacings of the block's columns, rows and
+    // planes, in LDS: read per plane from here rather than from global
+    // memory, whose loads would wait behind the velocity prefetch (memory
+    // counters drain in issue order, so waiting for a metric load issued
+    // after the prefetch waits for the prefetch too).  Level 1 uses the
+    // canonical x image, the outputs the true index (code/subgrid.f90:456);
+    // the z factors of plane p = kb-1+q sit at q (p's periodic image, which
+    // is p itself on the block's output planes).
     __shared__ T mx[6][CX];  // rdx, rdxg(i), rdxg(i+1) at wx(ci); the same at ci
     __shared__ T my[3][TYO + 2];  // rdy, rdyg(j), rdyg(j+1)
+    __shared__ T mz[3][DSM_KMAX + 3];  // rdz(p), rdzg(p), rdzg(p+1)
+    for (int q = tid; q < ke - kb + 3; q += NC) {
+        const int pc = wz(kb - 1 + q);
+        mz[0][q] = b.rdz[pc];
+        mz[1][q] = b.rdzg[pc];
+        mz[2][q] = b.rdzg[pc + 1];
+    }
     if (tid < CX) {
         const int ic = wx(i0 - 1 + tid), it = min(max(i0 - 1 + tid, 2), d.nxg - 1);
         mx[0][tid] = b.rdx[ic];
@@ -252,7 +264,6 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     // y taps of rows 2 and nyg-1 only with weight 0 and the other taps not
     // at all, so no branch is needed around them.
     auto level1 = [&](int p, T Q[15]) {
-        const int pc = wz(p);
         T a[3], ss[6], sm;
         {
             // code/subgrid.f90:394-396 centre velocities
@@ -263,8 +274,9 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
             T s[6];
             s[0] = (L0(0, p, 0, 0) - L0(0, p, -1, 0)) * mx[0][cx];
             s[1] = (L0(1, p, 0, 0) - L0(1, p, 0, -1)) * my[0][cy];
-            s[2] = (L0(2, p, 0, 0) - L0(2, p - 1, 0, 0)) * rdz[pc];
-            const T rz0 = rdzg[pc], rz1 = rdzg[pc + 1];
+            const int zq_ = p - kb + 1;
+            s[2] = (L0(2, p, 0, 0) - L0(2, p - 1, 0, 0)) * mz[0][zq_];
+            const T rz0 = mz[1][zq_], rz1 = mz[2][zq_];
             s[3] = strain_offdiag([&](int a, int c) { return L0(0, p, a, c); },
                                   [&](int a, int c) { return L0(1, p, a, c); }, mx[1][cx], mx[2][cx], my[1][cy],
                                   my[2][cy]);
@@ -368,8 +380,8 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
             T sh[6];
             sh[0] = (uf(0, 0, 0) - uf(-1, 0, 0)) * mx[3][cx];
             sh[1] = (vf(0, 0, 0) - vf(0, -1, 0)) * my[0][cy];
-            sh[2] = (wf(0, 0, 0) - wf(0, 0, -1)) * rdz[k];
-            const T rz0 = rdzg[k], rz1 = rdzg[k + 1];
+            sh[2] = (wf(0, 0, 0) - wf(0, 0, -1)) * mz[0][k - kb + 1];
+            const T rz0 = mz[1][k - kb + 1], rz1 = mz[2][k - kb + 1];
             sh[3] = strain_offdiag([&](int a, int c) { return uf(a, c, 0); }, [&](int a, int c) { return vf(a, c, 0); },
                                    mx[4][cx], mx[5][cx], my[1][cy], my[2][cy]);
             sh[4] = strain_offdiag([&](int a, int c) { return uf(a, 0, c); }, [&](int a, int c) { return wf(a, 0, c); },
@@ -440,15 +452,19 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
 // shape 2, 14 x 14 outputs in 256 threads (72 KB), puts two blocks of 4
 // waves on a CU, whose barriers overlap, for 7% more halo columns.  FP32:
 // shapes 0 and 1 are 256-thread blocks (40 KB), shape 2 is 30 x 14 outputs
-// in 512 threads (72 KB), the least halo per output.
+// in 512 threads (72 KB), the least halo per output.  Shape 3, both
+// precisions: 14 x 6 outputs in 128 threads (FP64 39 KB: four blocks of two
+// waves per CU), 1.52 columns per output, for small grids, where it gives
+// one channel more blocks.
 struct DsmShape { int txo, tyo; };
 template <typename T>
 constexpr DsmShape dsm_shape(int q)
 {
-    return sizeof(T) == 8 ? (q == 0 ? DsmShape{30, 14} : (q == 1 ? DsmShape{14, 30} : DsmShape{14, 14}))
-                          : (q == 0 ? DsmShape{30, 6} : (q == 1 ? DsmShape{14, 14} : DsmShape{30, 14}));
+    return q == 3 ? DsmShape{14, 6}
+                  : (sizeof(T) == 8 ? (q == 0 ? DsmShape{30, 14} : (q == 1 ? DsmShape{14, 30} : DsmShape{14, 14}))
+                                    : (q == 0 ? DsmShape{30, 6} : (q == 1 ? DsmShape{14, 14} : DsmShape{30, 14})));
 }
-int dsm_shape_count() { return 3; }
+int dsm_shape_count() { return 4; }
 
 template <typename T, int TXO, int TYO>
 int dsm_per_cu()
@@ -471,10 +487,11 @@ int dsm_fused_kchunk(const Dims& d, int device, int shape)
     const DsmShape sh = dsm_shape<T>(shape);
     const int per_cu = shape == 0   ? dsm_per_cu<T, dsm_shape<T>(0).txo, dsm_shape<T>(0).tyo>()
                        : shape == 1 ? dsm_per_cu<T, dsm_shape<T>(1).txo, dsm_shape<T>(1).tyo>()
-                                    : dsm_per_cu<T, dsm_shape<T>(2).txo, dsm_shape<T>(2).tyo>();
+                       : shape == 2 ? dsm_per_cu<T, dsm_shape<T>(2).txo, dsm_shape<T>(2).tyo>()
+                                    : dsm_per_cu<T, dsm_shape<T>(3).txo, dsm_shape<T>(3).tyo>();
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu < 1) ncu = 1;
     const long tiles = (long)cdiv(d.nxg - 2, sh.txo) * cdiv(d.nyg - 2, sh.tyo);
-    return round_filling_chunk(tiles, d.nzg - 2, (long)per_cu * ncu, 8, 64);
+    return round_filling_chunk(tiles, d.nzg - 2, (long)per_cu * ncu, 8, DSM_KMAX);
 }
 
 template <typename T>
@@ -487,6 +504,10 @@ template <typename T>
 void run_dsm_fused(const DsmArgs<T>& one, const DsmArgs<T>* tab, int nmem, const Dims& d, int kchunk, int shape,
                    hipStream_t s)
 {
+    if (kchunk < 1 || kchunk > DSM_KMAX) {
+        fprintf(stderr, "chanles: DSM z chunk %d outside 1..%d\n", kchunk, DSM_KMAX);
+        abort();
+    }
     PROF_BEGIN("k_dsm_fused", s);
 #define DSM_GO(Q)                                                                                                   \
     do {                                                                                                            \
@@ -496,7 +517,8 @@ void run_dsm_fused(const DsmArgs<T>& one, const DsmArgs<T>* tab, int nmem, const
         if (tab) k_dsm_fused<T, sh.txo, sh.tyo, true><<<grd, blk, 0, s>>>(one, tab, d, kchunk);                     \
         else k_dsm_fused<T, sh.txo, sh.tyo, false><<<grd, blk, 0, s>>>(one, tab, d, kchunk);                        \
     } while (0)
-    if (shape == 2) DSM_GO(2);
+    if (shape == 3) DSM_GO(3);
+    else if (shape == 2) DSM_GO(2);
     else if (shape == 1) DSM_GO(1);
     else DSM_GO(0);
 #undef DSM_GO

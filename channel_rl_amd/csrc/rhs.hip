@@ -149,10 +149,11 @@ __device__ __forceinline__ T rhs_w_gen(const A& F, const RhsXY<T>& g, const RhsZ
     return ((t1 + t2) + t3) + t4;
 }
 
-// mode 0: rhs_u/v/w into Ou/Ov/Ow on the reference's interior ranges;
-// mode s = 1,2,3: Wray substep s.  In RK modes the x and z ranges stop at the
-// last unique cell (nxg-2, nzg-2): the image columns the reference computes
-// there are overwritten by the boundary conditions that follow.
+// MODE 0: rhs_u/v/w into Ou/Ov/Ow on the reference's interior ranges;
+// MODE 1: Wray substep 1; MODE 2: substeps 2 and 3.  In RK modes the x and
+// z ranges stop at the last unique cell (nxg-2, nzg-2): the image columns
+// the reference computes there are overwritten by the boundary conditions
+// that follow.
 //
 // Pipelining: while plane k is computed from the LDS ring (four slots,
 // planes k-1, k, k+1 in use), the loads of planes k+2 and k+3 are in flight
@@ -161,25 +162,30 @@ __device__ __forceinline__ T rhs_w_gen(const A& F, const RhsXY<T>& g, const RhsZ
 // its reads, and every load has two planes of work to arrive.
 // In substeps 2 and 3 the thread's Uo and F1 values are fetched before the
 // stencil as well.
+// Every global load is issued unconditionally (positions outside the box
+// read a valid address and select 0; the last planes of a chunk re-read
+// plane ke+1): the memory counter drains in issue order, and with a fixed
+// number of loads per plane the compiler's wait for the oldest buffer
+// leaves the newer loads in flight.  A load skipped on some path would
+// make every such wait a wait for all outstanding loads.
 static constexpr int RHS_KMAX = 32;  // longest z chunk a block marches
 
 // Register budget: three waves per SIMD (168 VGPRs) in FP64 (two for the
 // 512-thread tile, which one block per CU holds by its LDS), four in FP32.
-template <typename T, int TX, int TY, bool G>
+template <typename T, int TX, int TY, int MODE, bool G>
 __global__ void __launch_bounds__(TX* TY)
     __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (TX * TY > 256 ? 2 : 3) : 4)))
-    k_rhs_rk(RhsArgs<T> one, const RhsArgs<T>* tab, Dims d, int mode, T c1, T c2, int kchunk)
+    k_rhs_rk(RhsArgs<T> one, const RhsArgs<T>* tab, Dims d, T c1, T c2, int kchunk)
 {
     constexpr int PX = TX + 2, PY = TY + 2, NP = PX * PY, NT = TX * TY, NL = (NP + NT - 1) / NT;
     __shared__ T sh[4][4][NP];  // [field][slot][position]
     // z reciprocal spacings of the block's planes kb..ke+1, read from LDS per
     // plane: a global load there would wait behind the prefetched planes
-    // (memory counters drain in issue order)
     constexpr int KZ = RHS_KMAX + 2;
     __shared__ T zt[2][KZ];  // 1/(z(k)-z(k-1)), 1/(zg(k)-zg(k-1)) at k = kb + q
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-    const int imax = mode == 0 ? d.nxg - 1 : d.nxg - 2;
-    const int kmax = mode == 0 ? d.nzg - 1 : d.nzg - 2;
+    const int imax = MODE == 0 ? d.nxg - 1 : d.nxg - 2;
+    const int kmax = MODE == 0 ? d.nzg - 1 : d.nzg - 2;
     const GBlk gb = gblk_xcd((kmax - 1 + kchunk - 1) / kchunk);
     const RhsArgs<T> arg = member_args<G>(one, tab, gb.m);
     const T* __restrict__ U = arg.U;
@@ -197,16 +203,20 @@ __global__ void __launch_bounds__(TX* TY)
     const int ke = min(kb + kchunk - 1, kmax);
     const int i = i0 + tx, j = j0 + ty;
     const T* src[4] = {U, V, W, nt};
-    // this thread's share of a halo plane: LDS index and box offset (-1: outside)
+    // this thread's share of a halo plane: LDS index (-1: none), box offset
+    // within a plane (a valid one, 0, where the position is outside the
+    // box) and whether the position is inside
     int lq[NL];
     long lo[NL];
+    bool lin[NL];
 #pragma unroll
     for (int r = 0; r < NL; r++) {
         const int q = tid + r * NT;
         const int lx = q % PX, ly = q / PX;
         const int ii = i0 - 1 + lx, jj = j0 - 1 + ly;
         lq[r] = q < NP ? q : -1;
-        lo[r] = (q < NP && ii <= d.nxg && jj <= d.nyg) ? at(d, ii, jj, 1) : -1;
+        lin[r] = q < NP && ii <= d.nxg && jj <= d.nyg;
+        lo[r] = lin[r] ? at(d, ii, jj, 1) : 0;
     }
     // two register buffers: the plane stored at the end of an iteration was
     // fetched two iterations earlier
@@ -216,7 +226,10 @@ __global__ void __launch_bounds__(TX* TY)
 #pragma unroll
         for (int r = 0; r < NL; r++)
 #pragma unroll
-            for (int f = 0; f < 4; f++) pre[r][f] = lo[r] >= 0 ? src[f][lo[r] + po] : T(0);
+            for (int f = 0; f < 4; f++) {
+                const T x = src[f][lo[r] + po];
+                pre[r][f] = lin[r] ? x : T(0);
+            }
     };
     auto slot = [&](int p) { return (p - kb + 4) & 3; };
     auto sstore = [&](int p, const T (&pre)[NL][4]) {
@@ -231,8 +244,8 @@ __global__ void __launch_bounds__(TX* TY)
     const T dPdx = T(sc->dPdx);
     const T dt = T(sc->dt);
     RhsXY<T> g;
+    const int ic = min(i, imax), jc = min(j, d.nyg - 1);
     {
-        const int ic = min(i, imax), jc = min(j, d.nyg - 1);
         g.rx_f = tb.rdx[ic]; g.rx_fp = tb.rdx[ic + 1]; g.rx_c = tb.rdxg[ic]; g.rx_cp = tb.rdxg[ic + 1];
         g.ry_f = tb.rdy[jc]; g.ry_fp = tb.rdy[jc + 1]; g.ry_c = tb.rdyg[jc]; g.ry_cp = tb.rdyg[jc + 1];
         g.w0j = tb.w0[jc]; g.w1j = tb.w1[jc]; g.w0m = tb.w0[jc - 1]; g.w1m = tb.w1[jc - 1];
@@ -248,59 +261,89 @@ __global__ void __launch_bounds__(TX* TY)
     sstore(kb, preA);
     gload(kb + 1, preA);
     sstore(kb + 1, preA);
-    if (kb + 2 <= ke + 1) gload(kb + 2, preA);
+    gload(min(kb + 2, ke + 1), preA);
     __syncthreads();
     const bool act = i <= imax && j <= d.nyg - 1;
+    const bool du0 = act && i <= d.nx - 1, dv0 = act && j <= d.ny - 1;
+    // the thread's own cell in plane 1 (clamped into the box for inactive
+    // threads, which load but do not store)
+    const long pc0 = at(d, ic, jc, 1);
+    // the y and x images of a stored value: the wall mirror rows (U, W), the
+    // first and last unique x columns; z images by plane, below
+    const bool yimg = j == 2 || j == d.nyg - 1;
+    const bool ximg_f = i == d.nx - 1 || i == 2;
+    const bool ximg_c = i == d.nxg - 2 || i == 2 || i == 3;
     // one plane of output from the ring (planes k-1, k, k+1).  The thread's
     // own Uo, F1 loads are issued before the prefetch (fetch): memory
     // counters drain in issue order, so waiting for them does not wait for
     // the prefetched planes.
     auto plane = [&](int k, auto&& fetch) {
-        const long p = at(d, i, j, k);
-        const bool du = act && (i <= d.nx - 1) && (k <= d.nzg - 1);
-        const bool dv = act && (j <= d.ny - 1) && (k <= d.nzg - 1);
+        const long p = pc0 + (long)(k - 1) * d.sxy;
+        const bool du = du0 && (k <= d.nzg - 1);
+        const bool dv = dv0 && (k <= d.nzg - 1);
         const bool dw = act && (k <= d.nz - 1);
         T uo = T(0), vo = T(0), wo = T(0), fu = T(0), fv = T(0), fw = T(0);
         // own-cell values read once and the substep-1 F written once: marked
         // non-temporal so that they do not displace the tiles' halo planes in L2
-        if (mode >= 2) {
-            if (du) { uo = __builtin_nontemporal_load(&Uo[p]); fu = __builtin_nontemporal_load(&F1u[p]); }
-            if (dv) { vo = __builtin_nontemporal_load(&Vo[p]); fv = __builtin_nontemporal_load(&F1v[p]); }
-            if (dw) { wo = __builtin_nontemporal_load(&Wo[p]); fw = __builtin_nontemporal_load(&F1w[p]); }
+        if constexpr (MODE == 2) {
+            uo = __builtin_nontemporal_load(&Uo[p]); fu = __builtin_nontemporal_load(&F1u[p]);
+            vo = __builtin_nontemporal_load(&Vo[p]); fv = __builtin_nontemporal_load(&F1v[p]);
+            wo = __builtin_nontemporal_load(&Wo[p]); fw = __builtin_nontemporal_load(&F1w[p]);
         }
+        asm volatile("" ::: "memory");  // the own-cell loads before the prefetch
         fetch();
+        // an empty asm with a memory clobber keeps the compiler from sinking
+        // the own-cell loads into the branches that use them (after the
+        // prefetch, where waiting for them would wait for the prefetch too)
+        asm volatile("" ::: "memory");
         RhsZ<T> gz;
         gz.rz_f = zt[0][k - kb]; gz.rz_fp = zt[0][k - kb + 1]; gz.rz_c = zt[1][k - kb]; gz.rz_cp = zt[1][k - kb + 1];
-        T ru = T(0), rv = T(0), rw = T(0);
-        if (du) ru = rhs_u_gen<T>(F, g, gz, nu, dPdx, i, j, k);
-        if (dv) rv = rhs_v_gen<T>(F, g, gz, nu, i, j, k);
-        if (dw) rw = rhs_w_gen<T>(F, g, gz, nu, i, j, k);
-        if (mode == 0) {
-            if (du) Ou[p] = ru;
-            if (dv) Ov[p] = rv;
-            if (dw) Ow[p] = rw;
-        } else if (mode == 1) {
-            // U = Uo + dt*rk_coef(1,1)*Fu1 with Uo = U at substep 1; the
-            // new field is stored with its boundary-condition images
-            if (du) { __builtin_nontemporal_store(ru, &F1u[p]); store_img(Ou, d, 0, i, j, k, F(0, i, j, k) + dt * c1 * ru); }
-            if (dv) { __builtin_nontemporal_store(rv, &F1v[p]); store_img(Ov, d, 1, i, j, k, F(1, i, j, k) + dt * c1 * rv); }
-            if (dw) { __builtin_nontemporal_store(rw, &F1w[p]); store_img(Ow, d, 2, i, j, k, F(2, i, j, k) + dt * c1 * rw); }
-        } else {
-            // U = Uo + dt*(c1*Fu1 + c2*Fu_s); rk_coef(3,2) = 0 so the second stage's F is not kept
-            if (du) store_img(Ou, d, 0, i, j, k, uo + dt * (c1 * fu + c2 * ru));
-            if (dv) store_img(Ov, d, 1, i, j, k, vo + dt * (c1 * fv + c2 * rv));
-            if (dw) store_img(Ow, d, 2, i, j, k, wo + dt * (c1 * fw + c2 * rw));
+        // each component is computed and stored before the next one, which
+        // keeps the live registers of one stencil at a time
+        const bool zimg_c = k == d.nzg - 2 || k == 2 || k == 3;
+        const bool zimg_f = k == d.nz - 1 || k == 2;
+        auto put = [&](int kind, bool on, bool img, T r, T old, T f1, T* O, T* F1) {
+            if constexpr (MODE == 0) {
+                if (on) O[p] = r;
+            } else {
+                T nv;
+                if constexpr (MODE == 1) {
+                    // U = Uo + dt*rk_coef(1,1)*Fu1 with Uo = U at substep 1
+                    nv = old + dt * c1 * r;
+                    if (on) __builtin_nontemporal_store(r, &F1[p]);
+                } else {
+                    // U = Uo + dt*(c1*Fu1 + c2*Fu_s); rk_coef(3,2) = 0 so the second stage's F is not kept
+                    nv = old + dt * (c1 * f1 + c2 * r);
+                }
+                // the new field with its boundary-condition images: the
+                // canonical store, then the images of the rare cells that have any
+                if (on) O[p] = nv;
+                if (on && img) store_img(O, d, kind, i, j, k, nv);
+            }
+        };
+        // (the branches hold no loads: the number of loads per plane is fixed)
+        if (du) {
+            const T r = rhs_u_gen<T>(F, g, gz, nu, dPdx, i, j, k);
+            put(0, true, ximg_f || yimg || zimg_c, r, MODE == 1 ? F(0, i, j, k) : uo, fu, Ou, F1u);
+        }
+        if (dv) {
+            const T r = rhs_v_gen<T>(F, g, gz, nu, i, j, k);
+            put(1, true, ximg_c || zimg_c, r, MODE == 1 ? F(1, i, j, k) : vo, fv, Ov, F1v);
+        }
+        if (dw) {
+            const T r = rhs_w_gen<T>(F, g, gz, nu, i, j, k);
+            put(2, true, ximg_c || yimg || zimg_f, r, MODE == 1 ? F(2, i, j, k) : wo, fw, Ow, F1w);
         }
     };
     // unrolled by two so the register buffers alternate without copies;
     // plane k+2 goes to the slot of plane k-2, last read before the
     // previous barrier
     for (int k = kb; k <= ke; k += 2) {
-        plane(k, [&] { if (k + 3 <= ke + 1) gload(k + 3, preB); });
+        plane(k, [&] { gload(min(k + 3, ke + 1), preB); });
         if (k + 2 <= ke + 1) sstore(k + 2, preA);
         __syncthreads();
         if (k + 1 > ke) break;
-        plane(k + 1, [&] { if (k + 4 <= ke + 1) gload(k + 4, preA); });
+        plane(k + 1, [&] { gload(min(k + 4, ke + 1), preA); });
         if (k + 3 <= ke + 1) sstore(k + 3, preB);
         __syncthreads();
     }
@@ -309,7 +352,9 @@ __global__ void __launch_bounds__(TX* TY)
 // Tile shapes (x cells, y cells) and z-chunk lengths the launcher offers;
 // chan_autotune times them on the channel at hand.  kchunk 0: the chunk that
 // fills the last round of blocks on this grid and device
-// (round_filling_chunk with the kernel's occupancy).
+// (round_filling_chunk with the kernel's occupancy), counting the tiles of
+// every member of a grouped launch (each cell is computed once whatever the
+// chunk, so the results do not depend on it).
 struct RhsShape { int tx, ty, kchunk; };
 static const RhsShape kRhsShapes[] = {
     {64, 4, 16}, {64, 4, 8}, {64, 8, 8}, {64, 8, 16}, {32, 8, 8}, {32, 8, 16}, {64, 4, 32}, {64, 8, 32},
@@ -317,16 +362,16 @@ static const RhsShape kRhsShapes[] = {
 };
 
 template <typename T, int TX, int TY>
-int rhs_round_chunk(int ni, int nj, int nk)
+int rhs_round_chunk(int ni, int nj, int nk, int nmem)
 {
     int per_cu = 0, ncu = 0, dev = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rhs_rk<T, TX, TY, false>, TX * TY, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rhs_rk<T, TX, TY, 2, false>, TX * TY, 0) != hipSuccess ||
         per_cu < 1)
         per_cu = 1;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
         ncu = 1;
-    return round_filling_chunk((long)cdiv(ni, TX) * cdiv(nj, TY), nk, (long)per_cu * ncu, 6, 32);
+    return round_filling_chunk((long)cdiv(ni, TX) * cdiv(nj, TY) * nmem, nk, (long)per_cu * ncu, 6, 32);
 }
 int rhs_shape_count() { return (int)(sizeof(kRhsShapes) / sizeof(kRhsShapes[0])); }
 
@@ -339,17 +384,24 @@ void run_rhs_rk(const RhsArgs<T>& one, const RhsArgs<T>* tab, int nmem, const Di
     const int nk = (mode == 0 ? d.nzg - 1 : d.nzg - 2) - 1;
     dim3 blk(sh.tx, sh.ty, 1);
     PROF_BEGIN("k_rhs_rk", s);
+#define RHS_MODE(TX, TY, M)                                                                                         \
+    do {                                                                                                            \
+        if (tab) k_rhs_rk<T, TX, TY, M, true><<<grd, blk, 0, s>>>(one, tab, d, T(c1), T(c2), kc);                   \
+        else k_rhs_rk<T, TX, TY, M, false><<<grd, blk, 0, s>>>(one, tab, d, T(c1), T(c2), kc);                      \
+    } while (0)
 #define RHS_GO(TX, TY)                                                                                              \
     do {                                                                                                            \
-        const int kc = std::min(RHS_KMAX, sh.kchunk > 0 ? sh.kchunk : rhs_round_chunk<T, TX, TY>(ni, d.nyg - 2, nk)); \
+        const int kc = std::min(RHS_KMAX, sh.kchunk > 0 ? sh.kchunk : rhs_round_chunk<T, TX, TY>(ni, d.nyg - 2, nk, nmem)); \
         dim3 grd(cdiv(ni, TX), cdiv(d.nyg - 2, TY), (unsigned)cdiv(nk, kc) * nmem);                                 \
-        if (tab) k_rhs_rk<T, TX, TY, true><<<grd, blk, 0, s>>>(one, tab, d, mode, T(c1), T(c2), kc);                \
-        else k_rhs_rk<T, TX, TY, false><<<grd, blk, 0, s>>>(one, tab, d, mode, T(c1), T(c2), kc);                   \
+        if (mode == 0) RHS_MODE(TX, TY, 0);                                                                         \
+        else if (mode == 1) RHS_MODE(TX, TY, 1);                                                                    \
+        else RHS_MODE(TX, TY, 2);                                                                                   \
     } while (0)
     if (sh.tx == 64 && sh.ty == 4) RHS_GO(64, 4);
     else if (sh.tx == 64 && sh.ty == 8) RHS_GO(64, 8);
     else RHS_GO(32, 8);
 #undef RHS_GO
+#undef RHS_MODE
     PROF_END(s);
 }
 

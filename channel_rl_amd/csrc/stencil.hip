@@ -276,13 +276,14 @@ __device__ __forceinline__ void final_sum2(const double* part, int nblk, double&
 template <typename T>
 __global__ void k_massflow_u(const T* U, Metric<double> m, Dims d, double* part, int nblk)
 {
-    const long n1 = d.nx - 2, n2 = d.nyg - 3, n3 = d.nzp;
-    const long tot = n1 * n2 * n3;
+    const int n1 = d.nx - 2, n2 = d.nyg - 3, n3 = d.nzp;
+    const int tot = n1 * n2 * n3;
     double q = 0.0, nrm = 0.0;
-    for (long t = (long)blockIdx.x * RED_THREADS + threadIdx.x; t < tot; t += (long)nblk * RED_THREADS) {
-        const int i = (int)(t % n1) + 2;
-        const int j = (int)((t / n1) % n2) + 3;
-        const int k = (int)(t / (n1 * n2)) + 2;
+    for (int t = blockIdx.x * RED_THREADS + threadIdx.x; t < tot; t += nblk * RED_THREADS) {
+        const int r = t / n1;
+        const int i = t - r * n1 + 2;
+        const int j = r % n2 + 3;
+        const int k = r / n2 + 2;
         const double dy = m.yg[j] - m.yg[j - 1];
         q += ((double)U[at(d, i, j, k)] + (double)U[at(d, i, j - 1, k)]) * 0.5 * dy;
         nrm += (1.0 + 1.0) * 0.5 * dy;
@@ -317,10 +318,11 @@ __global__ void k_dpx_partial(MassArgs<T> one, const MassArgs<T>* tab, Dims d, i
     const GBlk g = gblk(1);
     const MassArgs<T> ar = member_args(one, tab, g.m);
     const T* U = ar.U;
-    const long n1 = d.nx - 2, n3 = d.nzp, tot = n1 * n3;
+    const int n1 = d.nx - 2, n3 = d.nzp, tot = n1 * n3;
     double s = 0.0, c = 0.0;
-    for (long t = (long)g.x * RED_THREADS + threadIdx.x; t < tot; t += (long)nblk * RED_THREADS) {
-        const int i = (int)(t % n1) + 2, k = (int)(t / n1) + 2;
+    for (int t = g.x * RED_THREADS + threadIdx.x; t < tot; t += nblk * RED_THREADS) {
+        const int kq = t / n1;
+        const int i = t - kq * n1 + 2, k = kq + 2;
         s += 0.5 * ((double)U[at(d, i, 3, k)] + (double)U[at(d, i, d.nyg - 2, k)]);
         c += 1.0;
     }
@@ -371,13 +373,14 @@ __global__ void k_bulk_partial(MassArgs<T> one, const MassArgs<T>* tab, Dims d, 
     const MassArgs<T> ar = member_args(one, tab, g.m);
     const T* U = ar.U;
     const double* yg = ar.m.yg;
-    const long n1 = d.nx - 2, n2 = d.nyg - 3, n3 = d.nzp;
-    const long tot = n1 * n2 * n3;
+    const int n1 = d.nx - 2, n2 = d.nyg - 3, n3 = d.nzp;
+    const int tot = n1 * n2 * n3;
     double q = 0.0, nrm = 0.0;
-    for (long t = (long)g.x * RED_THREADS + threadIdx.x; t < tot; t += (long)nblk * RED_THREADS) {
-        const int i = (int)(t % n1) + 2;
-        const int j = (int)((t / n1) % n2) + 3;
-        const int k = (int)(t / (n1 * n2)) + 2;
+    for (int t = g.x * RED_THREADS + threadIdx.x; t < tot; t += nblk * RED_THREADS) {
+        const int r = t / n1;
+        const int i = t - r * n1 + 2;
+        const int j = r % n2 + 3;
+        const int k = r / n2 + 2;
         const double dy = yg[j] - yg[j - 1];
         q += ((double)U[at(d, i, j, k)] + (double)U[at(d, i, j - 1, k)]) * 0.5 * dy;
         nrm += (1.0 + 1.0) * 0.5 * dy;
@@ -435,9 +438,12 @@ __device__ __forceinline__ void block_max3(double a, double b, double c, double*
     }
 }
 
-// Blocks walk whole x rows (j, k), so the row's y and z spacings are uniform
-// and no per-element index division is needed; the maximum does not depend
-// on the order, so the result is the same as any other traversal.
+// The unique cells of the reference's loop (i, j, k in 2..nxg-1, 2..nyg-1,
+// 2..nzg-1), flattened with x fastest, are split into nblk equal runs, one per
+// block; every lane takes a cell (rows of a few dozen cells would leave most
+// of a 256-lane block idle), and each lane's next four cells are loaded
+// before any is used.  The maximum does not depend on the order, so the
+// result is the same as any other traversal.
 template <typename T>
 __global__ void k_dt_partial(DtArgs<T> one, const DtArgs<T>* tab, Dims d, int nblk)
 {
@@ -445,16 +451,34 @@ __global__ void k_dt_partial(DtArgs<T> one, const DtArgs<T>* tab, Dims d, int nb
     const DtArgs<T> a = member_args(one, tab, g.m);
     const T *U = a.U, *V = a.V, *W = a.W;
     const Metric<double>& m = a.m;
-    const int n2 = d.nyg - 2, rows = n2 * (d.nzg - 2);
+    const int n1 = d.nxg - 2, n2 = d.nyg - 2;
+    const int n = n1 * n2 * (d.nzg - 2);
+    const int per = (n + nblk - 1) / nblk;
+    const int q0 = g.x * per, q1 = min(q0 + per, n);
     double lu = 0.0, lv = 0.0, lw = 0.0;
-    for (int r = g.x; r < rows; r += nblk) {
-        const int j = r % n2 + 2, k = r / n2 + 2;
-        const double dyj = m.yg[j + 1] - m.yg[j], dzk = m.zg[k + 1] - m.zg[k];
-        const long base = at(d, 1, j, k);
-        for (int i = threadIdx.x + 2; i <= d.nxg - 1; i += RED_THREADS) {
-            lu = fmax(lu, (m.xg[i + 1] - m.xg[i]) / fabs((double)U[base + i - 1]));
-            lv = fmax(lv, dyj / fabs((double)V[base + i - 1]));
-            lw = fmax(lw, dzk / fabs((double)W[base + i - 1]));
+    constexpr int E = 4;
+    for (int qb = q0 + (int)threadIdx.x; qb < q1; qb += E * RED_THREADS) {
+        T u[E], v[E], w[E];
+        int ii[E], jj[E], kk[E];
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int q = min(qb + e * RED_THREADS, q1 - 1);
+            const int r = q / n1;
+            ii[e] = q - r * n1 + 2;
+            jj[e] = r % n2 + 2;
+            kk[e] = r / n2 + 2;
+            const long o = at(d, ii[e], jj[e], kk[e]);
+            u[e] = U[o];
+            v[e] = V[o];
+            w[e] = W[o];
+        }
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            // cells past the run repeat its last cell, which changes no maximum
+            const int i = ii[e], j = jj[e], k = kk[e];
+            lu = fmax(lu, (m.xg[i + 1] - m.xg[i]) / fabs((double)u[e]));
+            lv = fmax(lv, (m.yg[j + 1] - m.yg[j]) / fabs((double)v[e]));
+            lw = fmax(lw, (m.zg[k + 1] - m.zg[k]) / fabs((double)w[e]));
         }
     }
     block_max3(lu, lv, lw, a.part, nblk, g.x);
@@ -518,12 +542,13 @@ void launch_dt(const T* U, const T* V, const T* W, const Metric<double>& md, con
 template <typename T>
 __global__ void k_div_partial(const T* U, const T* V, const T* W, Metric<double> m, Dims d, double* part, int nblk)
 {
-    const long n1 = d.nxg - 2, n2 = d.nyg - 2, n3 = d.nzg - 2, tot = n1 * n2 * n3;
+    const int n1 = d.nxg - 2, n2 = d.nyg - 2, n3 = d.nzg - 2, tot = n1 * n2 * n3;
     double mx = 0.0, ma = 0.0;
-    for (long t = (long)blockIdx.x * RED_THREADS + threadIdx.x; t < tot; t += (long)nblk * RED_THREADS) {
-        const int i = (int)(t % n1) + 2;
-        const int j = (int)((t / n1) % n2) + 2;
-        const int k = (int)(t / (n1 * n2)) + 2;
+    for (int t = blockIdx.x * RED_THREADS + threadIdx.x; t < tot; t += nblk * RED_THREADS) {
+        const int r = t / n1;
+        const int i = t - r * n1 + 2;
+        const int j = r % n2 + 2;
+        const int k = r / n2 + 2;
         double v = ((double)U[at(d, i, j, k)] - (double)U[at(d, i - 1, j, k)]) / (m.x[i] - m.x[i - 1]) +
                    ((double)V[at(d, i, j, k)] - (double)V[at(d, i, j - 1, k)]) / (m.y[j] - m.y[j - 1]) +
                    ((double)W[at(d, i, j, k)] - (double)W[at(d, i, j, k - 1)]) / (m.z[k] - m.z[k - 1]);

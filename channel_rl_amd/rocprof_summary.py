@@ -25,6 +25,14 @@ are what `load_traffic` reads back.
 trace, its scaled HBM bytes per launch, the rate those bytes move at and
 that rate as a fraction of the 8 TB/s HBM peak.
 
+    python -m channel_rl_amd.rocprof_summary grids DIR/run_kernel_trace.csv [KERNEL_PREFIX]
+
+`grids` groups the dispatches of a kernel trace by kernel and grid size (a
+grouped ensemble launch stacks its members' grids along z, so each group of
+channels shows as its own grid): launches, average and total duration; and,
+over the whole trace, the sum of the kernels' durations against the union of
+their intervals (above 1 where launches overlapped).
+
 `sq` summarises the SQ counter passes listed in SQ_PASSES (medians per
 dispatch; each counter is a whole-device sum per dispatch).  Per wave:
 VALU and LDS instructions; as fractions of the waves' lifetime
@@ -169,6 +177,38 @@ def sq_table(paths):
     return "\n".join(out)
 
 
+def read_trace(path):
+    """[(short name, (grid x, y, z), start ns, end ns)] of a rocprofv3 kernel trace."""
+    rows = []
+    for r in csv.DictReader(open(path)):
+        g = tuple(int(r.get(f"Grid_Size_{a}", r.get(f"Grid_Size{a}", 0)) or 0) for a in "XYZ")
+        rows.append((short_name(r["Kernel_Name"]), g, int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+    return rows
+
+
+def grids_table(path, prefix=""):
+    rows = [r for r in read_trace(path) if base_name(r[0]).startswith(prefix) or not prefix]
+    groups = collections.defaultdict(list)
+    for n, g, t0, t1 in rows:
+        groups[(n, g)].append(t1 - t0)
+    busy = sum(t1 - t0 for _, _, t0, t1 in rows)
+    union, end = 0, None
+    for _, _, t0, t1 in sorted(rows, key=lambda r: r[2]):
+        if end is None or t0 > end:
+            union += t1 - t0
+            end = t1
+        elif t1 > end:
+            union += t1 - end
+            end = t1
+    out = [f"kernel time {busy / 1e6:.3f} ms over {union / 1e6:.3f} ms of intervals "
+           f"(overlap factor {busy / max(union, 1):.2f})", "",
+           "| kernel | grid (threads) | launches | average us | total ms |", "|---|---|---:|---:|---:|"]
+    for (n, g), d in sorted(groups.items(), key=lambda kv: -sum(kv[1])):
+        out.append(f"| {n} | {g[0]} x {g[1]} x {g[2]} | {len(d)} | {statistics.mean(d) / 1e3:.2f} | "
+                   f"{sum(d) / 1e6:.3f} |")
+    return "\n".join(out)
+
+
 def load_traffic(path):
     """Scaled HBM bytes per launch by kernel base name from a traffic table;
     of several instances of a kernel (tile shapes timed by the autotuner),
@@ -196,6 +236,8 @@ if __name__ == "__main__":
         print(traffic_table(sys.argv[2], sys.argv[3]))
     elif len(sys.argv) >= 5 and sys.argv[1] == "stages":
         print(stages_table(sys.argv[2], sys.argv[3], sys.argv[4]))
+    elif len(sys.argv) >= 3 and sys.argv[1] == "grids":
+        print(grids_table(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else ""))
     elif len(sys.argv) >= 3 and sys.argv[1] == "sq":
         print(sq_table(sys.argv[2:]))
     else:
