@@ -81,6 +81,9 @@ def parse():
                     help="steps of the batched ensemble (its wall-stress error is read after them)")
     ap.add_argument("--ensemble-timed", type=int, default=100,
                     help="steps timed for each form of the ensemble")
+    ap.add_argument("--tile-shapes", default="",
+                    help="RHS,DSM tile shape indices to use instead of the autotuner's picks (counter runs that "
+                         "must launch the same instances as a traced run)")
     ap.add_argument("--profile-compare", type=int, default=20000,
                     help="steps of the FP32-bulk vs FP64 mean-profile comparison (0: skip); the second "
                          "half is averaged")
@@ -321,11 +324,19 @@ def main():
     ch.set_scalars(dt=0.0, t=0.0, dPdx=case.dPdx, Qflow_x_0=q0)
     ch.set_wall_model(cr.WALL_EQUILIBRIUM, h_wm=case.h_wm)
     cells = case.cells
-    tune = ch.autotune(reps=5)
-    shapes = {"rhs": ch.tile_shape(cr.KERNEL_RHS), "dsm": ch.tile_shape(cr.KERNEL_DSM),
-              "rhs_us": [round(1e3 * t, 1) for t in tune["rhs"]], "dsm_us": [round(1e3 * t, 1) for t in tune["dsm"]]}
-    log(f"[bench] autotuned tile shapes: RHS {shapes['rhs']} ({' '.join(map(str, shapes['rhs_us']))} us), "
-        f"DSM {shapes['dsm']} ({' '.join(map(str, shapes['dsm_us']))} us)")
+    if args.tile_shapes:
+        rs, ds = (int(v) for v in args.tile_shapes.split(","))
+        ch.set_tile_shape(rs, cr.KERNEL_RHS)
+        if case.LES_model == 2:
+            ch.set_tile_shape(ds, cr.KERNEL_DSM)
+        shapes = {"rhs": ch.tile_shape(cr.KERNEL_RHS), "dsm": ch.tile_shape(cr.KERNEL_DSM), "set": "--tile-shapes"}
+        log(f"[bench] tile shapes set: RHS {shapes['rhs']}, DSM {shapes['dsm']}")
+    else:
+        tune = ch.autotune(reps=5)
+        shapes = {"rhs": ch.tile_shape(cr.KERNEL_RHS), "dsm": ch.tile_shape(cr.KERNEL_DSM),
+                  "rhs_us": [round(1e3 * t, 1) for t in tune["rhs"]], "dsm_us": [round(1e3 * t, 1) for t in tune["dsm"]]}
+        log(f"[bench] autotuned tile shapes: RHS {shapes['rhs']} ({' '.join(map(str, shapes['rhs_us']))} us), "
+            f"DSM {shapes['dsm']} ({' '.join(map(str, shapes['dsm_us']))} us)")
     log(f"[bench] {case.name} faces {case.nx} {case.ny} {case.nz}, {cells} cells, {args.precision}")
 
     ch.run_steps(args.warmup)
