@@ -74,6 +74,28 @@ __device__ __forceinline__ T strain_offdiag(const FA& A, const FB& B, T ra0, T r
     return T(0.125) * ((rb0 * sA0 + rb1 * sA1) + (ra0 * sB0 + ra1 * sB1));
 }
 
+// Square root of a sum of squares x >= 0: the hardware reciprocal square
+// root refined by the Newton-Raphson (Goldschmidt) steps of the library
+// sequence, without its rescaling of tiny and huge arguments (a strain
+// magnitude squared stays far inside the normal range; below 2^-960, where
+// the hardware estimate would overflow, the result is 0, as for x = 0).
+// Within an ulp of the correctly rounded root, in 13 instructions instead
+// of the library's 17.
+__device__ __forceinline__ double sqrt_sum_sq(double x)
+{
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r, h = 0.5 * r;
+    const double e = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, e, g);
+    h = __builtin_fma(h, e, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    return x > 0x1p-960 ? g : 0.0;
+}
+__device__ __forceinline__ float sqrt_sum_sq(float x) { return sqrtf(x); }
+
 // The register budget is held to two waves per SIMD (the FP64 kernels need
 // it; the FP32 ones fit in fewer registers and run three or four).
 template <typename T, int TXO, int TYO, bool G>
@@ -110,7 +132,6 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     auto wx = [&](int q) { return q < 2 ? q + nxp : (q > nxp + 1 ? q - nxp : q); };
     auto wz = [&](int q) { return q < 2 ? q + nzp : (q > nzp + 1 ? q - nzp : q); };
     const T h = T(0.5);
-    const T s6 = T(1.0 / 6.0), s23 = T(2.0 / 3.0), s13 = T(1.0 / 3.0);
     // x, y and z reciprocal spacings of the block's columns, rows and
     // planes, in LDS: read per plane from here rather than from global
     // memory, whose loads would wait behind the velocity prefetch (memory
@@ -202,18 +223,18 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     // column also at the halo position next to them (lx = 0, LX-1; the other
     // lanes repeat their own).  The y taps of a plane are taken once and
     // enter the z taps of three planes through two running sums (ga: the next
-    // plane's, two terms in; gb: the one after, one term in).  The y weights
-    // depend on the column's row only: zero outside the field's rows.
-    T wy[2][3];  // [U and W, V][below, centre, above]
-    {
-        const int jj = cj;
-        for (int t = 0; t < 2; t++) {
-            const int n2 = t == 0 ? d.nyg : d.ny;
-            const bool in = jj >= 1 && jj <= n2;
-            wy[t][0] = !in || jj == 1 ? T(0) : (jj == n2 ? s13 : s6);
-            wy[t][1] = in ? s23 : T(0);
-            wy[t][2] = !in || jj == n2 ? T(0) : (jj == 1 ? s13 : s6);
-        }
+    // plane's, two terms in; gb: the one after, one term in).  Every
+    // direction carries the weights (1, 4, 1), the factor 1/6 of each taken
+    // once at the end (1/216).  The one-sided rows (the first and last row
+    // of each field, weights (0, 4, 2) and (2, 4, 0)) read their inner
+    // neighbour for both taps: the y weights become a choice of rows, fixed
+    // per column.  Rows outside a field (the walls' halo rows, V's row nyg)
+    // produce values that no output reads.
+    int ym[2], yp[2];  // [U and W, V]: LDS offsets of the rows below and above
+    for (int t = 0; t < 2; t++) {
+        const int n2 = t == 0 ? d.nyg : d.ny;
+        ym[t] = cj == 1 ? LX : -LX;
+        yp[t] = cj == n2 ? -LX : LX;
     }
     const int qown = (cx + 1) + LX * (cy + 1);
     const int qext = (cx == 0 ? 0 : (cx == CX - 1 ? LX - 1 : cx + 1)) + LX * (cy + 1);
@@ -222,9 +243,9 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
         const int s = slot(p);
         for (int f = 0; f < 3; f++) {
             const T* P = l0[s][f];
-            const T* w = wy[f == 1 ? 1 : 0];
-            out[0][f] = w[0] * P[qown - LX] + w[1] * P[qown] + w[2] * P[qown + LX];
-            out[1][f] = w[0] * P[qext - LX] + w[1] * P[qext] + w[2] * P[qext + LX];
+            const int m = ym[f == 1 ? 1 : 0], q = yp[f == 1 ? 1 : 0];
+            out[0][f] = (P[qown + m] + P[qown + q]) + T(4) * P[qown];
+            out[1][f] = (P[qext + m] + P[qext + q]) + T(4) * P[qext];
         }
     };
     // filtered staggered velocities of the column at plane p into LDS: the
@@ -239,18 +260,18 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
         ycomb(p + 1, yc);
         const int sf = slot(p);
         for (int f = 0; f < 3; f++) {
-            const T own = ga[0][f] + s6 * yc[0][f];
-            const T ext = ga[1][f] + s6 * yc[1][f];
+            const T own = ga[0][f] + yc[0][f];
+            const T ext = ga[1][f] + yc[1][f];
             for (int e = 0; e < 2; e++) {
-                gb[e][f] = gb[e][f] + s23 * yc[e][f];
-                ga[e][f] = s6 * yc[e][f];
+                gb[e][f] = gb[e][f] + T(4) * yc[e][f];
+                ga[e][f] = yc[e][f];
             }
             // the moves run on every lane before the selection: inside a
             // conditional they would read disabled lanes as 0
             const T sl = lane_shift<DPP_WAVE_SHR1>(own), sr = lane_shift<DPP_WAVE_SHL1>(own);
             const T left = cx == 0 ? ext : sl;
             const T right = cx == CX - 1 ? ext : sr;
-            lf[sf][f][tid] = s6 * left + s23 * own + s6 * right;
+            lf[sf][f][tid] = ((left + right) + T(4) * own) * T(1.0 / 216.0);
         }
     };
 
@@ -290,7 +311,7 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
             acc = acc + T(4) * s[3] * s[3];
             acc = acc + T(4) * s[4] * s[4];
             acc = acc + T(4) * s[5] * s[5];
-            sm = sqrt(acc);
+            sm = sqrt_sum_sq(acc);
             for (int c = 0; c < 6; c++) ss[c] = sm * s[c];
         }
         for (int c = 0; c < 3; c++) Q[c] = a[c];
@@ -314,12 +335,12 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
         T yc[2][3];
         ycomb(kb - 2, yc);
         for (int e = 0; e < 2; e++)
-            for (int f = 0; f < 3; f++) gA[e][f] = s6 * yc[e][f];
+            for (int f = 0; f < 3; f++) gA[e][f] = yc[e][f];
         ycomb(kb - 1, yc);
         for (int e = 0; e < 2; e++)
             for (int f = 0; f < 3; f++) {
-                gA[e][f] = gA[e][f] + s23 * yc[e][f];
-                gB[e][f] = s6 * yc[e][f];
+                gA[e][f] = gA[e][f] + T(4) * yc[e][f];
+                gB[e][f] = yc[e][f];
             }
     }
     filt_vel(kb - 1, gA, gB);
@@ -344,11 +365,10 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
 
     double acc_lm = 0.0, acc_mm = 0.0;
     // y taps of the filter inputs for this column's row (one-sided at rows 2
-    // and nyg-1), taken on the output rows 1..TYO only
-    // six times the y weights (the one-sided rows' 1/3, 2/3 become 2, 4)
-    const T wym = (cj == 2) ? T(0) : (cj == d.nyg - 1 ? T(2) : T(1));
-    const T wyp = (cj == d.nyg - 1) ? T(0) : (cj == 2 ? T(2) : T(1));
-    const int om = cx + CX * max(cy - 1, 0), op = cx + CX * min(cy + 1, Tl::CY - 1);
+    // and nyg-1, whose weights (0, 4, 2) and (2, 4, 0) read the inner
+    // neighbour for both taps), taken on the output rows 1..TYO only
+    const int om0 = cx + CX * max(cy - 1, 0), op0 = cx + CX * min(cy + 1, Tl::CY - 1);
+    const int om = cj == 2 ? op0 : om0, op = cj == d.nyg - 1 ? om0 : op0;
 
     // one output plane k; za/zb and ga/gb as described above (roles swap)
     auto plane = [&](int k, T (&za)[15], T (&zb)[15], T (&ga)[2][3], T (&gb)[2][3]) {
@@ -367,7 +387,7 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
         T tp[15];
         // on the halo rows (cy 0 and CY-1, clamped indices) the taps are
         // taken as well and never used: no branch
-        for (int c = 0; c < 15; c++) tp[c] = wym * zq[c][om] + T(4) * zown[c] + wyp * zq[c][op];
+        for (int c = 0; c < 15; c++) tp[c] = (zq[c][om] + zq[c][op]) + T(4) * zown[c];
         for (int c = 0; c < 15; c++) {
             const T y = tp[c];
             tp[c] = (lane_shift<DPP_WAVE_SHR1>(y) + lane_shift<DPP_WAVE_SHL1>(y)) + T(4) * y;
@@ -394,7 +414,7 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
             acc = acc + T(4) * sh[3] * sh[3];
             acc = acc + T(4) * sh[4] * sh[4];
             acc = acc + T(4) * sh[5] * sh[5];
-            const T shm4 = T(4) * sqrt(acc);
+            const T shm4 = T(4) * sqrt_sum_sq(acc);
             // tp holds 216 times the filtered inputs: L = s (T_ab - s T_a T_b)
             // and M = 4|hat S| hat S_ab - s T(|S| S_ab), s = 1/216, the outer
             // s of L taken once on L:M
