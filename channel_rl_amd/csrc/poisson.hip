@@ -423,7 +423,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_fftz(PoissonBufs one, const Poi
 // form the two-ended kernel replaced; it is not instantiated.)
 static constexpr int TRI_THREADS = 64;
 static constexpr int TRI_GROUP = 8;
-static constexpr int TRI_KEEP = 96;
+static constexpr int TRI_KEEP = 192;
 
 __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_setup(Dims d, PoissonBufs pb)
 {
@@ -466,8 +466,11 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_setup(Dims d, PoissonBu
 // values by selection and re-store the half's last row with its own value.
 // (An empty asm on each new value keeps the compiler from moving its
 // computation into a branch of the selection.)
+// Register budget: two waves per SIMD (the 96-row half of a 192-row system
+// holds 192 VGPRs of rows; left free, the compiler takes 450 and one wave).
 template <int H, bool G>
-__global__ void __launch_bounds__(TRI_THREADS) k_tridiag_tw(PoissonBufs one, const PoissonBufs* tab, Dims d)
+__global__ void __launch_bounds__(TRI_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
+    k_tridiag_tw(PoissonBufs one, const PoissonBufs* tab, Dims d)
 {
     const GBlk gb = gblk(1);
     const PoissonBufs pb = member_args<G>(one, tab, gb.m);
@@ -485,7 +488,11 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_tw(PoissonBufs one, con
     // bottom half (slot H+u): row n-1-u, (A, C) = (u_r, l_r); an identity
     // row past a half's end
     __shared__ double sD[2 * H], sA[2 * H], sC[2 * H];
-    __shared__ double piv[H][TRI_THREADS];
+    // 1/den per (mode, half): the real and imaginary lanes of a half (lanes t
+    // and t^2) compute the same chain and share a slot (each writes the same
+    // value, each reads back what the pair wrote)
+    __shared__ double piv[H][TRI_THREADS / 2];
+    const int ps = ((threadIdx.x >> 2) << 1) | (threadIdx.x & 1);
     for (int q = threadIdx.x; q < 2 * H; q += TRI_THREADS) {
         const bool b = q >= H;
         const int u = b ? q - H : q;
@@ -511,10 +518,16 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_tw(PoissonBufs one, con
     double keep[H];
     {
         const double* pr = p0;
+        asm("" : "+v"(pr));
+        int hq = hl;
+        // each loop below compares the steps against its own opaque copy of
+        // hl: left free, the compiler keeps the H comparisons of the first
+        // loop as scalar masks for all four (2H scalar registers, spilled)
+        asm("" : "+v"(hq));
 #pragma unroll
         for (int u = 0; u < H; u++) {
             keep[u] = *pr;
-            pr = u + 1 < hl ? pr + step : pr;
+            pr = u + 1 < hq ? pr + step : pr;
         }
     }
     // keeps the compiler from sinking the loads below the chain
@@ -522,23 +535,27 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_tw(PoissonBufs one, con
     // pivot chain; row 0's diagonal of the 00 mode scaled by 3/2
     double c = 0.0;
     const bool scale0 = !bot && kx == 0 && kz == 0;
+    int hc = hl;
+    asm("" : "+v"(hc));
 #pragma unroll
     for (int u = 0; u < H; u++) {
         double den = ((D[u] + lx) + lz) - A[u] * c;
         if (u == 0) den = scale0 ? 3.0 / 2.0 * den : den;
         const double inv = 1.0 / den;
-        piv[u][threadIdx.x] = inv;
+        piv[u][ps] = inv;
         double cn = C[u] * inv;
         asm("" : "+v"(cn));
-        c = u < hl ? cn : c;
+        c = u < hc ? cn : c;
     }
     asm volatile("" ::: "memory");
     double y = 0.0;
+    int hf = hl;
+    asm("" : "+v"(hf));
 #pragma unroll
     for (int u = 0; u < H; u++) {
-        double yn = (keep[u] - A[u] * y) * piv[u][threadIdx.x];
+        double yn = (keep[u] - A[u] * y) * piv[u][ps];
         asm("" : "+v"(yn));
-        y = u < hl ? yn : y;
+        y = u < hf ? yn : y;
         keep[u] = y;
     }
     // the two middle rows from the pair's last values (lanes g and g^1)
@@ -551,11 +568,16 @@ __global__ void __launch_bounds__(TRI_THREADS) k_tridiag_tw(PoissonBufs one, con
     // back towards the end of the half: x = y - c' x(previous); pw moves
     // with the valid steps only, from row hl-2 of the half
     double* pw = p0 + (long)max(hl - 2, 0) * step;
+    // a fresh address chain for the stores (reusing the loads' addresses
+    // would keep 2H address registers live through the solve)
+    asm("" : "+v"(pw), "+v"(plast));
+    int hb = hl - 1;
+    asm("" : "+v"(hb));
 #pragma unroll
     for (int u = H - 2; u >= 0; u--) {
-        double xu = keep[u] - C[u] * piv[u][threadIdx.x] * xn;
+        double xu = keep[u] - C[u] * piv[u][ps] * xn;
         asm("" : "+v"(xu));
-        const bool in = u < hl - 1;
+        const bool in = u < hb;
         xn = in ? xu : xn;
         *(in ? pw : plast) = xn;
         if (u > 0) pw = in ? pw - step : pw;
@@ -840,6 +862,12 @@ void run_poisson_solve(const PoissonBufs& one, const PoissonBufs* tab, int nmem,
     if (n > TRI_KEEP) {
         if (tab) k_tridiag<true, 32, true><<<g1, TRI_THREADS, 0, s>>>(one, tab, d);
         else k_tridiag<true, 32, false><<<g1, TRI_THREADS, 0, s>>>(one, nullptr, d);
+    } else if (n > 128) {
+        if (tab) k_tridiag_tw<96, true><<<g2, TRI_THREADS, 0, s>>>(one, tab, d);
+        else k_tridiag_tw<96, false><<<g2, TRI_THREADS, 0, s>>>(one, nullptr, d);
+    } else if (n > 96) {
+        if (tab) k_tridiag_tw<64, true><<<g2, TRI_THREADS, 0, s>>>(one, tab, d);
+        else k_tridiag_tw<64, false><<<g2, TRI_THREADS, 0, s>>>(one, nullptr, d);
     } else if (n > 64) {
         if (tab) k_tridiag_tw<48, true><<<g2, TRI_THREADS, 0, s>>>(one, tab, d);
         else k_tridiag_tw<48, false><<<g2, TRI_THREADS, 0, s>>>(one, nullptr, d);

@@ -102,12 +102,13 @@ def test_equilibrium_wall_stress(gpu, n):
 
 
 @pytest.mark.parametrize("n", SIZES + [(10, 4, 10), (10, 5, 10), (10, 50, 12), (10, 80, 10), (12, 98, 10),
-                                       (10, 130, 12)])
+                                       (10, 130, 12), (10, 193, 10), (10, 194, 10)])
 def test_projection_matches_oracle(gpu, n):
-    """The y solve of ny - 1 rows: up to 96 two-ended in registers (3 and 4
+    """The y solve of ny - 1 rows: up to 192 two-ended in registers (3 and 4
     rows, the smallest systems; 49 and 79, odd, where the bottom half is a
-    row shorter; the small sizes), more streamed through memory before the
-    last 32 (97 rows: 65 streamed; 129 rows: 97 streamed, in groups of 8)."""
+    row shorter; 97 and 129, the 64- and 96-row halves, partly filled; 192,
+    the 96-row halves full; the small sizes), more streamed through memory
+    before the last 32 (193 rows: 161 streamed, in groups of 8)."""
     case, field = cases.small_case(*n, seed=11)
     o = oracle_for(case)
     o.set_state(*field)
@@ -324,7 +325,7 @@ def test_check_divergence_matches_oracle(gpu, n):
     assert md == 0.0 and abs(ma - 1.0) <= 1e-13
 
 
-@pytest.mark.parametrize("n", SIZES + [(12, 98, 10), (10, 130, 12)])
+@pytest.mark.parametrize("n", SIZES + [(12, 98, 10), (10, 130, 12), (10, 193, 10), (10, 194, 10)])
 def test_solve_poisson_equation_matches_oracle(gpu, n):
     """code/projection.f90:74-128 on a random right-hand side (the 00 mode
     incompatible, as the solve does not require compatibility: its D(2)*1.5
