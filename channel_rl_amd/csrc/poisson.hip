@@ -488,11 +488,15 @@ __global__ void __launch_bounds__(TRI_THREADS) __attribute__((amdgpu_waves_per_e
     // bottom half (slot H+u): row n-1-u, (A, C) = (u_r, l_r); an identity
     // row past a half's end
     __shared__ double sD[2 * H], sA[2 * H], sC[2 * H];
-    // 1/den per (mode, half): the real and imaginary lanes of a half (lanes t
-    // and t^2) compute the same chain and share a slot (each writes the same
-    // value, each reads back what the pair wrote)
-    __shared__ double piv[H][TRI_THREADS / 2];
-    const int ps = ((threadIdx.x >> 2) << 1) | (threadIdx.x & 1);
+    // 1/den per lane; for the 64- and 96-row halves per (mode, half) instead:
+    // the real and imaginary lanes of a half (lanes t and t^2) compute the
+    // same chain and share a slot (each writes the same value, each reads
+    // back what the pair wrote), which halves the LDS that limits those
+    // forms' occupancy (for the shorter ones the shared slots measured
+    // slower: 19.0 -> 21.3 us at 97 rows)
+    constexpr bool SHARE = H >= 64;
+    __shared__ double piv[H][SHARE ? TRI_THREADS / 2 : TRI_THREADS];
+    const int ps = SHARE ? (((threadIdx.x >> 2) << 1) | (threadIdx.x & 1)) : threadIdx.x;
     for (int q = threadIdx.x; q < 2 * H; q += TRI_THREADS) {
         const bool b = q >= H;
         const int u = b ? q - H : q;
@@ -518,12 +522,13 @@ __global__ void __launch_bounds__(TRI_THREADS) __attribute__((amdgpu_waves_per_e
     double keep[H];
     {
         const double* pr = p0;
-        asm("" : "+v"(pr));
         int hq = hl;
-        // each loop below compares the steps against its own opaque copy of
-        // hl: left free, the compiler keeps the H comparisons of the first
-        // loop as scalar masks for all four (2H scalar registers, spilled)
-        asm("" : "+v"(hq));
+        // for the long halves each loop below compares the steps against its
+        // own opaque copy of hl and the stores take a fresh address chain:
+        // left free, the compiler keeps the H comparisons of the first loop
+        // as scalar masks for all four and the loads' addresses for the
+        // stores (2H registers each), which spills the 96-row form
+        if constexpr (SHARE) asm("" : "+v"(pr), "+v"(hq));
 #pragma unroll
         for (int u = 0; u < H; u++) {
             keep[u] = *pr;
@@ -536,7 +541,7 @@ __global__ void __launch_bounds__(TRI_THREADS) __attribute__((amdgpu_waves_per_e
     double c = 0.0;
     const bool scale0 = !bot && kx == 0 && kz == 0;
     int hc = hl;
-    asm("" : "+v"(hc));
+    if constexpr (SHARE) asm("" : "+v"(hc));
 #pragma unroll
     for (int u = 0; u < H; u++) {
         double den = ((D[u] + lx) + lz) - A[u] * c;
@@ -550,7 +555,7 @@ __global__ void __launch_bounds__(TRI_THREADS) __attribute__((amdgpu_waves_per_e
     asm volatile("" ::: "memory");
     double y = 0.0;
     int hf = hl;
-    asm("" : "+v"(hf));
+    if constexpr (SHARE) asm("" : "+v"(hf));
 #pragma unroll
     for (int u = 0; u < H; u++) {
         double yn = (keep[u] - A[u] * y) * piv[u][ps];
@@ -568,11 +573,8 @@ __global__ void __launch_bounds__(TRI_THREADS) __attribute__((amdgpu_waves_per_e
     // back towards the end of the half: x = y - c' x(previous); pw moves
     // with the valid steps only, from row hl-2 of the half
     double* pw = p0 + (long)max(hl - 2, 0) * step;
-    // a fresh address chain for the stores (reusing the loads' addresses
-    // would keep 2H address registers live through the solve)
-    asm("" : "+v"(pw), "+v"(plast));
     int hb = hl - 1;
-    asm("" : "+v"(hb));
+    if constexpr (SHARE) asm("" : "+v"(pw), "+v"(plast), "+v"(hb));
 #pragma unroll
     for (int u = H - 2; u >= 0; u--) {
         double xu = keep[u] - C[u] * piv[u][ps] * xn;
