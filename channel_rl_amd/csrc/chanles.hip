@@ -116,6 +116,10 @@ struct chan_ctx {
     double* mins3 = nullptr;
     PoissonBufs pb{};
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+    // the same steps with compute_dt folded into substep 1 (dt_fused)
+    hipGraphExec_t graph_dt[3] = {nullptr, nullptr, nullptr};
+    double* dtpart = nullptr;  // the DSM pass's compute_dt maxima, [3][blocks]
+    int nblk_dt = 0;
     bool use_graph = true;
     int rhs_shape = 0;  // index into the RHS kernel's tile shapes
     // learned wall model
@@ -267,6 +271,10 @@ DsmBufs<T> dsm_bufs(chan_ctx* c)
     const int pi = sizeof(T) == sizeof(double) ? 0 : 1;
     b.rdx = (const T*)c->rcp[pi][0]; b.rdy = (const T*)c->rcp[pi][1]; b.rdz = (const T*)c->rcp[pi][2];
     b.rdxg = (const T*)c->rcp[pi][3]; b.rdyg = (const T*)c->rcp[pi][4]; b.rdzg = (const T*)c->rcp[pi][5];
+    b.dtpart = nullptr;  // set by the step that folds compute_dt into the pass
+    b.xg = c->md.xg; b.yg = c->md.yg; b.zg = c->md.zg;
+    b.mins3 = c->mins3;
+    b.nblk_dt = c->nblk_dt;
     return b;
 }
 
@@ -301,12 +309,13 @@ void enqueue_bc(chan_ctx* c, int buf)
 }
 
 template <typename T>
-void enqueue_eddy_viscosity(chan_ctx* c, int buf, int fuse_wm)
+void enqueue_eddy_viscosity(chan_ctx* c, int buf, int fuse_wm, bool with_dt = false)
 {
     T** s = (T**)c->state[buf];
     const T nu = T(c->nu);
     if (c->LES_model == 2) {
         DsmBufs<T> b = dsm_bufs<T>(c);
+        if (with_dt) b.dtpart = c->dtpart;
         launch_dsm<T>(s[0], s[1], s[2], (T*)c->nut, metric<T>(c), c->d, b, c->Dirichlet_nu_t == 1, fuse_wm,
                       c->alpha, nu, c->sc, c->stream);
     } else if (c->LES_model == 3) {
@@ -330,8 +339,10 @@ void enqueue_projection(chan_ctx* c, int buf)
 
 // code/time_integration.f90:24-96 with three rotating state buffers:
 // X (start of step, also Uo), Y, Z; substeps X->Y, Y->Z, Z->Y.
+// dt_fused: compute_dt of the step's start state is taken by the DSM pass
+// of substep 1 and its ratio kernel (dt_fusable), which need it first
 template <typename T>
-void enqueue_rk3(chan_ctx* c, int X)
+void enqueue_rk3(chan_ctx* c, int X, bool dt_fused = false)
 {
     const int Y = (X + 1) % 3, Z = (X + 2) % 3;
     const int in[3] = {X, Y, Z}, out[3] = {Y, Z, Y};
@@ -348,7 +359,7 @@ void enqueue_rk3(chan_ctx* c, int X)
                 launch_wm_mlp<T>(net, I[0], I[2], c->alpha, c->sc, c->md, c->d, c->nu, c->h_wm, c->kappa, c->Bc,
                                  c->x_mass_cte, c->wm_max_err, c->wm_stats, c->stream);
         }
-        enqueue_eddy_viscosity<T>(c, in[s], 1);
+        enqueue_eddy_viscosity<T>(c, in[s], 1, dt_fused && s == 0);
         double c1, c2 = 0.0;
         if (s == 0) c1 = c->rk_coef[0][0];
         else if (s == 1) { c1 = c->rk_coef[1][0]; c2 = c->rk_coef[1][1]; }
@@ -370,8 +381,10 @@ void enqueue_rk3(chan_ctx* c, int X)
 // the captured steps bake in launch shapes and options: drop them on change
 void drop_graphs(chan_ctx* c)
 {
-    for (int q = 0; q < 3; q++)
+    for (int q = 0; q < 3; q++) {
         if (c->graph[q]) { (void)hipGraphExecDestroy(c->graph[q]); c->graph[q] = nullptr; }
+        if (c->graph_dt[q]) { (void)hipGraphExecDestroy(c->graph_dt[q]); c->graph_dt[q] = nullptr; }
+    }
     c->generation++;
 }
 
@@ -383,24 +396,25 @@ int check_launch(chan_ctx* c)
 }
 
 template <typename T>
-int rk3_step(chan_ctx* c)
+int rk3_step(chan_ctx* c, bool dt_fused)
 {
     const int X = c->cur;
     if (!c->use_graph) {
-        enqueue_rk3<T>(c, X);
+        enqueue_rk3<T>(c, X, dt_fused);
         int rc = check_launch(c);
         if (rc) return rc;
     } else {
-        if (!c->graph[X]) {
+        hipGraphExec_t& gx = dt_fused ? c->graph_dt[X] : c->graph[X];
+        if (!gx) {
             hipGraph_t g;
             HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            enqueue_rk3<T>(c, X);
+            enqueue_rk3<T>(c, X, dt_fused);
             HIPCHK(c, hipStreamEndCapture(c->stream, &g));
-            hipError_t e = hipGraphInstantiate(&c->graph[X], g, nullptr, nullptr, 0);
+            hipError_t e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             if (e != hipSuccess) return fail(c, CHAN_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
         }
-        HIPCHK(c, hipGraphLaunch(c->graph[X], c->stream));
+        HIPCHK(c, hipGraphLaunch(gx, c->stream));
     }
     c->cur = (X + 1) % 3;
     return CHAN_OK;
@@ -433,11 +447,19 @@ void set_dsm_shape(chan_ctx* c, int q)
     if (c->prec == CHAN_FP64) {
         c->dsm_kchunk = dsm_fused_kchunk<double>(c->d, c->device, q);
         c->nblk_dsm = dsm_fused_nblk<double>(c->d, c->dsm_kchunk, q);
+        c->nblk_dt = dsm_fused_blocks<double>(c->d, c->dsm_kchunk, q);
     } else {
         c->dsm_kchunk = dsm_fused_kchunk<float>(c->d, c->device, q);
         c->nblk_dsm = dsm_fused_nblk<float>(c->d, c->dsm_kchunk, q);
+        c->nblk_dt = dsm_fused_blocks<float>(c->d, c->dsm_kchunk, q);
     }
 }
+
+// compute_dt folded into the step's first DSM pass: the dynamic Smagorinsky
+// model (its pass reads every cell compute_dt reads) and a wall stress that
+// does not read dt before that pass (the learned model's pressure-gradient
+// feature does)
+bool dt_fusable(const chan_ctx* c) { return c->LES_model == 2 && c->wall_mode != CHAN_WALL_MLP; }
 
 // the address of this device's copy of `scratch`, looked up once per device
 // with that device current (the caller has made it current)
@@ -641,11 +663,13 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
     }
     // partial sums: DSM rows use one slot per (x,z) block of the fused
     // kernel; room for the largest count any tile shape gives
-    size_t nblk_max = 0;
+    size_t nblk_max = 0, nblk_dt_max = 0;
     for (int q = dsm_shape_count() - 1; q >= 0; q--) {
         set_dsm_shape(c, q);
         nblk_max = std::max(nblk_max, (size_t)c->nblk_dsm);
+        nblk_dt_max = std::max(nblk_dt_max, (size_t)c->nblk_dt);
     }
+    if (c->LES_model == 2 && (rc = dalloc(c, (void**)&c->dtpart, 3 * nblk_dt_max * sizeof(double)))) return rc;
     // reduction blocks: about eight (y, z) rows each, at most 1024
     c->nblk_red = (int)std::max(64L, std::min(1024L, (long)(d.nyg - 2) * (d.nzg - 2) / 8));
     size_t npart = std::max((size_t)(d.nyg + 1) * 2 * nblk_max, (size_t)3 * c->nblk_red);
@@ -692,8 +716,10 @@ void chan_destroy(chan_ctx* c)
         return;
     }
     (void)hipSetDevice(c->device);
-    for (int q = 0; q < 3; q++)
+    for (int q = 0; q < 3; q++) {
         if (c->graph[q]) (void)hipGraphExecDestroy(c->graph[q]);
+        if (c->graph_dt[q]) (void)hipGraphExecDestroy(c->graph_dt[q]);
+    }
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->net) mlp_detach(c->net, c->stream);
     for (void* p : c->allocs) (void)hipFree(p);
@@ -960,7 +986,7 @@ int chan_compute_time_step_RK3(chan_ctx* c)
     int rc = set_device(c);
     if (rc) return rc;
     if ((rc = net_order(c))) return rc;
-    return DISPATCH(c, rk3_step, c);
+    return DISPATCH(c, rk3_step, c, false);
 }
 
 int chan_check_divergence(chan_ctx* c, double* max_div, double* max_abs)
@@ -1119,7 +1145,15 @@ int chan_run_steps(chan_ctx* c, int nsteps)
     if (!c || nsteps < 0) return CHAN_EINVAL;
     int rc = set_device(c);
     if (rc) return rc;
+    const bool fuse = dt_fusable(c);
     for (int n = 0; n < nsteps; n++) {
+        if (fuse) {
+            // compute_dt + compute_time_step_RK3 with the former inside the
+            // step (the same dt: its maxima are exact in any order)
+            if ((rc = net_order(c))) return rc;
+            if ((rc = DISPATCH(c, rk3_step, c, true))) return rc;
+            continue;
+        }
         DISPATCH(c, enqueue_dt, c);
         if ((rc = check_launch(c))) return rc;
         if ((rc = chan_compute_time_step_RK3(c))) return rc;
@@ -1137,10 +1171,16 @@ int chan_profile_steps(chan_ctx* c, int nsteps)
     const bool g = c->use_graph;
     c->use_graph = false;
     g_prof = &c->prof;
+    const bool fuse = dt_fusable(c);
     for (int n = 0; n < nsteps && rc == CHAN_OK; n++) {
-        DISPATCH(c, enqueue_dt, c);
-        rc = check_launch(c);
-        if (!rc) rc = chan_compute_time_step_RK3(c);
+        if (fuse) {
+            rc = net_order(c);
+            if (!rc) rc = DISPATCH(c, rk3_step, c, true);
+        } else {
+            DISPATCH(c, enqueue_dt, c);
+            rc = check_launch(c);
+            if (!rc) rc = chan_compute_time_step_RK3(c);
+        }
         hipError_t e = hipStreamSynchronize(c->stream);
         if (!rc && e != hipSuccess) rc = fail(c, CHAN_EHIP, std::string("profile sync: ") + hipGetErrorString(e));
         c->prof.collect();

@@ -143,11 +143,13 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     __shared__ T mx[6][CX];  // rdx, rdxg(i), rdxg(i+1) at wx(ci); the same at ci
     __shared__ T my[3][TYO + 2];  // rdy, rdyg(j), rdyg(j+1)
     __shared__ T mz[3][DSM_KMAX + 3];  // rdz(p), rdzg(p), rdzg(p+1)
+    __shared__ double mzd[DSM_KMAX + 3];  // zg(p+1) - zg(p), for compute_dt
     for (int q = tid; q < ke - kb + 3; q += NC) {
         const int pc = wz(kb - 1 + q);
         mz[0][q] = b.rdz[pc];
         mz[1][q] = b.rdzg[pc];
         mz[2][q] = b.rdzg[pc + 1];
+        if (b.dtpart) mzd[q] = b.zg[pc + 1] - b.zg[pc];
     }
     if (tid < CX) {
         const int ic = wx(i0 - 1 + tid), it = min(max(i0 - 1 + tid, 2), d.nxg - 1);
@@ -276,6 +278,13 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
     };
 
     const bool out_col = cx >= 1 && cx <= TXO && cy >= 1 && cy <= TYO && ci <= d.nxg - 1 && cj <= d.nyg - 1;
+    // compute_dt over this column's output cells (with b.dtpart): the
+    // smallest |U| and |V| (x and y spacings are the column's own, and a
+    // correctly rounded quotient falls as its divisor grows, so the largest
+    // dx/|U| is dx over the smallest |U|, exactly), and the largest dz/|W|
+    // plane by plane (the z spacings differ in their last bits)
+    T dt_mu = T(INFINITY), dt_mv = T(INFINITY);
+    double dt_lw = 0.0;
     const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
 
     // level 1 of this column at plane p: the filter inputs Q = a (3),
@@ -319,7 +328,14 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
             Q[3 + c] = a[pa[c]] * a[pb[c]];
             Q[9 + c] = ss[c];
         }
-        if (out_col && p >= kb && p <= ke) b.smag[at(d, ci, cj, p)] = sm;
+        if (out_col && p >= kb && p <= ke) {
+            b.smag[at(d, ci, cj, p)] = sm;
+            if (b.dtpart) {
+                dt_mu = fmin(dt_mu, fabs(L0(0, p, 0, 0)));
+                dt_mv = fmin(dt_mv, fabs(L0(1, p, 0, 0)));
+                dt_lw = fmax(dt_lw, mzd[p - kb + 1] / fabs((double)L0(2, p, 0, 0)));
+            }
+        }
     };
 
     // z taps of the filter inputs as running sums: for plane k (planes k-1
@@ -460,6 +476,27 @@ __global__ void __launch_bounds__((TXO + 2) * (TYO + 2)) __attribute__((amdgpu_w
             b.part[((long)(j - 1) * 2 + 1) * b.nblk + blk] = sm2;
         }
     }
+    if (b.dtpart) {
+        // the block's compute_dt maxima: per lane, then per wave (butterfly),
+        // then over the waves; one slot per block of the member's grid
+        double m3[3] = {0.0, 0.0, dt_lw};
+        if (out_col) {
+            m3[0] = (b.xg[ci + 1] - b.xg[ci]) / (double)dt_mu;
+            m3[1] = (b.yg[cj + 1] - b.yg[cj]) / (double)dt_mv;
+        }
+        for (int m = 32; m > 0; m >>= 1)
+            for (int c = 0; c < 3; c++) m3[c] = fmax(m3[c], __shfl_xor(m3[c], m, 64));
+        __shared__ double wmax[NC / 64][3];
+        if ((tid & 63) == 0)
+            for (int c = 0; c < 3; c++) wmax[tid >> 6][c] = m3[c];
+        __syncthreads();
+        if (tid < 3) {
+            double v = 0.0;
+            for (int w = 0; w < NC / 64; w++) v = fmax(v, wmax[w][tid]);
+            const int nb = gb.gx * gb.gy * gb.gz;
+            b.dtpart[(long)tid * nb + (tile.z * gb.gy + tile.y) * gb.gx + tile.x] = v;
+        }
+    }
 }
 
 // Tile shapes, (x outputs, y outputs): a row of TXO + 2 columns divides the
@@ -555,6 +592,15 @@ void launch_dsm_fused(const T* U, const T* V, const T* W, const Metric<T>& m, co
 
 template int dsm_fused_kchunk<double>(const Dims&, int, int);
 template int dsm_fused_kchunk<float>(const Dims&, int, int);
+template <typename T>
+int dsm_fused_blocks(const Dims& d, int kchunk, int shape)
+{
+    return cdiv(d.nxg - 2, dsm_shape<T>(shape).txo) * cdiv(d.nyg - 2, dsm_shape<T>(shape).tyo) *
+           cdiv(d.nzg - 2, kchunk);
+}
+
+template int dsm_fused_blocks<double>(const Dims&, int, int);
+template int dsm_fused_blocks<float>(const Dims&, int, int);
 template int dsm_fused_nblk<double>(const Dims&, int, int);
 template int dsm_fused_nblk<float>(const Dims&, int, int);
 template void launch_dsm_fused<double>(const double*, const double*, const double*, const Metric<double>&,

@@ -20,6 +20,13 @@ struct DsmBufs {
     int shape;     // tile shape of the fused pass A (dsm_shape_count)
     // reciprocal spacings, 1-based: 1/(x(i)-x(i-1)), 1/(xg(i)-xg(i-1)), ...
     const T *rdx, *rdy, *rdz, *rdxg, *rdyg, *rdzg;
+    // compute_dt's maxima over the pass's output cells, when dtpart is set:
+    // per block of the pass, [3][block] (code/time_integration.f90:113-121),
+    // with the centre coordinates they divide (FP64, 1-based)
+    double* dtpart;
+    const double *xg, *yg, *zg;
+    const double* mins3;  // compute_dt's smallest spacings (its viscous limit)
+    int nblk_dt;          // blocks of the pass (dtpart's row length)
 };
 
 template <typename T>
@@ -57,6 +64,8 @@ template <typename T>
 int dsm_fused_kchunk(const Dims& d, int device, int shape);
 template <typename T>
 int dsm_fused_nblk(const Dims& d, int kchunk, int shape);
+template <typename T>
+int dsm_fused_blocks(const Dims& d, int kchunk, int shape);
 int dsm_shape_count();
 template <typename T>
 void launch_dsm_fused(const T* U, const T* V, const T* W, const Metric<T>& m, const Dims& d, DsmBufs<T>& b,
@@ -87,6 +96,12 @@ void launch_equilibrium_alpha(const T* U, double* alpha_x, const Dims& d, double
 struct DsmRatioArgs {
     const double* part;
     double* ratio;
+    // with dtpart set, one more block reduces the DSM pass's compute_dt
+    // maxima (nblk_dt blocks) and sets sc->dt as k_dt_final does
+    const double* dtpart;
+    int nblk_dt;
+    Scalars* sc;
+    const double* mins3;
 };
 template <typename T>
 struct DsmNutArgs {

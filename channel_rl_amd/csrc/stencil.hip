@@ -79,17 +79,38 @@ void launch_bc(T* F, const Dims& d, int kind, hipStream_t s)
 // --------------------------------------------------------------- SGS: DSM
 // Pass A (filters, L_ij, M_ij, plane partial sums, |S|) is dsm_fused.hip.
 
+__device__ inline double dt_from_maxima(double mu, double mv, double mw, const Scalars* sc, const double* mins3);
+
 // <L:M>/<M:M> per row: one wave per row, the blocks' partials summed in a
 // fixed order (strided per lane, then a butterfly across the wave), so the
-// result does not depend on scheduling; no LDS and no barrier.
+// result does not depend on scheduling; no LDS and no barrier.  With the
+// DSM pass's compute_dt maxima (ar.dtpart), a last block reduces them and
+// sets the step's dt (the maximum does not depend on the order).
 __global__ void __launch_bounds__(64) k_dsm_ratio(DsmRatioArgs one, const DsmRatioArgs* tab, int nblk, int nyg)
 {
     const GBlk g = gblk(1);
     const DsmRatioArgs ar = member_args(one, tab, g.m);
+    const int t = threadIdx.x;
+    if (g.x == nyg - 2) {
+        const double* dp = ar.dtpart;
+        const int nb = ar.nblk_dt;
+        double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+        for (int q = t; q < nb; q += 64) {
+            m0 = fmax(m0, dp[q]);
+            m1 = fmax(m1, dp[nb + q]);
+            m2 = fmax(m2, dp[2L * nb + q]);
+        }
+        for (int m = 32; m > 0; m >>= 1) {
+            m0 = fmax(m0, __shfl_xor(m0, m, 64));
+            m1 = fmax(m1, __shfl_xor(m1, m, 64));
+            m2 = fmax(m2, __shfl_xor(m2, m, 64));
+        }
+        if (t == 0) ar.sc->dt = dt_from_maxima(m0, m1, m2, ar.sc, ar.mins3);
+        return;
+    }
     const double* part = ar.part;
     double* ratio = ar.ratio;
     const int j = g.x + 2;
-    const int t = threadIdx.x;
     double a = 0.0, c = 0.0;
     for (int q = t; q < nblk; q += 64) {
         a += part[((long)(j - 1) * 2 + 0) * nblk + q];
@@ -140,7 +161,7 @@ void run_dsm_ratio(const DsmRatioArgs& one, const DsmRatioArgs* tab, int nmem, c
                    hipStream_t s)
 {
     PROF_BEGIN("k_dsm_ratio", s);
-    k_dsm_ratio<<<dim3(d.nyg - 2, 1, nmem), 64, 0, s>>>(one, tab, nblk, d.nyg);
+    k_dsm_ratio<<<dim3(d.nyg - 2 + (one.dtpart ? 1 : 0), 1, nmem), 64, 0, s>>>(one, tab, nblk, d.nyg);
     PROF_END(s);
 }
 
@@ -159,7 +180,9 @@ void launch_dsm(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, 
                 int dirichlet, int fuse_wm, const double* alpha_x, T nu, const Scalars* sc, hipStream_t s)
 {
     launch_dsm_fused<T>(U, V, W, m, d, b, s);
-    run_dsm_ratio(DsmRatioArgs{b.part, b.ratio}, nullptr, 1, d, b.nblk, s);
+    run_dsm_ratio(DsmRatioArgs{b.part, b.ratio, b.dtpart, b.nblk_dt, const_cast<Scalars*>(sc),
+                               b.dtpart ? b.mins3 : nullptr},
+                  nullptr, 1, d, b.nblk, s);
     run_dsm_nut<T>(DsmNutArgs<T>{nut, b.smag, b.ratio, U, alpha_x, m, nu, dirichlet, sc}, nullptr, 1, d, fuse_wm, s);
 }
 
@@ -484,6 +507,20 @@ __global__ void k_dt_partial(DtArgs<T> one, const DtArgs<T>* tab, Dims d, int nb
     block_max3(lu, lv, lw, a.part, nblk, g.x);
 }
 
+// code/time_integration.f90:123-148 from the three maxima of dx/|u|: the
+// convective and viscous limits, the cap, a negative CFL as a fixed step
+__device__ inline double dt_from_maxima(double mu, double mv, double mw, const Scalars* sc, const double* mins3)
+{
+    const double cfl = sc->cfl, nu = sc->nu;
+    double dconv = fmin(fmin(cfl * mu, cfl * mv), cfl * mw);
+    double dvis = fmin(fmin(cfl * (mins3[0] * mins3[0]) / nu, cfl * (mins3[1] * mins3[1]) / nu),
+                       cfl * (mins3[2] * mins3[2]) / nu);
+    double dt = fmin(dconv, dvis);
+    dt = fmin(dt, sc->dt_max);
+    if (cfl < 0) dt = -cfl;
+    return dt;
+}
+
 template <typename T>
 __global__ void k_dt_final(DtArgs<T> one, const DtArgs<T>* tab, int nblk)
 {
@@ -508,16 +545,7 @@ __global__ void k_dt_final(DtArgs<T> one, const DtArgs<T>* tab, int nblk)
         }
         __syncthreads();
     }
-    if (t == 0) {
-        const double cfl = sc->cfl, nu = sc->nu;
-        double dconv = fmin(fmin(cfl * red[0][0], cfl * red[1][0]), cfl * red[2][0]);
-        double dvis = fmin(fmin(cfl * (mins3[0] * mins3[0]) / nu, cfl * (mins3[1] * mins3[1]) / nu),
-                           cfl * (mins3[2] * mins3[2]) / nu);
-        double dt = fmin(dconv, dvis);
-        dt = fmin(dt, sc->dt_max);
-        if (cfl < 0) dt = -cfl;
-        sc->dt = dt;
-    }
+    if (t == 0) sc->dt = dt_from_maxima(red[0][0], red[1][0], red[2][0], sc, mins3);
 }
 
 template <typename T>

@@ -430,3 +430,45 @@ def test_wall_viscosity_limit(gpu):
     for row, wall in ((0, 0), (1, 0), (-2, 1), (-1, 1)):
         assert np.max(np.abs(got[:, row, :] - ref[:, wall, :])) <= 1e-14 * numax
     assert np.array_equal(got[:, 2:-2, :], g.get_field(NU_T)[:, 2:-2, :])
+
+
+@pytest.mark.parametrize("precision", [FP64, "fp32"])
+def test_run_steps_folds_compute_dt_bitwise(gpu, precision):
+    """run_steps with the dynamic model folds compute_dt into the first DSM
+    pass of the step (per-column minima of |U| and |V|, per-plane maxima of
+    dz/|W|, a last block of the ratio kernel): three steps give the same dt
+    and fields, bit for bit, as compute_dt + compute_time_step_RK3, for every
+    DSM tile shape.  The fields are shifted so that the convective limit
+    binds (a small CFL and viscosity keep the viscous limit and the cap
+    away), once through U and once through W."""
+    from channel_rl_amd import FP32_BULK, KERNEL_DSM, WALL_EQUILIBRIUM, load_library
+    prec = FP64 if precision == FP64 else FP32_BULK
+    for u0, w0 in ((3.0, 50.0), (50.0, 3.0)):
+        case, field = cases.small_case(22, 13, 32, seed=9)
+        case.nu, case.CFL = 1e-4, 1e-3
+        rng = np.random.default_rng(4)
+        U = np.asfortranarray(u0 * (1.0 + 0.05 * rng.uniform(-1, 1, field[0].shape)))
+        V = np.asfortranarray(0.1 * rng.uniform(-1, 1, field[1].shape))
+        W = np.asfortranarray(w0 * (1.0 + 0.05 * rng.uniform(-1, 1, field[2].shape)))
+        for shape in range(load_library().chan_tile_shape_count(KERNEL_DSM)):
+            chans = []
+            for _ in range(2):
+                g = Channel(case.x, case.y, case.z, precision=prec, **case.kwargs())
+                g.set_state(U, V, W)
+                g.apply_boundary_conditions()
+                g.compute_projection_step()
+                g.set_scalars(dt=0.0, t=0.0, dPdx=0.0, Qflow_x_0=0.0)
+                g.set_wall_model(WALL_EQUILIBRIUM, h_wm=0.1)
+                g.set_tile_shape(shape, KERNEL_DSM)
+                chans.append(g)
+            a, b = chans
+            for step in range(3):
+                dt_a = a.compute_dt()
+                a.compute_time_step_RK3()
+                b.run_steps(1)
+                dt_b = b.scalars()["dt"]
+                assert dt_a == dt_b, (u0, w0, shape, step, dt_a, dt_b)
+                assert dt_a < 0.5 * case.CFL * (case.y[1] - case.y[0]) ** 2 / case.nu  # convective limit binds
+            for x, y in zip(a.state(), b.state()):
+                assert np.array_equal(x, y), (u0, w0, shape)
+            del a, b, chans
