@@ -308,16 +308,20 @@ void enqueue_bc(chan_ctx* c, int buf)
     launch_bc<T>(s[2], c->d, KIND_W, c->stream);
 }
 
+// with_eq: the DSM path also computes the equilibrium wall stress of this
+// state into c->alpha (in its ratio launch, before the pass that reads it)
 template <typename T>
-void enqueue_eddy_viscosity(chan_ctx* c, int buf, int fuse_wm, bool with_dt = false)
+void enqueue_eddy_viscosity(chan_ctx* c, int buf, int fuse_wm, bool with_dt = false, bool with_eq = false)
 {
     T** s = (T**)c->state[buf];
     const T nu = T(c->nu);
     if (c->LES_model == 2) {
         DsmBufs<T> b = dsm_bufs<T>(c);
         if (with_dt) b.dtpart = c->dtpart;
+        const EqAlphaArgs<T> eq{s[0], c->alpha, c->nu, wall_rows(c, c->h_wm), c->kappa, c->Bc,
+                                loglaw_crossover(c->kappa, c->Bc)};
         launch_dsm<T>(s[0], s[1], s[2], (T*)c->nut, metric<T>(c), c->d, b, c->Dirichlet_nu_t == 1, fuse_wm,
-                      c->alpha, nu, c->sc, c->stream);
+                      c->alpha, nu, c->sc, c->stream, with_eq ? &eq : nullptr);
     } else if (c->LES_model == 3) {
         launch_amd<T>(s[0], s[1], s[2], (T*)c->nut, metric<T>(c), c->d, c->Dirichlet_nu_t, c->sc, fuse_wm, c->alpha,
                       nu, c->stream);
@@ -350,7 +354,8 @@ void enqueue_rk3(chan_ctx* c, int X, bool dt_fused = false)
     for (int s = 0; s < 3; s++) {
         T** I = (T**)c->state[in[s]];
         T** O = (T**)c->state[out[s]];
-        if (c->wall_mode == CHAN_WALL_EQUILIBRIUM) {
+        const bool eq_in_dsm = c->wall_mode == CHAN_WALL_EQUILIBRIUM && c->LES_model == 2;
+        if (c->wall_mode == CHAN_WALL_EQUILIBRIUM && !eq_in_dsm) {
             launch_equilibrium_alpha<T>(I[0], c->alpha, c->d, c->nu, wall_rows(c, c->h_wm), c->kappa, c->Bc, c->stream);
         } else if (c->wall_mode == CHAN_WALL_MLP) {
             const void* net = nullptr;
@@ -359,7 +364,7 @@ void enqueue_rk3(chan_ctx* c, int X, bool dt_fused = false)
                 launch_wm_mlp<T>(net, I[0], I[2], c->alpha, c->sc, c->md, c->d, c->nu, c->h_wm, c->kappa, c->Bc,
                                  c->x_mass_cte, c->wm_max_err, c->wm_stats, c->stream);
         }
-        enqueue_eddy_viscosity<T>(c, in[s], 1, dt_fused && s == 0);
+        enqueue_eddy_viscosity<T>(c, in[s], 1, dt_fused && s == 0, eq_in_dsm);
         double c1, c2 = 0.0;
         if (s == 0) c1 = c->rk_coef[0][0];
         else if (s == 1) { c1 = c->rk_coef[1][0]; c2 = c->rk_coef[1][1]; }

@@ -9,6 +9,9 @@ namespace chan {
 // field kinds for the ghost fill
 enum FieldKind { KIND_U = 0, KIND_V = 1, KIND_W = 2, KIND_C_DIRICHLET = 3, KIND_C_NEUMANN = 4 };
 
+template <typename T>
+struct EqAlphaArgs;
+
 // Work buffers of the dynamic Smagorinsky model (box layout, type T).
 template <typename T>
 struct DsmBufs {
@@ -51,7 +54,7 @@ int rhs_shape_count();
 template <typename T>
 void launch_dsm(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, const Dims& d,
                 DsmBufs<T>& b, int dirichlet, int fuse_wm, const double* alpha_x, T nu, const Scalars* sc,
-                hipStream_t s);
+                hipStream_t s, const EqAlphaArgs<T>* eq = nullptr);
 
 // AMD eddy viscosity (amd.hip), dirichlet = Dirichlet_nu_t (0, 1, 2)
 template <typename T>
@@ -102,6 +105,14 @@ struct DsmRatioArgs {
     int nblk_dt;
     Scalars* sc;
     const double* mins3;
+    // with eq_alpha set, further blocks compute the equilibrium wall stress
+    // of k_equilibrium_alpha from eq_U (float when eq_fp32) into eq_alpha
+    const void* eq_U;
+    int eq_fp32;
+    double* eq_alpha;
+    double eq_nu, eq_kappa, eq_B, eq_yc;
+    int eq_jb;
+    double eq_wb, eq_hb, eq_ht;
 };
 template <typename T>
 struct DsmNutArgs {

@@ -86,11 +86,29 @@ __device__ inline double dt_from_maxima(double mu, double mv, double mw, const S
 // result does not depend on scheduling; no LDS and no barrier.  With the
 // DSM pass's compute_dt maxima (ar.dtpart), a last block reduces them and
 // sets the step's dt (the maximum does not depend on the order).
-__global__ void __launch_bounds__(64) k_dsm_ratio(DsmRatioArgs one, const DsmRatioArgs* tab, int nblk, int nyg)
+template <typename T>
+__device__ inline void eq_alpha_cell(int q, const T* U, double* alpha, const Dims& d, double nu, double kappa,
+                                     double B, double yc, const WallRows& ws);
+
+__global__ void __launch_bounds__(64) k_dsm_ratio(DsmRatioArgs one, const DsmRatioArgs* tab, int nblk, Dims d)
 {
     const GBlk g = gblk(1);
     const DsmRatioArgs ar = member_args(one, tab, g.m);
     const int t = threadIdx.x;
+    const int nyg = d.nyg;
+    const int neq0 = nyg - 2 + (ar.dtpart ? 1 : 0);  // the equilibrium stress's first block
+    if (g.x >= neq0) {
+        // the equilibrium wall stress, 64 cells per block (k_equilibrium_alpha)
+        const int q = (g.x - neq0) * 64 + t;
+        if (q >= 2 * d.nx * d.nzg) return;
+        const WallRows ws{ar.eq_jb, ar.eq_wb, ar.eq_hb, ar.eq_ht};
+        if (ar.eq_fp32)
+            eq_alpha_cell<float>(q, (const float*)ar.eq_U, ar.eq_alpha, d, ar.eq_nu, ar.eq_kappa, ar.eq_B, ar.eq_yc, ws);
+        else
+            eq_alpha_cell<double>(q, (const double*)ar.eq_U, ar.eq_alpha, d, ar.eq_nu, ar.eq_kappa, ar.eq_B, ar.eq_yc,
+                                  ws);
+        return;
+    }
     if (g.x == nyg - 2) {
         const double* dp = ar.dtpart;
         const int nb = ar.nblk_dt;
@@ -161,7 +179,8 @@ void run_dsm_ratio(const DsmRatioArgs& one, const DsmRatioArgs* tab, int nmem, c
                    hipStream_t s)
 {
     PROF_BEGIN("k_dsm_ratio", s);
-    k_dsm_ratio<<<dim3(d.nyg - 2 + (one.dtpart ? 1 : 0), 1, nmem), 64, 0, s>>>(one, tab, nblk, d.nyg);
+    const int neq = one.eq_alpha ? cdiv(2L * d.nx * d.nzg, 64) : 0;
+    k_dsm_ratio<<<dim3(d.nyg - 2 + (one.dtpart ? 1 : 0) + neq, 1, nmem), 64, 0, s>>>(one, tab, nblk, d);
     PROF_END(s);
 }
 
@@ -177,12 +196,21 @@ void run_dsm_nut(const DsmNutArgs<T>& one, const DsmNutArgs<T>* tab, int nmem, c
 
 template <typename T>
 void launch_dsm(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, const Dims& d, DsmBufs<T>& b,
-                int dirichlet, int fuse_wm, const double* alpha_x, T nu, const Scalars* sc, hipStream_t s)
+                int dirichlet, int fuse_wm, const double* alpha_x, T nu, const Scalars* sc, hipStream_t s,
+                const EqAlphaArgs<T>* eq)
 {
     launch_dsm_fused<T>(U, V, W, m, d, b, s);
-    run_dsm_ratio(DsmRatioArgs{b.part, b.ratio, b.dtpart, b.nblk_dt, const_cast<Scalars*>(sc),
-                               b.dtpart ? b.mins3 : nullptr},
-                  nullptr, 1, d, b.nblk, s);
+    DsmRatioArgs ra{b.part, b.ratio, b.dtpart, b.nblk_dt, const_cast<Scalars*>(sc), b.dtpart ? b.mins3 : nullptr};
+    if (eq) {
+        // the equilibrium wall stress in the same launch (it is read first by
+        // the pass B that follows)
+        ra.eq_U = eq->U;
+        ra.eq_fp32 = sizeof(T) == 4;
+        ra.eq_alpha = eq->alpha;
+        ra.eq_nu = eq->nu; ra.eq_kappa = eq->kappa; ra.eq_B = eq->B; ra.eq_yc = eq->yc;
+        ra.eq_jb = eq->ws.jb; ra.eq_wb = eq->ws.wb; ra.eq_hb = eq->ws.hb; ra.eq_ht = eq->ws.ht;
+    }
+    run_dsm_ratio(ra, nullptr, 1, d, b.nblk, s);
     run_dsm_nut<T>(DsmNutArgs<T>{nut, b.smag, b.ratio, U, alpha_x, m, nu, dirichlet, sc}, nullptr, 1, d, fuse_wm, s);
 }
 
@@ -217,16 +245,9 @@ void launch_wall_model(const T* U, T* nut, const double* alpha_x, const Metric<T
 
 // one thread per (x face, z centre, wall)
 template <typename T>
-__global__ void __launch_bounds__(256) k_equilibrium_alpha(EqAlphaArgs<T> one, const EqAlphaArgs<T>* tab, Dims d)
+__device__ inline void eq_alpha_cell(int q, const T* U, double* alpha, const Dims& d, double nu, double kappa,
+                                     double B, double yc, const WallRows& ws)
 {
-    const GBlk g = gblk(1);
-    const EqAlphaArgs<T> a = member_args(one, tab, g.m);
-    const T* U = a.U;
-    double* alpha = a.alpha;
-    const double nu = a.nu, kappa = a.kappa, B = a.B, yc = a.yc;
-    const WallRows& ws = a.ws;
-    const int q = g.x * blockDim.x + threadIdx.x;
-    if (q >= 2 * d.nx * d.nzg) return;
     const int i = q % d.nx + 1, r = q / d.nx;
     const int k = r % d.nzg + 1, wall = r / d.nzg;
     const int jb = ws.jb;
@@ -245,6 +266,16 @@ __global__ void __launch_bounds__(256) k_equilibrium_alpha(EqAlphaArgs<T> one, c
     const long ia = (i - 1) + (long)d.nx * (wall + 2L * (k - 1));
     const double t = loglaw_utau(uw, h, nu, kappa, B, yc, sqrt(fabs(alpha[ia])));
     alpha[ia] = (uw >= 0 ? 1.0 : -1.0) * t * t;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_equilibrium_alpha(EqAlphaArgs<T> one, const EqAlphaArgs<T>* tab, Dims d)
+{
+    const GBlk g = gblk(1);
+    const EqAlphaArgs<T> a = member_args(one, tab, g.m);
+    const int q = g.x * blockDim.x + threadIdx.x;
+    if (q >= 2 * d.nx * d.nzg) return;
+    eq_alpha_cell<T>(q, a.U, a.alpha, d, a.nu, a.kappa, a.B, a.yc, a.ws);
 }
 
 template <typename T>
@@ -698,7 +729,7 @@ void launch_wall_shear(const T* U, const T* nut, const Metric<double>& md, const
 #define CHAN_INST(T)                                                                                            \
     template void launch_bc<T>(T*, const Dims&, int, hipStream_t);                                              \
     template void launch_dsm<T>(const T*, const T*, const T*, T*, const Metric<T>&, const Dims&, DsmBufs<T>&,   \
-                                int, int, const double*, T, const Scalars*, hipStream_t);                       \
+                                int, int, const double*, T, const Scalars*, hipStream_t, const EqAlphaArgs<T>*); \
     template void launch_wall_model<T>(const T*, T*, const double*, const Metric<T>&, const Dims&, T,           \
                                        const Scalars*, hipStream_t);                                            \
     template void launch_equilibrium_alpha<T>(const T*, double*, const Dims&, double, const WallRows&,        \
