@@ -148,31 +148,49 @@ __global__ void __launch_bounds__(64) k_dsm_ratio(DsmRatioArgs one, const DsmRat
 // code/subgrid.f90:512-535: nu_t = max(-1/2 <LM>/<MM> |S|, 0), y ghost by
 // Dirichlet_nu_t; with fuse_wm the wall rows come from the wall model
 // (code/wallmodel.f90:39-42), which the reference applies right after.
+// Each lane takes one (x, y) position of NUT_PLANES consecutive z planes and
+// issues the planes' loads before any store (restrict pointers: the stores
+// cannot alias them), so a wave keeps several loads in flight.
+static constexpr int NUT_PLANES = 4;
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_dsm_nut(DsmNutArgs<T> one, const DsmNutArgs<T>* tab, Dims d, int fuse_wm)
 {
-    const GBlk g = gblk(d.nzg);
+    const int gz = (d.nzg + NUT_PLANES - 1) / NUT_PLANES;
+    const GBlk g = gblk(gz);
     const DsmNutArgs<T> a = member_args(one, tab, g.m);
-    T* nut = a.nut;
-    const T* U = a.U;
-    const double* alpha = a.alpha;
-    const Metric<T>& m = a.m;
-    const T nu = a.nu;
-    const int dirichlet = a.dirichlet;
-    // one (x, y) plane per block z, flattened so that every lane has a cell
+    T* __restrict__ nut = a.nut;
+    const T* __restrict__ smag = a.smag;
+    // one flattened (x, y) plane per block row, so that every lane has a cell
     const int q = g.x * blockDim.x + threadIdx.x;
     if (q >= d.nxg * d.nyg) return;
     const int jm = q / d.nxg;
-    const int i = q - jm * d.nxg + 1, j = jm + 1, k = g.z + 1;
-    const long p = at(d, i, j, k);
+    const int i = q - jm * d.nxg + 1, j = jm + 1;
+    const int k0 = g.z * NUT_PLANES + 1;
+    const int np = min(NUT_PLANES, d.nzg - k0 + 1);
+    const long p0 = at(d, i, j, k0);
     if (fuse_wm && (j <= 2 || j >= d.nyg - 1)) {
-        nut[p] = wm_nut_at<T>(U, alpha, m, d, nu, img_centre(i, d.nxg), k, j <= 2 ? 0 : 1, a.sc);
+        for (int t = 0; t < np; t++)
+            nut[p0 + t * d.sxy] =
+                wm_nut_at<T>(a.U, a.alpha, a.m, d, a.nu, img_centre(i, d.nxg), k0 + t, j <= 2 ? 0 : 1, a.sc);
         return;
     }
     const int jj = (j == 1) ? 2 : (j == d.nyg ? d.nyg - 1 : j);
+    const long s0 = at(d, img_centre(i, d.nxg), jj, 1);
+    T v[NUT_PLANES];
+#pragma unroll
+    for (int t = 0; t < NUT_PLANES; t++) {
+        // planes past the box read the last one and are not stored
+        const int k = k0 + min(t, np - 1);
+        v[t] = smag[s0 + (long)(img_centre(k, d.nzg) - 1) * d.sxy];
+    }
     const T r = T(a.ratio[jj - 1]);
-    const T v = fmax(T(-0.5) * r * a.smag[at(d, img_centre(i, d.nxg), jj, img_centre(k, d.nzg))], T(0));
-    nut[p] = (jj != j && dirichlet) ? -v : v;
+    const bool flip = jj != j && a.dirichlet;
+#pragma unroll
+    for (int t = 0; t < NUT_PLANES; t++) {
+        const T x = fmax(T(-0.5) * r * v[t], T(0));
+        if (t < np) nut[p0 + t * d.sxy] = flip ? -x : x;
+    }
 }
 
 void run_dsm_ratio(const DsmRatioArgs& one, const DsmRatioArgs* tab, int nmem, const Dims& d, int nblk,
@@ -188,7 +206,7 @@ template <typename T>
 void run_dsm_nut(const DsmNutArgs<T>& one, const DsmNutArgs<T>* tab, int nmem, const Dims& d, int fuse_wm,
                  hipStream_t s)
 {
-    dim3 grd(cdiv((long)d.nxg * d.nyg, 256), 1, (unsigned)d.nzg * nmem);
+    dim3 grd(cdiv((long)d.nxg * d.nyg, 256), 1, (unsigned)cdiv(d.nzg, NUT_PLANES) * nmem);
     PROF_BEGIN("k_dsm_nut", s);
     k_dsm_nut<T><<<grd, 256, 0, s>>>(one, tab, d, fuse_wm);
     PROF_END(s);
@@ -398,21 +416,31 @@ __global__ void k_dpx_final(MassArgs<T> one, const MassArgs<T>* tab, int nblk, i
     }
 }
 
-// U(2:nx-1,2:nyg-1,2:nzg-1) += dPdx (code/time_integration.f90:91)
+// U(2:nx-1,2:nyg-1,2:nzg-1) += dPdx (code/time_integration.f90:91).  Each
+// lane takes SHIFT_PLANES consecutive z planes of one (x, y) position, all
+// loads issued before the first store.
+static constexpr int SHIFT_PLANES = 4;
+
 template <typename T>
 __global__ void k_shift_u(MassArgs<T> one, const MassArgs<T>* tab, Dims d)
 {
-    const GBlk g = gblk(d.nzg - 3);
+    const GBlk g = gblk((d.nzg - 3 + SHIFT_PLANES - 1) / SHIFT_PLANES);
     const MassArgs<T> ar = member_args(one, tab, g.m);
     T* U = ar.U;
     const Scalars* sc = ar.sc;
     const int i = g.x * blockDim.x + threadIdx.x + 2;
     const int j = g.y * blockDim.y + threadIdx.y + 2;
-    const int k = g.z + 2;
+    const int k0 = g.z * SHIFT_PLANES + 2;
     if (i > d.nx - 1 || j > d.nyg - 1) return;
+    const int np = min(SHIFT_PLANES, d.nzg - 2 - k0 + 1);
+    const long p0 = at(d, i, j, k0);
+    T v[SHIFT_PLANES];
+#pragma unroll
+    for (int t = 0; t < SHIFT_PLANES; t++) v[t] = U[p0 + min(t, np - 1) * d.sxy];
     const T s = T(sc->last_dPdx);
-    const long p = at(d, i, j, k);
-    store_img(U, d, 0, i, j, k, U[p] + s);
+#pragma unroll
+    for (int t = 0; t < SHIFT_PLANES; t++)
+        if (t < np) store_img(U, d, 0, i, j, k0 + t, v[t] + s);
 }
 
 // Constant bulk flow (mass-flow mode 2; no reference counterpart: the
@@ -455,7 +483,7 @@ void run_mass_flow_correct(const MassArgs<T>& one, const MassArgs<T>* tab, int n
     PROF_END(s);
     if (apply) {
         // canonical cells only (k up to nzg-2), images written by store_img
-        dim3 blk(64, 4, 1), grd(cdiv(d.nx - 2, 64), cdiv(d.nyg - 2, 4), (unsigned)(d.nzg - 3) * nmem);
+        dim3 blk(64, 4, 1), grd(cdiv(d.nx - 2, 64), cdiv(d.nyg - 2, 4), (unsigned)cdiv(d.nzg - 3, SHIFT_PLANES) * nmem);
         PROF_BEGIN("k_shift_u", s);
         k_shift_u<T><<<grd, blk, 0, s>>>(one, tab, d);
         PROF_END(s);
