@@ -120,6 +120,10 @@ struct chan_ctx {
     hipGraphExec_t graph_dt[3] = {nullptr, nullptr, nullptr};
     double* dtpart = nullptr;  // the DSM pass's compute_dt maxima, [3][blocks]
     int nblk_dt = 0;
+    // substeps 1 and 2 of a step leave the dynamic model's pass B to the RHS
+    // kernel (DsmBufs::ntab): its row coefficients, [3][nyg] of the bulk type
+    void* ntab = nullptr;
+    bool defer_nut = true;
     bool use_graph = true;
     int rhs_shape = 0;  // index into the RHS kernel's tile shapes
     // learned wall model
@@ -272,6 +276,7 @@ DsmBufs<T> dsm_bufs(chan_ctx* c)
     b.rdx = (const T*)c->rcp[pi][0]; b.rdy = (const T*)c->rcp[pi][1]; b.rdz = (const T*)c->rcp[pi][2];
     b.rdxg = (const T*)c->rcp[pi][3]; b.rdyg = (const T*)c->rcp[pi][4]; b.rdzg = (const T*)c->rcp[pi][5];
     b.dtpart = nullptr;  // set by the step that folds compute_dt into the pass
+    b.ntab = nullptr;    // set by the substeps that defer pass B
     b.xg = c->md.xg; b.yg = c->md.yg; b.zg = c->md.zg;
     b.mins3 = c->mins3;
     b.nblk_dt = c->nblk_dt;
@@ -309,15 +314,19 @@ void enqueue_bc(chan_ctx* c, int buf)
 }
 
 // with_eq: the DSM path also computes the equilibrium wall stress of this
-// state into c->alpha (in its ratio launch, before the pass that reads it)
+// state into c->alpha (in its ratio launch, before the pass that reads it).
+// defer: the DSM path leaves its pass B to the RHS kernel, c->nut is not
+// written (the RHS then reads the |S| box with the coefficients c->ntab)
 template <typename T>
-void enqueue_eddy_viscosity(chan_ctx* c, int buf, int fuse_wm, bool with_dt = false, bool with_eq = false)
+void enqueue_eddy_viscosity(chan_ctx* c, int buf, int fuse_wm, bool with_dt = false, bool with_eq = false,
+                            bool defer = false)
 {
     T** s = (T**)c->state[buf];
     const T nu = T(c->nu);
     if (c->LES_model == 2) {
         DsmBufs<T> b = dsm_bufs<T>(c);
         if (with_dt) b.dtpart = c->dtpart;
+        if (defer) b.ntab = (T*)c->ntab;
         const EqAlphaArgs<T> eq{s[0], c->alpha, c->nu, wall_rows(c, c->h_wm), c->kappa, c->Bc,
                                 loglaw_crossover(c->kappa, c->Bc)};
         launch_dsm<T>(s[0], s[1], s[2], (T*)c->nut, metric<T>(c), c->d, b, c->Dirichlet_nu_t == 1, fuse_wm,
@@ -364,14 +373,16 @@ void enqueue_rk3(chan_ctx* c, int X, bool dt_fused = false)
                 launch_wm_mlp<T>(net, I[0], I[2], c->alpha, c->sc, c->md, c->d, c->nu, c->h_wm, c->kappa, c->Bc,
                                  c->x_mass_cte, c->wm_max_err, c->wm_stats, c->stream);
         }
-        enqueue_eddy_viscosity<T>(c, in[s], 1, dt_fused && s == 0, eq_in_dsm);
+        // nu_t is read by the RHS only, except after the last substep
+        const bool defer = c->defer_nut && c->LES_model == 2 && s < 2;
+        enqueue_eddy_viscosity<T>(c, in[s], 1, dt_fused && s == 0, eq_in_dsm, defer);
         double c1, c2 = 0.0;
         if (s == 0) c1 = c->rk_coef[0][0];
         else if (s == 1) { c1 = c->rk_coef[1][0]; c2 = c->rk_coef[1][1]; }
         else { c1 = c->rk_coef[2][0]; c2 = c->rk_coef[2][2]; }
-        launch_rhs_rk<T>(I[0], I[1], I[2], (const T*)c->nut, rhs_tabs<T>(c), c->d, T(c->nu), c->sc, s + 1, Xs[0],
-                         Xs[1], Xs[2], (T*)c->F1[0], (T*)c->F1[1], (T*)c->F1[2], O[0], O[1], O[2], c1, c2,
-                         c->rhs_shape, c->stream);
+        launch_rhs_rk<T>(I[0], I[1], I[2], defer ? (const T*)c->dsm[9] : (const T*)c->nut, rhs_tabs<T>(c), c->d,
+                         T(c->nu), c->sc, s + 1, Xs[0], Xs[1], Xs[2], (T*)c->F1[0], (T*)c->F1[1], (T*)c->F1[2], O[0],
+                         O[1], O[2], c1, c2, c->rhs_shape, c->stream, defer ? (const T*)c->ntab : nullptr);
         enqueue_projection<T>(c, out[s]);
     }
     if (c->x_mass_cte == 1 || c->x_mass_cte == 2) {
@@ -522,6 +533,7 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
     c->nu = cfg->nu; c->CFL = cfg->CFL; c->dPdx0 = cfg->dPdx;
     c->x_mass_cte = cfg->x_mass_cte; c->LES_model = cfg->LES_model; c->Dirichlet_nu_t = cfg->Dirichlet_nu_t;
     if (const char* e = getenv("CHANLES_NO_GRAPH")) c->use_graph = (e[0] == '0');
+    if (const char* e = getenv("CHANLES_NO_DEFERRED_NUT")) c->defer_nut = (e[0] == '0');
 
     Dims& d = c->d;
     d.nx = nx; d.ny = ny; d.nz = nz;
@@ -644,6 +656,7 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
     }
     if ((rc = dalloc(c, &c->nut, fb))) return rc;
     if (c->LES_model == 2 && (rc = dalloc(c, &c->dsm[9], fb))) return rc;  // |S|
+    if (c->LES_model == 2 && (rc = dalloc(c, &c->ntab, 3 * (size_t)d.nyg * sizeof(double)))) return rc;
     {
         // 1-based reciprocal spacings for the RHS and the dynamic model
         const std::vector<double>* f[6] = {&c->x, &c->y, &c->z, &c->xg, &c->yg, &c->zg};
@@ -1628,7 +1641,7 @@ int enqueue_group_step(chan_ensemble* e, const std::vector<int>& idx, TabSink& s
                 T** I = st(q, inb(q));
                 T** O = st(q, outb(q));
                 T** Xs = st(q, X[q]);
-                a[q] = RhsArgs<T>{I[0], I[1], I[2], (const T*)c->nut, rhs_tabs<T>(c), T(c->nu), c->sc, Xs[0], Xs[1],
+                a[q] = RhsArgs<T>{I[0], I[1], I[2], (const T*)c->nut, nullptr, rhs_tabs<T>(c), T(c->nu), c->sc, Xs[0], Xs[1],
                                   Xs[2], (T*)c->F1[0], (T*)c->F1[1], (T*)c->F1[2], O[0], O[1], O[2]};
             }
             const RhsArgs<T>* t = sink.put(a);

@@ -30,6 +30,12 @@ struct DsmBufs {
     const double *xg, *yg, *zg;
     const double* mins3;  // compute_dt's smallest spacings (its viscous limit)
     int nblk_dt;          // blocks of the pass (dtpart's row length)
+    // with ntab set, pass B is left to the consumer of nu_t: the ratio launch
+    // writes per row j the coefficients (A, B, S) at ntab[c*nyg + j-1] with
+    // nu_t = S max(A v, B) for the value v that smag then holds at a cell,
+    // and a small kernel completes smag's ghost rows (the wall model's nu_t
+    // there with fuse_wm) and periodic images; nu_t itself is not written
+    T* ntab;
 };
 
 template <typename T>
@@ -44,10 +50,12 @@ struct RhsTabs {
 };
 
 // mode 0: rhs only into R*; mode 1..3: RK substep (Wray, code/time_integration.f90:34-86)
+// ntab: nut holds the dynamic model's deferred pass-B values (DsmBufs::ntab)
 template <typename T>
 void launch_rhs_rk(const T* U, const T* V, const T* W, const T* nut, const RhsTabs<T>& m, const Dims& d,
                    T nu, const Scalars* sc, int mode, const T* Uo, const T* Vo, const T* Wo, T* F1u,
-                   T* F1v, T* F1w, T* Ou, T* Ov, T* Ow, double c1, double c2, int shape, hipStream_t s);
+                   T* F1v, T* F1w, T* Ou, T* Ov, T* Ow, double c1, double c2, int shape, hipStream_t s,
+                   const T* ntab = nullptr);
 // number of RHS tile shapes (shape index 0 .. count-1)
 int rhs_shape_count();
 
@@ -106,13 +114,18 @@ struct DsmRatioArgs {
     Scalars* sc;
     const double* mins3;
     // with eq_alpha set, further blocks compute the equilibrium wall stress
-    // of k_equilibrium_alpha from eq_U (float when eq_fp32) into eq_alpha
+    // of k_equilibrium_alpha from eq_U (float when fp32) into eq_alpha
     const void* eq_U;
-    int eq_fp32;
+    int fp32;
     double* eq_alpha;
     double eq_nu, eq_kappa, eq_B, eq_yc;
     int eq_jb;
     double eq_wb, eq_hb, eq_ht;
+    // with ntab set, the row blocks also write pass B's coefficients
+    // (DsmBufs::ntab; float when fp32), wall rows passed through with
+    // ntab_wm, ghost rows negated with ntab_dirichlet
+    void* ntab;
+    int ntab_wm, ntab_dirichlet;
 };
 template <typename T>
 struct DsmNutArgs {
@@ -157,6 +170,7 @@ struct DsmArgs {
 template <typename T>
 struct RhsArgs {
     const T *U, *V, *W, *nt;
+    const T* ntab;  // nt holds deferred pass-B values with these row coefficients (or null)
     RhsTabs<T> tb;
     T nu;
     const Scalars* sc;

@@ -232,13 +232,35 @@ __global__ void __launch_bounds__(TX* TY)
             }
     };
     auto slot = [&](int p) { return (p - kb + 4) & 3; };
+    // deferred pass B of the dynamic model (arg.ntab): the nu_t plane holds
+    // the value v of code/subgrid.f90:512-535 before the row's ratio, and
+    // nu_t = S max(A v, B) with the coefficients of the halo row, the same
+    // operations as pass B itself (A = -1/2 ratio; wall-model rows pass
+    // through with A = 1, B = -inf)
+    const T* ntab = arg.ntab;
+    __shared__ T ncf[3][PY];
+    if (ntab) {
+        for (int q = tid; q < 3 * PY; q += NT) {
+            const int c = q / PY, ly = q - c * PY;
+            const int jj = j0 - 1 + ly;
+            ncf[c][ly] = jj <= d.nyg ? ntab[c * d.nyg + jj - 1] : T(0);
+        }
+        __syncthreads();
+    }
     auto sstore = [&](int p, const T (&pre)[NL][4]) {
         const int s = slot(p);
 #pragma unroll
         for (int r = 0; r < NL; r++)
-            if (lq[r] >= 0)
+            if (lq[r] >= 0) {
 #pragma unroll
-                for (int f = 0; f < 4; f++) sh[f][s][lq[r]] = pre[r][f];
+                for (int f = 0; f < 3; f++) sh[f][s][lq[r]] = pre[r][f];
+                T v = pre[r][3];
+                if (ntab) {
+                    const int ly = lq[r] / PX;
+                    v = ncf[2][ly] * fmax(ncf[0][ly] * v, ncf[1][ly]);
+                }
+                sh[3][s][lq[r]] = v;
+            }
     };
     auto F = [&](int f, int a, int b, int c) -> T { return sh[f][slot(c)][(a - i0 + 1) + PX * (b - j0 + 1)]; };
     const T dPdx = T(sc->dPdx);
@@ -408,22 +430,23 @@ void run_rhs_rk(const RhsArgs<T>& one, const RhsArgs<T>* tab, int nmem, const Di
 template <typename T>
 void launch_rhs_rk(const T* U, const T* V, const T* W, const T* nut, const RhsTabs<T>& m, const Dims& d, T nu,
                    const Scalars* sc, int mode, const T* Uo, const T* Vo, const T* Wo, T* F1u, T* F1v, T* F1w,
-                   T* Ou, T* Ov, T* Ow, double c1, double c2, int shape, hipStream_t s)
+                   T* Ou, T* Ov, T* Ow, double c1, double c2, int shape, hipStream_t s, const T* ntab)
 {
-    run_rhs_rk<T>(RhsArgs<T>{U, V, W, nut, m, nu, sc, Uo, Vo, Wo, F1u, F1v, F1w, Ou, Ov, Ow}, nullptr, 1, d, mode, c1,
-                  c2, shape, s);
+    run_rhs_rk<T>(RhsArgs<T>{U, V, W, nut, ntab, m, nu, sc, Uo, Vo, Wo, F1u, F1v, F1w, Ou, Ov, Ow}, nullptr, 1, d,
+                  mode, c1, c2, shape, s);
 }
 
 template void launch_rhs_rk<double>(const double*, const double*, const double*, const double*,
                                     const RhsTabs<double>&, const Dims&, double, const Scalars*, int, const double*,
                                     const double*, const double*, double*, double*, double*, double*, double*,
-                                    double*, double, double, int, hipStream_t);
+                                    double*, double, double, int, hipStream_t, const double*);
 template void run_rhs_rk<double>(const RhsArgs<double>&, const RhsArgs<double>*, int, const Dims&, int, double,
                                  double, int, hipStream_t);
 template void run_rhs_rk<float>(const RhsArgs<float>&, const RhsArgs<float>*, int, const Dims&, int, double, double,
                                 int, hipStream_t);
 template void launch_rhs_rk<float>(const float*, const float*, const float*, const float*, const RhsTabs<float>&,
                                    const Dims&, float, const Scalars*, int, const float*, const float*, const float*,
-                                   float*, float*, float*, float*, float*, float*, double, double, int, hipStream_t);
+                                   float*, float*, float*, float*, float*, float*, double, double, int, hipStream_t,
+                                   const float*);
 
 }  // namespace chan

@@ -472,3 +472,47 @@ def test_run_steps_folds_compute_dt_bitwise(gpu, precision):
             for x, y in zip(a.state(), b.state()):
                 assert np.array_equal(x, y), (u0, w0, shape)
             del a, b, chans
+
+
+@pytest.mark.parametrize("precision", [FP64, "fp32"])
+def test_deferred_dsm_pass_b_bitwise(gpu, precision, monkeypatch):
+    """The step's first two substeps leave the dynamic model's pass B
+    (code/subgrid.f90:512-535 with the wall rows of code/wallmodel.f90:39-42)
+    to the RHS kernel: it reads |S| with per-row coefficients from the ratio
+    launch, and a small kernel completes |S|'s wall rows and periodic images.
+    Three steps give the same fields, nu_t and P, bit for bit, as with pass B
+    materialising nu_t in every substep (CHANLES_NO_DEFERRED_NUT=1), for the
+    equilibrium law and for a fixed alpha_x, on a grid whose tiles do not
+    divide the box and for every RHS tile shape."""
+    from channel_rl_amd import FP32_BULK, KERNEL_RHS, WALL_EQUILIBRIUM, load_library
+    prec = FP64 if precision == FP64 else FP32_BULK
+    case, field = cases.small_case(22, 13, 34, seed=11)
+    nshape = load_library().chan_tile_shape_count(KERNEL_RHS)
+    for wall in ("equilibrium", "fixed"):
+        for shape in range(0, nshape, 3):
+            chans = []
+            for deferred in (False, True):
+                if deferred:
+                    monkeypatch.delenv("CHANLES_NO_DEFERRED_NUT", raising=False)
+                else:
+                    monkeypatch.setenv("CHANLES_NO_DEFERRED_NUT", "1")
+                g = Channel(case.x, case.y, case.z, precision=prec, **case.kwargs())
+                g.set_state(*field)
+                g.apply_boundary_conditions()
+                g.compute_projection_step()
+                g.set_scalars(dt=0.0, t=0.0, dPdx=0.0, Qflow_x_0=0.0)
+                if wall == "equilibrium":
+                    g.set_wall_model(WALL_EQUILIBRIUM, h_wm=0.1)
+                else:
+                    g.set_field(ALPHA_X, np.full(g.shape(ALPHA_X), 2e-4))
+                g.set_tile_shape(shape, KERNEL_RHS)
+                chans.append(g)
+            a, b = chans
+            a.run_steps(3)
+            b.run_steps(3)
+            for x, y in zip(a.state(), b.state()):
+                assert np.array_equal(x, y), (wall, shape)
+            for f in (NU_T, P, ALPHA_X):
+                assert np.array_equal(a.get_field(f), b.get_field(f)), (wall, shape, f)
+            assert a.scalars() == b.scalars()
+            del a, b, chans
