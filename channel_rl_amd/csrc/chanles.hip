@@ -121,7 +121,7 @@ struct chan_ctx {
     double* dtpart = nullptr;  // the DSM pass's compute_dt maxima, [3][blocks]
     int nblk_dt = 0;
     // substeps 1 and 2 of a step leave the dynamic model's pass B to the RHS
-    // kernel (DsmBufs::ntab): its row coefficients, [3][nyg] of the bulk type
+    // kernel (DsmBufs::ntab): its row coefficients, [2][nyg] of the bulk type
     void* ntab = nullptr;
     bool defer_nut = true;
     bool use_graph = true;
@@ -656,7 +656,7 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
     }
     if ((rc = dalloc(c, &c->nut, fb))) return rc;
     if (c->LES_model == 2 && (rc = dalloc(c, &c->dsm[9], fb))) return rc;  // |S|
-    if (c->LES_model == 2 && (rc = dalloc(c, &c->ntab, 3 * (size_t)d.nyg * sizeof(double)))) return rc;
+    if (c->LES_model == 2 && (rc = dalloc(c, &c->ntab, 2 * (size_t)d.nyg * sizeof(double)))) return rc;
     {
         // 1-based reciprocal spacings for the RHS and the dynamic model
         const std::vector<double>* f[6] = {&c->x, &c->y, &c->z, &c->xg, &c->yg, &c->zg};

@@ -30,11 +30,12 @@ struct DsmBufs {
     const double *xg, *yg, *zg;
     const double* mins3;  // compute_dt's smallest spacings (its viscous limit)
     int nblk_dt;          // blocks of the pass (dtpart's row length)
-    // with ntab set, pass B is left to the consumer of nu_t: the ratio launch
-    // writes per row j the coefficients (A, B, S) at ntab[c*nyg + j-1] with
-    // nu_t = S max(A v, B) for the value v that smag then holds at a cell,
-    // and a small kernel completes smag's ghost rows (the wall model's nu_t
-    // there with fuse_wm) and periodic images; nu_t itself is not written
+    // with ntab set (and the wall rows from the wall model, fuse_wm), pass B
+    // is left to the consumer of nu_t: the ratio launch writes per row j the
+    // coefficients (A, B) at ntab[c*nyg + j-1] with nu_t = max(A v, B) for
+    // the value v that smag then holds at a cell, and a small kernel puts the
+    // wall model's nu_t into smag's wall rows and completes the periodic
+    // images of the others; nu_t itself is not written
     T* ntab;
 };
 
@@ -122,10 +123,8 @@ struct DsmRatioArgs {
     int eq_jb;
     double eq_wb, eq_hb, eq_ht;
     // with ntab set, the row blocks also write pass B's coefficients
-    // (DsmBufs::ntab; float when fp32), wall rows passed through with
-    // ntab_wm, ghost rows negated with ntab_dirichlet
+    // (DsmBufs::ntab; float when fp32), the wall rows passed through
     void* ntab;
-    int ntab_wm, ntab_dirichlet;
 };
 template <typename T>
 struct DsmNutArgs {

@@ -234,13 +234,13 @@ __global__ void __launch_bounds__(TX* TY)
     auto slot = [&](int p) { return (p - kb + 4) & 3; };
     // deferred pass B of the dynamic model (arg.ntab): the nu_t plane holds
     // the value v of code/subgrid.f90:512-535 before the row's ratio, and
-    // nu_t = S max(A v, B) with the coefficients of the halo row, the same
-    // operations as pass B itself (A = -1/2 ratio; wall-model rows pass
-    // through with A = 1, B = -inf)
+    // nu_t = max(A v, B) with the coefficients of the halo row, the same
+    // operations as pass B itself (A = -1/2 ratio, B = 0; the wall-model
+    // rows hold nu_t and pass through with A = 1, B = -inf)
     const T* ntab = arg.ntab;
-    __shared__ T ncf[3][PY];
+    __shared__ T ncf[2][PY];
     if (ntab) {
-        for (int q = tid; q < 3 * PY; q += NT) {
+        for (int q = tid; q < 2 * PY; q += NT) {
             const int c = q / PY, ly = q - c * PY;
             const int jj = j0 - 1 + ly;
             ncf[c][ly] = jj <= d.nyg ? ntab[c * d.nyg + jj - 1] : T(0);
@@ -257,7 +257,7 @@ __global__ void __launch_bounds__(TX* TY)
                 T v = pre[r][3];
                 if (ntab) {
                     const int ly = lq[r] / PX;
-                    v = ncf[2][ly] * fmax(ncf[0][ly] * v, ncf[1][ly]);
+                    v = fmax(ncf[0][ly] * v, ncf[1][ly]);
                 }
                 sh[3][s][lq[r]] = v;
             }

@@ -145,19 +145,19 @@ __global__ void __launch_bounds__(64) k_dsm_ratio(DsmRatioArgs one, const DsmRat
         ratio[nyg + j - 1] = a;
         ratio[2 * nyg + j - 1] = c;
         if (ar.ntab) {
-            // deferred pass B: row j (and its ghost row beyond rows 2 and
-            // nyg-1) as k_dsm_nut would compute it from the value there
-            auto put = [&](auto* tab, int row, bool ghost) {
+            // deferred pass B: row j (and the ghost row beyond rows 2 and
+            // nyg-1) as k_dsm_nut computes it from the value there; the
+            // wall-model rows 1, 2, nyg-1, nyg hold nu_t and pass through
+            auto put = [&](auto* tab, int row) {
                 using V = std::remove_pointer_t<decltype(tab)>;
-                const bool wm = ar.ntab_wm && (row <= 2 || row >= nyg - 1);
+                const bool wm = row <= 2 || row >= nyg - 1;
                 tab[row - 1] = wm ? V(1) : V(-0.5) * V(rt);
                 tab[nyg + row - 1] = wm ? V(-INFINITY) : V(0);
-                tab[2 * nyg + row - 1] = !wm && ghost && ar.ntab_dirichlet ? V(-1) : V(1);
             };
             auto rows = [&](auto* tab) {
-                put(tab, j, false);
-                if (j == 2) put(tab, 1, true);
-                if (j == nyg - 1) put(tab, nyg, true);
+                put(tab, j);
+                if (j == 2) put(tab, 1);
+                if (j == nyg - 1) put(tab, nyg);
             };
             if (ar.fp32) rows((float*)ar.ntab);
             else rows((double*)ar.ntab);
@@ -215,14 +215,13 @@ __global__ void __launch_bounds__(256) k_dsm_nut(DsmNutArgs<T> one, const DsmNut
 
 // The rest of a deferred pass B (DsmBufs::ntab): in the |S| box itself,
 // which pass A filled on the unique cells of rows 2..nyg-1,
-//   the wall-model rows (fuse_wm: 1, 2, nyg-1, nyg; the values k_dsm_nut
-//   writes there) or the ghost rows 1 and nyg (|S| of rows 2 and nyg-1),
-//   every cell with its periodic images; then
+//   the wall model's nu_t on rows 1, 2, nyg-1, nyg (the values k_dsm_nut
+//   writes there with fuse_wm), every cell with its periodic images; then
 //   the x image columns 1, nxg-1, nxg and z image planes 1, nzg-1, nzg of
-//   the other rows, from their unique cells.
+//   rows 3..nyg-2, from their unique cells.
 // No thread reads a cell that another one writes.
 template <typename T>
-__global__ void __launch_bounds__(256) k_dsm_edges(DsmNutArgs<T> a, Dims d, int fuse_wm)
+__global__ void __launch_bounds__(256) k_dsm_edges(DsmNutArgs<T> a, Dims d)
 {
     T* S = a.nut;  // the |S| box (the launcher passes it as nut)
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -230,23 +229,17 @@ __global__ void __launch_bounds__(256) k_dsm_edges(DsmNutArgs<T> a, Dims d, int 
     if (q < 2L * nxz) {
         const int side = (int)(q / nxz), r = (int)(q - (long)side * nxz);
         const int k = r / d.nxg + 1, i = r - (k - 1) * d.nxg + 1;
-        if (fuse_wm) {
-            const T v = wm_nut_at<T>(a.U, a.alpha, a.m, d, a.nu, img_centre(i, d.nxg), k, side, a.sc);
-            const int j0 = side == 0 ? 1 : d.nyg - 1;
-            S[at(d, i, j0, k)] = v;
-            S[at(d, i, j0 + 1, k)] = v;
-        } else {
-            const int jj = side == 0 ? 2 : d.nyg - 1;
-            S[at(d, i, side == 0 ? 1 : d.nyg, k)] = S[at(d, img_centre(i, d.nxg), jj, img_centre(k, d.nzg))];
-        }
+        const T v = wm_nut_at<T>(a.U, a.alpha, a.m, d, a.nu, img_centre(i, d.nxg), k, side, a.sc);
+        const int j0 = side == 0 ? 1 : d.nyg - 1;
+        S[at(d, i, j0, k)] = v;
+        S[at(d, i, j0 + 1, k)] = v;
         return;
     }
     const int npos = 3 * d.nzg + 3 * (d.nxg - 3);
-    const int nrow = fuse_wm ? d.nyg - 4 : d.nyg - 2;
     const long t = q - 2L * nxz;
-    if (t >= (long)npos * nrow) return;
+    if (t >= (long)npos * (d.nyg - 4)) return;
     const int row = (int)(t / npos), pos = (int)(t - (long)row * npos);
-    const int j = row + (fuse_wm ? 3 : 2);
+    const int j = row + 3;
     int i, k;
     if (pos < 3 * d.nzg) {
         const int c = pos / d.nzg;
@@ -295,22 +288,19 @@ void launch_dsm(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, 
         ra.eq_nu = eq->nu; ra.eq_kappa = eq->kappa; ra.eq_B = eq->B; ra.eq_yc = eq->yc;
         ra.eq_jb = eq->ws.jb; ra.eq_wb = eq->ws.wb; ra.eq_hb = eq->ws.hb; ra.eq_ht = eq->ws.ht;
     }
-    if (b.ntab) {
-        ra.ntab = b.ntab;
-        ra.ntab_wm = fuse_wm;
-        ra.ntab_dirichlet = dirichlet;
-    }
+    // pass B deferred to the consumer of nu_t (the wall rows from the wall
+    // model only; the step has them so)
+    const bool defer = b.ntab && fuse_wm;
+    if (defer) ra.ntab = b.ntab;
     run_dsm_ratio(ra, nullptr, 1, d, b.nblk, s);
     const DsmNutArgs<T> na{nut, b.smag, b.ratio, U, alpha_x, m, nu, dirichlet, sc};
-    if (b.ntab) {
-        // pass B deferred to the consumer: smag completed in place
+    if (defer) {
         const int npos = 3 * d.nzg + 3 * (d.nxg - 3);
-        const int nrow = fuse_wm ? d.nyg - 4 : d.nyg - 2;
-        const long n = 2L * d.nxg * d.nzg + (long)npos * nrow;
+        const long n = 2L * d.nxg * d.nzg + (long)npos * (d.nyg - 4);
         DsmNutArgs<T> ea = na;
         ea.nut = b.smag;  // completed in place
         PROF_BEGIN("k_dsm_edges", s);
-        k_dsm_edges<T><<<cdiv(n, 256), 256, 0, s>>>(ea, d, fuse_wm);
+        k_dsm_edges<T><<<cdiv(n, 256), 256, 0, s>>>(ea, d);
         PROF_END(s);
     } else {
         run_dsm_nut<T>(na, nullptr, 1, d, fuse_wm, s);
