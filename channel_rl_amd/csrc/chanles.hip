@@ -340,14 +340,21 @@ void enqueue_eddy_viscosity(chan_ctx* c, int buf, int fuse_wm, bool with_dt = fa
     }
 }
 
+// with_mass_flow: the step's last projection, also applying the correction
+// of mass-flow mode 1 (code/time_integration.f90:88-92): its functional is
+// taken of the projected values before they are stored, and the shift goes
+// into the projection's own store (the same values as projecting, then
+// shifting: one pass over U fewer)
 template <typename T>
-void enqueue_projection(chan_ctx* c, int buf)
+void enqueue_projection(chan_ctx* c, int buf, bool with_mass_flow = false)
 {
     T** s = (T**)c->state[buf];
     launch_poisson_forward<T>(s[0], s[1], s[2], c->md, c->d, c->pb, c->stream);
     launch_poisson_solve(c->d, c->pb, c->stream);
     launch_poisson_inverse(c->d, c->pb, c->stream);
-    launch_project<T>(s[0], s[1], s[2], c->pb.p, c->md, c->d, c->stream);
+    if (with_mass_flow)
+        launch_mass_flow_of_projection<T>(s[0], c->pb.p, c->md, c->d, c->part, c->nblk_red, c->sc, c->stream);
+    launch_project<T>(s[0], s[1], s[2], c->pb.p, c->md, c->d, c->stream, with_mass_flow ? c->sc : nullptr);
 }
 
 // code/time_integration.f90:24-96 with three rotating state buffers:
@@ -383,13 +390,14 @@ void enqueue_rk3(chan_ctx* c, int X, bool dt_fused = false)
         launch_rhs_rk<T>(I[0], I[1], I[2], defer ? (const T*)c->dsm[9] : (const T*)c->nut, rhs_tabs<T>(c), c->d,
                          T(c->nu), c->sc, s + 1, Xs[0], Xs[1], Xs[2], (T*)c->F1[0], (T*)c->F1[1], (T*)c->F1[2], O[0],
                          O[1], O[2], c1, c2, c->rhs_shape, c->stream, defer ? (const T*)c->ntab : nullptr);
-        enqueue_projection<T>(c, out[s]);
+        enqueue_projection<T>(c, out[s], s == 2 && c->x_mass_cte == 1);
     }
-    if (c->x_mass_cte == 1 || c->x_mass_cte == 2) {
-        // the correction's final reduction also advances the time
+    // the mass-flow correction's final reduction also advances the time (mode
+    // 1: in the last projection)
+    if (c->x_mass_cte == 2) {
         T** O = (T**)c->state[Y];
         launch_mass_flow_correct<T>(O[0], c->md, c->d, c->part, c->nblk_red, c->sc, c->x_mass_cte, 1, c->stream, 1);
-    } else {
+    } else if (c->x_mass_cte != 1) {
         k_advance_time<<<1, 1, 0, c->stream>>>(c->sc);
     }
 }

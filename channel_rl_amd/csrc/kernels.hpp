@@ -152,6 +152,9 @@ struct MassArgs {
     double* part;
     Scalars* sc;
     Metric<double> m;
+    // mode 1 with p set: U is the velocity before the projection by p, and
+    // the functional is taken of the projected values (k_project's arithmetic)
+    const double* p;
 };
 template <typename T>
 struct DtArgs {
@@ -219,6 +222,11 @@ void launch_mass_flow_init(const T* U, const Metric<double>& md, const Dims& d, 
 template <typename T>
 void launch_mass_flow_correct(T* U, const Metric<double>& md, const Dims& d, double* part, int nblk, Scalars* sc,
                               int mode, int apply, hipStream_t s, int advance);
+// the functional of mode 1 over the projection of U by p, before it is
+// applied (the shift then goes into the projection: launch_project's shift)
+template <typename T>
+void launch_mass_flow_of_projection(const T* U, const double* p, const Metric<double>& md, const Dims& d,
+                                    double* part, int nblk, Scalars* sc, hipStream_t s);
 
 template <typename T>
 void launch_dt(const T* U, const T* V, const T* W, const Metric<double>& md, const Dims& d, double* part,
@@ -285,6 +293,9 @@ struct ProjArgs {
     T *U, *V, *W;
     const double* p;
     Metric<double> m;
+    // with shift set, U also takes the mass-flow shift sc->last_dPdx
+    // (code/time_integration.f90:91) in the same store
+    const Scalars* shift;
 };
 // grouped forms of the projection's stages.  pb0 (forward) and `one`
 // (solve, inverse) are any member's buffers in a grouped launch: the group's
@@ -332,6 +343,7 @@ void launch_wm_mlp(const void* net, const T* U, const T* W, double* alpha, const
                    int x_mass_cte, double max_err, double* stats, hipStream_t s);
 
 template <typename T>
-void launch_project(T* U, T* V, T* W, const double* p, const Metric<double>& md, const Dims& d, hipStream_t s);
+void launch_project(T* U, T* V, T* W, const double* p, const Metric<double>& md, const Dims& d, hipStream_t s,
+                    const Scalars* shift = nullptr);
 
 }  // namespace chan

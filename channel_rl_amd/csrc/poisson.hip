@@ -926,7 +926,8 @@ __global__ void __launch_bounds__(256) k_project(ProjArgs<T> one, const ProjArgs
     const long q = at(d, i, j, k);
     const double p0 = p[q];
     // the spacings' reciprocals: within an ulp of the reference's divisions
-    store_img(U, d, 0, i, j, k, T((double)U[q] - (p[at(d, i + 1, j, k)] - p0) * m.rdxg[i + 1]));
+    const T un = T((double)U[q] - (p[at(d, i + 1, j, k)] - p0) * m.rdxg[i + 1]);
+    store_img(U, d, 0, i, j, k, arg.shift ? un + T(arg.shift->last_dPdx) : un);
     if (j <= d.ny - 1)
         store_img(V, d, 1, i, j, k, T((double)V[q] - (p[at(d, i, j + 1, k)] - p0) * m.rdyg[j + 1]));
     store_img(W, d, 2, i, j, k, T((double)W[q] - (p[at(d, i, j, k + 1)] - p0) * m.rdzg[k + 1]));
@@ -943,9 +944,10 @@ void run_project(const ProjArgs<T>& one, const ProjArgs<T>* tab, int nmem, const
 }
 
 template <typename T>
-void launch_project(T* U, T* V, T* W, const double* p, const Metric<double>& md, const Dims& d, hipStream_t s)
+void launch_project(T* U, T* V, T* W, const double* p, const Metric<double>& md, const Dims& d, hipStream_t s,
+                    const Scalars* shift)
 {
-    run_project<T>(ProjArgs<T>{U, V, W, p, md}, nullptr, 1, d, s);
+    run_project<T>(ProjArgs<T>{U, V, W, p, md, shift}, nullptr, 1, d, s);
 }
 
 template void launch_poisson_forward<double>(const double*, const double*, const double*, const Metric<double>&,
@@ -959,8 +961,8 @@ template void run_poisson_forward<float>(const FwdArgs<float>&, const FwdArgs<fl
 template void run_project<double>(const ProjArgs<double>&, const ProjArgs<double>*, int, const Dims&, hipStream_t);
 template void run_project<float>(const ProjArgs<float>&, const ProjArgs<float>*, int, const Dims&, hipStream_t);
 template void launch_project<double>(double*, double*, double*, const double*, const Metric<double>&, const Dims&,
-                                     hipStream_t);
+                                     hipStream_t, const Scalars*);
 template void launch_project<float>(float*, float*, float*, const double*, const Metric<double>&, const Dims&,
-                                    hipStream_t);
+                                    hipStream_t, const Scalars*);
 
 }  // namespace chan

@@ -465,12 +465,18 @@ __global__ void k_dpx_partial(MassArgs<T> one, const MassArgs<T>* tab, Dims d, i
     const GBlk g = gblk(1);
     const MassArgs<T> ar = member_args(one, tab, g.m);
     const T* U = ar.U;
+    const double* p = ar.p;
     const int n1 = d.nx - 2, n3 = d.nzp, tot = n1 * n3;
+    // U at face i of row j, or its projection by p as k_project computes it
+    auto u = [&](int i, int j, int k) {
+        const long q = at(d, i, j, k);
+        return p ? T((double)U[q] - (p[at(d, i + 1, j, k)] - p[q]) * ar.m.rdxg[i + 1]) : U[q];
+    };
     double s = 0.0, c = 0.0;
     for (int t = g.x * RED_THREADS + threadIdx.x; t < tot; t += nblk * RED_THREADS) {
         const int kq = t / n1;
         const int i = t - kq * n1 + 2, k = kq + 2;
-        s += 0.5 * ((double)U[at(d, i, 3, k)] + (double)U[at(d, i, d.nyg - 2, k)]);
+        s += 0.5 * ((double)u(i, 3, k) + (double)u(i, d.nyg - 2, k));
         c += 1.0;
     }
     block_sum2(s, c, ar.part, nblk, g.x);
@@ -570,6 +576,14 @@ void launch_mass_flow_correct(T* U, const Metric<double>& md, const Dims& d, dou
                               int mode, int apply, hipStream_t s, int advance)
 {
     run_mass_flow_correct<T>(MassArgs<T>{U, part, sc, md}, nullptr, 1, d, nblk, mode, apply, advance, s);
+}
+
+template <typename T>
+void launch_mass_flow_of_projection(const T* U, const double* p, const Metric<double>& md, const Dims& d,
+                                    double* part, int nblk, Scalars* sc, hipStream_t s)
+{
+    // mode 1, no shift applied here, the step's time advanced
+    run_mass_flow_correct<T>(MassArgs<T>{const_cast<T*>(U), part, sc, md, p}, nullptr, 1, d, nblk, 1, 0, 1, s);
 }
 
 // code/time_integration.f90:102-150 (max of dx/|u| as the reference; max is
@@ -841,6 +855,8 @@ void launch_wall_shear(const T* U, const T* nut, const Metric<double>& md, const
                                            Scalars*, hipStream_t);                                              \
     template void launch_mass_flow_correct<T>(T*, const Metric<double>&, const Dims&, double*, int, Scalars*,   \
                                               int, int, hipStream_t, int);                                      \
+    template void launch_mass_flow_of_projection<T>(const T*, const double*, const Metric<double>&, const Dims&, \
+                                                    double*, int, Scalars*, hipStream_t);                        \
     template void launch_dt<T>(const T*, const T*, const T*, const Metric<double>&, const Dims&, double*, int,  \
                                Scalars*, const double*, hipStream_t);                                           \
     template void launch_divergence_check<T>(const T*, const T*, const T*, const Metric<double>&, const Dims&, \
