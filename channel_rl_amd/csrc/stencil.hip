@@ -8,6 +8,7 @@
 #include "kernels.hpp"
 #include "prof.hpp"
 #include "wallmodel.hpp"
+#include <algorithm>
 #include <type_traits>
 
 namespace chan {
@@ -582,8 +583,10 @@ template <typename T>
 void launch_mass_flow_of_projection(const T* U, const double* p, const Metric<double>& md, const Dims& d,
                                     double* part, int nblk, Scalars* sc, hipStream_t s)
 {
-    // mode 1, no shift applied here, the step's time advanced
-    run_mass_flow_correct<T>(MassArgs<T>{const_cast<T*>(U), part, sc, md, p}, nullptr, 1, d, nblk, 1, 0, 1, s);
+    // mode 1, no shift applied here, the step's time advanced; one lane per
+    // term of the functional ((nx-2) nzp of them), at most nblk blocks
+    const int nb = std::min(nblk, cdiv((long)(d.nx - 2) * d.nzp, RED_THREADS));
+    run_mass_flow_correct<T>(MassArgs<T>{const_cast<T*>(U), part, sc, md, p}, nullptr, 1, d, nb, 1, 0, 1, s);
 }
 
 // code/time_integration.f90:102-150 (max of dx/|u| as the reference; max is
