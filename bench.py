@@ -46,7 +46,11 @@ ALG_FLOPS = {"k_dsm_fused": 519}
 #              registers between the sweeps): 2
 #   k_fftx_inv: reads p-hat (1), writes p (1)
 #   k_project: reads p, U,V,W, writes U,V,W: 7
-# A whole step moves 116 values per cell (928 B FP64, 584 B FP32 bulk).
+# A whole step of the dynamic model (mass-flow mode 1) moves 107 values per
+# cell (856 B FP64, 548 B FP32 bulk): compute_dt rides in the first pass A,
+# pass B runs in the last substep only (the RHS converts |S| in the others)
+# and the mass-flow shift in the last projection.  step_alg_gbs sums these
+# per launch over the launches the profiled steps make.
 ALG_VALUES = {
     "k_rhs_rk": (12, 0),
     "k_dsm_fused": (4, 0),
@@ -58,6 +62,10 @@ ALG_VALUES = {
     "k_project": (6, 1),
     # the AMD eddy viscosity (LES model 3): reads U,V,W, writes nu_t
     "k_amd": (4, 0),
+    # compute_dt on its own (reads U,V,W) and the mass-flow shift on its own
+    # (reads and writes U), where the step does not fold them
+    "k_dt_partial": (3, 0),
+    "k_shift_u": (2, 0),
 }
 
 
@@ -376,7 +384,11 @@ def main():
     # (where counted) and the HBM bytes of the counter passes, each as a
     # rate and as a fraction of the HBM peak
     stages = {}
+    step_alg_bytes = 0
     for name, (ms, n) in ranked:
+        if name in ALG_VALUES:
+            b0, b1 = ALG_VALUES[name]
+            step_alg_bytes += cells * (b0 * bulk_b + b1 * 8) * n / args.profile_steps
         us = 1e3 * ms / n
         st = {"us_per_launch": round(us, 2), "launches_per_step": round(n / args.profile_steps, 2)}
         if name in ALG_VALUES:
@@ -398,7 +410,8 @@ def main():
                 "avg_launch_us": round(avg_s * 1e6, 2), "alg_bytes_per_launch": alg_bytes,
                 "step_ms_profiled": round(tot / args.profile_steps, 3),
                 "copy_kernel_gbs": round(copy_gbs, 1),
-                "step_alg_gbs": round(cells * (928 if prec == cr.FP64 else 584) / (ms_step * 1e-3) / 1e9, 1)}
+                "step_alg_gbs": round(step_alg_bytes / (ms_step * 1e-3) / 1e9, 1),
+                "step_alg_bytes_per_cell": round(step_alg_bytes / cells, 1)}
     if dom in ALG_FLOPS:
         # the same kernel against the vector ALU peak of its arithmetic type
         # (no MFMA on this path): its bound is issue and latency, not HBM
