@@ -128,11 +128,21 @@ __device__ __forceinline__ T amd_gen(const A& F, const AmdXY<T>& g, const AmdZ<T
 // Dirichlet_nu_t (1 mirror, 0 copy, 2 mirror plus the offset
 // 0.38 sqrt(dPdx) (y(2)-y(1)), code/subgrid.f90:864-871).  With fuse_wm the
 // four wall rows are left to k_amd_wall_rows.
+// With DT (dt.part set), the launch also takes compute_dt's maxima over its
+// cells (code/time_integration.f90:113-121): per column the smallest |U|
+// and |V| (a correctly rounded quotient falls as its divisor grows, so the
+// largest dx/|U| is dx over the smallest |U|, exactly) and per plane the
+// largest dz/|W|, with the terms of the periodic image column nxg-1 and
+// plane nzg-1 from the canonical values they copy; one slot per block.
 static constexpr int AMD_KMAX = 32;
-template <typename T, int TX, int TY>
+struct AmdDt {
+    double* part;  // [3][blocks of the launch]
+    const double *xg, *yg, *zg;
+};
+template <typename T, int TX, int TY, bool DT>
 __global__ void __launch_bounds__(TX* TY) k_amd(const T* __restrict__ U, const T* __restrict__ V,
                                                 const T* __restrict__ W, T* __restrict__ nut, Metric<T> m, Dims d,
-                                                int dirichlet, const Scalars* sc, int fuse_wm, int kchunk)
+                                                int dirichlet, const Scalars* sc, int fuse_wm, int kchunk, AmdDt dt)
 {
     constexpr int PX = TX + 2, PY = TY + 2, NP = PX * PY, NT = TX * TY, NL = (NP + NT - 1) / NT;
     __shared__ T sh[3][4][NP];  // [field][slot][position]
@@ -184,7 +194,10 @@ __global__ void __launch_bounds__(TX* TY) k_amd(const T* __restrict__ U, const T
         zt[1][q] = m.rdzg[kk];
         zt[2][q] = m.rdzg[kk + 1];
     }
-    const bool act = i <= d.nxg - 2 && j <= d.nyg - 1 && !(fuse_wm && (j == 2 || j == d.nyg - 1));
+    const bool own = i <= d.nxg - 2 && j <= d.nyg - 1;  // a unique centre
+    const bool act = own && !(fuse_wm && (j == 2 || j == d.nyg - 1));
+    T dt_mu = T(INFINITY), dt_mv = T(INFINITY);
+    double dt_lw = 0.0;
     const bool ghost = !fuse_wm && (j == 2 || j == d.nyg - 1);
     const T off = dirichlet == 2 ? T(2.0 * 0.38 * sqrt(sc->dPdx) * (double)(m.y[2] - m.y[1]) / 2.0) : T(0);
     int xs[3], nxs = 1;
@@ -203,6 +216,13 @@ __global__ void __launch_bounds__(TX* TY) k_amd(const T* __restrict__ U, const T
         sstore(k + 1);
         if (k + 2 <= ke + 1) gload(k + 2);
         __syncthreads();
+        if (DT && own) {
+            dt_mu = fmin(dt_mu, fabs(F(0, i, j, k)));
+            dt_mv = fmin(dt_mv, fabs(F(1, i, j, k)));
+            const double aw = fabs((double)F(2, i, j, k));
+            dt_lw = fmax(dt_lw, (dt.zg[k + 1] - dt.zg[k]) / aw);
+            if (k == 2) dt_lw = fmax(dt_lw, (dt.zg[d.nzg] - dt.zg[d.nzg - 1]) / aw);
+        }
         if (act) {
             const AmdZ<T> gz{zt[0][k - kb], zt[1][k - kb], zt[2][k - kb]};
             const T val = amd_gen<T>(F, g, gz, i, j, k);
@@ -220,24 +240,83 @@ __global__ void __launch_bounds__(TX* TY) k_amd(const T* __restrict__ U, const T
                 }
         }
     }
+    if constexpr (DT) {
+        // the block's maxima: per lane, per wave (butterfly), over the waves
+        double m3[3] = {0.0, 0.0, dt_lw};
+        if (own) {
+            m3[0] = (dt.xg[i + 1] - dt.xg[i]) / (double)dt_mu;
+            if (i == 2) m3[0] = fmax(m3[0], (dt.xg[d.nxg] - dt.xg[d.nxg - 1]) / (double)dt_mu);
+            m3[1] = (dt.yg[j + 1] - dt.yg[j]) / (double)dt_mv;
+        }
+        for (int s = 32; s > 0; s >>= 1)
+            for (int c = 0; c < 3; c++) m3[c] = fmax(m3[c], __shfl_xor(m3[c], s, 64));
+        __shared__ double wmax[NT / 64][3];
+        if ((tid & 63) == 0)
+            for (int c = 0; c < 3; c++) wmax[tid >> 6][c] = m3[c];
+        __syncthreads();
+        if (tid < 3) {
+            double v = 0.0;
+            for (int w = 0; w < NT / 64; w++) v = fmax(v, wmax[w][tid]);
+            const int nb = gb.gx * gb.gy * gb.gz;
+            dt.part[(long)tid * nb + (gb.z * gb.gy + gb.y) * gb.gx + gb.x] = v;
+        }
+    }
 }
 
-// the four wall rows from the wall model, every box cell (i, k) of them
-// (code/wallmodel.f90:23-46 at x face pairs around the canonical centre)
+// The four wall rows from the wall model, every box cell (i, k) of them
+// (code/wallmodel.f90:23-46 at the x faces around the canonical centre):
+// one block per (z centre k, wall), whose threads first compute the
+// equilibrium wall stress of the row's faces when eq.alpha is set (the
+// values k_equilibrium_alpha computes: each face by one thread, then a
+// barrier), then nu_t of rows 1, 2 or nyg-1, nyg.  With dtpart set, one
+// more block reduces k_amd's compute_dt maxima and sets sc->dt as
+// k_dt_final does (the launcher sets it only when the wall rows do not read
+// dt: no wall viscosity limit).
+struct AmdWallDt {
+    const double* part;
+    int nblk;
+    const double* mins3;
+};
 template <typename T>
-__global__ void __launch_bounds__(256) k_amd_wall_rows(const T* U, T* nut, const double* alpha, Metric<T> m, Dims d,
-                                                       T nu, const Scalars* sc)
+__global__ void __launch_bounds__(256) k_amd_wall(const T* U, T* nut, double* alpha, Metric<T> m, Dims d, T nu,
+                                                  Scalars* sc, EqAlphaArgs<T> eq, AmdWallDt dt)
 {
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= 4L * d.nxg * d.nzg) return;
-    const int i = (int)(q % d.nxg) + 1, r = (int)((q / d.nxg) % 4), k = (int)(q / (4L * d.nxg)) + 1;
-    const int j = r < 2 ? r + 1 : d.nyg - 3 + r;
-    nut[at(d, i, j, k)] = wm_nut_at<T>(U, alpha, m, d, nu, img_centre(i, d.nxg), k, j <= 2 ? 0 : 1, sc);
+    const int t = threadIdx.x;
+    if ((int)blockIdx.x == 2 * d.nzg) {
+        double m3[3] = {0.0, 0.0, 0.0};
+        for (int q = t; q < dt.nblk; q += blockDim.x)
+            for (int c = 0; c < 3; c++) m3[c] = fmax(m3[c], dt.part[(long)c * dt.nblk + q]);
+        for (int s = 32; s > 0; s >>= 1)
+            for (int c = 0; c < 3; c++) m3[c] = fmax(m3[c], __shfl_xor(m3[c], s, 64));
+        __shared__ double wmax[4][3];
+        if ((t & 63) == 0)
+            for (int c = 0; c < 3; c++) wmax[t >> 6][c] = m3[c];
+        __syncthreads();
+        if (t == 0) {
+            for (int w = 1; w < (int)blockDim.x / 64; w++)
+                for (int c = 0; c < 3; c++) m3[c] = fmax(m3[c], wmax[w][c]);
+            sc->dt = dt_from_maxima(m3[0], m3[1], m3[2], sc, dt.mins3);
+        }
+        return;
+    }
+    const int k = (int)blockIdx.x % d.nzg + 1, wall = (int)blockIdx.x / d.nzg;
+    if (eq.alpha) {
+        for (int f = t; f < d.nx; f += blockDim.x)
+            eq_alpha_cell<T>(f + d.nx * ((k - 1) + d.nzg * wall), U, eq.alpha, d, eq.nu, eq.kappa, eq.B, eq.yc, eq.ws);
+        __syncthreads();
+    }
+    const int j0 = wall == 0 ? 1 : d.nyg - 1;
+    for (int i = t + 1; i <= d.nxg; i += blockDim.x) {
+        const T v = wm_nut_at<T>(U, alpha, m, d, nu, img_centre(i, d.nxg), k, wall, sc);
+        nut[at(d, i, j0, k)] = v;
+        nut[at(d, i, j0 + 1, k)] = v;
+    }
 }
 
 template <typename T>
 void launch_amd(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, const Dims& d, int dirichlet,
-                const Scalars* sc, int fuse_wm, const double* alpha, T nu, hipStream_t s)
+                const Scalars* sc, int fuse_wm, const double* alpha, T nu, hipStream_t s, const EqAlphaArgs<T>* eq,
+                double* dtpart, const Metric<double>* md, const double* mins3)
 {
     constexpr int TX = 32, TY = 8;
     const int nk = d.nzg - 3;
@@ -246,20 +325,39 @@ void launch_amd(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, 
     int kc = (int)std::min<long>(AMD_KMAX, std::max<long>(4, (tiles * nk + 2047) / 2048));
     kc = std::min(kc, nk);
     dim3 blk(TX, TY, 1), grd(cdiv(d.nxg - 3, TX), cdiv(d.nyg - 2, TY), (unsigned)cdiv(nk, kc));
+    // compute_dt rides along only with the wall rows' launch after it
+    const bool with_dt = dtpart && fuse_wm;
+    const AmdDt adt{with_dt ? dtpart : nullptr, md ? md->xg : nullptr, md ? md->yg : nullptr, md ? md->zg : nullptr};
     PROF_BEGIN("k_amd", s);
-    k_amd<T, TX, TY><<<grd, blk, 0, s>>>(U, V, W, nut, m, d, dirichlet, sc, fuse_wm, kc);
+    if (with_dt) k_amd<T, TX, TY, true><<<grd, blk, 0, s>>>(U, V, W, nut, m, d, dirichlet, sc, fuse_wm, kc, adt);
+    else k_amd<T, TX, TY, false><<<grd, blk, 0, s>>>(U, V, W, nut, m, d, dirichlet, sc, fuse_wm, kc, adt);
     PROF_END(s);
     if (fuse_wm) {
-        const long n = 4L * d.nxg * d.nzg;
-        PROF_BEGIN("k_amd_wall_rows", s);
-        k_amd_wall_rows<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(U, nut, alpha, m, d, nu, sc);
+        EqAlphaArgs<T> ea{};
+        if (eq) ea = *eq;
+        const AmdWallDt wdt{dtpart, (int)(grd.x * grd.y * grd.z), mins3};
+        PROF_BEGIN("k_amd_wall", s);
+        k_amd_wall<T><<<2 * d.nzg + (with_dt ? 1 : 0), 256, 0, s>>>(U, nut, const_cast<double*>(alpha), m, d, nu,
+                                                                    const_cast<Scalars*>(sc), ea, wdt);
         PROF_END(s);
     }
 }
 
+// blocks of k_amd (the compute_dt partials' row length)
+int amd_blocks(const Dims& d)
+{
+    const int nk = d.nzg - 3;
+    const long tiles = (long)cdiv(d.nxg - 3, 32) * cdiv(d.nyg - 2, 8);
+    int kc = (int)std::min<long>(AMD_KMAX, std::max<long>(4, (tiles * nk + 2047) / 2048));
+    kc = std::min(kc, nk);
+    return (int)(tiles * cdiv(nk, kc));
+}
+
 template void launch_amd<double>(const double*, const double*, const double*, double*, const Metric<double>&,
-                                 const Dims&, int, const Scalars*, int, const double*, double, hipStream_t);
+                                 const Dims&, int, const Scalars*, int, const double*, double, hipStream_t,
+                                 const EqAlphaArgs<double>*, double*, const Metric<double>*, const double*);
 template void launch_amd<float>(const float*, const float*, const float*, float*, const Metric<float>&, const Dims&,
-                                int, const Scalars*, int, const double*, float, hipStream_t);
+                                int, const Scalars*, int, const double*, float, hipStream_t,
+                                const EqAlphaArgs<float>*, double*, const Metric<double>*, const double*);
 
 }  // namespace chan

@@ -124,6 +124,7 @@ struct chan_ctx {
     // kernel (DsmBufs::ntab): its row coefficients, [2][nyg] of the bulk type
     void* ntab = nullptr;
     bool defer_nut = true;
+    double wm_limit = 0.0;  // the wall viscosity limit (a copy of sc->wm_nu_limit)
     bool use_graph = true;
     int rhs_shape = 0;  // index into the RHS kernel's tile shapes
     // learned wall model
@@ -332,8 +333,10 @@ void enqueue_eddy_viscosity(chan_ctx* c, int buf, int fuse_wm, bool with_dt = fa
         launch_dsm<T>(s[0], s[1], s[2], (T*)c->nut, metric<T>(c), c->d, b, c->Dirichlet_nu_t == 1, fuse_wm,
                       c->alpha, nu, c->sc, c->stream, with_eq ? &eq : nullptr);
     } else if (c->LES_model == 3) {
+        const EqAlphaArgs<T> eq{s[0], c->alpha, c->nu, wall_rows(c, c->h_wm), c->kappa, c->Bc,
+                                loglaw_crossover(c->kappa, c->Bc)};
         launch_amd<T>(s[0], s[1], s[2], (T*)c->nut, metric<T>(c), c->d, c->Dirichlet_nu_t, c->sc, fuse_wm, c->alpha,
-                      nu, c->stream);
+                      nu, c->stream, with_eq ? &eq : nullptr, with_dt ? c->dtpart : nullptr, &c->md, c->mins3);
     } else {
         k_fill<T><<<cdiv(c->d.nbox, 256), 256, 0, c->stream>>>((T*)c->nut, c->d.nbox, T(0));
         if (fuse_wm) launch_wall_model<T>(s[0], (T*)c->nut, c->alpha, metric<T>(c), c->d, nu, c->sc, c->stream);
@@ -370,7 +373,9 @@ void enqueue_rk3(chan_ctx* c, int X, bool dt_fused = false)
     for (int s = 0; s < 3; s++) {
         T** I = (T**)c->state[in[s]];
         T** O = (T**)c->state[out[s]];
-        const bool eq_in_dsm = c->wall_mode == CHAN_WALL_EQUILIBRIUM && c->LES_model == 2;
+        // the equilibrium wall stress rides in a launch of the eddy viscosity
+        // (the DSM ratio launch, or the AMD wall rows' launch)
+        const bool eq_in_dsm = c->wall_mode == CHAN_WALL_EQUILIBRIUM && (c->LES_model == 2 || c->LES_model == 3);
         if (c->wall_mode == CHAN_WALL_EQUILIBRIUM && !eq_in_dsm) {
             launch_equilibrium_alpha<T>(I[0], c->alpha, c->d, c->nu, wall_rows(c, c->h_wm), c->kappa, c->Bc, c->stream);
         } else if (c->wall_mode == CHAN_WALL_MLP) {
@@ -483,7 +488,15 @@ void set_dsm_shape(chan_ctx* c, int q)
 // model (its pass reads every cell compute_dt reads) and a wall stress that
 // does not read dt before that pass (the learned model's pressure-gradient
 // feature does)
-bool dt_fusable(const chan_ctx* c) { return c->LES_model == 2 && c->wall_mode != CHAN_WALL_MLP; }
+// compute_dt folded into the step's first eddy-viscosity launches: the DSM
+// pass A with its ratio launch, or the AMD kernel with its wall rows' launch
+// (whose wall viscosity limit, if any, reads dt in that launch); not with the
+// learned wall model, whose kernel runs before them and reads dt
+bool dt_fusable(const chan_ctx* c)
+{
+    if (c->wall_mode == CHAN_WALL_MLP) return false;
+    return c->LES_model == 2 || (c->LES_model == 3 && c->wm_limit == 0.0);
+}
 
 // the address of this device's copy of `scratch`, looked up once per device
 // with that device current (the caller has made it current)
@@ -695,7 +708,10 @@ int chan_create(const chan_config* cfg, chan_ctx** out)
         nblk_max = std::max(nblk_max, (size_t)c->nblk_dsm);
         nblk_dt_max = std::max(nblk_dt_max, (size_t)c->nblk_dt);
     }
-    if (c->LES_model == 2 && (rc = dalloc(c, (void**)&c->dtpart, 3 * nblk_dt_max * sizeof(double)))) return rc;
+    if (c->LES_model == 3) nblk_dt_max = amd_blocks(d);
+    if ((c->LES_model == 2 || c->LES_model == 3) &&
+        (rc = dalloc(c, (void**)&c->dtpart, 3 * nblk_dt_max * sizeof(double))))
+        return rc;
     // reduction blocks: about eight (y, z) rows each, at most 1024
     c->nblk_red = (int)std::max(64L, std::min(1024L, (long)(d.nyg - 2) * (d.nzg - 2) / 8));
     size_t npart = std::max((size_t)(d.nyg + 1) * 2 * nblk_max, (size_t)3 * c->nblk_red);
@@ -1059,6 +1075,7 @@ int chan_set_wall_viscosity_limit(chan_ctx* c, double limit)
     HIPCHK(c, hipMemcpy(&s0, c->sc, sizeof(s0), hipMemcpyDeviceToHost));
     s0.wm_nu_limit = limit;
     HIPCHK(c, hipMemcpy(c->sc, &s0, sizeof(s0), hipMemcpyHostToDevice));
+    c->wm_limit = limit;
     return CHAN_OK;
 }
 

@@ -95,6 +95,20 @@ struct Scalars {
     double maxdiv, maxabsdiv, last_dPdx, pad2;
 };
 
+// code/time_integration.f90:123-148 from the three maxima of dx/|u|: the
+// convective and viscous limits, the cap, a negative CFL as a fixed step
+__device__ inline double dt_from_maxima(double mu, double mv, double mw, const Scalars* sc, const double* mins3)
+{
+    const double cfl = sc->cfl, nu = sc->nu;
+    double dconv = fmin(fmin(cfl * mu, cfl * mv), cfl * mw);
+    double dvis = fmin(fmin(cfl * (mins3[0] * mins3[0]) / nu, cfl * (mins3[1] * mins3[1]) / nu),
+                       cfl * (mins3[2] * mins3[2]) / nu);
+    double dt = fmin(dconv, dvis);
+    dt = fmin(dt, sc->dt_max);
+    if (cfl < 0) dt = -cfl;
+    return dt;
+}
+
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
 // z-chunk length in [lo, hi] for a kernel whose blocks each march `kc` of

@@ -1,6 +1,7 @@
 // kernels.hpp -- host-callable launchers of the channel kernels.
 #pragma once
 #include "common.hpp"
+#include "wallmodel.hpp"
 
 struct chan_mlp;
 
@@ -65,10 +66,17 @@ void launch_dsm(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, 
                 DsmBufs<T>& b, int dirichlet, int fuse_wm, const double* alpha_x, T nu, const Scalars* sc,
                 hipStream_t s, const EqAlphaArgs<T>* eq = nullptr);
 
-// AMD eddy viscosity (amd.hip), dirichlet = Dirichlet_nu_t (0, 1, 2)
+// AMD eddy viscosity (amd.hip), dirichlet = Dirichlet_nu_t (0, 1, 2).  With
+// fuse_wm, eq set: the wall rows' launch also computes the equilibrium wall
+// stress into alpha first; dtpart set (fuse_wm, no wall viscosity limit):
+// the two launches also take compute_dt ([3][amd_blocks] partials; md's
+// centre coordinates, mins3 the smallest spacings) and set sc->dt
 template <typename T>
 void launch_amd(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, const Dims& d, int dirichlet,
-                const Scalars* sc, int fuse_wm, const double* alpha, T nu, hipStream_t s);
+                const Scalars* sc, int fuse_wm, const double* alpha, T nu, hipStream_t s,
+                const EqAlphaArgs<T>* eq = nullptr, double* dtpart = nullptr, const Metric<double>* md = nullptr,
+                const double* mins3 = nullptr);
+int amd_blocks(const Dims& d);
 
 // fused pass A of the dynamic Smagorinsky model (dsm_fused.hip): z-chunk
 // length for this grid and device, and the partial-sum blocks per row it gives
@@ -87,14 +95,6 @@ template <typename T>
 void launch_wall_model(const T* U, T* nut, const double* alpha_x, const Metric<T>& m, const Dims& d, T nu,
                        const Scalars* sc, hipStream_t s);
 
-// Where the equilibrium wall model samples U: h_wm above each wall,
-// interpolated between centre rows jb and jb+1 with weight wb on row jb
-// (h_wm = 0: row 2 itself, wb = 1); hb is the bottom sampling height, ht the
-// top one.  Depends on the grid only, so the host computes it.
-struct WallRows {
-    int jb;
-    double wb, hb, ht;
-};
 
 template <typename T>
 void launch_equilibrium_alpha(const T* U, double* alpha_x, const Dims& d, double nu, const WallRows& ws,

@@ -81,16 +81,12 @@ void launch_bc(T* F, const Dims& d, int kind, hipStream_t s)
 // --------------------------------------------------------------- SGS: DSM
 // Pass A (filters, L_ij, M_ij, plane partial sums, |S|) is dsm_fused.hip.
 
-__device__ inline double dt_from_maxima(double mu, double mv, double mw, const Scalars* sc, const double* mins3);
 
 // <L:M>/<M:M> per row: one wave per row, the blocks' partials summed in a
 // fixed order (strided per lane, then a butterfly across the wave), so the
 // result does not depend on scheduling; no LDS and no barrier.  With the
 // DSM pass's compute_dt maxima (ar.dtpart), a last block reduces them and
 // sets the step's dt (the maximum does not depend on the order).
-template <typename T>
-__device__ inline void eq_alpha_cell(int q, const T* U, double* alpha, const Dims& d, double nu, double kappa,
-                                     double B, double yc, const WallRows& ws);
 
 __global__ void __launch_bounds__(64) k_dsm_ratio(DsmRatioArgs one, const DsmRatioArgs* tab, int nblk, Dims d)
 {
@@ -337,30 +333,6 @@ void launch_wall_model(const T* U, T* nut, const double* alpha_x, const Metric<T
 
 // ------------------------------------------------ equilibrium wall stress
 
-// one thread per (x face, z centre, wall)
-template <typename T>
-__device__ inline void eq_alpha_cell(int q, const T* U, double* alpha, const Dims& d, double nu, double kappa,
-                                     double B, double yc, const WallRows& ws)
-{
-    const int i = q % d.nx + 1, r = q / d.nx;
-    const int k = r % d.nzg + 1, wall = r / d.nzg;
-    const int jb = ws.jb;
-    const double wb = ws.wb;
-    double uw, h;
-    if (wall == 0) {
-        uw = wb * (double)U[at(d, i, jb, k)] + (1.0 - wb) * (double)U[at(d, i, jb + 1, k)];
-        h = ws.hb;
-    } else {
-        const int jt = d.nyg + 1 - jb;
-        uw = wb * (double)U[at(d, i, jt, k)] + (1.0 - wb) * (double)U[at(d, i, jt - 1, k)];
-        h = ws.ht;
-    }
-    // Newton starts from the cell's previous u_tau (|alpha_x| = u_tau^2): the
-    // flow moves little in a step, so a few iterations reach the same root
-    const long ia = (i - 1) + (long)d.nx * (wall + 2L * (k - 1));
-    const double t = loglaw_utau(uw, h, nu, kappa, B, yc, sqrt(fabs(alpha[ia])));
-    alpha[ia] = (uw >= 0 ? 1.0 : -1.0) * t * t;
-}
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_equilibrium_alpha(EqAlphaArgs<T> one, const EqAlphaArgs<T>* tab, Dims d)
@@ -658,19 +630,6 @@ __global__ void k_dt_partial(DtArgs<T> one, const DtArgs<T>* tab, Dims d, int nb
     block_max3(lu, lv, lw, a.part, nblk, g.x);
 }
 
-// code/time_integration.f90:123-148 from the three maxima of dx/|u|: the
-// convective and viscous limits, the cap, a negative CFL as a fixed step
-__device__ inline double dt_from_maxima(double mu, double mv, double mw, const Scalars* sc, const double* mins3)
-{
-    const double cfl = sc->cfl, nu = sc->nu;
-    double dconv = fmin(fmin(cfl * mu, cfl * mv), cfl * mw);
-    double dvis = fmin(fmin(cfl * (mins3[0] * mins3[0]) / nu, cfl * (mins3[1] * mins3[1]) / nu),
-                       cfl * (mins3[2] * mins3[2]) / nu);
-    double dt = fmin(dconv, dvis);
-    dt = fmin(dt, sc->dt_max);
-    if (cfl < 0) dt = -cfl;
-    return dt;
-}
 
 template <typename T>
 __global__ void k_dt_final(DtArgs<T> one, const DtArgs<T>* tab, int nblk)

@@ -112,4 +112,39 @@ __device__ inline WallSample wall_sample(const T* U, const T* W, const Metric<do
     return s;
 }
 
+// Where the equilibrium wall model samples U: h_wm above each wall,
+// interpolated between centre rows jb and jb+1 with weight wb on row jb
+// (h_wm = 0: row 2 itself, wb = 1); hb is the bottom sampling height, ht the
+// top one.  Depends on the grid only, so the host computes it.
+struct WallRows {
+    int jb;
+    double wb, hb, ht;
+};
+
+// The equilibrium wall stress of one wall cell, q = (x face) + nx ((z centre)
+// + nzg wall), from U at the matching height, in alpha_x (code/wallmodel.f90:23-46)
+template <typename T>
+__device__ inline void eq_alpha_cell(int q, const T* U, double* alpha, const Dims& d, double nu, double kappa,
+                                     double B, double yc, const WallRows& ws)
+{
+    const int i = q % d.nx + 1, r = q / d.nx;
+    const int k = r % d.nzg + 1, wall = r / d.nzg;
+    const int jb = ws.jb;
+    const double wb = ws.wb;
+    double uw, h;
+    if (wall == 0) {
+        uw = wb * (double)U[at(d, i, jb, k)] + (1.0 - wb) * (double)U[at(d, i, jb + 1, k)];
+        h = ws.hb;
+    } else {
+        const int jt = d.nyg + 1 - jb;
+        uw = wb * (double)U[at(d, i, jt, k)] + (1.0 - wb) * (double)U[at(d, i, jt - 1, k)];
+        h = ws.ht;
+    }
+    // Newton starts from the cell's previous u_tau (|alpha_x| = u_tau^2): the
+    // flow moves little in a step, so a few iterations reach the same root
+    const long ia = (i - 1) + (long)d.nx * (wall + 2L * (k - 1));
+    const double t = loglaw_utau(uw, h, nu, kappa, B, yc, sqrt(fabs(alpha[ia])));
+    alpha[ia] = (uw >= 0 ? 1.0 : -1.0) * t * t;
+}
+
 }  // namespace chan

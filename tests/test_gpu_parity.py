@@ -516,3 +516,47 @@ def test_deferred_dsm_pass_b_bitwise(gpu, precision, monkeypatch):
                 assert np.array_equal(a.get_field(f), b.get_field(f)), (wall, shape, f)
             assert a.scalars() == b.scalars()
             del a, b, chans
+
+
+@pytest.mark.parametrize("precision", [FP64, "fp32"])
+def test_amd_step_folds_compute_dt_and_wall_stress_bitwise(gpu, precision):
+    """With the AMD model (LES_model 3), run_steps folds compute_dt into the
+    first substep's AMD kernel (per-column minima of |U| and |V|, per-plane
+    maxima of dz/|W|, the terms of the image column and plane included) and
+    a last block of the wall rows' launch, which also computes the
+    equilibrium wall stress before the wall rows: three steps give the same
+    dt and fields, bit for bit, as compute_dt + compute_time_step_RK3, with
+    the convective limit binding once through U and once through W (the
+    folded wall stress itself is checked against the oracle by
+    test_rk3_with_amd)."""
+    from channel_rl_amd import FP32_BULK, WALL_EQUILIBRIUM
+    prec = FP64 if precision == FP64 else FP32_BULK
+    for u0, w0 in ((3.0, 50.0), (50.0, 3.0)):
+        case, field = cases.small_case(22, 13, 32, seed=9, LES_model=3)
+        case.nu, case.CFL = 1e-4, 1e-3
+        rng = np.random.default_rng(4)
+        U = np.asfortranarray(u0 * (1.0 + 0.05 * rng.uniform(-1, 1, field[0].shape)))
+        V = np.asfortranarray(0.1 * rng.uniform(-1, 1, field[1].shape))
+        W = np.asfortranarray(w0 * (1.0 + 0.05 * rng.uniform(-1, 1, field[2].shape)))
+        chans = []
+        for _ in range(2):
+            g = Channel(case.x, case.y, case.z, precision=prec, **case.kwargs())
+            g.set_state(U, V, W)
+            g.apply_boundary_conditions()
+            g.compute_projection_step()
+            g.set_scalars(dt=0.0, t=0.0, dPdx=0.0, Qflow_x_0=0.0)
+            g.set_wall_model(WALL_EQUILIBRIUM, h_wm=0.1)
+            chans.append(g)
+        a, b = chans
+        for step in range(3):
+            dt_a = a.compute_dt()
+            a.compute_time_step_RK3()
+            b.run_steps(1)
+            dt_b = b.scalars()["dt"]
+            assert dt_a == dt_b, (u0, w0, step, dt_a, dt_b)
+            assert dt_a < 0.5 * case.CFL * (case.y[1] - case.y[0]) ** 2 / case.nu  # convective limit binds
+        for x, y in zip(a.state(), b.state()):
+            assert np.array_equal(x, y), (u0, w0)
+        assert np.array_equal(a.get_field(NU_T), b.get_field(NU_T))
+        assert np.array_equal(a.get_field(ALPHA_X), b.get_field(ALPHA_X))
+        del a, b, chans
