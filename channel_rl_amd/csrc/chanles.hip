@@ -1631,7 +1631,9 @@ int enqueue_group_step(chan_ensemble* e, const std::vector<int>& idx, TabSink& s
             const WmArgs<T>* t = sink.put(a);
             if (sink.launch) run_wm_mlp<T>(a[0], t, n, d, s);
         }
-        // dynamic Smagorinsky (pass A, plane ratios, pass B with the wall rows)
+        // dynamic Smagorinsky (pass A, plane ratios, pass B with the wall rows;
+        // pass B left to the RHS in substeps 1 and 2, as enqueue_rk3 does)
+        const bool defer = c0->defer_nut && sub < 2;
         {
             std::vector<DsmArgs<T>> a(n);
             std::vector<DsmRatioArgs> r(n);
@@ -1642,8 +1644,10 @@ int enqueue_group_step(chan_ensemble* e, const std::vector<int>& idx, TabSink& s
                 const DsmBufs<T> b = dsm_bufs<T>(c);
                 a[q] = DsmArgs<T>{u[0], u[1], u[2], b};
                 r[q] = DsmRatioArgs{b.part, b.ratio};
-                v[q] = DsmNutArgs<T>{(T*)c->nut, b.smag, b.ratio, u[0], c->alpha, metric<T>(c), T(c->nu),
-                                     c->Dirichlet_nu_t == 1, c->sc};
+                r[q].fp32 = sizeof(T) == 4;
+                if (defer) r[q].ntab = c->ntab;
+                v[q] = DsmNutArgs<T>{defer ? b.smag : (T*)c->nut, b.smag, b.ratio, u[0], c->alpha, metric<T>(c),
+                                     T(c->nu), c->Dirichlet_nu_t == 1, c->sc};
             }
             const DsmArgs<T>* ta = sink.put(a);
             const DsmRatioArgs* tr = sink.put(r);
@@ -1651,7 +1655,8 @@ int enqueue_group_step(chan_ensemble* e, const std::vector<int>& idx, TabSink& s
             if (sink.launch) {
                 run_dsm_fused<T>(a[0], ta, n, d, c0->dsm_kchunk, c0->dsm_shape, s);
                 run_dsm_ratio(r[0], tr, n, d, c0->nblk_dsm, s);
-                run_dsm_nut<T>(v[0], tv, n, d, 1, s);
+                if (defer) run_dsm_edges<T>(v[0], tv, n, d, s);
+                else run_dsm_nut<T>(v[0], tv, n, d, 1, s);
             }
         }
         // RHS + RK combine
@@ -1666,7 +1671,8 @@ int enqueue_group_step(chan_ensemble* e, const std::vector<int>& idx, TabSink& s
                 T** I = st(q, inb(q));
                 T** O = st(q, outb(q));
                 T** Xs = st(q, X[q]);
-                a[q] = RhsArgs<T>{I[0], I[1], I[2], (const T*)c->nut, nullptr, rhs_tabs<T>(c), T(c->nu), c->sc, Xs[0], Xs[1],
+                a[q] = RhsArgs<T>{I[0], I[1], I[2], defer ? (const T*)c->dsm[9] : (const T*)c->nut,
+                                  defer ? (const T*)c->ntab : nullptr, rhs_tabs<T>(c), T(c->nu), c->sc, Xs[0], Xs[1],
                                   Xs[2], (T*)c->F1[0], (T*)c->F1[1], (T*)c->F1[2], O[0], O[1], O[2]};
             }
             const RhsArgs<T>* t = sink.put(a);

@@ -196,6 +196,9 @@ void run_dsm_ratio(const DsmRatioArgs& one, const DsmRatioArgs* tab, int nmem, c
 template <typename T>
 void run_dsm_nut(const DsmNutArgs<T>& one, const DsmNutArgs<T>* tab, int nmem, const Dims& d, int fuse_wm,
                  hipStream_t s);
+// the rest of a deferred pass B (DsmBufs::ntab) in the |S| box passed as nut
+template <typename T>
+void run_dsm_edges(const DsmNutArgs<T>& one, const DsmNutArgs<T>* tab, int nmem, const Dims& d, hipStream_t s);
 template <typename T>
 void run_equilibrium_alpha(const EqAlphaArgs<T>& one, const EqAlphaArgs<T>* tab, int nmem, const Dims& d,
                            hipStream_t s);

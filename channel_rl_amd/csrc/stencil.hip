@@ -218,10 +218,12 @@ __global__ void __launch_bounds__(256) k_dsm_nut(DsmNutArgs<T> one, const DsmNut
 //   rows 3..nyg-2, from their unique cells.
 // No thread reads a cell that another one writes.
 template <typename T>
-__global__ void __launch_bounds__(256) k_dsm_edges(DsmNutArgs<T> a, Dims d)
+__global__ void __launch_bounds__(256) k_dsm_edges(DsmNutArgs<T> one, const DsmNutArgs<T>* tab, Dims d)
 {
+    const GBlk g = gblk(1);
+    const DsmNutArgs<T> a = member_args(one, tab, g.m);
     T* S = a.nut;  // the |S| box (the launcher passes it as nut)
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long q = (long)g.x * blockDim.x + threadIdx.x;
     const int nxz = d.nxg * d.nzg;
     if (q < 2L * nxz) {
         const int side = (int)(q / nxz), r = (int)(q - (long)side * nxz);
@@ -248,6 +250,16 @@ __global__ void __launch_bounds__(256) k_dsm_edges(DsmNutArgs<T> a, Dims d)
         i = p2 - c * (d.nxg - 3) + 2;
     }
     S[at(d, i, j, k)] = S[at(d, img_centre(i, d.nxg), j, img_centre(k, d.nzg))];
+}
+
+template <typename T>
+void run_dsm_edges(const DsmNutArgs<T>& one, const DsmNutArgs<T>* tab, int nmem, const Dims& d, hipStream_t s)
+{
+    const int npos = 3 * d.nzg + 3 * (d.nxg - 3);
+    const long n = 2L * d.nxg * d.nzg + (long)npos * (d.nyg - 4);
+    PROF_BEGIN("k_dsm_edges", s);
+    k_dsm_edges<T><<<dim3(cdiv(n, 256), 1, nmem), 256, 0, s>>>(one, tab, d);
+    PROF_END(s);
 }
 
 void run_dsm_ratio(const DsmRatioArgs& one, const DsmRatioArgs* tab, int nmem, const Dims& d, int nblk,
@@ -292,13 +304,9 @@ void launch_dsm(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, 
     run_dsm_ratio(ra, nullptr, 1, d, b.nblk, s);
     const DsmNutArgs<T> na{nut, b.smag, b.ratio, U, alpha_x, m, nu, dirichlet, sc};
     if (defer) {
-        const int npos = 3 * d.nzg + 3 * (d.nxg - 3);
-        const long n = 2L * d.nxg * d.nzg + (long)npos * (d.nyg - 4);
         DsmNutArgs<T> ea = na;
         ea.nut = b.smag;  // completed in place
-        PROF_BEGIN("k_dsm_edges", s);
-        k_dsm_edges<T><<<cdiv(n, 256), 256, 0, s>>>(ea, d);
-        PROF_END(s);
+        run_dsm_edges<T>(ea, nullptr, 1, d, s);
     } else {
         run_dsm_nut<T>(na, nullptr, 1, d, fuse_wm, s);
     }
@@ -829,6 +837,7 @@ void launch_wall_shear(const T* U, const T* nut, const Metric<double>& md, const
                                        int, double*, hipStream_t);                                              \
     template void run_dsm_nut<T>(const DsmNutArgs<T>&, const DsmNutArgs<T>*, int, const Dims&, int,             \
                                  hipStream_t);                                                                  \
+    template void run_dsm_edges<T>(const DsmNutArgs<T>&, const DsmNutArgs<T>*, int, const Dims&, hipStream_t);  \
     template void run_equilibrium_alpha<T>(const EqAlphaArgs<T>&, const EqAlphaArgs<T>*, int, const Dims&,      \
                                            hipStream_t);                                                        \
     template void run_mass_flow_correct<T>(const MassArgs<T>&, const MassArgs<T>*, int, const Dims&, int, int,  \
