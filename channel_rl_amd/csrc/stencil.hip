@@ -81,13 +81,13 @@ void launch_bc(T* F, const Dims& d, int kind, hipStream_t s)
 // --------------------------------------------------------------- SGS: DSM
 // Pass A (filters, L_ij, M_ij, plane partial sums, |S|) is dsm_fused.hip.
 
-
 // <L:M>/<M:M> per row: one wave per row, the blocks' partials summed in a
 // fixed order (strided per lane, then a butterfly across the wave), so the
-// result does not depend on scheduling; no LDS and no barrier.  With the
-// DSM pass's compute_dt maxima (ar.dtpart), a last block reduces them and
-// sets the step's dt (the maximum does not depend on the order).
-
+// result does not depend on scheduling; no LDS and no barrier.  The row
+// waves also write pass B's coefficients when it is deferred (ar.ntab).
+// With the DSM pass's compute_dt maxima (ar.dtpart), one more block reduces
+// them and sets the step's dt (the maximum does not depend on the order);
+// with ar.eq_alpha, the blocks after it compute the equilibrium wall stress.
 __global__ void __launch_bounds__(64) k_dsm_ratio(DsmRatioArgs one, const DsmRatioArgs* tab, int nblk, Dims d)
 {
     const GBlk g = gblk(1);
