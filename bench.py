@@ -200,7 +200,7 @@ def measure_batched(ch, case, args, cr, cases):
             c.close()
         log(f"[bench] ensemble {name}: {len(mem) * k / times[name]:.1f} channel-steps/s")
     chans = [E.make_channel(c, f, precision=cr.FP32_BULK, wall=cr.WALL_MLP, net=net) for c, f in mem]
-    tuned["batched"] = E.tune_members(chans)
+    tuned["batched"] = E.tune_groups(chans, E.tune_members(chans))
     ens = cr.Ensemble(chans)
     ens.set_grouping(False)
     times["batched_member_kernels"] = E.time_batched(ens, k)

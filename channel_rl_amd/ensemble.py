@@ -5,7 +5,7 @@ import time
 
 import numpy as np
 
-from . import FP64, FP32_BULK, WALL_EQUILIBRIUM, WALL_MLP, Channel, Ensemble, cases
+from . import ALPHA_X, FP64, FP32_BULK, WALL_EQUILIBRIUM, WALL_MLP, Channel, Ensemble, cases
 
 
 def make_channel(case, field, precision=FP64, wall=WALL_EQUILIBRIUM, net=None, max_rel_err=0.05):
@@ -40,6 +40,57 @@ def tune_members(channels, reps=3):
             ch.set_tile_shape(picked[key][0], KERNEL_RHS)
             ch.set_tile_shape(picked[key][1], KERNEL_DSM)
     return picked
+
+
+def tune_groups(channels, picked, steps=12):
+    """The tile shapes of each grid's members refined on their grouped launch:
+    a group runs many times the blocks of one channel, so the shapes the
+    autotuner picks for one channel (tune_members, `picked`) need not be the
+    group's best.  For every grid with two or more members, the grouped step
+    is timed over `steps` steps for each DSM shape (the RHS shape fixed), then
+    for each RHS shape (the best DSM shape fixed); the fastest pair is set on
+    the members.  Every member's fields, scalars and wall stress are restored
+    after each trial, so the tuning leaves the state as it was.  Returns
+    {(nx, ny, nz, precision): (rhs shape, dsm shape)} with the refined picks."""
+    from . import KERNEL_DSM, KERNEL_RHS, load_library
+    L = load_library()
+    nr, nd = L.chan_tile_shape_count(KERNEL_RHS), L.chan_tile_shape_count(KERNEL_DSM)
+    out = dict(picked)
+    for key in picked:
+        members = [ch for ch in channels if (ch.nx, ch.ny, ch.nz, ch.precision) == key]
+        if len(members) < 2:
+            continue
+        saved = [(ch.state(), ch.scalars(), ch.get_field(ALPHA_X)) for ch in members]
+
+        def trial(r, d):
+            for ch in members:
+                ch.set_tile_shape(r, KERNEL_RHS)
+                ch.set_tile_shape(d, KERNEL_DSM)
+            ens = Ensemble(members)
+            try:
+                ens.run_steps(2)
+                ens.synchronize()
+                t0 = time.perf_counter()
+                ens.run_steps(steps)
+                ens.synchronize()
+                return time.perf_counter() - t0
+            finally:
+                ens.close()
+                for ch, (st, sc, a) in zip(members, saved):
+                    ch.set_state(*st)
+                    ch.set_scalars(**sc)
+                    ch.set_field(ALPHA_X, a)
+
+        r0, d0 = picked[key]
+        td = [trial(r0, d) for d in range(nd)]
+        d1 = int(np.argmin(td))
+        tr = [trial(r, d1) for r in range(nr)]
+        r1 = int(np.argmin(tr))
+        for ch in members:
+            ch.set_tile_shape(r1, KERNEL_RHS)
+            ch.set_tile_shape(d1, KERNEL_DSM)
+        out[key] = (r1, d1)
+    return out
 
 
 def time_sequential(channels, steps):
@@ -96,7 +147,7 @@ def main(argv=None):
         print(f"no member with {a.faces} x faces", file=sys.stderr)
         return 2
     chans = [make_channel(c, f, precision=FP32_BULK, wall=WALL_MLP, net=net) for c, f in mem]
-    tune_members(chans)
+    tune_groups(chans, tune_members(chans))
     ens = Ensemble(chans)
     ens.run_steps(10)
     ens.synchronize()

@@ -172,3 +172,37 @@ def test_grouped_ensemble_matches_members_alone(gpu):
             assert np.array_equal(x, z), f"member {q} ungrouped"
     eg.close()
     es.close()
+
+
+def test_group_tuning_leaves_the_state_and_picks_valid_shapes(gpu):
+    """tune_groups times each grid's grouped step under every DSM and RHS
+    tile shape and restores the members' fields, scalars and wall stress
+    after each trial: afterwards the members step exactly as untuned copies
+    with the picked shapes set."""
+    from channel_rl_amd import ALPHA_X, KERNEL_DSM, KERNEL_RHS, load_library
+    from channel_rl_amd.ensemble import tune_groups, tune_members
+    L = load_library()
+    case, field = cases.small_case(18, 17, 18, seed=21)
+    tuned = [make_channel(case, field) for _ in range(2)]
+    ref = [make_channel(case, field) for _ in range(2)]
+    before = [_state(ch) + [ch.get_field(ALPHA_X)] for ch in tuned]
+    picks = tune_groups(tuned, tune_members(tuned, reps=1), steps=2)
+    (r, d), = picks.values()
+    assert 0 <= r < L.chan_tile_shape_count(KERNEL_RHS) and 0 <= d < L.chan_tile_shape_count(KERNEL_DSM)
+    for ch, b in zip(tuned, before):
+        assert ch.tile_shape(KERNEL_RHS) == r and ch.tile_shape(KERNEL_DSM) == d
+        for x, y in zip(_state(ch) + [ch.get_field(ALPHA_X)], b):
+            assert np.array_equal(x, y)
+    for ch in ref:
+        ch.set_tile_shape(r, KERNEL_RHS)
+        ch.set_tile_shape(d, KERNEL_DSM)
+    e1, e2 = Ensemble(tuned), Ensemble(ref)
+    e1.run_steps(3)
+    e2.run_steps(3)
+    e1.synchronize()
+    e2.synchronize()
+    for a, b in zip(tuned, ref):
+        for x, y in zip(_state(a), _state(b)):
+            assert np.array_equal(x, y)
+    e1.close()
+    e2.close()
