@@ -560,3 +560,38 @@ def test_amd_step_folds_compute_dt_and_wall_stress_bitwise(gpu, precision):
         assert np.array_equal(a.get_field(NU_T), b.get_field(NU_T))
         assert np.array_equal(a.get_field(ALPHA_X), b.get_field(ALPHA_X))
         del a, b, chans
+
+
+@pytest.mark.parametrize("les", [2, 3])
+def test_captured_and_launched_steps_agree_bitwise(gpu, les, monkeypatch):
+    """run_steps replays a captured graph of the step; with
+    CHANLES_NO_GRAPH=1 the same launches go to the stream one by one, and
+    profile_steps times them one by one: all three give the same fields, dt
+    and nu_t, bit for bit, with every fold of the step in use (compute_dt,
+    the wall stress and, for the dynamic model, the deferred pass B and the
+    mass-flow shift)."""
+    from channel_rl_amd import WALL_EQUILIBRIUM
+    case, field = cases.small_case(22, 13, 34, seed=12, LES_model=les)
+    chans = []
+    for graph in (True, False, True):
+        if graph:
+            monkeypatch.delenv("CHANLES_NO_GRAPH", raising=False)
+        else:
+            monkeypatch.setenv("CHANLES_NO_GRAPH", "1")
+        g = Channel(case.x, case.y, case.z, precision=FP64, **case.kwargs())
+        g.set_state(*field)
+        g.apply_boundary_conditions()
+        g.compute_projection_step()
+        g.initialize_mass_flow()
+        g.set_scalars(dt=0.0, t=0.0, dPdx=case.dPdx, Qflow_x_0=g.scalars()["Qflow_x_0"])
+        g.set_wall_model(WALL_EQUILIBRIUM, h_wm=0.1)
+        chans.append(g)
+    a, b, c = chans
+    a.run_steps(4)
+    b.run_steps(4)
+    c.profile_steps(4)
+    for other in (b, c):
+        for x, y in zip(a.state(), other.state()):
+            assert np.array_equal(x, y), les
+        assert np.array_equal(a.get_field(NU_T), other.get_field(NU_T))
+        assert a.scalars() == other.scalars()
