@@ -313,18 +313,26 @@ __global__ void __launch_bounds__(256) k_amd_wall(const T* U, T* nut, double* al
     }
 }
 
+// the AMD kernel's tile (32x8 centres) and z chunk: at most AMD_KMAX
+// planes, enough blocks for two rounds on 256 CUs
+static constexpr int AMD_TX = 32, AMD_TY = 8;
+static dim3 amd_grid(const Dims& d, int& kc)
+{
+    const int nk = d.nzg - 3;
+    const long tiles = (long)cdiv(d.nxg - 3, AMD_TX) * cdiv(d.nyg - 2, AMD_TY);
+    kc = (int)std::min<long>(AMD_KMAX, std::max<long>(4, (tiles * nk + 2047) / 2048));
+    kc = std::min(kc, nk);
+    return dim3(cdiv(d.nxg - 3, AMD_TX), cdiv(d.nyg - 2, AMD_TY), (unsigned)cdiv(nk, kc));
+}
+
 template <typename T>
 void launch_amd(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, const Dims& d, int dirichlet,
                 const Scalars* sc, int fuse_wm, const double* alpha, T nu, hipStream_t s, const EqAlphaArgs<T>* eq,
                 double* dtpart, const Metric<double>* md, const double* mins3)
 {
-    constexpr int TX = 32, TY = 8;
-    const int nk = d.nzg - 3;
-    // z chunks of at most AMD_KMAX planes, enough blocks for two rounds on 256 CUs
-    const long tiles = (long)cdiv(d.nxg - 3, TX) * cdiv(d.nyg - 2, TY);
-    int kc = (int)std::min<long>(AMD_KMAX, std::max<long>(4, (tiles * nk + 2047) / 2048));
-    kc = std::min(kc, nk);
-    dim3 blk(TX, TY, 1), grd(cdiv(d.nxg - 3, TX), cdiv(d.nyg - 2, TY), (unsigned)cdiv(nk, kc));
+    constexpr int TX = AMD_TX, TY = AMD_TY;
+    int kc;
+    const dim3 blk(TX, TY, 1), grd = amd_grid(d, kc);
     // compute_dt rides along only with the wall rows' launch after it
     const bool with_dt = dtpart && fuse_wm;
     const AmdDt adt{with_dt ? dtpart : nullptr, md ? md->xg : nullptr, md ? md->yg : nullptr, md ? md->zg : nullptr};
@@ -346,11 +354,9 @@ void launch_amd(const T* U, const T* V, const T* W, T* nut, const Metric<T>& m, 
 // blocks of k_amd (the compute_dt partials' row length)
 int amd_blocks(const Dims& d)
 {
-    const int nk = d.nzg - 3;
-    const long tiles = (long)cdiv(d.nxg - 3, 32) * cdiv(d.nyg - 2, 8);
-    int kc = (int)std::min<long>(AMD_KMAX, std::max<long>(4, (tiles * nk + 2047) / 2048));
-    kc = std::min(kc, nk);
-    return (int)(tiles * cdiv(nk, kc));
+    int kc;
+    const dim3 g = amd_grid(d, kc);
+    return (int)(g.x * g.y * g.z);
 }
 
 template void launch_amd<double>(const double*, const double*, const double*, double*, const Metric<double>&,
