@@ -103,3 +103,26 @@ def test_rocprof_summary_sq_fractions(tmp_path):
     p.write_text(hdr + "".join(f'"void chan::k_dsm_fused<double, 30, 14>(x)",128,{c},{v}\n' for c, v in vals.items()))
     rows = [r for r in rs.sq_table([str(p)]).splitlines() if r.startswith("| k_")]
     assert rows == ["| k_dsm_fused<double, 30, 14> | 2 | 300 | 15 | 0.40 | 0.20 | 0.40 | 0.50 | 0.50 | 0.05 |"]
+
+
+def test_step_algorithmic_bytes_of_the_dynamic_model_step():
+    """bench.py sums each stage's algorithmic values per launch over the
+    launches of a step: the dynamic-model step with mass-flow mode 1 (pass A,
+    RHS and the projection's five kernels per substep, pass B in the last
+    substep only, compute_dt and the shift folded) moves 107 values per cell,
+    856 B in FP64 and 548 B with the FP32 bulk (spectral values stay FP64)."""
+    import importlib.util
+    import os
+    spec = importlib.util.spec_from_file_location(
+        "bench_mod", os.path.join(os.path.dirname(__file__), "..", "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    per_step = {"k_dsm_fused": 3, "k_rhs_rk": 3, "k_dsm_nut": 1, "k_fftx_fwd": 3, "k_fftz": 6,
+                "k_tridiag": 3, "k_fftx_inv": 3, "k_project": 3}
+
+    def nbytes(bulk):
+        return sum(n * (bench.ALG_VALUES[k][0] * bulk + bench.ALG_VALUES[k][1] * 8) for k, n in per_step.items())
+
+    assert sum(n * sum(bench.ALG_VALUES[k]) for k, n in per_step.items()) == 107
+    assert nbytes(8) == 856
+    assert nbytes(4) == 548
