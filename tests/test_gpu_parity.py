@@ -482,14 +482,16 @@ def test_deferred_dsm_pass_b_bitwise(gpu, precision, monkeypatch):
     launch, and a small kernel completes |S|'s wall rows and periodic images.
     Three steps give the same fields, nu_t and P, bit for bit, as with pass B
     materialising nu_t in every substep (CHANLES_NO_DEFERRED_NUT=1), for the
-    equilibrium law and for a fixed alpha_x, on a grid whose tiles do not
-    divide the box and for every RHS tile shape."""
-    from channel_rl_amd import FP32_BULK, KERNEL_RHS, WALL_EQUILIBRIUM, load_library
+    equilibrium law, for a fixed alpha_x and for the learned wall model
+    (an untrained network, checked against the law), on a grid whose tiles do
+    not divide the box and for every RHS tile shape."""
+    from channel_rl_amd import FP32_BULK, KERNEL_RHS, WALL_EQUILIBRIUM, WALL_MLP, WallNetwork, load_library
     prec = FP64 if precision == FP64 else FP32_BULK
     case, field = cases.small_case(22, 13, 34, seed=11)
     nshape = load_library().chan_tile_shape_count(KERNEL_RHS)
-    for wall in ("equilibrium", "fixed"):
-        for shape in range(0, nshape, 3):
+    net = WallNetwork(seed=3)
+    for wall in ("equilibrium", "fixed", "network"):
+        for shape in range(0, nshape, 3 if wall != "network" else 5):
             chans = []
             for deferred in (False, True):
                 if deferred:
@@ -503,6 +505,9 @@ def test_deferred_dsm_pass_b_bitwise(gpu, precision, monkeypatch):
                 g.set_scalars(dt=0.0, t=0.0, dPdx=0.0, Qflow_x_0=0.0)
                 if wall == "equilibrium":
                     g.set_wall_model(WALL_EQUILIBRIUM, h_wm=0.1)
+                elif wall == "network":
+                    g.set_wall_network(net, max_rel_err=0.05)
+                    g.set_wall_model(WALL_MLP, h_wm=0.1)
                 else:
                     g.set_field(ALPHA_X, np.full(g.shape(ALPHA_X), 2e-4))
                 g.set_tile_shape(shape, KERNEL_RHS)
@@ -515,6 +520,8 @@ def test_deferred_dsm_pass_b_bitwise(gpu, precision, monkeypatch):
             for f in (NU_T, P, ALPHA_X):
                 assert np.array_equal(a.get_field(f), b.get_field(f)), (wall, shape, f)
             assert a.scalars() == b.scalars()
+            a.close()
+            b.close()
             del a, b, chans
 
 
