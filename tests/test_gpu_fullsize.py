@@ -144,3 +144,39 @@ def test_fullsize_fp32_bulk_step_keeps_invariants(gpu):
     q = held_flow(U)
     assert abs(q - q0) <= 1e-6 * abs(q0), (q, q0)
     ch.close()
+
+
+@pytest.mark.parametrize("les", [2, 3])
+def test_step_folds_bitwise_at_full_size(gpu, wmles, les, monkeypatch):
+    """At 192x97x192 FP64 (every kernel over many tiles and rounds of
+    blocks): run_steps, with compute_dt in the first eddy-viscosity launches
+    and the dynamic model's pass B deferred to the RHS, against compute_dt +
+    compute_time_step_RK3 with pass B materialised in every substep
+    (CHANLES_NO_DEFERRED_NUT=1): two steps agree bit for bit (the wall stress
+    and the mass-flow shift ride in the same launches on both sides)."""
+    case, field, (U, V, W) = wmles
+    chans = []
+    for folded in (True, False):
+        if folded:
+            monkeypatch.delenv("CHANLES_NO_DEFERRED_NUT", raising=False)
+        else:
+            monkeypatch.setenv("CHANLES_NO_DEFERRED_NUT", "1")
+        kw = case.kwargs()
+        kw["LES_model"] = les
+        g = Channel(case.x, case.y, case.z, precision=FP64, **kw)
+        g.set_state(U, V, W)
+        g.initialize_mass_flow()
+        g.set_scalars(dt=0.0, t=0.0, dPdx=case.dPdx, Qflow_x_0=g.scalars()["Qflow_x_0"])
+        g.set_wall_model(WALL_EQUILIBRIUM, h_wm=case.h_wm)
+        chans.append(g)
+    a, b = chans
+    a.run_steps(2)
+    for _ in range(2):
+        b.compute_dt()
+        b.compute_time_step_RK3()
+    for x, y in zip(a.state(), b.state()):
+        assert np.array_equal(x, y), les
+    assert np.array_equal(a.get_field(NU_T), b.get_field(NU_T))
+    assert a.scalars() == b.scalars()
+    a.close()
+    b.close()
