@@ -1793,6 +1793,11 @@ void chan_ensemble_destroy(chan_ensemble* e)
     if (!e) return;
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    // the members' streams wait on this ensemble's events (run_steps orders
+    // their later work after its steps): let them pass those waits before
+    // the events go
+    for (chan_ctx* c : e->ch)
+        if (c && c->stream) (void)hipStreamSynchronize(c->stream);
     ens_drop(e);
     for (hipEvent_t ev : e->join)
         if (ev) (void)hipEventDestroy(ev);

@@ -61,12 +61,14 @@ def tune_groups(channels, picked, steps=12):
         if len(members) < 2:
             continue
         saved = [(ch.state(), ch.scalars(), ch.get_field(ALPHA_X)) for ch in members]
+        # one ensemble for all the trials: a change of shape makes it capture
+        # its step again
+        ens = Ensemble(members)
 
         def trial(r, d):
             for ch in members:
                 ch.set_tile_shape(r, KERNEL_RHS)
                 ch.set_tile_shape(d, KERNEL_DSM)
-            ens = Ensemble(members)
             try:
                 ens.run_steps(2)
                 ens.synchronize()
@@ -75,17 +77,19 @@ def tune_groups(channels, picked, steps=12):
                 ens.synchronize()
                 return time.perf_counter() - t0
             finally:
-                ens.close()
                 for ch, (st, sc, a) in zip(members, saved):
                     ch.set_state(*st)
                     ch.set_scalars(**sc)
                     ch.set_field(ALPHA_X, a)
 
-        r0, d0 = picked[key]
-        td = [trial(r0, d) for d in range(nd)]
-        d1 = int(np.argmin(td))
-        tr = [trial(r, d1) for r in range(nr)]
-        r1 = int(np.argmin(tr))
+        try:
+            r0, d0 = picked[key]
+            td = [trial(r0, d) for d in range(nd)]
+            d1 = int(np.argmin(td))
+            tr = [trial(r, d1) for r in range(nr)]
+            r1 = int(np.argmin(tr))
+        finally:
+            ens.close()
         for ch in members:
             ch.set_tile_shape(r1, KERNEL_RHS)
             ch.set_tile_shape(d1, KERNEL_DSM)
@@ -139,6 +143,9 @@ def main(argv=None):
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--faces", type=int, default=0, help="only the members with this many x faces (0: all)")
     ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--shapes-file", default="",
+                    help="tile shapes per grid: read from this JSON file when it exists (no tuning), "
+                         "else tuned and written there")
     a = ap.parse_args(argv)
     X, Y = wallnet.solver_samples(200_000, seed=7)
     net, _ = wallnet.train(X, Y, epochs=4)
@@ -147,7 +154,21 @@ def main(argv=None):
         print(f"no member with {a.faces} x faces", file=sys.stderr)
         return 2
     chans = [make_channel(c, f, precision=FP32_BULK, wall=WALL_MLP, net=net) for c, f in mem]
-    tune_groups(chans, tune_members(chans))
+    import json
+    import os
+    from . import KERNEL_DSM, KERNEL_RHS
+    if a.shapes_file and os.path.exists(a.shapes_file):
+        with open(a.shapes_file) as fh:
+            picks = {tuple(map(int, k.split())): v for k, v in json.load(fh).items()}
+        for ch in chans:
+            r, d = picks[(ch.nx, ch.ny, ch.nz, ch.precision)]
+            ch.set_tile_shape(r, KERNEL_RHS)
+            ch.set_tile_shape(d, KERNEL_DSM)
+    else:
+        picks = tune_groups(chans, tune_members(chans))
+        if a.shapes_file:
+            with open(a.shapes_file, "w") as fh:
+                json.dump({" ".join(map(str, k)): list(v) for k, v in picks.items()}, fh)
     ens = Ensemble(chans)
     ens.run_steps(10)
     ens.synchronize()
