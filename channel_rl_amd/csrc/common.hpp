@@ -111,6 +111,14 @@ __device__ inline double dt_from_maxima(double mu, double mv, double mw, const S
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
+// A 256-thread block over x rows of n points: 64 x 4, or 32 x 8 where a
+// 64-wide tile would leave more lanes idle (96 points: a half-empty second
+// tile, a quarter of the launch's lanes).
+inline dim3 row_block(int n)
+{
+    return (cdiv(n, 32) * 32 < cdiv(n, 64) * 64) ? dim3(32, 8, 1) : dim3(64, 4, 1);
+}
+
 // z-chunk length in [lo, hi] for a kernel whose blocks each march `kc` of
 // nk planes over one of `tiles` (x,y) tiles, with two planes of pre-roll:
 // blocks run in rounds of `slots` (resident blocks per CU x CUs), so the

@@ -936,7 +936,8 @@ __global__ void __launch_bounds__(256) k_project(ProjArgs<T> one, const ProjArgs
 template <typename T>
 void run_project(const ProjArgs<T>& one, const ProjArgs<T>* tab, int nmem, const Dims& d, hipStream_t s)
 {
-    dim3 blk(64, 4, 1), grd(cdiv(d.nxg - 3, 64), cdiv(d.nyg - 2, 4), (unsigned)(d.nzg - 3) * nmem);
+    const dim3 blk = row_block(d.nxg - 3);
+    const dim3 grd(cdiv(d.nxg - 3, blk.x), cdiv(d.nyg - 2, blk.y), (unsigned)(d.nzg - 3) * nmem);
     PROF_BEGIN("k_project", s);
     if (tab) k_project<T, true><<<grd, blk, 0, s>>>(one, tab, d);
     else k_project<T, false><<<grd, blk, 0, s>>>(one, nullptr, d);

@@ -545,7 +545,8 @@ void run_mass_flow_correct(const MassArgs<T>& one, const MassArgs<T>* tab, int n
     PROF_END(s);
     if (apply) {
         // canonical cells only (k up to nzg-2), images written by store_img
-        dim3 blk(64, 4, 1), grd(cdiv(d.nx - 2, 64), cdiv(d.nyg - 2, 4), (unsigned)cdiv(d.nzg - 3, SHIFT_PLANES) * nmem);
+        const dim3 blk = row_block(d.nx - 2);
+        const dim3 grd(cdiv(d.nx - 2, blk.x), cdiv(d.nyg - 2, blk.y), (unsigned)cdiv(d.nzg - 3, SHIFT_PLANES) * nmem);
         PROF_BEGIN("k_shift_u", s);
         k_shift_u<T><<<grd, blk, 0, s>>>(one, tab, d);
         PROF_END(s);
