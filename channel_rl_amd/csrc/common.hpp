@@ -111,6 +111,34 @@ __device__ inline double dt_from_maxima(double mu, double mv, double mw, const S
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
+// A linear cell index t = i + n1 (j + n2 k) (0-based, x fastest) walked in
+// steps of a fixed stride S: with S = sr n1 + s1 and sr = sk n2 + sj the
+// next cell is (i + s1, j + sj, k + sk) plus at most one carry out of i and
+// one out of j, so a loop needs no division per cell.
+struct CellIdx { int i, j, k; };
+struct StrideCarry { int s1, sj, sk; };
+__host__ __device__ inline CellIdx cell_of(int t, int n1, int n2)
+{
+    const int r = t / n1;
+    return CellIdx{t - r * n1, r % n2, r / n2};
+}
+__host__ __device__ inline StrideCarry stride_carry(int S, int n1, int n2)
+{
+    const int sr = S / n1;
+    return StrideCarry{S % n1, sr % n2, sr / n2};
+}
+__host__ __device__ inline CellIdx advance(CellIdx c, const StrideCarry& s, int n1, int n2)
+{
+    c.i += s.s1;
+    const int ci = c.i >= n1;
+    if (ci) c.i -= n1;
+    c.j += s.sj + ci;
+    const int cj = c.j >= n2;
+    if (cj) c.j -= n2;
+    c.k += s.sk + cj;
+    return c;
+}
+
 // A 256-thread block over x rows of n points: 64 x 4, or 32 x 8 where a
 // 64-wide tile would leave more lanes idle (96 points: a half-empty second
 // tile, a quarter of the launch's lanes).

@@ -520,14 +520,17 @@ __global__ void k_bulk_partial(MassArgs<T> one, const MassArgs<T>* tab, Dims d, 
     const int n1 = d.nx - 2, n2 = d.nyg - 3, n3 = d.nzp;
     const int tot = n1 * n2 * n3;
     double q = 0.0, nrm = 0.0;
-    for (int t = g.x * RED_THREADS + threadIdx.x; t < tot; t += nblk * RED_THREADS) {
-        const int r = t / n1;
-        const int i = t - r * n1 + 2;
-        const int j = r % n2 + 3;
-        const int k = r / n2 + 2;
+    // term t is the cell (t % n1 + 2, (t / n1) % n2 + 3, t / (n1 n2) + 2); a
+    // thread's terms are S apart, so its indices follow by carries
+    const StrideCarry sc = stride_carry(nblk * RED_THREADS, n1, n2);
+    int t = g.x * RED_THREADS + threadIdx.x;
+    CellIdx c = cell_of(t, n1, n2);
+    for (; t < tot; t += nblk * RED_THREADS) {
+        const int i = c.i + 2, j = c.j + 3, k = c.k + 2;
         const double dy = yg[j] - yg[j - 1];
         q += ((double)U[at(d, i, j, k)] + (double)U[at(d, i, j - 1, k)]) * 0.5 * dy;
         nrm += (1.0 + 1.0) * 0.5 * dy;
+        c = advance(c, sc, n1, n2);
     }
     block_sum2(q, nrm, ar.part, nblk, g.x);
 }
